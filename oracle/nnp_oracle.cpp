@@ -1,0 +1,1302 @@
+// CPU FP64 oracle of the 2G-HDNNP energy/force path -- TEST INFRASTRUCTURE.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+// this library; it is the checker, never the product.  It restates n2p2's
+// algorithm from scratch; every block cites the reference file:line it
+// follows (paths relative to the n2p2 source tree).  Expressions keep the
+// reference's association order and the library is compiled with
+// -ffp-contract=off so that, with glibc libm, results reproduce the
+// reference's golden vectors bit for bit (test/cpp/ExamplePrediction.h).
+#include "nnp_oracle.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using std::string;
+using std::vector;
+
+thread_local string g_err;
+
+// ---------------------------------------------------------------- strings --
+// utility.cpp:33-77 (split(reduce(line))): tokens separated by runs of
+// blanks/tabs.
+vector<string> words(const string& line)
+{
+    vector<string> out;
+    size_t i = 0, n = line.size();
+    while (i < n) {
+        while (i < n && (line[i] == ' ' || line[i] == '\t')) ++i;
+        if (i >= n) break;
+        size_t j = i;
+        while (j < n && line[j] != ' ' && line[j] != '\t') ++j;
+        out.push_back(line.substr(i, j - i));
+        i = j;
+    }
+    return out;
+}
+
+string fmt(const char* f, ...) __attribute__((format(printf, 1, 2)));
+string fmt(const char* f, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, f);
+    vsnprintf(buf, sizeof buf, f, ap);
+    va_end(ap);
+    return buf;
+}
+
+// Element symbols ordered by atomic number (Z = position + 1).
+const char* kSymbols[] = {
+    "H", "He", "Li", "Be", "B", "C", "N", "O", "F", "Ne", "Na", "Mg", "Al",
+    "Si", "P", "S", "Cl", "Ar", "K", "Ca", "Sc", "Ti", "V", "Cr", "Mn", "Fe",
+    "Co", "Ni", "Cu", "Zn", "Ga", "Ge", "As", "Se", "Br", "Kr", "Rb", "Sr",
+    "Y", "Zr", "Nb", "Mo", "Tc", "Ru", "Rh", "Pd", "Ag", "Cd", "In", "Sn",
+    "Sb", "Te", "I", "Xe", "Cs", "Ba", "La", "Ce", "Pr", "Nd", "Pm", "Sm",
+    "Eu", "Gd", "Tb", "Dy", "Ho", "Er", "Tm", "Yb", "Lu", "Hf", "Ta", "W",
+    "Re", "Os", "Ir", "Pt", "Au", "Hg", "Tl", "Pb", "Bi", "Po", "At", "Rn",
+    "Fr", "Ra", "Ac", "Th", "Pa", "U", "Np", "Pu", "Am", "Cm", "Bk", "Cf",
+    "Es", "Fm", "Md", "No"};
+
+int atomic_number(const string& s)
+{
+    for (size_t i = 0; i < sizeof kSymbols / sizeof kSymbols[0]; ++i)
+        if (s == kSymbols[i]) return (int)i + 1;
+    throw std::runtime_error("unknown element symbol \"" + s + "\"");
+}
+
+// ---------------------------------------------------------------- cutoff --
+// CutoffFunction.cpp:25-28 constants, 95-112 set*, 114-213 f/df,
+// CutoffFunction.h:210-220 fdf guard, CoreFunction.h:214-319 cores.
+struct Cutoff {
+    int type = 0;
+    double rc = 0, rcinv = 0, rci = 0, alpha = 0, iw = 0;
+    void set(int t, double r, double a)
+    {
+        if (t < 0 || t > 8) throw std::invalid_argument("unknown cutoff type");
+        if (a < 0.0 || a >= 1.0)
+            throw std::invalid_argument("0 <= alpha < 1.0 is required");
+        type = t;
+        rc = r;
+        rcinv = 1.0 / r;
+        alpha = a;
+        rci = rc * alpha;
+        iw = 1.0 / (rc - rci);
+    }
+    void fdf(double r, double& fc, double& dfc) const
+    {
+        static const double PI = 4.0 * atan(1.0);
+        static const double E = exp(1.0);
+        static const double TANH_PRE = pow((E + 1 / E) / (E - 1 / E), 3);
+        if (r >= rc) { fc = 0.0; dfc = 0.0; return; }
+        switch (type) {
+        case 0: fc = 1.0; dfc = 0.0; return;
+        case 1: {
+            if (r < rci) { fc = 1.0; dfc = 0.0; return; }
+            double const x = (r - rci) * iw;
+            double const t = cos(PI * x);
+            fc = 0.5 * (t + 1.0);
+            dfc = -0.5 * iw * PI * sqrt(1.0 - t * t);
+            return;
+        }
+        case 2: {
+            double const t = tanh(1.0 - r * rcinv);
+            double const t2 = t * t;
+            fc = t * t2;
+            dfc = 3.0 * t2 * (t2 - 1.0) * rcinv;
+            return;
+        }
+        case 3: {
+            double const t = tanh(1.0 - r * rcinv);
+            double const t2 = t * t;
+            fc = TANH_PRE * t * t2;
+            dfc = 3.0 * TANH_PRE * t2 * (t2 - 1.0) * rcinv;
+            return;
+        }
+        default: break;
+        }
+        // Polynomial / exponential cores on x in [0, 1).
+        if (r < rci) { fc = 1.0; dfc = 0.0; return; }
+        double const x = (r - rci) * iw;
+        if (type == 4) {
+            double const t = 1.0 / (x * x - 1.0);
+            double const t2 = exp(t);
+            fc = E * t2;
+            dfc = -2.0 * E * x * t * t * t2;
+        } else if (type == 5) {
+            fc = (2.0 * x - 3.0) * x * x + 1.0;
+            dfc = x * (6.0 * x - 6.0);
+        } else if (type == 6) {
+            double const x2 = x * x;
+            fc = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
+            dfc = x2 * ((60.0 - 30.0 * x) * x - 30.0);
+        } else if (type == 7) {
+            double const x2 = x * x;
+            fc = (x * (x * (20.0 * x - 70.0) + 84.0) - 35.0) * x2 * x2 + 1.0;
+            dfc = x2 * x * (x * (x * (140.0 * x - 420.0) + 420.0) - 140.0);
+        } else {
+            double x4 = x * x;
+            x4 *= x4;
+            fc = (x * (x * ((315.0 - 70.0 * x) * x - 540.0) + 420.0) - 126.0) *
+                 x * x4 + 1.0;
+            dfc = x4 *
+                  (x * (x * ((2520.0 - 630.0 * x) * x - 3780.0) + 2520.0) - 630.0);
+        }
+        dfc *= iw;
+    }
+};
+
+// ------------------------------------------------------- symmetry functions --
+enum Scaling { ST_NONE, ST_SCALE, ST_CENTER, ST_SCALECENTER, ST_SCALESIGMA };
+
+struct SF {
+    int type = 0;  // 2 radial, 3 angular (narrow)
+    int ec = 0, e1 = 0, e2 = 0;
+    double eta = 0, rs = 0, rc = 0, lambda = 0, zeta = 0;
+    int zetaInt = 0;
+    bool useIntPow = false;
+    int index = 0;                // sorted index within the element
+    vector<int> indexPerElement;  // compact derivative slot per neighbour element
+    // scaling (SymFnc.cpp:52-98)
+    int scalingType = ST_NONE;
+    double Smin = 0, Smax = 0, Gmin = 0, Gmax = 0, Gmean = 0, Gsigma = 0;
+    double scalingFactor = 1.0;
+
+    bool relevant(int e) const  // SymFnc*::checkRelevantElement
+    {
+        return type == 2 ? e == e1 : (e == e1 || e == e2);
+    }
+    // SymFnc.cpp:169-195
+    double scale(double v) const
+    {
+        switch (scalingType) {
+        case ST_NONE: return v;
+        case ST_SCALE: return Smin + scalingFactor * (v - Gmin);
+        case ST_CENTER: return v - Gmean;
+        case ST_SCALECENTER: return Smin + scalingFactor * (v - Gmean);
+        case ST_SCALESIGMA: return Smin + scalingFactor * (v - Gmean);
+        }
+        return 0.0;
+    }
+};
+
+// Sort key of SymFncExpRad::operator< (SymFncExpRad.cpp:57-75) and
+// SymFncExpAngn::operator< (SymFncExpAngn.cpp:51-77).  The cutoff type and
+// alpha are equal for all functions of one model.
+bool sf_less(const SF& a, const SF& b)
+{
+    if (a.ec != b.ec) return a.ec < b.ec;
+    if (a.type != b.type) return a.type < b.type;
+    if (a.rc != b.rc) return a.rc < b.rc;
+    if (a.eta != b.eta) return a.eta < b.eta;
+    if (a.rs != b.rs) return a.rs < b.rs;
+    if (a.type == 2) return a.e1 < b.e1;
+    if (a.zeta != b.zeta) return a.zeta < b.zeta;
+    if (a.lambda != b.lambda) return a.lambda < b.lambda;
+    if (a.e1 != b.e1) return a.e1 < b.e1;
+    return a.e2 < b.e2;
+}
+
+// One SF group: members sharing (ec, type, rc, e1[, e2])
+// (Element.cpp:194-287, SymGrpExpRad.cpp:110-136, SymGrpExpAngn.cpp:108-148,
+// SymGrpBaseExpAng.cpp:45-57).
+struct Group {
+    int type = 0, ec = 0, e1 = 0, e2 = 0;
+    double rc = 0;
+    Cutoff fc;
+    vector<int> mem;  // SF indices, sorted like the SF order
+    // angular per-member constants
+    vector<double> factorNorm, factorDeriv, zetaLambda, scalingFactors;
+    vector<char> calcExp;
+};
+
+// ------------------------------------------------------------ neural net --
+enum Act { AF_IDENTITY, AF_TANH, AF_LOGISTIC, AF_SOFTPLUS, AF_RELU, AF_GAUSSIAN,
+           AF_COS, AF_REVLOGISTIC, AF_EXP, AF_HARMONIC };
+
+int act_from(const string& c)  // NeuralNetwork.cpp:1078-1097
+{
+    if (c == "l") return AF_IDENTITY;
+    if (c == "t") return AF_TANH;
+    if (c == "s") return AF_LOGISTIC;
+    if (c == "p") return AF_SOFTPLUS;
+    if (c == "r") return AF_RELU;
+    if (c == "g") return AF_GAUSSIAN;
+    if (c == "c") return AF_COS;
+    if (c == "S") return AF_REVLOGISTIC;
+    if (c == "e") return AF_EXP;
+    if (c == "h") return AF_HARMONIC;
+    throw std::runtime_error("unknown activation function \"" + c + "\"");
+}
+
+struct NN {
+    vector<int> n;    // neurons per layer (layer 0 = input)
+    vector<int> act;  // per layer
+    vector<vector<double>> W;  // W[l][k*n[l-1]+j], l >= 1
+    vector<vector<double>> b;  // b[l][k]
+    bool normalize = false;
+    size_t numConnections() const
+    {
+        size_t c = 0;
+        for (size_t l = 1; l < n.size(); ++l) c += (size_t)n[l] * (n[l - 1] + 1);
+        return c;
+    }
+    // NeuralNetwork.cpp:147-169 (setConnections order).
+    void setConnections(const vector<double>& c)
+    {
+        size_t count = 0;
+        W.assign(n.size(), {});
+        b.assign(n.size(), {});
+        for (size_t l = 1; l < n.size(); ++l) {
+            W[l].assign((size_t)n[l] * n[l - 1], 0.0);
+            b[l].assign(n[l], 0.0);
+            for (int j = 0; j < n[l - 1]; ++j)
+                for (int k = 0; k < n[l]; ++k) W[l][(size_t)k * n[l - 1] + j] = c.at(count++);
+            for (int k = 0; k < n[l]; ++k) b[l][k] = c.at(count++);
+        }
+    }
+};
+
+// Activation (NeuralNetwork.cpp:778-889); returns value, sets dfdx.
+double activate(int a, double x, double& dfdx)
+{
+    const double EXP_LIMIT = 35.0;
+    double v;
+    switch (a) {
+    case AF_IDENTITY: dfdx = 1.0; return x;
+    case AF_TANH: v = tanh(x); dfdx = 1.0 - v * v; return v;
+    case AF_LOGISTIC:
+        if (x > EXP_LIMIT) { dfdx = 0.0; return 1.0; }
+        if (x < -EXP_LIMIT) { dfdx = 0.0; return 0.0; }
+        v = 1.0 / (1.0 + exp(-x)); dfdx = v * (1.0 - v); return v;
+    case AF_SOFTPLUS:
+        if (x > EXP_LIMIT) { dfdx = 1.0; return x; }
+        if (x < -EXP_LIMIT) { dfdx = 0.0; return 0.0; }
+        {
+            double const t = exp(x);
+            v = log(1.0 + t);
+            dfdx = 1.0 / (1.0 + 1.0 / t);
+            return v;
+        }
+    case AF_RELU:
+        if (x > 0.0) { dfdx = 1.0; return x; }
+        dfdx = 0.0; return 0.0;
+    case AF_GAUSSIAN: { double const t = exp(-0.5 * x * x); dfdx = -x * t; return t; }
+    case AF_COS: dfdx = -sin(x); return cos(x);
+    case AF_REVLOGISTIC:
+        v = 1.0 / (1.0 + exp(-x)); dfdx = v * (v - 1.0); return 1.0 - v;
+    case AF_EXP: v = exp(-x); dfdx = -v; return v;
+    case AF_HARMONIC: dfdx = 2.0 * x; return x * x;
+    }
+    dfdx = 0.0;
+    return 0.0;
+}
+
+// --------------------------------------------------------------- elements --
+struct Element {
+    string symbol;
+    int Z = 0;
+    double energyOffset = 0.0;
+    vector<SF> sfs;      // sorted
+    vector<Group> grps;  // sorted
+    vector<vector<int>> table;  // symmetryFunctionTable[e]
+    NN nn;
+};
+
+struct Vec { double x = 0, y = 0, z = 0; };
+
+struct Neighbor {
+    int j;
+    int bc[3];
+    int element;
+    double d;
+    double dr[3];
+    vector<double> dGdr;  // 3 * n_SFD(element)
+};
+
+struct Atom {
+    int element = 0;
+    double r[3] = {0, 0, 0};
+    vector<Neighbor> nbr;
+    vector<int> unique;  // neighborsUnique (first entry = self)
+    vector<double> G, dEdG, dGdr;  // dGdr: 3 * n_SF
+    double energy = 0.0;
+    double f[3] = {0, 0, 0};
+};
+
+}  // namespace
+
+struct oracle_ctx {
+    vector<Element> el;
+    int cutoffType = 0;
+    double cutoffAlpha = 0.0;
+    double maxCutoff = 0.0;
+    bool normalize = false;
+    double meanEnergy = 0, convEnergy = 1, convLength = 1;
+    // structure
+    vector<Atom> atoms;
+    bool periodic = false;
+    double box[3][3] = {{0}}, invbox[3][3] = {{0}};
+    double energy = 0.0;
+    double phase[5] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+// ----------------------------------------------------------- model setup --
+void load_model(oracle_ctx& c, const string& dir)
+{
+    // Settings::readFile / parseLines (Settings.cpp:188-392).
+    std::ifstream in(dir + "/input.nn");
+    if (!in) throw std::runtime_error("could not open " + dir + "/input.nn");
+    std::multimap<string, string> kv;
+    vector<std::pair<string, string>> ordered;
+    string line;
+    while (std::getline(in, line)) {
+        if (line.empty() || line == "\r") continue;
+        size_t p = line.find('#');
+        if (p != string::npos) line.erase(p);
+        p = line.find('!');
+        if (p != string::npos) line.erase(p);
+        vector<string> w = words(line);
+        if (w.empty()) continue;
+        string key = w[0], val;
+        for (size_t i = 1; i < w.size(); ++i) val += (i > 1 ? " " : "") + w[i];
+        if (key == "symfunction") key = "symfunction_short";
+        kv.emplace(key, val);
+        ordered.emplace_back(key, val);
+    }
+    auto has = [&](const char* k) { return kv.count(k) > 0; };
+    auto get = [&](const char* k) {
+        auto it = kv.find(k);
+        if (it == kv.end()) throw std::runtime_error(string("keyword \"") + k + "\" not found");
+        return it->second;
+    };
+    if (has("nnp_type")) {
+        string t = get("nnp_type");
+        if (t != "2G-HDNNP" && t != "2")
+            throw std::runtime_error("only 2G-HDNNP models are supported");
+    }
+    // Mode::setupNormalization (Mode.cpp:239-300).
+    int nn_keys = has("mean_energy") + has("conv_energy") + has("conv_length");
+    if (nn_keys == 3) {
+        c.normalize = true;
+        c.meanEnergy = atof(get("mean_energy").c_str());
+        c.convEnergy = atof(get("conv_energy").c_str());
+        c.convLength = atof(get("conv_length").c_str());
+    } else if (nn_keys != 0) {
+        throw std::runtime_error("incorrect usage of normalization keywords");
+    }
+    // ElementMap::registerElements (ElementMap.cpp:36-50): sort by Z.
+    vector<string> syms = words(get("elements"));
+    std::sort(syms.begin(), syms.end(), [](const string& a, const string& b) {
+        return atomic_number(a) < atomic_number(b);
+    });
+    if ((size_t)atoi(get("number_of_elements").c_str()) != syms.size())
+        throw std::runtime_error("inconsistent number of elements");
+    auto eidx = [&](const string& s) {
+        for (size_t i = 0; i < syms.size(); ++i) if (syms[i] == s) return (int)i;
+        throw std::runtime_error("element \"" + s + "\" not in element map");
+    };
+    c.el.assign(syms.size(), Element());
+    for (size_t i = 0; i < syms.size(); ++i) {
+        c.el[i].symbol = syms[i];
+        c.el[i].Z = atomic_number(syms[i]);
+    }
+    for (auto it = kv.lower_bound("atom_energy"); it != kv.upper_bound("atom_energy"); ++it) {
+        vector<string> a = words(it->second);
+        c.el.at(eidx(a.at(0))).energyOffset = atof(a.at(1).c_str());
+    }
+    // Mode::setupCutoff (Mode.cpp:589-682).
+    vector<string> ct = words(get("cutoff_type"));
+    c.cutoffType = atoi(ct.at(0).c_str());
+    if (ct.size() > 1) c.cutoffAlpha = atof(ct.at(1).c_str());
+    // Mode::setupSymmetryFunctions (Mode.cpp:684-759), parameter parsing
+    // SymFncExpRad.cpp:77-96, SymFncBaseExpAng.cpp:50-93.
+    for (auto& kvp : ordered) {
+        if (kvp.first != "symfunction_short") continue;
+        vector<string> a = words(kvp.second);
+        SF s;
+        s.ec = eidx(a.at(0));
+        s.type = atoi(a.at(1).c_str());
+        if (s.type == 2) {
+            s.e1 = eidx(a.at(2));
+            s.eta = atof(a.at(3).c_str());
+            s.rs = atof(a.at(4).c_str());
+            s.rc = atof(a.at(5).c_str());
+        } else if (s.type == 3) {
+            s.e1 = eidx(a.at(2));
+            s.e2 = eidx(a.at(3));
+            s.eta = atof(a.at(4).c_str());
+            s.lambda = atof(a.at(5).c_str());
+            s.zeta = atof(a.at(6).c_str());
+            s.rc = atof(a.at(7).c_str());
+            if (a.size() > 8) s.rs = atof(a.at(8).c_str());
+            if (s.e1 > s.e2) std::swap(s.e1, s.e2);
+            s.zetaInt = (int)std::round(s.zeta);
+            s.useIntPow = fabs(s.zeta - s.zetaInt) <= std::numeric_limits<double>::min();
+        } else {
+            throw std::runtime_error(fmt("symmetry function type %d not supported", s.type));
+        }
+        c.el.at(s.ec).sfs.push_back(s);
+    }
+    c.maxCutoff = 0.0;
+    for (auto& e : c.el) {
+        if (c.normalize) {  // SymFnc*::changeLengthUnit
+            for (auto& s : e.sfs) {
+                s.eta /= c.convLength * c.convLength;
+                s.rs *= c.convLength;
+                s.rc *= c.convLength;
+            }
+        }
+        std::sort(e.sfs.begin(), e.sfs.end(), sf_less);
+        for (size_t i = 0; i < e.sfs.size(); ++i) {
+            e.sfs[i].index = (int)i;
+            c.maxCutoff = std::max(c.maxCutoff, e.sfs[i].rc);
+        }
+        // Element::setupSymmetryFunctionMemory (Element.cpp:289-311).
+        e.table.assign(c.el.size(), {});
+        for (auto& s : e.sfs) {
+            s.indexPerElement.assign(c.el.size(), -1);
+            for (size_t ne = 0; ne < c.el.size(); ++ne)
+                if (s.relevant((int)ne)) {
+                    s.indexPerElement[ne] = (int)e.table[ne].size();
+                    e.table[ne].push_back(s.index);
+                }
+        }
+        // Element::setupSymmetryFunctionGroups (Element.cpp:194-287).
+        for (auto& s : e.sfs) {
+            bool added = false;
+            for (auto& g : e.grps) {
+                if (g.type == s.type && g.rc == s.rc && g.e1 == s.e1 &&
+                    (s.type == 2 || g.e2 == s.e2)) {
+                    g.mem.push_back(s.index);
+                    added = true;
+                    break;
+                }
+            }
+            if (!added) {
+                Group g;
+                g.type = s.type; g.ec = s.ec; g.e1 = s.e1; g.e2 = s.e2; g.rc = s.rc;
+                g.fc.set(c.cutoffType, s.rc, c.cutoffAlpha);
+                g.mem.push_back(s.index);
+                e.grps.push_back(g);
+            }
+        }
+        // Members are in SF order already (SF sort refines the member sort).
+        for (auto& g : e.grps) {
+            if (g.type != 3) continue;
+            for (size_t i = 0; i < g.mem.size(); ++i) {
+                const SF& s = e.sfs[g.mem[i]];
+                g.factorNorm.push_back(pow(2.0, 1.0 - s.zeta));
+                g.factorDeriv.push_back(2.0 * s.eta / s.zeta / s.lambda);
+                g.zetaLambda.push_back(s.zeta * s.lambda);
+                const SF* prev = i ? &e.sfs[g.mem[i - 1]] : nullptr;
+                g.calcExp.push_back(!prev || prev->eta != s.eta || prev->rs != s.rs);
+            }
+        }
+    }
+    // Mode::setupNeuralNetwork (Mode.cpp:1227-1512), 2G, global keywords.
+    int nh = atoi(get("global_hidden_layers_short").c_str());
+    vector<string> nodes = words(get("global_nodes_short"));
+    vector<string> acts = words(get("global_activation_short"));
+    if ((int)nodes.size() != nh || (int)acts.size() != nh + 1)
+        throw std::runtime_error("inconsistent global NN topology");
+    for (auto& e : c.el) {
+        e.nn.n.assign(nh + 2, 0);
+        e.nn.act.assign(nh + 2, AF_IDENTITY);
+        e.nn.n[0] = (int)e.sfs.size();
+        for (int l = 1; l <= nh; ++l) {
+            e.nn.n[l] = atoi(nodes[l - 1].c_str());
+            e.nn.act[l] = act_from(acts[l - 1]);
+        }
+        e.nn.n[nh + 1] = 1;
+        e.nn.act[nh + 1] = act_from(acts[nh]);
+        e.nn.normalize = has("normalize_nodes");
+    }
+    for (auto it = kv.lower_bound("element_nodes_short"); it != kv.upper_bound("element_nodes_short"); ++it) {
+        vector<string> a = words(it->second);
+        Element& e = c.el.at(eidx(a.at(0)));
+        if ((int)a.size() - 1 != nh) throw std::runtime_error("inconsistent per-element NN topology");
+        for (int l = 1; l <= nh; ++l) e.nn.n[l] = atoi(a.at(l).c_str());
+    }
+    // Mode::setupSymmetryFunctionScaling (Mode.cpp:781-909).
+    int st = ST_NONE;
+    bool sc = has("scale_symmetry_functions"), ce = has("center_symmetry_functions");
+    if (sc && !ce) st = ST_SCALE;
+    else if (!sc && ce) st = ST_CENTER;
+    else if (sc && ce) st = ST_SCALECENTER;
+    else if (has("scale_symmetry_functions_sigma")) st = ST_SCALESIGMA;
+    double Smin = 0.0, Smax = 0.0;
+    if (st == ST_SCALE || st == ST_SCALECENTER || st == ST_SCALESIGMA) {
+        Smin = has("scale_min_short") ? atof(get("scale_min_short").c_str()) : 0.0;
+        Smax = has("scale_max_short") ? atof(get("scale_max_short").c_str()) : 1.0;
+    }
+    std::ifstream sf(dir + "/scaling.data");
+    if (!sf) throw std::runtime_error("could not open " + dir + "/scaling.data");
+    vector<string> lines;
+    while (std::getline(sf, line))
+        if (!line.empty() && line[0] != '#') lines.push_back(line);
+    size_t off = 0;
+    for (auto& e : c.el) {
+        for (auto& s : e.sfs) {  // SymFnc::setScalingType (SymFnc.cpp:52-98)
+            vector<string> a = words(lines.at(off + s.index));
+            if ((size_t)atoi(a.at(0).c_str()) != (size_t)s.ec + 1 &&
+                (size_t)atoi(a.at(1).c_str()) != (size_t)s.index + 1)
+                throw std::runtime_error("inconsistent scaling statistics");
+            s.scalingType = st;
+            s.Gmin = atof(a.at(2).c_str());
+            s.Gmax = atof(a.at(3).c_str());
+            s.Gmean = atof(a.at(4).c_str());
+            if (a.size() > 5) s.Gsigma = atof(a.at(5).c_str());
+            s.Smin = Smin;
+            s.Smax = Smax;
+            if (st == ST_SCALE || st == ST_SCALECENTER) s.scalingFactor = (Smax - Smin) / (s.Gmax - s.Gmin);
+            else if (st == ST_SCALESIGMA) s.scalingFactor = (Smax - Smin) / s.Gsigma;
+            else s.scalingFactor = 1.0;
+        }
+        off += e.sfs.size();
+        for (auto& g : e.grps) {  // SymGrp*::setScalingFactors
+            g.scalingFactors.clear();
+            for (size_t i = 0; i < g.mem.size(); ++i) {
+                g.scalingFactors.push_back(e.sfs[g.mem[i]].scalingFactor);
+                if (g.type == 3) g.factorNorm[i] *= g.scalingFactors[i];
+            }
+        }
+    }
+    // Mode::readNeuralNetworkWeights (Mode.cpp:2331-2350),
+    // readColumnsFromFile (utility.cpp:247-290), column 0.
+    for (auto& e : c.el) {
+        string fn = dir + fmt("/weights.%03d.data", e.Z);
+        std::ifstream wf(fn);
+        if (!wf) throw std::runtime_error("could not open " + fn);
+        vector<double> w;
+        while (std::getline(wf, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            vector<string> a = words(line);
+            w.push_back(a.empty() ? std::numeric_limits<double>::quiet_NaN() : atof(a[0].c_str()));
+        }
+        if (w.size() < e.nn.numConnections())
+            throw std::runtime_error("too few weights in " + fn);
+        e.nn.setConnections(w);
+    }
+}
+
+// ------------------------------------------------------------- structure --
+// Structure::calculateInverseBox (Structure.cpp:518-544).
+void inverse_box(oracle_ctx& c)
+{
+    double (&b)[3][3] = c.box;
+    double (&ib)[3][3] = c.invbox;
+    double invdet = b[0][0] * b[1][1] * b[2][2] + b[1][0] * b[2][1] * b[0][2] +
+                    b[2][0] * b[0][1] * b[1][2] - b[2][0] * b[1][1] * b[0][2] -
+                    b[0][0] * b[2][1] * b[1][2] - b[1][0] * b[0][1] * b[2][2];
+    invdet = 1.0 / invdet;
+    ib[0][0] = b[1][1] * b[2][2] - b[2][1] * b[1][2];
+    ib[0][1] = b[2][1] * b[0][2] - b[0][1] * b[2][2];
+    ib[0][2] = b[0][1] * b[1][2] - b[1][1] * b[0][2];
+    ib[1][0] = b[2][0] * b[1][2] - b[1][0] * b[2][2];
+    ib[1][1] = b[0][0] * b[2][2] - b[2][0] * b[0][2];
+    ib[1][2] = b[1][0] * b[0][2] - b[0][0] * b[1][2];
+    ib[2][0] = b[1][0] * b[2][1] - b[2][0] * b[1][1];
+    ib[2][1] = b[2][0] * b[0][1] - b[0][0] * b[2][1];
+    ib[2][2] = b[0][0] * b[1][1] - b[1][0] * b[0][1];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) ib[i][k] *= invdet;
+}
+
+// Structure::remap (Structure.cpp:1224-1248).
+void remap(oracle_ctx& c, Atom& a)
+{
+    double f[3];
+    for (int k = 0; k < 3; ++k)
+        f[k] = a.r[0] * c.invbox[0][k] + a.r[1] * c.invbox[1][k] + a.r[2] * c.invbox[2][k];
+    for (int k = 0; k < 3; ++k) if (f[k] > 1.0) f[k] -= (int)f[k];
+    for (int k = 0; k < 3; ++k) if (f[k] < 0.0) f[k] += 1.0 - (int)f[k];
+    for (int k = 0; k < 3; ++k) if (f[k] == 1.0) f[k] = 0.0;
+    for (int k = 0; k < 3; ++k)
+        a.r[k] = f[0] * c.box[0][k] + f[1] * c.box[1][k] + f[2] * c.box[2][k];
+}
+
+// Structure::toNormalizedUnits (Structure.cpp:1250-1277) for positions/box.
+void to_normalized(oracle_ctx& c)
+{
+    if (!c.normalize) return;
+    if (c.periodic) {
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) c.box[i][k] *= c.convLength;
+        double const inv = 1.0 / c.convLength;  // Vec3D::operator/=
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) c.invbox[i][k] *= inv;
+    }
+    for (auto& a : c.atoms)
+        for (int k = 0; k < 3; ++k) a.r[k] *= c.convLength;
+}
+
+// Structure::calculatePbcCopies (Structure.cpp:485-516).
+void pbc_copies(const oracle_ctx& c, double rc, int pbc[3])
+{
+    auto cross = [](const double* u, const double* v, double* w) {
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    auto normalize = [](double* w) {
+        double n = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        w[0] /= n; w[1] /= n; w[2] /= n;
+    };
+    auto dot = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    double axb[3], axc[3], bxc[3];
+    cross(c.box[0], c.box[1], axb); normalize(axb);
+    cross(c.box[0], c.box[2], axc); normalize(axc);
+    cross(c.box[1], c.box[2], bxc); normalize(bxc);
+    double proj[3] = {fabs(dot(c.box[0], bxc)), fabs(dot(c.box[1], axc)), fabs(dot(c.box[2], axb))};
+    for (int k = 0; k < 3; ++k) {
+        pbc[k] = 0;
+        while (pbc[k] * proj[k] <= rc) pbc[k]++;
+    }
+}
+
+// Distance vector of the (j, image) pair, in the reference's evaluation order
+// (Structure.cpp:322-325): ((r_i - r_j) + bc0*a) + bc1*b + bc2*c.
+inline void pair_dr(const oracle_ctx& c, const Atom& ai, const Atom& aj, int b0, int b1, int b2, double* dr)
+{
+    for (int k = 0; k < 3; ++k)
+        dr[k] = ai.r[k] - aj.r[k] + b0 * c.box[0][k] + b1 * c.box[1][k] + b2 * c.box[2][k];
+}
+
+void push_neighbor(Atom& ai, int i, int j, const int* bc, int ej, const double* dr)
+{
+    Neighbor n;
+    n.j = j;
+    n.bc[0] = bc[0]; n.bc[1] = bc[1]; n.bc[2] = bc[2];
+    n.element = ej;
+    n.d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+    n.dr[0] = dr[0]; n.dr[1] = dr[1]; n.dr[2] = dr[2];
+    ai.nbr.push_back(n);
+    if (ai.unique.back() != j && i != j) ai.unique.push_back(j);
+}
+
+// Structure::calculateNeighborList (Structure.cpp:287-398), brute force.
+void neighbors_brute(oracle_ctx& c, double rc)
+{
+    const int N = (int)c.atoms.size();
+    int pbc[3] = {0, 0, 0};
+    if (c.periodic) pbc_copies(c, rc, pbc);
+    double const rc2 = rc * rc;
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < N; ++i) {
+        Atom& ai = c.atoms[i];
+        ai.nbr.clear();
+        ai.unique.assign(1, i);
+        for (int j = 0; j < N; ++j) {
+            const Atom& aj = c.atoms[j];
+            for (int b0 = -pbc[0]; b0 <= pbc[0]; ++b0)
+                for (int b1 = -pbc[1]; b1 <= pbc[1]; ++b1)
+                    for (int b2 = -pbc[2]; b2 <= pbc[2]; ++b2) {
+                        if (i == j && b0 == 0 && b1 == 0 && b2 == 0) continue;
+                        double dr[3];
+                        pair_dr(c, ai, aj, b0, b1, b2, dr);
+                        if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2) {
+                            int bc[3] = {b0, b1, b2};
+                            push_neighbor(ai, i, j, bc, aj.element, dr);
+                        }
+                    }
+        }
+    }
+}
+
+// Cell list producing the same (j, image) set in the same order.  Cells are
+// taken in fractional coordinates; each axis has nc_k >= 3 cells of
+// perpendicular width >= rc.  Candidates from the 27 surrounding cells are
+// tested with the same expression and collected, then sorted by
+// (j, bc0, bc1, bc2), which is the brute-force loop order.  Requires
+// positions inside the cell (fractional coordinates in [0, 1)), which
+// Structure::remap guarantees; falls back to brute force otherwise.
+void neighbors_cell(oracle_ctx& c, double rc)
+{
+    const int N = (int)c.atoms.size();
+    if (!c.periodic) { neighbors_brute(c, rc); return; }
+    int pbc[3];
+    pbc_copies(c, rc, pbc);
+    // perpendicular widths: proj_k from the same construction
+    int nc[3];
+    {
+        auto cross = [](const double* u, const double* v, double* w) {
+            w[0] = u[1] * v[2] - u[2] * v[1];
+            w[1] = u[2] * v[0] - u[0] * v[2];
+            w[2] = u[0] * v[1] - u[1] * v[0];
+        };
+        double n[3][3];
+        cross(c.box[1], c.box[2], n[0]);
+        cross(c.box[2], c.box[0], n[1]);
+        cross(c.box[0], c.box[1], n[2]);
+        for (int k = 0; k < 3; ++k) {
+            double len = sqrt(n[k][0] * n[k][0] + n[k][1] * n[k][1] + n[k][2] * n[k][2]);
+            double proj = fabs(c.box[k][0] * n[k][0] + c.box[k][1] * n[k][1] + c.box[k][2] * n[k][2]) / len;
+            nc[k] = (int)floor(proj / (rc * (1.0 + 1e-12) + 1e-12));
+        }
+    }
+    if (nc[0] < 3 || nc[1] < 3 || nc[2] < 3) { neighbors_brute(c, rc); return; }
+    vector<int> cell(N);
+    vector<double> frac(3 * (size_t)N);
+    for (int i = 0; i < N; ++i) {
+        const Atom& a = c.atoms[i];
+        int ci[3];
+        for (int k = 0; k < 3; ++k) {
+            double f = a.r[0] * c.invbox[0][k] + a.r[1] * c.invbox[1][k] + a.r[2] * c.invbox[2][k];
+            if (!(f >= 0.0 && f < 1.0)) { neighbors_brute(c, rc); return; }
+            frac[3 * i + k] = f;
+            ci[k] = std::min((int)(f * nc[k]), nc[k] - 1);
+        }
+        cell[i] = (ci[0] * nc[1] + ci[1]) * nc[2] + ci[2];
+    }
+    const int ncell = nc[0] * nc[1] * nc[2];
+    vector<int> start(ncell + 1, 0), order(N);
+    for (int i = 0; i < N; ++i) start[cell[i] + 1]++;
+    for (int k = 0; k < ncell; ++k) start[k + 1] += start[k];
+    {
+        vector<int> fill(start.begin(), start.end() - 1);
+        for (int i = 0; i < N; ++i) order[fill[cell[i]]++] = i;
+    }
+    double const rc2 = rc * rc;
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < N; ++i) {
+        Atom& ai = c.atoms[i];
+        int ci[3] = {cell[i] / (nc[1] * nc[2]), (cell[i] / nc[2]) % nc[1], cell[i] % nc[2]};
+        struct Cand { long key; int j; int bc[3]; double dr[3]; };
+        vector<Cand> cand;
+        for (int d0 = -1; d0 <= 1; ++d0)
+            for (int d1 = -1; d1 <= 1; ++d1)
+                for (int d2 = -1; d2 <= 1; ++d2) {
+                    int cc[3] = {ci[0] + d0, ci[1] + d1, ci[2] + d2};
+                    int bc[3];
+                    for (int k = 0; k < 3; ++k) {
+                        // neighbour cell wraps: image of j is r_j + m*box,
+                        // i.e. bc = -m in the reference's dr expression
+                        int m = cc[k] < 0 ? -1 : (cc[k] >= nc[k] ? 1 : 0);
+                        cc[k] -= m * nc[k];
+                        bc[k] = -m;
+                    }
+                    int id = (cc[0] * nc[1] + cc[1]) * nc[2] + cc[2];
+                    for (int p = start[id]; p < start[id + 1]; ++p) {
+                        int j = order[p];
+                        if (i == j && bc[0] == 0 && bc[1] == 0 && bc[2] == 0) continue;
+                        double dr[3];
+                        pair_dr(c, ai, c.atoms[j], bc[0], bc[1], bc[2], dr);
+                        if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2) {
+                            Cand x;
+                            x.j = j;
+                            x.bc[0] = bc[0]; x.bc[1] = bc[1]; x.bc[2] = bc[2];
+                            x.dr[0] = dr[0]; x.dr[1] = dr[1]; x.dr[2] = dr[2];
+                            x.key = (((long)j * 3 + (bc[0] + 1)) * 3 + (bc[1] + 1)) * 3 + (bc[2] + 1);
+                            cand.push_back(x);
+                        }
+                    }
+                }
+        std::sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.key < b.key; });
+        ai.nbr.clear();
+        ai.unique.assign(1, i);
+        for (auto& x : cand) push_neighbor(ai, i, x.j, x.bc, c.atoms[x.j].element, x.dr);
+    }
+    (void)pbc;
+}
+
+// ------------------------------------------------------ SF group kernels --
+// SymGrpExpRad::calculate (SymGrpExpRad.cpp:144-239).
+void calc_radial(const Element& e, const Group& g, Atom& a)
+{
+    const size_t M = g.mem.size();
+    vector<double> result(M, 0.0);
+    for (auto& n : a.nbr) {
+        if (!(g.e1 == n.element && n.d < g.rc)) continue;
+        double const rij = n.d;
+        double pfc, pdfc;
+        g.fc.fdf(rij, pfc, pdfc);
+        for (size_t k = 0; k < M; ++k) {
+            const SF& s = e.sfs[g.mem[k]];
+            double const pexp = exp(-s.eta * (rij - s.rs) * (rij - s.rs));
+            result[k] += pexp * pfc;
+            double const p1 = g.scalingFactors[k] * (pdfc - 2.0 * s.eta * (rij - s.rs) * pfc) * pexp / rij;
+            double const px = p1 * n.dr[0], py = p1 * n.dr[1], pz = p1 * n.dr[2];
+            double* d = &a.dGdr[3 * s.index];
+            d[0] += px; d[1] += py; d[2] += pz;
+            d = &n.dGdr[3 * s.indexPerElement[g.e1]];
+            d[0] -= px; d[1] -= py; d[2] -= pz;
+        }
+    }
+    for (size_t k = 0; k < M; ++k) {
+        const SF& s = e.sfs[g.mem[k]];
+        a.G[s.index] = s.scale(result[k]);
+    }
+}
+
+// pow_int (utility.cpp:292-316): exponentiation by squaring.
+double pow_int(double x, int n)
+{
+    unsigned int m;
+    if (n < 0) { x = 1.0 / x; m = -n; }
+    else m = n;
+    double r = 1.0;
+    do {
+        if (m & 1) r *= x;
+        m >>= 1;
+        x *= x;
+    } while (m);
+    return r;
+}
+
+// SymGrpExpAngn::calculate (SymGrpExpAngn.cpp:156-403).
+void calc_angular(const Element& e, const Group& g, Atom& a)
+{
+    const size_t M = g.mem.size();
+    vector<double> result(M, 0.0);
+    double const rc = g.rc, rc2 = rc * rc;
+    size_t nn = a.nbr.size();
+    if (nn == 0) nn = 1;
+    for (size_t j = 0; j + 1 < nn; ++j) {
+        Neighbor& nj = a.nbr[j];
+        int const nej = nj.element;
+        double const rij = nj.d;
+        if (!((g.e1 == nej || g.e2 == nej) && rij < rc)) continue;
+        double const r2ij = rij * rij;
+        double pfcij, pdfcij;
+        g.fc.fdf(rij, pfcij, pdfcij);
+        const double* dr1 = nj.dr;
+        for (size_t k = j + 1; k < nn; ++k) {
+            Neighbor& nk = a.nbr[k];
+            int const nek = nk.element;
+            if (!((g.e1 == nej && g.e2 == nek) || (g.e2 == nej && g.e1 == nek))) continue;
+            double const rik = nk.d;
+            if (!(rik < rc)) continue;
+            const double* dr2 = nk.dr;
+            double dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1], dr2[2] - dr1[2]};
+            double rjk = dr3[0] * dr3[0] + dr3[1] * dr3[1] + dr3[2] * dr3[2];
+            if (!(rjk < rc2)) continue;
+            double pfcik, pdfcik;
+            g.fc.fdf(rik, pfcik, pdfcik);
+            rjk = sqrt(rjk);
+            double pfcjk, pdfcjk;
+            g.fc.fdf(rjk, pfcjk, pdfcjk);
+            double const rinvijik = 1.0 / rij / rik;
+            double const costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
+            double const pfc = pfcij * pfcik * pfcjk;
+            double const r2ik = rik * rik;
+            double const r2sum = r2ij + r2ik + rjk * rjk;
+            double const pr1 = pfcik * pfcjk * pdfcij / rij;
+            double const pr2 = pfcij * pfcjk * pdfcik / rik;
+            double const pr3 = pfcij * pfcik * pdfcjk / rjk;
+            double vexp = 0.0, rijs = 0.0, riks = 0.0, rjks = 0.0;
+            for (size_t l = 0; l < M; ++l) {
+                const SF& s = e.sfs[g.mem[l]];
+                if (g.calcExp[l]) {
+                    if (s.rs > 0.0) {
+                        rijs = rij - s.rs;
+                        riks = rik - s.rs;
+                        rjks = rjk - s.rs;
+                        vexp = exp(-s.eta * (rijs * rijs + riks * riks + rjks * rjks));
+                    } else {
+                        vexp = exp(-s.eta * r2sum);
+                    }
+                }
+                double const plambda = 1.0 + s.lambda * costijk;
+                double fg = vexp;
+                if (plambda <= 0.0) fg = 0.0;
+                else if (s.useIntPow) fg *= pow_int(plambda, s.zetaInt - 1);
+                else fg *= pow(plambda, s.zeta - 1.0);
+                result[l] += fg * plambda * pfc;
+                fg *= g.factorNorm[l];
+                double const pfczl = pfc * g.zetaLambda[l];
+                double const p2etapl = plambda * g.factorDeriv[l];
+                double p1, p2, p3;
+                if (s.rs > 0.0) {
+                    p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl * rijs / rij) + pr1 * plambda);
+                    p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl * riks / rik) + pr2 * plambda);
+                    p3 = fg * (pfczl * (rinvijik + p2etapl * rjks / rjk) - pr3 * plambda);
+                } else {
+                    p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl) + pr1 * plambda);
+                    p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl) + pr2 * plambda);
+                    p3 = fg * (pfczl * (rinvijik + p2etapl) - pr3 * plambda);
+                }
+                double const p1x = p1 * dr1[0], p1y = p1 * dr1[1], p1z = p1 * dr1[2];
+                double const p2x = p2 * dr2[0], p2y = p2 * dr2[1], p2z = p2 * dr2[2];
+                double const p3x = p3 * dr3[0], p3y = p3 * dr3[1], p3z = p3 * dr3[2];
+                double* d = &a.dGdr[3 * s.index];
+                d[0] += p1x + p2x; d[1] += p1y + p2y; d[2] += p1z + p2z;
+                d = &nj.dGdr[3 * s.indexPerElement[nej]];
+                d[0] -= p1x + p3x; d[1] -= p1y + p3y; d[2] -= p1z + p3z;
+                d = &nk.dGdr[3 * s.indexPerElement[nek]];
+                d[0] -= p2x - p3x; d[1] -= p2y - p3y; d[2] -= p2z - p3z;
+            }
+        }
+    }
+    for (size_t l = 0; l < M; ++l) {
+        const SF& s = e.sfs[g.mem[l]];
+        result[l] *= g.factorNorm[l] / g.scalingFactors[l];
+        a.G[s.index] = s.scale(result[l]);
+    }
+}
+
+// Mode::calculateSymmetryFunctionGroups (Mode.cpp:1630-1709) with
+// Atom::allocate (Atom.cpp:168-240).
+void calc_sf(oracle_ctx& c)
+{
+    const int N = (int)c.atoms.size();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < N; ++i) {
+        Atom& a = c.atoms[i];
+        const Element& e = c.el[a.element];
+        const size_t ns = e.sfs.size();
+        a.G.assign(ns, 0.0);
+        a.dEdG.assign(ns, 0.0);
+        a.dGdr.assign(3 * ns, 0.0);
+        for (auto& n : a.nbr) n.dGdr.assign(3 * e.table[n.element].size(), 0.0);
+        for (auto& g : e.grps) {
+            if (g.type == 2) calc_radial(e, g, a);
+            else calc_angular(e, g, a);
+        }
+    }
+}
+
+// Mode::calculateAtomicNeuralNetworks (Mode.cpp:1711-1732) with
+// NeuralNetwork::propagateLayer (NeuralNetwork.cpp:761-896) and the
+// forward-mode NeuralNetwork::calculateDEdG (NeuralNetwork.cpp:396-442).
+void calc_nn(oracle_ctx& c)
+{
+    const int N = (int)c.atoms.size();
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        Atom& a = c.atoms[i];
+        const NN& nn = c.el[a.element].nn;
+        const int L = (int)nn.n.size();
+        vector<vector<double>> val(L), dfdx(L);
+        val[0] = a.G;
+        for (int l = 1; l < L; ++l) {
+            val[l].assign(nn.n[l], 0.0);
+            dfdx[l].assign(nn.n[l], 0.0);
+            for (int k = 0; k < nn.n[l]; ++k) {
+                double t = 0.0;
+                const double* w = &nn.W[l][(size_t)k * nn.n[l - 1]];
+                for (int j = 0; j < nn.n[l - 1]; ++j) t += w[j] * val[l - 1][j];
+                t += nn.b[l][k];
+                if (nn.normalize) t /= nn.n[l - 1];
+                val[l][k] = activate(nn.act[l], t, dfdx[l][k]);
+            }
+        }
+        a.energy = val[L - 1][0];
+        const int nh = L - 2;
+        vector<vector<double>> inner(nh), outer(nh);
+        for (int l = 0; l < nh; ++l) {
+            inner[l].assign(nn.n[l + 1], 0.0);
+            outer[l].assign(nn.n[l + 2], 0.0);
+        }
+        for (int k = 0; k < nn.n[0]; ++k) {
+            for (int q = 0; q < nn.n[1]; ++q) {
+                inner[0][q] = nn.W[1][(size_t)q * nn.n[0] + k] * dfdx[1][q];
+                if (nn.normalize) inner[0][q] /= nn.n[0];
+            }
+            for (int l = 1; l < nh + 1; ++l) {
+                for (int i2 = 0; i2 < nn.n[l + 1]; ++i2) {
+                    outer[l - 1][i2] = 0.0;
+                    for (int i1 = 0; i1 < nn.n[l]; ++i1)
+                        outer[l - 1][i2] += nn.W[l + 1][(size_t)i2 * nn.n[l] + i1] * inner[l - 1][i1];
+                    outer[l - 1][i2] *= dfdx[l + 1][i2];
+                    if (nn.normalize) outer[l - 1][i2] /= nn.n[l];
+                    if (l < nh) inner[l][i2] = outer[l - 1][i2];
+                }
+            }
+            a.dEdG[k] = outer[nh - 1][0];
+        }
+    }
+}
+
+// Mode::calculateForces (Mode.cpp:1922-1984), Atom::calculateSelfForceShort
+// and calculatePairForceShort (Atom.cpp:371-401).
+void calc_forces_pull(oracle_ctx& c)
+{
+    const int N = (int)c.atoms.size();
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        Atom& ai = c.atoms[i];
+        double f[3] = {0.0, 0.0, 0.0};
+        double s[3] = {0.0, 0.0, 0.0};
+        for (size_t q = 0; q < ai.dEdG.size(); ++q)
+            for (int k = 0; k < 3; ++k) s[k] -= ai.dEdG[q] * ai.dGdr[3 * q + k];
+        for (int k = 0; k < 3; ++k) f[k] += s[k];
+        for (size_t u = 1; u < ai.unique.size(); ++u) {
+            const Atom& aj = c.atoms[ai.unique[u]];
+            const vector<vector<int>>& table = c.el[aj.element].table;
+            for (auto& n : aj.nbr) {
+                if (n.j != i) continue;
+                const vector<int>& t = table[n.element];
+                double p[3] = {0.0, 0.0, 0.0};
+                for (size_t q = 0; q < t.size(); ++q)
+                    for (int k = 0; k < 3; ++k) p[k] -= aj.dEdG[t[q]] * n.dGdr[3 * q + k];
+                for (int k = 0; k < 3; ++k) f[k] += p[k];
+            }
+        }
+        for (int k = 0; k < 3; ++k) ai.f[k] = f[k];
+    }
+}
+
+// InterfaceLammps::getForces (InterfaceLammps.cpp:679-740) without unit
+// factors: every neighbour entry, self images included, scatters.
+void calc_forces_push(oracle_ctx& c)
+{
+    for (auto& a : c.atoms) a.f[0] = a.f[1] = a.f[2] = 0.0;
+    for (size_t i = 0; i < c.atoms.size(); ++i) {
+        Atom& a = c.atoms[i];
+        const vector<vector<int>>& table = c.el[a.element].table;
+        for (auto& n : a.nbr) {
+            const vector<int>& t = table[n.element];
+            Atom& b = c.atoms[n.j];
+            for (size_t q = 0; q < t.size(); ++q)
+                for (int k = 0; k < 3; ++k) b.f[k] -= a.dEdG[t[q]] * n.dGdr[3 * q + k];
+        }
+        for (size_t q = 0; q < a.dEdG.size(); ++q)
+            for (int k = 0; k < 3; ++k) a.f[k] -= a.dEdG[q] * a.dGdr[3 * q + k];
+    }
+}
+
+double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+void set_box(oracle_ctx& c, const double* box)
+{
+    c.periodic = box != nullptr;
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) c.box[i][k] = box ? box[3 * i + k] : 0.0;
+    if (c.periodic) inverse_box(c);
+}
+
+}  // namespace
+
+// ==================================================================== C API
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+#define ORACLE_TRY(...)                                   \
+    try { __VA_ARGS__ } catch (const std::exception& ex) { \
+        g_err = ex.what(); return 1; }
+
+oracle_ctx* oracle_create(const char* model_dir)
+{
+    oracle_ctx* c = new oracle_ctx();
+    try {
+        load_model(*c, model_dir);
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        delete c;
+        return nullptr;
+    }
+    return c;
+}
+
+void oracle_destroy(oracle_ctx* c) { delete c; }
+
+int oracle_num_elements(const oracle_ctx* c) { return (int)c->el.size(); }
+int oracle_num_sf(const oracle_ctx* c, int e) { return (int)c->el.at(e).sfs.size(); }
+int oracle_atomic_number(const oracle_ctx* c, int e) { return c->el.at(e).Z; }
+double oracle_max_cutoff(const oracle_ctx* c) { return c->maxCutoff; }
+int oracle_normalize(const oracle_ctx* c) { return c->normalize ? 1 : 0; }
+
+// Structure::readFromLines (Structure.cpp:142-271) for the first structure,
+// then Prediction::readStructureFromFile (Prediction.cpp:50-65).
+int oracle_read_structure(oracle_ctx* c, const char* path)
+{
+    ORACLE_TRY({
+        std::ifstream in(path);
+        if (!in) throw std::runtime_error(string("could not open ") + path);
+        string line;
+        vector<double> pos, box;
+        vector<int> elem;
+        bool begun = false;
+        while (std::getline(in, line)) {
+            vector<string> w = words(line);
+            if (w.empty()) continue;
+            if (w[0] == "begin") { begun = true; continue; }
+            if (!begun) throw std::runtime_error("expected \"begin\"");
+            if (w[0] == "lattice") {
+                for (int k = 1; k <= 3; ++k) box.push_back(atof(w.at(k).c_str()));
+            } else if (w[0] == "atom") {
+                for (int k = 1; k <= 3; ++k) pos.push_back(atof(w.at(k).c_str()));
+                int e = -1;
+                for (size_t q = 0; q < c->el.size(); ++q) if (c->el[q].symbol == w.at(4)) e = (int)q;
+                if (e < 0) throw std::runtime_error("unknown element " + w.at(4));
+                elem.push_back(e);
+            } else if (w[0] == "end") {
+                break;
+            }
+        }
+        if (!(box.empty() || box.size() == 9)) throw std::runtime_error("strange number of box vectors");
+        int rc = oracle_set_structure(c, (int)elem.size(), pos.data(), elem.data(), box.empty() ? nullptr : box.data());
+        if (rc) return rc;
+    })
+    return 0;
+}
+
+int oracle_set_structure(oracle_ctx* c, int n, const double* pos, const int* elem, const double* box)
+{
+    ORACLE_TRY({
+        c->atoms.assign(n, Atom());
+        set_box(*c, box);
+        for (int i = 0; i < n; ++i) {
+            if (elem[i] < 0 || elem[i] >= (int)c->el.size()) throw std::runtime_error("element index out of range");
+            c->atoms[i].element = elem[i];
+            for (int k = 0; k < 3; ++k) c->atoms[i].r[k] = pos[3 * i + k];
+            if (c->periodic) remap(*c, c->atoms[i]);
+        }
+        to_normalized(*c);
+    })
+    return 0;
+}
+
+long oracle_build_neighbors(oracle_ctx* c, int nbr_mode, int nthreads)
+{
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    try {
+        if (nbr_mode == ORACLE_NBR_CELL) neighbors_cell(*c, c->maxCutoff);
+        else neighbors_brute(*c, c->maxCutoff);
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+    long tot = 0;
+    for (auto& a : c->atoms) tot += (long)a.nbr.size();
+    return tot;
+}
+
+// Prediction::predict (Prediction.cpp:67-78) -> Mode::evaluateNNP
+// (Mode.cpp:2029-2077), then toPhysicalUnits and energy offsets.
+int oracle_predict(oracle_ctx* c, int force_mode, int nbr_mode, int nthreads)
+{
+    ORACLE_TRY({
+#ifdef _OPENMP
+        if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+        double t0 = now();
+        if (nbr_mode == ORACLE_NBR_CELL) neighbors_cell(*c, c->maxCutoff);
+        else neighbors_brute(*c, c->maxCutoff);
+        double t1 = now();
+        calc_sf(*c);
+        double t2 = now();
+        calc_nn(*c);
+        double t3 = now();
+        double E = 0.0;  // Mode::calculateEnergy (Mode.cpp:1876-1901)
+        for (auto& a : c->atoms) E += a.energy;
+        c->energy = E;
+        double t4 = now();
+        if (force_mode == ORACLE_FORCES_PUSH) calc_forces_push(*c);
+        else calc_forces_pull(*c);
+        double t5 = now();
+        c->phase[0] = t1 - t0; c->phase[1] = t2 - t1; c->phase[2] = t3 - t2;
+        c->phase[3] = t4 - t3; c->phase[4] = t5 - t4;
+        // Structure::toPhysicalUnits (Structure.cpp:1279-1306),
+        // Atom::toPhysicalUnits (Atom.cpp:123-150).
+        const size_t N = c->atoms.size();
+        if (c->normalize) {
+            c->energy = c->energy / c->convEnergy + N * c->meanEnergy;
+            double const ff = 1.0 / (c->convEnergy / c->convLength);
+            double const fr = 1.0 / c->convLength;
+            for (auto& a : c->atoms)
+                for (int k = 0; k < 3; ++k) { a.f[k] *= ff; a.r[k] *= fr; }
+            if (c->periodic) {
+                double const inv = 1.0 / c->convLength;
+                for (int i = 0; i < 3; ++i)
+                    for (int k = 0; k < 3; ++k) { c->box[i][k] *= inv; c->invbox[i][k] *= c->convLength; }
+            }
+        }
+        // Mode::addEnergyOffset (Mode.cpp:2079-2096).
+        for (size_t e = 0; e < c->el.size(); ++e) {
+            size_t cnt = 0;
+            for (auto& a : c->atoms) cnt += (a.element == (int)e);
+            c->energy += cnt * c->el[e].energyOffset;
+        }
+    })
+    return 0;
+}
+
+int oracle_num_atoms(const oracle_ctx* c) { return (int)c->atoms.size(); }
+double oracle_energy(const oracle_ctx* c) { return c->energy; }
+
+int oracle_get_forces(const oracle_ctx* c, double* f)
+{
+    for (size_t i = 0; i < c->atoms.size(); ++i)
+        for (int k = 0; k < 3; ++k) f[3 * i + k] = c->atoms[i].f[k];
+    return 0;
+}
+
+int oracle_get_atomic_energies(const oracle_ctx* c, double* e)
+{
+    for (size_t i = 0; i < c->atoms.size(); ++i) e[i] = c->atoms[i].energy;
+    return 0;
+}
+
+int oracle_get_positions(const oracle_ctx* c, double* r)
+{
+    for (size_t i = 0; i < c->atoms.size(); ++i)
+        for (int k = 0; k < 3; ++k) r[3 * i + k] = c->atoms[i].r[k];
+    return 0;
+}
+
+int oracle_get_G(const oracle_ctx* c, double* G, int ld)
+{
+    for (size_t i = 0; i < c->atoms.size(); ++i)
+        for (size_t s = 0; s < c->atoms[i].G.size() && (int)s < ld; ++s) G[i * ld + s] = c->atoms[i].G[s];
+    return 0;
+}
+
+int oracle_get_dEdG(const oracle_ctx* c, double* d, int ld)
+{
+    for (size_t i = 0; i < c->atoms.size(); ++i)
+        for (size_t s = 0; s < c->atoms[i].dEdG.size() && (int)s < ld; ++s) d[i * ld + s] = c->atoms[i].dEdG[s];
+    return 0;
+}
+
+int oracle_num_neighbors(const oracle_ctx* c, int i) { return (int)c->atoms.at(i).nbr.size(); }
+
+int oracle_get_neighbors(const oracle_ctx* c, int i, int* j, int* bc)
+{
+    const Atom& a = c->atoms.at(i);
+    for (size_t q = 0; q < a.nbr.size(); ++q) {
+        j[q] = a.nbr[q].j;
+        for (int k = 0; k < 3; ++k) bc[3 * q + k] = a.nbr[q].bc[k];
+    }
+    return 0;
+}
+
+double oracle_phase_time(const oracle_ctx* c, int p) { return (p >= 0 && p < 5) ? c->phase[p] : 0.0; }
+
+int oracle_cutoff_fdf(int type, double rc, double alpha, double r, double* fc, double* dfc)
+{
+    ORACLE_TRY({
+        Cutoff f;
+        f.set(type, rc, alpha);
+        f.fdf(r, *fc, *dfc);
+    })
+    return 0;
+}
+
+}  // extern "C"
