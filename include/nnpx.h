@@ -1,0 +1,172 @@
+/*
+ * nnpx -- MI355X (gfx950) Behler-Parrinello (2G-HDNNP) energy/force path.
+ *
+ * C ABI of libnnpx.so.  Plain pointers and sizes only; every entry point
+ * returns an nnpx_status and never throws.  One context drives one GPU on
+ * one host thread (HIP stream per context).  Citations refer to the n2p2
+ * sources the entry point stands in for (src/libnnp/...).
+ *
+ * Units: positions, box vectors, energies and forces are in the model's
+ * internal units -- the same units nnp::Mode's calculate* functions see
+ * (normalised units when input.nn holds mean_energy/conv_energy/conv_length,
+ * Structure::toNormalizedUnits, Structure.cpp:1250-1277).  nnpx_model_units
+ * returns the conversion constants so a caller can convert exactly as
+ * Prediction::readStructureFromFile / Prediction::predict do.
+ */
+#ifndef NNPX_H
+#define NNPX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nnpx_ctx nnpx_ctx;
+
+typedef enum {
+    NNPX_OK = 0,
+    NNPX_ERR_ARGUMENT = 1,   /* std::invalid_argument in the reference      */
+    NNPX_ERR_FILE = 2,       /* runtime_error "Could not open file"         */
+    NNPX_ERR_MODEL = 3,      /* runtime_error from setup (bad input.nn ...)  */
+    NNPX_ERR_RANGE = 4,      /* std::range_error / out_of_range             */
+    NNPX_ERR_STATE = 5,      /* call order violated (no model, no structure) */
+    NNPX_ERR_HIP = 6,        /* HIP runtime failure                         */
+    NNPX_ERR_UNSUPPORTED = 7 /* model feature outside the supported subset  */
+} nnpx_status;
+
+/* Precision forms. */
+typedef enum {
+    NNPX_PLAIN_FP64 = 0, /* FP64 throughout, stored dG/dr (reference layout) */
+    NNPX_MIXED = 1       /* FP32 SF (compensated), BF16 MFMA NN, fused
+                            recomputing force kernel, FP64 reductions        */
+} nnpx_precision;
+
+/* Force semantics. */
+typedef enum {
+    NNPX_FORCES_PULL = 0, /* nnp::Mode::calculateForces, Mode.cpp:1922-1984:
+                             contributions of an atom's own periodic images
+                             are not added (loop starts at neighborsUnique+1) */
+    NNPX_FORCES_PUSH = 1  /* InterfaceLammps::getForces,
+                             InterfaceLammps.cpp:679-740: every neighbour
+                             entry scatters, self images included            */
+} nnpx_force_mode;
+
+/* Options for nnpx_set_option. */
+typedef enum {
+    NNPX_OPT_FORCE_MODE = 0,  /* nnpx_force_mode, default NNPX_FORCES_PULL  */
+    NNPX_OPT_TIMING = 1,      /* 1: record HIP events around every kernel   */
+    NNPX_OPT_NEIGHBOR_ALGO = 2 /* 0 auto (cell list where every axis holds
+                                 >= 3 cells), 1 brute force, 2 cell list    */
+} nnpx_option;
+
+/* Flags for nnpx_evaluate. */
+#define NNPX_EVAL_FORCES 1u  /* compute forces (else energy + dE/dG only) */
+
+/* ---- context ---------------------------------------------------------- */
+/* No counterpart: creates the HIP stream and device state of one Mode. */
+nnpx_status nnpx_create(nnpx_ctx** ctx, int device);
+nnpx_status nnpx_destroy(nnpx_ctx* ctx);
+/* Last error message of this context (or of the last failed create). */
+const char* nnpx_last_error(const nnpx_ctx* ctx);
+nnpx_status nnpx_set_option(nnpx_ctx* ctx, int option, int64_t value);
+nnpx_status nnpx_synchronize(nnpx_ctx* ctx);
+
+/* ---- model ------------------------------------------------------------ */
+/* Replaces Prediction::setup (Prediction.cpp:36-48): Mode::loadSettingsFile,
+ * Mode::setupGeneric (Mode.cpp:213-237), setupSymmetryFunctionScaling
+ * (Mode.cpp:781-909), setupNeuralNetworkWeights (Mode.cpp:1514-1547).
+ * dir holds input.nn, scaling.data and weights.%03d.data. */
+nnpx_status nnpx_load_model(nnpx_ctx* ctx, const char* dir);
+/* Mode::getNumElements / Element::numSymmetryFunctions /
+ * Mode::getMaxCutoffRadius (Mode.h). */
+nnpx_status nnpx_model_info(const nnpx_ctx* ctx, int* num_elements,
+                            int* max_sf, double* max_cutoff);
+nnpx_status nnpx_num_sf(const nnpx_ctx* ctx, int element, int* num_sf);
+/* Atomic number of element index e (ElementMap::atomicNumber). */
+nnpx_status nnpx_atomic_number(const nnpx_ctx* ctx, int element, int* z);
+/* Element index of a symbol (ElementMap::operator[]). */
+nnpx_status nnpx_element_index(const nnpx_ctx* ctx, const char* symbol, int* element);
+/* normalize flag, mean_energy, conv_energy, conv_length and the per-element
+ * atom_energy offsets (Mode.cpp:239-300, 2079-2096); offsets has
+ * num_elements entries. */
+nnpx_status nnpx_model_units(const nnpx_ctx* ctx, int* normalize,
+                             double* mean_energy, double* conv_energy,
+                             double* conv_length, double* offsets);
+
+/* ---- structures ------------------------------------------------------- */
+/* Replaces filling nnp::Structure (Structure::readFromLines,
+ * Structure.cpp:142-271) for a batch of structures packed back to back.
+ *   atom_offsets[n_struct+1]: first atom of each structure
+ *   pos[3*N]: positions, elem[N]: element indices (order of atomic number)
+ *   box[9*n_struct]: lattice vectors as rows (ignored where !periodic)
+ *   periodic[n_struct]: 1 periodic, 0 isolated
+ * Positions of periodic structures are used as given (callers remap, as
+ * Structure::remap does at read time).  Data is copied to device memory. */
+nnpx_status nnpx_set_structures(nnpx_ctx* ctx, int n_struct,
+                                const int64_t* atom_offsets,
+                                const double* pos, const int* elem,
+                                const double* box, const int* periodic);
+/* Overwrite positions only (same atoms, same boxes), e.g. for finite
+ * differences or MD driven from the host. */
+nnpx_status nnpx_update_positions(nnpx_ctx* ctx, const double* pos);
+
+/* ---- compute (the nnp::Mode boundary) --------------------------------- */
+/* Structure::calculateNeighborList(maxCutoffRadius, false)
+ * (Structure.cpp:287-398): same (j, image) set, same order. */
+nnpx_status nnpx_calculate_neighbor_list(nnpx_ctx* ctx);
+/* Mode::calculateSymmetryFunctionGroups (Mode.cpp:1630-1709), FP64.
+ * derivatives != 0 also stores dG/dr in the reference layout
+ * (Atom::dGdr, Atom::Neighbor::dGdr). */
+nnpx_status nnpx_calculate_symmetry_function_groups(nnpx_ctx* ctx, int derivatives);
+/* Mode::calculateAtomicNeuralNetworks (Mode.cpp:1711-1732), FP64;
+ * derivatives != 0 also computes dE/dG (adjoint). */
+nnpx_status nnpx_calculate_atomic_neural_networks(nnpx_ctx* ctx, int derivatives);
+/* Mode::calculateEnergy (Mode.cpp:1876-1901): per-structure sums. */
+nnpx_status nnpx_calculate_energy(nnpx_ctx* ctx);
+/* Mode::calculateForces (Mode.cpp:1922-1984) with the force mode option. */
+nnpx_status nnpx_calculate_forces(nnpx_ctx* ctx);
+/* Mode::evaluateNNP (Mode.cpp:2029-2077) in either precision form:
+ * neighbour list, SF, NN, energy and (flag NNPX_EVAL_FORCES) forces.
+ * Asynchronous on the context's stream; getters synchronise. */
+nnpx_status nnpx_evaluate(nnpx_ctx* ctx, int precision, unsigned flags);
+
+/* ---- results ---------------------------------------------------------- */
+nnpx_status nnpx_get_energy(nnpx_ctx* ctx, double* energy /* n_struct */);
+nnpx_status nnpx_get_atomic_energies(nnpx_ctx* ctx, double* e /* N */);
+nnpx_status nnpx_get_forces(nnpx_ctx* ctx, double* f /* 3N */);
+/* Scaled symmetry functions and dE/dG of every atom, row stride ld >= max_sf
+ * (Atom::G, Atom::dEdG). */
+nnpx_status nnpx_get_G(nnpx_ctx* ctx, double* G, int ld);
+nnpx_status nnpx_get_dEdG(nnpx_ctx* ctx, double* dEdG, int ld);
+/* Neighbour list: total entry count, then per-atom counts and, for every
+ * entry in list order, the neighbour index and its image shift
+ * (bc0, bc1, bc2).  Pass NULL arrays to query the total only. */
+nnpx_status nnpx_get_neighbors(nnpx_ctx* ctx, int64_t* total, int* counts,
+                               int* j, int* shift);
+/* Stored dG/dr of the plain form (after SF with derivatives):
+ *   self[N*ld_self*3] -- Atom::dGdr, row stride ld_self >= max_sf
+ *   nbr[total*ld_nbr*3] -- Atom::Neighbor::dGdr per neighbour entry, slots
+ *   in Element::symmetryFunctionTable order (Element.cpp:289-311).
+ * Either pointer may be NULL. */
+nnpx_status nnpx_get_dGdr(nnpx_ctx* ctx, double* self, int ld_self,
+                          double* nbr, int ld_nbr);
+
+/* ---- measurement ------------------------------------------------------ */
+/* With NNPX_OPT_TIMING set, summed event time and launch count of the named
+ * kernel since the last reset ("nbr", "sf64", "nn64", "force64", "energy",
+ * "sf32", "nnbf16", "force32", ...). */
+nnpx_status nnpx_kernel_time(nnpx_ctx* ctx, const char* name, double* ms,
+                             int64_t* launches);
+nnpx_status nnpx_reset_kernel_times(nnpx_ctx* ctx);
+/* Algorithmic work counters of the current structures (after a neighbour
+ * list): P pairs within the SF cutoff, T triplets, member evaluations. */
+nnpx_status nnpx_work_counts(nnpx_ctx* ctx, int64_t* pairs,
+                             int64_t* triplets, int64_t* pair_members,
+                             int64_t* triplet_members, int64_t* triplet_exps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNPX_H */

@@ -1,0 +1,343 @@
+"""n2p2_amd -- MI355X (gfx950) Behler-Parrinello energy/force path.
+
+Python mirror of the ``nnp::Mode`` compute boundary over the C ABI of
+``libnnpx.so`` (``include/nnpx.h``).  The compute runs only in the HIP
+library; importing this package never falls back to a CPU path, and every
+call raises :class:`NNPError` when the library or a GPU is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["NNPError", "Mode", "library", "PLAIN_FP64", "MIXED", "FORCES_PULL", "FORCES_PUSH",
+           "read_input_data", "remap", "write_output_data"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libnnpx.so")
+
+PLAIN_FP64 = 0
+MIXED = 1
+FORCES_PULL = 0
+FORCES_PUSH = 1
+OPT_FORCE_MODE = 0
+OPT_TIMING = 1
+OPT_NEIGHBOR_ALGO = 2
+EVAL_FORCES = 1
+
+_STATUS = {0: "OK", 1: "ARGUMENT", 2: "FILE", 3: "MODEL", 4: "RANGE", 5: "STATE", 6: "HIP", 7: "UNSUPPORTED"}
+
+_lib = None
+
+
+class NNPError(RuntimeError):
+    """A non-zero nnpx_status; ``status`` holds the code."""
+
+    def __init__(self, status, msg):
+        super().__init__("%s: %s" % (_STATUS.get(status, status), msg))
+        self.status = status
+
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_c_i64_p = ctypes.POINTER(ctypes.c_int64)
+
+_SIGS = {
+    "nnpx_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int],
+    "nnpx_destroy": [ctypes.c_void_p],
+    "nnpx_set_option": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64],
+    "nnpx_synchronize": [ctypes.c_void_p],
+    "nnpx_load_model": [ctypes.c_void_p, ctypes.c_char_p],
+    "nnpx_model_info": [ctypes.c_void_p, _c_int_p, _c_int_p, _c_dbl_p],
+    "nnpx_num_sf": [ctypes.c_void_p, ctypes.c_int, _c_int_p],
+    "nnpx_atomic_number": [ctypes.c_void_p, ctypes.c_int, _c_int_p],
+    "nnpx_element_index": [ctypes.c_void_p, ctypes.c_char_p, _c_int_p],
+    "nnpx_model_units": [ctypes.c_void_p, _c_int_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "nnpx_set_structures": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_update_positions": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_calculate_neighbor_list": [ctypes.c_void_p],
+    "nnpx_calculate_symmetry_function_groups": [ctypes.c_void_p, ctypes.c_int],
+    "nnpx_calculate_atomic_neural_networks": [ctypes.c_void_p, ctypes.c_int],
+    "nnpx_calculate_energy": [ctypes.c_void_p],
+    "nnpx_calculate_forces": [ctypes.c_void_p],
+    "nnpx_evaluate": [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint],
+    "nnpx_get_energy": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_get_atomic_energies": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_get_forces": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_get_G": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+    "nnpx_get_dEdG": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int],
+    "nnpx_get_neighbors": [ctypes.c_void_p, _c_i64_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_get_dGdr": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int],
+    "nnpx_kernel_time": [ctypes.c_void_p, ctypes.c_char_p, _c_dbl_p, _c_i64_p],
+    "nnpx_reset_kernel_times": [ctypes.c_void_p],
+    "nnpx_work_counts": [ctypes.c_void_p, _c_i64_p, _c_i64_p, _c_i64_p, _c_i64_p, _c_i64_p],
+}
+
+
+def library():
+    """Load libnnpx.so (built by ``__graft_entry__.build()``); raise if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NNPError(6, "HIP library %s is not built (run __graft_entry__.build())" % LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        lib.nnpx_last_error.argtypes = [ctypes.c_void_p]
+        lib.nnpx_last_error.restype = ctypes.c_char_p
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class Mode:
+    """One GPU context with a loaded model: the nnp::Mode compute calls.
+
+    Method names follow nnp::Mode (Mode.h:313-387): calculate_neighbor_list,
+    calculate_symmetry_function_groups, calculate_atomic_neural_networks,
+    calculate_energy, calculate_forces and evaluate (evaluateNNP).
+    """
+
+    def __init__(self, model_dir=None, device=0):
+        self._lib = library()
+        h = ctypes.c_void_p()
+        st = self._lib.nnpx_create(ctypes.byref(h), device)
+        if st:
+            raise NNPError(st, self._lib.nnpx_last_error(None).decode())
+        self._h = h
+        self.n_atoms = 0
+        self.n_struct = 0
+        if model_dir is not None:
+            self.load_model(model_dir)
+
+    def _ck(self, st):
+        if st:
+            raise NNPError(st, self._lib.nnpx_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.nnpx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- model ---------------------------------------------------------
+    def load_model(self, model_dir):
+        self._ck(self._lib.nnpx_load_model(self._h, os.fsencode(model_dir)))
+        ne, ms, rc = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        self._ck(self._lib.nnpx_model_info(self._h, ctypes.byref(ne), ctypes.byref(ms), ctypes.byref(rc)))
+        self.num_elements, self.max_sf, self.max_cutoff = ne.value, ms.value, rc.value
+        self.num_sf = []
+        self.atomic_numbers = []
+        for e in range(self.num_elements):
+            v = ctypes.c_int()
+            self._ck(self._lib.nnpx_num_sf(self._h, e, ctypes.byref(v)))
+            self.num_sf.append(v.value)
+            self._ck(self._lib.nnpx_atomic_number(self._h, e, ctypes.byref(v)))
+            self.atomic_numbers.append(v.value)
+        norm = ctypes.c_int()
+        me, ce, cl = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        off = np.zeros(self.num_elements)
+        self._ck(self._lib.nnpx_model_units(self._h, ctypes.byref(norm), ctypes.byref(me), ctypes.byref(ce),
+                                            ctypes.byref(cl), off.ctypes.data_as(_c_dbl_p)))
+        self.normalize = bool(norm.value)
+        self.mean_energy, self.conv_energy, self.conv_length = me.value, ce.value, cl.value
+        self.energy_offsets = off
+
+    def element_index(self, symbol):
+        v = ctypes.c_int()
+        self._ck(self._lib.nnpx_element_index(self._h, symbol.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_option(self, option, value):
+        self._ck(self._lib.nnpx_set_option(self._h, option, int(value)))
+
+    # -- structures ----------------------------------------------------
+    def set_structures(self, positions, elements, boxes=None, atom_offsets=None, periodic=None):
+        """Set one or many structures (model units, already remapped).
+
+        positions: (N, 3) float64; elements: (N,) element indices;
+        boxes: (S, 3, 3) lattice vectors as rows or None (isolated);
+        atom_offsets: (S+1,) first atom of each structure (default one structure).
+        """
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3)
+        el = np.ascontiguousarray(elements, dtype=np.int32).reshape(-1)
+        n = pos.shape[0]
+        if atom_offsets is None:
+            atom_offsets = [0, n]
+        off = np.ascontiguousarray(atom_offsets, dtype=np.int64)
+        ns = len(off) - 1
+        if boxes is None:
+            box = np.zeros((ns, 9))
+            per = np.zeros(ns, dtype=np.int32) if periodic is None else np.asarray(periodic, np.int32)
+        else:
+            box = np.ascontiguousarray(boxes, dtype=np.float64).reshape(ns, 9)
+            per = np.ones(ns, dtype=np.int32) if periodic is None else np.ascontiguousarray(periodic, np.int32)
+        self._ck(self._lib.nnpx_set_structures(self._h, ns, _ptr(off), _ptr(pos), _ptr(el), _ptr(box), _ptr(per)))
+        self.n_atoms, self.n_struct = n, ns
+        self._pos_shape = pos.shape
+
+    def update_positions(self, positions):
+        pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(self._pos_shape)
+        self._ck(self._lib.nnpx_update_positions(self._h, _ptr(pos)))
+
+    # -- compute (nnp::Mode) -------------------------------------------
+    def calculate_neighbor_list(self):
+        self._ck(self._lib.nnpx_calculate_neighbor_list(self._h))
+
+    def calculate_symmetry_function_groups(self, derivatives=True):
+        self._ck(self._lib.nnpx_calculate_symmetry_function_groups(self._h, int(derivatives)))
+
+    def calculate_atomic_neural_networks(self, derivatives=True):
+        self._ck(self._lib.nnpx_calculate_atomic_neural_networks(self._h, int(derivatives)))
+
+    def calculate_energy(self):
+        self._ck(self._lib.nnpx_calculate_energy(self._h))
+
+    def calculate_forces(self):
+        self._ck(self._lib.nnpx_calculate_forces(self._h))
+
+    def evaluate(self, precision=PLAIN_FP64, forces=True):
+        self._ck(self._lib.nnpx_evaluate(self._h, precision, EVAL_FORCES if forces else 0))
+
+    def synchronize(self):
+        self._ck(self._lib.nnpx_synchronize(self._h))
+
+    # -- results -------------------------------------------------------
+    def energies(self):
+        e = np.zeros(self.n_struct)
+        self._ck(self._lib.nnpx_get_energy(self._h, _ptr(e)))
+        return e
+
+    def atomic_energies(self):
+        e = np.zeros(self.n_atoms)
+        self._ck(self._lib.nnpx_get_atomic_energies(self._h, _ptr(e)))
+        return e
+
+    def forces(self):
+        f = np.zeros((self.n_atoms, 3))
+        self._ck(self._lib.nnpx_get_forces(self._h, _ptr(f)))
+        return f
+
+    def G(self):
+        g = np.zeros((self.n_atoms, self.max_sf))
+        self._ck(self._lib.nnpx_get_G(self._h, _ptr(g), self.max_sf))
+        return g
+
+    def dEdG(self):
+        g = np.zeros((self.n_atoms, self.max_sf))
+        self._ck(self._lib.nnpx_get_dEdG(self._h, _ptr(g), self.max_sf))
+        return g
+
+    def neighbors(self):
+        """(counts, j, shifts) of the neighbour list in list order."""
+        tot = ctypes.c_int64()
+        self._ck(self._lib.nnpx_get_neighbors(self._h, ctypes.byref(tot), None, None, None))
+        counts = np.zeros(self.n_atoms, np.int32)
+        j = np.zeros(tot.value, np.int32)
+        sh = np.zeros((tot.value, 3), np.int32)
+        self._ck(self._lib.nnpx_get_neighbors(self._h, ctypes.byref(tot), _ptr(counts), _ptr(j), _ptr(sh)))
+        return counts, j, sh
+
+    def dGdr(self, ld_nbr):
+        """Stored dG/dr: (self [N, max_sf, 3], neighbour entries [P, ld_nbr, 3])."""
+        counts, _, _ = self.neighbors()
+        s = np.zeros((self.n_atoms, self.max_sf, 3))
+        nb = np.zeros((int(counts.sum()), ld_nbr, 3))
+        self._ck(self._lib.nnpx_get_dGdr(self._h, _ptr(s), self.max_sf, _ptr(nb), ld_nbr))
+        return s, nb
+
+    def kernel_time(self, name):
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self._ck(self._lib.nnpx_kernel_time(self._h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def reset_kernel_times(self):
+        self._ck(self._lib.nnpx_reset_kernel_times(self._h))
+
+    def work_counts(self):
+        v = [ctypes.c_int64() for _ in range(5)]
+        self._ck(self._lib.nnpx_work_counts(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(["pairs", "triplets", "pair_members", "triplet_members", "triplet_exps"],
+                        [x.value for x in v]))
+
+
+# ---------------------------------------------------------------- files --
+def read_input_data(path, symbols):
+    """First structure of an n2p2 input.data file (Structure::readFromLines,
+    Structure.cpp:142-271) without the remap: (pos, elem, box or None)."""
+    pos, el, box = [], [], []
+    index = {s: i for i, s in enumerate(symbols)}
+    with open(path) as f:
+        begun = False
+        for line in f:
+            w = line.split()
+            if not w:
+                continue
+            if w[0] == "begin":
+                begun = True
+            elif not begun:
+                raise ValueError("expected 'begin'")
+            elif w[0] == "lattice":
+                box.append([float(x) for x in w[1:4]])
+            elif w[0] == "atom":
+                pos.append([float(x) for x in w[1:4]])
+                el.append(index[w[4]])
+            elif w[0] == "end":
+                break
+    if box and len(box) != 3:
+        raise ValueError("strange number of box vectors")
+    return np.array(pos), np.array(el, np.int32), (np.array(box) if box else None)
+
+
+def inverse_box(box):
+    """Structure::calculateInverseBox (Structure.cpp:518-544), same operation order."""
+    b = box
+    invdet = (b[0][0] * b[1][1] * b[2][2] + b[1][0] * b[2][1] * b[0][2] + b[2][0] * b[0][1] * b[1][2]
+              - b[2][0] * b[1][1] * b[0][2] - b[0][0] * b[2][1] * b[1][2] - b[1][0] * b[0][1] * b[2][2])
+    invdet = 1.0 / invdet
+    ib = np.zeros((3, 3))
+    ib[0, 0] = b[1][1] * b[2][2] - b[2][1] * b[1][2]
+    ib[0, 1] = b[2][1] * b[0][2] - b[0][1] * b[2][2]
+    ib[0, 2] = b[0][1] * b[1][2] - b[1][1] * b[0][2]
+    ib[1, 0] = b[2][0] * b[1][2] - b[1][0] * b[2][2]
+    ib[1, 1] = b[0][0] * b[2][2] - b[2][0] * b[0][2]
+    ib[1, 2] = b[1][0] * b[0][2] - b[0][0] * b[1][2]
+    ib[2, 0] = b[1][0] * b[2][1] - b[2][0] * b[1][1]
+    ib[2, 1] = b[2][0] * b[0][1] - b[0][0] * b[2][1]
+    ib[2, 2] = b[0][0] * b[1][1] - b[1][0] * b[0][1]
+    return ib * invdet
+
+
+def remap(pos, box):
+    """Structure::remap (Structure.cpp:1224-1248) elementwise, same rounding."""
+    ib = inverse_box(box)
+    r = np.asarray(pos, dtype=np.float64)
+    f = r[:, 0:1] * ib[0] + r[:, 1:2] * ib[1] + r[:, 2:3] * ib[2]
+    f = np.where(f > 1.0, f - np.trunc(f), f)
+    f = np.where(f < 0.0, f + (1.0 - np.trunc(f)), f)
+    f = np.where(f == 1.0, 0.0, f)
+    return f[:, 0:1] * box[0] + f[:, 1:2] * box[1] + f[:, 2:3] * box[2]
+
+
+def write_output_data(path, pos, elem, symbols, forces, energy, box=None, comment=""):
+    """Structure::writeToFile (Structure.cpp:1475-1532) for predicted results."""
+    with open(path, "w") as f:
+        f.write("begin\ncomment %s\n" % comment)
+        if box is not None:
+            for v in box:
+                f.write("lattice %24.16E %24.16E %24.16E\n" % tuple(v))
+        for r, e, fo in zip(pos, elem, forces):
+            f.write("atom %24.16E %24.16E %24.16E %2s %24.16E %24.16E %24.16E %24.16E %24.16E\n"
+                    % (r[0], r[1], r[2], symbols[e], 0.0, 0.0, fo[0], fo[1], fo[2]))
+        f.write("energy %24.16E\ncharge %24.16E\nend\n" % (energy, 0.0))

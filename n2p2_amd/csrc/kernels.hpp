@@ -1,0 +1,76 @@
+// Host-side launchers of the gfx950 kernels (all asynchronous on `stream`).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "device_common.hpp"
+#include "model.hpp"
+
+namespace nnpx {
+
+// Device view of the structures and the neighbour list.
+struct SysView {
+    int natoms;
+    int nstruct;
+    const double* pos;     // [3N]
+    const int* elem;       // [N]
+    const int* sid;        // [N] structure of each atom
+    const StructT* st;     // [nstruct]
+    // neighbour list, fixed capacity per atom, entries sorted by (j, image)
+    int cap;
+    int* nb_n;             // [N]
+    int* nb_j;             // [N*cap]
+    int* nb_img;           // [N*cap]
+    int* nb_rev;           // [N*cap] entry (j*cap + k) holding (i, -image)
+    int* flags;            // [0] overflow, [1] max count
+    // cell list
+    int ncells;
+    int* atom_cell;        // [N]
+    int* cell_count;       // [ncells]
+    int* cell_start;       // [ncells+1]
+    int* cell_atoms;       // [N]
+};
+
+void fill_int(int* p, int v, size_t n, hipStream_t stream);
+
+// Neighbour list (Structure::calculateNeighborList, Structure.cpp:287-398):
+// cell assignment, per-atom build (brute force or cell list, same set and
+// order), reverse entries.
+void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream);
+
+// FP64 symmetry functions; with deriv also dG/dr in the reference layout.
+struct SF64Out {
+    double* G;          // [N*ldG]
+    double* dGself;     // [N*ldG*3]
+    double* dGnbr;      // [N*cap*S*3]
+    int ldG, S;
+};
+void launch_sf64(const ModelT* m, const ModelT& hm, const SysView& s, const SF64Out& o,
+                 bool deriv, int max_nb, hipStream_t stream);
+
+// FP64 atomic NNs over per-element atom lists; dE/dG by the adjoint.
+struct NNIO {
+    const double* G;       // [N*ldG] scaled SFs
+    double* energy;        // [N]
+    double* dEdG;          // [N*ldG] (may be null)
+    int ldG;
+};
+void launch_nn64(const ModelT* m, const ModelT& hm, const double* w, const int* elem_atoms,
+                 const int* elem_count_host, const int* elem_off_host, const NNIO& io,
+                 hipStream_t stream);
+
+// Per-structure energy sums (fixed-order tree reduction).
+void launch_energy(const double* atomE, const int* atom_off, int nstruct, double* E,
+                   hipStream_t stream);
+
+// FP64 force contraction from stored dG/dr.
+void launch_force64(const ModelT* m, const SysView& s, const double* dEdG, int ldG,
+                    const double* dGself, const double* dGnbr, int S, int push,
+                    double* f, hipStream_t stream);
+
+// Algorithmic work counters (pairs, triplets, member evaluations).
+void launch_work_counts(const ModelT* m, const SysView& s, unsigned long long* out5,
+                        hipStream_t stream);
+
+}  // namespace nnpx

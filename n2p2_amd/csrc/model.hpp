@@ -1,0 +1,124 @@
+// Model tables of the 2G-HDNNP path: host loader and the flat, POD layout
+// the HIP kernels read.  The loader follows the setup sequence of
+// Prediction::setup (Prediction.cpp:36-48): Mode::setupGeneric
+// (Mode.cpp:213-237), setupSymmetryFunctionScaling (Mode.cpp:781-909) and
+// setupNeuralNetworkWeights (Mode.cpp:1514-1547, 2331-2350).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nnpx {
+
+constexpr int kMaxElem = 8;      // elements per model
+constexpr int kMaxSF = 128;      // symmetry functions per element
+constexpr int kMaxGroups = 64;   // SF groups per element
+constexpr int kMaxMembers = 32;  // members per SF group
+constexpr int kMaxLayers = 6;    // NN layers incl. input and output
+constexpr int kMaxWidth = 128;   // NN layer width
+
+enum ScalingType { ST_NONE = 0, ST_SCALE, ST_CENTER, ST_SCALECENTER, ST_SCALESIGMA };
+enum Activation {
+    AF_IDENTITY = 0, AF_TANH, AF_LOGISTIC, AF_SOFTPLUS, AF_RELU, AF_GAUSSIAN,
+    AF_COS, AF_REVLOGISTIC, AF_EXP, AF_HARMONIC
+};
+
+// One SF group: members share (central element, type, rc, e1[, e2]).
+struct GroupT {
+    int type;        // 2 radial, 3 angular
+    int e1, e2;      // neighbour elements (e1 <= e2)
+    int nmem;        // members
+    int mem0;        // first member in ModelT::mem
+    int pad;
+    double rc, rc2;
+    double rcinv, rci, iw;  // cutoff constants (CutoffFunction.cpp:95-112)
+};
+
+// One member (one symmetry function) of a group.
+struct MemberT {
+    int sf;          // SF index of the central element (row of G)
+    int slot1;       // derivative slot for neighbour element e1
+    int slot2;       // derivative slot for neighbour element e2 (angular)
+    int calcExp;     // recompute exp (eta or rs differ from previous member)
+    int useIntPow;
+    int zetaInt;
+    int pad0, pad1;
+    double eta, rs, lambda, zeta;
+    double factorNorm;   // angular: 2^(1-zeta) * scaling factor
+    double factorDeriv;  // angular: 2 eta / zeta / lambda
+    double zetaLambda;   // angular: zeta * lambda
+    double sfac;         // scaling factor (SymFnc::scalingFactor)
+    double gnorm;        // angular: factorNorm / sfac
+};
+
+// SymFnc::scale (SymFnc.cpp:169-195): Smin + factor * (v - shift) or v - shift.
+struct ScaleT {
+    int type;
+    int pad;
+    double Smin, Gmin, Gmean, factor;
+};
+
+struct NNT {
+    int nl;                 // layers incl. input/output
+    int normalize;          // normalize_nodes
+    int n[kMaxLayers];      // neurons per layer
+    int act[kMaxLayers];    // activation per layer
+    int woff[kMaxLayers];   // offset of W_l (row-major [n_l][n_{l-1}]) in the weight buffer
+    int boff[kMaxLayers];   // offset of b_l
+};
+
+struct ElementT {
+    int nsf;               // symmetry functions
+    int ngrp;              // groups (radial first, then angular)
+    int grp0;              // first group in ModelT::grp
+    int nrad;              // radial groups
+    int sf0;               // first SF in ModelT::scale
+    int pad;
+    int nsfd[kMaxElem];    // derivative slots per neighbour element
+    int tab0[kMaxElem];    // first entry of the slot->SF table per neighbour element
+    NNT nn;
+};
+
+struct ModelT {
+    int nelem;
+    int cutoffType;
+    int ngroups;
+    int nmembers;
+    int max_sf;             // max SF count over elements
+    int max_sfd;            // max derivative slots (over central and neighbour element)
+    int max_mem_ang;        // max members of an angular group
+    int max_mem_rad;
+    int max_width;          // max NN layer width
+    int nweights;           // FP64 weights+biases in the weight buffer
+    double maxCutoff;
+    double alpha;
+    double tanhPre;         // CT_TANH prefactor (CutoffFunction.cpp:28)
+    ElementT el[kMaxElem];
+    GroupT grp[kMaxElem * kMaxGroups];
+    MemberT mem[kMaxElem * kMaxSF];
+    ScaleT scale[kMaxElem * kMaxSF];
+    int table[kMaxElem * kMaxElem * kMaxSF];  // slot -> SF index
+};
+
+// Host-side model: the device tables plus what the host needs.
+struct HostModel {
+    ModelT t{};
+    std::vector<double> weights;        // NN parameters, FP64
+    std::vector<std::string> symbols;   // by element index
+    std::vector<int> Z;
+    std::vector<double> energyOffset;
+    bool normalize = false;
+    double meanEnergy = 0.0, convEnergy = 1.0, convLength = 1.0;
+};
+
+// Parse input.nn, scaling.data and weights.%03d.data from dir.
+// Throws nnpx::Error on failure.
+void load_model(const std::string& dir, HostModel& m);
+
+struct Error {
+    int status;
+    std::string msg;
+};
+
+}  // namespace nnpx

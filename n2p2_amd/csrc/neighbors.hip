@@ -1,0 +1,250 @@
+// Neighbour list on gfx950, set- and order-identical to
+// Structure::calculateNeighborList(rc, false) (Structure.cpp:287-398):
+// for atom i every (j, bc0, bc1, bc2) with |dr|^2 <= rc^2, ordered by j, then
+// bc0, bc1, bc2, with dr evaluated by the reference's expression.
+//
+// Two builders share the output layout (fixed capacity per atom):
+//  * brute force -- one wave per atom, lanes over j, serial images; hits are
+//    emitted in loop order with a wave prefix sum (no sort needed);
+//  * cell list  -- for structures whose cell holds >= 3 cells of width >= rc
+//    on every axis: atoms binned by fractional coordinate, candidates from
+//    the 27 surrounding cells, hits collected in LDS and bitonic-sorted by
+//    (j, image) so the order equals the brute-force loop order.
+#include "kernels.hpp"
+
+#include <algorithm>
+
+namespace nnpx {
+namespace {
+
+constexpr int kNbMaxLds = 1024;  // max entries sorted in LDS per atom
+
+__global__ void k_cell_count(SysView s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.natoms) return;
+    const StructT& S = s.st[s.sid[i]];
+    if (S.algo != 1) { s.atom_cell[i] = -1; return; }
+    const double* r = s.pos + 3 * (size_t)i;
+    int c[3];
+    for (int k = 0; k < 3; ++k) {
+        // fractional coordinate as in Structure::remap (Structure.cpp:1226-1228)
+        double f = r[0] * S.invbox[k] + r[1] * S.invbox[3 + k] + r[2] * S.invbox[6 + k];
+        int q = (int)(f * S.nc[k]);
+        c[k] = q < 0 ? 0 : (q >= S.nc[k] ? S.nc[k] - 1 : q);
+    }
+    const int cell = S.cell0 + (c[0] * S.nc[1] + c[1]) * S.nc[2] + c[2];
+    s.atom_cell[i] = cell;
+    atomicAdd(&s.cell_count[cell], 1);
+}
+
+// Exclusive scan of cell counts (single block; ncells is small).
+__global__ __launch_bounds__(1024) void k_scan_cells(SysView s)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int n = s.ncells;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    int sum = 0;
+    for (int q = b; q < e; ++q) sum += s.cell_count[q];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int q = b; q < e; ++q) {
+        s.cell_start[q] = run;
+        run += s.cell_count[q];
+        s.cell_count[q] = 0;  // reused as fill cursor
+    }
+    if (t == 1023) s.cell_start[n] = part[1023];
+}
+
+__global__ void k_cell_fill(SysView s)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.natoms) return;
+    const int c = s.atom_cell[i];
+    if (c < 0) return;
+    const int p = atomicAdd(&s.cell_count[c], 1);
+    s.cell_atoms[s.cell_start[c] + p] = i;
+}
+
+__device__ inline void emit(const SysView& s, int i, int slot, int j, int img)
+{
+    if (slot < s.cap) {
+        s.nb_j[(size_t)i * s.cap + slot] = j;
+        s.nb_img[(size_t)i * s.cap + slot] = img;
+    }
+}
+
+// One wave per atom.
+__global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
+{
+    __shared__ long long keys[kNbMaxLds];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const StructT& S = s.st[s.sid[i]];
+    const double* ri = s.pos + 3 * (size_t)i;
+    int count = 0;
+    if (S.algo != 1) {
+        const int p0 = S.periodic ? S.pbc[0] : 0;
+        const int p1 = S.periodic ? S.pbc[1] : 0;
+        const int p2 = S.periodic ? S.pbc[2] : 0;
+        for (int jb = 0; jb < S.natoms; jb += kWave) {
+            const int jl = jb + lane;
+            const int j = S.atom0 + jl;
+            int hits = 0;
+            if (jl < S.natoms) {
+                const double* rj = s.pos + 3 * (size_t)j;
+                for (int b0 = -p0; b0 <= p0; ++b0)
+                    for (int b1 = -p1; b1 <= p1; ++b1)
+                        for (int b2 = -p2; b2 <= p2; ++b2) {
+                            if (i == j && b0 == 0 && b1 == 0 && b2 == 0) continue;
+                            double dr[3];
+                            pair_vector(ri, rj, S.box, b0, b1, b2, dr);
+                            if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2) ++hits;
+                        }
+            }
+            const int incl = wave_incl_scan(hits);
+            int slot = count + incl - hits;
+            if (hits) {
+                const double* rj = s.pos + 3 * (size_t)j;
+                for (int b0 = -p0; b0 <= p0; ++b0)
+                    for (int b1 = -p1; b1 <= p1; ++b1)
+                        for (int b2 = -p2; b2 <= p2; ++b2) {
+                            if (i == j && b0 == 0 && b1 == 0 && b2 == 0) continue;
+                            double dr[3];
+                            pair_vector(ri, rj, S.box, b0, b1, b2, dr);
+                            if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2)
+                                emit(s, i, slot++, j, pack_image(b0, b1, b2));
+                        }
+            }
+            count += __shfl(incl, kWave - 1, kWave);
+        }
+    } else {
+        const int c = s.atom_cell[i] - S.cell0;
+        const int c0 = c / (S.nc[1] * S.nc[2]), c1 = (c / S.nc[2]) % S.nc[1], c2 = c % S.nc[2];
+        for (int d0 = -1; d0 <= 1; ++d0)
+            for (int d1 = -1; d1 <= 1; ++d1)
+                for (int d2 = -1; d2 <= 1; ++d2) {
+                    int q[3] = {c0 + d0, c1 + d1, c2 + d2};
+                    int b[3];
+                    for (int k = 0; k < 3; ++k) {
+                        // wrapped cell: neighbour image r_j + m*box, i.e. bc = -m
+                        const int m = q[k] < 0 ? -1 : (q[k] >= S.nc[k] ? 1 : 0);
+                        q[k] -= m * S.nc[k];
+                        b[k] = -m;
+                    }
+                    const int cell = S.cell0 + (q[0] * S.nc[1] + q[1]) * S.nc[2] + q[2];
+                    const int beg = s.cell_start[cell], end = s.cell_start[cell + 1];
+                    for (int p = beg; p < end; p += kWave) {
+                        bool hit = false;
+                        int j = 0;
+                        if (p + lane < end) {
+                            j = s.cell_atoms[p + lane];
+                            if (!(i == j && b[0] == 0 && b[1] == 0 && b[2] == 0)) {
+                                double dr[3];
+                                pair_vector(ri, s.pos + 3 * (size_t)j, S.box, b[0], b[1], b[2], dr);
+                                hit = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2;
+                            }
+                        }
+                        const int incl = wave_incl_scan(hit ? 1 : 0);
+                        if (hit) {
+                            const int slot = count + incl - 1;
+                            if (slot < kNbMaxLds)
+                                keys[slot] = ((long long)j << 21) | pack_image(b[0], b[1], b[2]);
+                        }
+                        count += __shfl(incl, kWave - 1, kWave);
+                    }
+                }
+        // Bitonic sort of the collected keys (padded to a power of two).
+        const int n = min(count, kNbMaxLds);
+        int np = 1;
+        while (np < n) np <<= 1;
+        for (int q = n + lane; q < np; q += kWave) keys[q] = 0x7fffffffffffffffLL;
+        __syncthreads();
+        for (int k = 2; k <= np; k <<= 1)
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                for (int x = lane; x < np; x += kWave) {
+                    const int y = x ^ jj;
+                    if (y > x) {
+                        const long long a = keys[x], bb = keys[y];
+                        const bool up = (x & k) == 0;
+                        if ((a > bb) == up) { keys[x] = bb; keys[y] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int q = lane; q < n; q += kWave)
+            emit(s, i, q, (int)(keys[q] >> 21), (int)(keys[q] & ((1 << 21) - 1)));
+        if (count > kNbMaxLds && lane == 0) atomicOr(&s.flags[0], 1);
+    }
+    if (lane == 0) {
+        s.nb_n[i] = min(count, s.cap);
+        if (count > s.cap) atomicOr(&s.flags[0], 1);
+        atomicMax(&s.flags[1], count);
+    }
+}
+
+// Reverse entry: (i -> j, image) is stored in j's list as (i, -image).
+__global__ void k_reverse(SysView s)
+{
+    const int i = blockIdx.x;
+    const int n = s.nb_n[i];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const size_t e = (size_t)i * s.cap + k;
+        const int j = s.nb_j[e];
+        const long long want = ((long long)i << 21) | negate_image(s.nb_img[e]);
+        int lo = 0, hi = s.nb_n[j] - 1, found = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) >> 1;
+            const size_t m = (size_t)j * s.cap + mid;
+            const long long key = ((long long)s.nb_j[m] << 21) | s.nb_img[m];
+            if (key == want) { found = mid; break; }
+            if (key < want) lo = mid + 1;
+            else hi = mid - 1;
+        }
+        s.nb_rev[e] = found < 0 ? -1 : (int)((size_t)j * s.cap + found);
+        if (found < 0) atomicOr(&s.flags[0], 2);
+    }
+}
+
+__global__ void k_fill_int(int* p, int v, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        p[q] = v;
+}
+
+}  // namespace
+
+// Memory clears use a fill kernel (not hipMemsetAsync) so the sequence can be
+// captured in a HIP graph.
+void fill_int(int* p, int v, size_t n, hipStream_t stream)
+{
+    if (n == 0) return;
+    const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_fill_int, dim3(nb), dim3(256), 0, stream, p, v, n);
+}
+
+void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream)
+{
+    if (s.natoms == 0) return;
+    fill_int(s.flags, 0, 2, stream);
+    if (any_cell) {
+        fill_int(s.cell_count, 0, (size_t)s.ncells, stream);
+        const int nb = (s.natoms + 255) / 256;
+        hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, stream, s);
+        hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, s);
+        hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(256), 0, stream, s);
+    }
+    hipLaunchKernelGGL(k_build, dim3(s.natoms), dim3(kWave), 0, stream, s, rc * rc);
+    hipLaunchKernelGGL(k_reverse, dim3(s.natoms), dim3(kWave), 0, stream, s);
+}
+
+}  // namespace nnpx

@@ -1,0 +1,952 @@
+// C ABI of libnnpx (include/nnpx.h): context, model and structure state, and
+// the nnp::Mode compute sequence on one GPU.
+#include "nnpx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "mixed.hpp"
+#include "model.hpp"
+
+using namespace nnpx;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct Timer {
+    std::string name;
+    hipEvent_t a, b;
+};
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct nnpx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // options
+    int force_mode = NNPX_FORCES_PULL;
+    bool timing = false;
+    int nbr_algo = 0;
+    // model
+    bool has_model = false;
+    HostModel hm;
+    ModelT* d_model = nullptr;
+    double* d_w = nullptr;       // FP64 weights, then the transposed copy
+    MixedModel mixed;            // BF16/FP32 tables of the mixed form
+    // structures
+    int N = 0, S = 0;
+    std::vector<int64_t> atom_off;
+    std::vector<int> elem_host;
+    std::vector<StructT> st_host;
+    std::vector<int> elem_count, elem_off;  // per element, into elem_atoms
+    bool any_cell = false;
+    int ncells = 0;
+    // device buffers
+    std::map<std::string, DevBuf> bufs;
+    // neighbour list state
+    int cap = 0;
+    int max_nb = 0;
+    bool nbr_valid = false;
+    // results state
+    bool sf_valid = false, sfd_valid = false, nn_valid = false, dedg_valid = false;
+    bool e_valid = false, f_valid = false;
+    int last_precision = NNPX_PLAIN_FP64;
+    // timing
+    std::vector<Timer> pending;
+    std::map<std::string, std::pair<double, int64_t>> ktime;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace {
+
+struct Fail {
+    nnpx_status st;
+    std::string msg;
+};
+
+[[noreturn]] void fail(nnpx_status st, const std::string& m) { throw Fail{st, m}; }
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) fail(NNPX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename F>
+nnpx_status guard(nnpx_ctx* c, F&& f)
+{
+    if (!c) return NNPX_ERR_ARGUMENT;
+    try {
+        hip_check(hipSetDevice(c->device), "hipSetDevice");
+        f();
+        return NNPX_OK;
+    } catch (const Fail& x) {
+        c->err = x.msg;
+        return x.st;
+    } catch (const Error& x) {
+        c->err = x.msg;
+        return (nnpx_status)x.status;
+    } catch (const std::exception& x) {
+        c->err = x.what();
+        return NNPX_ERR_ARGUMENT;
+    }
+}
+
+template <typename T>
+T* buf(nnpx_ctx* c, const std::string& name, size_t count)
+{
+    DevBuf& b = c->bufs[name];
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (b.bytes < bytes) {
+        if (b.p) hip_check(hipFree(b.p), "hipFree");
+        b.p = nullptr;
+        b.bytes = 0;
+        hip_check(hipMalloc(&b.p, bytes), ("hipMalloc " + name).c_str());
+        b.bytes = bytes;
+    }
+    return static_cast<T*>(b.p);
+}
+
+template <typename T>
+T* existing(nnpx_ctx* c, const std::string& name)
+{
+    auto it = c->bufs.find(name);
+    return it == c->bufs.end() ? nullptr : static_cast<T*>(it->second.p);
+}
+
+hipEvent_t get_event(nnpx_ctx* c)
+{
+    if (!c->event_pool.empty()) {
+        hipEvent_t e = c->event_pool.back();
+        c->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    return e;
+}
+
+// Brackets one launcher call with events when timing is enabled.
+template <typename F>
+void timed(nnpx_ctx* c, const char* name, F&& f)
+{
+    if (!c->timing) {
+        f();
+        return;
+    }
+    Timer t{name, get_event(c), get_event(c)};
+    hip_check(hipEventRecord(t.a, c->stream), "hipEventRecord");
+    f();
+    hip_check(hipEventRecord(t.b, c->stream), "hipEventRecord");
+    c->pending.push_back(t);
+}
+
+void collect_times(nnpx_ctx* c)
+{
+    if (c->pending.empty()) return;
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (auto& t : c->pending) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, t.a, t.b), "hipEventElapsedTime");
+        auto& r = c->ktime[t.name];
+        r.first += ms;
+        r.second += 1;
+        c->event_pool.push_back(t.a);
+        c->event_pool.push_back(t.b);
+    }
+    c->pending.clear();
+}
+
+SysView sysview(nnpx_ctx* c)
+{
+    SysView s{};
+    s.natoms = c->N;
+    s.nstruct = c->S;
+    s.pos = existing<double>(c, "pos");
+    s.elem = existing<int>(c, "elem");
+    s.sid = existing<int>(c, "sid");
+    s.st = existing<StructT>(c, "st");
+    s.cap = c->cap;
+    s.nb_n = existing<int>(c, "nb_n");
+    s.nb_j = existing<int>(c, "nb_j");
+    s.nb_img = existing<int>(c, "nb_img");
+    s.nb_rev = existing<int>(c, "nb_rev");
+    s.flags = existing<int>(c, "flags");
+    s.ncells = c->ncells;
+    s.atom_cell = existing<int>(c, "atom_cell");
+    s.cell_count = existing<int>(c, "cell_count");
+    s.cell_start = existing<int>(c, "cell_start");
+    s.cell_atoms = existing<int>(c, "cell_atoms");
+    return s;
+}
+
+void need_model(nnpx_ctx* c)
+{
+    if (!c->has_model) fail(NNPX_ERR_STATE, "no model loaded");
+}
+void need_structures(nnpx_ctx* c)
+{
+    need_model(c);
+    if (c->N <= 0) fail(NNPX_ERR_STATE, "no structures set");
+}
+
+void invalidate(nnpx_ctx* c)
+{
+    c->sf_valid = c->sfd_valid = c->nn_valid = c->dedg_valid = c->e_valid = c->f_valid = false;
+}
+
+// Geometry tables of one structure: inverse box (Structure.cpp:518-544),
+// image range (Structure::calculatePbcCopies, Structure.cpp:485-516) and the
+// cell grid of the cell list.
+void setup_struct(StructT& t, const double* box, bool periodic, double rc)
+{
+    std::memset(&t, 0, sizeof t);
+    t.periodic = periodic ? 1 : 0;
+    if (!periodic) return;
+    for (int q = 0; q < 9; ++q) t.box[q] = box[q];
+    const double(*b)[3] = reinterpret_cast<const double(*)[3]>(t.box);
+    double ib[3][3];
+    double invdet = b[0][0] * b[1][1] * b[2][2] + b[1][0] * b[2][1] * b[0][2] +
+                    b[2][0] * b[0][1] * b[1][2] - b[2][0] * b[1][1] * b[0][2] -
+                    b[0][0] * b[2][1] * b[1][2] - b[1][0] * b[0][1] * b[2][2];
+    if (invdet == 0.0) fail(NNPX_ERR_ARGUMENT, "singular box");
+    invdet = 1.0 / invdet;
+    ib[0][0] = b[1][1] * b[2][2] - b[2][1] * b[1][2];
+    ib[0][1] = b[2][1] * b[0][2] - b[0][1] * b[2][2];
+    ib[0][2] = b[0][1] * b[1][2] - b[1][1] * b[0][2];
+    ib[1][0] = b[2][0] * b[1][2] - b[1][0] * b[2][2];
+    ib[1][1] = b[0][0] * b[2][2] - b[2][0] * b[0][2];
+    ib[1][2] = b[1][0] * b[0][2] - b[0][0] * b[1][2];
+    ib[2][0] = b[1][0] * b[2][1] - b[2][0] * b[1][1];
+    ib[2][1] = b[2][0] * b[0][1] - b[0][0] * b[2][1];
+    ib[2][2] = b[0][0] * b[1][1] - b[1][0] * b[0][1];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) t.invbox[3 * i + k] = ib[i][k] * invdet;
+    auto cross = [](const double* u, const double* v, double* w) {
+        w[0] = u[1] * v[2] - u[2] * v[1];
+        w[1] = u[2] * v[0] - u[0] * v[2];
+        w[2] = u[0] * v[1] - u[1] * v[0];
+    };
+    auto unit = [](double* w) {
+        const double n = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        w[0] /= n; w[1] /= n; w[2] /= n;
+    };
+    auto dot = [](const double* u, const double* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    double axb[3], axc[3], bxc[3];
+    cross(b[0], b[1], axb); unit(axb);
+    cross(b[0], b[2], axc); unit(axc);
+    cross(b[1], b[2], bxc); unit(bxc);
+    const double proj[3] = {std::fabs(dot(b[0], bxc)), std::fabs(dot(b[1], axc)), std::fabs(dot(b[2], axb))};
+    for (int k = 0; k < 3; ++k) {
+        t.pbc[k] = 0;
+        while (t.pbc[k] * proj[k] <= rc) t.pbc[k]++;
+        if (t.pbc[k] > 60) fail(NNPX_ERR_UNSUPPORTED, "cell too small for the cutoff (image range > 60)");
+        // cells of perpendicular width >= rc (with a relative margin)
+        t.nc[k] = (int)std::floor(proj[k] / (rc * (1.0 + 1e-9) + 1e-12));
+    }
+}
+
+// Exact fractional coordinates as Structure::remap evaluates them.
+bool inside_cell(const StructT& t, const double* r)
+{
+    for (int k = 0; k < 3; ++k) {
+        const double f = r[0] * t.invbox[k] + r[1] * t.invbox[3 + k] + r[2] * t.invbox[6 + k];
+        if (!(f >= 0.0 && f < 1.0)) return false;
+    }
+    return true;
+}
+
+void choose_algorithms(nnpx_ctx* c, const double* pos)
+{
+    c->ncells = 0;
+    c->any_cell = false;
+    for (int s = 0; s < c->S; ++s) {
+        StructT& t = c->st_host[s];
+        t.algo = 0;
+        t.cell0 = 0;
+        bool cells = t.periodic && t.nc[0] >= 3 && t.nc[1] >= 3 && t.nc[2] >= 3 && c->nbr_algo != 1;
+        if (cells)
+            for (int64_t i = c->atom_off[s]; i < c->atom_off[s + 1] && cells; ++i)
+                cells = inside_cell(t, pos + 3 * i);
+        if (cells) {
+            t.algo = 1;
+            t.cell0 = c->ncells;
+            c->ncells += t.nc[0] * t.nc[1] * t.nc[2];
+            c->any_cell = true;
+        }
+    }
+}
+
+int estimate_cap(nnpx_ctx* c)
+{
+    const double rc = c->hm.t.maxCutoff;
+    int cap = 32;
+    for (int s = 0; s < c->S; ++s) {
+        const StructT& t = c->st_host[s];
+        const int64_t n = c->atom_off[s + 1] - c->atom_off[s];
+        double est;
+        if (t.periodic) {
+            const double* b = t.box;
+            const double vol = std::fabs(b[0] * (b[4] * b[8] - b[5] * b[7]) - b[1] * (b[3] * b[8] - b[5] * b[6]) +
+                                         b[2] * (b[3] * b[7] - b[4] * b[6]));
+            est = 4.0 / 3.0 * M_PI * rc * rc * rc * n / vol;
+            const double maxn = (double)n * (2 * t.pbc[0] + 1) * (2 * t.pbc[1] + 1) * (2 * t.pbc[2] + 1);
+            est = std::min(maxn, 1.5 * est + 32.0);
+        } else {
+            est = (double)n;
+        }
+        cap = std::max(cap, (int)est);
+    }
+    return ((cap + 31) / 32) * 32;
+}
+
+void alloc_neighbors(nnpx_ctx* c)
+{
+    const size_t n = (size_t)c->N * c->cap;
+    buf<int>(c, "nb_n", c->N);
+    buf<int>(c, "nb_j", n);
+    buf<int>(c, "nb_img", n);
+    buf<int>(c, "nb_rev", n);
+    buf<int>(c, "flags", 4);
+    buf<int>(c, "atom_cell", c->N);
+    buf<int>(c, "cell_count", std::max(c->ncells, 1));
+    buf<int>(c, "cell_start", c->ncells + 1);
+    buf<int>(c, "cell_atoms", c->N);
+}
+
+void read_flags(nnpx_ctx* c, int* f)
+{
+    hip_check(hipMemcpyAsync(f, existing<int>(c, "flags"), 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+              "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+// Builds the list; grows the capacity and rebuilds when an atom overflowed.
+void build_neighbors(nnpx_ctx* c, bool check)
+{
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        alloc_neighbors(c);
+        SysView s = sysview(c);
+        timed(c, "nbr", [&] { launch_neighbors(s, c->hm.t.maxCutoff, c->any_cell, c->stream); });
+        if (!check) break;
+        int f[2];
+        read_flags(c, f);
+        c->max_nb = f[1];
+        if (f[0] & 2) fail(NNPX_ERR_HIP, "neighbour list is not symmetric");
+        if (!(f[0] & 1)) break;
+        if (f[1] > 1024) fail(NNPX_ERR_UNSUPPORTED, "more than 1024 neighbours per atom");
+        c->cap = ((f[1] + f[1] / 4 + 31) / 32) * 32;
+    }
+    c->nbr_valid = true;
+}
+
+void compute_sf64(nnpx_ctx* c, bool deriv)
+{
+    const ModelT& t = c->hm.t;
+    SF64Out o{};
+    o.ldG = t.max_sf;
+    o.S = std::max(t.max_sfd, 1);
+    o.G = buf<double>(c, "G", (size_t)c->N * o.ldG);
+    if (deriv) {
+        o.dGself = buf<double>(c, "dGself", (size_t)c->N * o.ldG * 3);
+        o.dGnbr = buf<double>(c, "dGnbr", (size_t)c->N * c->cap * o.S * 3);
+    }
+    SysView s = sysview(c);
+    timed(c, "sf64", [&] { launch_sf64(c->d_model, t, s, o, deriv, c->cap, c->stream); });
+    c->sf_valid = true;
+    c->sfd_valid = deriv;
+}
+
+void compute_nn64(nnpx_ctx* c, bool deriv)
+{
+    const ModelT& t = c->hm.t;
+    NNIO io{};
+    io.ldG = t.max_sf;
+    io.G = existing<double>(c, "G");
+    io.energy = buf<double>(c, "Eatom", c->N);
+    io.dEdG = deriv ? buf<double>(c, "dEdG", (size_t)c->N * io.ldG) : nullptr;
+    const int* atoms = existing<int>(c, "elem_atoms");
+    timed(c, "nn64", [&] {
+        launch_nn64(c->d_model, t, c->d_w, atoms, c->elem_count.data(), c->elem_off.data(), io, c->stream);
+    });
+    c->nn_valid = true;
+    c->dedg_valid = deriv;
+}
+
+void compute_energy(nnpx_ctx* c)
+{
+    double* E = buf<double>(c, "E", c->S);
+    timed(c, "energy", [&] { launch_energy(existing<double>(c, "Eatom"), existing<int>(c, "atom_off"), c->S, E, c->stream); });
+    c->e_valid = true;
+}
+
+void compute_force64(nnpx_ctx* c)
+{
+    const ModelT& t = c->hm.t;
+    double* f = buf<double>(c, "F", (size_t)3 * c->N);
+    SysView s = sysview(c);
+    timed(c, "force64", [&] {
+        launch_force64(c->d_model, s, existing<double>(c, "dEdG"), t.max_sf, existing<double>(c, "dGself"),
+                       existing<double>(c, "dGnbr"), std::max(t.max_sfd, 1),
+                       c->force_mode == NNPX_FORCES_PUSH ? 1 : 0, f, c->stream);
+    });
+    c->f_valid = true;
+}
+
+void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check)
+{
+    const bool forces = flags & NNPX_EVAL_FORCES;
+    invalidate(c);
+    build_neighbors(c, check);
+    if (precision == NNPX_MIXED) {
+        MixedBuffers mb = mixed_buffers(c);
+        run_mixed(c->mixed, c->hm.t, sysview(c), mb, forces, c->force_mode == NNPX_FORCES_PUSH,
+                  c->stream, [&](const char* name, const std::function<void()>& f) { timed(c, name, f); });
+        c->sf_valid = true;
+        c->nn_valid = true;
+        c->dedg_valid = true;
+        c->e_valid = true;
+        c->f_valid = forces;
+    } else {
+        compute_sf64(c, forces);
+        compute_nn64(c, true);
+        compute_energy(c);
+        if (forces) compute_force64(c);
+    }
+    c->last_precision = precision;
+}
+
+// Results are read only after the overflow flag of the last neighbour build
+// was checked; an overflow re-runs the evaluation with a larger capacity.
+void settle(nnpx_ctx* c)
+{
+    int f[2];
+    read_flags(c, f);
+    if (f[0] & 2) fail(NNPX_ERR_HIP, "neighbour list is not symmetric");
+    if (f[0] & 1) {
+        c->cap = ((f[1] + f[1] / 4 + 31) / 32) * 32;
+        evaluate(c, c->last_precision, c->f_valid ? NNPX_EVAL_FORCES : 0u, true);
+    }
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+void copy_out(nnpx_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+}  // namespace
+
+MixedBuffers mixed_buffers(nnpx_ctx* c)
+{
+    const ModelT& t = c->hm.t;
+    MixedBuffers m{};
+    m.ldG = t.max_sf;
+    m.G = buf<double>(c, "G", (size_t)c->N * m.ldG);
+    m.Gbf = buf<uint16_t>(c, "Gbf", (size_t)c->N * c->mixed.ldK);
+    m.dEdG = buf<double>(c, "dEdG", (size_t)c->N * m.ldG);
+    m.dEdGf = buf<float>(c, "dEdGf", (size_t)c->N * m.ldG);
+    m.Eatom = buf<double>(c, "Eatom", c->N);
+    m.E = buf<double>(c, "E", c->S);
+    m.F = buf<double>(c, "F", (size_t)3 * c->N);
+    m.Fpart = buf<double>(c, "Fpart", (size_t)3 * c->N * c->cap);
+    m.elem_atoms = existing<int>(c, "elem_atoms");
+    m.atom_off = existing<int>(c, "atom_off");
+    for (int e = 0; e <= t.nelem; ++e) m.elem_off[e] = c->elem_off[e];
+    m.nstruct = c->S;
+    m.cap = c->cap;
+    return m;
+}
+
+// ======================================================================== ABI
+extern "C" {
+
+nnpx_status nnpx_create(nnpx_ctx** ctx, int device)
+{
+    if (!ctx) return NNPX_ERR_ARGUMENT;
+    *ctx = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        g_create_error = "no HIP device available";
+        return NNPX_ERR_HIP;
+    }
+    if (device < 0 || device >= n) {
+        g_create_error = "device index out of range";
+        return NNPX_ERR_ARGUMENT;
+    }
+    nnpx_ctx* c = new nnpx_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_error = "could not create a HIP stream";
+        delete c;
+        return NNPX_ERR_HIP;
+    }
+    *ctx = c;
+    return NNPX_OK;
+}
+
+nnpx_status nnpx_destroy(nnpx_ctx* c)
+{
+    if (!c) return NNPX_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    if (c->d_model) (void)hipFree(c->d_model);
+    if (c->d_w) (void)hipFree(c->d_w);
+    free_mixed(c->mixed);
+    for (auto& t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto e : c->event_pool) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return NNPX_OK;
+}
+
+const char* nnpx_last_error(const nnpx_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
+{
+    return guard(c, [&] {
+        switch (option) {
+        case NNPX_OPT_FORCE_MODE:
+            if (value != NNPX_FORCES_PULL && value != NNPX_FORCES_PUSH) fail(NNPX_ERR_ARGUMENT, "bad force mode");
+            c->force_mode = (int)value;
+            c->f_valid = false;
+            break;
+        case NNPX_OPT_TIMING: c->timing = value != 0; break;
+        case NNPX_OPT_NEIGHBOR_ALGO:
+            if (value < 0 || value > 2) fail(NNPX_ERR_ARGUMENT, "bad neighbour algorithm");
+            c->nbr_algo = (int)value;
+            break;
+        default: fail(NNPX_ERR_ARGUMENT, "unknown option");
+        }
+    });
+}
+
+nnpx_status nnpx_synchronize(nnpx_ctx* c)
+{
+    return guard(c, [&] { hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); });
+}
+
+nnpx_status nnpx_load_model(nnpx_ctx* c, const char* dir)
+{
+    return guard(c, [&] {
+        if (!dir) fail(NNPX_ERR_ARGUMENT, "null model directory");
+        HostModel hm;
+        load_model(dir, hm);
+        c->hm = std::move(hm);
+        const ModelT& t = c->hm.t;
+        if (!c->d_model) hip_check(hipMalloc(&c->d_model, sizeof(ModelT)), "hipMalloc model");
+        hip_check(hipMemcpy(c->d_model, &t, sizeof(ModelT), hipMemcpyHostToDevice), "hipMemcpy model");
+        // weights in model layout followed by a transposed copy ([in][out])
+        std::vector<double> w2(2 * (size_t)t.nweights, 0.0);
+        std::copy(c->hm.weights.begin(), c->hm.weights.end(), w2.begin());
+        for (int e = 0; e < t.nelem; ++e) {
+            const NNT& N = t.el[e].nn;
+            for (int l = 1; l < N.nl; ++l)
+                for (int k = 0; k < N.n[l]; ++k)
+                    for (int j = 0; j < N.n[l - 1]; ++j)
+                        w2[t.nweights + N.woff[l] + (size_t)j * N.n[l] + k] =
+                            c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j];
+        }
+        if (c->d_w) hip_check(hipFree(c->d_w), "hipFree");
+        c->d_w = nullptr;
+        hip_check(hipMalloc(&c->d_w, sizeof(double) * w2.size()), "hipMalloc weights");
+        hip_check(hipMemcpy(c->d_w, w2.data(), sizeof(double) * w2.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        setup_mixed(c->mixed, c->hm);
+        c->has_model = true;
+        c->N = c->S = 0;
+        c->nbr_valid = false;
+        invalidate(c);
+    });
+}
+
+nnpx_status nnpx_model_info(const nnpx_ctx* cc, int* ne, int* max_sf, double* rc)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (ne) *ne = c->hm.t.nelem;
+        if (max_sf) *max_sf = c->hm.t.max_sf;
+        if (rc) *rc = c->hm.t.maxCutoff;
+    });
+}
+
+nnpx_status nnpx_num_sf(const nnpx_ctx* cc, int e, int* n)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (e < 0 || e >= c->hm.t.nelem || !n) fail(NNPX_ERR_ARGUMENT, "bad element index");
+        *n = c->hm.t.el[e].nsf;
+    });
+}
+
+nnpx_status nnpx_atomic_number(const nnpx_ctx* cc, int e, int* z)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (e < 0 || e >= c->hm.t.nelem || !z) fail(NNPX_ERR_ARGUMENT, "bad element index");
+        *z = c->hm.Z[e];
+    });
+}
+
+nnpx_status nnpx_element_index(const nnpx_ctx* cc, const char* sym, int* e)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (!sym || !e) fail(NNPX_ERR_ARGUMENT, "null argument");
+        for (int q = 0; q < c->hm.t.nelem; ++q)
+            if (c->hm.symbols[q] == sym) { *e = q; return; }
+        fail(NNPX_ERR_RANGE, std::string("element not in model: ") + sym);
+    });
+}
+
+nnpx_status nnpx_model_units(const nnpx_ctx* cc, int* normalize, double* mean, double* ce, double* cl,
+                             double* offsets)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (normalize) *normalize = c->hm.normalize ? 1 : 0;
+        if (mean) *mean = c->hm.meanEnergy;
+        if (ce) *ce = c->hm.convEnergy;
+        if (cl) *cl = c->hm.convLength;
+        if (offsets)
+            for (int e = 0; e < c->hm.t.nelem; ++e) offsets[e] = c->hm.energyOffset[e];
+    });
+}
+
+nnpx_status nnpx_set_structures(nnpx_ctx* c, int ns, const int64_t* off, const double* pos, const int* elem,
+                                const double* box, const int* periodic)
+{
+    return guard(c, [&] {
+        need_model(c);
+        if (ns < 1 || !off || !pos || !elem || !periodic) fail(NNPX_ERR_ARGUMENT, "null or empty structure input");
+        if (off[0] != 0) fail(NNPX_ERR_ARGUMENT, "atom_offsets[0] must be 0");
+        for (int s = 0; s < ns; ++s)
+            if (off[s + 1] < off[s]) fail(NNPX_ERR_ARGUMENT, "atom_offsets must be non-decreasing");
+        const int64_t N = off[ns];
+        if (N <= 0 || N > (int64_t)1 << 30) fail(NNPX_ERR_ARGUMENT, "bad atom count");
+        for (int64_t i = 0; i < N; ++i)
+            if (elem[i] < 0 || elem[i] >= c->hm.t.nelem) fail(NNPX_ERR_RANGE, "element index out of range");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        c->N = (int)N;
+        c->S = ns;
+        c->atom_off.assign(off, off + ns + 1);
+        c->elem_host.assign(elem, elem + N);
+        c->st_host.assign(ns, StructT{});
+        for (int s = 0; s < ns; ++s) {
+            if (periodic[s] && !box) fail(NNPX_ERR_ARGUMENT, "periodic structure without box");
+            setup_struct(c->st_host[s], periodic[s] ? box + 9 * s : nullptr, periodic[s] != 0, c->hm.t.maxCutoff);
+            c->st_host[s].atom0 = (int)off[s];
+            c->st_host[s].natoms = (int)(off[s + 1] - off[s]);
+        }
+        choose_algorithms(c, pos);
+        std::vector<int> sid(N);
+        for (int s = 0; s < ns; ++s)
+            for (int64_t i = off[s]; i < off[s + 1]; ++i) sid[i] = s;
+        const int ne = c->hm.t.nelem;
+        c->elem_count.assign(ne, 0);
+        c->elem_off.assign(ne + 1, 0);
+        for (int64_t i = 0; i < N; ++i) c->elem_count[elem[i]]++;
+        for (int e = 0; e < ne; ++e) c->elem_off[e + 1] = c->elem_off[e] + c->elem_count[e];
+        std::vector<int> atoms(N);
+        {
+            std::vector<int> fillp(c->elem_off.begin(), c->elem_off.end() - 1);
+            for (int64_t i = 0; i < N; ++i) atoms[fillp[elem[i]]++] = (int)i;
+        }
+        std::vector<int> off32(ns + 1);
+        for (int s = 0; s <= ns; ++s) off32[s] = (int)off[s];
+        auto up = [&](const std::string& name, const void* src, size_t bytes) {
+            void* d = buf<char>(c, name, bytes);
+            hip_check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        };
+        up("pos", pos, sizeof(double) * 3 * N);
+        up("elem", elem, sizeof(int) * N);
+        up("sid", sid.data(), sizeof(int) * N);
+        up("st", c->st_host.data(), sizeof(StructT) * ns);
+        up("elem_atoms", atoms.data(), sizeof(int) * N);
+        up("atom_off", off32.data(), sizeof(int) * (ns + 1));
+        c->cap = estimate_cap(c);
+        c->nbr_valid = false;
+        invalidate(c);
+        // First build sizes the capacity to the actual maximum.
+        build_neighbors(c, true);
+        const int want = ((c->max_nb + c->max_nb / 8 + 8 + 31) / 32) * 32;
+        if (want < c->cap) {
+            c->cap = std::max(want, 32);
+            build_neighbors(c, true);
+        }
+    });
+}
+
+nnpx_status nnpx_update_positions(nnpx_ctx* c, const double* pos)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!pos) fail(NNPX_ERR_ARGUMENT, "null positions");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        for (int s = 0; s < c->S; ++s) {
+            StructT& t = c->st_host[s];
+            if (t.algo == 1)
+                for (int64_t i = c->atom_off[s]; i < c->atom_off[s + 1]; ++i)
+                    if (!inside_cell(t, pos + 3 * i)) fail(NNPX_ERR_ARGUMENT, "position outside the cell (remap first)");
+        }
+        hip_check(hipMemcpy(existing<double>(c, "pos"), pos, sizeof(double) * 3 * c->N, hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        c->nbr_valid = false;
+        invalidate(c);
+    });
+}
+
+nnpx_status nnpx_calculate_neighbor_list(nnpx_ctx* c)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        invalidate(c);
+        build_neighbors(c, true);
+    });
+}
+
+nnpx_status nnpx_calculate_symmetry_function_groups(nnpx_ctx* c, int derivatives)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->nbr_valid) build_neighbors(c, true);
+        // Mode.cpp:1634-1635: skip when results are already there
+        if (c->sfd_valid || (c->sf_valid && !derivatives)) return;
+        compute_sf64(c, derivatives != 0);
+        c->nn_valid = c->dedg_valid = c->e_valid = c->f_valid = false;
+    });
+}
+
+nnpx_status nnpx_calculate_atomic_neural_networks(nnpx_ctx* c, int derivatives)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->sf_valid) fail(NNPX_ERR_STATE, "symmetry functions not calculated");
+        compute_nn64(c, derivatives != 0);
+        c->e_valid = c->f_valid = false;
+    });
+}
+
+nnpx_status nnpx_calculate_energy(nnpx_ctx* c)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->nn_valid) fail(NNPX_ERR_STATE, "atomic neural networks not calculated");
+        compute_energy(c);
+    });
+}
+
+nnpx_status nnpx_calculate_forces(nnpx_ctx* c)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->sfd_valid) fail(NNPX_ERR_STATE, "symmetry function derivatives not calculated");
+        if (!c->dedg_valid) fail(NNPX_ERR_STATE, "dE/dG not calculated");
+        compute_force64(c);
+    });
+}
+
+nnpx_status nnpx_evaluate(nnpx_ctx* c, int precision, unsigned flags)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (precision != NNPX_PLAIN_FP64 && precision != NNPX_MIXED) fail(NNPX_ERR_ARGUMENT, "bad precision");
+        evaluate(c, precision, flags, false);
+    });
+}
+
+nnpx_status nnpx_get_energy(nnpx_ctx* c, double* E)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!E) fail(NNPX_ERR_ARGUMENT, "null output");
+        if (!c->e_valid) fail(NNPX_ERR_STATE, "energy not calculated");
+        settle(c);
+        copy_out(c, E, existing<double>(c, "E"), sizeof(double) * c->S);
+    });
+}
+
+nnpx_status nnpx_get_atomic_energies(nnpx_ctx* c, double* e)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!e) fail(NNPX_ERR_ARGUMENT, "null output");
+        if (!c->nn_valid) fail(NNPX_ERR_STATE, "atomic neural networks not calculated");
+        settle(c);
+        copy_out(c, e, existing<double>(c, "Eatom"), sizeof(double) * c->N);
+    });
+}
+
+nnpx_status nnpx_get_forces(nnpx_ctx* c, double* f)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!f) fail(NNPX_ERR_ARGUMENT, "null output");
+        if (!c->f_valid) fail(NNPX_ERR_STATE, "forces not calculated");
+        settle(c);
+        copy_out(c, f, existing<double>(c, "F"), sizeof(double) * 3 * c->N);
+    });
+}
+
+static void get_rows(nnpx_ctx* c, double* out, int ld, const char* name)
+{
+    const int w = c->hm.t.max_sf;
+    if (ld < w) fail(NNPX_ERR_ARGUMENT, "row stride smaller than the maximum SF count");
+    std::vector<double> tmp((size_t)c->N * w);
+    copy_out(c, tmp.data(), existing<double>(c, name), sizeof(double) * tmp.size());
+    for (int i = 0; i < c->N; ++i) {
+        const int ns = c->hm.t.el[c->elem_host[i]].nsf;
+        for (int q = 0; q < ld; ++q) out[(size_t)i * ld + q] = q < ns ? tmp[(size_t)i * w + q] : 0.0;
+    }
+}
+
+nnpx_status nnpx_get_G(nnpx_ctx* c, double* G, int ld)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!G) fail(NNPX_ERR_ARGUMENT, "null output");
+        if (!c->sf_valid) fail(NNPX_ERR_STATE, "symmetry functions not calculated");
+        settle(c);
+        get_rows(c, G, ld, "G");
+    });
+}
+
+nnpx_status nnpx_get_dEdG(nnpx_ctx* c, double* d, int ld)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!d) fail(NNPX_ERR_ARGUMENT, "null output");
+        if (!c->dedg_valid) fail(NNPX_ERR_STATE, "dE/dG not calculated");
+        settle(c);
+        get_rows(c, d, ld, "dEdG");
+    });
+}
+
+nnpx_status nnpx_get_neighbors(nnpx_ctx* c, int64_t* total, int* counts, int* j, int* shift)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->nbr_valid) build_neighbors(c, true);
+        settle(c);
+        std::vector<int> n(c->N);
+        copy_out(c, n.data(), existing<int>(c, "nb_n"), sizeof(int) * c->N);
+        int64_t tot = 0;
+        for (int v : n) tot += v;
+        if (total) *total = tot;
+        if (counts) std::copy(n.begin(), n.end(), counts);
+        if (j || shift) {
+            std::vector<int> jj((size_t)c->N * c->cap), im((size_t)c->N * c->cap);
+            copy_out(c, jj.data(), existing<int>(c, "nb_j"), sizeof(int) * jj.size());
+            copy_out(c, im.data(), existing<int>(c, "nb_img"), sizeof(int) * im.size());
+            int64_t p = 0;
+            for (int i = 0; i < c->N; ++i)
+                for (int k = 0; k < n[i]; ++k, ++p) {
+                    const size_t e = (size_t)i * c->cap + k;
+                    if (j) j[p] = jj[e];
+                    if (shift) unpack_image(im[e], shift[3 * p], shift[3 * p + 1], shift[3 * p + 2]);
+                }
+        }
+    });
+}
+
+nnpx_status nnpx_get_dGdr(nnpx_ctx* c, double* self, int ld_self, double* nbr, int ld_nbr)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->sfd_valid || c->last_precision != NNPX_PLAIN_FP64)
+            fail(NNPX_ERR_STATE, "stored dG/dr not available (plain form with derivatives required)");
+        settle(c);
+        const ModelT& t = c->hm.t;
+        if (self) {
+            if (ld_self < t.max_sf) fail(NNPX_ERR_ARGUMENT, "ld_self too small");
+            std::vector<double> tmp((size_t)c->N * t.max_sf * 3);
+            copy_out(c, tmp.data(), existing<double>(c, "dGself"), sizeof(double) * tmp.size());
+            for (int i = 0; i < c->N; ++i)
+                for (int q = 0; q < ld_self; ++q)
+                    for (int k = 0; k < 3; ++k)
+                        self[((size_t)i * ld_self + q) * 3 + k] =
+                            q < t.el[c->elem_host[i]].nsf ? tmp[((size_t)i * t.max_sf + q) * 3 + k] : 0.0;
+        }
+        if (nbr) {
+            const int S = std::max(t.max_sfd, 1);
+            if (ld_nbr < S) fail(NNPX_ERR_ARGUMENT, "ld_nbr too small");
+            std::vector<int> n(c->N), jj((size_t)c->N * c->cap);
+            copy_out(c, n.data(), existing<int>(c, "nb_n"), sizeof(int) * c->N);
+            copy_out(c, jj.data(), existing<int>(c, "nb_j"), sizeof(int) * jj.size());
+            std::vector<double> tmp((size_t)c->N * c->cap * S * 3);
+            copy_out(c, tmp.data(), existing<double>(c, "dGnbr"), sizeof(double) * tmp.size());
+            int64_t p = 0;
+            for (int i = 0; i < c->N; ++i) {
+                const ElementT& E = t.el[c->elem_host[i]];
+                for (int k = 0; k < n[i]; ++k, ++p) {
+                    const int ns = E.nsfd[c->elem_host[jj[(size_t)i * c->cap + k]]];
+                    for (int q = 0; q < ld_nbr; ++q)
+                        for (int x = 0; x < 3; ++x)
+                            nbr[((size_t)p * ld_nbr + q) * 3 + x] =
+                                q < ns ? tmp[(((size_t)i * c->cap + k) * S + q) * 3 + x] : 0.0;
+                }
+            }
+        }
+    });
+}
+
+nnpx_status nnpx_kernel_time(nnpx_ctx* c, const char* name, double* ms, int64_t* launches)
+{
+    return guard(c, [&] {
+        if (!name) fail(NNPX_ERR_ARGUMENT, "null name");
+        collect_times(c);
+        auto it = c->ktime.find(name);
+        if (ms) *ms = it == c->ktime.end() ? 0.0 : it->second.first;
+        if (launches) *launches = it == c->ktime.end() ? 0 : it->second.second;
+    });
+}
+
+nnpx_status nnpx_reset_kernel_times(nnpx_ctx* c)
+{
+    return guard(c, [&] {
+        collect_times(c);
+        c->ktime.clear();
+    });
+}
+
+nnpx_status nnpx_work_counts(nnpx_ctx* c, int64_t* pairs, int64_t* triplets, int64_t* pm, int64_t* tm,
+                             int64_t* te)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->nbr_valid) build_neighbors(c, true);
+        settle(c);
+        unsigned long long* d = buf<unsigned long long>(c, "work", 5);
+        hip_check(hipMemsetAsync(d, 0, 5 * sizeof(unsigned long long), c->stream), "hipMemsetAsync");
+        launch_work_counts(c->d_model, sysview(c), d, c->stream);
+        unsigned long long h[5];
+        copy_out(c, h, d, sizeof h);
+        if (pairs) *pairs = (int64_t)h[0];
+        if (triplets) *triplets = (int64_t)h[1];
+        if (pm) *pm = (int64_t)h[2];
+        if (tm) *tm = (int64_t)h[3];
+        if (te) *te = (int64_t)h[4];
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+"""ctypes access to the CPU checkers (test infrastructure only).
+
+* Oracle   -- oracle/lib/liboracle.so, the from-scratch FP64 restatement.
+* RefLib   -- oracle/_ref/libnnpref.so, n2p2's own compute classes (built only
+              where the reference tree exists; tests using it skip otherwise).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "lib", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libnnpref.so")
+
+PULL, PUSH = 0, 1
+BRUTE, CELL = 0, 1
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Oracle:
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            L = ctypes.CDLL(ORACLE_SO)
+            L.oracle_create.restype = ctypes.c_void_p
+            L.oracle_create.argtypes = [ctypes.c_char_p]
+            L.oracle_last_error.restype = ctypes.c_char_p
+            for n in ("oracle_energy", "oracle_max_cutoff", "oracle_phase_time"):
+                getattr(L, n).restype = ctypes.c_double
+            L.oracle_build_neighbors.restype = ctypes.c_long
+            L.oracle_cutoff_fdf.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, model_dir):
+        L = self.lib()
+        h = L.oracle_create(os.fsencode(model_dir))
+        if not h:
+            raise RuntimeError(L.oracle_last_error().decode())
+        self.h = ctypes.c_void_p(h)
+        self.ne = L.oracle_num_elements(self.h)
+        self.nsf = [L.oracle_num_sf(self.h, e) for e in range(self.ne)]
+        self.max_sf = max(self.nsf)
+
+    def __del__(self):
+        try:
+            self.lib().oracle_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_structure(self, pos, elem, box=None):
+        L = self.lib()
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        b = None if box is None else np.ascontiguousarray(box, dtype=np.float64)
+        if L.oracle_set_structure(self.h, len(e), _vp(p), _vp(e), _vp(b)):
+            raise RuntimeError(L.oracle_last_error().decode())
+        self.n = len(e)
+
+    def read_structure(self, path):
+        L = self.lib()
+        if L.oracle_read_structure(self.h, os.fsencode(path)):
+            raise RuntimeError(L.oracle_last_error().decode())
+        self.n = L.oracle_num_atoms(self.h)
+
+    def predict(self, force_mode=PULL, nbr=BRUTE, threads=0):
+        L = self.lib()
+        if L.oracle_predict(self.h, force_mode, nbr, threads):
+            raise RuntimeError(L.oracle_last_error().decode())
+
+    def build_neighbors(self, nbr=BRUTE, threads=0):
+        return self.lib().oracle_build_neighbors(self.h, nbr, threads)
+
+    def energy(self):
+        return self.lib().oracle_energy(self.h)
+
+    def forces(self):
+        f = np.zeros((self.n, 3))
+        self.lib().oracle_get_forces(self.h, _vp(f))
+        return f
+
+    def positions(self):
+        r = np.zeros((self.n, 3))
+        self.lib().oracle_get_positions(self.h, _vp(r))
+        return r
+
+    def atomic_energies(self):
+        e = np.zeros(self.n)
+        self.lib().oracle_get_atomic_energies(self.h, _vp(e))
+        return e
+
+    def G(self):
+        g = np.zeros((self.n, self.max_sf))
+        self.lib().oracle_get_G(self.h, _vp(g), self.max_sf)
+        return g
+
+    def dEdG(self):
+        g = np.zeros((self.n, self.max_sf))
+        self.lib().oracle_get_dEdG(self.h, _vp(g), self.max_sf)
+        return g
+
+    def neighbors(self):
+        L = self.lib()
+        counts = np.array([L.oracle_num_neighbors(self.h, i) for i in range(self.n)], np.int32)
+        js, shs = [], []
+        for i in range(self.n):
+            j = np.zeros(counts[i], np.int32)
+            sh = np.zeros((counts[i], 3), np.int32)
+            L.oracle_get_neighbors(self.h, i, _vp(j), _vp(sh))
+            js.append(j)
+            shs.append(sh)
+        return counts, np.concatenate(js), np.concatenate(shs)
+
+    def phase_times(self):
+        return [self.lib().oracle_phase_time(self.h, p) for p in range(5)]
+
+
+def cutoff_fdf(ctype, rc, alpha, r):
+    L = Oracle.lib()
+    fc, dfc = ctypes.c_double(), ctypes.c_double()
+    if L.oracle_cutoff_fdf(ctype, rc, alpha, r, ctypes.byref(fc), ctypes.byref(dfc)):
+        raise RuntimeError(L.oracle_last_error().decode())
+    return fc.value, dfc.value
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+class RefLib:
+    """n2p2's compute classes driven like Mode (only where built)."""
+    _lib = None
+
+    @classmethod
+    def lib(cls):
+        if cls._lib is None:
+            L = ctypes.CDLL(REF_SO)
+            L.ref_create.restype = ctypes.c_void_p
+            L.ref_create.argtypes = [ctypes.c_char_p]
+            L.ref_last_error.restype = ctypes.c_char_p
+            L.ref_energy.restype = ctypes.c_double
+            L.ref_phase_time.restype = ctypes.c_double
+            cls._lib = L
+        return cls._lib
+
+    def __init__(self, model_dir):
+        L = self.lib()
+        h = L.ref_create(os.fsencode(model_dir))
+        if not h:
+            raise RuntimeError(L.ref_last_error().decode())
+        self.h = ctypes.c_void_p(h)
+
+    def set_structure(self, pos, elem, box=None):
+        p = np.ascontiguousarray(pos, dtype=np.float64)
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        b = None if box is None else np.ascontiguousarray(box, dtype=np.float64)
+        if self.lib().ref_set_structure(self.h, len(e), _vp(p), _vp(e), _vp(b)):
+            raise RuntimeError(self.lib().ref_last_error().decode())
+        self.n = len(e)
+
+    def predict(self):
+        if self.lib().ref_predict(self.h):
+            raise RuntimeError(self.lib().ref_last_error().decode())
+
+    def energy(self):
+        return self.lib().ref_energy(self.h)
+
+    def forces(self):
+        f = np.zeros((self.n, 3))
+        self.lib().ref_get_forces(self.h, _vp(f))
+        return f
+
+    def G(self, ld):
+        g = np.zeros((self.n, ld))
+        self.lib().ref_get_G(self.h, _vp(g), ld)
+        return g
+
+    def dEdG(self, ld):
+        g = np.zeros((self.n, ld))
+        self.lib().ref_get_dEdG(self.h, _vp(g), ld)
+        return g
+
+    def phase_times(self):
+        return [self.lib().ref_phase_time(self.h, p) for p in range(5)]

@@ -1,0 +1,200 @@
+"""Parity of the plain FP64 GPU form against the CPU oracle and the
+reference's golden vectors (run on an MI355X: ``pytest -m gpu``).
+
+Tolerances: energy 1e-10 relative, forces 1e-8 (model units), symmetry
+functions 1e-12 relative; neighbour lists bit-exact (same entries, same order).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import n2p2_amd
+from n2p2_amd import read_input_data, remap, synth
+from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "H2O_RPBE-D3")
+E_REL = 1e-10
+F_ABS = 1e-8
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def water_dir(tmp_path_factory):
+    return synth.water_model(str(tmp_path_factory.mktemp("water")))
+
+
+def predict_golden(mode):
+    """Prediction::readStructureFromFile + predict (Prediction.cpp:50-78) on the GPU."""
+    pos, elem, box = read_input_data(os.path.join(GOLD, "input.data"), ["H", "O"])
+    pos = remap(pos, box)
+    cl, ce = mode.conv_length, mode.conv_energy
+    mode.set_structures(pos * cl, elem, (box * cl)[None])
+    mode.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    n = len(elem)
+    E = mode.energies()[0] / ce + n * mode.mean_energy
+    E += sum(np.count_nonzero(elem == e) * mode.energy_offsets[e] for e in range(mode.num_elements))
+    F = mode.forces() * (1.0 / (ce / cl))
+    return E, F
+
+
+def test_golden_h2o_plain_fp64():
+    say("case: H2O_RPBE-D3 golden, plain FP64")
+    exp = json.load(open(os.path.join(GOLD, "expected.json")))
+    m = n2p2_amd.Mode(GOLD)
+    E, F = predict_golden(m)
+    Fg = np.array(exp["forces"])
+    dE = abs(E - exp["energy"])
+    dF = np.abs(F - Fg).max()
+    say("  |dE| = %.3e (rel %.3e), max|dF| = %.3e" % (dE, dE / abs(exp["energy"]), dF))
+    assert dE / abs(exp["energy"]) <= E_REL
+    assert dF <= F_ABS
+
+
+def compare_neighbors(m, o):
+    c1, j1, s1 = m.neighbors()
+    c2, j2, s2 = o.neighbors()
+    assert (c1 == c2).all()
+    assert (j1 == j2).all()
+    assert (s1 == s2).all()
+    return int(c1.sum())
+
+
+@pytest.mark.parametrize("case", ["water192", "water1536_cells", "golden_cells", "small_cells"])
+def test_neighbor_list_identical(case, water_dir):
+    say("case: neighbour list", case)
+    if case == "golden_cells":
+        d = GOLD
+        pos, elem, box = read_input_data(os.path.join(GOLD, "input.data"), ["H", "O"])
+        o = Oracle(d)
+        o.set_structure(pos, elem, box)
+        m = n2p2_amd.Mode(d)
+        cl = m.conv_length
+        m.set_structures(remap(pos, box) * cl, elem, (box * cl)[None])
+        o.build_neighbors(BRUTE)
+    else:
+        if case == "water192":
+            pos, elem, box = synth.water_box(4)
+        elif case == "water1536_cells":
+            pos, elem, box = synth.water_box(8)
+        else:
+            rng = np.random.default_rng(11)
+            box = np.eye(3) * 4.2
+            pos = remap(rng.random((9, 3)) * 4.2, box)
+            elem = np.array([1, 1, 1, 0, 0, 0, 0, 0, 0], np.int32)
+        o = Oracle(water_dir)
+        o.set_structure(pos, elem, box)
+        o.build_neighbors(BRUTE)
+        m = n2p2_amd.Mode(water_dir)
+        m.set_structures(remap(pos, box), elem, box[None])
+    m.calculate_neighbor_list()
+    n = compare_neighbors(m, o)
+    say("  %d entries identical" % n)
+
+
+def check_against_oracle(m, o, force_mode=PULL):
+    o.predict(force_mode, BRUTE)
+    G, Go = m.G(), o.G()
+    # scaled G are O(1) (Smin + factor * (G - mean)); relative to max(1, |G|)
+    gerr = (np.abs(G - Go) / np.maximum(np.abs(Go), 1.0)).max()
+    E, Eo = m.energies()[0], o.energy()
+    F, Fo = m.forces(), o.forces()
+    dEdG, dEdGo = m.dEdG(), o.dEdG()
+    derr = np.abs(dEdG - dEdGo).max() / max(np.abs(dEdGo).max(), 1e-30)
+    ferr = np.abs(F - Fo).max()
+    say("  G rel %.2e  dEdG rel %.2e  E rel %.2e  F abs %.2e" % (gerr, derr, abs(E - Eo) / abs(Eo), ferr))
+    assert gerr <= 1e-12
+    assert derr <= 1e-10
+    assert abs(E - Eo) <= E_REL * abs(Eo)
+    assert ferr <= F_ABS
+
+
+def test_water192_plain_vs_oracle(water_dir):
+    say("case: water-192 plain FP64 vs oracle")
+    pos, elem, box = synth.water_box(4)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    o = Oracle(water_dir)
+    o.set_structure(pos, elem, box)
+    check_against_oracle(m, o)
+
+
+def test_water12288_plain_vs_oracle(water_dir):
+    say("case: water-12288 plain FP64 vs oracle (cell list)")
+    pos, elem, box = synth.water_box(16)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    o = Oracle(water_dir)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, CELL)
+    E, Eo = m.energies()[0], o.energy()
+    F, Fo = m.forces(), o.forces()
+    say("  E rel %.2e  F abs %.2e" % (abs(E - Eo) / abs(Eo), np.abs(F - Fo).max()))
+    assert abs(E - Eo) <= E_REL * abs(Eo)
+    assert np.abs(F - Fo).max() <= F_ABS
+
+
+def test_step_api_equals_evaluate(water_dir):
+    """calculate_symmetry_function_groups / _atomic_neural_networks / _energy /
+    _forces in sequence give the evaluate results."""
+    say("case: step-by-step Mode calls")
+    pos, elem, box = synth.water_box(4)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(pos, elem, box[None])
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    E1, F1 = m.energies(), m.forces()
+    m.calculate_neighbor_list()
+    m.calculate_symmetry_function_groups(True)
+    m.calculate_atomic_neural_networks(True)
+    m.calculate_energy()
+    m.calculate_forces()
+    # dG/dr pair terms are summed with LDS atomics: equal up to summation order
+    assert np.abs(m.energies() - E1).max() <= 1e-14 * np.abs(E1).max()
+    assert np.abs(m.forces() - F1).max() <= 1e-12
+
+
+@pytest.mark.parametrize("force_mode", [PULL, PUSH])
+def test_batched_small_cells(force_mode, water_dir):
+    """Several uneven structures in one call, including cells smaller than rc
+    (self images); pull form reproduces Mode::calculateForces, push form the
+    InterfaceLammps scatter."""
+    say("case: batch of 12 structures, force mode", force_mode)
+    pos, elem, boxes, off = synth.random_batch(n_struct=12, nmin=16, nmax=90)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_option(n2p2_amd.OPT_FORCE_MODE, force_mode)
+    gpos = np.vstack([remap(pos[off[s]:off[s + 1]], boxes[s]) for s in range(12)])
+    m.set_structures(gpos, elem, boxes, off)
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    E, F = m.energies(), m.forces()
+    for s in range(12):
+        a, b = off[s], off[s + 1]
+        o = Oracle(water_dir)
+        o.set_structure(pos[a:b], elem[a:b], boxes[s])
+        o.predict(force_mode, BRUTE)
+        assert abs(E[s] - o.energy()) <= E_REL * abs(o.energy()), s
+        assert np.abs(F[a:b] - o.forces()).max() <= F_ABS, s
+
+
+def test_update_positions(water_dir):
+    say("case: update_positions")
+    pos, elem, box = synth.water_box(4)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(pos, elem, box[None])
+    rng = np.random.default_rng(5)
+    p2 = remap(pos + rng.normal(0, 0.02, pos.shape), box)
+    m.update_positions(remap(p2, box))
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    o = Oracle(water_dir)
+    o.set_structure(p2, elem, box)
+    o.predict(PULL, BRUTE)
+    assert abs(m.energies()[0] - o.energy()) <= E_REL * abs(o.energy())
+    assert np.abs(m.forces() - o.forces()).max() <= F_ABS
