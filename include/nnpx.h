@@ -56,8 +56,11 @@ typedef enum {
 typedef enum {
     NNPX_OPT_FORCE_MODE = 0,  /* nnpx_force_mode, default NNPX_FORCES_PULL  */
     NNPX_OPT_TIMING = 1,      /* 1: record HIP events around every kernel   */
-    NNPX_OPT_NEIGHBOR_ALGO = 2 /* 0 auto (cell list where every axis holds
-                                 >= 3 cells), 1 brute force, 2 cell list    */
+    NNPX_OPT_NEIGHBOR_ALGO = 2, /* 0 auto (cell list where every axis holds
+                                  >= 3 cells), 1 brute force, 2 cell list   */
+    NNPX_OPT_BF16_TERMS = 3     /* mixed form NN: 1 = plain bf16 operands,
+                                  3 = hi/lo split, three bf16 MFMA products
+                                  (default 3)                              */
 } nnpx_option;
 
 /* Flags for nnpx_evaluate. */
@@ -153,6 +156,10 @@ nnpx_status nnpx_get_dGdr(nnpx_ctx* ctx, double* self, int ld_self,
                           double* nbr, int ld_nbr);
 
 /* ---- measurement ------------------------------------------------------ */
+/* No counterpart in the reference: multiplies two known 16x16 matrices with
+ * the bf16 MFMA operand layout the NN kernels use and returns the max abs
+ * error against the exact product (a lane-layout check of the device). */
+nnpx_status nnpx_mfma_selftest(nnpx_ctx* ctx, double* max_abs_err);
 /* With NNPX_OPT_TIMING set, summed event time and launch count of the named
  * kernel since the last reset ("nbr", "sf64", "nn64", "force64", "energy",
  * "sf32", "nnbf16", "force32", ...). */

@@ -23,6 +23,7 @@ FORCES_PUSH = 1
 OPT_FORCE_MODE = 0
 OPT_TIMING = 1
 OPT_NEIGHBOR_ALGO = 2
+OPT_BF16_TERMS = 3
 EVAL_FORCES = 1
 
 _STATUS = {0: "OK", 1: "ARGUMENT", 2: "FILE", 3: "MODEL", 4: "RANGE", 5: "STATE", 6: "HIP", 7: "UNSUPPORTED"}
@@ -72,6 +73,7 @@ _SIGS = {
     "nnpx_kernel_time": [ctypes.c_void_p, ctypes.c_char_p, _c_dbl_p, _c_i64_p],
     "nnpx_reset_kernel_times": [ctypes.c_void_p],
     "nnpx_work_counts": [ctypes.c_void_p, _c_i64_p, _c_i64_p, _c_i64_p, _c_i64_p, _c_i64_p],
+    "nnpx_mfma_selftest": [ctypes.c_void_p, _c_dbl_p],
 }
 
 
@@ -264,6 +266,11 @@ class Mode:
 
     def reset_kernel_times(self):
         self._ck(self._lib.nnpx_reset_kernel_times(self._h))
+
+    def mfma_selftest(self):
+        v = ctypes.c_double()
+        self._ck(self._lib.nnpx_mfma_selftest(self._h, ctypes.byref(v)))
+        return v.value
 
     def work_counts(self):
         v = [ctypes.c_int64() for _ in range(5)]

@@ -1,26 +1,880 @@
-// Mixed-precision form (see mixed.hpp).
+// Mixed-precision form of the energy/force path (see mixed.hpp).
+//
+// k_sf32     symmetry-function values in FP32, Kahan sums per lane, lanes
+//            combined in FP64, FP64 scaling (SymGrpExpRad::calculate /
+//            SymGrpExpAngn::calculate without derivatives,
+//            SymGrpExpRad.cpp:144-239, SymGrpExpAngn.cpp:156-403).
+// k_nnbf16   NeuralNetwork::propagate (NeuralNetwork.cpp:761-896) for 16
+//            atoms per wave as v_mfma_f32_16x16x16_bf16 tiles, and the
+//            adjoint dE/dG as the transposed GEMMs (replacing the reference's
+//            forward-mode NeuralNetwork::calculateDEdG, NeuralNetwork.cpp:396-442).
+// k_force32  fused dG/dr backward: recomputes each pair/triplet term and
+//            contracts it with dE/dG (Mode::calculateForces semantics,
+//            Mode.cpp:1922-1984) without storing dG/dr; FP64 accumulation.
+// k_fsum     per-atom gather of the per-entry contributions (reverse
+//            entries), fixed order.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
 #include "mixed.hpp"
+#include "nnpx.h"
 
 namespace nnpx {
+namespace {
+
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int kNNRows = 16;  // atoms per wave in the NN kernel
+
+// float -> bf16, round to nearest even
+__host__ __device__ inline uint16_t to_bf16(float f)
+{
+    uint32_t u;
+#ifdef __HIP_DEVICE_COMPILE__
+    u = __float_as_uint(f);
+#else
+    std::memcpy(&u, &f, 4);
+#endif
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float from_bf16(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+#ifdef __HIP_DEVICE_COMPILE__
+    return __uint_as_float(u);
+#else
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+#endif
+}
+
+// Cutoff function in FP32.  CT_COS uses sin(pi x) for the derivative (the
+// reference's sqrt(1 - cos^2) form cancels in single precision).
+__device__ inline void cutoff32(int type, float rc, float rci, float iw, float rcinv, float tanhPre,
+                                float r, float& fc, float& dfc)
+{
+    if (r >= rc) { fc = 0.f; dfc = 0.f; return; }
+    if (type == 1) {
+        if (r < rci) { fc = 1.f; dfc = 0.f; return; }
+        float s, c;
+        sincospif((r - rci) * iw, &s, &c);
+        fc = 0.5f * (c + 1.f);
+        dfc = -0.5f * iw * (float)M_PI * s;
+        return;
+    }
+    if (type == 2 || type == 3) {
+        const float t = tanhf(1.f - r * rcinv);
+        const float t2 = t * t;
+        const float pre = type == 3 ? tanhPre : 1.f;
+        fc = pre * t * t2;
+        dfc = 3.f * pre * t2 * (t2 - 1.f) * rcinv;
+        return;
+    }
+    if (type == 0) { fc = 1.f; dfc = 0.f; return; }
+    if (r < rci) { fc = 1.f; dfc = 0.f; return; }
+    const float x = (r - rci) * iw;
+    if (type == 4) {
+        const float t = 1.f / (x * x - 1.f);
+        const float t2 = __expf(t);
+        fc = (float)M_E * t2;
+        dfc = -2.f * (float)M_E * x * t * t * t2;
+    } else if (type == 5) {
+        fc = (2.f * x - 3.f) * x * x + 1.f;
+        dfc = x * (6.f * x - 6.f);
+    } else if (type == 6) {
+        const float x2 = x * x;
+        fc = ((15.f - 6.f * x) * x - 10.f) * x * x2 + 1.f;
+        dfc = x2 * ((60.f - 30.f * x) * x - 30.f);
+    } else if (type == 7) {
+        const float x2 = x * x;
+        fc = (x * (x * (20.f * x - 70.f) + 84.f) - 35.f) * x2 * x2 + 1.f;
+        dfc = x2 * x * (x * (x * (140.f * x - 420.f) + 420.f) - 140.f);
+    } else {
+        float x4 = x * x;
+        x4 *= x4;
+        fc = (x * (x * ((315.f - 70.f * x) * x - 540.f) + 420.f) - 126.f) * x * x4 + 1.f;
+        dfc = x4 * (x * (x * ((2520.f - 630.f * x) * x - 3780.f) + 2520.f) - 630.f);
+    }
+    dfc *= iw;
+}
+
+struct GroupF {
+    float rc, rc2, rci, iw, rcinv;
+};
+
+__device__ inline GroupF group_f(const GroupT& g)
+{
+    return GroupF{(float)g.rc, (float)g.rc2, (float)g.rci, (float)g.iw, (float)g.rcinv};
+}
+
+// LDS staging of one atom's neighbours (FP32 geometry).
+struct Stage {
+    float* x; float* y; float* z;  // dr
+    float* d;                      // |dr|
+    float* rinv;                   // 1/|dr|
+    float* fc; float* dfc;         // cutoff of the current group
+    int* el;                       // neighbour element
+    int* la; int* lb;              // group lists
+};
+
+__device__ inline Stage carve(float* base, int capL)
+{
+    Stage S;
+    S.x = base; S.y = S.x + capL; S.z = S.y + capL; S.d = S.z + capL; S.rinv = S.d + capL;
+    S.fc = S.rinv + capL; S.dfc = S.fc + capL;
+    S.el = reinterpret_cast<int*>(S.dfc + capL);
+    S.la = S.el + capL; S.lb = S.la + capL;
+    return S;
+}
+constexpr int kStageWords = 10;  // 32-bit words per neighbour slot
+
+__device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
+{
+    const int lane = threadIdx.x;
+    const StructT& St = s.st[s.sid[i]];
+    const int n = min(s.nb_n[i], capL);
+    const double* ri = s.pos + 3 * (size_t)i;
+    const size_t base = (size_t)i * s.cap;
+    for (int k = lane; k < n; k += kWave) {
+        const int j = s.nb_j[base + k];
+        int b0, b1, b2;
+        unpack_image(s.nb_img[base + k], b0, b1, b2);
+        double dr[3];
+        pair_vector(ri, s.pos + 3 * (size_t)j, St.box, b0, b1, b2, dr);
+        const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        S.x[k] = (float)dr[0];
+        S.y[k] = (float)dr[1];
+        S.z[k] = (float)dr[2];
+        S.d[k] = (float)d;
+        S.rinv[k] = (float)(1.0 / d);
+        S.el[k] = s.elem[j];
+    }
+    __syncthreads();
+    return n;
+}
+
+// Ordered list of neighbours with element `elem` inside rc.
+__device__ inline int make_list(const Stage& S, int n, int elem, float rc, int* out)
+{
+    const int lane = threadIdx.x;
+    int cnt = 0;
+    for (int b = 0; b < n; b += kWave) {
+        const int k = b + lane;
+        const bool ok = k < n && S.el[k] == elem && S.d[k] < rc;
+        const int incl = wave_incl_scan(ok ? 1 : 0);
+        if (ok) out[cnt + incl - 1] = k;
+        cnt += __shfl(incl, kWave - 1, kWave);
+    }
+    __syncthreads();
+    return cnt;
+}
+
+__device__ inline void group_cutoffs(const Stage& S, int n, int e1, int e2, int ctype, const GroupF& g,
+                                     float tanhPre)
+{
+    for (int k = threadIdx.x; k < n; k += kWave) {
+        float f = 0.f, df = 0.f;
+        if (S.el[k] == e1 || S.el[k] == e2) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, S.d[k], f, df);
+        S.fc[k] = f;
+        S.dfc[k] = df;
+    }
+    __syncthreads();
+}
+
+// Decode pair t of the group's enumeration into neighbour slots j < k.
+__device__ inline void pair_of(int t, bool same, int nA, int full, const int* la, const int* lb, int& j, int& k)
+{
+    int a, b;
+    if (same) {
+        if (t < full) {
+            a = t % nA;
+            b = (a + t / nA + 1) % nA;
+        } else {
+            a = t - full;
+            b = a + nA / 2;
+        }
+    } else {
+        a = t % nA;
+        b = t / nA;
+    }
+    j = la[a];
+    k = lb[b];
+    if (j > k) { const int x = j; j = k; k = x; }
+}
+
+__device__ inline int pair_count(bool same, int nA, int nB, int& full)
+{
+    if (same) {
+        const int D = (nA - 1) / 2;
+        full = nA * D;
+        return full + ((nA % 2 == 0) ? nA / 2 : 0);
+    }
+    full = 0;
+    return nA * nB;
+}
+
+__device__ inline float powi(float x, int n) { return pow_int<float>(x, n); }
+
+// Kahan summation step.
+__device__ inline void kahan(float& s, float& c, float x)
+{
+    const float y = x - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// ------------------------------------------------------------- SF values --
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
+                                             SysView s, double* __restrict__ G, float* __restrict__ Gin, int ldG,
+                                             int ldK, int capL)
+{
+    extern __shared__ float sm[];
+    const Stage S = carve(sm, capL);
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = s.elem[i];
+    const ElementT& E = M->el[e];
+    const int ctype = M->cutoffType;
+    const float tanhPre = (float)M->tanhPre;
+    const int n = stage_atom(s, i, capL, S);
+    double* Gi = G + (size_t)i * ldG;
+    float* Ii = Gin + (size_t)i * ldK;
+    for (int q = E.nsf + lane; q < ldK; q += kWave) Ii[q] = 0.f;
+
+    for (int gi = 0; gi < E.ngrp; ++gi) {
+        const GroupT& GT = M->grp[E.grp0 + gi];
+        const GroupF g = group_f(GT);
+        const int nmem = GT.nmem;
+        const MemberF* mf = MF + GT.mem0;
+        group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        float acc[MAXM], cmp[MAXM];
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
+        if (GT.type == 2) {
+            for (int k = lane; k < n; k += kWave) {
+                if (S.el[k] != GT.e1 || !(S.d[k] < g.rc)) continue;
+                const float r = S.d[k], fc = S.fc[k];
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q)
+                    if (q < nmem) {
+                        const float dr = r - mf[q].rs;
+                        kahan(acc[q], cmp[q], __expf(-mf[q].eta * dr * dr) * fc);
+                    }
+            }
+        } else {
+            const bool same = GT.e1 == GT.e2;
+            const int nA = make_list(S, n, GT.e1, g.rc, S.la);
+            const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+            const int* lb = same ? S.la : S.lb;
+            int full;
+            const int total = pair_count(same, nA, nB, full);
+            for (int t = lane; t < total; t += kWave) {
+                int j, k;
+                pair_of(t, same, nA, full, S.la, lb, j, k);
+                const float x1 = S.x[j], y1 = S.y[j], z1 = S.z[j];
+                const float x2 = S.x[k], y2 = S.y[k], z2 = S.z[k];
+                const float x3 = x2 - x1, y3 = y2 - y1, z3 = z2 - z1;
+                const float r2jk = x3 * x3 + y3 * y3 + z3 * z3;
+                if (!(r2jk < g.rc2)) continue;
+                const float rij = S.d[j], rik = S.d[k];
+                const float rjk = sqrtf(r2jk);
+                float fcjk, dfcjk;
+                cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
+                const float cost = (x1 * x2 + y1 * y2 + z1 * z2) * S.rinv[j] * S.rinv[k];
+                const float pfc = S.fc[j] * S.fc[k] * fcjk;
+                const float r2sum = rij * rij + rik * rik + r2jk;
+                float vexp = 0.f;
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q)
+                    if (q < nmem) {
+                        const MemberF& m = mf[q];
+                        if (m.calcExp) {
+                            if (m.rs > 0.f) {
+                                const float a = rij - m.rs, b = rik - m.rs, c = rjk - m.rs;
+                                vexp = __expf(-m.eta * (a * a + b * b + c * c));
+                            } else {
+                                vexp = __expf(-m.eta * r2sum);
+                            }
+                        }
+                        const float pl = 1.f + m.lambda * cost;
+                        float fg = 0.f;
+                        if (pl > 0.f) fg = vexp * (m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f));
+                        kahan(acc[q], cmp[q], fg * pl * pfc);
+                    }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q)
+            if (q < nmem) {
+                double v = wave_sum((double)acc[q] - (double)cmp[q]);
+                if (lane == 0) {
+                    const MemberT& mt = M->mem[GT.mem0 + q];
+                    if (GT.type == 3) v *= mt.gnorm;
+                    const double gs = scale_sf(M->scale[E.sf0 + mt.sf], v);
+                    Gi[mt.sf] = gs;
+                    Ii[mt.sf] = (float)gs;
+                }
+            }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ BF16 MFMA NN --
+struct NNMix {
+    const uint16_t* w;
+    const float* b;
+    const float* sffac;
+    const int* atoms;
+    int blk_off[kMaxElem + 1];
+    int atom_off[kMaxElem + 1];
+    int wW[kMaxElem][kMaxLayers], wT[kMaxElem][kMaxLayers], bo[kMaxElem][kMaxLayers];
+    int pin[kMaxElem][kMaxLayers], pout[kMaxElem][kMaxLayers];
+    int nelem, ldK, ldG, stride, terms;
+};
+
+__device__ inline float act32(int a, float x, float& d)
+{
+    float v;
+    switch (a) {
+    case AF_IDENTITY: d = 1.f; return x;
+    case AF_TANH: v = tanhf(x); d = 1.f - v * v; return v;
+    case AF_LOGISTIC:
+        if (x > 35.f) { d = 0.f; return 1.f; }
+        if (x < -35.f) { d = 0.f; return 0.f; }
+        v = 1.f / (1.f + __expf(-x)); d = v * (1.f - v); return v;
+    case AF_SOFTPLUS:
+        if (x > 35.f) { d = 1.f; return x; }
+        if (x < -35.f) { d = 0.f; return 0.f; }
+        { const float t = __expf(x); d = 1.f / (1.f + 1.f / t); return log1pf(t); }
+    case AF_RELU: if (x > 0.f) { d = 1.f; return x; } d = 0.f; return 0.f;
+    case AF_GAUSSIAN: { const float t = __expf(-0.5f * x * x); d = -x * t; return t; }
+    case AF_COS: d = -sinf(x); return cosf(x);
+    case AF_REVLOGISTIC: v = 1.f / (1.f + __expf(-x)); d = v * (v - 1.f); return 1.f - v;
+    case AF_EXP: v = __expf(-x); d = -v; return v;
+    case AF_HARMONIC: d = 2.f * x; return x * x;
+    }
+    d = 0.f;
+    return 0.f;
+}
+
+// A fragment: rows = atoms (lane % 16), k = k0 + 4*(lane/16) + 0..3 from an
+// LDS row-major [16][stride] FP32 tile, split into bf16 hi and lo.
+__device__ inline void a_frag(const float* tile, int stride, int k0, s4& hi, s4& lo)
+{
+    const int lane = threadIdx.x;
+    const float* p = tile + (lane & 15) * stride + k0 + 4 * (lane >> 4);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const uint16_t h = to_bf16(p[v]);
+        hi[v] = (short)h;
+        lo[v] = (short)to_bf16(p[v] - from_bf16(h));
+    }
+}
+
+// B fragment: B[k][n] = Wt[n][k] for n = n0 + lane % 16, k = k0 + 4*(lane/16)
+// + 0..3, from a row-major bf16 matrix with row length ld (hi; lo at +lo_off).
+__device__ inline void b_frag(const uint16_t* __restrict__ w, int ld, int lo_off, int n0, int k0, s4& hi, s4& lo)
+{
+    const int lane = threadIdx.x;
+    const uint16_t* p = w + (size_t)(n0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4);
+    const uint2 h = *reinterpret_cast<const uint2*>(p);
+    const uint2 l = *reinterpret_cast<const uint2*>(p + lo_off);
+    hi = __builtin_bit_cast(s4, h);
+    lo = __builtin_bit_cast(s4, l);
+}
+
+__device__ inline f4 mma(f4 c, const s4& ah, const s4& al, const s4& bh, const s4& bl, int terms)
+{
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
+    if (terms > 1) {
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, c, 0, 0, 0);
+    }
+    return c;
+}
+
+__global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNMix a, const float* __restrict__ Gin,
+                                               double* __restrict__ Eatom, double* __restrict__ dEdG,
+                                               float* __restrict__ dEdGf)
+{
+    extern __shared__ float sm[];
+    const int lane = threadIdx.x;
+    int e = 0;
+    while (e + 1 < a.nelem && (int)blockIdx.x >= a.blk_off[e + 1]) ++e;
+    const NNT& N = M->el[e].nn;
+    const int nl = N.nl;
+    const int st = a.stride;
+    float* bufA = sm;
+    float* bufB = bufA + kNNRows * st;
+    float* dfd = bufB + kNNRows * st;  // [layer][16][st]
+    const int tile = (int)blockIdx.x - a.blk_off[e];
+    const int first = a.atom_off[e] + tile * kNNRows;
+    const int cnt = min(kNNRows, a.atom_off[e + 1] - first);
+
+    // layer-0 input rows
+    const int pin0 = a.pin[e][1];
+    for (int x = lane; x < kNNRows * pin0; x += kWave) {
+        const int r = x / pin0, k = x % pin0;
+        bufA[r * st + k] = r < cnt ? Gin[(size_t)a.atoms[first + r] * a.ldK + k] : 0.f;
+    }
+    __syncthreads();
+    float* in = bufA;
+    float* out = bufB;
+    for (int l = 1; l < nl; ++l) {
+        const int pi = a.pin[e][l], po = a.pout[e][l];
+        const uint16_t* W = a.w + a.wW[e][l];
+        const float* bias = a.b + a.bo[e][l];
+        const float norm = N.normalize ? 1.f / N.n[l - 1] : 1.f;
+        for (int n0 = 0; n0 < po; n0 += 16) {
+            f4 c = {0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < pi; k0 += 16) {
+                s4 ah, al, bh, bl;
+                a_frag(in, st, k0, ah, al);
+                b_frag(W, pi, po * pi, n0, k0, bh, bl);
+                c = mma(c, ah, al, bh, bl, a.terms);
+            }
+            const int n = n0 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 4 * (lane >> 4) + v;
+                float d;
+                const float x = (c[v] + bias[n]) * norm;
+                const float y = act32(N.act[l], x, d);
+                out[r * st + n] = n < N.n[l] ? y : 0.f;
+                dfd[(l * kNNRows + r) * st + n] = n < N.n[l] ? d : 0.f;
+            }
+        }
+        __syncthreads();
+        float* t = in; in = out; out = t;
+    }
+    if (lane < cnt) Eatom[a.atoms[first + lane]] = (double)in[lane * st];
+    // adjoint: delta of the output layer is f'_out for the energy neuron
+    float* dl = out;
+    const int pl = a.pout[e][nl - 1];
+    for (int x = lane; x < kNNRows * pl; x += kWave) {
+        const int r = x / pl, n = x % pl;
+        dl[r * st + n] = n == 0 ? dfd[((nl - 1) * kNNRows + r) * st] : 0.f;
+    }
+    __syncthreads();
+    float* dn = in;
+    for (int l = nl - 1; l >= 1; --l) {
+        const int pi = a.pin[e][l], po = a.pout[e][l];
+        const uint16_t* WT = a.w + a.wT[e][l];
+        const float norm = N.normalize ? 1.f / N.n[l - 1] : 1.f;
+        for (int m0 = 0; m0 < pi; m0 += 16) {
+            f4 c = {0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < po; k0 += 16) {
+                s4 ah, al, bh, bl;
+                a_frag(dl, st, k0, ah, al);
+                b_frag(WT, po, po * pi, m0, k0, bh, bl);
+                c = mma(c, ah, al, bh, bl, a.terms);
+            }
+            const int m = m0 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 4 * (lane >> 4) + v;
+                float g = c[v] * norm;
+                if (l > 1) {
+                    dn[r * st + m] = g * dfd[((l - 1) * kNNRows + r) * st + m];
+                } else if (r < cnt && m < N.n[0]) {
+                    const size_t at = (size_t)a.atoms[first + r];
+                    dEdG[at * a.ldG + m] = (double)g;
+                    dEdGf[at * a.ldG + m] = g * a.sffac[e * kMaxSF + m];
+                }
+            }
+        }
+        __syncthreads();
+        float* t = dl; dl = dn; dn = t;
+    }
+}
+
+// --------------------------------------------------- fused force backward --
+template <int MAXM>
+__global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
+                                                SysView s, const float* __restrict__ fw, int ldG,
+                                                double* __restrict__ F, double* __restrict__ Fpart, int capL)
+{
+    extern __shared__ float sm[];
+    const Stage S = carve(sm, capL);
+    double* ax = reinterpret_cast<double*>(sm + kStageWords * capL);
+    double* ay = ax + capL;
+    double* az = ay + capL;
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = s.elem[i];
+    const ElementT& E = M->el[e];
+    const int ctype = M->cutoffType;
+    const float tanhPre = (float)M->tanhPre;
+    const int n = stage_atom(s, i, capL, S);
+    for (int k = lane; k < n; k += kWave) { ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0; }
+    const float* wi = fw + (size_t)i * ldG;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    __syncthreads();
+
+    for (int gi = 0; gi < E.ngrp; ++gi) {
+        const GroupT& GT = M->grp[E.grp0 + gi];
+        const GroupF g = group_f(GT);
+        const int nmem = GT.nmem;
+        const MemberF* mf = MF + GT.mem0;
+        float w[MAXM];
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q) w[q] = q < nmem ? wi[mf[q].sf] : 0.f;
+        group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        if (GT.type == 2) {
+            // radial: each lane owns its neighbours, no conflicts
+            for (int k = lane; k < n; k += kWave) {
+                if (S.el[k] != GT.e1 || !(S.d[k] < g.rc)) continue;
+                const float r = S.d[k], fc = S.fc[k], dfc = S.dfc[k];
+                float P = 0.f;
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q)
+                    if (q < nmem) {
+                        const float dr = r - mf[q].rs;
+                        const float ex = __expf(-mf[q].eta * dr * dr);
+                        P += w[q] * (dfc - mf[q].twoEta * dr * fc) * ex;
+                    }
+                P *= S.rinv[k];
+                const double vx = (double)(P * S.x[k]), vy = (double)(P * S.y[k]), vz = (double)(P * S.z[k]);
+                fx -= vx; fy -= vy; fz -= vz;
+                ax[k] += vx; ay[k] += vy; az[k] += vz;
+            }
+            __syncthreads();
+            continue;
+        }
+        const bool same = GT.e1 == GT.e2;
+        const int nA = make_list(S, n, GT.e1, g.rc, S.la);
+        const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+        const int* lb = same ? S.la : S.lb;
+        int full;
+        const int total = pair_count(same, nA, nB, full);
+        for (int t = lane; t < total; t += kWave) {
+            int j, k;
+            pair_of(t, same, nA, full, S.la, lb, j, k);
+            const float x1 = S.x[j], y1 = S.y[j], z1 = S.z[j];
+            const float x2 = S.x[k], y2 = S.y[k], z2 = S.z[k];
+            const float x3 = x2 - x1, y3 = y2 - y1, z3 = z2 - z1;
+            const float r2jk = x3 * x3 + y3 * y3 + z3 * z3;
+            if (!(r2jk < g.rc2)) continue;
+            const float rij = S.d[j], rik = S.d[k];
+            const float ij = S.rinv[j], ik = S.rinv[k];
+            const float rjk = sqrtf(r2jk);
+            const float jk = 1.f / rjk;
+            float fcjk, dfcjk;
+            cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
+            const float fcij = S.fc[j], fcik = S.fc[k];
+            const float rinvijik = ij * ik;
+            const float cost = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
+            const float pfc = fcij * fcik * fcjk;
+            const float r2sum = rij * rij + rik * rik + r2jk;
+            const float pr1 = fcik * fcjk * S.dfc[j] * ij;
+            const float pr2 = fcij * fcjk * S.dfc[k] * ik;
+            const float pr3 = fcij * fcik * dfcjk * jk;
+            float A = 0.f, B = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, C3 = 0.f;
+            float vexp = 0.f;
+#pragma unroll
+            for (int q = 0; q < MAXM; ++q)
+                if (q < nmem) {
+                    const MemberF& m = mf[q];
+                    if (m.calcExp) {
+                        if (m.rs > 0.f) {
+                            const float a = rij - m.rs, b = rik - m.rs, c = rjk - m.rs;
+                            vexp = __expf(-m.eta * (a * a + b * b + c * c));
+                        } else {
+                            vexp = __expf(-m.eta * r2sum);
+                        }
+                    }
+                    const float pl = 1.f + m.lambda * cost;
+                    if (pl > 0.f) {
+                        const float u = w[q] * vexp * (m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f));
+                        A += u * m.zetaLambda;
+                        const float b = u * pl;
+                        B += b;
+                        const float c = b * m.twoEta;
+                        if (m.rs > 0.f) {
+                            C1 += c * (rij - m.rs) * ij;
+                            C2 += c * (rik - m.rs) * ik;
+                            C3 += c * (rjk - m.rs) * jk;
+                        } else {
+                            C0 += c;
+                        }
+                    }
+                }
+            C1 += C0; C2 += C0; C3 += C0;
+            const float P1 = pfc * (A * (rinvijik - cost * ij * ij) - C1) + B * pr1;
+            const float P2 = pfc * (A * (rinvijik - cost * ik * ik) - C2) + B * pr2;
+            const float P3 = pfc * (A * rinvijik + C3) - B * pr3;
+            const float p1x = P1 * x1, p1y = P1 * y1, p1z = P1 * z1;
+            const float p2x = P2 * x2, p2y = P2 * y2, p2z = P2 * z2;
+            const float p3x = P3 * x3, p3y = P3 * y3, p3z = P3 * z3;
+            fx -= (double)(p1x + p2x);
+            fy -= (double)(p1y + p2y);
+            fz -= (double)(p1z + p2z);
+            lds_add(ax + j, (double)(p1x + p3x));
+            lds_add(ay + j, (double)(p1y + p3y));
+            lds_add(az + j, (double)(p1z + p3z));
+            lds_add(ax + k, (double)(p2x - p3x));
+            lds_add(ay + k, (double)(p2y - p3y));
+            lds_add(az + k, (double)(p2z - p3z));
+        }
+        __syncthreads();
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (lane == 0) {
+        F[3 * (size_t)i] = fx;
+        F[3 * (size_t)i + 1] = fy;
+        F[3 * (size_t)i + 2] = fz;
+    }
+    double* fp = Fpart + (size_t)i * s.cap * 3;
+    for (int k = lane; k < n; k += kWave) {
+        fp[3 * k] = ax[k];
+        fp[3 * k + 1] = ay[k];
+        fp[3 * k + 2] = az[k];
+    }
+}
+
+// F_i += sum over i's entries (j, image) of the contribution stored on the
+// reverse entry (i, -image) of j's list; own images skipped in the pull form.
+__global__ __launch_bounds__(64) void k_fsum(SysView s, const double* __restrict__ Fpart, int push,
+                                             double* __restrict__ F)
+{
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int n = s.nb_n[i];
+    const size_t base = (size_t)i * s.cap;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int k = lane; k < n; k += kWave) {
+        const int j = s.nb_j[base + k];
+        if (j == i && !push) continue;
+        const int r = s.nb_rev[base + k];
+        if (r < 0) continue;
+        fx += Fpart[3 * (size_t)r];
+        fy += Fpart[3 * (size_t)r + 1];
+        fz += Fpart[3 * (size_t)r + 2];
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (lane == 0) {
+        F[3 * (size_t)i] += fx;
+        F[3 * (size_t)i + 1] += fy;
+        F[3 * (size_t)i + 2] += fz;
+    }
+}
+
+__global__ void k_mfma_probe(const float* A, const float* B, float* D)
+{
+    // A, B: 16x16 row-major; D = A * B (16x16 row-major)
+    const int lane = threadIdx.x;
+    s4 ah, al, bh, bl;
+    a_frag(A, 16, 0, ah, al);
+    // B fragment from B^T (row n, column k)
+    float bt[4];
+    for (int v = 0; v < 4; ++v) bt[v] = B[(4 * (lane >> 4) + v) * 16 + (lane & 15)];
+    for (int v = 0; v < 4; ++v) {
+        const uint16_t h = to_bf16(bt[v]);
+        bh[v] = (short)h;
+        bl[v] = (short)to_bf16(bt[v] - from_bf16(h));
+    }
+    f4 c = {0.f, 0.f, 0.f, 0.f};
+    c = mma(c, ah, al, bh, bl, 3);
+    for (int v = 0; v < 4; ++v) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = c[v];
+}
+
+int pow2_members(const ModelT& t)
+{
+    const int m = std::max(std::max(t.max_mem_ang, t.max_mem_rad), 1);
+    return m <= 4 ? 4 : m <= 8 ? 8 : m <= 16 ? 16 : 32;
+}
+
+}  // namespace
 
 void setup_mixed(MixedModel& mm, const HostModel& hm)
 {
-    (void)hm;
-    mm.ready = false;
+    free_mixed(mm);
+    const ModelT& t = hm.t;
+    std::vector<MemberF> mf(std::max(t.nmembers, 1));
+    std::vector<float> sffac((size_t)kMaxElem * kMaxSF, 0.f);
+    for (int e = 0; e < t.nelem; ++e) {
+        const ElementT& E = t.el[e];
+        for (int g = E.grp0; g < E.grp0 + E.ngrp; ++g) {
+            const GroupT& G = t.grp[g];
+            for (int q = G.mem0; q < G.mem0 + G.nmem; ++q) {
+                const MemberT& M = t.mem[q];
+                MemberF& f = mf[q];
+                f.eta = (float)M.eta;
+                f.rs = (float)M.rs;
+                f.lambda = (float)M.lambda;
+                f.zeta = (float)M.zeta;
+                f.gnorm = G.type == 3 ? (float)M.gnorm : 1.f;
+                f.twoEta = (float)(2.0 * M.eta);
+                f.zetaLambda = (float)M.zetaLambda;
+                f.sf = M.sf;
+                f.calcExp = M.calcExp;
+                f.zetaInt = M.zetaInt;
+                f.useIntPow = M.useIntPow;
+                sffac[(size_t)e * kMaxSF + M.sf] = (float)(G.type == 3 ? M.factorNorm : M.sfac);
+            }
+        }
+    }
+    // bf16 weights, padded to 16: W [out][in] and WT [in][out], hi then lo.
+    std::vector<uint16_t> w;
+    std::vector<float> b;
+    auto pad16 = [](int x) { return (x + 15) / 16 * 16; };
+    mm.maxpad = 16;
+    mm.ldK = 16;
+    for (int e = 0; e < t.nelem; ++e) {
+        const NNT& N = t.el[e].nn;
+        mm.ldK = std::max(mm.ldK, pad16(N.n[0]));
+        for (int l = 1; l < N.nl; ++l) {
+            const int nin = N.n[l - 1], nout = N.n[l];
+            const int pi = pad16(nin), po = pad16(nout);
+            mm.pin[e][l] = pi;
+            mm.pout[e][l] = po;
+            mm.maxpad = std::max(mm.maxpad, std::max(pi, po));
+            std::vector<float> W((size_t)po * pi, 0.f);
+            for (int k = 0; k < nout; ++k)
+                for (int j = 0; j < nin; ++j) W[(size_t)k * pi + j] = (float)hm.weights[N.woff[l] + (size_t)k * nin + j];
+            mm.wW[e][l] = (int)w.size();
+            std::vector<uint16_t> hi(W.size()), lo(W.size());
+            for (size_t q = 0; q < W.size(); ++q) {
+                hi[q] = to_bf16(W[q]);
+                lo[q] = to_bf16(W[q] - from_bf16(hi[q]));
+            }
+            w.insert(w.end(), hi.begin(), hi.end());
+            w.insert(w.end(), lo.begin(), lo.end());
+            mm.wT[e][l] = (int)w.size();
+            for (int j = 0; j < pi; ++j)
+                for (int k = 0; k < po; ++k) hi[(size_t)j * po + k] = to_bf16(W[(size_t)k * pi + j]);
+            for (int j = 0; j < pi; ++j)
+                for (int k = 0; k < po; ++k) {
+                    const float x = W[(size_t)k * pi + j];
+                    lo[(size_t)j * po + k] = to_bf16(x - from_bf16(to_bf16(x)));
+                }
+            w.insert(w.end(), hi.begin(), hi.end());
+            w.insert(w.end(), lo.begin(), lo.end());
+            mm.bo[e][l] = (int)b.size();
+            for (int k = 0; k < po; ++k) b.push_back(k < nout ? (float)hm.weights[N.boff[l] + k] : 0.f);
+        }
+    }
+    auto up = [](void** dst, const void* src, size_t bytes) {
+        if (hipMalloc(dst, std::max<size_t>(bytes, 16)) != hipSuccess) throw Error{NNPX_ERR_HIP, "hipMalloc mixed"};
+        if (hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) throw Error{NNPX_ERR_HIP, "hipMemcpy"};
+    };
+    up((void**)&mm.d_mem, mf.data(), sizeof(MemberF) * mf.size());
+    up((void**)&mm.d_sffac, sffac.data(), sizeof(float) * sffac.size());
+    up((void**)&mm.d_w, w.data(), sizeof(uint16_t) * w.size());
+    up((void**)&mm.d_b, b.data(), sizeof(float) * b.size());
+    mm.ready = true;
 }
 
 void free_mixed(MixedModel& mm)
 {
     if (mm.d_mem) (void)hipFree(mm.d_mem);
+    if (mm.d_sffac) (void)hipFree(mm.d_sffac);
     if (mm.d_w) (void)hipFree(mm.d_w);
     if (mm.d_b) (void)hipFree(mm.d_b);
+    const int terms = mm.terms;
     mm = MixedModel{};
+    mm.terms = terms;
 }
 
-void run_mixed(const MixedModel& mm, const ModelT&, const SysView&, const MixedBuffers&, bool, bool,
-               hipStream_t, const TimedFn&)
+void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
+               const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed)
 {
-    if (!mm.ready) throw Error{7, "mixed-precision form not available in this build"};
+    if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
+    if (s.natoms == 0) return;
+    const int capL = s.cap;
+    const int P = pow2_members(t);
+    const size_t lds_sf = sizeof(float) * (size_t)kStageWords * capL;
+    const size_t lds_f = lds_sf + sizeof(double) * 3 * (size_t)capL;
+    if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
+    timed("sf32", [&] {
+        switch (P) {
+        case 4: hipLaunchKernelGGL(k_sf32<4>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
+        case 8: hipLaunchKernelGGL(k_sf32<8>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
+        case 16: hipLaunchKernelGGL(k_sf32<16>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
+        default: hipLaunchKernelGGL(k_sf32<32>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
+        }
+    });
+    NNMix a{};
+    a.w = mm.d_w;
+    a.b = mm.d_b;
+    a.sffac = mm.d_sffac;
+    a.atoms = b.elem_atoms;
+    a.nelem = t.nelem;
+    a.ldK = mm.ldK;
+    a.ldG = b.ldG;
+    a.stride = mm.maxpad + 4;
+    a.terms = mm.terms;
+    int blocks = 0;
+    for (int e = 0; e < t.nelem; ++e) {
+        a.blk_off[e] = blocks;
+        a.atom_off[e] = b.elem_off[e];
+        blocks += (b.elem_off[e + 1] - b.elem_off[e] + kNNRows - 1) / kNNRows;
+        for (int l = 0; l < kMaxLayers; ++l) {
+            a.wW[e][l] = mm.wW[e][l];
+            a.wT[e][l] = mm.wT[e][l];
+            a.bo[e][l] = mm.bo[e][l];
+            a.pin[e][l] = mm.pin[e][l];
+            a.pout[e][l] = mm.pout[e][l];
+        }
+    }
+    a.blk_off[t.nelem] = blocks;
+    a.atom_off[t.nelem] = b.elem_off[t.nelem];
+    const size_t lds_nn = sizeof(float) * (size_t)kNNRows * a.stride * (2 + kMaxLayers);
+    timed("nnbf16", [&] {
+        hipLaunchKernelGGL(k_nnbf16, dim3(blocks), dim3(kWave), lds_nn, stream, d_model, a, b.Gin, b.Eatom, b.dEdG,
+                           b.dEdGf);
+    });
+    timed("energy", [&] { launch_energy(b.Eatom, b.atom_off, b.nstruct, b.E, stream); });
+    if (!forces) return;
+    timed("force32", [&] {
+        switch (P) {
+        case 4: hipLaunchKernelGGL(k_force32<4>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
+        case 8: hipLaunchKernelGGL(k_force32<8>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
+        case 16: hipLaunchKernelGGL(k_force32<16>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
+        default: hipLaunchKernelGGL(k_force32<32>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
+        }
+    });
+    timed("fsum", [&] {
+        hipLaunchKernelGGL(k_fsum, dim3(s.natoms), dim3(kWave), 0, stream, s, b.Fpart, push ? 1 : 0, b.F);
+    });
+}
+
+double mfma_selftest()
+{
+    std::vector<float> A(256), B(256), D(256, 0.f);
+    for (int i = 0; i < 16; ++i)
+        for (int k = 0; k < 16; ++k) {
+            A[i * 16 + k] = (float)((i * 7 + k * 3) % 11) - 5.f;
+            B[i * 16 + k] = (float)((i * 5 + k * 13) % 9) - 4.f;
+        }
+    float *dA, *dB, *dD;
+    if (hipMalloc(&dA, 1024) != hipSuccess || hipMalloc(&dB, 1024) != hipSuccess || hipMalloc(&dD, 1024) != hipSuccess)
+        throw Error{NNPX_ERR_HIP, "hipMalloc"};
+    (void)hipMemcpy(dA, A.data(), 1024, hipMemcpyHostToDevice);
+    (void)hipMemcpy(dB, B.data(), 1024, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, dA, dB, dD);
+    (void)hipMemcpy(D.data(), dD, 1024, hipMemcpyDeviceToHost);
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dD);
+    double err = 0.0;
+    for (int i = 0; i < 16; ++i)
+        for (int j = 0; j < 16; ++j) {
+            double ref = 0.0;
+            for (int k = 0; k < 16; ++k) ref += (double)A[i * 16 + k] * B[k * 16 + j];
+            err = std::max(err, std::fabs(ref - D[i * 16 + j]));
+        }
+    return err;
 }
 
 }  // namespace nnpx

@@ -1,11 +1,13 @@
 // Mixed-precision form of the energy/force path:
-//   * symmetry functions in FP32 with compensated (Neumaier) pair sums,
-//     scaling in FP64;
-//   * atomic NNs as BF16 MFMA GEMMs with FP32 accumulation over all atoms
-//     of an element, adjoint dE/dG by the transposed GEMMs;
-//   * a fused force kernel that recomputes the radial and angular terms,
-//     contracts them with dE/dG on the fly (no dG/dr tensor) and
-//     accumulates forces in FP64;
+//   * symmetry functions in FP32 with compensated (Kahan) pair sums per lane,
+//     lanes combined and scaled in FP64;
+//   * atomic NNs as BF16 MFMA GEMMs (v_mfma_f32_16x16x16_bf16) with FP32
+//     accumulation over all atoms of an element; the adjoint dE/dG runs as
+//     the transposed GEMMs.  Operands are split into bf16 hi + lo parts and
+//     the three leading products are accumulated (1 term = plain BF16);
+//   * a fused force kernel that recomputes the radial and angular terms and
+//     contracts them with dE/dG on the fly -- per triplet three scalars
+//     replace the per-member dG/dr -- with FP64 force accumulation;
 //   * per-structure energies reduced in FP64.
 #pragma once
 
@@ -22,31 +24,35 @@ namespace nnpx {
 // FP32 copy of one group member's constants.
 struct MemberF {
     float eta, rs, lambda, zeta;
-    float factorNorm, factorDeriv, zetaLambda, gnorm;
-    float sfac;
-    int sf, calcExp, zetaInt;
-    int useIntPow, slot1, slot2, pad;
+    float gnorm;       // angular: 2^(1-zeta) (G normalisation)
+    float twoEta;      // 2 eta
+    float zetaLambda;  // zeta * lambda
+    float pad;
+    int sf, calcExp, zetaInt, useIntPow;
 };
 
 struct MixedModel {
-    int ldK = 0;                // padded input width (multiple of 32)
-    int ldH = 0;                // padded hidden width (multiple of 32)
-    MemberF* d_mem = nullptr;   // [nmembers]
-    uint16_t* d_w = nullptr;    // BF16 weights per element/layer, padded tiles
-    float* d_b = nullptr;       // FP32 biases per element/layer, padded
-    int woff[kMaxElem][kMaxLayers] = {};  // offsets into d_w (elements of BF16)
-    int boff[kMaxElem][kMaxLayers] = {};  // offsets into d_b
-    int kin[kMaxElem][kMaxLayers] = {};   // padded in-width of layer l
-    int kout[kMaxElem][kMaxLayers] = {};  // padded out-width of layer l
+    int ldK = 0;                // padded NN input width (multiple of 16)
+    int maxpad = 0;             // max padded layer width
+    int terms = 3;              // bf16 product terms (1 or 3)
+    MemberF* d_mem = nullptr;   // [nmembers], same order as ModelT::mem
+    float* d_sffac = nullptr;   // [nelem*kMaxSF] force weight factor per SF
+    uint16_t* d_w = nullptr;    // bf16 weights: per element/layer W hi, W lo, WT hi, WT lo
+    float* d_b = nullptr;       // FP32 biases, padded
+    int wW[kMaxElem][kMaxLayers] = {};   // offset of W [out_pad][in_pad] (hi; lo follows)
+    int wT[kMaxElem][kMaxLayers] = {};   // offset of WT [in_pad][out_pad] (hi; lo follows)
+    int bo[kMaxElem][kMaxLayers] = {};   // offset of the bias
+    int pin[kMaxElem][kMaxLayers] = {};  // padded in-width of layer l
+    int pout[kMaxElem][kMaxLayers] = {}; // padded out-width of layer l
     bool ready = false;
 };
 
 struct MixedBuffers {
     int ldG;
     double* G;         // [N*ldG] scaled SFs (FP64 scaling of FP32 sums)
-    uint16_t* Gbf;     // [N*ldK] BF16 NN input
+    float* Gin;        // [N*ldK] FP32 NN input (split into bf16 operands by the NN kernel)
     double* dEdG;      // [N*ldG] dE/dG (FP64 copy of the FP32 adjoint)
-    float* dEdGf;      // [N*ldG] dE/dG * scaling factor, FP32, for the force kernel
+    float* dEdGf;      // [N*ldG] dE/dG times the SF's force factor, FP32
     double* Eatom;     // [N]
     double* E;         // [nstruct]
     double* F;         // [3N]
@@ -62,8 +68,11 @@ using TimedFn = std::function<void(const char*, const std::function<void()>&)>;
 
 void setup_mixed(MixedModel& mm, const HostModel& hm);
 void free_mixed(MixedModel& mm);
-void run_mixed(const MixedModel& mm, const ModelT& t, const SysView& s, const MixedBuffers& b,
-               bool forces, bool push, hipStream_t stream, const TimedFn& timed);
+void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
+               const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed);
+// Checks the MFMA lane layout assumed by the NN kernels on a known product;
+// returns the max abs error.
+double mfma_selftest();
 
 }  // namespace nnpx
 

@@ -413,7 +413,7 @@ void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check)
     build_neighbors(c, check);
     if (precision == NNPX_MIXED) {
         MixedBuffers mb = mixed_buffers(c);
-        run_mixed(c->mixed, c->hm.t, sysview(c), mb, forces, c->force_mode == NNPX_FORCES_PUSH,
+        run_mixed(c->mixed, c->hm.t, c->d_model, sysview(c), mb, forces, c->force_mode == NNPX_FORCES_PUSH,
                   c->stream, [&](const char* name, const std::function<void()>& f) { timed(c, name, f); });
         c->sf_valid = true;
         c->nn_valid = true;
@@ -457,7 +457,7 @@ MixedBuffers mixed_buffers(nnpx_ctx* c)
     MixedBuffers m{};
     m.ldG = t.max_sf;
     m.G = buf<double>(c, "G", (size_t)c->N * m.ldG);
-    m.Gbf = buf<uint16_t>(c, "Gbf", (size_t)c->N * c->mixed.ldK);
+    m.Gin = buf<float>(c, "Gin", (size_t)c->N * c->mixed.ldK);
     m.dEdG = buf<double>(c, "dEdG", (size_t)c->N * m.ldG);
     m.dEdGf = buf<float>(c, "dEdGf", (size_t)c->N * m.ldG);
     m.Eatom = buf<double>(c, "Eatom", c->N);
@@ -531,6 +531,11 @@ nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
         case NNPX_OPT_NEIGHBOR_ALGO:
             if (value < 0 || value > 2) fail(NNPX_ERR_ARGUMENT, "bad neighbour algorithm");
             c->nbr_algo = (int)value;
+            break;
+        case NNPX_OPT_BF16_TERMS:
+            if (value != 1 && value != 3) fail(NNPX_ERR_ARGUMENT, "bf16 terms must be 1 or 3");
+            c->mixed.terms = (int)value;
+            invalidate(c);
             break;
         default: fail(NNPX_ERR_ARGUMENT, "unknown option");
         }
@@ -907,6 +912,14 @@ nnpx_status nnpx_get_dGdr(nnpx_ctx* c, double* self, int ld_self, double* nbr, i
                 }
             }
         }
+    });
+}
+
+nnpx_status nnpx_mfma_selftest(nnpx_ctx* c, double* err)
+{
+    return guard(c, [&] {
+        if (!err) fail(NNPX_ERR_ARGUMENT, "null output");
+        *err = mfma_selftest();
     });
 }
 

@@ -1,0 +1,153 @@
+"""Mixed-precision form (FP32 symmetry functions, BF16 MFMA networks, fused
+force kernel, FP64 reductions) against the FP64 oracle on the GPU.
+
+Stated bar: force RMS error <= 1e-3 eV/A against the FP64 oracle; the errors
+of each case are printed so the test log states them per configuration.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import n2p2_amd
+from n2p2_amd import read_input_data, remap, synth
+from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "H2O_RPBE-D3")
+F_RMS = 1e-3
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def water_dir(tmp_path_factory):
+    return synth.water_model(str(tmp_path_factory.mktemp("water")))
+
+
+def test_mfma_lane_layout():
+    m = n2p2_amd.Mode()
+    err = m.mfma_selftest()
+    say("mfma self-test max abs err %.3e" % err)
+    assert err <= 1e-6
+
+
+def errors(m, o):
+    F, Fo = m.forces(), o.forces()
+    E, Eo = m.energies(), np.atleast_1d(o.energy())
+    rms = float(np.sqrt(np.mean((F - Fo) ** 2)))
+    mx = float(np.abs(F - Fo).max())
+    erel = float(np.abs(E[0] - Eo[0]) / abs(Eo[0]))
+    return rms, mx, erel
+
+
+@pytest.mark.parametrize("terms", [3, 1])
+def test_water192_mixed(terms, water_dir):
+    say("case: water-192 mixed, bf16 terms", terms)
+    pos, elem, box = synth.water_box(4)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_option(n2p2_amd.OPT_BF16_TERMS, terms)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    o = Oracle(water_dir)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE)
+    G, Go = m.G(), o.G()
+    gerr = float((np.abs(G - Go) / np.maximum(np.abs(Go), 1.0)).max())
+    dEdG, dEdGo = m.dEdG(), o.dEdG()
+    derr = float(np.abs(dEdG - dEdGo).max() / np.abs(dEdGo).max())
+    rms, mx, erel = errors(m, o)
+    say("  G err %.2e  dEdG rel %.2e  E rel %.2e  F rms %.2e max %.2e (|F| rms %.3f)"
+        % (gerr, derr, erel, rms, mx, np.sqrt(np.mean(o.forces() ** 2))))
+    assert gerr <= 1e-5
+    assert rms <= F_RMS
+    if terms == 3:
+        assert derr <= 1e-4
+        assert erel <= 1e-5
+
+
+def test_water12288_mixed(water_dir):
+    say("case: water-12288 mixed")
+    pos, elem, box = synth.water_box(16)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    o = Oracle(water_dir)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, CELL)
+    rms, mx, erel = errors(m, o)
+    say("  E rel %.2e  F rms %.2e max %.2e" % (erel, rms, mx))
+    assert rms <= F_RMS
+    assert erel <= 1e-5
+
+
+def test_golden_h2o_mixed():
+    """The H2O_RPBE-D3 model (CT_TANHU, rs > 0 radial terms, normalised units)."""
+    say("case: H2O_RPBE-D3 mixed")
+    pos, elem, box = read_input_data(os.path.join(GOLD, "input.data"), ["H", "O"])
+    m = n2p2_amd.Mode(GOLD)
+    cl, ce = m.conv_length, m.conv_energy
+    m.set_structures(remap(pos, box) * cl, elem, (box * cl)[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    o = Oracle(GOLD)
+    o.read_structure(os.path.join(GOLD, "input.data"))
+    o.predict(PULL, BRUTE)
+    F = m.forces() * (1.0 / (ce / cl))  # physical units (eV/A here: Ha/bohr scaled)
+    Fo = o.forces()
+    rms = float(np.sqrt(np.mean((F - Fo) ** 2)))
+    say("  F rms %.2e max %.2e (|F| rms %.3e)" % (rms, np.abs(F - Fo).max(), np.sqrt(np.mean(Fo ** 2))))
+    assert rms <= F_RMS
+
+
+@pytest.mark.parametrize("force_mode", [PULL, PUSH])
+def test_batched_mixed(force_mode, water_dir):
+    say("case: mixed batch of 24 structures, force mode", force_mode)
+    pos, elem, boxes, off = synth.random_batch(n_struct=24, nmin=16, nmax=200)
+    gpos = np.vstack([remap(pos[off[s]:off[s + 1]], boxes[s]) for s in range(24)])
+    m = n2p2_amd.Mode(water_dir)
+    m.set_option(n2p2_amd.OPT_FORCE_MODE, force_mode)
+    m.set_structures(gpos, elem, boxes, off)
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    E, F = m.energies(), m.forces()
+    Fo = np.zeros_like(F)
+    for s in range(24):
+        a, b = off[s], off[s + 1]
+        o = Oracle(water_dir)
+        o.set_structure(pos[a:b], elem[a:b], boxes[s])
+        o.predict(force_mode, BRUTE)
+        Fo[a:b] = o.forces()
+        assert abs(E[s] - o.energy()) <= 1e-5 * max(1.0, abs(o.energy())), s
+    rms = float(np.sqrt(np.mean((F - Fo) ** 2)))
+    say("  F rms %.2e max %.2e" % (rms, np.abs(F - Fo).max()))
+    assert rms <= F_RMS
+
+
+def test_plain_forces_match_finite_differences(water_dir):
+    """FP64 GPU forces vs central differences of the FP64 GPU energy
+    (nnp-checkf.cpp:173-295 semantics, threshold 10*h^2)."""
+    say("case: plain FP64 forces vs finite differences")
+    pos, elem, box = synth.water_box(4)
+    pos = remap(pos, box)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(pos, elem, box[None])
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    F = m.forces()
+    h = 1e-4
+    worst = 0.0
+    for i in (0, 1, 2, 100):
+        for c in range(3):
+            e = []
+            for sgn in (-1, 1):
+                p = pos.copy()
+                p[i, c] += sgn * h
+                m.update_positions(remap(p, box))
+                m.evaluate(n2p2_amd.PLAIN_FP64, forces=False)
+                e.append(m.energies()[0])
+            fd = -(e[1] - e[0]) / (2 * h)
+            worst = max(worst, abs(fd - F[i, c]))
+    say("  max |F - F_fd| = %.2e (threshold %.1e)" % (worst, 10 * h * h))
+    assert worst <= 10 * h * h
