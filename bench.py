@@ -55,7 +55,7 @@ def time_form(m, form, steps, warmup):
     m.synchronize()
     dt = time.perf_counter() - t0
     times = {}
-    for k in ("nbr", "sf64", "nn64", "energy", "force64", "sf32", "nnbf16", "force32", "fsum"):
+    for k in ("nbr", "rev", "sf64", "nn64", "energy", "force64", "sf32", "nnbf16", "force32"):
         ms, n = m.kernel_time(k)
         if n:
             times[k] = (ms, n)
@@ -145,7 +145,7 @@ def main():
     plain_rate = n * max(5, args.steps // 5) / dt_plain
 
     sf_flop, nn_flop = flop_counts(w, nn_macs, n)
-    dom = max((k for k in times if k != "nbr"), key=lambda k: times[k][0])
+    dom = max((k for k in times if k not in ("nbr", "rev")), key=lambda k: times[k][0])
     ms, launches = times[dom]
     avg_s = ms / launches / 1e3
     if dom in ("sf64", "sf32", "force32"):

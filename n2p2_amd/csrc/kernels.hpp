@@ -30,14 +30,17 @@ struct SysView {
     int* cell_count;       // [ncells]
     int* cell_start;       // [ncells+1]
     int* cell_atoms;       // [N]
+    double* cell_pos;      // [3N] positions in cell order
 };
 
 void fill_int(int* p, int v, size_t n, hipStream_t stream);
 
 // Neighbour list (Structure::calculateNeighborList, Structure.cpp:287-398):
-// cell assignment, per-atom build (brute force or cell list, same set and
-// order), reverse entries.
+// cell assignment and per-atom build (brute force or cell list, same set
+// and order).
 void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream);
+// Reverse entries: for (i -> j, image) the slot of (j -> i, -image).
+void launch_reverse(const SysView& s, hipStream_t stream);
 
 // FP64 symmetry functions; with deriv also dG/dr in the reference layout.
 struct SF64Out {

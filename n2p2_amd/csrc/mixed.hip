@@ -10,9 +10,9 @@
 //            forward-mode NeuralNetwork::calculateDEdG, NeuralNetwork.cpp:396-442).
 // k_force32  fused dG/dr backward: recomputes each pair/triplet term and
 //            contracts it with dE/dG (Mode::calculateForces semantics,
-//            Mode.cpp:1922-1984) without storing dG/dr; FP64 accumulation.
-// k_fsum     per-atom gather of the per-entry contributions (reverse
-//            entries), fixed order.
+//            Mode.cpp:1922-1984) without storing dG/dr; FP64 accumulation
+//            in LDS per neighbour entry, pushed with FP64 atomics.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -52,20 +52,23 @@ __host__ __device__ inline float from_bf16(uint16_t h)
 #endif
 }
 
-// Cutoff function in FP32.  CT_COS uses sin(pi x) for the derivative (the
-// reference's sqrt(1 - cos^2) form cancels in single precision).
+// Cutoff function in FP32, branch-light.  CT_COS uses the hardware sine and
+// cosine on x in [0, 1] (arguments in revolutions) and sin(pi x) for the
+// derivative (the reference's sqrt(1 - cos^2) form cancels in single
+// precision).
 __device__ inline void cutoff32(int type, float rc, float rci, float iw, float rcinv, float tanhPre,
                                 float r, float& fc, float& dfc)
 {
-    if (r >= rc) { fc = 0.f; dfc = 0.f; return; }
     if (type == 1) {
-        if (r < rci) { fc = 1.f; dfc = 0.f; return; }
-        float s, c;
-        sincospif((r - rci) * iw, &s, &c);
-        fc = 0.5f * (c + 1.f);
-        dfc = -0.5f * iw * (float)M_PI * s;
+        const float x = fminf(fmaxf((r - rci) * iw, 0.f), 1.f);
+        const float c = __builtin_amdgcn_cosf(0.5f * x);
+        const float s = __builtin_amdgcn_sinf(0.5f * x);
+        const bool in = r < rc;
+        fc = in ? 0.5f * (c + 1.f) : 0.f;
+        dfc = in ? -0.5f * iw * (float)M_PI * s : 0.f;
         return;
     }
+    if (r >= rc) { fc = 0.f; dfc = 0.f; return; }
     if (type == 2 || type == 3) {
         const float t = tanhf(1.f - r * rcinv);
         const float t2 = t * t;
@@ -111,26 +114,32 @@ __device__ inline GroupF group_f(const GroupT& g)
     return GroupF{(float)g.rc, (float)g.rc2, (float)g.rci, (float)g.iw, (float)g.rcinv};
 }
 
-// LDS staging of one atom's neighbours (FP32 geometry).
+constexpr int kTrip = 512;  // compacted triplets buffered per wave
+
+// LDS staging of one atom's neighbours (FP32 geometry) and the compacted
+// triplet buffer.
 struct Stage {
-    float* x; float* y; float* z;  // dr
-    float* d;                      // |dr|
-    float* rinv;                   // 1/|dr|
-    float* fc; float* dfc;         // cutoff of the current group
-    int* el;                       // neighbour element
-    int* la; int* lb;              // group lists
+    float4* g;       // x, y, z, |dr|
+    float4* c;       // 1/|dr|, fc, dfc (cutoff of the current group), unused
+    int* el;         // neighbour element
+    int* la; int* lb;  // group lists
+    unsigned* tjk;   // compacted pair (j | k << 16)
+    float* tr2;      // its r_jk^2
 };
 
 __device__ inline Stage carve(float* base, int capL)
 {
     Stage S;
-    S.x = base; S.y = S.x + capL; S.z = S.y + capL; S.d = S.z + capL; S.rinv = S.d + capL;
-    S.fc = S.rinv + capL; S.dfc = S.fc + capL;
-    S.el = reinterpret_cast<int*>(S.dfc + capL);
-    S.la = S.el + capL; S.lb = S.la + capL;
+    S.g = reinterpret_cast<float4*>(base);
+    S.c = S.g + capL;
+    S.el = reinterpret_cast<int*>(S.c + capL);
+    S.la = S.el + capL;
+    S.lb = S.la + capL;
+    S.tjk = reinterpret_cast<unsigned*>(S.lb + capL);
+    S.tr2 = reinterpret_cast<float*>(S.tjk + kTrip);
     return S;
 }
-constexpr int kStageWords = 10;  // 32-bit words per neighbour slot
+__host__ __device__ constexpr size_t stage_bytes(int capL) { return (size_t)capL * (32 + 12) + (size_t)kTrip * 8; }
 
 __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
 {
@@ -146,11 +155,8 @@ __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage&
         double dr[3];
         pair_vector(ri, s.pos + 3 * (size_t)j, St.box, b0, b1, b2, dr);
         const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-        S.x[k] = (float)dr[0];
-        S.y[k] = (float)dr[1];
-        S.z[k] = (float)dr[2];
-        S.d[k] = (float)d;
-        S.rinv[k] = (float)(1.0 / d);
+        S.g[k] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
+        S.c[k] = make_float4((float)(1.0 / d), 0.f, 0.f, 0.f);
         S.el[k] = s.elem[j];
     }
     __syncthreads();
@@ -164,7 +170,7 @@ __device__ inline int make_list(const Stage& S, int n, int elem, float rc, int* 
     int cnt = 0;
     for (int b = 0; b < n; b += kWave) {
         const int k = b + lane;
-        const bool ok = k < n && S.el[k] == elem && S.d[k] < rc;
+        const bool ok = k < n && S.el[k] == elem && S.g[k].w < rc;
         const int incl = wave_incl_scan(ok ? 1 : 0);
         if (ok) out[cnt + incl - 1] = k;
         cnt += __shfl(incl, kWave - 1, kWave);
@@ -178,43 +184,11 @@ __device__ inline void group_cutoffs(const Stage& S, int n, int e1, int e2, int 
 {
     for (int k = threadIdx.x; k < n; k += kWave) {
         float f = 0.f, df = 0.f;
-        if (S.el[k] == e1 || S.el[k] == e2) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, S.d[k], f, df);
-        S.fc[k] = f;
-        S.dfc[k] = df;
+        if (S.el[k] == e1 || S.el[k] == e2) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, S.g[k].w, f, df);
+        S.c[k].y = f;
+        S.c[k].z = df;
     }
     __syncthreads();
-}
-
-// Decode pair t of the group's enumeration into neighbour slots j < k.
-__device__ inline void pair_of(int t, bool same, int nA, int full, const int* la, const int* lb, int& j, int& k)
-{
-    int a, b;
-    if (same) {
-        if (t < full) {
-            a = t % nA;
-            b = (a + t / nA + 1) % nA;
-        } else {
-            a = t - full;
-            b = a + nA / 2;
-        }
-    } else {
-        a = t % nA;
-        b = t / nA;
-    }
-    j = la[a];
-    k = lb[b];
-    if (j > k) { const int x = j; j = k; k = x; }
-}
-
-__device__ inline int pair_count(bool same, int nA, int nB, int& full)
-{
-    if (same) {
-        const int D = (nA - 1) / 2;
-        full = nA * D;
-        return full + ((nA % 2 == 0) ? nA / 2 : 0);
-    }
-    full = 0;
-    return nA * nB;
 }
 
 __device__ inline float powi(float x, int n) { return pow_int<float>(x, n); }
@@ -226,6 +200,68 @@ __device__ inline void kahan(float& s, float& c, float x)
     const float t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+// Calls work(j, k, r2jk) for every triplet (j < k in neighbour order) of the
+// group's element pair with r_jk^2 < rc^2.  Pairs are enumerated without
+// divergence (like elements: the folded order (a, a+d mod n), a balanced
+// rectangle; unlike elements: the full product), tested, and the passing
+// ones compacted through LDS so the member loops run with full waves.
+template <typename Work>
+__device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work)
+{
+    const int lane = threadIdx.x;
+    int full, total;
+    if (same) {
+        full = nA * ((nA - 1) / 2);
+        total = full + ((nA % 2 == 0) ? nA / 2 : 0);
+    } else {
+        full = 0;
+        total = nA * nB;
+    }
+    const int* lb = same ? S.la : S.lb;
+    const float inv = nA > 0 ? 1.f / (float)nA : 0.f;
+    int cnt = 0;
+    for (int t0 = 0; t0 < total; t0 += kWave) {
+        const int t = t0 + lane;
+        bool ok = false;
+        unsigned jk = 0;
+        float r2 = 0.f;
+        if (t < total) {
+            const int u = same && t >= full ? t - full : t;
+            int q = (int)((float)u * inv);
+            int r = u - q * nA;
+            if (r < 0) { --q; r += nA; }
+            else if (r >= nA) { ++q; r -= nA; }
+            int a, b;
+            if (!same) { a = r; b = q; }
+            else if (t < full) { a = r; b = r + q + 1; if (b >= nA) b -= nA; }
+            else { a = u; b = u + nA / 2; }
+            int j = S.la[a], k = lb[b];
+            if (j > k) { const int x = j; j = k; k = x; }
+            const float4 gj = S.g[j], gk = S.g[k];
+            const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
+            r2 = dx * dx + dy * dy + dz * dz;
+            ok = r2 < rc2;
+            jk = (unsigned)j | ((unsigned)k << 16);
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok) {
+            const int pos = cnt + __popcll(m & __lanemask_lt());
+            S.tjk[pos] = jk;
+            S.tr2[pos] = r2;
+        }
+        cnt += __popcll(m);
+        if (cnt > kTrip - kWave) {
+            __syncthreads();
+            for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+            __syncthreads();
+            cnt = 0;
+        }
+    }
+    __syncthreads();
+    for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+    __syncthreads();
 }
 
 // ------------------------------------------------------------- SF values --
@@ -258,8 +294,9 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
         for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
         if (GT.type == 2) {
             for (int k = lane; k < n; k += kWave) {
-                if (S.el[k] != GT.e1 || !(S.d[k] < g.rc)) continue;
-                const float r = S.d[k], fc = S.fc[k];
+                const float r = S.g[k].w;
+                if (S.el[k] != GT.e1 || !(r < g.rc)) continue;
+                const float fc = S.c[k].y;
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q)
                     if (q < nmem) {
@@ -271,43 +308,33 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             const bool same = GT.e1 == GT.e2;
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
-            const int* lb = same ? S.la : S.lb;
-            int full;
-            const int total = pair_count(same, nA, nB, full);
-            for (int t = lane; t < total; t += kWave) {
-                int j, k;
-                pair_of(t, same, nA, full, S.la, lb, j, k);
-                const float x1 = S.x[j], y1 = S.y[j], z1 = S.z[j];
-                const float x2 = S.x[k], y2 = S.y[k], z2 = S.z[k];
-                const float x3 = x2 - x1, y3 = y2 - y1, z3 = z2 - z1;
-                const float r2jk = x3 * x3 + y3 * y3 + z3 * z3;
-                if (!(r2jk < g.rc2)) continue;
-                const float rij = S.d[j], rik = S.d[k];
+            for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
+                const float4 gj = S.g[j], gk = S.g[k];
+                const float4 cj = S.c[j], ck = S.c[k];
                 const float rjk = sqrtf(r2jk);
                 float fcjk, dfcjk;
                 cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
-                const float cost = (x1 * x2 + y1 * y2 + z1 * z2) * S.rinv[j] * S.rinv[k];
-                const float pfc = S.fc[j] * S.fc[k] * fcjk;
-                const float r2sum = rij * rij + rik * rik + r2jk;
+                const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
+                const float pfc = cj.y * ck.y * fcjk;
+                const float r2sum = gj.w * gj.w + gk.w * gk.w + r2jk;
                 float vexp = 0.f;
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q)
                     if (q < nmem) {
                         const MemberF& m = mf[q];
                         if (m.calcExp) {
+                            float arg = r2sum;
                             if (m.rs > 0.f) {
-                                const float a = rij - m.rs, b = rik - m.rs, c = rjk - m.rs;
-                                vexp = __expf(-m.eta * (a * a + b * b + c * c));
-                            } else {
-                                vexp = __expf(-m.eta * r2sum);
+                                const float a = gj.w - m.rs, b = gk.w - m.rs, c = rjk - m.rs;
+                                arg = a * a + b * b + c * c;
                             }
+                            vexp = __expf(-m.eta * arg) * pfc;
                         }
                         const float pl = 1.f + m.lambda * cost;
-                        float fg = 0.f;
-                        if (pl > 0.f) fg = vexp * (m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f));
-                        kahan(acc[q], cmp[q], fg * pl * pfc);
+                        const float pw = m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f);
+                        kahan(acc[q], cmp[q], pl > 0.f ? vexp * pw * pl : 0.f);
                     }
-            }
+            });
         }
 #pragma unroll
         for (int q = 0; q < MAXM; ++q)
@@ -495,14 +522,18 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
 }
 
 // --------------------------------------------------- fused force backward --
+// Forces on the atom's neighbours are accumulated per neighbour entry in LDS
+// (FP64) and pushed to the force array with FP64 atomics; contributions to
+// the atom's own periodic images are dropped in the pull form
+// (Mode::calculateForces skips them, Mode.cpp:1957) and kept in the push form.
 template <int MAXM>
 __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                                 SysView s, const float* __restrict__ fw, int ldG,
-                                                double* __restrict__ F, double* __restrict__ Fpart, int capL)
+                                                double* __restrict__ F, int push, int capL)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL);
-    double* ax = reinterpret_cast<double*>(sm + kStageWords * capL);
+    double* ax = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + stage_bytes(capL));
     double* ay = ax + capL;
     double* az = ay + capL;
     const int i = blockIdx.x;
@@ -529,18 +560,19 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         if (GT.type == 2) {
             // radial: each lane owns its neighbours, no conflicts
             for (int k = lane; k < n; k += kWave) {
-                if (S.el[k] != GT.e1 || !(S.d[k] < g.rc)) continue;
-                const float r = S.d[k], fc = S.fc[k], dfc = S.dfc[k];
+                const float4 gk = S.g[k];
+                if (S.el[k] != GT.e1 || !(gk.w < g.rc)) continue;
+                const float4 ck = S.c[k];
                 float P = 0.f;
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q)
                     if (q < nmem) {
-                        const float dr = r - mf[q].rs;
+                        const float dr = gk.w - mf[q].rs;
                         const float ex = __expf(-mf[q].eta * dr * dr);
-                        P += w[q] * (dfc - mf[q].twoEta * dr * fc) * ex;
+                        P += w[q] * (ck.z - mf[q].twoEta * dr * ck.y) * ex;
                     }
-                P *= S.rinv[k];
-                const double vx = (double)(P * S.x[k]), vy = (double)(P * S.y[k]), vz = (double)(P * S.z[k]);
+                P *= ck.x;
+                const double vx = (double)(P * gk.x), vy = (double)(P * gk.y), vz = (double)(P * gk.z);
                 fx -= vx; fy -= vy; fz -= vz;
                 ax[k] += vx; ay[k] += vy; az[k] += vz;
             }
@@ -550,31 +582,20 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const bool same = GT.e1 == GT.e2;
         const int nA = make_list(S, n, GT.e1, g.rc, S.la);
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
-        const int* lb = same ? S.la : S.lb;
-        int full;
-        const int total = pair_count(same, nA, nB, full);
-        for (int t = lane; t < total; t += kWave) {
-            int j, k;
-            pair_of(t, same, nA, full, S.la, lb, j, k);
-            const float x1 = S.x[j], y1 = S.y[j], z1 = S.z[j];
-            const float x2 = S.x[k], y2 = S.y[k], z2 = S.z[k];
-            const float x3 = x2 - x1, y3 = y2 - y1, z3 = z2 - z1;
-            const float r2jk = x3 * x3 + y3 * y3 + z3 * z3;
-            if (!(r2jk < g.rc2)) continue;
-            const float rij = S.d[j], rik = S.d[k];
-            const float ij = S.rinv[j], ik = S.rinv[k];
+        for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
+            const float4 gj = S.g[j], gk = S.g[k];
+            const float4 cj = S.c[j], ck = S.c[k];
             const float rjk = sqrtf(r2jk);
-            const float jk = 1.f / rjk;
+            const float ijk = 1.f / rjk;
             float fcjk, dfcjk;
             cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
-            const float fcij = S.fc[j], fcik = S.fc[k];
-            const float rinvijik = ij * ik;
-            const float cost = (x1 * x2 + y1 * y2 + z1 * z2) * rinvijik;
-            const float pfc = fcij * fcik * fcjk;
-            const float r2sum = rij * rij + rik * rik + r2jk;
-            const float pr1 = fcik * fcjk * S.dfc[j] * ij;
-            const float pr2 = fcij * fcjk * S.dfc[k] * ik;
-            const float pr3 = fcij * fcik * dfcjk * jk;
+            const float rinvijik = cj.x * ck.x;
+            const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * rinvijik;
+            const float pfc = cj.y * ck.y * fcjk;
+            const float r2sum = gj.w * gj.w + gk.w * gk.w + r2jk;
+            const float pr1 = ck.y * fcjk * cj.z * cj.x;
+            const float pr2 = cj.y * fcjk * ck.z * ck.x;
+            const float pr3 = cj.y * ck.y * dfcjk * ijk;
             float A = 0.f, B = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, C3 = 0.f;
             float vexp = 0.f;
 #pragma unroll
@@ -582,35 +603,35 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                 if (q < nmem) {
                     const MemberF& m = mf[q];
                     if (m.calcExp) {
+                        float arg = r2sum;
                         if (m.rs > 0.f) {
-                            const float a = rij - m.rs, b = rik - m.rs, c = rjk - m.rs;
-                            vexp = __expf(-m.eta * (a * a + b * b + c * c));
-                        } else {
-                            vexp = __expf(-m.eta * r2sum);
+                            const float a = gj.w - m.rs, b = gk.w - m.rs, c = rjk - m.rs;
+                            arg = a * a + b * b + c * c;
                         }
+                        vexp = __expf(-m.eta * arg);
                     }
                     const float pl = 1.f + m.lambda * cost;
-                    if (pl > 0.f) {
-                        const float u = w[q] * vexp * (m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f));
-                        A += u * m.zetaLambda;
-                        const float b = u * pl;
-                        B += b;
-                        const float c = b * m.twoEta;
-                        if (m.rs > 0.f) {
-                            C1 += c * (rij - m.rs) * ij;
-                            C2 += c * (rik - m.rs) * ik;
-                            C3 += c * (rjk - m.rs) * jk;
-                        } else {
-                            C0 += c;
-                        }
+                    const float pw = m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f);
+                    const float u = pl > 0.f ? w[q] * vexp * pw : 0.f;
+                    A += u * m.zetaLambda;
+                    const float b = u * pl;
+                    B += b;
+                    const float c = b * m.twoEta;
+                    if (m.rs > 0.f) {
+                        C1 += c * (gj.w - m.rs) * cj.x;
+                        C2 += c * (gk.w - m.rs) * ck.x;
+                        C3 += c * (rjk - m.rs) * ijk;
+                    } else {
+                        C0 += c;
                     }
                 }
             C1 += C0; C2 += C0; C3 += C0;
-            const float P1 = pfc * (A * (rinvijik - cost * ij * ij) - C1) + B * pr1;
-            const float P2 = pfc * (A * (rinvijik - cost * ik * ik) - C2) + B * pr2;
+            const float P1 = pfc * (A * (rinvijik - cost * cj.x * cj.x) - C1) + B * pr1;
+            const float P2 = pfc * (A * (rinvijik - cost * ck.x * ck.x) - C2) + B * pr2;
             const float P3 = pfc * (A * rinvijik + C3) - B * pr3;
-            const float p1x = P1 * x1, p1y = P1 * y1, p1z = P1 * z1;
-            const float p2x = P2 * x2, p2y = P2 * y2, p2z = P2 * z2;
+            const float x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
+            const float p1x = P1 * gj.x, p1y = P1 * gj.y, p1z = P1 * gj.z;
+            const float p2x = P2 * gk.x, p2y = P2 * gk.y, p2z = P2 * gk.z;
             const float p3x = P3 * x3, p3y = P3 * y3, p3z = P3 * z3;
             fx -= (double)(p1x + p2x);
             fy -= (double)(p1y + p2y);
@@ -621,52 +642,30 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             lds_add(ax + k, (double)(p2x - p3x));
             lds_add(ay + k, (double)(p2y - p3y));
             lds_add(az + k, (double)(p2z - p3z));
-        }
-        __syncthreads();
+        });
     }
     fx = wave_sum(fx);
     fy = wave_sum(fy);
     fz = wave_sum(fz);
     if (lane == 0) {
-        F[3 * (size_t)i] = fx;
-        F[3 * (size_t)i + 1] = fy;
-        F[3 * (size_t)i + 2] = fz;
+        atomicAdd(F + 3 * (size_t)i, fx);
+        atomicAdd(F + 3 * (size_t)i + 1, fy);
+        atomicAdd(F + 3 * (size_t)i + 2, fz);
     }
-    double* fp = Fpart + (size_t)i * s.cap * 3;
-    for (int k = lane; k < n; k += kWave) {
-        fp[3 * k] = ax[k];
-        fp[3 * k + 1] = ay[k];
-        fp[3 * k + 2] = az[k];
-    }
-}
-
-// F_i += sum over i's entries (j, image) of the contribution stored on the
-// reverse entry (i, -image) of j's list; own images skipped in the pull form.
-__global__ __launch_bounds__(64) void k_fsum(SysView s, const double* __restrict__ Fpart, int push,
-                                             double* __restrict__ F)
-{
-    const int i = blockIdx.x;
-    const int lane = threadIdx.x;
-    const int n = s.nb_n[i];
     const size_t base = (size_t)i * s.cap;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
     for (int k = lane; k < n; k += kWave) {
         const int j = s.nb_j[base + k];
         if (j == i && !push) continue;
-        const int r = s.nb_rev[base + k];
-        if (r < 0) continue;
-        fx += Fpart[3 * (size_t)r];
-        fy += Fpart[3 * (size_t)r + 1];
-        fz += Fpart[3 * (size_t)r + 2];
+        atomicAdd(F + 3 * (size_t)j, ax[k]);
+        atomicAdd(F + 3 * (size_t)j + 1, ay[k]);
+        atomicAdd(F + 3 * (size_t)j + 2, az[k]);
     }
-    fx = wave_sum(fx);
-    fy = wave_sum(fy);
-    fz = wave_sum(fz);
-    if (lane == 0) {
-        F[3 * (size_t)i] += fx;
-        F[3 * (size_t)i + 1] += fy;
-        F[3 * (size_t)i + 2] += fz;
-    }
+}
+
+__global__ void k_zero(double* p, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        p[q] = 0.0;
 }
 
 __global__ void k_mfma_probe(const float* A, const float* B, float* D)
@@ -793,7 +792,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     if (s.natoms == 0) return;
     const int capL = s.cap;
     const int P = pow2_members(t);
-    const size_t lds_sf = sizeof(float) * (size_t)kStageWords * capL;
+    const size_t lds_sf = stage_bytes(capL);
     const size_t lds_f = lds_sf + sizeof(double) * 3 * (size_t)capL;
     if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     timed("sf32", [&] {
@@ -837,15 +836,16 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     timed("energy", [&] { launch_energy(b.Eatom, b.atom_off, b.nstruct, b.E, stream); });
     if (!forces) return;
     timed("force32", [&] {
+        const size_t n3 = (size_t)3 * s.natoms;
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0, stream,
+                           b.F, n3);
+        const int ps = push ? 1 : 0;
         switch (P) {
-        case 4: hipLaunchKernelGGL(k_force32<4>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
-        case 8: hipLaunchKernelGGL(k_force32<8>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
-        case 16: hipLaunchKernelGGL(k_force32<16>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
-        default: hipLaunchKernelGGL(k_force32<32>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, b.Fpart, capL); break;
+        case 4: hipLaunchKernelGGL(k_force32<4>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
+        case 8: hipLaunchKernelGGL(k_force32<8>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
+        case 16: hipLaunchKernelGGL(k_force32<16>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
+        default: hipLaunchKernelGGL(k_force32<32>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
         }
-    });
-    timed("fsum", [&] {
-        hipLaunchKernelGGL(k_fsum, dim3(s.natoms), dim3(kWave), 0, stream, s, b.Fpart, push ? 1 : 0, b.F);
     });
 }
 

@@ -56,7 +56,6 @@ struct MixedBuffers {
     double* Eatom;     // [N]
     double* E;         // [nstruct]
     double* F;         // [3N]
-    double* Fpart;     // [3*N*cap] per-entry force contributions
     const int* elem_atoms;
     const int* atom_off;
     int elem_off[kMaxElem + 1];

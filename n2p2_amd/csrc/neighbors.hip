@@ -71,8 +71,12 @@ __global__ void k_cell_fill(SysView s)
     if (i >= s.natoms) return;
     const int c = s.atom_cell[i];
     if (c < 0) return;
-    const int p = atomicAdd(&s.cell_count[c], 1);
-    s.cell_atoms[s.cell_start[c] + p] = i;
+    const int p = s.cell_start[c] + atomicAdd(&s.cell_count[c], 1);
+    s.cell_atoms[p] = i;
+    // cell-ordered copy of the positions: candidate loads become contiguous
+    s.cell_pos[3 * (size_t)p] = s.pos[3 * (size_t)i];
+    s.cell_pos[3 * (size_t)p + 1] = s.pos[3 * (size_t)i + 1];
+    s.cell_pos[3 * (size_t)p + 2] = s.pos[3 * (size_t)i + 2];
 }
 
 __device__ inline void emit(const SysView& s, int i, int slot, int j, int img)
@@ -128,41 +132,66 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
             count += __shfl(incl, kWave - 1, kWave);
         }
     } else {
+        // The 27 surrounding cells are flattened into one candidate range so
+        // the wave tests 64 candidates per step with contiguous loads.
+        __shared__ int cs[27], co[28], cim[27];
         const int c = s.atom_cell[i] - S.cell0;
         const int c0 = c / (S.nc[1] * S.nc[2]), c1 = (c / S.nc[2]) % S.nc[1], c2 = c % S.nc[2];
-        for (int d0 = -1; d0 <= 1; ++d0)
-            for (int d1 = -1; d1 <= 1; ++d1)
-                for (int d2 = -1; d2 <= 1; ++d2) {
-                    int q[3] = {c0 + d0, c1 + d1, c2 + d2};
-                    int b[3];
-                    for (int k = 0; k < 3; ++k) {
-                        // wrapped cell: neighbour image r_j + m*box, i.e. bc = -m
-                        const int m = q[k] < 0 ? -1 : (q[k] >= S.nc[k] ? 1 : 0);
-                        q[k] -= m * S.nc[k];
-                        b[k] = -m;
-                    }
-                    const int cell = S.cell0 + (q[0] * S.nc[1] + q[1]) * S.nc[2] + q[2];
-                    const int beg = s.cell_start[cell], end = s.cell_start[cell + 1];
-                    for (int p = beg; p < end; p += kWave) {
-                        bool hit = false;
-                        int j = 0;
-                        if (p + lane < end) {
-                            j = s.cell_atoms[p + lane];
-                            if (!(i == j && b[0] == 0 && b[1] == 0 && b[2] == 0)) {
-                                double dr[3];
-                                pair_vector(ri, s.pos + 3 * (size_t)j, S.box, b[0], b[1], b[2], dr);
-                                hit = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2;
-                            }
-                        }
-                        const int incl = wave_incl_scan(hit ? 1 : 0);
-                        if (hit) {
-                            const int slot = count + incl - 1;
-                            if (slot < kNbMaxLds)
-                                keys[slot] = ((long long)j << 21) | pack_image(b[0], b[1], b[2]);
-                        }
-                        count += __shfl(incl, kWave - 1, kWave);
-                    }
+        int start = 0, len = 0, img = 0;
+        if (lane < 27) {
+            int q[3] = {c0 + lane / 9 - 1, c1 + (lane / 3) % 3 - 1, c2 + lane % 3 - 1};
+            int b[3];
+            for (int k = 0; k < 3; ++k) {
+                // wrapped cell: neighbour image r_j + m*box, i.e. bc = -m
+                const int m = q[k] < 0 ? -1 : (q[k] >= S.nc[k] ? 1 : 0);
+                q[k] -= m * S.nc[k];
+                b[k] = -m;
+            }
+            const int cell = S.cell0 + (q[0] * S.nc[1] + q[1]) * S.nc[2] + q[2];
+            start = s.cell_start[cell];
+            len = s.cell_start[cell + 1] - start;
+            img = pack_image(b[0], b[1], b[2]);
+        }
+        const int incl = wave_incl_scan(len);
+        if (lane < 27) {
+            cs[lane] = start;
+            co[lane] = incl - len;
+            cim[lane] = img;
+        }
+        if (lane == 26) co[27] = incl;
+        __syncthreads();
+        const int total = co[27];
+        const int self_img = pack_image(0, 0, 0);
+        for (int b0 = 0; b0 < total; b0 += kWave) {
+            const int x = b0 + lane;
+            bool hit = false;
+            long long key = 0;
+            if (x < total) {
+                int lo = 0, hi = 26;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (co[mid] <= x) lo = mid;
+                    else hi = mid - 1;
                 }
+                const int p = cs[lo] + x - co[lo];
+                const int j = s.cell_atoms[p];
+                const int im = cim[lo];
+                if (!(i == j && im == self_img)) {
+                    int q0, q1, q2;
+                    unpack_image(im, q0, q1, q2);
+                    double dr[3];
+                    pair_vector(ri, s.cell_pos + 3 * (size_t)p, S.box, q0, q1, q2, dr);
+                    hit = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2;
+                    key = ((long long)j << 21) | im;
+                }
+            }
+            const int inc = wave_incl_scan(hit ? 1 : 0);
+            if (hit) {
+                const int slot = count + inc - 1;
+                if (slot < kNbMaxLds) keys[slot] = key;
+            }
+            count += __shfl(inc, kWave - 1, kWave);
+        }
         // Bitonic sort of the collected keys (padded to a power of two).
         const int n = min(count, kNbMaxLds);
         int np = 1;
@@ -244,6 +273,11 @@ void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t st
         hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(256), 0, stream, s);
     }
     hipLaunchKernelGGL(k_build, dim3(s.natoms), dim3(kWave), 0, stream, s, rc * rc);
+}
+
+void launch_reverse(const SysView& s, hipStream_t stream)
+{
+    if (s.natoms == 0) return;
     hipLaunchKernelGGL(k_reverse, dim3(s.natoms), dim3(kWave), 0, stream, s);
 }
 

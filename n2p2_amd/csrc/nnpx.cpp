@@ -61,6 +61,7 @@ struct nnpx_ctx {
     int cap = 0;
     int max_nb = 0;
     bool nbr_valid = false;
+    bool rev_valid = false;
     // results state
     bool sf_valid = false, sfd_valid = false, nn_valid = false, dedg_valid = false;
     bool e_valid = false, f_valid = false;
@@ -190,6 +191,7 @@ SysView sysview(nnpx_ctx* c)
     s.cell_count = existing<int>(c, "cell_count");
     s.cell_start = existing<int>(c, "cell_start");
     s.cell_atoms = existing<int>(c, "cell_atoms");
+    s.cell_pos = existing<double>(c, "cell_pos");
     return s;
 }
 
@@ -325,6 +327,7 @@ void alloc_neighbors(nnpx_ctx* c)
     buf<int>(c, "cell_count", std::max(c->ncells, 1));
     buf<int>(c, "cell_start", c->ncells + 1);
     buf<int>(c, "cell_atoms", c->N);
+    buf<double>(c, "cell_pos", (size_t)3 * c->N);
 }
 
 void read_flags(nnpx_ctx* c, int* f)
@@ -341,11 +344,11 @@ void build_neighbors(nnpx_ctx* c, bool check)
         alloc_neighbors(c);
         SysView s = sysview(c);
         timed(c, "nbr", [&] { launch_neighbors(s, c->hm.t.maxCutoff, c->any_cell, c->stream); });
+        c->rev_valid = false;
         if (!check) break;
         int f[2];
         read_flags(c, f);
         c->max_nb = f[1];
-        if (f[0] & 2) fail(NNPX_ERR_HIP, "neighbour list is not symmetric");
         if (!(f[0] & 1)) break;
         if (f[1] > 1024) fail(NNPX_ERR_UNSUPPORTED, "more than 1024 neighbours per atom");
         c->cap = ((f[1] + f[1] / 4 + 31) / 32) * 32;
@@ -398,6 +401,10 @@ void compute_force64(nnpx_ctx* c)
     const ModelT& t = c->hm.t;
     double* f = buf<double>(c, "F", (size_t)3 * c->N);
     SysView s = sysview(c);
+    if (!c->rev_valid) {
+        timed(c, "rev", [&] { launch_reverse(s, c->stream); });
+        c->rev_valid = true;
+    }
     timed(c, "force64", [&] {
         launch_force64(c->d_model, s, existing<double>(c, "dEdG"), t.max_sf, existing<double>(c, "dGself"),
                        existing<double>(c, "dGnbr"), std::max(t.max_sfd, 1),
@@ -463,7 +470,6 @@ MixedBuffers mixed_buffers(nnpx_ctx* c)
     m.Eatom = buf<double>(c, "Eatom", c->N);
     m.E = buf<double>(c, "E", c->S);
     m.F = buf<double>(c, "F", (size_t)3 * c->N);
-    m.Fpart = buf<double>(c, "Fpart", (size_t)3 * c->N * c->cap);
     m.elem_atoms = existing<int>(c, "elem_atoms");
     m.atom_off = existing<int>(c, "atom_off");
     for (int e = 0; e <= t.nelem; ++e) m.elem_off[e] = c->elem_off[e];

@@ -32,7 +32,7 @@ struct Lds {
     int* el;       // [cap]
     int* la;       // [cap] list of neighbours with element e1 inside rc
     int* lb;       // [cap] list with element e2
-    double* acc;   // [cap][nmem][3]
+    double* acc;   // [nmem][3][cap] (neighbour index fastest: conflict-free atomics)
     double* red;   // [kSFWaves][4*kMaxMembers]
 };
 
@@ -175,7 +175,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
             L.dfc[k] = df;
         }
         if (deriv)
-            for (int x = tid; x < n * nmem * 3; x += kSFThreads) L.acc[x] = 0.0;
+            for (int x = tid; x < nmem * 3 * capL; x += kSFThreads) L.acc[x] = 0.0;
         __syncthreads();
 
         double gacc[MAXM];
@@ -262,12 +262,11 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                     p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl) + pr2 * plambda);
                     p3 = fg * (pfczl * (rinvijik + p2etapl) - pr3 * plambda);
                 }
-                double* aj = L.acc + ((size_t)j * nmem + q) * 3;
-                double* ak = L.acc + ((size_t)k * nmem + q) * 3;
                 for (int c = 0; c < 3; ++c) {
                     const double p1c = p1 * dr1[c], p2c = p2 * dr2[c], p3c = p3 * dr3[c];
-                    lds_add(aj + c, -(p1c + p3c));
-                    lds_add(ak + c, -(p2c - p3c));
+                    double* a = L.acc + (size_t)(q * 3 + c) * capL;
+                    lds_add(a + j, -(p1c + p3c));
+                    lds_add(a + k, -(p2c - p3c));
                 }
                 }
                 }
@@ -294,14 +293,14 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                 const MemberT& mb = M->mem[G.mem0 + q];
                 const int slot = ek == G.e1 ? mb.slot1 : mb.slot2;
                 double* d = dGn + ((size_t)k * o.S + slot) * 3;
-                const double* a = L.acc + (size_t)x * 3;
-                d[0] = a[0]; d[1] = a[1]; d[2] = a[2];
+                const double* a = L.acc + (size_t)q * 3 * capL + k;
+                d[0] = a[0]; d[1] = a[capL]; d[2] = a[2 * capL];
             }
             // The atom's own dG/dr: minus the sum over its neighbour entries.
             if (tid < nmem * 3) {
                 const int q = tid / 3, c = tid % 3;
                 double v = 0.0;
-                for (int k = 0; k < n; ++k) v += L.acc[((size_t)k * nmem + q) * 3 + c];
+                for (int k = 0; k < n; ++k) v += L.acc[(size_t)(q * 3 + c) * capL + k];
                 dGi[M->mem[G.mem0 + q].sf * 3 + c] = -v;
             }
         }
