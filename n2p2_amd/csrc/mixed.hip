@@ -14,6 +14,7 @@
 //            in LDS per neighbour entry, pushed with FP64 atomics.
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -125,6 +126,7 @@ struct Stage {
     int* la; int* lb;  // group lists
     unsigned* tjk;   // compacted pair (j | k << 16)
     float* tr2;      // its r_jk^2
+    float4* mp;      // [2*kMaxMembers] member constants of the current group
 };
 
 __device__ inline Stage carve(float* base, int capL)
@@ -137,9 +139,13 @@ __device__ inline Stage carve(float* base, int capL)
     S.lb = S.la + capL;
     S.tjk = reinterpret_cast<unsigned*>(S.lb + capL);
     S.tr2 = reinterpret_cast<float*>(S.tjk + kTrip);
+    S.mp = reinterpret_cast<float4*>(S.tr2 + kTrip);
     return S;
 }
-__host__ __device__ constexpr size_t stage_bytes(int capL) { return (size_t)capL * (32 + 12) + (size_t)kTrip * 8; }
+__host__ __device__ constexpr size_t stage_bytes(int capL)
+{
+    return (size_t)capL * (32 + 12) + (size_t)kTrip * 8 + (size_t)2 * kMaxMembers * 16;
+}
 
 __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
 {
@@ -193,14 +199,62 @@ __device__ inline void group_cutoffs(const Stage& S, int n, int e1, int e2, int 
 
 __device__ inline float powi(float x, int n) { return pow_int<float>(x, n); }
 
-// Kahan summation step.
+// Compensated summation step (Fast2Sum error term accumulated in c; the
+// terms are non-negative, so the running sum dominates after the first).
 __device__ inline void kahan(float& s, float& c, float x)
 {
-    const float y = x - c;
-    const float t = s + y;
-    c = (t - s) - y;
+    const float t = s + x;
+    c += (s - t) + x;
     s = t;
 }
+
+// Value of a per-triplet table at a wave-uniform index.
+template <int MAXS>
+__device__ inline float pick(const float* v, int s)
+{
+    float r = v[0];
+    if (MAXS > 1) r = s == 1 ? v[1] : r;
+    if (MAXS > 2) r = s == 2 ? v[2] : r;
+    if (MAXS > 3) r = s == 3 ? v[3] : r;
+    return r;
+}
+
+template <int MAXS>
+__device__ inline float pickb(const float* v, unsigned b1, unsigned b2)
+{
+    if (MAXS <= 2) return b1 ? v[1] : v[0];
+    const float lo = b1 ? v[1] : v[0];
+    const float hi = b1 ? v[3] : v[2];
+    return b2 ? hi : lo;
+}
+
+// Member constants of one group: flags as wave-uniform bit masks, values
+// staged in LDS (mp[q] = zeta, eta log2 e, rs; mp[32+q] = the force-pass
+// weights w zeta lambda, w, 2 eta w), read as broadcasts in the triplet loops.
+struct MemberMasks {
+    unsigned calc, s1, s2;
+};
+
+__device__ inline MemberMasks load_members(const Stage& S, const MemberF* __restrict__ mf, int nmem,
+                                           const float* w)
+{
+    const int lane = threadIdx.x;
+    if (lane < nmem) {
+        const MemberF m = mf[lane];
+        const float wq = w ? w[m.sf] : 0.f;
+        S.mp[lane] = make_float4(m.zeta, m.etaL, m.rs, (float)(m.sidx & 1));
+        S.mp[kMaxMembers + lane] = make_float4(wq * m.zetaLambda, wq, wq * m.twoEta, (float)((m.sidx >> 1) & 1));
+    }
+    MemberMasks R;
+    R.calc = (unsigned)__ballot(lane < nmem && mf[min(lane, nmem - 1)].calcExp);
+    R.s1 = (unsigned)__ballot(lane < nmem && (mf[min(lane, nmem - 1)].sidx & 1));
+    R.s2 = (unsigned)__ballot(lane < nmem && (mf[min(lane, nmem - 1)].sidx & 2));
+    __syncthreads();
+    return R;
+}
+
+__device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
 
 // Calls work(j, k, r2jk) for every triplet (j < k in neighbour order) of the
 // group's element pair with r_jk^2 < rc^2.  Pairs are enumerated without
@@ -221,54 +275,61 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
     }
     const int* lb = same ? S.la : S.lb;
     const float inv = nA > 0 ? 1.f / (float)nA : 0.f;
-    int cnt = 0;
-    for (int t0 = 0; t0 < total; t0 += kWave) {
-        const int t = t0 + lane;
-        bool ok = false;
-        unsigned jk = 0;
-        float r2 = 0.f;
-        if (t < total) {
-            const int u = same && t >= full ? t - full : t;
-            int q = (int)((float)u * inv);
-            int r = u - q * nA;
-            if (r < 0) { --q; r += nA; }
-            else if (r >= nA) { ++q; r -= nA; }
-            int a, b;
-            if (!same) { a = r; b = q; }
-            else if (t < full) { a = r; b = r + q + 1; if (b >= nA) b -= nA; }
-            else { a = u; b = u + nA / 2; }
-            int j = S.la[a], k = lb[b];
-            if (j > k) { const int x = j; j = k; k = x; }
-            const float4 gj = S.g[j], gk = S.g[k];
-            const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
-            r2 = dx * dx + dy * dy + dz * dz;
-            ok = r2 < rc2;
-            jk = (unsigned)j | ((unsigned)k << 16);
+    int t0 = 0;
+    while (t0 < total) {
+        // fill the buffer with passing pairs ...
+        int cnt = 0;
+        while (t0 < total && cnt <= kTrip - kWave) {
+            const int t = t0 + lane;
+            bool ok = false;
+            unsigned jk = 0;
+            float r2 = 0.f;
+            if (t < total) {
+                const int u = same && t >= full ? t - full : t;
+                int q = (int)((float)u * inv);
+                int r = u - q * nA;
+                if (r < 0) { --q; r += nA; }
+                else if (r >= nA) { ++q; r -= nA; }
+                int a, b;
+                if (!same) { a = r; b = q; }
+                else if (t < full) { a = r; b = r + q + 1; if (b >= nA) b -= nA; }
+                else { a = u; b = u + nA / 2; }
+                int j = S.la[a], k = lb[b];
+                if (j > k) { const int x = j; j = k; k = x; }
+                const float4 gj = S.g[j], gk = S.g[k];
+                const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
+                r2 = dx * dx + dy * dy + dz * dz;
+                ok = r2 < rc2;
+                jk = (unsigned)j | ((unsigned)k << 16);
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = cnt + __popcll(m & __lanemask_lt());
+                S.tjk[pos] = jk;
+                S.tr2[pos] = r2;
+            }
+            cnt += __popcll(m);
+            t0 += kWave;
         }
-        const unsigned long long m = __ballot(ok);
-        if (ok) {
-            const int pos = cnt + __popcll(m & __lanemask_lt());
-            S.tjk[pos] = jk;
-            S.tr2[pos] = r2;
-        }
-        cnt += __popcll(m);
-        if (cnt > kTrip - kWave) {
-            __syncthreads();
-            for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
-            __syncthreads();
-            cnt = 0;
-        }
+        __syncthreads();
+        // ... then run the member loops with full waves
+        for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        __syncthreads();
     }
-    __syncthreads();
-    for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
-    __syncthreads();
 }
 
 // ------------------------------------------------------------- SF values --
-template <int MAXM>
+// Angular terms: vexp * pfc * pl^zeta evaluated as one exp2 of
+// zeta * log2(pl) + log2(pfc) - eta log2(e) * sum(r - rs)^2, with log2(pl)
+// shared by all members with the same lambda (pl <= 0 gives log2 = -inf, the
+// reference's zero term).
+// FAST: every angular group has exactly MAXM members, at most two distinct
+// lambda and rs = 0 -- the member loop is then branch-free.  Otherwise the
+// general loop handles shifted Gaussians and up to four lambda values.
+template <int MAXM, bool FAST>
 __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
-                                             SysView s, double* __restrict__ G, float* __restrict__ Gin, int ldG,
-                                             int ldK, int capL)
+                                             const GroupMixF* __restrict__ GM, SysView s, double* __restrict__ G,
+                                             float* __restrict__ Gin, int ldG, int ldK, int capL)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL);
@@ -308,6 +369,8 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             const bool same = GT.e1 == GT.e2;
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+            const GroupMixF gm = GM[E.grp0 + gi];
+            const MemberMasks R = load_members(S, mf, nmem, nullptr);
             for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
                 const float4 gj = S.g[j], gk = S.g[k];
                 const float4 cj = S.c[j], ck = S.c[k];
@@ -315,31 +378,50 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 float fcjk, dfcjk;
                 cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
                 const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
-                const float pfc = cj.y * ck.y * fcjk;
+                const float lpfc = flog2(cj.y * ck.y * fcjk);
                 const float r2sum = gj.w * gj.w + gk.w * gk.w + r2jk;
-                float vexp = 0.f;
+                constexpr int MAXS = FAST ? 2 : 4;
+                float L[MAXS];
 #pragma unroll
-                for (int q = 0; q < MAXM; ++q)
-                    if (q < nmem) {
-                        const MemberF& m = mf[q];
-                        if (m.calcExp) {
-                            float arg = r2sum;
-                            if (m.rs > 0.f) {
-                                const float a = gj.w - m.rs, b = gk.w - m.rs, c = rjk - m.rs;
-                                arg = a * a + b * b + c * c;
-                            }
-                            vexp = __expf(-m.eta * arg) * pfc;
-                        }
-                        const float pl = 1.f + m.lambda * cost;
-                        const float pw = m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f);
-                        kahan(acc[q], cmp[q], pl > 0.f ? vexp * pw * pl : 0.f);
+                for (int t = 0; t < MAXS; ++t) {
+                    const float pl = 1.f + gm.lam[t] * cost;
+                    L[t] = pl > 0.f ? flog2(pl) : -INFINITY;
+                }
+                if constexpr (FAST) {
+#pragma unroll
+                    for (int q = 0; q < MAXM; ++q) {
+                        const float4 mq = S.mp[q];
+                        const float Ls = mq.w != 0.f ? L[1] : L[0];
+                        const float ec = fmaf(-mq.y, r2sum, lpfc);
+                        kahan(acc[q], cmp[q], fexp2(fmaf(mq.x, Ls, ec)));
                     }
+                } else {
+                    float ec = 0.f;
+#pragma unroll
+                    for (int q = 0; q < MAXM; ++q)
+                        if (q < nmem) {
+                            const float4 mq = S.mp[q];
+                            const float4 mw = S.mp[kMaxMembers + q];
+                            if ((R.calc >> q) & 1u) {
+                                float arg = r2sum;
+                                if (mq.z > 0.f) {
+                                    const float a = gj.w - mq.z, b = gk.w - mq.z, c = rjk - mq.z;
+                                    arg = a * a + b * b + c * c;
+                                }
+                                ec = fmaf(-mq.y, arg, lpfc);
+                            }
+                            const float lo = mq.w != 0.f ? L[1] : L[0];
+                            const float hi = mq.w != 0.f ? L[3] : L[2];
+                            const float Ls = mw.w != 0.f ? hi : lo;
+                            kahan(acc[q], cmp[q], fexp2(fmaf(mq.x, Ls, ec)));
+                        }
+                }
             });
         }
 #pragma unroll
         for (int q = 0; q < MAXM; ++q)
             if (q < nmem) {
-                double v = wave_sum((double)acc[q] - (double)cmp[q]);
+                double v = wave_sum((double)acc[q] + (double)cmp[q]);
                 if (lane == 0) {
                     const MemberT& mt = M->mem[GT.mem0 + q];
                     if (GT.type == 3) v *= mt.gnorm;
@@ -526,10 +608,11 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
 // (FP64) and pushed to the force array with FP64 atomics; contributions to
 // the atom's own periodic images are dropped in the pull form
 // (Mode::calculateForces skips them, Mode.cpp:1957) and kept in the push form.
-template <int MAXM>
+template <int MAXM, bool FAST>
 __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
-                                                SysView s, const float* __restrict__ fw, int ldG,
-                                                double* __restrict__ F, int push, int capL)
+                                                const GroupMixF* __restrict__ GM, SysView s,
+                                                const float* __restrict__ fw, int ldG, double* __restrict__ F,
+                                                int push, int capL)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL);
@@ -553,9 +636,6 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const GroupF g = group_f(GT);
         const int nmem = GT.nmem;
         const MemberF* mf = MF + GT.mem0;
-        float w[MAXM];
-#pragma unroll
-        for (int q = 0; q < MAXM; ++q) w[q] = q < nmem ? wi[mf[q].sf] : 0.f;
         group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         if (GT.type == 2) {
             // radial: each lane owns its neighbours, no conflicts
@@ -569,7 +649,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                     if (q < nmem) {
                         const float dr = gk.w - mf[q].rs;
                         const float ex = __expf(-mf[q].eta * dr * dr);
-                        P += w[q] * (ck.z - mf[q].twoEta * dr * ck.y) * ex;
+                        P += wi[mf[q].sf] * (ck.z - mf[q].twoEta * dr * ck.y) * ex;
                     }
                 P *= ck.x;
                 const double vx = (double)(P * gk.x), vy = (double)(P * gk.y), vz = (double)(P * gk.z);
@@ -582,6 +662,8 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const bool same = GT.e1 == GT.e2;
         const int nA = make_list(S, n, GT.e1, g.rc, S.la);
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+        const GroupMixF gm = GM[E.grp0 + gi];
+        const MemberMasks R = load_members(S, mf, nmem, wi);
         for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
@@ -596,35 +678,66 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             const float pr1 = ck.y * fcjk * cj.z * cj.x;
             const float pr2 = cj.y * fcjk * ck.z * ck.x;
             const float pr3 = cj.y * ck.y * dfcjk * ijk;
-            float A = 0.f, B = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, C3 = 0.f;
-            float vexp = 0.f;
+            // per distinct lambda: log2(pl) (-inf where pl <= 0) and 1/pl
+            constexpr int MAXS = FAST ? 2 : 4;
+            float L[MAXS], Rp[MAXS];
 #pragma unroll
-            for (int q = 0; q < MAXM; ++q)
-                if (q < nmem) {
-                    const MemberF& m = mf[q];
-                    if (m.calcExp) {
-                        float arg = r2sum;
-                        if (m.rs > 0.f) {
-                            const float a = gj.w - m.rs, b = gk.w - m.rs, c = rjk - m.rs;
-                            arg = a * a + b * b + c * c;
-                        }
-                        vexp = __expf(-m.eta * arg);
-                    }
-                    const float pl = 1.f + m.lambda * cost;
-                    const float pw = m.useIntPow ? powi(pl, m.zetaInt - 1) : powf(pl, m.zeta - 1.f);
-                    const float u = pl > 0.f ? w[q] * vexp * pw : 0.f;
-                    A += u * m.zetaLambda;
-                    const float b = u * pl;
-                    B += b;
-                    const float c = b * m.twoEta;
-                    if (m.rs > 0.f) {
-                        C1 += c * (gj.w - m.rs) * cj.x;
-                        C2 += c * (gk.w - m.rs) * ck.x;
-                        C3 += c * (rjk - m.rs) * ijk;
-                    } else {
-                        C0 += c;
-                    }
+            for (int t = 0; t < MAXS; ++t) {
+                const float pl = 1.f + gm.lam[t] * cost;
+                const bool pos = pl > 0.f;
+                L[t] = pos ? flog2(pl) : -INFINITY;
+                Rp[t] = pos ? __builtin_amdgcn_rcpf(pl) : 0.f;
+            }
+            // t_q = vexp * pl^zeta: u_q * pl = w t_q and u_q = w t_q / pl,
+            // u_q = w vexp pl^(zeta-1) being the reference's fg times w
+            float A = 0.f, B = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, C3 = 0.f;
+            if constexpr (FAST) {
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q) {
+                    const float4 mq = S.mp[q];
+                    const float4 wq = S.mp[kMaxMembers + q];
+                    const bool s1 = mq.w != 0.f;
+                    const float t = fexp2(fmaf(mq.x, s1 ? L[1] : L[0], -mq.y * r2sum));
+                    B = fmaf(wq.y, t, B);
+                    C0 = fmaf(wq.z, t, C0);
+                    A = fmaf(wq.x, t * (s1 ? Rp[1] : Rp[0]), A);
                 }
+            } else {
+                float ec = 0.f, fij = 1.f, fik = 1.f, fjk = 1.f;
+                bool shifted = false;
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q)
+                    if (q < nmem) {
+                        const float4 mq = S.mp[q];
+                        const float4 wq = S.mp[kMaxMembers + q];
+                        if ((R.calc >> q) & 1u) {
+                            float arg = r2sum;
+                            shifted = mq.z > 0.f;
+                            if (shifted) {
+                                const float a = gj.w - mq.z, b = gk.w - mq.z, c = rjk - mq.z;
+                                arg = a * a + b * b + c * c;
+                                fij = a * cj.x;
+                                fik = b * ck.x;
+                                fjk = c * ijk;
+                            }
+                            ec = -mq.y * arg;
+                        }
+                        const bool s1 = mq.w != 0.f, s2 = wq.w != 0.f;
+                        const float Ls = s2 ? (s1 ? L[3] : L[2]) : (s1 ? L[1] : L[0]);
+                        const float Rs = s2 ? (s1 ? Rp[3] : Rp[2]) : (s1 ? Rp[1] : Rp[0]);
+                        const float t = fexp2(fmaf(mq.x, Ls, ec));
+                        B = fmaf(wq.y, t, B);
+                        const float c = wq.z * t;
+                        if (shifted) {
+                            C1 = fmaf(c, fij, C1);
+                            C2 = fmaf(c, fik, C2);
+                            C3 = fmaf(c, fjk, C3);
+                        } else {
+                            C0 += c;
+                        }
+                        A = fmaf(wq.x, t * Rs, A);
+                    }
+            }
             C1 += C0; C2 += C0; C3 += C0;
             const float P1 = pfc * (A * (rinvijik - cost * cj.x * cj.x) - C1) + B * pr1;
             const float P2 = pfc * (A * (rinvijik - cost * ck.x * ck.x) - C2) + B * pr2;
@@ -687,6 +800,25 @@ __global__ void k_mfma_probe(const float* A, const float* B, float* D)
     for (int v = 0; v < 4; ++v) D[(4 * (lane >> 4) + v) * 16 + (lane & 15)] = c[v];
 }
 
+// Calls f(integral_constant MAXM, bool_constant FAST) for the kernel
+// instance matching the model: exact member counts {4, 8, 10, 16} on the fast
+// path, powers of two up to 32 on the general one.
+template <typename F>
+void dispatch_members(int exact, int maxm, F&& f)
+{
+    using std::integral_constant;
+    switch (exact) {
+    case 4: f(integral_constant<int, 4>{}, std::true_type{}); return;
+    case 8: f(integral_constant<int, 8>{}, std::true_type{}); return;
+    case 10: f(integral_constant<int, 10>{}, std::true_type{}); return;
+    case 16: f(integral_constant<int, 16>{}, std::true_type{}); return;
+    default: break;
+    }
+    if (maxm <= 8) f(integral_constant<int, 8>{}, std::false_type{});
+    else if (maxm <= 16) f(integral_constant<int, 16>{}, std::false_type{});
+    else f(integral_constant<int, 32>{}, std::false_type{});
+}
+
 int pow2_members(const ModelT& t)
 {
     const int m = std::max(std::max(t.max_mem_ang, t.max_mem_rad), 1);
@@ -700,11 +832,16 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
     free_mixed(mm);
     const ModelT& t = hm.t;
     std::vector<MemberF> mf(std::max(t.nmembers, 1));
+    std::vector<GroupMixF> gmix(std::max(t.ngroups, 1));
     std::vector<float> sffac((size_t)kMaxElem * kMaxSF, 0.f);
+    mm.maxS = 1;
+    mm.hasRs = false;
     for (int e = 0; e < t.nelem; ++e) {
         const ElementT& E = t.el[e];
         for (int g = E.grp0; g < E.grp0 + E.ngrp; ++g) {
             const GroupT& G = t.grp[g];
+            GroupMixF& gm = gmix[g];
+            gm = GroupMixF{};
             for (int q = G.mem0; q < G.mem0 + G.nmem; ++q) {
                 const MemberT& M = t.mem[q];
                 MemberF& f = mf[q];
@@ -715,13 +852,39 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 f.gnorm = G.type == 3 ? (float)M.gnorm : 1.f;
                 f.twoEta = (float)(2.0 * M.eta);
                 f.zetaLambda = (float)M.zetaLambda;
+                f.etaL = (float)(M.eta * 1.4426950408889634);  // eta * log2(e)
                 f.sf = M.sf;
                 f.calcExp = M.calcExp;
-                f.zetaInt = M.zetaInt;
-                f.useIntPow = M.useIntPow;
+                f.sidx = 0;
+                if (G.type == 3) {
+                    if (M.rs > 0.0) mm.hasRs = true;
+                    int k = 0;
+                    while (k < gm.nlam && gm.lam[k] != f.lambda) ++k;
+                    if (k == gm.nlam) {
+                        if (gm.nlam == 4) throw Error{NNPX_ERR_UNSUPPORTED, "more than 4 distinct lambda per SF group"};
+                        gm.lam[gm.nlam++] = f.lambda;
+                    }
+                    f.sidx = k;
+                }
                 sffac[(size_t)e * kMaxSF + M.sf] = (float)(G.type == 3 ? M.factorNorm : M.sfac);
             }
+            mm.maxS = std::max(mm.maxS, gm.nlam);
+            // unused table slots repeat the first lambda (evaluated, never picked)
+            for (int k = gm.nlam; k < 4; ++k) gm.lam[k] = gm.nlam ? gm.lam[0] : 0.f;
         }
+    }
+    mm.maxS = mm.maxS <= 2 ? 2 : 4;
+    // fast path: all angular groups share one member count, <= 2 lambda, rs = 0
+    mm.fastM = 0;
+    if (mm.maxS == 2 && !mm.hasRs) {
+        int cnt = -1;
+        bool same = true;
+        for (int g = 0; g < t.ngroups; ++g)
+            if (t.grp[g].type == 3) {
+                if (cnt < 0) cnt = t.grp[g].nmem;
+                same = same && t.grp[g].nmem == cnt;
+            }
+        if (same && (cnt == 4 || cnt == 8 || cnt == 10 || cnt == 16)) mm.fastM = cnt;
     }
     // bf16 weights, padded to 16: W [out][in] and WT [in][out], hi then lo.
     std::vector<uint16_t> w;
@@ -768,6 +931,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
         if (hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) throw Error{NNPX_ERR_HIP, "hipMemcpy"};
     };
     up((void**)&mm.d_mem, mf.data(), sizeof(MemberF) * mf.size());
+    up((void**)&mm.d_gmix, gmix.data(), sizeof(GroupMixF) * gmix.size());
     up((void**)&mm.d_sffac, sffac.data(), sizeof(float) * sffac.size());
     up((void**)&mm.d_w, w.data(), sizeof(uint16_t) * w.size());
     up((void**)&mm.d_b, b.data(), sizeof(float) * b.size());
@@ -777,6 +941,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
 void free_mixed(MixedModel& mm)
 {
     if (mm.d_mem) (void)hipFree(mm.d_mem);
+    if (mm.d_gmix) (void)hipFree(mm.d_gmix);
     if (mm.d_sffac) (void)hipFree(mm.d_sffac);
     if (mm.d_w) (void)hipFree(mm.d_w);
     if (mm.d_b) (void)hipFree(mm.d_b);
@@ -796,12 +961,11 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     const size_t lds_f = lds_sf + sizeof(double) * 3 * (size_t)capL;
     if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     timed("sf32", [&] {
-        switch (P) {
-        case 4: hipLaunchKernelGGL(k_sf32<4>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
-        case 8: hipLaunchKernelGGL(k_sf32<8>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
-        case 16: hipLaunchKernelGGL(k_sf32<16>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
-        default: hipLaunchKernelGGL(k_sf32<32>, dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, s, b.G, b.Gin, b.ldG, mm.ldK, capL); break;
-        }
+        dispatch_members(mm.fastM, P, [&](auto m, auto fast) {
+            hipLaunchKernelGGL((k_sf32<decltype(m)::value, decltype(fast)::value>),
+                               dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, mm.d_gmix, s, b.G,
+                               b.Gin, b.ldG, mm.ldK, capL);
+        });
     });
     NNMix a{};
     a.w = mm.d_w;
@@ -840,12 +1004,11 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
         hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0, stream,
                            b.F, n3);
         const int ps = push ? 1 : 0;
-        switch (P) {
-        case 4: hipLaunchKernelGGL(k_force32<4>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
-        case 8: hipLaunchKernelGGL(k_force32<8>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
-        case 16: hipLaunchKernelGGL(k_force32<16>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
-        default: hipLaunchKernelGGL(k_force32<32>, dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, s, b.dEdGf, b.ldG, b.F, ps, capL); break;
-        }
+        dispatch_members(mm.fastM, P, [&](auto m, auto fast) {
+            hipLaunchKernelGGL((k_force32<decltype(m)::value, decltype(fast)::value>),
+                               dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, mm.d_gmix, s, b.dEdGf,
+                               b.ldG, b.F, ps, capL);
+        });
     });
 }
 

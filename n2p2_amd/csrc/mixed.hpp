@@ -27,8 +27,15 @@ struct MemberF {
     float gnorm;       // angular: 2^(1-zeta) (G normalisation)
     float twoEta;      // 2 eta
     float zetaLambda;  // zeta * lambda
-    float pad;
-    int sf, calcExp, zetaInt, useIntPow;
+    float etaL;        // eta * log2(e): exponent factor for exp2
+    int sf, calcExp, sidx, pad;  // sidx: index of lambda in the group's lambda table
+};
+
+// Per-group table of the distinct lambda values (members index it by sidx).
+struct GroupMixF {
+    float lam[4];
+    int nlam;
+    int pad[3];
 };
 
 struct MixedModel {
@@ -36,6 +43,10 @@ struct MixedModel {
     int maxpad = 0;             // max padded layer width
     int terms = 3;              // bf16 product terms (1 or 3)
     MemberF* d_mem = nullptr;   // [nmembers], same order as ModelT::mem
+    GroupMixF* d_gmix = nullptr; // [ngroups]
+    int maxS = 2;               // max distinct lambda values per angular group (2 or 4)
+    bool hasRs = false;         // some angular member has rs > 0
+    int fastM = 0;              // exact angular member count of the branch-free path (0: general path)
     float* d_sffac = nullptr;   // [nelem*kMaxSF] force weight factor per SF
     uint16_t* d_w = nullptr;    // bf16 weights: per element/layer W hi, W lo, WT hi, WT lo
     float* d_b = nullptr;       // FP32 biases, padded
