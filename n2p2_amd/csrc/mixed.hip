@@ -115,15 +115,15 @@ __device__ inline GroupF group_f(const GroupT& g)
     return GroupF{(float)g.rc, (float)g.rc2, (float)g.rci, (float)g.iw, (float)g.rcinv};
 }
 
-constexpr int kTrip = 512;  // compacted triplets buffered per wave
+constexpr int kTrip = 384;  // compacted triplets buffered per wave
 
 // LDS staging of one atom's neighbours (FP32 geometry) and the compacted
 // triplet buffer.
 struct Stage {
     float4* g;       // x, y, z, |dr|
     float4* c;       // 1/|dr|, fc, dfc (cutoff of the current group), unused
-    int* el;         // neighbour element
-    int* la; int* lb;  // group lists
+    unsigned char* el;  // neighbour element
+    unsigned short* la; unsigned short* lb;  // group lists
     unsigned* tjk;   // compacted pair (j | k << 16)
     float* tr2;      // its r_jk^2
     float4* mp;      // [2*kMaxMembers] member constants of the current group
@@ -134,17 +134,19 @@ __device__ inline Stage carve(float* base, int capL)
     Stage S;
     S.g = reinterpret_cast<float4*>(base);
     S.c = S.g + capL;
-    S.el = reinterpret_cast<int*>(S.c + capL);
-    S.la = S.el + capL;
-    S.lb = S.la + capL;
-    S.tjk = reinterpret_cast<unsigned*>(S.lb + capL);
+    S.tjk = reinterpret_cast<unsigned*>(S.c + capL);
     S.tr2 = reinterpret_cast<float*>(S.tjk + kTrip);
     S.mp = reinterpret_cast<float4*>(S.tr2 + kTrip);
+    S.la = reinterpret_cast<unsigned short*>(S.mp + 2 * kMaxMembers);
+    S.lb = S.la + capL;
+    S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
     return S;
 }
+// bytes of the staging area (capL is a multiple of 32, so the byte arrays end
+// 8-byte aligned)
 __host__ __device__ constexpr size_t stage_bytes(int capL)
 {
-    return (size_t)capL * (32 + 12) + (size_t)kTrip * 8 + (size_t)2 * kMaxMembers * 16;
+    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)kTrip * 8 + (size_t)2 * kMaxMembers * 16;
 }
 
 __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
@@ -163,14 +165,14 @@ __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage&
         const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
         S.g[k] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
         S.c[k] = make_float4((float)(1.0 / d), 0.f, 0.f, 0.f);
-        S.el[k] = s.elem[j];
+        S.el[k] = (unsigned char)s.elem[j];
     }
     __syncthreads();
     return n;
 }
 
 // Ordered list of neighbours with element `elem` inside rc.
-__device__ inline int make_list(const Stage& S, int n, int elem, float rc, int* out)
+__device__ inline int make_list(const Stage& S, int n, int elem, float rc, unsigned short* out)
 {
     const int lane = threadIdx.x;
     int cnt = 0;
@@ -178,7 +180,7 @@ __device__ inline int make_list(const Stage& S, int n, int elem, float rc, int* 
         const int k = b + lane;
         const bool ok = k < n && S.el[k] == elem && S.g[k].w < rc;
         const int incl = wave_incl_scan(ok ? 1 : 0);
-        if (ok) out[cnt + incl - 1] = k;
+        if (ok) out[cnt + incl - 1] = (unsigned short)k;
         cnt += __shfl(incl, kWave - 1, kWave);
     }
     __syncthreads();
@@ -273,7 +275,7 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
         full = 0;
         total = nA * nB;
     }
-    const int* lb = same ? S.la : S.lb;
+    const unsigned short* lb = same ? S.la : S.lb;
     const float inv = nA > 0 ? 1.f / (float)nA : 0.f;
     int t0 = 0;
     while (t0 < total) {
