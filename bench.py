@@ -63,6 +63,38 @@ def time_form(m, form, steps, warmup):
     return dt, times
 
 
+def traffic_from_profile(kernel):
+    """HBM bytes per launch of `kernel` as recorded by the FETCH_SIZE /
+    WRITE_SIZE counter runs (scaled by a copy-kernel calibration) in
+    profiles/hbm_traffic_water12288.txt; None when that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_water12288.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 3 and f[0] == kernel and f[1] == "traffic_bytes_per_launch":
+            return float(f[2])
+    return None
+
+
+def calibrate_copy(mib):
+    """A plain device copy of known size (torch's copy kernel), five
+    launches: the FETCH_SIZE / WRITE_SIZE counters of this run give the
+    scale factor between counter readings and bytes actually moved."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.rand(n, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                      "copy_GBps": 2 * n * 4 / dt / 1e9}))
+
+
 def cpu_baseline(model_dir, pos, elem, box, budget_s=15.0):
     """n2p2's compute path on the host cores: reference classes (SF groups,
     NN, force contraction compiled from n2p2's sources) when built, else the
@@ -111,11 +143,17 @@ def main():
     ap.add_argument("--form", choices=["mixed", "plain"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     import n2p2_amd
     from n2p2_amd import remap, synth
@@ -152,12 +190,13 @@ def main():
         achieved = sf_flop / avg_s / 1e12
         peak = PEAK_FP64 if dom == "sf64" else PEAK_FP32
         roof = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                "frac": achieved / peak, "traffic": None,
+                "frac": achieved / peak, "traffic": traffic_from_profile("k_" + dom),
                 "flop_per_launch": sf_flop, "avg_launch_ms": avg_s * 1e3}
     else:
         achieved = nn_flop / avg_s / 1e12
         roof = {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": PEAK_FP32, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32, "traffic": None, "flop_per_launch": nn_flop,
+                "frac": achieved / PEAK_FP32, "traffic": traffic_from_profile("k_" + dom),
+                "flop_per_launch": nn_flop,
                 "avg_launch_ms": avg_s * 1e3}
 
     out = {
