@@ -90,6 +90,13 @@ nnpx_status nnpx_num_sf(const nnpx_ctx* ctx, int element, int* num_sf);
 nnpx_status nnpx_atomic_number(const nnpx_ctx* ctx, int element, int* z);
 /* Element index of a symbol (ElementMap::operator[]). */
 nnpx_status nnpx_element_index(const nnpx_ctx* ctx, const char* symbol, int* element);
+/* Number of SFs of a centre atom of element e that depend on a neighbour of
+ * element e_nbr: the length of Atom::Neighbor::dGdr
+ * (Element::getSymmetryFunctionNumTable, Element.cpp:289-311). */
+nnpx_status nnpx_num_sf_derivatives(const nnpx_ctx* ctx, int element, int element_nbr, int* n);
+/* Symbol of element index e (ElementMap::operator[](size_t)), written with
+ * its terminating NUL into buf of len bytes. */
+nnpx_status nnpx_element_symbol(const nnpx_ctx* ctx, int element, char* buf, int len);
 /* normalize flag, mean_energy, conv_energy, conv_length and the per-element
  * atom_energy offsets (Mode.cpp:239-300, 2079-2096); offsets has
  * num_elements entries. */

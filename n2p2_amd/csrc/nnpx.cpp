@@ -629,6 +629,29 @@ nnpx_status nnpx_element_index(const nnpx_ctx* cc, const char* sym, int* e)
     });
 }
 
+nnpx_status nnpx_num_sf_derivatives(const nnpx_ctx* cc, int e, int e_nbr, int* n)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        const int ne = c->hm.t.nelem;
+        if (e < 0 || e >= ne || e_nbr < 0 || e_nbr >= ne || !n) fail(NNPX_ERR_ARGUMENT, "bad element index");
+        *n = c->hm.t.el[e].nsfd[e_nbr];
+    });
+}
+
+nnpx_status nnpx_element_symbol(const nnpx_ctx* cc, int e, char* buf, int len)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (e < 0 || e >= c->hm.t.nelem || !buf || len <= 0) fail(NNPX_ERR_ARGUMENT, "bad element index or buffer");
+        const std::string& s = c->hm.symbols[e];
+        if ((int)s.size() >= len) fail(NNPX_ERR_RANGE, "symbol buffer too small");
+        std::memcpy(buf, s.c_str(), s.size() + 1);
+    });
+}
+
 nnpx_status nnpx_model_units(const nnpx_ctx* cc, int* normalize, double* mean, double* ce, double* cl,
                              double* offsets)
 {
