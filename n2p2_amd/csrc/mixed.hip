@@ -116,6 +116,7 @@ __device__ inline GroupF group_f(const GroupT& g)
 }
 
 constexpr int kTrip = 384;  // compacted triplets buffered per wave
+constexpr int kRadChunk = 8;  // radial members accumulated per pass
 
 // LDS staging of one atom's neighbours (FP32 geometry) and the compacted
 // triplet buffer.
@@ -352,22 +353,42 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
         const int nmem = GT.nmem;
         const MemberF* mf = MF + GT.mem0;
         group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        if (GT.type == 2) {
+            // radial members in chunks of kRadChunk (any member count)
+            for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
+                float acc[kRadChunk], cmp[kRadChunk];
+#pragma unroll
+                for (int q = 0; q < kRadChunk; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
+                for (int k = lane; k < n; k += kWave) {
+                    const float r = S.g[k].w;
+                    if (S.el[k] != GT.e1 || !(r < g.rc)) continue;
+                    const float fc = S.c[k].y;
+#pragma unroll
+                    for (int q = 0; q < kRadChunk; ++q)
+                        if (q0 + q < nmem) {
+                            const float dr = r - mf[q0 + q].rs;
+                            kahan(acc[q], cmp[q], __expf(-mf[q0 + q].eta * dr * dr) * fc);
+                        }
+                }
+#pragma unroll
+                for (int q = 0; q < kRadChunk; ++q)
+                    if (q0 + q < nmem) {
+                        const double v = wave_sum((double)acc[q] + (double)cmp[q]);
+                        if (lane == 0) {
+                            const MemberT& mt = M->mem[GT.mem0 + q0 + q];
+                            const double gs = scale_sf(M->scale[E.sf0 + mt.sf], v);
+                            Gi[mt.sf] = gs;
+                            Ii[mt.sf] = (float)gs;
+                        }
+                    }
+            }
+            __syncthreads();
+            continue;
+        }
         float acc[MAXM], cmp[MAXM];
 #pragma unroll
         for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
-        if (GT.type == 2) {
-            for (int k = lane; k < n; k += kWave) {
-                const float r = S.g[k].w;
-                if (S.el[k] != GT.e1 || !(r < g.rc)) continue;
-                const float fc = S.c[k].y;
-#pragma unroll
-                for (int q = 0; q < MAXM; ++q)
-                    if (q < nmem) {
-                        const float dr = r - mf[q].rs;
-                        kahan(acc[q], cmp[q], __expf(-mf[q].eta * dr * dr) * fc);
-                    }
-            }
-        } else {
+        {
             const bool same = GT.e1 == GT.e2;
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
@@ -426,7 +447,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 double v = wave_sum((double)acc[q] + (double)cmp[q]);
                 if (lane == 0) {
                     const MemberT& mt = M->mem[GT.mem0 + q];
-                    if (GT.type == 3) v *= mt.gnorm;
+                    v *= mt.gnorm;
                     const double gs = scale_sf(M->scale[E.sf0 + mt.sf], v);
                     Gi[mt.sf] = gs;
                     Ii[mt.sf] = (float)gs;
@@ -646,13 +667,12 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                 if (S.el[k] != GT.e1 || !(gk.w < g.rc)) continue;
                 const float4 ck = S.c[k];
                 float P = 0.f;
-#pragma unroll
-                for (int q = 0; q < MAXM; ++q)
-                    if (q < nmem) {
-                        const float dr = gk.w - mf[q].rs;
-                        const float ex = __expf(-mf[q].eta * dr * dr);
-                        P += wi[mf[q].sf] * (ck.z - mf[q].twoEta * dr * ck.y) * ex;
-                    }
+#pragma unroll 2
+                for (int q = 0; q < nmem; ++q) {
+                    const float dr = gk.w - mf[q].rs;
+                    const float ex = __expf(-mf[q].eta * dr * dr);
+                    P += wi[mf[q].sf] * (ck.z - mf[q].twoEta * dr * ck.y) * ex;
+                }
                 P *= ck.x;
                 const double vx = (double)(P * gk.x), vy = (double)(P * gk.y), vz = (double)(P * gk.z);
                 fx -= vx; fy -= vy; fz -= vz;

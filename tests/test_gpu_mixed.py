@@ -126,6 +126,27 @@ def test_batched_mixed(force_mode, water_dir):
     assert rms <= F_RMS
 
 
+def test_fcc_four_species_mixed(tmp_path_factory):
+    """Four-species FCC model (80 SFs: 10-member radial groups, 4-member
+    angular groups, cutoff 6.0 A): 5^3 cells, 500 atoms, cell list."""
+    say("case: FCC 500 atoms mixed")
+    d = synth.fcc_model(str(tmp_path_factory.mktemp("fcc")))
+    pos, elem, box = synth.fcc_lattice(n_cells=5)
+    m = n2p2_amd.Mode(d)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE)
+    G, Go = m.G(), o.G()
+    gerr = float((np.abs(G - Go) / np.maximum(np.abs(Go), 1.0)).max())
+    rms, mx, erel = errors(m, o)
+    say("  G err %.2e  E rel %.2e  F rms %.2e max %.2e" % (gerr, erel, rms, mx))
+    assert gerr <= 1e-5
+    assert rms <= F_RMS
+    assert erel <= 1e-5
+
+
 def test_plain_forces_match_finite_differences(water_dir):
     """FP64 GPU forces vs central differences of the FP64 GPU energy
     (nnp-checkf.cpp:173-295 semantics, threshold 10*h^2)."""

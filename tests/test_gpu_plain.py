@@ -143,6 +143,24 @@ def test_water12288_plain_vs_oracle(water_dir):
     assert np.abs(F - Fo).max() <= F_ABS
 
 
+def test_fcc_four_species_plain(tmp_path_factory):
+    """Four-species FCC model (80 SFs, 80-30-30-1 NN, rc 6.0): 5^3 cells."""
+    say("case: FCC 500 atoms plain FP64")
+    d = synth.fcc_model(str(tmp_path_factory.mktemp("fcc")))
+    pos, elem, box = synth.fcc_lattice(n_cells=5)
+    m = n2p2_amd.Mode(d)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE)
+    E, Eo = m.energies()[0], o.energy()
+    dF = float(np.abs(m.forces() - o.forces()).max())
+    say("  E rel %.2e  max|dF| %.2e" % (abs(E - Eo) / abs(Eo), dF))
+    assert abs(E - Eo) / abs(Eo) <= E_REL
+    assert dF <= F_ABS
+
+
 def test_step_api_equals_evaluate(water_dir):
     """calculate_symmetry_function_groups / _atomic_neural_networks / _energy /
     _forces in sequence give the evaluate results."""

@@ -184,8 +184,8 @@ def _water_dir():
 
 
 @pytest.mark.skipif(not ref_available(), reason="reference compute classes not built (no reference tree)")
-@pytest.mark.parametrize("case", ["golden", "water192", "small_cell"])
-def test_oracle_bitwise_equals_reference_classes(case, water_model_dir):
+@pytest.mark.parametrize("case", ["golden", "water192", "small_cell", "fcc108"])
+def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path):
     """The oracle reproduces n2p2's own SymGrp/NeuralNetwork/Atom code bit for bit."""
     from n2p2_amd import read_input_data, synth
     if case == "golden":
@@ -194,6 +194,9 @@ def test_oracle_bitwise_equals_reference_classes(case, water_model_dir):
     elif case == "water192":
         d = water_model_dir
         pos, elem, box = synth.water_box(4)
+    elif case == "fcc108":
+        d = synth.fcc_model(str(tmp_path / "fcc"))
+        pos, elem, box = synth.fcc_lattice(n_cells=3)
     else:
         d = water_model_dir
         rng = np.random.default_rng(3)
