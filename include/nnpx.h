@@ -162,6 +162,30 @@ nnpx_status nnpx_get_neighbors(nnpx_ctx* ctx, int64_t* total, int* counts,
 nnpx_status nnpx_get_dGdr(nnpx_ctx* ctx, double* self, int ld_self,
                           double* nbr, int ld_nbr);
 
+/* ---- molecular dynamics ----------------------------------------------- */
+/* No counterpart in nnp::Mode: the velocity-Verlet loop a driver runs around
+ * the energy/force call (LAMMPS fix nve with pair_style hdnnp,
+ * InterfaceLammps.cpp:463-500 per step).  One structure per context.
+ *   masses[num_elements]: amu; vel[3N]: A/fs (physical); dt: fs;
+ *   skin: A (physical), added to the list cutoff; the list is rebuilt on the
+ *   device when an atom moved more than skin/2 since the last build.
+ * Computes the forces of the initial positions. */
+nnpx_status nnpx_md_init(nnpx_ctx* ctx, int precision, const double* masses,
+                         const double* vel, double dt, double skin);
+/* Runs nsteps steps.  graph_steps > 0 captures that many steps in a HIP
+ * graph (once, re-captured when graph_steps changes or a buffer grows) and
+ * replays it; the remainder runs eagerly.  Fails with NNPX_ERR_RANGE if the
+ * neighbour capacity overflowed during the run. */
+nnpx_status nnpx_md_run(nnpx_ctx* ctx, int nsteps, int graph_steps);
+/* Current positions (model units, unwrapped since the last rebuild) and
+ * velocities (A/fs); either pointer may be NULL. */
+nnpx_status nnpx_md_state(nnpx_ctx* ctx, double* pos, double* vel);
+/* Potential and kinetic energy (model energy units) of the first
+ * min(max, steps) steps since nnpx_md_init, the step count and the number
+ * of list rebuilds. */
+nnpx_status nnpx_md_energies(nnpx_ctx* ctx, double* epot, double* ekin,
+                             int64_t max, int64_t* nsteps, int64_t* rebuilds);
+
 /* ---- measurement ------------------------------------------------------ */
 /* No counterpart in the reference: multiplies two known 16x16 matrices with
  * the bf16 MFMA operand layout the NN kernels use and returns the max abs

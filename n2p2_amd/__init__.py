@@ -74,6 +74,13 @@ _SIGS = {
     "nnpx_reset_kernel_times": [ctypes.c_void_p],
     "nnpx_work_counts": [ctypes.c_void_p, _c_i64_p, _c_i64_p, _c_i64_p, _c_i64_p, _c_i64_p],
     "nnpx_mfma_selftest": [ctypes.c_void_p, _c_dbl_p],
+    "nnpx_element_symbol": [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
+    "nnpx_num_sf_derivatives": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_int_p],
+    "nnpx_md_init": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                     ctypes.c_double],
+    "nnpx_md_run": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int],
+    "nnpx_md_state": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_md_energies": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _c_i64_p, _c_i64_p],
 }
 
 
@@ -271,6 +278,34 @@ class Mode:
         v = ctypes.c_double()
         self._ck(self._lib.nnpx_mfma_selftest(self._h, ctypes.byref(v)))
         return v.value
+
+    # ------------------------------------------------------------ MD --
+    def md_init(self, masses, velocities, dt, skin, precision=MIXED):
+        """Velocity Verlet set-up: masses per element (amu), velocities
+        [N, 3] (A/fs), dt (fs), list skin (A); computes the initial forces."""
+        m = np.ascontiguousarray(masses, dtype=np.float64)
+        v = np.ascontiguousarray(velocities, dtype=np.float64).reshape(-1)
+        if len(m) != self.num_elements or len(v) != 3 * self.n_atoms:
+            raise NNPError(1, "masses or velocities have the wrong size")
+        self._ck(self._lib.nnpx_md_init(self._h, precision, _ptr(m), _ptr(v), float(dt), float(skin)))
+
+    def md_run(self, nsteps, graph_steps=0):
+        self._ck(self._lib.nnpx_md_run(self._h, int(nsteps), int(graph_steps)))
+
+    def md_state(self):
+        p = np.zeros((self.n_atoms, 3))
+        v = np.zeros((self.n_atoms, 3))
+        self._ck(self._lib.nnpx_md_state(self._h, _ptr(p), _ptr(v)))
+        return p, v
+
+    def md_energies(self, max_steps=1 << 16):
+        ep = np.zeros(max_steps)
+        ek = np.zeros(max_steps)
+        n, r = ctypes.c_int64(), ctypes.c_int64()
+        self._ck(self._lib.nnpx_md_energies(self._h, _ptr(ep), _ptr(ek), max_steps, ctypes.byref(n),
+                                            ctypes.byref(r)))
+        k = min(n.value, max_steps)
+        return ep[:k], ek[:k], r.value
 
     def work_counts(self):
         v = [ctypes.c_int64() for _ in range(5)]

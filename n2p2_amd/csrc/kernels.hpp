@@ -31,14 +31,18 @@ struct SysView {
     int* cell_start;       // [ncells+1]
     int* cell_atoms;       // [N]
     double* cell_pos;      // [3N] positions in cell order
+    // when non-null, the list kernels run only if *gate != 0 (device-side
+    // rebuild decision inside captured MD steps)
+    const int* gate;
 };
 
-void fill_int(int* p, int v, size_t n, hipStream_t stream);
+void fill_int(int* p, int v, size_t n, hipStream_t stream, const int* gate = nullptr);
 
 // Neighbour list (Structure::calculateNeighborList, Structure.cpp:287-398):
 // cell assignment and per-atom build (brute force or cell list, same set
 // and order).
-void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream);
+// clear_flags: reset the overflow flags first (MD keeps them across rebuilds).
+void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream, bool clear_flags = true);
 // Reverse entries: for (i -> j, image) the slot of (j -> i, -image).
 void launch_reverse(const SysView& s, hipStream_t stream);
 

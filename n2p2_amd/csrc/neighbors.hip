@@ -19,8 +19,11 @@ namespace {
 
 constexpr int kNbMaxLds = 1024;  // max entries sorted in LDS per atom
 
+__device__ inline bool gated_off(const SysView& s) { return s.gate && *s.gate == 0; }
+
 __global__ void k_cell_count(SysView s)
 {
+    if (gated_off(s)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.natoms) return;
     const StructT& S = s.st[s.sid[i]];
@@ -41,6 +44,7 @@ __global__ void k_cell_count(SysView s)
 // Exclusive scan of cell counts (single block; ncells is small).
 __global__ __launch_bounds__(1024) void k_scan_cells(SysView s)
 {
+    if (gated_off(s)) return;
     __shared__ int part[1024];
     const int t = threadIdx.x;
     const int n = s.ncells;
@@ -67,6 +71,7 @@ __global__ __launch_bounds__(1024) void k_scan_cells(SysView s)
 
 __global__ void k_cell_fill(SysView s)
 {
+    if (gated_off(s)) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.natoms) return;
     const int c = s.atom_cell[i];
@@ -90,6 +95,7 @@ __device__ inline void emit(const SysView& s, int i, int slot, int j, int img)
 // One wave per atom.
 __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
 {
+    if (gated_off(s)) return;
     __shared__ long long keys[kNbMaxLds];
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
@@ -224,6 +230,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
 // Reverse entry: (i -> j, image) is stored in j's list as (i, -image).
 __global__ void k_reverse(SysView s)
 {
+    if (gated_off(s)) return;
     const int i = blockIdx.x;
     const int n = s.nb_n[i];
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
@@ -244,8 +251,9 @@ __global__ void k_reverse(SysView s)
     }
 }
 
-__global__ void k_fill_int(int* p, int v, size_t n)
+__global__ void k_fill_int(int* p, int v, size_t n, const int* gate)
 {
+    if (gate && *gate == 0) return;
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
         p[q] = v;
 }
@@ -254,19 +262,19 @@ __global__ void k_fill_int(int* p, int v, size_t n)
 
 // Memory clears use a fill kernel (not hipMemsetAsync) so the sequence can be
 // captured in a HIP graph.
-void fill_int(int* p, int v, size_t n, hipStream_t stream)
+void fill_int(int* p, int v, size_t n, hipStream_t stream, const int* gate)
 {
     if (n == 0) return;
     const int nb = (int)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_fill_int, dim3(nb), dim3(256), 0, stream, p, v, n);
+    hipLaunchKernelGGL(k_fill_int, dim3(nb), dim3(256), 0, stream, p, v, n, gate);
 }
 
-void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream)
+void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream, bool clear_flags)
 {
     if (s.natoms == 0) return;
-    fill_int(s.flags, 0, 2, stream);
+    if (clear_flags) fill_int(s.flags, 0, 2, stream, s.gate);
     if (any_cell) {
-        fill_int(s.cell_count, 0, (size_t)s.ncells, stream);
+        fill_int(s.cell_count, 0, (size_t)s.ncells, stream, s.gate);
         const int nb = (s.natoms + 255) / 256;
         hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, stream, s);
         hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, s);

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "md.hpp"
 #include "mixed.hpp"
 #include "model.hpp"
 
@@ -66,6 +67,14 @@ struct nnpx_ctx {
     bool sf_valid = false, sfd_valid = false, nn_valid = false, dedg_valid = false;
     bool e_valid = false, f_valid = false;
     int last_precision = NNPX_PLAIN_FP64;
+    // MD (velocity Verlet with a skin-buffered list)
+    double skin = 0.0;           // extra list cutoff, model length units
+    bool md_ready = false;
+    int md_precision = NNPX_MIXED;
+    MDConst mdc{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
     // timing
     std::vector<Timer> pending;
     std::map<std::string, std::pair<double, int64_t>> ktime;
@@ -106,12 +115,24 @@ nnpx_status guard(nnpx_ctx* c, F&& f)
     }
 }
 
+void drop_graph(nnpx_ctx* c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->gexec = nullptr;
+    c->graph = nullptr;
+    c->graph_steps = 0;
+}
+
+// Named device buffer of at least count elements; growing one invalidates a
+// captured MD graph (its kernels hold the old addresses).
 template <typename T>
 T* buf(nnpx_ctx* c, const std::string& name, size_t count)
 {
     DevBuf& b = c->bufs[name];
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     if (b.bytes < bytes) {
+        drop_graph(c);
         if (b.p) hip_check(hipFree(b.p), "hipFree");
         b.p = nullptr;
         b.bytes = 0;
@@ -210,6 +231,8 @@ void invalidate(nnpx_ctx* c)
     c->sf_valid = c->sfd_valid = c->nn_valid = c->dedg_valid = c->e_valid = c->f_valid = false;
 }
 
+double list_cutoff(const nnpx_ctx* c) { return c->hm.t.maxCutoff + c->skin; }
+
 // Geometry tables of one structure: inverse box (Structure.cpp:518-544),
 // image range (Structure::calculatePbcCopies, Structure.cpp:485-516) and the
 // cell grid of the cell list.
@@ -294,7 +317,7 @@ void choose_algorithms(nnpx_ctx* c, const double* pos)
 
 int estimate_cap(nnpx_ctx* c)
 {
-    const double rc = c->hm.t.maxCutoff;
+    const double rc = list_cutoff(c);
     int cap = 32;
     for (int s = 0; s < c->S; ++s) {
         const StructT& t = c->st_host[s];
@@ -343,7 +366,7 @@ void build_neighbors(nnpx_ctx* c, bool check)
     for (int attempt = 0; attempt < 3; ++attempt) {
         alloc_neighbors(c);
         SysView s = sysview(c);
-        timed(c, "nbr", [&] { launch_neighbors(s, c->hm.t.maxCutoff, c->any_cell, c->stream); });
+        timed(c, "nbr", [&] { launch_neighbors(s, list_cutoff(c), c->any_cell, c->stream); });
         c->rev_valid = false;
         if (!check) break;
         int f[2];
@@ -417,6 +440,15 @@ void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check)
 {
     const bool forces = flags & NNPX_EVAL_FORCES;
     invalidate(c);
+    if (c->md_ready) {
+        // MD leaves positions unwrapped between rebuilds: wrap them (and
+        // reset the displacement reference) before a full list build.
+        int* gate = existing<int>(c, "md_gate");
+        md_set_gate(gate, 1, c->stream);
+        md_wrap(existing<double>(c, "pos"), existing<double>(c, "md_ref"), existing<int>(c, "sid"),
+                existing<StructT>(c, "st"), c->N, gate, c->stream);
+        md_set_gate(gate, 0, c->stream);
+    }
     build_neighbors(c, check);
     if (precision == NNPX_MIXED) {
         MixedBuffers mb = mixed_buffers(c);
@@ -515,6 +547,7 @@ nnpx_status nnpx_destroy(nnpx_ctx* c)
     if (c->d_model) (void)hipFree(c->d_model);
     if (c->d_w) (void)hipFree(c->d_w);
     free_mixed(c->mixed);
+    drop_graph(c);
     for (auto& t : c->pending) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : c->event_pool) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(c->stream);
@@ -560,6 +593,9 @@ nnpx_status nnpx_load_model(nnpx_ctx* c, const char* dir)
         HostModel hm;
         load_model(dir, hm);
         c->hm = std::move(hm);
+        c->skin = 0.0;
+        c->md_ready = false;
+        drop_graph(c);
         const ModelT& t = c->hm.t;
         if (!c->d_model) hip_check(hipMalloc(&c->d_model, sizeof(ModelT)), "hipMalloc model");
         hip_check(hipMemcpy(c->d_model, &t, sizeof(ModelT), hipMemcpyHostToDevice), "hipMemcpy model");
@@ -678,6 +714,10 @@ nnpx_status nnpx_set_structures(nnpx_ctx* c, int ns, const int64_t* off, const d
             if (off[s + 1] < off[s]) fail(NNPX_ERR_ARGUMENT, "atom_offsets must be non-decreasing");
         const int64_t N = off[ns];
         if (N <= 0 || N > (int64_t)1 << 30) fail(NNPX_ERR_ARGUMENT, "bad atom count");
+        // a new structure set ends any MD set-up (skin, graph)
+        c->skin = 0.0;
+        c->md_ready = false;
+        drop_graph(c);
         for (int64_t i = 0; i < N; ++i)
             if (elem[i] < 0 || elem[i] >= c->hm.t.nelem) fail(NNPX_ERR_RANGE, "element index out of range");
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -688,7 +728,7 @@ nnpx_status nnpx_set_structures(nnpx_ctx* c, int ns, const int64_t* off, const d
         c->st_host.assign(ns, StructT{});
         for (int s = 0; s < ns; ++s) {
             if (periodic[s] && !box) fail(NNPX_ERR_ARGUMENT, "periodic structure without box");
-            setup_struct(c->st_host[s], periodic[s] ? box + 9 * s : nullptr, periodic[s] != 0, c->hm.t.maxCutoff);
+            setup_struct(c->st_host[s], periodic[s] ? box + 9 * s : nullptr, periodic[s] != 0, list_cutoff(c));
             c->st_host[s].atom0 = (int)off[s];
             c->st_host[s].natoms = (int)(off[s + 1] - off[s]);
         }
@@ -941,6 +981,199 @@ nnpx_status nnpx_get_dGdr(nnpx_ctx* c, double* self, int ld_self, double* nbr, i
                 }
             }
         }
+    });
+}
+
+namespace {
+
+// One velocity-Verlet step, enqueued without host synchronisation: half kick
+// and drift, displacement check, gated wrap and list rebuild, forces, second
+// half kick with the kinetic energy, step bookkeeping.
+void md_step(nnpx_ctx* c)
+{
+    double* pos = existing<double>(c, "pos");
+    double* vel = existing<double>(c, "md_vel");
+    double* ref = existing<double>(c, "md_ref");
+    int* gate = existing<int>(c, "md_gate");
+    int* counters = existing<int>(c, "md_counters");
+    const double* F = existing<double>(c, "F");
+    const int* elem = existing<int>(c, "elem");
+    const double half = 0.5 * c->skin;
+    md_kick_drift(pos, vel, F, elem, c->mdc, c->N, c->stream);
+    md_check(pos, ref, c->N, half * half, gate, c->stream);
+    md_wrap(pos, ref, existing<int>(c, "sid"), existing<StructT>(c, "st"), c->N, gate, c->stream);
+    SysView s = sysview(c);
+    s.gate = gate;
+    launch_neighbors(s, list_cutoff(c), c->any_cell, c->stream, false);
+    if (c->md_precision == NNPX_MIXED) {
+        MixedBuffers mb = mixed_buffers(c);
+        run_mixed(c->mixed, c->hm.t, c->d_model, sysview(c), mb, true, c->force_mode == NNPX_FORCES_PUSH, c->stream,
+                  [&](const char* name, const std::function<void()>& f) { timed(c, name, f); });
+    } else {
+        launch_reverse(s, c->stream);
+        c->rev_valid = true;
+        compute_sf64(c, true);
+        compute_nn64(c, true);
+        compute_energy(c);
+        compute_force64(c);
+    }
+    md_kick(vel, existing<double>(c, "F"), elem, c->mdc, c->N, counters, existing<double>(c, "md_ekin"), c->stream);
+    md_step_end(gate, counters, existing<double>(c, "E"), existing<double>(c, "md_epot"),
+                existing<double>(c, "md_ekin"), c->stream);
+}
+
+void need_md(nnpx_ctx* c)
+{
+    need_structures(c);
+    if (!c->md_ready) fail(NNPX_ERR_STATE, "MD not initialised (nnpx_md_init)");
+}
+
+}  // namespace
+
+nnpx_status nnpx_md_init(nnpx_ctx* c, int precision, const double* masses, const double* vel, double dt,
+                         double skin)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (c->S != 1) fail(NNPX_ERR_UNSUPPORTED, "MD runs one structure per context");
+        if (precision != NNPX_PLAIN_FP64 && precision != NNPX_MIXED) fail(NNPX_ERR_ARGUMENT, "bad precision");
+        if (!masses || !vel) fail(NNPX_ERR_ARGUMENT, "null masses or velocities");
+        if (!(dt > 0.0) || !(skin >= 0.0)) fail(NNPX_ERR_ARGUMENT, "dt must be > 0 and skin >= 0");
+        const int ne = c->hm.t.nelem;
+        for (int e = 0; e < ne; ++e)
+            if (!(masses[e] > 0.0)) fail(NNPX_ERR_ARGUMENT, "masses must be > 0");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        drop_graph(c);
+        c->md_ready = false;
+        // Skin-extended list: image ranges and cell grid for rc + skin.
+        const double cl = c->hm.normalize ? c->hm.convLength : 1.0;
+        const double ce = c->hm.normalize ? c->hm.convEnergy : 1.0;
+        c->skin = skin * cl;
+        std::vector<double> pos((size_t)3 * c->N);
+        copy_out(c, pos.data(), existing<double>(c, "pos"), sizeof(double) * pos.size());
+        StructT& t = c->st_host[0];
+        const int a0 = t.atom0, na = t.natoms;
+        double box[9];
+        std::memcpy(box, t.box, sizeof box);
+        setup_struct(t, box, t.periodic != 0, list_cutoff(c));
+        t.atom0 = a0;
+        t.natoms = na;
+        choose_algorithms(c, pos.data());
+        hip_check(hipMemcpy(existing<StructT>(c, "st"), c->st_host.data(), sizeof(StructT), hipMemcpyHostToDevice),
+                  "hipMemcpy");
+        c->cap = estimate_cap(c);
+        invalidate(c);
+        build_neighbors(c, true);
+        // headroom for density fluctuations between rebuilds
+        c->cap = std::max(32, ((c->max_nb + c->max_nb / 4 + 32 + 31) / 32) * 32);
+        build_neighbors(c, true);
+        // Velocity-Verlet constants: a = F/m with F in eV/A, m in amu, t in
+        // fs gives 9.64853321e-3 A/fs^2 per eV/(A amu); converted to the
+        // model's length and energy units.
+        const double kappa = 9.64853321e-3;
+        for (int e = 0; e < ne; ++e) {
+            c->mdc.hk[e] = 0.5 * dt * kappa * cl * cl / (ce * masses[e]);
+            c->mdc.half_m[e] = 0.5 * masses[e] * ce / (kappa * cl * cl);
+        }
+        c->mdc.dt = dt;
+        std::vector<double> v((size_t)3 * c->N);
+        for (size_t q = 0; q < v.size(); ++q) v[q] = vel[q] * cl;
+        double* dv = buf<double>(c, "md_vel", v.size());
+        hip_check(hipMemcpy(dv, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        double* ref = buf<double>(c, "md_ref", (size_t)3 * c->N);
+        hip_check(hipMemcpy(ref, existing<double>(c, "pos"), sizeof(double) * 3 * c->N, hipMemcpyDeviceToDevice),
+                  "hipMemcpy");
+        int* gate = buf<int>(c, "md_gate", 16);
+        int* counters = buf<int>(c, "md_counters", 16);
+        buf<double>(c, "md_epot", kMDHistory);
+        double* ekin = buf<double>(c, "md_ekin", kMDHistory);
+        c->md_precision = precision;
+        // forces of the initial positions (allocates every buffer a step uses)
+        evaluate(c, precision, NNPX_EVAL_FORCES, true);
+        settle(c);
+        md_reset(gate, counters, ekin, c->stream);
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        c->md_ready = true;
+    });
+}
+
+nnpx_status nnpx_md_run(nnpx_ctx* c, int nsteps, int graph_steps)
+{
+    return guard(c, [&] {
+        need_md(c);
+        if (nsteps < 0 || graph_steps < 0) fail(NNPX_ERR_ARGUMENT, "negative step count");
+        const bool timing = c->timing;
+        if (graph_steps > 0 && (!c->gexec || c->graph_steps != graph_steps)) {
+            drop_graph(c);
+            c->timing = false;  // no event records inside the captured sequence
+            hip_check(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                for (int k = 0; k < graph_steps; ++k) md_step(c);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(c->stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                c->timing = timing;
+                throw;
+            }
+            c->timing = timing;
+            hip_check(hipStreamEndCapture(c->stream, &c->graph), "hipStreamEndCapture");
+            hip_check(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+            c->graph_steps = graph_steps;
+        }
+        c->timing = false;
+        hipEvent_t a = nullptr, b = nullptr;
+        if (timing) {
+            a = get_event(c);
+            b = get_event(c);
+            hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+        }
+        int done = 0;
+        if (graph_steps > 0)
+            for (; done + graph_steps <= nsteps; done += graph_steps)
+                hip_check(hipGraphLaunch(c->gexec, c->stream), "hipGraphLaunch");
+        for (; done < nsteps; ++done) md_step(c);
+        if (timing) {
+            hip_check(hipEventRecord(b, c->stream), "hipEventRecord");
+            c->pending.push_back(Timer{"md", a, b});
+        }
+        c->timing = timing;
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        int f[2];
+        read_flags(c, f);
+        if (f[0] & 1) fail(NNPX_ERR_RANGE, "neighbour capacity exceeded during MD (raise the skin or re-initialise)");
+        c->sf_valid = c->nn_valid = c->dedg_valid = c->e_valid = c->f_valid = true;
+        c->sfd_valid = c->md_precision == NNPX_PLAIN_FP64;
+        c->last_precision = c->md_precision;
+    });
+}
+
+nnpx_status nnpx_md_state(nnpx_ctx* c, double* pos, double* vel)
+{
+    return guard(c, [&] {
+        need_md(c);
+        if (pos) copy_out(c, pos, existing<double>(c, "pos"), sizeof(double) * 3 * c->N);
+        if (vel) {
+            copy_out(c, vel, existing<double>(c, "md_vel"), sizeof(double) * 3 * c->N);
+            const double cl = c->hm.normalize ? c->hm.convLength : 1.0;
+            for (int64_t q = 0; q < 3 * (int64_t)c->N; ++q) vel[q] /= cl;
+        }
+    });
+}
+
+nnpx_status nnpx_md_energies(nnpx_ctx* c, double* epot, double* ekin, int64_t max, int64_t* nsteps,
+                             int64_t* rebuilds)
+{
+    return guard(c, [&] {
+        need_md(c);
+        int cnt[2];
+        copy_out(c, cnt, existing<int>(c, "md_counters"), sizeof cnt);
+        if (nsteps) *nsteps = cnt[0];
+        if (rebuilds) *rebuilds = cnt[1];
+        const int64_t n = std::min<int64_t>(std::min<int64_t>(cnt[0], kMDHistory), std::max<int64_t>(max, 0));
+        if (n > 0 && cnt[0] > kMDHistory) fail(NNPX_ERR_RANGE, "more steps than the kept energy history");
+        if (epot && n > 0) copy_out(c, epot, existing<double>(c, "md_epot"), sizeof(double) * n);
+        if (ekin && n > 0) copy_out(c, ekin, existing<double>(c, "md_ekin"), sizeof(double) * n);
     });
 }
 

@@ -28,6 +28,7 @@ SEED_FCC = 20240603
 SEED_WATERSET = 20240604
 SEED_WEIGHTS = 20240610
 SEED_TEACHER = 20240611
+SEED_VELOCITY = 20240612
 
 ETA_RAD = np.geomspace(0.001, 0.5, 10)
 
@@ -171,6 +172,20 @@ def fcc_sf_lines(rc=6.0, symbols=FCC_SYMBOLS):
 
 
 ATOMIC_NUMBER = {"H": 1, "O": 8, "Al": 13, "Ni": 28, "Cu": 29, "Pd": 46}
+# standard atomic weights (amu) used as MD masses
+MASS = {"H": 1.008, "O": 15.999, "Al": 26.9815, "Ni": 58.6934, "Cu": 63.546, "Pd": 106.42}
+KB_EV = 8.617333262e-5          # Boltzmann constant, eV/K
+ACCEL = 9.64853321e-3           # (eV/A)/amu in A/fs^2
+
+
+def maxwell_velocities(masses, temperature=300.0, seed=SEED_VELOCITY):
+    """Seeded Maxwell-Boltzmann velocities (A/fs) for per-atom masses (amu),
+    net momentum removed."""
+    rng = np.random.default_rng(seed)
+    m = np.asarray(masses, dtype=np.float64)
+    v = rng.normal(size=(len(m), 3)) * np.sqrt(KB_EV * temperature * ACCEL / m)[:, None]
+    v -= (m[:, None] * v).sum(0) / m.sum()
+    return v
 
 
 def _parse(line, symbols):
