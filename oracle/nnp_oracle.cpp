@@ -1093,6 +1093,203 @@ void set_box(oracle_ctx& c, const double* box)
 // ==================================================================== C API
 extern "C" {
 
+// ----------------------------------------------------------------------
+// Training gradients (config "force-training gradient"): dE/dc and dF/dc of
+// the connections c, restated from NeuralNetwork::calculateDEdc
+// (NeuralNetwork.cpp:444-488), calculateDFdc (490-527), calculateDEdb
+// (593-626), calculateDxdG (628-657), calculateD2EdGdc (659-719) and the
+// activation derivatives of propagateLayer (761-896); the per-component
+// Jacobian dG_i/dx_{a,c} is gathered as Training::collectDGdxia does
+// (Training.cpp:3586-3625) and contracted with the force residual field.
+namespace {
+
+struct NNState {
+    vector<vector<double>> val, dfdx, d2fdx2, dxdG;
+};
+
+// Value and first two derivatives of an activation (propagateLayer).
+double activate2(int a, double x, double& d1, double& d2)
+{
+    const double EXP_LIMIT = 35.0;
+    double v;
+    switch (a) {
+    case AF_IDENTITY: d1 = 1.0; d2 = 0.0; return x;
+    case AF_TANH: v = tanh(x); d1 = 1.0 - v * v; d2 = -2.0 * v * (1.0 - v * v); return v;
+    case AF_LOGISTIC:
+        if (x > EXP_LIMIT) { d1 = 0.0; d2 = 0.0; return 1.0; }
+        if (x < -EXP_LIMIT) { d1 = 0.0; d2 = 0.0; return 0.0; }
+        v = 1.0 / (1.0 + exp(-x)); d1 = v * (1.0 - v); d2 = v * (1.0 - v) * (1.0 - 2.0 * v); return v;
+    case AF_SOFTPLUS:
+        if (x > EXP_LIMIT) { d1 = 1.0; d2 = 0.0; return x; }
+        if (x < -EXP_LIMIT) { d1 = 0.0; d2 = 0.0; return 0.0; }
+        {
+            const double t = exp(x);
+            v = log(1.0 + t);
+            const double s = 1.0 / (1.0 + 1.0 / t);
+            d1 = s; d2 = s * (1.0 - s);
+            return v;
+        }
+    case AF_RELU:
+        d2 = 0.0;
+        if (x > 0.0) { d1 = 1.0; return x; }
+        d1 = 0.0; return 0.0;
+    case AF_GAUSSIAN: { const double t = exp(-0.5 * x * x); d1 = -x * t; d2 = (x * x - 1.0) * t; return t; }
+    case AF_COS: { const double t = cos(x); d1 = -sin(x); d2 = -t; return t; }
+    case AF_REVLOGISTIC:
+        v = 1.0 / (1.0 + exp(-x)); d1 = v * (v - 1.0); d2 = v * (v - 1.0) * (1.0 - 2.0 * v); return 1.0 - v;
+    case AF_EXP: v = exp(-x); d1 = -v; d2 = v; return v;
+    case AF_HARMONIC: d1 = 2.0 * x; d2 = 2.0; return x * x;
+    }
+    throw std::runtime_error("activation without second derivative");
+}
+
+struct ConnOffsets {
+    vector<size_t> w, b;   // connection offsets of weights / biases of layer l
+    vector<size_t> bo;     // offsets of layer l in the bias-only array
+    size_t n = 0, nb = 0;
+};
+
+ConnOffsets conn_offsets(const NN& nn)
+{
+    ConnOffsets o;
+    const size_t L = nn.n.size();
+    o.w.assign(L, 0); o.b.assign(L, 0); o.bo.assign(L, 0);
+    for (size_t l = 1; l < L; ++l) {
+        o.w[l] = o.n;
+        o.n += (size_t)nn.n[l - 1] * nn.n[l];
+        o.b[l] = o.n;
+        o.n += nn.n[l];
+        o.bo[l] = o.nb;
+        o.nb += nn.n[l];
+    }
+    return o;
+}
+
+void nn_forward2(const NN& nn, const vector<double>& G, NNState& st)
+{
+    const int L = (int)nn.n.size();
+    st.val.assign(L, {}); st.dfdx.assign(L, {}); st.d2fdx2.assign(L, {}); st.dxdG.assign(L, {});
+    st.val[0] = G;
+    for (int l = 1; l < L; ++l) {
+        st.val[l].assign(nn.n[l], 0.0);
+        st.dfdx[l].assign(nn.n[l], 0.0);
+        st.d2fdx2[l].assign(nn.n[l], 0.0);
+        st.dxdG[l].assign(nn.n[l], 0.0);
+        for (int k = 0; k < nn.n[l]; ++k) {
+            double t = 0.0;
+            const double* w = &nn.W[l][(size_t)k * nn.n[l - 1]];
+            for (int j = 0; j < nn.n[l - 1]; ++j) t += w[j] * st.val[l - 1][j];
+            t += nn.b[l][k];
+            if (nn.normalize) t /= nn.n[l - 1];
+            st.val[l][k] = activate2(nn.act[l], t, st.dfdx[l][k], st.d2fdx2[l][k]);
+        }
+    }
+}
+
+// NeuralNetwork::calculateDEdc: dE/dc in connection order.
+void nn_dEdc(const NN& nn, const NNState& st, const ConnOffsets& o, double* d)
+{
+    const int L = (int)nn.n.size();
+    for (size_t q = 0; q < o.n; ++q) d[q] = 0.0;
+    for (int i = 0; i < nn.n[L - 1]; ++i) {
+        d[o.b[L - 1] + i] = st.dfdx[L - 1][i];
+        if (nn.normalize) d[o.b[L - 1] + i] /= nn.n[L - 2];
+    }
+    for (int i = L - 2; i >= 0; --i) {
+        for (int j = 0; j < nn.n[i]; ++j) {
+            for (int k = 0; k < nn.n[i + 1]; ++k) {
+                d[o.w[i + 1] + (size_t)j * nn.n[i + 1] + k] = d[o.b[i + 1] + k] * st.val[i][j];
+                if (i >= 1)
+                    d[o.b[i] + j] += d[o.b[i + 1] + k] * nn.W[i + 1][(size_t)k * nn.n[i] + j] * st.dfdx[i][j];
+            }
+            if (nn.normalize && i >= 1) d[o.b[i] + j] /= nn.n[i - 1];
+        }
+    }
+}
+
+// NeuralNetwork::calculateDEdb: dE/db in bias-only order.
+void nn_dEdb(const NN& nn, const NNState& st, const ConnOffsets& o, vector<double>& d)
+{
+    const int L = (int)nn.n.size();
+    d.assign(o.nb, 0.0);
+    for (int i = 0; i < nn.n[L - 1]; ++i) {
+        d[o.bo[L - 1] + i] = st.dfdx[L - 1][i];
+        if (nn.normalize) d[o.bo[L - 1] + i] /= nn.n[L - 2];
+    }
+    for (int i = L - 2; i >= 0; --i) {
+        for (int j = 0; j < nn.n[i]; ++j) {
+            for (int k = 0; k < nn.n[i + 1]; ++k)
+                if (i >= 1)
+                    d[o.bo[i] + j] += d[o.bo[i + 1] + k] * nn.W[i + 1][(size_t)k * nn.n[i] + j] * st.dfdx[i][j];
+            if (nn.normalize && i >= 1) d[o.bo[i] + j] /= nn.n[i - 1];
+        }
+    }
+}
+
+// NeuralNetwork::calculateDxdG for input `index`.
+void nn_dxdG(const NN& nn, NNState& st, int index)
+{
+    const int L = (int)nn.n.size();
+    for (int i = 0; i < nn.n[1]; ++i) {
+        st.dxdG[1][i] = nn.W[1][(size_t)i * nn.n[0] + index];
+        if (nn.normalize) st.dxdG[1][i] /= nn.n[0];
+    }
+    for (int l = 2; l < L; ++l)
+        for (int j = 0; j < nn.n[l]; ++j) {
+            double t = 0.0;
+            for (int k = 0; k < nn.n[l - 1]; ++k)
+                t += nn.W[l][(size_t)j * nn.n[l - 1] + k] * st.dfdx[l - 1][k] * st.dxdG[l - 1][k];
+            if (nn.normalize) t /= nn.n[l - 1];
+            st.dxdG[l][j] = t;
+        }
+}
+
+// NeuralNetwork::calculateD2EdGdc for input `index`.
+void nn_d2EdGdc(const NN& nn, const NNState& st, const ConnOffsets& o, int index, const vector<double>& dEdb,
+                double* d2)
+{
+    const int L = (int)nn.n.size();
+    for (int i = 0; i < nn.n[L - 1]; ++i) {
+        d2[o.b[L - 1] + i] = st.d2fdx2[L - 1][i] * st.dxdG[L - 1][i];
+        if (nn.normalize) d2[o.b[L - 1] + i] /= nn.n[L - 2];
+    }
+    for (int i = L - 2; i >= 0; --i) {
+        for (int j = 0; j < nn.n[i]; ++j) {
+            for (int k = 0; k < nn.n[i + 1]; ++k) {
+                double& w = d2[o.w[i + 1] + (size_t)j * nn.n[i + 1] + k];
+                if (i == 0) {
+                    w = d2[o.b[i + 1] + k] * st.val[i][j];
+                    if (j == index) w += dEdb[o.bo[i + 1] + k];
+                } else {
+                    w = d2[o.b[i + 1] + k] * st.val[i][j] + dEdb[o.bo[i + 1] + k] * st.dfdx[i][j] * st.dxdG[i][j];
+                }
+                if (i >= 1)
+                    d2[o.b[i] + j] += nn.W[i + 1][(size_t)k * nn.n[i] + j] *
+                                      (d2[o.b[i + 1] + k] * st.dfdx[i][j] +
+                                       dEdb[o.bo[i + 1] + k] * st.d2fdx2[i][j] * st.dxdG[i][j]);
+            }
+            if (nn.normalize && i >= 1) d2[o.b[i] + j] /= nn.n[i - 1];
+        }
+    }
+}
+
+// NeuralNetwork::calculateDFdc: dF/dc = -sum_i d2E/dG_i dc * dGdxyz_i.
+void nn_dFdc(const NN& nn, NNState& st, const ConnOffsets& o, const double* dGdxyz, double* dFdc)
+{
+    vector<double> dEdb;
+    nn_dEdb(nn, st, o, dEdb);
+    vector<double> d2(o.n);
+    for (size_t q = 0; q < o.n; ++q) dFdc[q] = 0.0;
+    for (int i = 0; i < nn.n[0]; ++i) {
+        std::fill(d2.begin(), d2.end(), 0.0);
+        nn_dxdG(nn, st, i);
+        nn_d2EdGdc(nn, st, o, i, dEdb, d2.data());
+        for (size_t q = 0; q < o.n; ++q) dFdc[q] -= d2[q] * dGdxyz[i];
+    }
+}
+
+}  // namespace
+
 const char* oracle_last_error(void) { return g_err.c_str(); }
 
 #define ORACLE_TRY(...)                                   \
@@ -1300,3 +1497,95 @@ int oracle_cutoff_fdf(int type, double rc, double alpha, double r, double* fc, d
 }
 
 }  // extern "C"
+
+int oracle_num_connections(const oracle_ctx* c, int e)
+{
+    if (e < 0 || e >= (int)c->el.size()) return -1;
+    return (int)c->el[e].nn.numConnections();
+}
+
+int oracle_get_weights(const oracle_ctx* c, double* w)
+{
+    ORACLE_TRY({
+        size_t p = 0;
+        for (auto const& e : c->el) {
+            const NN& nn = e.nn;
+            for (size_t l = 1; l < nn.n.size(); ++l) {
+                for (int j = 0; j < nn.n[l - 1]; ++j)
+                    for (int k = 0; k < nn.n[l]; ++k) w[p++] = nn.W[l][(size_t)k * nn.n[l - 1] + j];
+                for (int k = 0; k < nn.n[l]; ++k) w[p++] = nn.b[l][k];
+            }
+        }
+    });
+    return 0;
+}
+
+int oracle_set_weights(oracle_ctx* c, const double* w)
+{
+    ORACLE_TRY({
+        size_t p = 0;
+        for (auto& e : c->el) {
+            const size_t n = e.nn.numConnections();
+            e.nn.setConnections(vector<double>(w + p, w + p + n));
+            p += n;
+        }
+    });
+    return 0;
+}
+
+int oracle_loss_gradient(oracle_ctx* c, double E_ref, const double* F_ref, double wE, double wF, double* loss,
+                         double* grad)
+{
+    ORACLE_TRY({
+        if (c->normalize) throw std::runtime_error("loss gradient: normalised models are not supported");
+        const size_t N = c->atoms.size();
+        for (auto const& a : c->atoms)
+            if (a.dEdG.empty() || a.dGdr.empty()) throw std::runtime_error("loss gradient: run oracle_predict first");
+        vector<size_t> off(c->el.size() + 1, 0);
+        for (size_t e = 0; e < c->el.size(); ++e) off[e + 1] = off[e] + c->el[e].nn.numConnections();
+        for (size_t q = 0; q < off.back(); ++q) grad[q] = 0.0;
+        const double r = c->energy - E_ref;
+        double lf = 0.0;
+        vector<double> u(3 * N);
+        for (size_t a = 0; a < N; ++a)
+            for (int k = 0; k < 3; ++k) {
+                const double d = c->atoms[a].f[k] - F_ref[3 * a + k];
+                lf += d * d;
+                u[3 * a + k] = 2.0 * wF * d;
+            }
+        loss[1] = wE * r * r;
+        loss[2] = wF * lf;
+        loss[0] = loss[1] + loss[2];
+        // Training::collectDGdxia contracted with u: every neighbour entry of
+        // atom i (self images included) and atom i's own derivative.
+        vector<vector<double>> gdot(N);
+        for (size_t i = 0; i < N; ++i) {
+            const Atom& a = c->atoms[i];
+            const size_t ns = a.G.size();
+            vector<double>& g = gdot[i];
+            g.assign(ns, 0.0);
+            for (const Neighbor& n : a.nbr) {
+                const vector<int>& t = c->el[a.element].table[n.element];
+                for (size_t q = 0; q < t.size(); ++q)
+                    for (int k = 0; k < 3; ++k) g[t[q]] += n.dGdr[3 * q + k] * u[3 * n.j + k];
+            }
+            for (size_t q = 0; q < ns; ++q)
+                for (int k = 0; k < 3; ++k) g[q] += a.dGdr[3 * q + k] * u[3 * i + k];
+        }
+        // Training::dPdc (Training.cpp:3319-3389) weighted by the loss adjoints.
+        const double cE = 2.0 * wE * r;
+        for (size_t i = 0; i < N; ++i) {
+            const Atom& a = c->atoms[i];
+            const NN& nn = c->el[a.element].nn;
+            const ConnOffsets o = conn_offsets(nn);
+            NNState st;
+            nn_forward2(nn, a.G, st);
+            vector<double> dEdc(o.n), dFdc(o.n);
+            nn_dEdc(nn, st, o, dEdc.data());
+            nn_dFdc(nn, st, o, gdot[i].data(), dFdc.data());
+            double* g = grad + off[a.element];
+            for (size_t q = 0; q < o.n; ++q) g[q] += cE * dEdc[q] + dFdc[q];
+        }
+    });
+    return 0;
+}

@@ -95,6 +95,20 @@ long   oracle_build_neighbors(oracle_ctx* c, int nbr_mode, int nthreads);
 int    oracle_cutoff_fdf(int type, double rc, double alpha, double r,
                          double* fc, double* dfc);
 
+/* ---- training gradient (restated from NeuralNetwork::calculateDEdc /
+ * calculateDFdc, Training::dPdc and Training::collectDGdxia) ---- */
+/* Connections of element e's network (setConnections order). */
+int oracle_num_connections(const oracle_ctx* c, int e);
+/* All networks' connections, elements in order (weights files' order). */
+int oracle_get_weights(const oracle_ctx* c, double* w);
+int oracle_set_weights(oracle_ctx* c, const double* w);
+/* After oracle_predict (forces, pull form) of a non-normalised model:
+ * loss = wE (E - E_ref)^2 + wF sum_{a,c} (F_ac - F_ref,ac)^2 as
+ * loss[0] total, loss[1] energy part, loss[2] force part, and its gradient
+ * with respect to all connections (layout of oracle_get_weights). */
+int oracle_loss_gradient(oracle_ctx* c, double E_ref, const double* F_ref, double wE, double wF,
+                         double* loss, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,35 @@ class Oracle:
     def phase_times(self):
         return [self.lib().oracle_phase_time(self.h, p) for p in range(5)]
 
+    # training gradient (NeuralNetwork::calculateDEdc / calculateDFdc restated)
+    def num_connections(self):
+        return sum(self.lib().oracle_num_connections(self.h, e) for e in range(self.ne))
+
+    def weights(self):
+        w = np.zeros(self.num_connections())
+        self._ck(self.lib().oracle_get_weights(self.h, _vp(w)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        self._ck(self.lib().oracle_set_weights(self.h, _vp(w)))
+
+    def loss_gradient(self, E_ref, F_ref, wE, wF):
+        """After predict(PULL): (loss[3], gradient) of
+        wE (E - E_ref)^2 + wF sum (F - F_ref)^2."""
+        L = self.lib()
+        L.oracle_loss_gradient.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        loss = np.zeros(3)
+        g = np.zeros(self.num_connections())
+        F = np.ascontiguousarray(F_ref, dtype=np.float64)
+        self._ck(L.oracle_loss_gradient(self.h, float(E_ref), _vp(F), float(wE), float(wF), _vp(loss), _vp(g)))
+        return loss, g
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise RuntimeError(self.lib().oracle_last_error().decode())
+
 
 def cutoff_fdf(ctype, rc, alpha, r):
     L = Oracle.lib()

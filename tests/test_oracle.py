@@ -213,3 +213,50 @@ def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path
     assert (o.forces() == r.forces()).all()
     assert (o.G() == r.G(o.max_sf)).all()
     assert (o.dEdG() == r.dEdG(o.max_sf)).all()
+
+
+@pytest.mark.parametrize("part", ["energy", "force"])
+def test_loss_gradient_matches_finite_differences(tmp_path, part):
+    """Training gradient of wE (E - E_ref)^2 + wF sum (F - F_ref)^2 (the
+    reference's calculateDEdc / calculateDFdc / collectDGdxia, restated)
+    against central differences with h = 1e-5 on 48 connections spread over
+    both networks and all layers, for the energy and the force term
+    separately; 81-atom water box (no self images), targets from a second
+    (teacher) network."""
+    from n2p2_amd import synth
+    d = synth.water_model(str(tmp_path / "student"))
+    t = synth.water_model(str(tmp_path / "teacher"), seed=synth.SEED_TEACHER)
+    pos, elem, box = synth.water_box(3)
+    teach = Oracle(t)
+    teach.set_structure(pos, elem, box)
+    teach.predict(PULL, BRUTE)
+    E_ref, F_ref = teach.energy(), teach.forces()
+    n = len(elem)
+    wE, wF = (1.0 / n, 0.0) if part == "energy" else (0.0, 0.7 / (3 * n))
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE)
+    loss, g = o.loss_gradient(E_ref, F_ref, wE, wF)
+    assert loss[0] > 0
+    w0 = o.weights()
+    rng = np.random.default_rng(5)
+    idx = np.unique(np.concatenate([rng.choice(len(w0), 40, replace=False),
+                                    [0, 1, 1250, 1275, 1950, len(w0) // 2, len(w0) - 1, len(w0) - 26]]))
+
+    def L(w):
+        o.set_weights(w)
+        o.set_structure(pos, elem, box)
+        o.predict(PULL, BRUTE)
+        return o.loss_gradient(E_ref, F_ref, wE, wF)[0][0]
+
+    h = 1e-5
+    worst = 0.0
+    for k in idx:
+        wp, wm = w0.copy(), w0.copy()
+        wp[k] += h
+        wm[k] -= h
+        fd = (L(wp) - L(wm)) / (2 * h)
+        worst = max(worst, abs(fd - g[k]))
+    o.set_weights(w0)
+    scale = np.abs(g).max()
+    assert worst <= 1e-6 * scale, (worst, scale)
