@@ -162,6 +162,28 @@ nnpx_status nnpx_get_neighbors(nnpx_ctx* ctx, int64_t* total, int* counts,
 nnpx_status nnpx_get_dGdr(nnpx_ctx* ctx, double* self, int ld_self,
                           double* nbr, int ld_nbr);
 
+/* ---- training gradient ------------------------------------------------ */
+/* Connections (weights and biases) of element e's network, or of all
+ * networks for e = -1 (Training::getNumConnections). */
+nnpx_status nnpx_num_connections(const nnpx_ctx* ctx, int element, int* n);
+/* All connections, elements in order, each network in
+ * NeuralNetwork::setConnections order (NeuralNetwork.cpp:147-169) -- the
+ * order of the weights.%03d.data files. */
+nnpx_status nnpx_get_weights(const nnpx_ctx* ctx, double* w);
+nnpx_status nnpx_set_weights(nnpx_ctx* ctx, const double* w);
+/* Energy+force loss of the current structures and its gradient with respect
+ * to every connection (layout of nnpx_get_weights):
+ *   L = sum_s (E_s - E_ref[s])^2 / N_s + beta/(3N) sum_{a,c} (F_ac - F_ref)^2
+ * loss[0] = L, loss[1] = energy term, loss[2] = force term.  The force term
+ * is differentiated through the forces (second order) as Training::dPdc
+ * with NeuralNetwork::calculateDFdc and Training::collectDGdxia do
+ * (Training.cpp:3319-3389, 3586-3625).  NNPX_PLAIN_FP64: FP64 from the stored
+ * dG/dr; NNPX_MIXED: FP32 recomputed tangents and FP32 network adjoints,
+ * gradient summed in FP64.  Model units; normalize_nodes is not supported. */
+nnpx_status nnpx_loss_gradient(nnpx_ctx* ctx, int precision, const double* E_ref,
+                               const double* F_ref, double beta, double* loss,
+                               double* grad);
+
 /* ---- molecular dynamics ----------------------------------------------- */
 /* No counterpart in nnp::Mode: the velocity-Verlet loop a driver runs around
  * the energy/force call (LAMMPS fix nve with pair_style hdnnp,

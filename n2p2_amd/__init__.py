@@ -79,6 +79,11 @@ _SIGS = {
     "nnpx_md_init": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                      ctypes.c_double],
     "nnpx_md_run": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int],
+    "nnpx_num_connections": [ctypes.c_void_p, ctypes.c_int, _c_int_p],
+    "nnpx_get_weights": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_set_weights": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_loss_gradient": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                           ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_md_state": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_md_energies": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _c_i64_p, _c_i64_p],
 }
@@ -278,6 +283,35 @@ class Mode:
         v = ctypes.c_double()
         self._ck(self._lib.nnpx_mfma_selftest(self._h, ctypes.byref(v)))
         return v.value
+
+    # ------------------------------------------------------ training --
+    def num_connections(self, element=-1):
+        n = ctypes.c_int()
+        self._ck(self._lib.nnpx_num_connections(self._h, element, ctypes.byref(n)))
+        return n.value
+
+    def weights(self):
+        w = np.zeros(self.num_connections())
+        self._ck(self._lib.nnpx_get_weights(self._h, _ptr(w)))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if len(w) != self.num_connections():
+            raise NNPError(1, "wrong number of connections")
+        self._ck(self._lib.nnpx_set_weights(self._h, _ptr(w)))
+
+    def loss_gradient(self, E_ref, F_ref, beta, precision=PLAIN_FP64):
+        """(loss[3], gradient) of sum_s (E_s - E_ref)^2/N_s + beta/(3N) sum (F - F_ref)^2."""
+        E = np.ascontiguousarray(E_ref, dtype=np.float64).reshape(-1)
+        F = np.ascontiguousarray(F_ref, dtype=np.float64).reshape(-1)
+        if len(E) != self.n_struct or len(F) != 3 * self.n_atoms:
+            raise NNPError(1, "targets have the wrong size")
+        loss = np.zeros(3)
+        g = np.zeros(self.num_connections())
+        self._ck(self._lib.nnpx_loss_gradient(self._h, precision, _ptr(E), _ptr(F), float(beta), _ptr(loss),
+                                              _ptr(g)))
+        return loss, g
 
     # ------------------------------------------------------------ MD --
     def md_init(self, masses, velocities, dt, skin, precision=MIXED):

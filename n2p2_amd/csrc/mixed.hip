@@ -797,6 +797,179 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     }
 }
 
+
+// ------------------------------------------- tangent of G along a field --
+// dG_i/dx . u for every SF of every atom (the contraction of
+// Training::collectDGdxia with a displacement field u), recomputing the
+// radial and angular terms: per triplet the three geometric scalars
+// s1 = r_ij.(u_i - u_j), s2 = r_ik.(u_i - u_k), s3 = r_jk.(u_k - u_j) are
+// combined into three coefficients, and each member adds t_q times their
+// member-weighted sum.  FP32 compensated sums per lane, FP64 across lanes,
+// scaled like the SFs themselves (scaling factor and angular normalisation).
+template <int MAXM, bool FAST>
+__global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
+                                                  const GroupMixF* __restrict__ GM, SysView s,
+                                                  const double* __restrict__ u, const float* __restrict__ sffac,
+                                                  double* __restrict__ Gdot, int ldG, int capL)
+{
+    extern __shared__ float sm[];
+    const Stage S = carve(sm, capL);
+    float4* du = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL));
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = s.elem[i];
+    const ElementT& E = M->el[e];
+    const int ctype = M->cutoffType;
+    const float tanhPre = (float)M->tanhPre;
+    const int n = stage_atom(s, i, capL, S);
+    {
+        const double ux = u[3 * (size_t)i], uy = u[3 * (size_t)i + 1], uz = u[3 * (size_t)i + 2];
+        const size_t base = (size_t)i * s.cap;
+        for (int k = lane; k < n; k += kWave) {
+            const int j = s.nb_j[base + k];
+            const float dx = (float)(ux - u[3 * (size_t)j]);
+            const float dy = (float)(uy - u[3 * (size_t)j + 1]);
+            const float dz = (float)(uz - u[3 * (size_t)j + 2]);
+            const float4 g = S.g[k];
+            du[k] = make_float4(dx, dy, dz, g.x * dx + g.y * dy + g.z * dz);
+        }
+    }
+    double* Gi = Gdot + (size_t)i * ldG;
+    __syncthreads();
+
+    for (int gi = 0; gi < E.ngrp; ++gi) {
+        const GroupT& GT = M->grp[E.grp0 + gi];
+        const GroupF g = group_f(GT);
+        const int nmem = GT.nmem;
+        const MemberF* mf = MF + GT.mem0;
+        group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        if (GT.type == 2) {
+            for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
+                float acc[kRadChunk], cmp[kRadChunk];
+#pragma unroll
+                for (int q = 0; q < kRadChunk; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
+                for (int k = lane; k < n; k += kWave) {
+                    const float4 gk = S.g[k];
+                    if (S.el[k] != GT.e1 || !(gk.w < g.rc)) continue;
+                    const float4 ck = S.c[k];
+                    const float sk = du[k].w * ck.x;  // (r_ij / r).(u_i - u_j)
+#pragma unroll
+                    for (int q = 0; q < kRadChunk; ++q)
+                        if (q0 + q < nmem) {
+                            const float dr = gk.w - mf[q0 + q].rs;
+                            const float ex = __expf(-mf[q0 + q].eta * dr * dr);
+                            kahan(acc[q], cmp[q], (ck.z - mf[q0 + q].twoEta * dr * ck.y) * ex * sk);
+                        }
+                }
+#pragma unroll
+                for (int q = 0; q < kRadChunk; ++q)
+                    if (q0 + q < nmem) {
+                        const double v = wave_sum((double)acc[q] + (double)cmp[q]);
+                        if (lane == 0) {
+                            const int sf = mf[q0 + q].sf;
+                            Gi[sf] = v * (double)sffac[e * kMaxSF + sf];
+                        }
+                    }
+            }
+            __syncthreads();
+            continue;
+        }
+        float acc[MAXM], cmp[MAXM];
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
+        const bool same = GT.e1 == GT.e2;
+        const int nA = make_list(S, n, GT.e1, g.rc, S.la);
+        const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+        const GroupMixF gm = GM[E.grp0 + gi];
+        const MemberMasks R = load_members(S, mf, nmem, nullptr);
+        for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
+            const float4 gj = S.g[j], gk = S.g[k];
+            const float4 cj = S.c[j], ck = S.c[k];
+            const float4 uj = du[j], uk = du[k];
+            const float rjk = sqrtf(r2jk);
+            const float ijk = 1.f / rjk;
+            float fcjk, dfcjk;
+            cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
+            const float rinvijik = cj.x * ck.x;
+            const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * rinvijik;
+            const float pfc = cj.y * ck.y * fcjk;
+            const float r2sum = gj.w * gj.w + gk.w * gk.w + r2jk;
+            const float pr1 = ck.y * fcjk * cj.z * cj.x;
+            const float pr2 = cj.y * fcjk * ck.z * ck.x;
+            const float pr3 = cj.y * ck.y * dfcjk * ijk;
+            // geometric scalars of the field
+            const float s1 = uj.w, s2 = uk.w;
+            const float s3 = (gk.x - gj.x) * (uj.x - uk.x) + (gk.y - gj.y) * (uj.y - uk.y) +
+                             (gk.z - gj.z) * (uj.z - uk.z);
+            const float al = pfc * ((rinvijik - cost * cj.x * cj.x) * s1 + (rinvijik - cost * ck.x * ck.x) * s2 +
+                                    rinvijik * s3);
+            const float be = pr1 * s1 + pr2 * s2 - pr3 * s3;
+            const float ga = pfc * (s3 - s1 - s2);
+            constexpr int MAXS = FAST ? 2 : 4;
+            float L[MAXS], Rp[MAXS];
+#pragma unroll
+            for (int t = 0; t < MAXS; ++t) {
+                const float pl = 1.f + gm.lam[t] * cost;
+                const bool pos = pl > 0.f;
+                L[t] = pos ? flog2(pl) : -INFINITY;
+                Rp[t] = pos ? __builtin_amdgcn_rcpf(pl) : 0.f;
+            }
+            if constexpr (FAST) {
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q) {
+                    const float4 mq = S.mp[q];
+                    const bool s1b = mq.w != 0.f;
+                    const float t = fexp2(fmaf(mq.x, s1b ? L[1] : L[0], -mq.y * r2sum));
+                    const float zl = mq.x * gm.lam[s1b ? 1 : 0];
+                    const float c = fmaf(zl * (s1b ? Rp[1] : Rp[0]), al, be);
+                    kahan(acc[q], cmp[q], t * fmaf(2.f * mq.y * 0.69314718f, ga, c));
+                }
+            } else {
+                float ec = 0.f, fij = 1.f, fik = 1.f, fjk = 1.f;
+                bool shifted = false;
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q)
+                    if (q < nmem) {
+                        const float4 mq = S.mp[q];
+                        if ((R.calc >> q) & 1u) {
+                            float arg = r2sum;
+                            shifted = mq.z > 0.f;
+                            if (shifted) {
+                                const float a = gj.w - mq.z, b = gk.w - mq.z, c = rjk - mq.z;
+                                arg = a * a + b * b + c * c;
+                                fij = a * cj.x;
+                                fik = b * ck.x;
+                                fjk = c * ijk;
+                            }
+                            ec = -mq.y * arg;
+                        }
+                        const unsigned b1 = (R.s1 >> q) & 1u, b2 = (R.s2 >> q) & 1u;
+                        const int sidx = (int)b1 | ((int)b2 << 1);
+                        const float Ls = pickb<MAXS>(L, b1, b2);
+                        const float Rs = pickb<MAXS>(Rp, b1, b2);
+                        const float t = fexp2(fmaf(mq.x, Ls, ec));
+                        const float zl = mq.x * pick<MAXS>(gm.lam, sidx);
+                        const float twoEta = 2.f * mq.y * 0.69314718f;
+                        float v = fmaf(zl * Rs, al, be);
+                        if (shifted) v += twoEta * pfc * (fjk * s3 - fij * s1 - fik * s2);
+                        else v += twoEta * ga;
+                        kahan(acc[q], cmp[q], t * v);
+                    }
+            }
+        });
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q)
+            if (q < nmem) {
+                const double v = wave_sum((double)acc[q] + (double)cmp[q]);
+                if (lane == 0) {
+                    const int sf = mf[q].sf;
+                    Gi[sf] = v * (double)sffac[e * kMaxSF + sf];
+                }
+            }
+        __syncthreads();
+    }
+}
+
 __global__ void k_zero(double* p, size_t n)
 {
     for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
@@ -1031,6 +1204,20 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                                dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, mm.d_gmix, s, b.dEdGf,
                                b.ldG, b.F, ps, capL);
         });
+    });
+}
+
+void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
+                      const double* u, double* Gdot, int ldG, hipStream_t stream)
+{
+    if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
+    if (s.natoms == 0) return;
+    const int capL = s.cap;
+    const size_t lds = stage_bytes(capL) + sizeof(float4) * (size_t)capL;
+    if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
+    dispatch_members(mm.fastM, pow2_members(t), [&](auto m, auto fast) {
+        hipLaunchKernelGGL((k_tangent32<decltype(m)::value, decltype(fast)::value>), dim3(s.natoms), dim3(kWave), lds,
+                           stream, d_model, mm.d_mem, mm.d_gmix, s, u, mm.d_sffac, Gdot, ldG, capL);
     });
 }
 

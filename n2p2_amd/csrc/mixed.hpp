@@ -80,6 +80,10 @@ void setup_mixed(MixedModel& mm, const HostModel& hm);
 void free_mixed(MixedModel& mm);
 void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
                const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed);
+// Tangent of the scaled SFs along a displacement field u [3N]:
+// Gdot[i*ldG + q] = sum_a dG_iq/dx_a . u_a, recomputed in FP32.
+void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
+                      const double* u, double* Gdot, int ldG, hipStream_t stream);
 // Checks the MFMA lane layout assumed by the NN kernels on a known product;
 // returns the max abs error.
 double mfma_selftest();

@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 #include "md.hpp"
 #include "mixed.hpp"
+#include "train.hpp"
 #include "model.hpp"
 
 using namespace nnpx;
@@ -379,6 +380,35 @@ void build_neighbors(nnpx_ctx* c, bool check)
     c->nbr_valid = true;
 }
 
+// FP64 weights in model layout followed by a transposed copy ([in][out]),
+// and the BF16/FP32 tables of the mixed form.
+void upload_weights(nnpx_ctx* c)
+{
+    const ModelT& t = c->hm.t;
+    std::vector<double> w2(2 * (size_t)t.nweights, 0.0);
+    std::copy(c->hm.weights.begin(), c->hm.weights.end(), w2.begin());
+    for (int e = 0; e < t.nelem; ++e) {
+        const NNT& N = t.el[e].nn;
+        for (int l = 1; l < N.nl; ++l)
+            for (int k = 0; k < N.n[l]; ++k)
+                for (int j = 0; j < N.n[l - 1]; ++j)
+                    w2[t.nweights + N.woff[l] + (size_t)j * N.n[l] + k] =
+                        c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j];
+    }
+    if (!c->d_w) hip_check(hipMalloc(&c->d_w, sizeof(double) * w2.size()), "hipMalloc weights");
+    hip_check(hipMemcpy(c->d_w, w2.data(), sizeof(double) * w2.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    setup_mixed(c->mixed, c->hm);
+}
+
+// Connections per element in NeuralNetwork::setConnections order
+// (NeuralNetwork.cpp:147-169): per layer the weights j-major, then biases.
+int num_connections(const NNT& N)
+{
+    int n = 0;
+    for (int l = 1; l < N.nl; ++l) n += N.n[l - 1] * N.n[l] + N.n[l];
+    return n;
+}
+
 void compute_sf64(nnpx_ctx* c, bool deriv)
 {
     const ModelT& t = c->hm.t;
@@ -599,22 +629,17 @@ nnpx_status nnpx_load_model(nnpx_ctx* c, const char* dir)
         const ModelT& t = c->hm.t;
         if (!c->d_model) hip_check(hipMalloc(&c->d_model, sizeof(ModelT)), "hipMalloc model");
         hip_check(hipMemcpy(c->d_model, &t, sizeof(ModelT), hipMemcpyHostToDevice), "hipMemcpy model");
-        // weights in model layout followed by a transposed copy ([in][out])
-        std::vector<double> w2(2 * (size_t)t.nweights, 0.0);
-        std::copy(c->hm.weights.begin(), c->hm.weights.end(), w2.begin());
-        for (int e = 0; e < t.nelem; ++e) {
-            const NNT& N = t.el[e].nn;
-            for (int l = 1; l < N.nl; ++l)
-                for (int k = 0; k < N.n[l]; ++k)
-                    for (int j = 0; j < N.n[l - 1]; ++j)
-                        w2[t.nweights + N.woff[l] + (size_t)j * N.n[l] + k] =
-                            c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j];
-        }
         if (c->d_w) hip_check(hipFree(c->d_w), "hipFree");
         c->d_w = nullptr;
-        hip_check(hipMalloc(&c->d_w, sizeof(double) * w2.size()), "hipMalloc weights");
-        hip_check(hipMemcpy(c->d_w, w2.data(), sizeof(double) * w2.size(), hipMemcpyHostToDevice), "hipMemcpy");
-        setup_mixed(c->mixed, c->hm);
+        upload_weights(c);
+        // SF -> derivative slot per (centre element, neighbour element)
+        std::vector<int> islot((size_t)kMaxElem * kMaxElem * kMaxSF, -1);
+        for (int e = 0; e < t.nelem; ++e)
+            for (int n = 0; n < t.nelem; ++n)
+                for (int q = 0; q < t.el[e].nsfd[n]; ++q)
+                    islot[((size_t)e * kMaxElem + n) * kMaxSF + t.table[t.el[e].tab0[n] + q]] = q;
+        int* di = buf<int>(c, "islot", islot.size());
+        hip_check(hipMemcpy(di, islot.data(), sizeof(int) * islot.size(), hipMemcpyHostToDevice), "hipMemcpy");
         c->has_model = true;
         c->N = c->S = 0;
         c->nbr_valid = false;
@@ -981,6 +1006,139 @@ nnpx_status nnpx_get_dGdr(nnpx_ctx* c, double* self, int ld_self, double* nbr, i
                 }
             }
         }
+    });
+}
+
+nnpx_status nnpx_num_connections(const nnpx_ctx* cc, int element, int* n)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (!n || element < -1 || element >= c->hm.t.nelem) fail(NNPX_ERR_ARGUMENT, "bad element or null output");
+        int total = 0;
+        for (int e = 0; e < c->hm.t.nelem; ++e)
+            if (element < 0 || element == e) total += num_connections(c->hm.t.el[e].nn);
+        *n = total;
+    });
+}
+
+nnpx_status nnpx_get_weights(const nnpx_ctx* cc, double* w)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        need_model(c);
+        if (!w) fail(NNPX_ERR_ARGUMENT, "null output");
+        size_t p = 0;
+        for (int e = 0; e < c->hm.t.nelem; ++e) {
+            const NNT& N = c->hm.t.el[e].nn;
+            for (int l = 1; l < N.nl; ++l) {
+                for (int j = 0; j < N.n[l - 1]; ++j)
+                    for (int k = 0; k < N.n[l]; ++k) w[p++] = c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j];
+                for (int k = 0; k < N.n[l]; ++k) w[p++] = c->hm.weights[N.boff[l] + k];
+            }
+        }
+    });
+}
+
+nnpx_status nnpx_set_weights(nnpx_ctx* c, const double* w)
+{
+    return guard(c, [&] {
+        need_model(c);
+        if (!w) fail(NNPX_ERR_ARGUMENT, "null weights");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        size_t p = 0;
+        for (int e = 0; e < c->hm.t.nelem; ++e) {
+            const NNT& N = c->hm.t.el[e].nn;
+            for (int l = 1; l < N.nl; ++l) {
+                for (int j = 0; j < N.n[l - 1]; ++j)
+                    for (int k = 0; k < N.n[l]; ++k) c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j] = w[p++];
+                for (int k = 0; k < N.n[l]; ++k) c->hm.weights[N.boff[l] + k] = w[p++];
+            }
+        }
+        upload_weights(c);
+        drop_graph(c);
+        invalidate(c);
+    });
+}
+
+nnpx_status nnpx_loss_gradient(nnpx_ctx* c, int precision, const double* E_ref, const double* F_ref, double beta,
+                               double* loss, double* grad)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (precision != NNPX_PLAIN_FP64 && precision != NNPX_MIXED) fail(NNPX_ERR_ARGUMENT, "bad precision");
+        if (!E_ref || !F_ref || !loss || !grad) fail(NNPX_ERR_ARGUMENT, "null argument");
+        if (!(beta >= 0.0)) fail(NNPX_ERR_ARGUMENT, "beta must be >= 0");
+        const ModelT& t = c->hm.t;
+        for (int e = 0; e < t.nelem; ++e)
+            if (t.el[e].nn.normalize) fail(NNPX_ERR_UNSUPPORTED, "loss gradient with normalize_nodes");
+        evaluate(c, precision, NNPX_EVAL_FORCES, true);
+        settle(c);
+        const int N = c->N, S = c->S, ldG = t.max_sf;
+        double* Eref = buf<double>(c, "t_Eref", S);
+        double* Fref = buf<double>(c, "t_Fref", (size_t)3 * N);
+        hip_check(hipMemcpyAsync(Eref, E_ref, sizeof(double) * S, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(Fref, F_ref, sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream),
+                  "hipMemcpyAsync");
+        double* u = buf<double>(c, "t_u", (size_t)3 * N);
+        double* cE = buf<double>(c, "t_cE", S);
+        double* lossd = buf<double>(c, "t_loss", 4);
+        double* Gdot = buf<double>(c, "t_Gdot", (size_t)N * ldG);
+        int total = 0;
+        GradArgs a{};
+        for (int e = 0; e < t.nelem; ++e) {
+            a.coff[e] = total;
+            a.nconn[e] = num_connections(t.el[e].nn);
+            a.nconn_max = std::max(a.nconn_max, a.nconn[e]);
+            total += a.nconn[e];
+        }
+        double* gd = buf<double>(c, "t_grad", total);
+        hip_check(hipMemsetAsync(lossd, 0, 4 * sizeof(double), c->stream), "hipMemsetAsync");
+        hip_check(hipMemsetAsync(gd, 0, sizeof(double) * total, c->stream), "hipMemsetAsync");
+        hip_check(hipMemsetAsync(Gdot, 0, sizeof(double) * (size_t)N * ldG, c->stream), "hipMemsetAsync");
+        timed(c, "residual", [&] {
+            launch_residual(existing<double>(c, "F"), Fref, existing<double>(c, "E"), Eref, existing<int>(c, "atom_off"),
+                            N, S, beta / (3.0 * N), u, cE, lossd, c->stream);
+        });
+        SysView s = sysview(c);
+        if (precision == NNPX_PLAIN_FP64) {
+            timed(c, "tangent64", [&] {
+                launch_tangent64(c->d_model, s, existing<double>(c, "dGself"), existing<double>(c, "dGnbr"),
+                                 std::max(t.max_sfd, 1), existing<int>(c, "islot"), u, Gdot, ldG, c->stream);
+            });
+        } else {
+            timed(c, "tangent32", [&] { launch_tangent32(c->mixed, t, c->d_model, s, u, Gdot, ldG, c->stream); });
+        }
+        a.w = c->d_w;
+        a.nweights = t.nweights;
+        a.G = existing<double>(c, "G");
+        a.Gdot = Gdot;
+        a.ldG = ldG;
+        a.cE = cE;
+        a.sid = existing<int>(c, "sid");
+        a.atoms = existing<int>(c, "elem_atoms");
+        a.nelem = t.nelem;
+        a.maxw = std::max(t.max_width, t.max_sf);
+        a.grad = gd;
+        int blocks = 0;
+        for (int e = 0; e < t.nelem; ++e) {
+            a.blk_off[e] = blocks;
+            a.atom_off[e] = c->elem_off[e];
+            blocks += (c->elem_count[e] + kGradAtoms - 1) / kGradAtoms;
+        }
+        a.blk_off[t.nelem] = blocks;
+        a.atom_off[t.nelem] = c->elem_off[t.nelem];
+        if (grad_lds_bytes(a, precision == NNPX_MIXED) > 65536)
+            fail(NNPX_ERR_UNSUPPORTED, "network too large for the gradient kernel's LDS");
+        timed(c, precision == NNPX_MIXED ? "grad32" : "grad64",
+              [&] { launch_grad(c->d_model, a, blocks, precision == NNPX_MIXED, c->stream); });
+        double l[4];
+        copy_out(c, l, lossd, sizeof l);
+        loss[1] = l[1];
+        loss[2] = l[2];
+        loss[0] = l[1] + l[2];
+        copy_out(c, grad, gd, sizeof(double) * total);
+        collect_times(c);
     });
 }
 

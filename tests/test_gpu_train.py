@@ -1,0 +1,108 @@
+"""Energy+force training gradient on the device (config "force-training
+gradient step"): FP64 and mixed (FP32) forms against the FP64 oracle's
+restatement of Training::dPdc / NeuralNetwork::calculateDFdc, which is itself
+checked against central finite differences on the CPU."""
+import numpy as np
+import pytest
+
+import n2p2_amd
+from n2p2_amd import synth
+from oracle_lib import BRUTE, PULL, Oracle
+
+pytestmark = pytest.mark.gpu
+BETA = 0.7
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def models(tmp_path_factory):
+    d = tmp_path_factory.mktemp("train")
+    return synth.water_model(str(d / "student")), synth.water_model(str(d / "teacher"), seed=synth.SEED_TEACHER)
+
+
+def targets(teacher, pos, elem, boxes, off):
+    E, F = [], []
+    for s in range(len(off) - 1):
+        a, b = off[s], off[s + 1]
+        o = Oracle(teacher)
+        o.set_structure(pos[a:b], elem[a:b], boxes[s])
+        o.predict(PULL, BRUTE)
+        E.append(o.energy())
+        F.append(o.forces())
+    return np.array(E), np.vstack(F)
+
+
+def oracle_gradient(student, pos, elem, boxes, off, E_ref, F_ref):
+    N = off[-1]
+    loss, grad = np.zeros(3), None
+    for s in range(len(off) - 1):
+        a, b = off[s], off[s + 1]
+        o = Oracle(student)
+        o.set_structure(pos[a:b], elem[a:b], boxes[s])
+        o.predict(PULL, BRUTE)
+        l, g = o.loss_gradient(E_ref[s], F_ref[a:b], 1.0 / (b - a), BETA / (3.0 * N))
+        loss += l
+        grad = g if grad is None else grad + g
+    return loss, grad
+
+
+@pytest.mark.parametrize("precision", [n2p2_amd.PLAIN_FP64, n2p2_amd.MIXED])
+def test_loss_gradient_vs_oracle(models, precision):
+    say("case: loss gradient, 6 water structures (66-90 atoms), precision", precision)
+    student, teacher = models
+    pos, elem, boxes, off = synth.water_set(n_struct=6, mmin=22, mmax=30)
+    E_ref, F_ref = targets(teacher, pos, elem, boxes, off)
+    lo, go = oracle_gradient(student, pos, elem, boxes, off, E_ref, F_ref)
+    m = n2p2_amd.Mode(student)
+    m.set_structures(pos, elem, boxes, off)
+    loss, g = m.loss_gradient(E_ref, F_ref, BETA, precision)
+    rel = float(np.abs(g - go).max() / np.abs(go).max())
+    lrel = np.abs(loss - lo) / np.abs(lo)
+    say("  loss %s (oracle %s)  loss rel %s  grad max rel %.2e" % (loss, lo, lrel, rel))
+    if precision == n2p2_amd.PLAIN_FP64:
+        assert lrel.max() <= 1e-10
+        assert rel <= 1e-9
+    else:
+        assert lrel.max() <= 1e-4
+        assert rel <= 2e-3
+
+
+def test_force_term_gradient_vs_oracle(models):
+    """Force term alone (energy targets equal to the model's own energies),
+    where the second-order path carries the whole gradient."""
+    say("case: force-term gradient, FP64 and mixed")
+    student, teacher = models
+    pos, elem, boxes, off = synth.water_set(n_struct=4, mmin=22, mmax=26, seed=synth.SEED_WATERSET + 1)
+    _, F_ref = targets(teacher, pos, elem, boxes, off)
+    m = n2p2_amd.Mode(student)
+    m.set_structures(pos, elem, boxes, off)
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    E_own = m.energies()
+    lo, go = oracle_gradient(student, pos, elem, boxes, off, E_own, F_ref)
+    for precision, tol in ((n2p2_amd.PLAIN_FP64, 1e-9), (n2p2_amd.MIXED, 2e-3)):
+        loss, g = m.loss_gradient(E_own, F_ref, BETA, precision)
+        rel = float(np.abs(g - go).max() / np.abs(go).max())
+        say("  precision %d: force loss %.6e (oracle %.6e)  grad max rel %.2e" % (precision, loss[2], lo[2], rel))
+        assert rel <= tol
+
+
+def test_gradient_step_reduces_loss(models):
+    """A small step against the FP32 gradient lowers the FP64 loss."""
+    say("case: gradient step")
+    student, teacher = models
+    pos, elem, boxes, off = synth.water_set(n_struct=8, mmin=22, mmax=40, seed=synth.SEED_WATERSET + 2)
+    E_ref, F_ref = targets(teacher, pos, elem, boxes, off)
+    m = n2p2_amd.Mode(student)
+    m.set_structures(pos, elem, boxes, off)
+    l0, g = m.loss_gradient(E_ref, F_ref, BETA, n2p2_amd.MIXED)
+    w = m.weights()
+    m.set_weights(w - 1e-3 * g / np.abs(g).max())
+    l1, _ = m.loss_gradient(E_ref, F_ref, BETA, n2p2_amd.PLAIN_FP64)
+    say("  loss %.6e -> %.6e" % (l0[0], l1[0]))
+    assert l1[0] < l0[0]
+    m.set_weights(w)
+    l2, _ = m.loss_gradient(E_ref, F_ref, BETA, n2p2_amd.PLAIN_FP64)
+    assert abs(l2[0] - l0[0]) <= 1e-3 * l0[0]
