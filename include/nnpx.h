@@ -121,6 +121,22 @@ nnpx_status nnpx_set_structures(nnpx_ctx* ctx, int n_struct,
  * differences or MD driven from the host. */
 nnpx_status nnpx_update_positions(nnpx_ctx* ctx, const double* pos);
 
+/* Caller-provided neighbour lists, the LAMMPS path
+ * (InterfaceLammps::setLocalAtoms / addNeighbor, InterfaceLammps.cpp:
+ * 410-441): nlocal atoms (elem[nlocal]) are evaluated; their neighbour
+ * entries p in [nb_offsets[i], nb_offsets[i+1]) give the index of the
+ * neighbour in the caller's atom array (local or ghost, 0..ntotal-1), its
+ * element and the distance vector r_i - r_j (nb_dr[3p..3p+2], model length
+ * units).  Periodicity is carried by the ghosts.  Replaces the structures of
+ * the context; forces then use the push form and cover all ntotal atoms
+ * (nnpx_get_forces returns 3*ntotal values, ghosts included, as
+ * InterfaceLammps::getForces fills LAMMPS' local+ghost force array,
+ * InterfaceLammps.cpp:679-740).  No neighbour list is built on the device. */
+nnpx_status nnpx_set_neighbors_external(nnpx_ctx* ctx, int nlocal, const int* elem,
+                                        int ntotal, const int64_t* nb_offsets,
+                                        const int* nb_index, const int* nb_elem,
+                                        const double* nb_dr);
+
 /* ---- compute (the nnp::Mode boundary) --------------------------------- */
 /* Structure::calculateNeighborList(maxCutoffRadius, false)
  * (Structure.cpp:287-398): same (j, image) set, same order. */

@@ -80,6 +80,8 @@ _SIGS = {
                      ctypes.c_double],
     "nnpx_md_run": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int],
     "nnpx_num_connections": [ctypes.c_void_p, ctypes.c_int, _c_int_p],
+    "nnpx_set_neighbors_external": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_get_weights": [ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_set_weights": [ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_loss_gradient": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
@@ -198,8 +200,21 @@ class Mode:
             box = np.ascontiguousarray(boxes, dtype=np.float64).reshape(ns, 9)
             per = np.ones(ns, dtype=np.int32) if periodic is None else np.ascontiguousarray(periodic, np.int32)
         self._ck(self._lib.nnpx_set_structures(self._h, ns, _ptr(off), _ptr(pos), _ptr(el), _ptr(box), _ptr(per)))
-        self.n_atoms, self.n_struct = n, ns
+        self.n_atoms, self.n_struct, self.n_force = n, ns, None
         self._pos_shape = pos.shape
+
+    def set_neighbors_external(self, elem_local, n_total, offsets, index, nb_elem, dr):
+        """Caller-provided neighbour lists (the LAMMPS path): local atoms'
+        elements, total atom count (local + ghosts), per-atom entry offsets,
+        neighbour indices, their elements and distance vectors r_i - r_j."""
+        e = np.ascontiguousarray(elem_local, dtype=np.int32)
+        o = np.ascontiguousarray(offsets, dtype=np.int64)
+        j = np.ascontiguousarray(index, dtype=np.int32)
+        ne = np.ascontiguousarray(nb_elem, dtype=np.int32)
+        d = np.ascontiguousarray(dr, dtype=np.float64).reshape(-1)
+        self._ck(self._lib.nnpx_set_neighbors_external(self._h, len(e), _ptr(e), int(n_total), _ptr(o), _ptr(j),
+                                                       _ptr(ne), _ptr(d)))
+        self.n_atoms, self.n_struct, self.n_force = len(e), 1, int(n_total)
 
     def update_positions(self, positions):
         pos = np.ascontiguousarray(positions, dtype=np.float64).reshape(self._pos_shape)
@@ -239,7 +254,7 @@ class Mode:
         return e
 
     def forces(self):
-        f = np.zeros((self.n_atoms, 3))
+        f = np.zeros((getattr(self, "n_force", None) or self.n_atoms, 3))
         self._ck(self._lib.nnpx_get_forces(self._h, _ptr(f)))
         return f
 

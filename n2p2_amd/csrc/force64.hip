@@ -9,6 +9,8 @@
 // reference's loop over neighborsUnique+1 does, and included in the push
 // form (InterfaceLammps::getForces, InterfaceLammps.cpp:679-740).
 // One wave per atom; no atomics, fixed summation order.
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace nnpx {
@@ -81,13 +83,10 @@ __global__ __launch_bounds__(64) void k_work(const ModelT* __restrict__ M, SysVi
     const ElementT& E = M->el[e];
     const StructT& S = s.st[s.sid[i]];
     const int n = min(s.nb_n[i], 1024);
-    const double* ri = s.pos + 3 * (size_t)i;
     for (int k = lane; k < n; k += kWave) {
         const int j = s.nb_j[(size_t)i * s.cap + k];
-        int b0, b1, b2;
-        unpack_image(s.nb_img[(size_t)i * s.cap + k], b0, b1, b2);
         double dr[3];
-        pair_vector(ri, s.pos + 3 * (size_t)j, S.box, b0, b1, b2, dr);
+        entry_vector(s, i, (size_t)i * s.cap + k, S, dr);
         sx[0][k] = dr[0]; sx[1][k] = dr[1]; sx[2][k] = dr[2];
         sd[k] = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
         se[k] = s.elem[j];
@@ -129,7 +128,72 @@ __global__ __launch_bounds__(64) void k_work(const ModelT* __restrict__ M, SysVi
     }
 }
 
+// Push form (InterfaceLammps::getForces, InterfaceLammps.cpp:679-740): atom i
+// scatters -dE_i/dG . dG_i/dr_j onto every neighbour entry j (ghosts of an
+// external list included) and -dE_i/dG . dG_i/dr_i onto itself.
+__global__ __launch_bounds__(64) void k_force64_push(const ModelT* __restrict__ M, SysView s,
+                                                      const double* __restrict__ dEdG, int ldG,
+                                                      const double* __restrict__ dGself,
+                                                      const double* __restrict__ dGnbr, int S, double* f)
+{
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = s.elem[i];
+    const ElementT& E = M->el[e];
+    const double* dE = dEdG + (size_t)i * ldG;
+    const double* ds = dGself + (size_t)i * ldG * 3;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int q = lane; q < E.nsf; q += kWave) {
+        fx -= dE[q] * ds[3 * q];
+        fy -= dE[q] * ds[3 * q + 1];
+        fz -= dE[q] * ds[3 * q + 2];
+    }
+    fx = wave_sum(fx);
+    fy = wave_sum(fy);
+    fz = wave_sum(fz);
+    if (lane == 0) {
+        atomicAdd(f + 3 * (size_t)i, fx);
+        atomicAdd(f + 3 * (size_t)i + 1, fy);
+        atomicAdd(f + 3 * (size_t)i + 2, fz);
+    }
+    const int n = s.nb_n[i];
+    const size_t base = (size_t)i * s.cap;
+    for (int k = lane; k < n; k += kWave) {
+        const int j = s.nb_j[base + k];
+        const int ej = s.elem[j];
+        const int* tab = M->table + E.tab0[ej];
+        const int nt = E.nsfd[ej];
+        const double* d = dGnbr + (base + k) * S * 3;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        for (int t = 0; t < nt; ++t) {
+            const double w = dE[tab[t]];
+            px -= w * d[3 * t];
+            py -= w * d[3 * t + 1];
+            pz -= w * d[3 * t + 2];
+        }
+        atomicAdd(f + 3 * (size_t)j, px);
+        atomicAdd(f + 3 * (size_t)j + 1, py);
+        atomicAdd(f + 3 * (size_t)j + 2, pz);
+    }
+}
+
+__global__ void k_zero64(double* p, size_t n)
+{
+    for (size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x)
+        p[q] = 0.0;
+}
+
 }  // namespace
+
+void launch_force64_push(const ModelT* m, const SysView& s, const double* dEdG, int ldG, const double* dGself,
+                         const double* dGnbr, int S, double* f, hipStream_t stream)
+{
+    if (s.natoms == 0) return;
+    const size_t n3 = (size_t)3 * s.nforce;
+    hipLaunchKernelGGL(k_zero64, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0, stream, f,
+                       n3);
+    hipLaunchKernelGGL(k_force64_push, dim3(s.natoms), dim3(kWave), 0, stream, m, s, dEdG, ldG, dGself, dGnbr, S, f);
+}
 
 void launch_force64(const ModelT* m, const SysView& s, const double* dEdG, int ldG,
                     const double* dGself, const double* dGnbr, int S, int push, double* f,

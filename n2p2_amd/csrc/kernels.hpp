@@ -34,7 +34,27 @@ struct SysView {
     // when non-null, the list kernels run only if *gate != 0 (device-side
     // rebuild decision inside captured MD steps)
     const int* gate;
+    // caller-provided neighbour lists (the LAMMPS path): distance vector
+    // r_i - r_j of every entry [N*cap*3]; null when lists are built here
+    const double* nb_dr;
+    // force entries (natoms, or local + ghost atoms with external lists)
+    int nforce;
 };
+
+// Distance vector r_i - r_j of entry e (= i*cap + k) of atom i: from the
+// positions and the image shift, or as given with an external list.
+__device__ inline void entry_vector(const SysView& s, int i, size_t e, const StructT& S, double* dr)
+{
+    if (s.nb_dr) {
+        dr[0] = s.nb_dr[3 * e];
+        dr[1] = s.nb_dr[3 * e + 1];
+        dr[2] = s.nb_dr[3 * e + 2];
+        return;
+    }
+    int b0, b1, b2;
+    unpack_image(s.nb_img[e], b0, b1, b2);
+    pair_vector(s.pos + 3 * (size_t)i, s.pos + 3 * (size_t)s.nb_j[e], S.box, b0, b1, b2, dr);
+}
 
 void fill_int(int* p, int v, size_t n, hipStream_t stream, const int* gate = nullptr);
 
@@ -70,6 +90,12 @@ void launch_nn64(const ModelT* m, const ModelT& hm, const double* w, const int* 
 // Per-structure energy sums (fixed-order tree reduction).
 void launch_energy(const double* atomE, const int* atom_off, int nstruct, double* E,
                    hipStream_t stream);
+
+// FP64 push-form forces from stored dG/dr scattered onto every neighbour
+// entry (InterfaceLammps::getForces, InterfaceLammps.cpp:679-740): works
+// with external lists whose ghost atoms have no lists of their own.
+void launch_force64_push(const ModelT* m, const SysView& s, const double* dEdG, int ldG, const double* dGself,
+                         const double* dGnbr, int S, double* f, hipStream_t stream);
 
 // FP64 force contraction from stored dG/dr.
 void launch_force64(const ModelT* m, const SysView& s, const double* dEdG, int ldG,

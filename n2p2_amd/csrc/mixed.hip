@@ -155,14 +155,11 @@ __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage&
     const int lane = threadIdx.x;
     const StructT& St = s.st[s.sid[i]];
     const int n = min(s.nb_n[i], capL);
-    const double* ri = s.pos + 3 * (size_t)i;
     const size_t base = (size_t)i * s.cap;
     for (int k = lane; k < n; k += kWave) {
         const int j = s.nb_j[base + k];
-        int b0, b1, b2;
-        unpack_image(s.nb_img[base + k], b0, b1, b2);
         double dr[3];
-        pair_vector(ri, s.pos + 3 * (size_t)j, St.box, b0, b1, b2, dr);
+        entry_vector(s, i, base + k, St, dr);
         const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
         S.g[k] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
         S.c[k] = make_float4((float)(1.0 / d), 0.f, 0.f, 0.f);
@@ -1195,7 +1192,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     timed("energy", [&] { launch_energy(b.Eatom, b.atom_off, b.nstruct, b.E, stream); });
     if (!forces) return;
     timed("force32", [&] {
-        const size_t n3 = (size_t)3 * s.natoms;
+        const size_t n3 = (size_t)3 * s.nforce;
         hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0, stream,
                            b.F, n3);
         const int ps = push ? 1 : 0;

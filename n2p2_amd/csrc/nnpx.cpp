@@ -76,6 +76,9 @@ struct nnpx_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
+    // caller-provided neighbour lists (LAMMPS path): forces on ntotal atoms
+    bool external = false;
+    int ntotal = 0;
     // timing
     std::vector<Timer> pending;
     std::map<std::string, std::pair<double, int64_t>> ktime;
@@ -214,8 +217,12 @@ SysView sysview(nnpx_ctx* c)
     s.cell_start = existing<int>(c, "cell_start");
     s.cell_atoms = existing<int>(c, "cell_atoms");
     s.cell_pos = existing<double>(c, "cell_pos");
+    s.nb_dr = c->external ? existing<double>(c, "nb_dr") : nullptr;
+    s.nforce = c->external ? c->ntotal : c->N;
     return s;
 }
+
+int force_entries(const nnpx_ctx* c) { return c->external ? c->ntotal : c->N; }
 
 void need_model(nnpx_ctx* c)
 {
@@ -364,6 +371,10 @@ void read_flags(nnpx_ctx* c, int* f)
 // Builds the list; grows the capacity and rebuilds when an atom overflowed.
 void build_neighbors(nnpx_ctx* c, bool check)
 {
+    if (c->external) {  // the caller's list stays
+        c->nbr_valid = true;
+        return;
+    }
     for (int attempt = 0; attempt < 3; ++attempt) {
         alloc_neighbors(c);
         SysView s = sysview(c);
@@ -452,8 +463,16 @@ void compute_energy(nnpx_ctx* c)
 void compute_force64(nnpx_ctx* c)
 {
     const ModelT& t = c->hm.t;
-    double* f = buf<double>(c, "F", (size_t)3 * c->N);
+    double* f = buf<double>(c, "F", (size_t)3 * force_entries(c));
     SysView s = sysview(c);
+    if (c->external) {
+        timed(c, "force64", [&] {
+            launch_force64_push(c->d_model, s, existing<double>(c, "dEdG"), t.max_sf, existing<double>(c, "dGself"),
+                                existing<double>(c, "dGnbr"), std::max(t.max_sfd, 1), f, c->stream);
+        });
+        c->f_valid = true;
+        return;
+    }
     if (!c->rev_valid) {
         timed(c, "rev", [&] { launch_reverse(s, c->stream); });
         c->rev_valid = true;
@@ -482,8 +501,9 @@ void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check)
     build_neighbors(c, check);
     if (precision == NNPX_MIXED) {
         MixedBuffers mb = mixed_buffers(c);
-        run_mixed(c->mixed, c->hm.t, c->d_model, sysview(c), mb, forces, c->force_mode == NNPX_FORCES_PUSH,
-                  c->stream, [&](const char* name, const std::function<void()>& f) { timed(c, name, f); });
+        run_mixed(c->mixed, c->hm.t, c->d_model, sysview(c), mb, forces,
+                  c->external || c->force_mode == NNPX_FORCES_PUSH, c->stream,
+                  [&](const char* name, const std::function<void()>& f) { timed(c, name, f); });
         c->sf_valid = true;
         c->nn_valid = true;
         c->dedg_valid = true;
@@ -531,7 +551,7 @@ MixedBuffers mixed_buffers(nnpx_ctx* c)
     m.dEdGf = buf<float>(c, "dEdGf", (size_t)c->N * m.ldG);
     m.Eatom = buf<double>(c, "Eatom", c->N);
     m.E = buf<double>(c, "E", c->S);
-    m.F = buf<double>(c, "F", (size_t)3 * c->N);
+    m.F = buf<double>(c, "F", (size_t)3 * force_entries(c));
     m.elem_atoms = existing<int>(c, "elem_atoms");
     m.atom_off = existing<int>(c, "atom_off");
     for (int e = 0; e <= t.nelem; ++e) m.elem_off[e] = c->elem_off[e];
@@ -739,7 +759,10 @@ nnpx_status nnpx_set_structures(nnpx_ctx* c, int ns, const int64_t* off, const d
             if (off[s + 1] < off[s]) fail(NNPX_ERR_ARGUMENT, "atom_offsets must be non-decreasing");
         const int64_t N = off[ns];
         if (N <= 0 || N > (int64_t)1 << 30) fail(NNPX_ERR_ARGUMENT, "bad atom count");
-        // a new structure set ends any MD set-up (skin, graph)
+        // a new structure set ends any MD set-up (skin, graph) and any
+        // caller-provided neighbour list
+        c->external = false;
+        c->ntotal = 0;
         c->skin = 0.0;
         c->md_ready = false;
         drop_graph(c);
@@ -796,11 +819,104 @@ nnpx_status nnpx_set_structures(nnpx_ctx* c, int ns, const int64_t* off, const d
     });
 }
 
+nnpx_status nnpx_set_neighbors_external(nnpx_ctx* c, int nlocal, const int* elem, int ntotal,
+                                        const int64_t* nb_offsets, const int* nb_index, const int* nb_elem,
+                                        const double* nb_dr)
+{
+    return guard(c, [&] {
+        need_model(c);
+        if (nlocal <= 0 || ntotal < nlocal || !elem || !nb_offsets) fail(NNPX_ERR_ARGUMENT, "bad atom counts");
+        const int ne = c->hm.t.nelem;
+        const int64_t P = nb_offsets[nlocal];
+        if (nb_offsets[0] != 0 || P < 0 || (P > 0 && (!nb_index || !nb_elem || !nb_dr)))
+            fail(NNPX_ERR_ARGUMENT, "bad neighbour offsets or null arrays");
+        std::vector<int> el_all(ntotal, -1);
+        for (int i = 0; i < nlocal; ++i) {
+            if (elem[i] < 0 || elem[i] >= ne) fail(NNPX_ERR_RANGE, "element index out of range");
+            el_all[i] = elem[i];
+        }
+        int maxn = 0;
+        for (int i = 0; i < nlocal; ++i) {
+            const int64_t a = nb_offsets[i], b = nb_offsets[i + 1];
+            if (b < a) fail(NNPX_ERR_ARGUMENT, "neighbour offsets must be non-decreasing");
+            maxn = std::max<int64_t>(maxn, b - a);
+            for (int64_t p = a; p < b; ++p) {
+                const int j = nb_index[p];
+                if (j < 0 || j >= ntotal) fail(NNPX_ERR_RANGE, "neighbour index out of range");
+                if (nb_elem[p] < 0 || nb_elem[p] >= ne) fail(NNPX_ERR_RANGE, "neighbour element out of range");
+                if (el_all[j] >= 0 && el_all[j] != nb_elem[p]) fail(NNPX_ERR_ARGUMENT, "inconsistent neighbour element");
+                el_all[j] = nb_elem[p];
+            }
+        }
+        if (maxn > 1024) fail(NNPX_ERR_UNSUPPORTED, "more than 1024 neighbours per atom");
+        for (auto& x : el_all) x = std::max(x, 0);
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        drop_graph(c);
+        c->md_ready = false;
+        c->skin = 0.0;
+        c->external = true;
+        c->ntotal = ntotal;
+        c->N = nlocal;
+        c->S = 1;
+        c->atom_off.assign({0, (int64_t)nlocal});
+        c->elem_host = el_all;  // local atoms, then ghosts
+        c->st_host.assign(1, StructT{});
+        c->st_host[0].atom0 = 0;
+        c->st_host[0].natoms = nlocal;
+        c->any_cell = false;
+        c->ncells = 0;
+        c->elem_count.assign(ne, 0);
+        c->elem_off.assign(ne + 1, 0);
+        for (int i = 0; i < nlocal; ++i) c->elem_count[elem[i]]++;
+        for (int e = 0; e < ne; ++e) c->elem_off[e + 1] = c->elem_off[e] + c->elem_count[e];
+        std::vector<int> atoms(nlocal);
+        {
+            std::vector<int> fillp(c->elem_off.begin(), c->elem_off.end() - 1);
+            for (int i = 0; i < nlocal; ++i) atoms[fillp[elem[i]]++] = i;
+        }
+        c->cap = std::max(32, ((maxn + 31) / 32) * 32);
+        c->max_nb = maxn;
+        const size_t slots = (size_t)nlocal * c->cap;
+        std::vector<int> nn(nlocal), jj(slots, 0), im(slots, pack_image(0, 0, 0));
+        std::vector<double> dr(3 * slots, 0.0);
+        for (int i = 0; i < nlocal; ++i) {
+            nn[i] = (int)(nb_offsets[i + 1] - nb_offsets[i]);
+            for (int k = 0; k < nn[i]; ++k) {
+                const int64_t p = nb_offsets[i] + k;
+                jj[(size_t)i * c->cap + k] = nb_index[p];
+                for (int x = 0; x < 3; ++x) dr[3 * ((size_t)i * c->cap + k) + x] = nb_dr[3 * p + x];
+            }
+        }
+        std::vector<int> sid(nlocal, 0), off32 = {0, nlocal}, flags(4, 0);
+        std::vector<double> pos(3 * (size_t)nlocal, 0.0);
+        auto up = [&](const std::string& name, const void* src, size_t bytes) {
+            void* d = buf<char>(c, name, bytes);
+            hip_check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+        };
+        up("pos", pos.data(), sizeof(double) * pos.size());
+        up("elem", el_all.data(), sizeof(int) * ntotal);
+        up("sid", sid.data(), sizeof(int) * nlocal);
+        up("st", c->st_host.data(), sizeof(StructT));
+        up("elem_atoms", atoms.data(), sizeof(int) * nlocal);
+        up("atom_off", off32.data(), sizeof(int) * 2);
+        up("nb_n", nn.data(), sizeof(int) * nlocal);
+        up("nb_j", jj.data(), sizeof(int) * slots);
+        up("nb_img", im.data(), sizeof(int) * slots);
+        up("nb_dr", dr.data(), sizeof(double) * dr.size());
+        up("flags", flags.data(), sizeof(int) * flags.size());
+        buf<int>(c, "nb_rev", slots);
+        invalidate(c);
+        c->nbr_valid = true;
+        c->rev_valid = true;
+    });
+}
+
 nnpx_status nnpx_update_positions(nnpx_ctx* c, const double* pos)
 {
     return guard(c, [&] {
         need_structures(c);
         if (!pos) fail(NNPX_ERR_ARGUMENT, "null positions");
+        if (c->external) fail(NNPX_ERR_STATE, "positions are not used with an external neighbour list");
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         for (int s = 0; s < c->S; ++s) {
             StructT& t = c->st_host[s];
@@ -903,7 +1019,7 @@ nnpx_status nnpx_get_forces(nnpx_ctx* c, double* f)
         if (!f) fail(NNPX_ERR_ARGUMENT, "null output");
         if (!c->f_valid) fail(NNPX_ERR_STATE, "forces not calculated");
         settle(c);
-        copy_out(c, f, existing<double>(c, "F"), sizeof(double) * 3 * c->N);
+        copy_out(c, f, existing<double>(c, "F"), sizeof(double) * 3 * force_entries(c));
     });
 }
 
@@ -1194,6 +1310,7 @@ nnpx_status nnpx_md_init(nnpx_ctx* c, int precision, const double* masses, const
     return guard(c, [&] {
         need_structures(c);
         if (c->S != 1) fail(NNPX_ERR_UNSUPPORTED, "MD runs one structure per context");
+        if (c->external) fail(NNPX_ERR_STATE, "MD needs positions (not an external neighbour list)");
         if (precision != NNPX_PLAIN_FP64 && precision != NNPX_MIXED) fail(NNPX_ERR_ARGUMENT, "bad precision");
         if (!masses || !vel) fail(NNPX_ERR_ARGUMENT, "null masses or velocities");
         if (!(dt > 0.0) || !(skin >= 0.0)) fail(NNPX_ERR_ARGUMENT, "dt must be > 0 and skin >= 0");

@@ -91,15 +91,12 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
     const double tanhPre = M->tanhPre;
     const StructT& S = s.st[s.sid[i]];
     const int n = min(s.nb_n[i], capL);
-    const double* ri = s.pos + 3 * (size_t)i;
     const size_t ebase = (size_t)i * s.cap;
 
     for (int k = tid; k < n; k += kSFThreads) {
         const int j = s.nb_j[ebase + k];
-        int b0, b1, b2;
-        unpack_image(s.nb_img[ebase + k], b0, b1, b2);
         double dr[3];
-        pair_vector(ri, s.pos + 3 * (size_t)j, S.box, b0, b1, b2, dr);
+        entry_vector(s, i, ebase + k, S, dr);
         L.dr[k] = dr[0];
         L.dr[capL + k] = dr[1];
         L.dr[2 * capL + k] = dr[2];
