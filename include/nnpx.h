@@ -224,6 +224,18 @@ nnpx_status nnpx_md_state(nnpx_ctx* ctx, double* pos, double* vel);
 nnpx_status nnpx_md_energies(nnpx_ctx* ctx, double* epot, double* ekin,
                              int64_t max, int64_t* nsteps, int64_t* rebuilds);
 
+/* ---- launch-shape autotuner ------------------------------------------- */
+/* No counterpart in the reference.  Times candidate launch shapes of the
+ * mixed-form kernels on the current structures (HIP events, 3 launches
+ * each) and keeps the fastest in the context: the triplet-buffer size of the
+ * SF, force and tangent kernels (LDS per wave against batching) and the
+ * atoms per block of the gradient kernel.  Results are not changed by the
+ * choice beyond summation order.  values[4] = {trip_sf, trip_force,
+ * trip_tangent, grad_atoms}. */
+nnpx_status nnpx_autotune(nnpx_ctx* ctx);
+nnpx_status nnpx_get_tuning(const nnpx_ctx* ctx, int* values);
+nnpx_status nnpx_set_tuning(nnpx_ctx* ctx, const int* values);
+
 /* ---- measurement ------------------------------------------------------ */
 /* No counterpart in the reference: multiplies two known 16x16 matrices with
  * the bf16 MFMA operand layout the NN kernels use and returns the max abs

@@ -80,6 +80,9 @@ _SIGS = {
                      ctypes.c_double],
     "nnpx_md_run": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int],
     "nnpx_num_connections": [ctypes.c_void_p, ctypes.c_int, _c_int_p],
+    "nnpx_autotune": [ctypes.c_void_p],
+    "nnpx_get_tuning": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_set_tuning": [ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_set_neighbors_external": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_get_weights": [ctypes.c_void_p, ctypes.c_void_p],
@@ -298,6 +301,22 @@ class Mode:
         v = ctypes.c_double()
         self._ck(self._lib.nnpx_mfma_selftest(self._h, ctypes.byref(v)))
         return v.value
+
+    # ------------------------------------------------------ autotuner --
+    def autotune(self):
+        """Time candidate launch shapes on the current structures and keep the
+        fastest; returns the chosen {trip_sf, trip_force, trip_tangent, grad_atoms}."""
+        self._ck(self._lib.nnpx_autotune(self._h))
+        return self.tuning()
+
+    def tuning(self):
+        v = np.zeros(4, np.int32)
+        self._ck(self._lib.nnpx_get_tuning(self._h, _ptr(v)))
+        return dict(zip(("trip_sf", "trip_force", "trip_tangent", "grad_atoms"), (int(x) for x in v)))
+
+    def set_tuning(self, trip_sf, trip_force, trip_tangent, grad_atoms):
+        v = np.array([trip_sf, trip_force, trip_tangent, grad_atoms], np.int32)
+        self._ck(self._lib.nnpx_set_tuning(self._h, _ptr(v)))
 
     # ------------------------------------------------------ training --
     def num_connections(self, element=-1):

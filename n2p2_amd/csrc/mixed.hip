@@ -115,7 +115,7 @@ __device__ inline GroupF group_f(const GroupT& g)
     return GroupF{(float)g.rc, (float)g.rc2, (float)g.rci, (float)g.iw, (float)g.rcinv};
 }
 
-constexpr int kTrip = 384;  // compacted triplets buffered per wave
+constexpr int kTripDefault = 384;  // compacted triplets buffered per wave (tunable)
 constexpr int kRadChunk = 8;  // radial members accumulated per pass
 
 // LDS staging of one atom's neighbours (FP32 geometry) and the compacted
@@ -128,16 +128,18 @@ struct Stage {
     unsigned* tjk;   // compacted pair (j | k << 16)
     float* tr2;      // its r_jk^2
     float4* mp;      // [2*kMaxMembers] member constants of the current group
+    int trip;        // capacity of the triplet buffer
 };
 
-__device__ inline Stage carve(float* base, int capL)
+__device__ inline Stage carve(float* base, int capL, int trip)
 {
     Stage S;
+    S.trip = trip;
     S.g = reinterpret_cast<float4*>(base);
     S.c = S.g + capL;
     S.tjk = reinterpret_cast<unsigned*>(S.c + capL);
-    S.tr2 = reinterpret_cast<float*>(S.tjk + kTrip);
-    S.mp = reinterpret_cast<float4*>(S.tr2 + kTrip);
+    S.tr2 = reinterpret_cast<float*>(S.tjk + trip);
+    S.mp = reinterpret_cast<float4*>(S.tr2 + trip);
     S.la = reinterpret_cast<unsigned short*>(S.mp + 2 * kMaxMembers);
     S.lb = S.la + capL;
     S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
@@ -145,9 +147,9 @@ __device__ inline Stage carve(float* base, int capL)
 }
 // bytes of the staging area (capL is a multiple of 32, so the byte arrays end
 // 8-byte aligned)
-__host__ __device__ constexpr size_t stage_bytes(int capL)
+__host__ __device__ constexpr size_t stage_bytes(int capL, int trip)
 {
-    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)kTrip * 8 + (size_t)2 * kMaxMembers * 16;
+    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembers * 16;
 }
 
 __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
@@ -279,7 +281,7 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
     while (t0 < total) {
         // fill the buffer with passing pairs ...
         int cnt = 0;
-        while (t0 < total && cnt <= kTrip - kWave) {
+        while (t0 < total && cnt <= S.trip - kWave) {
             const int t = t0 + lane;
             bool ok = false;
             unsigned jk = 0;
@@ -329,10 +331,10 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
 template <int MAXM, bool FAST>
 __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                              const GroupMixF* __restrict__ GM, SysView s, double* __restrict__ G,
-                                             float* __restrict__ Gin, int ldG, int ldK, int capL)
+                                             float* __restrict__ Gin, int ldG, int ldK, int capL, int trip)
 {
     extern __shared__ float sm[];
-    const Stage S = carve(sm, capL);
+    const Stage S = carve(sm, capL, trip);
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
@@ -632,11 +634,11 @@ template <int MAXM, bool FAST>
 __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                                 const GroupMixF* __restrict__ GM, SysView s,
                                                 const float* __restrict__ fw, int ldG, double* __restrict__ F,
-                                                int push, int capL)
+                                                int push, int capL, int trip)
 {
     extern __shared__ float sm[];
-    const Stage S = carve(sm, capL);
-    double* ax = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + stage_bytes(capL));
+    const Stage S = carve(sm, capL, trip);
+    double* ax = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip));
     double* ay = ax + capL;
     double* az = ay + capL;
     const int i = blockIdx.x;
@@ -807,11 +809,11 @@ template <int MAXM, bool FAST>
 __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                                   const GroupMixF* __restrict__ GM, SysView s,
                                                   const double* __restrict__ u, const float* __restrict__ sffac,
-                                                  double* __restrict__ Gdot, int ldG, int capL)
+                                                  double* __restrict__ Gdot, int ldG, int capL, int trip)
 {
     extern __shared__ float sm[];
-    const Stage S = carve(sm, capL);
-    float4* du = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL));
+    const Stage S = carve(sm, capL, trip);
+    float4* du = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip));
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
@@ -1138,8 +1140,10 @@ void free_mixed(MixedModel& mm)
     if (mm.d_w) (void)hipFree(mm.d_w);
     if (mm.d_b) (void)hipFree(mm.d_b);
     const int terms = mm.terms;
+    const Tuning tune = mm.tune;
     mm = MixedModel{};
     mm.terms = terms;
+    mm.tune = tune;
 }
 
 void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
@@ -1149,14 +1153,16 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     if (s.natoms == 0) return;
     const int capL = s.cap;
     const int P = pow2_members(t);
-    const size_t lds_sf = stage_bytes(capL);
-    const size_t lds_f = lds_sf + sizeof(double) * 3 * (size_t)capL;
-    if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
+    const int tsf = mm.tune.trip_sf, tf = mm.tune.trip_force;
+    const size_t lds_sf = stage_bytes(capL, tsf);
+    const size_t lds_f = stage_bytes(capL, tf) + sizeof(double) * 3 * (size_t)capL;
+    if (lds_f > 65536 || lds_sf > 65536)
+        throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     timed("sf32", [&] {
         dispatch_members(mm.fastM, P, [&](auto m, auto fast) {
             hipLaunchKernelGGL((k_sf32<decltype(m)::value, decltype(fast)::value>),
                                dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, mm.d_gmix, s, b.G,
-                               b.Gin, b.ldG, mm.ldK, capL);
+                               b.Gin, b.ldG, mm.ldK, capL, tsf);
         });
     });
     NNMix a{};
@@ -1199,7 +1205,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
         dispatch_members(mm.fastM, P, [&](auto m, auto fast) {
             hipLaunchKernelGGL((k_force32<decltype(m)::value, decltype(fast)::value>),
                                dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, mm.d_gmix, s, b.dEdGf,
-                               b.ldG, b.F, ps, capL);
+                               b.ldG, b.F, ps, capL, tf);
         });
     });
 }
@@ -1210,11 +1216,12 @@ void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_mod
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
     if (s.natoms == 0) return;
     const int capL = s.cap;
-    const size_t lds = stage_bytes(capL) + sizeof(float4) * (size_t)capL;
+    const int tt = mm.tune.trip_tangent;
+    const size_t lds = stage_bytes(capL, tt) + sizeof(float4) * (size_t)capL;
     if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     dispatch_members(mm.fastM, pow2_members(t), [&](auto m, auto fast) {
         hipLaunchKernelGGL((k_tangent32<decltype(m)::value, decltype(fast)::value>), dim3(s.natoms), dim3(kWave), lds,
-                           stream, d_model, mm.d_mem, mm.d_gmix, s, u, mm.d_sffac, Gdot, ldG, capL);
+                           stream, d_model, mm.d_mem, mm.d_gmix, s, u, mm.d_sffac, Gdot, ldG, capL, tt);
     });
 }
 

@@ -38,6 +38,15 @@ struct GroupMixF {
     int pad[3];
 };
 
+// Launch shapes of the mixed-form kernels, set by the autotuner
+// (nnpx_autotune) or left at these defaults.
+struct Tuning {
+    int trip_sf = 384;       // triplet buffer of the SF kernel (multiple of 64)
+    int trip_force = 384;    // ... of the force kernel
+    int trip_tangent = 384;  // ... of the tangent kernel
+    int grad_atoms = 32;     // atoms per block of the gradient kernel
+};
+
 struct MixedModel {
     int ldK = 0;                // padded NN input width (multiple of 16)
     int maxpad = 0;             // max padded layer width
@@ -55,6 +64,7 @@ struct MixedModel {
     int bo[kMaxElem][kMaxLayers] = {};   // offset of the bias
     int pin[kMaxElem][kMaxLayers] = {};  // padded in-width of layer l
     int pout[kMaxElem][kMaxLayers] = {}; // padded out-width of layer l
+    Tuning tune;
     bool ready = false;
 };
 

@@ -1235,12 +1235,13 @@ nnpx_status nnpx_loss_gradient(nnpx_ctx* c, int precision, const double* E_ref, 
         a.atoms = existing<int>(c, "elem_atoms");
         a.nelem = t.nelem;
         a.maxw = std::max(t.max_width, t.max_sf);
+        a.per_block = c->mixed.tune.grad_atoms;
         a.grad = gd;
         int blocks = 0;
         for (int e = 0; e < t.nelem; ++e) {
             a.blk_off[e] = blocks;
             a.atom_off[e] = c->elem_off[e];
-            blocks += (c->elem_count[e] + kGradAtoms - 1) / kGradAtoms;
+            blocks += (c->elem_count[e] + a.per_block - 1) / a.per_block;
         }
         a.blk_off[t.nelem] = blocks;
         a.atom_off[t.nelem] = c->elem_off[t.nelem];
@@ -1255,6 +1256,140 @@ nnpx_status nnpx_loss_gradient(nnpx_ctx* c, int precision, const double* E_ref, 
         loss[0] = l[1] + l[2];
         copy_out(c, grad, gd, sizeof(double) * total);
         collect_times(c);
+    });
+}
+
+namespace {
+
+// Mean event time (ms) of `name` over the launches since the last reset.
+double mean_time(nnpx_ctx* c, const char* name)
+{
+    collect_times(c);
+    auto it = c->ktime.find(name);
+    return it == c->ktime.end() || it->second.second == 0 ? 1e30 : it->second.first / it->second.second;
+}
+
+}  // namespace
+
+nnpx_status nnpx_autotune(nnpx_ctx* c)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (c->md_ready) fail(NNPX_ERR_STATE, "autotune before nnpx_md_init");
+        const bool timing = c->timing;
+        Tuning& tu = c->mixed.tune;
+        const Tuning keep = tu;
+        const int capL = c->cap;
+        const ModelT& t = c->hm.t;
+        auto fits = [&](int trip, size_t extra) {
+            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembers * 16 + extra <= 65536;
+        };
+        c->timing = true;
+        double best_sf = 1e30, best_f = 1e30, best_t = 1e30;
+        Tuning best = keep;
+        const int ldG = t.max_sf;
+        double* u = buf<double>(c, "t_u", (size_t)3 * c->N);
+        double* Gdot = buf<double>(c, "t_Gdot", (size_t)c->N * ldG);
+        hip_check(hipMemsetAsync(u, 0, sizeof(double) * 3 * c->N, c->stream), "hipMemsetAsync");
+        try {
+            for (int trip : {128, 256, 384, 512, 768, 1024}) {
+                if (!fits(trip, sizeof(double) * 3 * (size_t)capL)) continue;
+                tu.trip_sf = tu.trip_force = tu.trip_tangent = trip;
+                evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, true);  // warm-up
+                settle(c);
+                c->ktime.clear();
+                for (int r = 0; r < 3; ++r) evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, false);
+                SysView s = sysview(c);
+                for (int r = 0; r < 3; ++r)
+                    timed(c, "tangent32", [&] { launch_tangent32(c->mixed, t, c->d_model, s, u, Gdot, ldG, c->stream); });
+                const double a = mean_time(c, "sf32"), b = mean_time(c, "force32"), d = mean_time(c, "tangent32");
+                if (a < best_sf) { best_sf = a; best.trip_sf = trip; }
+                if (b < best_f) { best_f = b; best.trip_force = trip; }
+                if (d < best_t) { best_t = d; best.trip_tangent = trip; }
+            }
+            tu = best;
+            // atoms per block of the gradient kernel (FP32 network pass on the
+            // current G with a zero tangent)
+            GradArgs a{};
+            int total = 0;
+            for (int e = 0; e < t.nelem; ++e) {
+                a.coff[e] = total;
+                a.nconn[e] = num_connections(t.el[e].nn);
+                a.nconn_max = std::max(a.nconn_max, a.nconn[e]);
+                total += a.nconn[e];
+            }
+            double* cE = buf<double>(c, "t_cE", c->S);
+            double* gd = buf<double>(c, "t_grad", total);
+            hip_check(hipMemsetAsync(cE, 0, sizeof(double) * c->S, c->stream), "hipMemsetAsync");
+            hip_check(hipMemsetAsync(Gdot, 0, sizeof(double) * (size_t)c->N * ldG, c->stream), "hipMemsetAsync");
+            a.w = c->d_w;
+            a.nweights = t.nweights;
+            a.G = existing<double>(c, "G");
+            a.Gdot = Gdot;
+            a.ldG = ldG;
+            a.cE = cE;
+            a.sid = existing<int>(c, "sid");
+            a.atoms = existing<int>(c, "elem_atoms");
+            a.nelem = t.nelem;
+            a.maxw = std::max(t.max_width, t.max_sf);
+            a.grad = gd;
+            double best_g = 1e30;
+            if (grad_lds_bytes(a, true) <= 65536) {
+                for (int per : {8, 16, 32, 64, 128}) {
+                    a.per_block = per;
+                    int blocks = 0;
+                    for (int e = 0; e < t.nelem; ++e) {
+                        a.blk_off[e] = blocks;
+                        a.atom_off[e] = c->elem_off[e];
+                        blocks += (c->elem_count[e] + per - 1) / per;
+                    }
+                    a.blk_off[t.nelem] = blocks;
+                    a.atom_off[t.nelem] = c->elem_off[t.nelem];
+                    c->ktime.clear();
+                    launch_grad(c->d_model, a, blocks, true, c->stream);
+                    for (int r = 0; r < 3; ++r) timed(c, "grad32", [&] { launch_grad(c->d_model, a, blocks, true, c->stream); });
+                    const double g = mean_time(c, "grad32");
+                    if (g < best_g) { best_g = g; tu.grad_atoms = per; }
+                }
+            }
+        } catch (...) {
+            tu = keep;
+            c->timing = timing;
+            throw;
+        }
+        c->ktime.clear();
+        c->timing = timing;
+        invalidate(c);
+    });
+}
+
+nnpx_status nnpx_get_tuning(const nnpx_ctx* cc, int* v)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        if (!v) fail(NNPX_ERR_ARGUMENT, "null output");
+        const Tuning& tu = c->mixed.tune;
+        v[0] = tu.trip_sf;
+        v[1] = tu.trip_force;
+        v[2] = tu.trip_tangent;
+        v[3] = tu.grad_atoms;
+    });
+}
+
+nnpx_status nnpx_set_tuning(nnpx_ctx* c, const int* v)
+{
+    return guard(c, [&] {
+        if (!v) fail(NNPX_ERR_ARGUMENT, "null input");
+        for (int k = 0; k < 3; ++k)
+            if (v[k] < 64 || v[k] > 4096 || v[k] % 64) fail(NNPX_ERR_ARGUMENT, "triplet buffer: multiple of 64 in [64, 4096]");
+        if (v[3] < 1 || v[3] > 1024) fail(NNPX_ERR_ARGUMENT, "gradient atoms per block out of range");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        drop_graph(c);
+        Tuning& tu = c->mixed.tune;
+        tu.trip_sf = v[0];
+        tu.trip_force = v[1];
+        tu.trip_tangent = v[2];
+        tu.grad_atoms = v[3];
     });
 }
 

@@ -120,7 +120,7 @@ __global__ __launch_bounds__(64) void k_tangent64(const ModelT* __restrict__ M, 
 
 // Network forward with the tangent and one reverse pass for
 // Phi_i = cE_s E_i - dE_i/dG . Gdot_i; dPhi/dw accumulated per block in LDS
-// (connection order) and added to grad.  One wave per block, kGradAtoms
+// (connection order) and added to grad.  One wave per block, a.per_block
 // atoms of one element per block.
 template <typename T>
 __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradArgs a)
@@ -162,8 +162,8 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
         }
     }
     const double* wt = a.w + a.nweights;  // transposed copy [in][out]
-    const int first = a.atom_off[e] + ((int)blockIdx.x - a.blk_off[e]) * kGradAtoms;
-    const int last = min(first + kGradAtoms, a.atom_off[e + 1]);
+    const int first = a.atom_off[e] + ((int)blockIdx.x - a.blk_off[e]) * a.per_block;
+    const int last = min(first + a.per_block, a.atom_off[e + 1]);
     __syncthreads();
     for (int p = first; p < last; ++p) {
         const int i = a.atoms[p];

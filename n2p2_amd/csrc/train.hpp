@@ -10,7 +10,6 @@
 
 namespace nnpx {
 
-constexpr int kGradAtoms = 32;  // atoms of one element per gradient block
 
 struct GradArgs {
     const double* w;       // FP64 weights (model layout), transposed copy after nweights
@@ -28,6 +27,7 @@ struct GradArgs {
     int coff[kMaxElem];    // offset of each element's connections in grad
     int nconn_max;
     int maxw;              // max layer width
+    int per_block;         // atoms of one element per block
     double* grad;          // [sum nconn], accumulated
 };
 

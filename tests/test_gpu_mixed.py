@@ -172,3 +172,27 @@ def test_plain_forces_match_finite_differences(water_dir):
             worst = max(worst, abs(fd - F[i, c]))
     say("  max |F - F_fd| = %.2e (threshold %.1e)" % (worst, 10 * h * h))
     assert worst <= 10 * h * h
+
+
+def test_autotune_keeps_results(water_dir):
+    """The autotuner only changes launch shapes: forces before and after
+    agree to summation-order noise, and explicit shapes round-trip."""
+    say("case: autotune on water-1536")
+    pos, elem, box = synth.water_box(8)
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    F0 = m.forces()
+    tu = m.autotune()
+    say("  chosen", tu)
+    assert all(v % 64 == 0 for v in (tu["trip_sf"], tu["trip_force"], tu["trip_tangent"]))
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    d = float(np.abs(m.forces() - F0).max())
+    say("  max|dF| after tuning %.2e" % d)
+    assert d <= 1e-7
+    m.set_tuning(128, 256, 512, 16)
+    assert m.tuning() == {"trip_sf": 128, "trip_force": 256, "trip_tangent": 512, "grad_atoms": 16}
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    assert float(np.abs(m.forces() - F0).max()) <= 1e-7
+    with pytest.raises(n2p2_amd.NNPError):
+        m.set_tuning(100, 256, 512, 16)
