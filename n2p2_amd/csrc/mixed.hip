@@ -320,6 +320,72 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
     }
 }
 
+// Row-major variant for the force kernel: candidates are enumerated a-major
+// (all partners of la[a] consecutively), compacted the same way, and each lane
+// takes a contiguous chunk of the compacted buffer, so it sees long runs of
+// one first atom j and can keep j's force contribution in registers.
+template <typename Work>
+__device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work)
+{
+    const int lane = threadIdx.x;
+    int full, total, R;
+    if (same) {
+        R = (nA - 1) / 2;
+        full = nA * R;
+        total = full + ((nA % 2 == 0) ? nA / 2 : 0);
+    } else {
+        R = nB;
+        full = nA * nB;
+        total = full;
+    }
+    const unsigned short* lb = same ? S.la : S.lb;
+    const float inv = R > 0 ? 1.f / (float)R : 0.f;
+    int t0 = 0;
+    while (t0 < total) {
+        int cnt = 0;
+        while (t0 < total && cnt <= S.trip - kWave) {
+            const int t = t0 + lane;
+            bool ok = false;
+            unsigned jk = 0;
+            float r2 = 0.f;
+            if (t < total) {
+                int a, b;
+                if (t < full) {
+                    int q = (int)((float)t * inv);
+                    int r = t - q * R;
+                    if (r < 0) { --q; r += R; }
+                    else if (r >= R) { ++q; r -= R; }
+                    a = q;
+                    if (same) { b = a + r + 1; if (b >= nA) b -= nA; }
+                    else b = r;
+                } else {
+                    a = t - full;
+                    b = a + nA / 2;
+                }
+                const int j = S.la[a], k = lb[b];
+                const float4 gj = S.g[j], gk = S.g[k];
+                const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
+                r2 = dx * dx + dy * dy + dz * dz;
+                ok = r2 < rc2;
+                jk = (unsigned)j | ((unsigned)k << 16);
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = cnt + __popcll(m & __lanemask_lt());
+                S.tjk[pos] = jk;
+                S.tr2[pos] = r2;
+            }
+            cnt += __popcll(m);
+            t0 += kWave;
+        }
+        __syncthreads();
+        const int per = (cnt + kWave - 1) / kWave;
+        const int x0 = lane * per, x1 = min(cnt, x0 + per);
+        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------- SF values --
 // Angular terms: vexp * pfc * pl^zeta evaluated as one exp2 of
 // zeta * log2(pl) + log2(pfc) - eta log2(e) * sum(r - rs)^2, with log2(pl)
@@ -685,7 +751,11 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
         const MemberMasks R = load_members(S, mf, nmem, wi);
-        for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
+        // contributions to the row atom j are summed in registers over the
+        // lane's run of equal j and flushed when j changes
+        int cur = -1;
+        double rx = 0.0, ry = 0.0, rz = 0.0;
+        for_each_triplet_rows(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
             const float rjk = sqrtf(r2jk);
@@ -770,13 +840,28 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             fx -= (double)(p1x + p2x);
             fy -= (double)(p1y + p2y);
             fz -= (double)(p1z + p2z);
-            lds_add(ax + j, (double)(p1x + p3x));
-            lds_add(ay + j, (double)(p1y + p3y));
-            lds_add(az + j, (double)(p1z + p3z));
+            if (j != cur) {
+                if (cur >= 0) {
+                    lds_add(ax + cur, rx);
+                    lds_add(ay + cur, ry);
+                    lds_add(az + cur, rz);
+                }
+                cur = j;
+                rx = 0.0; ry = 0.0; rz = 0.0;
+            }
+            rx += (double)(p1x + p3x);
+            ry += (double)(p1y + p3y);
+            rz += (double)(p1z + p3z);
             lds_add(ax + k, (double)(p2x - p3x));
             lds_add(ay + k, (double)(p2y - p3y));
             lds_add(az + k, (double)(p2z - p3z));
         });
+        if (cur >= 0) {
+            lds_add(ax + cur, rx);
+            lds_add(ay + cur, ry);
+            lds_add(az + cur, rz);
+        }
+        __syncthreads();
     }
     fx = wave_sum(fx);
     fy = wave_sum(fy);
