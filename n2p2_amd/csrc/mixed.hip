@@ -187,6 +187,19 @@ __device__ inline int make_list(const Stage& S, int n, int elem, float rc, unsig
     return cnt;
 }
 
+// Cutoff values of every staged neighbour for one radius (elements whose
+// groups all share it compute them once per atom).
+__device__ inline void all_cutoffs(const Stage& S, int n, int ctype, const GroupF& g, float tanhPre)
+{
+    for (int k = threadIdx.x; k < n; k += kWave) {
+        float f = 0.f, df = 0.f;
+        cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, S.g[k].w, f, df);
+        S.c[k].y = f;
+        S.c[k].z = df;
+    }
+    __syncthreads();
+}
+
 __device__ inline void group_cutoffs(const Stage& S, int n, int e1, int e2, int ctype, const GroupF& g,
                                      float tanhPre)
 {
@@ -417,7 +430,11 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
         const GroupF g = group_f(GT);
         const int nmem = GT.nmem;
         const MemberF* mf = MF + GT.mem0;
-        group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        if (E.ucut) {
+            if (gi == 0) all_cutoffs(S, n, ctype, g, tanhPre);
+        } else {
+            group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        }
         if (GT.type == 2) {
             // radial members in chunks of kRadChunk (any member count)
             for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
@@ -724,7 +741,11 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const GroupF g = group_f(GT);
         const int nmem = GT.nmem;
         const MemberF* mf = MF + GT.mem0;
-        group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        if (E.ucut) {
+            if (gi == 0) all_cutoffs(S, n, ctype, g, tanhPre);
+        } else {
+            group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        }
         if (GT.type == 2) {
             // radial: each lane owns its neighbours, no conflicts
             for (int k = lane; k < n; k += kWave) {
@@ -926,7 +947,11 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
         const GroupF g = group_f(GT);
         const int nmem = GT.nmem;
         const MemberF* mf = MF + GT.mem0;
-        group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        if (E.ucut) {
+            if (gi == 0) all_cutoffs(S, n, ctype, g, tanhPre);
+        } else {
+            group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
+        }
         if (GT.type == 2) {
             for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
                 float acc[kRadChunk], cmp[kRadChunk];

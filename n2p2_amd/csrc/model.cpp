@@ -272,6 +272,9 @@ void load_model(const std::string& dir, HostModel& hm)
         E.grp0 = gcount;
         E.ngrp = (int)groups.size();
         E.nrad = 0;
+        E.ucut = 1;
+        for (auto& g : groups)
+            if (v[g[0]].rc != v[groups[0][0]].rc) E.ucut = 0;
         for (auto& g : groups) {
             const SF& h = v[g[0]];
             if ((int)g.size() > kMaxMembers) fail(NNPX_ERR_UNSUPPORTED, "SF group too large");

@@ -74,7 +74,7 @@ struct ElementT {
     int grp0;              // first group in ModelT::grp
     int nrad;              // radial groups
     int sf0;               // first SF in ModelT::scale
-    int pad;
+    int ucut;              // 1: every group of the element has the same cutoff radius
     int nsfd[kMaxElem];    // derivative slots per neighbour element
     int tab0[kMaxElem];    // first entry of the slot->SF table per neighbour element
     NNT nn;
