@@ -167,7 +167,7 @@ struct Cutoff {
 enum Scaling { ST_NONE, ST_SCALE, ST_CENTER, ST_SCALECENTER, ST_SCALESIGMA };
 
 struct SF {
-    int type = 0;  // 2 radial, 3 angular (narrow)
+    int type = 0;  // 2 radial, 3 angular (narrow), 9 angular (wide)
     int ec = 0, e1 = 0, e2 = 0;
     double eta = 0, rs = 0, rc = 0, lambda = 0, zeta = 0;
     int zetaInt = 0;
@@ -441,7 +441,7 @@ void load_model(oracle_ctx& c, const string& dir)
             s.eta = atof(a.at(3).c_str());
             s.rs = atof(a.at(4).c_str());
             s.rc = atof(a.at(5).c_str());
-        } else if (s.type == 3) {
+        } else if (s.type == 3 || s.type == 9) {  // SymFncExpAngw.cpp parameters match type 3
             s.e1 = eidx(a.at(2));
             s.e2 = eidx(a.at(3));
             s.eta = atof(a.at(4).c_str());
@@ -502,7 +502,7 @@ void load_model(oracle_ctx& c, const string& dir)
         }
         // Members are in SF order already (SF sort refines the member sort).
         for (auto& g : e.grps) {
-            if (g.type != 3) continue;
+            if (g.type == 2) continue;  // SymGrpExpAngn/SymGrpExpAngw.cpp:108-148 (same member set-up)
             for (size_t i = 0; i < g.mem.size(); ++i) {
                 const SF& s = e.sfs[g.mem[i]];
                 g.factorNorm.push_back(pow(2.0, 1.0 - s.zeta));
@@ -577,7 +577,7 @@ void load_model(oracle_ctx& c, const string& dir)
             g.scalingFactors.clear();
             for (size_t i = 0; i < g.mem.size(); ++i) {
                 g.scalingFactors.push_back(e.sfs[g.mem[i]].scalingFactor);
-                if (g.type == 3) g.factorNorm[i] *= g.scalingFactors[i];
+                if (g.type != 2) g.factorNorm[i] *= g.scalingFactors[i];
             }
         }
     }
@@ -954,6 +954,90 @@ void calc_angular(const Element& e, const Group& g, Atom& a)
     }
 }
 
+// SymGrpExpAngw::calculate (SymGrpExpAngw.cpp:156-380): every pair j < k of
+// the group's elements inside rc, no r_jk cutoff and no r_jk term.
+void calc_angular_wide(const Element& e, const Group& g, Atom& a)
+{
+    const size_t M = g.mem.size();
+    vector<double> result(M, 0.0);
+    double const rc = g.rc;
+    size_t nn = a.nbr.size();
+    if (nn == 0) nn = 1;
+    for (size_t j = 0; j + 1 < nn; ++j) {
+        Neighbor& nj = a.nbr[j];
+        int const nej = nj.element;
+        double const rij = nj.d;
+        if (!((g.e1 == nej || g.e2 == nej) && rij < rc)) continue;
+        double const r2ij = rij * rij;
+        double pfcij, pdfcij;
+        g.fc.fdf(rij, pfcij, pdfcij);
+        const double* dr1 = nj.dr;
+        for (size_t k = j + 1; k < nn; ++k) {
+            Neighbor& nk = a.nbr[k];
+            int const nek = nk.element;
+            if (!((g.e1 == nej && g.e2 == nek) || (g.e2 == nej && g.e1 == nek))) continue;
+            double const rik = nk.d;
+            if (!(rik < rc)) continue;
+            const double* dr2 = nk.dr;
+            double const dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1], dr2[2] - dr1[2]};
+            double pfcik, pdfcik;
+            g.fc.fdf(rik, pfcik, pdfcik);
+            double const rinvijik = 1.0 / rij / rik;
+            double const costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
+            double const pfc = pfcij * pfcik;
+            double const r2ik = rik * rik;
+            double const r2sum = r2ij + r2ik;
+            double const pr1 = pfcik * pdfcij / rij;
+            double const pr2 = pfcij * pdfcik / rik;
+            double vexp = 0.0, rijs = 0.0, riks = 0.0;
+            for (size_t l = 0; l < M; ++l) {
+                const SF& s = e.sfs[g.mem[l]];
+                if (g.calcExp[l]) {
+                    if (s.rs > 0.0) {
+                        rijs = rij - s.rs;
+                        riks = rik - s.rs;
+                        vexp = exp(-s.eta * (rijs * rijs + riks * riks));
+                    } else {
+                        vexp = exp(-s.eta * r2sum);
+                    }
+                }
+                double const plambda = 1.0 + s.lambda * costijk;
+                double fg = vexp;
+                if (plambda <= 0.0) fg = 0.0;
+                else if (s.useIntPow) fg *= pow_int(plambda, s.zetaInt - 1);
+                else fg *= pow(plambda, s.zeta - 1.0);
+                result[l] += fg * plambda * pfc;
+                fg *= g.factorNorm[l];
+                double const pfczl = pfc * g.zetaLambda[l];
+                double const p2etapl = plambda * g.factorDeriv[l];
+                double p1, p2;
+                if (s.rs > 0.0) {
+                    p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl * rijs / rij) + pr1 * plambda);
+                    p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl * riks / rik) + pr2 * plambda);
+                } else {
+                    p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl) + pr1 * plambda);
+                    p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl) + pr2 * plambda);
+                }
+                double const p3 = fg * pfczl * rinvijik;
+                double const p1x = p1 * dr1[0], p1y = p1 * dr1[1], p1z = p1 * dr1[2];
+                double const p2x = p2 * dr2[0], p2y = p2 * dr2[1], p2z = p2 * dr2[2];
+                double const p3x = p3 * dr3[0], p3y = p3 * dr3[1], p3z = p3 * dr3[2];
+                double* d = &a.dGdr[3 * s.index];
+                d[0] += p1x + p2x; d[1] += p1y + p2y; d[2] += p1z + p2z;
+                d = &nj.dGdr[3 * s.indexPerElement[nej]];
+                d[0] -= p1x + p3x; d[1] -= p1y + p3y; d[2] -= p1z + p3z;
+                d = &nk.dGdr[3 * s.indexPerElement[nek]];
+                d[0] -= p2x - p3x; d[1] -= p2y - p3y; d[2] -= p2z - p3z;
+            }
+        }
+    }
+    for (size_t l = 0; l < M; ++l) {
+        const SF& s = e.sfs[g.mem[l]];
+        result[l] *= g.factorNorm[l] / g.scalingFactors[l];
+        a.G[s.index] = s.scale(result[l]);
+    }
+}
+
 // Mode::calculateSymmetryFunctionGroups (Mode.cpp:1630-1709) with
 // Atom::allocate (Atom.cpp:168-240).
 void calc_sf(oracle_ctx& c)
@@ -970,6 +1054,7 @@ void calc_sf(oracle_ctx& c)
         for (auto& n : a.nbr) n.dGdr.assign(3 * e.table[n.element].size(), 0.0);
         for (auto& g : e.grps) {
             if (g.type == 2) calc_radial(e, g, a);
+            else if (g.type == 9) calc_angular_wide(e, g, a);
             else calc_angular(e, g, a);
         }
     }

@@ -12,7 +12,10 @@ What it writes (data only -- numbers and the example's input files):
   ``expected.json`` with the energy and the 1080 force vectors that
   ``test/cpp/ExamplePrediction.h:34-1117`` holds for them (checked by the
   reference at 1000*DBL_EPSILON, ``test/cpp/test_Prediction.cpp:25,44-56``).
-* ``sf_kat.json`` -- the type-2/type-3 symmetry-function known answers of
+* ``Cu2S_PBE/`` -- the same for ``examples/nnp-predict/Cu2S_PBE`` and
+  ``ExamplePrediction.h:1118-1265`` (SF types 2, 3 and 9, CT_POLY2 cutoff,
+  SCALESIGMA scaling, softplus networks).
+* ``sf_kat.json`` -- the type-2/3/9 symmetry-function known answers of
   ``test/cpp/fixtures.h:151-258`` with the cluster geometries of
   ``fixtures.h:11-149`` (values for atom 0, CT_TANHU, no scaling, 1000*eps).
 * ``cutoff_kat.json`` -- f and df of all cutoff types at rc=3, alpha=0.2,
@@ -39,24 +42,32 @@ def block(text, name):
     return m.group(1)
 
 
-def main(ref):
-    ex = os.path.join(ref, "examples", "nnp-predict", "H2O_RPBE-D3")
-    out = os.path.join(HERE, "H2O_RPBE-D3")
+def example(ref, name, count):
+    """Copy an nnp-predict example's files and write its golden energy and
+    forces (the ExamplePrediction.h block for `name`)."""
+    ex = os.path.join(ref, "examples", "nnp-predict", name)
+    out = os.path.join(HERE, name)
     os.makedirs(out, exist_ok=True)
-    for fn in ("input.nn", "scaling.data", "weights.001.data", "weights.008.data", "input.data"):
-        shutil.copyfile(os.path.join(ex, fn), os.path.join(out, fn))
-
+    for fn in sorted(os.listdir(ex)):
+        if fn in ("input.nn", "scaling.data", "input.data") or re.match(r"weights\.\d{3}\.data$", fn):
+            shutil.copyfile(os.path.join(ex, fn), os.path.join(out, fn))
     text = open(os.path.join(ref, "test", "cpp", "ExamplePrediction.h")).read()
-    start = text.index('ExamplePrediction("H2O_RPBE-D3")')
-    stop = text.index("examples.push_back", start + 10)
-    part = text[start:stop]
+    start = text.index('ExamplePrediction("%s")' % name)
+    stop = text.find("examples.push_back", start + 10)
+    part = text[start:stop if stop > 0 else len(text)]
+    line0 = text[:start].count("\n") + 1
     energy = float(re.search(r"e->energy\s*=\s*([-+0-9.eE]+);", part).group(1))
     forces = [floats_in(m) for m in re.findall(r"Vec3D\(([^)]*)\)", part)]
-    assert len(forces) == 1080 and all(len(f) == 3 for f in forces)
-    json.dump({"source": "test/cpp/ExamplePrediction.h:34-1117",
+    assert len(forces) == count and all(len(f) == 3 for f in forces), (name, len(forces))
+    json.dump({"source": "test/cpp/ExamplePrediction.h:%d-%d" % (line0, line0 + part.count("\n") - 1),
                "tolerance": "1000*DBL_EPSILON absolute (test/cpp/test_Prediction.cpp:25)",
                "energy": energy, "forces": forces},
               open(os.path.join(out, "expected.json"), "w"), indent=0)
+
+
+def main(ref):
+    example(ref, "H2O_RPBE-D3", 1080)
+    example(ref, "Cu2S_PBE", 144)
 
     fx = open(os.path.join(ref, "test", "cpp", "fixtures.h")).read()
     kat = {"source": "test/cpp/fixtures.h:11-258; test/cpp/test_SymmetryFunctions.cpp:36,68-74",
@@ -73,7 +84,7 @@ def main(ref):
         lines = re.findall(r'"([^"]*)"', block(fx, "setupLines" + name))
         vals = floats_in(block(fx, "values" + name))
         assert len(types) == len(lines) == len(vals)
-        rows = [{"type": t, "line": l, "value": v} for t, l, v in zip(types, lines, vals) if t in (2, 3)]
+        rows = [{"type": t, "line": l, "value": v} for t, l, v in zip(types, lines, vals) if t in (2, 3, 9)]
         kat["sets"][name] = dict(geoms[name], functions=rows)
     json.dump(kat, open(os.path.join(HERE, "sf_kat.json"), "w"), indent=1)
 

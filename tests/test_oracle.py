@@ -15,9 +15,12 @@ GOLD = os.path.join(HERE, "golden")
 EPS = np.finfo(float).eps
 
 
-def test_h2o_golden_energy_and_forces():
-    """test/cpp/test_Prediction.cpp:25,44-56: E and all 1080 forces at 1000*eps."""
-    d = os.path.join(GOLD, "H2O_RPBE-D3")
+@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+def test_golden_energy_and_forces(name):
+    """test/cpp/test_Prediction.cpp:25,44-56: E and all forces at 1000*eps
+    (H2O_RPBE-D3: SF types 2/3, CT_TANHU; Cu2S_PBE: types 2/3/9, CT_POLY2,
+    SCALESIGMA, softplus)."""
+    d = os.path.join(GOLD, name)
     exp = json.load(open(os.path.join(d, "expected.json")))
     o = Oracle(d)
     o.read_structure(os.path.join(d, "input.data"))
@@ -184,7 +187,7 @@ def _water_dir():
 
 
 @pytest.mark.skipif(not ref_available(), reason="reference compute classes not built (no reference tree)")
-@pytest.mark.parametrize("case", ["golden", "water192", "small_cell", "fcc108"])
+@pytest.mark.parametrize("case", ["golden", "water192", "small_cell", "fcc108", "cu2s"])
 def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path):
     """The oracle reproduces n2p2's own SymGrp/NeuralNetwork/Atom code bit for bit."""
     from n2p2_amd import read_input_data, synth
@@ -194,6 +197,9 @@ def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path
     elif case == "water192":
         d = water_model_dir
         pos, elem, box = synth.water_box(4)
+    elif case == "cu2s":
+        d = os.path.join(GOLD, "Cu2S_PBE")
+        pos, elem, box = read_input_data(os.path.join(d, "input.data"), ["S", "Cu"])
     elif case == "fcc108":
         d = synth.fcc_model(str(tmp_path / "fcc"))
         pos, elem, box = synth.fcc_lattice(n_cells=3)
