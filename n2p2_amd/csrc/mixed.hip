@@ -472,19 +472,22 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
         for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
         {
             const bool same = GT.e1 == GT.e2;
+        const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
             const GroupMixF gm = GM[E.grp0 + gi];
             const MemberMasks R = load_members(S, mf, nmem, nullptr);
-            for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
+            for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
                 const float4 gj = S.g[j], gk = S.g[k];
                 const float4 cj = S.c[j], ck = S.c[k];
                 const float rjk = sqrtf(r2jk);
                 float fcjk, dfcjk;
-                cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
+                fcjk = 1.f;
+                dfcjk = 0.f;
+                if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
                 const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
                 const float lpfc = flog2(cj.y * ck.y * fcjk);
-                const float r2sum = gj.w * gj.w + gk.w * gk.w + r2jk;
+                const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
                 constexpr int MAXS = FAST ? 2 : 4;
                 float L[MAXS];
 #pragma unroll
@@ -511,7 +514,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                                 float arg = r2sum;
                                 if (mq.z > 0.f) {
                                     const float a = gj.w - mq.z, b = gk.w - mq.z, c = rjk - mq.z;
-                                    arg = a * a + b * b + c * c;
+                                    arg = a * a + b * b + (wide ? 0.f : c * c);
                                 }
                                 ec = fmaf(-mq.y, arg, lpfc);
                             }
@@ -768,6 +771,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             continue;
         }
         const bool same = GT.e1 == GT.e2;
+        const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
         const int nA = make_list(S, n, GT.e1, g.rc, S.la);
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
@@ -776,17 +780,19 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         // lane's run of equal j and flushed when j changes
         int cur = -1;
         double rx = 0.0, ry = 0.0, rz = 0.0;
-        for_each_triplet_rows(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
+        for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
             const float rjk = sqrtf(r2jk);
             const float ijk = 1.f / rjk;
             float fcjk, dfcjk;
-            cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
+            fcjk = 1.f;
+            dfcjk = 0.f;
+            if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
             const float rinvijik = cj.x * ck.x;
             const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * rinvijik;
             const float pfc = cj.y * ck.y * fcjk;
-            const float r2sum = gj.w * gj.w + gk.w * gk.w + r2jk;
+            const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
             const float pr1 = ck.y * fcjk * cj.z * cj.x;
             const float pr2 = cj.y * fcjk * ck.z * ck.x;
             const float pr3 = cj.y * ck.y * dfcjk * ijk;
@@ -827,10 +833,10 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                             shifted = mq.z > 0.f;
                             if (shifted) {
                                 const float a = gj.w - mq.z, b = gk.w - mq.z, c = rjk - mq.z;
-                                arg = a * a + b * b + c * c;
+                                arg = a * a + b * b + (wide ? 0.f : c * c);
                                 fij = a * cj.x;
                                 fik = b * ck.x;
-                                fjk = c * ijk;
+                                fjk = wide ? 0.f : c * ijk;
                             }
                             ec = -mq.y * arg;
                         }
@@ -850,7 +856,8 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                         A = fmaf(wq.x, t * Rs, A);
                     }
             }
-            C1 += C0; C2 += C0; C3 += C0;
+            C1 += C0; C2 += C0;
+            if (!wide) C3 += C0;
             const float P1 = pfc * (A * (rinvijik - cost * cj.x * cj.x) - C1) + B * pr1;
             const float P2 = pfc * (A * (rinvijik - cost * ck.x * ck.x) - C2) + B * pr2;
             const float P3 = pfc * (A * rinvijik + C3) - B * pr3;
@@ -987,22 +994,25 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
 #pragma unroll
         for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
         const bool same = GT.e1 == GT.e2;
+        const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
         const int nA = make_list(S, n, GT.e1, g.rc, S.la);
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
         const MemberMasks R = load_members(S, mf, nmem, nullptr);
-        for_each_triplet(S, nA, nB, same, g.rc2, [&](int j, int k, float r2jk) {
+        for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
             const float4 uj = du[j], uk = du[k];
             const float rjk = sqrtf(r2jk);
             const float ijk = 1.f / rjk;
             float fcjk, dfcjk;
-            cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
+            fcjk = 1.f;
+            dfcjk = 0.f;
+            if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
             const float rinvijik = cj.x * ck.x;
             const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * rinvijik;
             const float pfc = cj.y * ck.y * fcjk;
-            const float r2sum = gj.w * gj.w + gk.w * gk.w + r2jk;
+            const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
             const float pr1 = ck.y * fcjk * cj.z * cj.x;
             const float pr2 = cj.y * fcjk * ck.z * ck.x;
             const float pr3 = cj.y * ck.y * dfcjk * ijk;
@@ -1013,7 +1023,7 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
             const float al = pfc * ((rinvijik - cost * cj.x * cj.x) * s1 + (rinvijik - cost * ck.x * ck.x) * s2 +
                                     rinvijik * s3);
             const float be = pr1 * s1 + pr2 * s2 - pr3 * s3;
-            const float ga = pfc * (s3 - s1 - s2);
+            const float ga = pfc * ((wide ? 0.f : s3) - s1 - s2);
             constexpr int MAXS = FAST ? 2 : 4;
             float L[MAXS], Rp[MAXS];
 #pragma unroll
@@ -1045,10 +1055,10 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
                             shifted = mq.z > 0.f;
                             if (shifted) {
                                 const float a = gj.w - mq.z, b = gk.w - mq.z, c = rjk - mq.z;
-                                arg = a * a + b * b + c * c;
+                                arg = a * a + b * b + (wide ? 0.f : c * c);
                                 fij = a * cj.x;
                                 fik = b * ck.x;
-                                fjk = c * ijk;
+                                fjk = wide ? 0.f : c * ijk;
                             }
                             ec = -mq.y * arg;
                         }
@@ -1153,14 +1163,14 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 f.rs = (float)M.rs;
                 f.lambda = (float)M.lambda;
                 f.zeta = (float)M.zeta;
-                f.gnorm = G.type == 3 ? (float)M.gnorm : 1.f;
+                f.gnorm = G.type != 2 ? (float)M.gnorm : 1.f;
                 f.twoEta = (float)(2.0 * M.eta);
                 f.zetaLambda = (float)M.zetaLambda;
                 f.etaL = (float)(M.eta * 1.4426950408889634);  // eta * log2(e)
                 f.sf = M.sf;
                 f.calcExp = M.calcExp;
                 f.sidx = 0;
-                if (G.type == 3) {
+                if (G.type != 2) {
                     if (M.rs > 0.0) mm.hasRs = true;
                     int k = 0;
                     while (k < gm.nlam && gm.lam[k] != f.lambda) ++k;
@@ -1170,7 +1180,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                     }
                     f.sidx = k;
                 }
-                sffac[(size_t)e * kMaxSF + M.sf] = (float)(G.type == 3 ? M.factorNorm : M.sfac);
+                sffac[(size_t)e * kMaxSF + M.sf] = (float)(G.type != 2 ? M.factorNorm : M.sfac);
             }
             mm.maxS = std::max(mm.maxS, gm.nlam);
             // unused table slots repeat the first lambda (evaluated, never picked)
@@ -1184,7 +1194,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
         int cnt = -1;
         bool same = true;
         for (int g = 0; g < t.ngroups; ++g)
-            if (t.grp[g].type == 3) {
+            if (t.grp[g].type != 2) {
                 if (cnt < 0) cnt = t.grp[g].nmem;
                 same = same && t.grp[g].nmem == cnt;
             }

@@ -197,7 +197,7 @@ void load_model(const std::string& dir, HostModel& hm)
             f.eta = num(w.at(3));
             f.rs = num(w.at(4));
             f.rc = num(w.at(5));
-        } else if (f.type == 3) {
+        } else if (f.type == 3 || f.type == 9) {  // narrow / wide angular, same parameters
             f.e1 = eidx(w.at(2));
             f.e2 = eidx(w.at(3));
             f.eta = num(w.at(4));
@@ -210,7 +210,7 @@ void load_model(const std::string& dir, HostModel& hm)
             f.useIntPow = std::fabs(f.zeta - f.zetaInt) <= std::numeric_limits<double>::min();
         } else {
             fail(NNPX_ERR_UNSUPPORTED, "symmetry function type " + w.at(1) +
-                                           " not supported (types 2 and 3 are)");
+                                           " not supported (types 2, 3 and 9 are)");
         }
         if (hm.normalize) {  // SymFnc*::changeLengthUnit
             f.eta /= hm.convLength * hm.convLength;
@@ -309,7 +309,7 @@ void load_model(const std::string& dir, HostModel& hm)
                 M.rs = f.rs;
                 M.lambda = f.lambda;
                 M.zeta = f.zeta;
-                if (f.type == 3) {  // SymGrpExpAngn::sortMembers (SymGrpExpAngn.cpp:108-148)
+                if (f.type != 2) {  // SymGrpExpAngn/SymGrpExpAngw::sortMembers (SymGrpExpAngn.cpp:108-148)
                     M.factorNorm = std::pow(2.0, 1.0 - f.zeta);
                     M.factorDeriv = 2.0 * f.eta / f.zeta / f.lambda;
                     M.zetaLambda = f.zeta * f.lambda;
@@ -417,7 +417,7 @@ void load_model(const std::string& dir, HostModel& hm)
             for (int q = G.mem0; q < G.mem0 + G.nmem; ++q) {
                 MemberT& M = t.mem[q];
                 M.sfac = t.scale[E.sf0 + M.sf].factor;
-                if (G.type == 3) {
+                if (G.type != 2) {
                     M.factorNorm *= M.sfac;
                     M.gnorm = M.factorNorm / M.sfac;
                 }

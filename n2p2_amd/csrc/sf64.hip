@@ -161,6 +161,9 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
         const GroupT& G = M->grp[E.grp0 + gi];
         const int nmem = G.nmem;
         const bool same = G.e1 == G.e2;
+        // wide angular groups (SymGrpExpAngw.cpp:156-380): no r_jk cutoff and
+        // no r_jk term in the exponent
+        const bool wide = G.type == 9;
         const int nA = build_list(L, n, G.e1, G.rc, L.la);
         const int nB = same ? nA : build_list(L, n, G.e2, G.rc, L.lb);
         const int* lb = same ? L.la : L.lb;
@@ -208,22 +211,30 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
             const double dr2[3] = {L.dr[k], L.dr[capL + k], L.dr[2 * capL + k]};
             const double dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1], dr2[2] - dr1[2]};
             double rjk = dr3[0] * dr3[0] + dr3[1] * dr3[1] + dr3[2] * dr3[2];
-            if (!(rjk < rc2)) continue;
+            if (!wide && !(rjk < rc2)) continue;
             const double rij = L.d[j], rik = L.d[k];
             const double pfcij = L.fc[j], pdfcij = L.dfc[j];
             const double pfcik = L.fc[k], pdfcik = L.dfc[k];
-            rjk = sqrt(rjk);
-            double pfcjk, pdfcjk;
-            cutoff_fdf(ctype, G, tanhPre, rjk, pfcjk, pdfcjk);
             const double r2ij = rij * rij;
             const double rinvijik = 1.0 / rij / rik;
             const double costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
-            const double pfc = pfcij * pfcik * pfcjk;
             const double r2ik = rik * rik;
-            const double r2sum = r2ij + r2ik + rjk * rjk;
-            const double pr1 = pfcik * pfcjk * pdfcij / rij;
-            const double pr2 = pfcij * pfcjk * pdfcik / rik;
-            const double pr3 = pfcij * pfcik * pdfcjk / rjk;
+            double pfc, r2sum, pr1, pr2, pr3 = 0.0;
+            if (wide) {
+                pfc = pfcij * pfcik;
+                r2sum = r2ij + r2ik;
+                pr1 = pfcik * pdfcij / rij;
+                pr2 = pfcij * pdfcik / rik;
+            } else {
+                rjk = sqrt(rjk);
+                double pfcjk, pdfcjk;
+                cutoff_fdf(ctype, G, tanhPre, rjk, pfcjk, pdfcjk);
+                pfc = pfcij * pfcik * pfcjk;
+                r2sum = r2ij + r2ik + rjk * rjk;
+                pr1 = pfcik * pfcjk * pdfcij / rij;
+                pr2 = pfcij * pfcjk * pdfcik / rik;
+                pr3 = pfcij * pfcik * pdfcjk / rjk;
+            }
             double vexp = 0.0, rijs = 0.0, riks = 0.0, rjks = 0.0;
 #pragma unroll
             for (int q = 0; q < MAXM; ++q) {
@@ -233,8 +244,12 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                     if (mb.rs > 0.0) {
                         rijs = rij - mb.rs;
                         riks = rik - mb.rs;
-                        rjks = rjk - mb.rs;
-                        vexp = exp(-mb.eta * (rijs * rijs + riks * riks + rjks * rjks));
+                        if (wide) {
+                            vexp = exp(-mb.eta * (rijs * rijs + riks * riks));
+                        } else {
+                            rjks = rjk - mb.rs;
+                            vexp = exp(-mb.eta * (rijs * rijs + riks * riks + rjks * rjks));
+                        }
                     } else {
                         vexp = exp(-mb.eta * r2sum);
                     }
@@ -253,11 +268,12 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                 if (mb.rs > 0.0) {
                     p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl * rijs / rij) + pr1 * plambda);
                     p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl * riks / rik) + pr2 * plambda);
-                    p3 = fg * (pfczl * (rinvijik + p2etapl * rjks / rjk) - pr3 * plambda);
+                    p3 = wide ? fg * pfczl * rinvijik
+                              : fg * (pfczl * (rinvijik + p2etapl * rjks / rjk) - pr3 * plambda);
                 } else {
                     p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl) + pr1 * plambda);
                     p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl) + pr2 * plambda);
-                    p3 = fg * (pfczl * (rinvijik + p2etapl) - pr3 * plambda);
+                    p3 = wide ? fg * pfczl * rinvijik : fg * (pfczl * (rinvijik + p2etapl) - pr3 * plambda);
                 }
                 for (int c = 0; c < 3; ++c) {
                     const double p1c = p1 * dr1[c], p2c = p2 * dr2[c], p3c = p3 * dr3[c];

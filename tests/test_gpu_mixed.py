@@ -85,6 +85,24 @@ def test_water12288_mixed(water_dir):
     assert erel <= 1e-5
 
 
+def test_golden_cu2s_mixed():
+    """Cu2S_PBE golden (wide angular type 9, CT_POLY2, SCALESIGMA, softplus)
+    in the mixed form against the reference's forces."""
+    import json
+    say("case: Cu2S_PBE mixed")
+    gdir = os.path.join(HERE, "golden", "Cu2S_PBE")
+    pos, elem, box = read_input_data(os.path.join(gdir, "input.data"), ["S", "Cu"])
+    m = n2p2_amd.Mode(gdir)
+    cl, ce = m.conv_length, m.conv_energy
+    m.set_structures(remap(pos, box) * cl, elem, (box * cl)[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    F = m.forces() * (1.0 / (ce / cl))
+    Fg = np.array(json.load(open(os.path.join(gdir, "expected.json")))["forces"])
+    rms = float(np.sqrt(np.mean((F - Fg) ** 2)))
+    say("  F rms %.2e max %.2e (|F| rms %.3e)" % (rms, np.abs(F - Fg).max(), np.sqrt(np.mean(Fg ** 2))))
+    assert rms <= F_RMS
+
+
 def test_golden_h2o_mixed():
     """The H2O_RPBE-D3 model (CT_TANHU, rs > 0 radial terms, normalised units)."""
     say("case: H2O_RPBE-D3 mixed")

@@ -31,13 +31,16 @@ def water_dir(tmp_path_factory):
     return synth.water_model(str(tmp_path_factory.mktemp("water")))
 
 
-def predict_golden(mode):
+GOLDEN = {"H2O_RPBE-D3": ["H", "O"], "Cu2S_PBE": ["S", "Cu"]}
+
+
+def predict_golden(mode, gdir=GOLD, symbols=("H", "O"), precision=n2p2_amd.PLAIN_FP64):
     """Prediction::readStructureFromFile + predict (Prediction.cpp:50-78) on the GPU."""
-    pos, elem, box = read_input_data(os.path.join(GOLD, "input.data"), ["H", "O"])
+    pos, elem, box = read_input_data(os.path.join(gdir, "input.data"), list(symbols))
     pos = remap(pos, box)
     cl, ce = mode.conv_length, mode.conv_energy
     mode.set_structures(pos * cl, elem, (box * cl)[None])
-    mode.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    mode.evaluate(precision, forces=True)
     n = len(elem)
     E = mode.energies()[0] / ce + n * mode.mean_energy
     E += sum(np.count_nonzero(elem == e) * mode.energy_offsets[e] for e in range(mode.num_elements))
@@ -45,11 +48,15 @@ def predict_golden(mode):
     return E, F
 
 
-def test_golden_h2o_plain_fp64():
-    say("case: H2O_RPBE-D3 golden, plain FP64")
-    exp = json.load(open(os.path.join(GOLD, "expected.json")))
-    m = n2p2_amd.Mode(GOLD)
-    E, F = predict_golden(m)
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_golden_plain_fp64(name):
+    """test/cpp/test_Prediction.cpp examples: H2O_RPBE-D3 (SF types 2/3,
+    CT_TANHU) and Cu2S_PBE (types 2/3/9, CT_POLY2, SCALESIGMA, softplus)."""
+    say("case: %s golden, plain FP64" % name)
+    gdir = os.path.join(HERE, "golden", name)
+    exp = json.load(open(os.path.join(gdir, "expected.json")))
+    m = n2p2_amd.Mode(gdir)
+    E, F = predict_golden(m, gdir, GOLDEN[name])
     Fg = np.array(exp["forces"])
     dE = abs(E - exp["energy"])
     dF = np.abs(F - Fg).max()
