@@ -103,6 +103,10 @@ __device__ inline void cutoff32(int type, float rc, float rci, float iw, float r
         fc = (x * (x * ((315.f - 70.f * x) * x - 540.f) + 420.f) - 126.f) * x * x4 + 1.f;
         dfc = x4 * (x * (x * ((2520.f - 630.f * x) * x - 3780.f) + 2520.f) - 630.f);
     }
+    // The polynomials vanish at x = 1; in FP32 they can round to tiny
+    // negative values just below it, which the log2 form of the angular
+    // terms must not see.
+    fc = fmaxf(fc, 0.f);
     dfc *= iw;
 }
 

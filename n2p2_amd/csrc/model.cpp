@@ -251,9 +251,9 @@ void load_model(const std::string& dir, HostModel& hm)
             t.max_sfd = std::max(t.max_sfd, E.nsfd[n]);
         }
         // Groups (Element::setupSymmetryFunctionGroups, Element.cpp:194-287):
-        // radial groups first, angular after; members in SF order.
+        // radial groups first, then narrow and wide angular; members in SF order.
         std::vector<std::vector<int>> groups;
-        for (int pass = 2; pass <= 3; ++pass) {
+        for (int pass : {2, 3, 9}) {
             for (auto& f : v) {
                 if (f.type != pass) continue;
                 bool added = false;
