@@ -192,6 +192,13 @@ def _parse(line, symbols):
     w = line.split()
     idx = {s: i for i, s in enumerate(symbols)}
     t = int(w[1])
+    if t == 20:
+        return dict(type=20, ec=idx[w[0]], e1=idx[w[2]], e2=idx[w[2]], rl=float(w[3]), rc=float(w[4]),
+                    sub=w[5], al=0.0, ar=0.0)
+    if t in (21, 22):
+        e1, e2 = sorted((idx[w[2]], idx[w[3]]))
+        return dict(type=t, ec=idx[w[0]], e1=e1, e2=e2, rl=float(w[4]), rc=float(w[5]), al=float(w[6]),
+                    ar=float(w[7]), sub=w[8])
     if t == 2:
         return dict(type=2, ec=idx[w[0]], e1=idx[w[2]], e2=idx[w[2]], eta=float(w[3]), rs=float(w[4]),
                     rc=float(w[5]), lam=0.0, zeta=0.0)
@@ -200,15 +207,23 @@ def _parse(line, symbols):
                 rc=float(w[7]), rs=float(w[8]) if len(w) > 8 else 0.0)
 
 
+def _sort_key(s):
+    if s["type"] >= 20:  # SymFncCompRad.cpp:48-68, SymFncCompAngw.cpp:56-82
+        return (s["type"], s["sub"], s["e1"], s["e2"], s["rc"], s["rl"], s["al"], s["ar"])
+    if s["type"] == 2:
+        return (s["type"], s["rc"], s["eta"], s["rs"], s["e1"])
+    return (s["type"], s["rc"], s["eta"], s["rs"], s["zeta"], s["lam"], s["e1"], s["e2"])
+
+
 def sorted_sfs(lines, symbols):
     """SFs per element in n2p2's sorted order (SymFncExpRad/SymFncExpAngn
-    operator<, SymFncExpRad.cpp:57-75, SymFncExpAngn.cpp:51-77)."""
+    operator<, SymFncExpRad.cpp:57-75, SymFncExpAngn.cpp:51-77; compact
+    types by subtype, elements, rc, rl, angles)."""
     sfs = [_parse(l, symbols) for l in lines]
     out = []
     for e in range(len(symbols)):
         mine = [s for s in sfs if s["ec"] == e]
-        mine.sort(key=lambda s: (s["type"], s["rc"], s["eta"], s["rs"], s["e1"]) if s["type"] == 2 else
-                  (s["type"], s["rc"], s["eta"], s["rs"], s["zeta"], s["lam"], s["e1"], s["e2"]))
+        mine.sort(key=_sort_key)
         out.append(mine)
     return out
 
@@ -269,10 +284,15 @@ def sf_values(sfs, pos, elem, box, atoms=None):
 def scaling_rows(sfs, pos, elem, box, atoms=None):
     """nnp-scaling statistics rows (e_index, sf_index, min, max, mean, sigma)."""
     vals = sf_values(sfs, pos, elem, box, atoms)
+    return rows_from_values([np.array([v for i, v in vals.items() if elem[i] == e]) for e in range(len(sfs))])
+
+
+def rows_from_values(mats):
+    """Scaling rows from per-element matrices of unscaled SF values
+    (atoms x functions)."""
     rows = []
-    for e in range(len(sfs)):
-        mat = np.array([v for i, v in vals.items() if elem[i] == e])
-        for k in range(len(sfs[e])):
+    for e, mat in enumerate(mats):
+        for k in range(mat.shape[1]):
             col = mat[:, k]
             mx = col.max()
             mn = col.min()
@@ -283,7 +303,7 @@ def scaling_rows(sfs, pos, elem, box, atoms=None):
 
 
 def write_model(directory, symbols, sf_lines, hidden, activations, rows, seed=SEED_WEIGHTS, cutoff_type="1 0.0",
-                weights=None):
+                weights=None, scaling=True):
     """Write input.nn, scaling.data and weights.%03d.data.
 
     Weights and biases ~ U(-1/sqrt(fan_in), +1/sqrt(fan_in)) drawn per element
@@ -294,7 +314,8 @@ def write_model(directory, symbols, sf_lines, hidden, activations, rows, seed=SE
         f.write("number_of_elements %d\n" % len(symbols))
         f.write("elements %s\n" % " ".join(symbols))
         f.write("cutoff_type %s\n" % cutoff_type)
-        f.write("scale_symmetry_functions\ncenter_symmetry_functions\n")
+        if scaling:
+            f.write("scale_symmetry_functions\ncenter_symmetry_functions\n")
         f.write("scale_min_short 0.0\nscale_max_short 1.0\n")
         f.write("global_hidden_layers_short %d\n" % len(hidden))
         f.write("global_nodes_short %s\n" % " ".join(str(h) for h in hidden))
@@ -327,15 +348,52 @@ def write_model(directory, symbols, sf_lines, hidden, activations, rows, seed=SE
     return all_w
 
 
-def water_model(directory, seed=SEED_WEIGHTS):
-    """The 50-25-25-1 water model, scaling statistics from the seeded 192-atom box."""
+def water_model(directory, seed=SEED_WEIGHTS, cutoff_type="1 0.0", activations=("t", "t", "l")):
+    """The 50-25-25-1 water model, scaling statistics from the seeded 192-atom
+    box (computed with the cosine cutoff whatever ``cutoff_type`` says: they
+    are only the model's scaling constants)."""
     symbols = ("H", "O")
     lines = water_sf_lines()
     if os.path.exists(os.path.join(directory, "weights.008.data")):
         return directory
     pos, elem, box = water_box(4)
     rows = scaling_rows(sorted_sfs(lines, symbols), pos, elem, box)
-    write_model(directory, symbols, lines, (25, 25), ("t", "t", "l"), rows, seed=seed)
+    write_model(directory, symbols, lines, (25, 25), activations, rows, seed=seed, cutoff_type=cutoff_type)
+    return directory
+
+
+def compact_sf_lines():
+    """Water SFs built from compact functions (types 20, 21, 22): every
+    polynomial order, the asymmetric and exponential subtypes, inner bounds
+    below and above zero, angle windows centred at 0 and 180 degrees, and
+    narrow (type 21) members of different subtypes sharing (rl, rc) next to
+    each other in sort order."""
+    lines = []
+    for c, o in (("H", "O"), ("O", "H")):
+        lines += ["%s 20 H -6.0 6.0 p2" % c, "%s 20 H 0.0 3.0 p2a" % c, "%s 20 O 0.5 4.0 p3" % c,
+                  "%s 20 O -6.0 6.0 e" % c, "%s 20 H 1.0 5.0 p4a" % c, "%s 20 O 0.0 6.0 p1" % c,
+                  "%s 20 %s 2.0 5.5 p4" % (c, o)]
+        lines += ["%s 21 H O -6.0 6.0 0.0 180.0 p2" % c, "%s 21 H O -6.0 6.0 60.0 150.0 p2" % c,
+                  "%s 21 H O -6.0 6.0 -90.0 90.0 p2a" % c, "%s 21 H O -6.0 6.0 0.0 180.0 p2a" % c,
+                  "%s 21 H H 0.0 6.0 90.0 270.0 p3" % c, "%s 21 H H 0.0 6.0 0.0 120.0 p3a" % c,
+                  "%s 21 O O -6.0 6.0 30.0 160.0 e" % c]
+        lines += ["%s 22 H O -6.0 6.0 0.0 180.0 p2" % c, "%s 22 H O -6.0 6.0 0.0 180.0 p2a" % c,
+                  "%s 22 O O 1.0 6.0 40.0 140.0 e" % c, "%s 22 H H -6.0 6.0 -90.0 90.0 p4" % c,
+                  "%s 22 H H 0.5 5.0 20.0 100.0 p1" % c, "%s 22 H O -5.0 5.0 90.0 270.0 p3a" % c]
+    return lines
+
+
+def compact_water_model(directory, rows=None, seed=SEED_WEIGHTS):
+    """A water model on compact_sf_lines() (30-30-1 networks, tanh).  With
+    `rows` (nnp-scaling rows for the sorted functions) the functions are
+    scaled and centred; without, unscaled (used to measure the rows)."""
+    symbols = ("H", "O")
+    lines = compact_sf_lines()
+    sfs = sorted_sfs(lines, symbols)
+    scaled = rows is not None
+    if not scaled:
+        rows = [(e + 1, k + 1, 0.0, 1.0, 0.0, 1.0) for e in range(2) for k in range(len(sfs[e]))]
+    write_model(directory, symbols, lines, (30, 30), ("t", "t", "l"), rows, seed=seed, scaling=scaled)
     return directory
 
 

@@ -82,6 +82,71 @@ int atomic_number(const string& s)
     throw std::runtime_error("unknown element symbol \"" + s + "\"");
 }
 
+// ------------------------------------------------------- core functions --
+// CoreFunction.h:214-319 (fdfEXP, fdfPOLY1..4) on x in [0, 1):
+// core 0 exponential, 1..4 the polynomials of increasing order.
+enum Core { CORE_EXP = 0, CORE_POLY1, CORE_POLY2, CORE_POLY3, CORE_POLY4 };
+
+void core_fdf(int core, double x, double& fx, double& dfx)
+{
+    static const double E = exp(1.0);
+    if (core == CORE_EXP) {
+        double const t = 1.0 / (x * x - 1.0);
+        double const t2 = exp(t);
+        fx = E * t2;
+        dfx = -2.0 * E * x * t * t * t2;
+    } else if (core == CORE_POLY1) {
+        fx = (2.0 * x - 3.0) * x * x + 1.0;
+        dfx = x * (6.0 * x - 6.0);
+    } else if (core == CORE_POLY2) {
+        double const x2 = x * x;
+        fx = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
+        dfx = x2 * ((60.0 - 30.0 * x) * x - 30.0);
+    } else if (core == CORE_POLY3) {
+        double const x2 = x * x;
+        fx = (x * (x * (20.0 * x - 70.0) + 84.0) - 35.0) * x2 * x2 + 1.0;
+        dfx = x2 * x * (x * (x * (140.0 * x - 420.0) + 420.0) - 140.0);
+    } else {
+        double x4 = x * x;
+        x4 *= x4;
+        fx = (x * (x * ((315.0 - 70.0 * x) * x - 540.0) + 420.0) - 126.0) *
+             x * x4 + 1.0;
+        dfx = x4 *
+              (x * (x * ((2520.0 - 630.0 * x) * x - 3780.0) + 2520.0) - 630.0);
+    }
+}
+
+// A compact function on [left, right]: the core evaluated at
+// |x - center| / width, optionally through the asymmetric map x -> (2 - x) x
+// (CompactFunction.cpp:58-77 setLeftRight, CompactFunction.h:150-172 fdf,
+// CoreFunction.h:197-212 asymmetric fdf).
+struct Compact {
+    int core = CORE_POLY2;
+    bool asym = false;
+    double center = 0, width = 0, left = 0, right = 0, scale = 0;
+    void set_left_right(double l, double r)
+    {
+        if (l >= r) throw std::invalid_argument("compact function: left >= right");
+        left = l;
+        right = r;
+        center = (r + l) / 2.0;
+        width = (r - l) / 2.0;
+        scale = 1.0 / width;
+    }
+    void fdf(double a, double& fa, double& dfa) const
+    {
+        a = (a - center) * scale;
+        double const x = std::abs(a);
+        if (asym) {
+            core_fdf(core, (2.0 - x) * x, fa, dfa);
+            dfa *= 2.0 - 2.0 * x;
+        } else {
+            core_fdf(core, x, fa, dfa);
+        }
+        dfa *= std::copysign(scale, a);
+    }
+};
+
 // ---------------------------------------------------------------- cutoff --
 // CutoffFunction.cpp:25-28 constants, 95-112 set*, 114-213 f/df,
 // CutoffFunction.h:210-220 fdf guard, CoreFunction.h:214-319 cores.
@@ -135,30 +200,7 @@ struct Cutoff {
         // Polynomial / exponential cores on x in [0, 1).
         if (r < rci) { fc = 1.0; dfc = 0.0; return; }
         double const x = (r - rci) * iw;
-        if (type == 4) {
-            double const t = 1.0 / (x * x - 1.0);
-            double const t2 = exp(t);
-            fc = E * t2;
-            dfc = -2.0 * E * x * t * t * t2;
-        } else if (type == 5) {
-            fc = (2.0 * x - 3.0) * x * x + 1.0;
-            dfc = x * (6.0 * x - 6.0);
-        } else if (type == 6) {
-            double const x2 = x * x;
-            fc = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
-            dfc = x2 * ((60.0 - 30.0 * x) * x - 30.0);
-        } else if (type == 7) {
-            double const x2 = x * x;
-            fc = (x * (x * (20.0 * x - 70.0) + 84.0) - 35.0) * x2 * x2 + 1.0;
-            dfc = x2 * x * (x * (x * (140.0 * x - 420.0) + 420.0) - 140.0);
-        } else {
-            double x4 = x * x;
-            x4 *= x4;
-            fc = (x * (x * ((315.0 - 70.0 * x) * x - 540.0) + 420.0) - 126.0) *
-                 x * x4 + 1.0;
-            dfc = x4 *
-                  (x * (x * ((2520.0 - 630.0 * x) * x - 3780.0) + 2520.0) - 630.0);
-        }
+        core_fdf(type - 4, x, fc, dfc);
         dfc *= iw;
     }
 };
@@ -166,10 +208,21 @@ struct Cutoff {
 // ------------------------------------------------------- symmetry functions --
 enum Scaling { ST_NONE, ST_SCALE, ST_CENTER, ST_SCALECENTER, ST_SCALESIGMA };
 
+// Types: 2 radial, 3 angular (narrow), 9 angular (wide) with Gaussian terms
+// and a cutoff function; 20 radial, 21 angular (narrow), 22 angular (wide)
+// built from compact functions (SymFncCompRad, SymFncCompAngn,
+// SymFncCompAngw).
+inline bool is_compact(int type) { return type >= 20; }
+inline bool is_radial(int type) { return type == 2 || type == 20; }
+
 struct SF {
-    int type = 0;  // 2 radial, 3 angular (narrow), 9 angular (wide)
+    int type = 0;
     int ec = 0, e1 = 0, e2 = 0;
     double eta = 0, rs = 0, rc = 0, lambda = 0, zeta = 0;
+    // compact types: lower radial bound, angle window (degrees), subtype
+    double rl = 0, angleLeft = 0, angleRight = 0;
+    string subtype;
+    Compact cr, ca;  // radial and angular compact functions
     int zetaInt = 0;
     bool useIntPow = false;
     int index = 0;                // sorted index within the element
@@ -181,7 +234,7 @@ struct SF {
 
     bool relevant(int e) const  // SymFnc*::checkRelevantElement
     {
-        return type == 2 ? e == e1 : (e == e1 || e == e2);
+        return is_radial(type) ? e == e1 : (e == e1 || e == e2);
     }
     // SymFnc.cpp:169-195
     double scale(double v) const
@@ -204,6 +257,17 @@ bool sf_less(const SF& a, const SF& b)
 {
     if (a.ec != b.ec) return a.ec < b.ec;
     if (a.type != b.type) return a.type < b.type;
+    if (is_compact(a.type)) {
+        // SymFncCompRad.cpp:48-68, SymFncCompAngn/SymFncCompAngw.cpp:56-82
+        if (a.subtype != b.subtype) return a.subtype < b.subtype;
+        if (a.e1 != b.e1) return a.e1 < b.e1;
+        if (a.type != 20 && a.e2 != b.e2) return a.e2 < b.e2;
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.rl != b.rl) return a.rl < b.rl;
+        if (a.type == 20) return false;
+        if (a.angleLeft != b.angleLeft) return a.angleLeft < b.angleLeft;
+        return a.angleRight < b.angleRight;
+    }
     if (a.rc != b.rc) return a.rc < b.rc;
     if (a.eta != b.eta) return a.eta < b.eta;
     if (a.rs != b.rs) return a.rs < b.rs;
@@ -217,6 +281,9 @@ bool sf_less(const SF& a, const SF& b)
 // One SF group: members sharing (ec, type, rc, e1[, e2])
 // (Element.cpp:194-287, SymGrpExpRad.cpp:110-136, SymGrpExpAngn.cpp:108-148,
 // SymGrpBaseExpAng.cpp:45-57).
+// Compact groups share (ec, type, e1[, e2]) only; their radial range is
+// (rmin, rmax) over the members (SymGrpCompRad.cpp:45-77,
+// SymGrpCompAngn/SymGrpCompAngw.cpp:46-80, SymGrpBaseComp.cpp:24-31).
 struct Group {
     int type = 0, ec = 0, e1 = 0, e2 = 0;
     double rc = 0;
@@ -225,6 +292,10 @@ struct Group {
     // angular per-member constants
     vector<double> factorNorm, factorDeriv, zetaLambda, scalingFactors;
     vector<char> calcExp;
+    // compact groups
+    double rmin = 0, rmax = 0;
+    vector<double> mrl, mrc, mal, mar;  // member ranges (angles in radians)
+    vector<char> calcComp;  // narrow: r_jk term recomputed (SymGrpCompAngn.cpp:108-126)
 };
 
 // ------------------------------------------------------------ neural net --
@@ -361,6 +432,28 @@ struct oracle_ctx {
 namespace {
 
 // ----------------------------------------------------------- model setup --
+// SymFncBaseComp::setCompactFunction (SymFncBaseComp.cpp:40-88): "p1".."p4"
+// polynomials, a trailing "a" for the asymmetric form, "e" exponential.
+void set_compact_subtype(SF& s)
+{
+    const string& t = s.subtype;
+    if (t.size() < 1 || t.size() > 3)
+        throw std::runtime_error("invalid compact function type specification: \"" + t + "\"");
+    if (t[0] == 'p') {
+        if (t.size() < 2 || t[1] < '1' || t[1] > '4')
+            throw std::runtime_error("invalid polynom type: \"" + t + "\"");
+        s.cr.core = CORE_POLY1 + (t[1] - '1');
+        if (t.size() == 3) {
+            if (t[2] != 'a') throw std::runtime_error("invalid polynom specifier: \"" + t + "\"");
+            s.cr.asym = true;
+        }
+    } else if (t[0] == 'e') {
+        s.cr.core = CORE_EXP;
+    } else {
+        throw std::runtime_error("unknown compact SF type: \"" + t + "\"");
+    }
+}
+
 void load_model(oracle_ctx& c, const string& dir)
 {
     // Settings::readFile / parseLines (Settings.cpp:188-392).
@@ -452,6 +545,34 @@ void load_model(oracle_ctx& c, const string& dir)
             if (s.e1 > s.e2) std::swap(s.e1, s.e2);
             s.zetaInt = (int)std::round(s.zeta);
             s.useIntPow = fabs(s.zeta - s.zetaInt) <= std::numeric_limits<double>::min();
+        } else if (s.type == 20) {  // SymFncCompRad.cpp:85-112
+            s.e1 = eidx(a.at(2));
+            s.rl = atof(a.at(3).c_str());
+            s.rc = atof(a.at(4).c_str());
+            s.subtype = a.at(5);
+            if (s.rl > s.rc) throw std::runtime_error("lower radial boundary >= upper radial boundary");
+            set_compact_subtype(s);
+        } else if (s.type == 21 || s.type == 22) {  // SymFncBaseCompAng.cpp:47-110
+            s.e1 = eidx(a.at(2));
+            s.e2 = eidx(a.at(3));
+            s.rl = atof(a.at(4).c_str());
+            s.rc = atof(a.at(5).c_str());
+            s.angleLeft = atof(a.at(6).c_str());
+            s.angleRight = atof(a.at(7).c_str());
+            s.subtype = a.at(8);
+            if (s.e1 > s.e2) std::swap(s.e1, s.e2);
+            if (s.rl > s.rc) throw std::runtime_error("lower radial boundary >= upper radial boundary");
+            set_compact_subtype(s);
+            if (s.angleLeft >= s.angleRight)
+                throw std::runtime_error("left angle boundary right of or equal to right angle boundary");
+            double const center = (s.angleLeft + s.angleRight) / 2.0;
+            if ((s.angleLeft < 0.0 && center != 0.0) || (s.angleRight > 180.0 && center != 180.0))
+                throw std::runtime_error("angle boundary out of [0,180] and center of angular function /= 0 or /= 180");
+            if (s.angleRight - s.angleLeft > 360.0)
+                throw std::runtime_error("periodic symmetry function cannot spread over domain > 360 degrees");
+            // the angular core has the radial core's type, never asymmetric
+            s.ca.core = s.cr.core;
+            s.ca.set_left_right(s.angleLeft * M_PI / 180.0, s.angleRight * M_PI / 180.0);
         } else {
             throw std::runtime_error(fmt("symmetry function type %d not supported", s.type));
         }
@@ -464,8 +585,11 @@ void load_model(oracle_ctx& c, const string& dir)
                 s.eta /= c.convLength * c.convLength;
                 s.rs *= c.convLength;
                 s.rc *= c.convLength;
+                s.rl *= c.convLength;
             }
         }
+        for (auto& s : e.sfs)
+            if (is_compact(s.type)) s.cr.set_left_right(s.rl, s.rc);
         std::sort(e.sfs.begin(), e.sfs.end(), sf_less);
         for (size_t i = 0; i < e.sfs.size(); ++i) {
             e.sfs[i].index = (int)i;
@@ -485,8 +609,8 @@ void load_model(oracle_ctx& c, const string& dir)
         for (auto& s : e.sfs) {
             bool added = false;
             for (auto& g : e.grps) {
-                if (g.type == s.type && g.rc == s.rc && g.e1 == s.e1 &&
-                    (s.type == 2 || g.e2 == s.e2)) {
+                if (g.type == s.type && (is_compact(s.type) || g.rc == s.rc) && g.e1 == s.e1 &&
+                    (is_radial(s.type) || g.e2 == s.e2)) {
                     g.mem.push_back(s.index);
                     added = true;
                     break;
@@ -495,14 +619,32 @@ void load_model(oracle_ctx& c, const string& dir)
             if (!added) {
                 Group g;
                 g.type = s.type; g.ec = s.ec; g.e1 = s.e1; g.e2 = s.e2; g.rc = s.rc;
-                g.fc.set(c.cutoffType, s.rc, c.cutoffAlpha);
+                if (!is_compact(s.type)) g.fc.set(c.cutoffType, s.rc, c.cutoffAlpha);
                 g.mem.push_back(s.index);
                 e.grps.push_back(g);
             }
         }
+        // Compact groups: member ranges and the r_jk recompute flags
+        // (SymGrpCompRad.cpp:79-107, SymGrpCompAngn.cpp:77-126,
+        // SymGrpCompAngw.cpp:82-115).
+        for (auto& g : e.grps) {
+            if (!is_compact(g.type)) continue;
+            for (size_t i = 0; i < g.mem.size(); ++i) {
+                const SF& s = e.sfs[g.mem[i]];
+                if (s.rl <= 0.0) g.rmin = 0.0;
+                else g.rmin = std::min(g.rmin, s.rl);
+                g.rmax = std::max(g.rmax, s.rc);
+                g.mrl.push_back(s.rl);
+                g.mrc.push_back(s.rc);
+                g.mal.push_back(s.angleLeft * M_PI / 180.0);
+                g.mar.push_back(s.angleRight * M_PI / 180.0);
+                const SF* prev = i ? &e.sfs[g.mem[i - 1]] : nullptr;
+                g.calcComp.push_back(g.type != 21 || !prev || prev->rc != s.rc || prev->rl != s.rl);
+            }
+        }
         // Members are in SF order already (SF sort refines the member sort).
         for (auto& g : e.grps) {
-            if (g.type == 2) continue;  // SymGrpExpAngn/SymGrpExpAngw.cpp:108-148 (same member set-up)
+            if (g.type != 3 && g.type != 9) continue;  // SymGrpExpAngn/SymGrpExpAngw.cpp:108-148 (same member set-up)
             for (size_t i = 0; i < g.mem.size(); ++i) {
                 const SF& s = e.sfs[g.mem[i]];
                 g.factorNorm.push_back(pow(2.0, 1.0 - s.zeta));
@@ -577,7 +719,7 @@ void load_model(oracle_ctx& c, const string& dir)
             g.scalingFactors.clear();
             for (size_t i = 0; i < g.mem.size(); ++i) {
                 g.scalingFactors.push_back(e.sfs[g.mem[i]].scalingFactor);
-                if (g.type != 2) g.factorNorm[i] *= g.scalingFactors[i];
+                if (g.type == 3 || g.type == 9) g.factorNorm[i] *= g.scalingFactors[i];
             }
         }
     }
@@ -1038,6 +1180,151 @@ void calc_angular_wide(const Element& e, const Group& g, Atom& a)
     }
 }
 
+// SymGrpCompRad::calculate (SymGrpCompRad.cpp:129-199).
+void calc_comp_radial(const Element& e, const Group& g, Atom& a)
+{
+    const size_t M = g.mem.size();
+    vector<double> result(M, 0.0);
+    for (auto& n : a.nbr) {
+        double const rij = n.d;
+        if (!(g.e1 == n.element && rij < g.rmax && rij > g.rmin)) continue;
+        for (size_t k = 0; k < M; ++k) {
+            if (rij <= g.mrl[k] || rij >= g.mrc[k]) continue;
+            const SF& s = e.sfs[g.mem[k]];
+            double rad, drad;
+            s.cr.fdf(rij, rad, drad);
+            result[k] += rad;
+            if (drad == 0.0) continue;
+            drad *= g.scalingFactors[k] / rij;
+            double const px = drad * n.dr[0], py = drad * n.dr[1], pz = drad * n.dr[2];
+            double* d = &a.dGdr[3 * s.index];
+            d[0] += px; d[1] += py; d[2] += pz;
+            d = &n.dGdr[3 * s.indexPerElement[g.e1]];
+            d[0] -= px; d[1] -= py; d[2] -= pz;
+        }
+    }
+    for (size_t k = 0; k < M; ++k) {
+        const SF& s = e.sfs[g.mem[k]];
+        a.G[s.index] = s.scale(result[k]);
+    }
+}
+
+// SymGrpCompAngw::calculate (SymGrpCompAngw.cpp:117-341) and, with
+// `narrow`, SymGrpCompAngn::calculate (SymGrpCompAngn.cpp:128-390): the
+// product of compact radial functions of r_ij, r_ik (narrow: and r_jk) with
+// a compact function of the angle acos(cos theta_ijk).  In the narrow form
+// the r_jk factor is only re-evaluated when a member's (rl, rc) differ from
+// the previous member's or the previous member was skipped for this
+// triplet; otherwise the previous member's value is reused.
+void calc_comp_angular(const Element& e, const Group& g, Atom& a, bool narrow)
+{
+    const size_t M = g.mem.size();
+    vector<double> result(M, 0.0), radij(M, 0.0), dradij(M, 0.0);
+    double const rmin = g.rmin, rmax = g.rmax;
+    double const r2min = rmin * rmin, r2max = rmax * rmax;
+    size_t nn = a.nbr.size();
+    if (nn == 0) nn = 1;
+    for (size_t j = 0; j + 1 < nn; ++j) {
+        Neighbor& nj = a.nbr[j];
+        int const nej = nj.element;
+        double const rij = nj.d;
+        if (!((g.e1 == nej || g.e2 == nej) && rij < rmax && rij > rmin)) continue;
+        for (size_t l = 0; l < M; ++l) {
+            if (rij > g.mrl[l] && rij < g.mrc[l]) e.sfs[g.mem[l]].cr.fdf(rij, radij[l], dradij[l]);
+            else radij[l] = 0.0;
+        }
+        const double* dr1 = nj.dr;
+        for (size_t k = j + 1; k < nn; ++k) {
+            Neighbor& nk = a.nbr[k];
+            int const nek = nk.element;
+            if (!((g.e1 == nej && g.e2 == nek) || (g.e2 == nej && g.e1 == nek))) continue;
+            double const rik = nk.d;
+            if (!(rik < rmax && rik > rmin)) continue;
+            const double* dr2 = nk.dr;
+            double const dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1], dr2[2] - dr1[2]};
+            double rjk = 0.0;
+            if (narrow) {
+                rjk = dr3[0] * dr3[0] + dr3[1] * dr3[1] + dr3[2] * dr3[2];
+                if (rjk >= r2max || rjk <= r2min) continue;
+                rjk = sqrt(rjk);
+            }
+            double const rinvijik = 1.0 / (rij * rik);
+            double const costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
+            // the angular polynomial vanishes at 0 and 180 degrees
+            if (costijk <= -1.0 || costijk >= 1.0) continue;
+            double const acostijk = acos(costijk);
+            double const rinvij = rinvijik * rik;
+            double const rinvik = rinvijik * rij;
+            double const rinvjk = narrow ? 1.0 / rjk : 0.0;
+            double const phiijik0 = rinvij * (rinvik - rinvij * costijk);
+            double const phiikij0 = rinvik * (rinvij - rinvik * costijk);
+            double const dacostijk = -1.0 / sqrt(1.0 - costijk * costijk);
+            bool skip = false;
+            double radjk = 0.0, dradjk = 0.0;
+            for (size_t l = 0; l < M; ++l) {
+                if (radij[l] == 0.0 || rik <= g.mrl[l] || rik >= g.mrc[l] ||
+                    (narrow && (rjk <= g.mrl[l] || rjk >= g.mrc[l])) ||
+                    acostijk <= g.mal[l] || acostijk >= g.mar[l]) {
+                    skip = true;
+                    continue;
+                }
+                const SF& s = e.sfs[g.mem[l]];
+                double radik, dradik, ang, dang;
+                s.cr.fdf(rik, radik, dradik);
+                if (narrow && (g.calcComp[l] || skip)) {
+                    s.cr.fdf(rjk, radjk, dradjk);
+                    skip = false;
+                }
+                s.ca.fdf(acostijk, ang, dang);
+                double rad = narrow ? radij[l] * radik * radjk : radij[l] * radik;
+                result[l] += rad * ang;
+                dang *= dacostijk;
+                double const phiijik = phiijik0 * dang;
+                double const phiikij = phiikij0 * dang;
+                double const psiijik = rinvijik * dang;
+                double chiij, chiik, chijk = 0.0;
+                if (narrow) {
+                    chiij = rinvij * dradij[l] * radik * radjk;
+                    chiik = rinvik * radij[l] * dradik * radjk;
+                    chijk = -rinvjk * radij[l] * radik * dradjk;
+                } else {
+                    chiij = rinvij * radik * dradij[l];
+                    chiik = rinvik * radij[l] * dradik;
+                }
+                double p1, p2, p3;
+                double const sfac = g.scalingFactors[l];
+                if (dang != 0.0 && rad != 0.0) {
+                    rad *= sfac;
+                    ang *= sfac;
+                    p1 = rad * phiijik + ang * chiij;
+                    p2 = rad * phiikij + ang * chiik;
+                    p3 = narrow ? rad * psiijik + ang * chijk : rad * psiijik;
+                } else if (ang != 0.0) {
+                    ang *= sfac;
+                    p1 = ang * chiij;
+                    p2 = ang * chiik;
+                    p3 = narrow ? ang * chijk : 0.0;
+                } else {
+                    continue;
+                }
+                double const p1x = p1 * dr1[0], p1y = p1 * dr1[1], p1z = p1 * dr1[2];
+                double const p2x = p2 * dr2[0], p2y = p2 * dr2[1], p2z = p2 * dr2[2];
+                double const p3x = p3 * dr3[0], p3y = p3 * dr3[1], p3z = p3 * dr3[2];
+                double* d = &a.dGdr[3 * s.index];
+                d[0] += p1x + p2x; d[1] += p1y + p2y; d[2] += p1z + p2z;
+                d = &nj.dGdr[3 * s.indexPerElement[nej]];
+                d[0] -= p1x + p3x; d[1] -= p1y + p3y; d[2] -= p1z + p3z;
+                d = &nk.dGdr[3 * s.indexPerElement[nek]];
+                d[0] -= p2x - p3x; d[1] -= p2y - p3y; d[2] -= p2z - p3z;
+            }
+        }
+    }
+    for (size_t l = 0; l < M; ++l) {
+        const SF& s = e.sfs[g.mem[l]];
+        a.G[s.index] = s.scale(result[l]);
+    }
+}
+
 // Mode::calculateSymmetryFunctionGroups (Mode.cpp:1630-1709) with
 // Atom::allocate (Atom.cpp:168-240).
 void calc_sf(oracle_ctx& c)
@@ -1055,6 +1342,8 @@ void calc_sf(oracle_ctx& c)
         for (auto& g : e.grps) {
             if (g.type == 2) calc_radial(e, g, a);
             else if (g.type == 9) calc_angular_wide(e, g, a);
+            else if (g.type == 20) calc_comp_radial(e, g, a);
+            else if (g.type == 21 || g.type == 22) calc_comp_angular(e, g, a, g.type == 21);
             else calc_angular(e, g, a);
         }
     }

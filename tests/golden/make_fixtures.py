@@ -15,7 +15,11 @@ What it writes (data only -- numbers and the example's input files):
 * ``Cu2S_PBE/`` -- the same for ``examples/nnp-predict/Cu2S_PBE`` and
   ``ExamplePrediction.h:1118-1265`` (SF types 2, 3 and 9, CT_POLY2 cutoff,
   SCALESIGMA scaling, softplus networks).
-* ``sf_kat.json`` -- the type-2/3/9 symmetry-function known answers of
+* ``Anisole_SCAN/``, ``DMABN_SCAN/``, ``Ethylbenzene_SCAN/`` -- the same for
+  the compact-function models (SF types 20 and 22, polynomial and asymmetric
+  polynomial subtypes, normalised units; ``ExamplePrediction.h:1266-1525,
+  1526-1550, 1551-1846``).
+* ``sf_kat.json`` -- the type-2/3/9/20/21/22 symmetry-function known answers of
   ``test/cpp/fixtures.h:151-258`` with the cluster geometries of
   ``fixtures.h:11-149`` (values for atom 0, CT_TANHU, no scaling, 1000*eps).
 * ``cutoff_kat.json`` -- f and df of all cutoff types at rc=3, alpha=0.2,
@@ -68,6 +72,9 @@ def example(ref, name, count):
 def main(ref):
     example(ref, "H2O_RPBE-D3", 1080)
     example(ref, "Cu2S_PBE", 144)
+    example(ref, "Anisole_SCAN", 256)
+    example(ref, "DMABN_SCAN", 21)
+    example(ref, "Ethylbenzene_SCAN", 288)
 
     fx = open(os.path.join(ref, "test", "cpp", "fixtures.h")).read()
     kat = {"source": "test/cpp/fixtures.h:11-258; test/cpp/test_SymmetryFunctions.cpp:36,68-74",
@@ -84,7 +91,7 @@ def main(ref):
         lines = re.findall(r'"([^"]*)"', block(fx, "setupLines" + name))
         vals = floats_in(block(fx, "values" + name))
         assert len(types) == len(lines) == len(vals)
-        rows = [{"type": t, "line": l, "value": v} for t, l, v in zip(types, lines, vals) if t in (2, 3, 9)]
+        rows = [{"type": t, "line": l, "value": v} for t, l, v in zip(types, lines, vals) if t in (2, 3, 9, 20, 21, 22)]
         kat["sets"][name] = dict(geoms[name], functions=rows)
     json.dump(kat, open(os.path.join(HERE, "sf_kat.json"), "w"), indent=1)
 

@@ -218,3 +218,19 @@ class RefLib:
 
     def phase_times(self):
         return [self.lib().ref_phase_time(self.h, p) for p in range(5)]
+
+
+def compact_model(directory):
+    """The compact-function water model (SF types 20/21/22) with scaling
+    statistics measured by the oracle on the seeded 192-atom box."""
+    from n2p2_amd import synth
+    if os.path.exists(os.path.join(directory, "weights.008.data")):
+        return directory
+    raw = synth.compact_water_model(directory + "_raw")
+    pos, elem, box = synth.water_box(4)
+    o = Oracle(raw)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE)
+    G = o.G()
+    mats = [G[elem == e][:, :o.nsf[e]] for e in range(o.ne)]
+    return synth.compact_water_model(directory, rows=synth.rows_from_values(mats))

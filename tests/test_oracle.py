@@ -8,18 +8,19 @@ import os
 import numpy as np
 import pytest
 
-from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, RefLib, cutoff_fdf, ref_available
+from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, RefLib, compact_model, cutoff_fdf, ref_available
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
 EPS = np.finfo(float).eps
 
 
-@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE"])
+@pytest.mark.parametrize("name", ["H2O_RPBE-D3", "Cu2S_PBE", "Anisole_SCAN", "DMABN_SCAN", "Ethylbenzene_SCAN"])
 def test_golden_energy_and_forces(name):
     """test/cpp/test_Prediction.cpp:25,44-56: E and all forces at 1000*eps
     (H2O_RPBE-D3: SF types 2/3, CT_TANHU; Cu2S_PBE: types 2/3/9, CT_POLY2,
-    SCALESIGMA, softplus)."""
+    SCALESIGMA, softplus; the three SCAN models: compact types 20/22 with
+    polynomial and asymmetric polynomial subtypes, normalised units)."""
     d = os.path.join(GOLD, name)
     exp = json.load(open(os.path.join(d, "expected.json")))
     o = Oracle(d)
@@ -82,7 +83,7 @@ def _z(sym):
 
 @pytest.mark.parametrize("name", ["Mono", "Mixed"])
 def test_sf_known_answers(tmp_path, name):
-    """test/cpp/fixtures.h:151-258 values of atom 0 for types 2 and 3 at 1000*eps
+    """test/cpp/fixtures.h:151-258 values of atom 0 for types 2, 3, 9, 20, 21, 22 at 1000*eps
     (test/cpp/test_SymmetryFunctions.cpp:36, 162-190)."""
     from n2p2_amd.synth import sorted_sfs
     k = json.load(open(os.path.join(GOLD, "sf_kat.json")))["sets"][name]
@@ -187,7 +188,7 @@ def _water_dir():
 
 
 @pytest.mark.skipif(not ref_available(), reason="reference compute classes not built (no reference tree)")
-@pytest.mark.parametrize("case", ["golden", "water192", "small_cell", "fcc108", "cu2s"])
+@pytest.mark.parametrize("case", ["golden", "water192", "small_cell", "fcc108", "cu2s", "dmabn", "compact"])
 def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path):
     """The oracle reproduces n2p2's own SymGrp/NeuralNetwork/Atom code bit for bit."""
     from n2p2_amd import read_input_data, synth
@@ -200,6 +201,14 @@ def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path
     elif case == "cu2s":
         d = os.path.join(GOLD, "Cu2S_PBE")
         pos, elem, box = read_input_data(os.path.join(d, "input.data"), ["S", "Cu"])
+    elif case == "dmabn":
+        d = os.path.join(GOLD, "DMABN_SCAN")
+        pos, elem, box = read_input_data(os.path.join(d, "input.data"), ["H", "C", "N"])
+    elif case == "compact":
+        # every compact subtype, and narrow members of different subtypes
+        # sharing (rl, rc), whose r_jk factor the reference reuses
+        d = compact_model(str(tmp_path / "compact"))
+        pos, elem, box = synth.water_box(4)
     elif case == "fcc108":
         d = synth.fcc_model(str(tmp_path / "fcc"))
         pos, elem, box = synth.fcc_lattice(n_cells=3)
