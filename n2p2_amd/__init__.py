@@ -430,7 +430,10 @@ def inverse_box(box):
 
 
 def remap(pos, box):
-    """Structure::remap (Structure.cpp:1224-1248) elementwise, same rounding."""
+    """Structure::remap (Structure.cpp:1224-1248) elementwise, same rounding;
+    positions of an isolated structure (box None) are returned unchanged."""
+    if box is None:
+        return np.array(pos, dtype=np.float64)
     ib = inverse_box(box)
     r = np.asarray(pos, dtype=np.float64)
     f = r[:, 0:1] * ib[0] + r[:, 1:2] * ib[1] + r[:, 2:3] * ib[2]

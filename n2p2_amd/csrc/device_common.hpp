@@ -57,6 +57,34 @@ __device__ inline void pair_vector(const double* __restrict__ ri, const double* 
         dr[k] = ri[k] - rj[k] + f0 * box[k] + f1 * box[3 + k] + f2 * box[6 + k];
 }
 
+// Core functions on x in [0, 1) (CoreFunction.h:214-319): 0 exponential
+// e*exp(1/(x^2 - 1)), 1..4 the polynomials of increasing order.
+__device__ inline void core_fdf(int core, double x, double& f, double& df)
+{
+    if (core == 0) {
+        double const t = 1.0 / (x * x - 1.0);
+        double const t2 = exp(t);
+        f = M_E * t2;
+        df = -2.0 * M_E * x * t * t * t2;
+    } else if (core == 1) {
+        f = (2.0 * x - 3.0) * x * x + 1.0;
+        df = x * (6.0 * x - 6.0);
+    } else if (core == 2) {
+        double const x2 = x * x;
+        f = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
+        df = x2 * ((60.0 - 30.0 * x) * x - 30.0);
+    } else if (core == 3) {
+        double const x2 = x * x;
+        f = (x * (x * (20.0 * x - 70.0) + 84.0) - 35.0) * x2 * x2 + 1.0;
+        df = x2 * x * (x * (x * (140.0 * x - 420.0) + 420.0) - 140.0);
+    } else {
+        double x4 = x * x;
+        x4 *= x4;
+        f = (x * (x * ((315.0 - 70.0 * x) * x - 540.0) + 420.0) - 126.0) * x * x4 + 1.0;
+        df = x4 * (x * (x * ((2520.0 - 630.0 * x) * x - 3780.0) + 2520.0) - 630.0);
+    }
+}
+
 // Cutoff function value and derivative (CutoffFunction::fdf,
 // CutoffFunction.h:210-220, CutoffFunction.cpp:114-213, CoreFunction.h).
 __device__ inline void cutoff_fdf(int type, const GroupT& g, double tanhPre, double r,
@@ -91,29 +119,25 @@ __device__ inline void cutoff_fdf(int type, const GroupT& g, double tanhPre, dou
     }
     if (r < g.rci) { fc = 1.0; dfc = 0.0; return; }
     double const x = (r - g.rci) * g.iw;
-    if (type == 4) {
-        double const t = 1.0 / (x * x - 1.0);
-        double const t2 = exp(t);
-        fc = M_E * t2;
-        dfc = -2.0 * M_E * x * t * t * t2;
-    } else if (type == 5) {
-        fc = (2.0 * x - 3.0) * x * x + 1.0;
-        dfc = x * (6.0 * x - 6.0);
-    } else if (type == 6) {
-        double const x2 = x * x;
-        fc = ((15.0 - 6.0 * x) * x - 10.0) * x * x2 + 1.0;
-        dfc = x2 * ((60.0 - 30.0 * x) * x - 30.0);
-    } else if (type == 7) {
-        double const x2 = x * x;
-        fc = (x * (x * (20.0 * x - 70.0) + 84.0) - 35.0) * x2 * x2 + 1.0;
-        dfc = x2 * x * (x * (x * (140.0 * x - 420.0) + 420.0) - 140.0);
-    } else {
-        double x4 = x * x;
-        x4 *= x4;
-        fc = (x * (x * ((315.0 - 70.0 * x) * x - 540.0) + 420.0) - 126.0) * x * x4 + 1.0;
-        dfc = x4 * (x * (x * ((2520.0 - 630.0 * x) * x - 3780.0) + 2520.0) - 630.0);
-    }
+    core_fdf(type - 4, x, fc, dfc);
     dfc *= g.iw;
+}
+
+// Compact function (CompactFunction::fdf, CompactFunction.h:150-172): the
+// core at |a - cen| * scl, through x -> (2 - x) x when asymmetric
+// (CoreFunction.h:197-212).
+__device__ inline void compact_fdf(int core, int asym, double cen, double scl, double a, double& fa,
+                                   double& dfa)
+{
+    a = (a - cen) * scl;
+    double const x = fabs(a);
+    if (asym) {
+        core_fdf(core, (2.0 - x) * x, fa, dfa);
+        dfa *= 2.0 - 2.0 * x;
+    } else {
+        core_fdf(core, x, fa, dfa);
+    }
+    dfa *= copysign(scl, a);
 }
 
 // Exponentiation by squaring (utility.cpp:292-316).

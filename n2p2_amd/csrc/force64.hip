@@ -110,7 +110,7 @@ __global__ __launch_bounds__(64) void k_work(const ModelT* __restrict__ M, SysVi
                 if (!((se[j] == G.e1 && se[k] == G.e2) || (se[j] == G.e2 && se[k] == G.e1))) continue;
                 if (!(sd[k] < G.rc)) continue;
                 const double a = sx[0][k] - sx[0][j], b = sx[1][k] - sx[1][j], c = sx[2][k] - sx[2][j];
-                if (G.type == 9 || a * a + b * b + c * c < G.rc2) {
+                if (G.type == 9 || G.type == 22 || a * a + b * b + c * c < G.rc2) {
                     ++T;
                     Tm += G.nmem;
                     Te += nexp;

@@ -75,6 +75,9 @@ struct SF64Out {
 };
 void launch_sf64(const ModelT* m, const ModelT& hm, const SysView& s, const SF64Out& o,
                  bool deriv, int max_nb, hipStream_t stream);
+// The compact-function groups (types 20, 21, 22) into the same outputs.
+void launch_sfcomp64(const ModelT* m, const ModelT& hm, const SysView& s, const SF64Out& o,
+                     bool deriv, int max_nb, hipStream_t stream);
 
 // FP64 atomic NNs over per-element atom lists; dE/dG by the adjoint.
 struct NNIO {

@@ -131,7 +131,7 @@ struct Stage {
     unsigned short* la; unsigned short* lb;  // group lists
     unsigned* tjk;   // compacted pair (j | k << 16)
     float* tr2;      // its r_jk^2
-    float4* mp;      // [2*kMaxMembers] member constants of the current group
+    float4* mp;      // [2*kMaxMembersExp] member constants of the current group
     int trip;        // capacity of the triplet buffer
 };
 
@@ -144,7 +144,7 @@ __device__ inline Stage carve(float* base, int capL, int trip)
     S.tjk = reinterpret_cast<unsigned*>(S.c + capL);
     S.tr2 = reinterpret_cast<float*>(S.tjk + trip);
     S.mp = reinterpret_cast<float4*>(S.tr2 + trip);
-    S.la = reinterpret_cast<unsigned short*>(S.mp + 2 * kMaxMembers);
+    S.la = reinterpret_cast<unsigned short*>(S.mp + 2 * kMaxMembersExp);
     S.lb = S.la + capL;
     S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
     return S;
@@ -153,7 +153,7 @@ __device__ inline Stage carve(float* base, int capL, int trip)
 // 8-byte aligned)
 __host__ __device__ constexpr size_t stage_bytes(int capL, int trip)
 {
-    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembers * 16;
+    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembersExp * 16;
 }
 
 __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
@@ -262,7 +262,7 @@ __device__ inline MemberMasks load_members(const Stage& S, const MemberF* __rest
         const MemberF m = mf[lane];
         const float wq = w ? w[m.sf] : 0.f;
         S.mp[lane] = make_float4(m.zeta, m.etaL, m.rs, (float)(m.sidx & 1));
-        S.mp[kMaxMembers + lane] = make_float4(wq * m.zetaLambda, wq, wq * m.twoEta, (float)((m.sidx >> 1) & 1));
+        S.mp[kMaxMembersExp + lane] = make_float4(wq * m.zetaLambda, wq, wq * m.twoEta, (float)((m.sidx >> 1) & 1));
     }
     MemberMasks R;
     R.calc = (unsigned)__ballot(lane < nmem && mf[min(lane, nmem - 1)].calcExp);
@@ -513,7 +513,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                     for (int q = 0; q < MAXM; ++q)
                         if (q < nmem) {
                             const float4 mq = S.mp[q];
-                            const float4 mw = S.mp[kMaxMembers + q];
+                            const float4 mw = S.mp[kMaxMembersExp + q];
                             if ((R.calc >> q) & 1u) {
                                 float arg = r2sum;
                                 if (mq.z > 0.f) {
@@ -817,7 +817,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q) {
                     const float4 mq = S.mp[q];
-                    const float4 wq = S.mp[kMaxMembers + q];
+                    const float4 wq = S.mp[kMaxMembersExp + q];
                     const bool s1 = mq.w != 0.f;
                     const float t = fexp2(fmaf(mq.x, s1 ? L[1] : L[0], -mq.y * r2sum));
                     B = fmaf(wq.y, t, B);
@@ -831,7 +831,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                 for (int q = 0; q < MAXM; ++q)
                     if (q < nmem) {
                         const float4 mq = S.mp[q];
-                        const float4 wq = S.mp[kMaxMembers + q];
+                        const float4 wq = S.mp[kMaxMembersExp + q];
                         if ((R.calc >> q) & 1u) {
                             float arg = r2sum;
                             shifted = mq.z > 0.f;
@@ -1167,14 +1167,14 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 f.rs = (float)M.rs;
                 f.lambda = (float)M.lambda;
                 f.zeta = (float)M.zeta;
-                f.gnorm = G.type != 2 ? (float)M.gnorm : 1.f;
+                f.gnorm = (G.type == 3 || G.type == 9) ? (float)M.gnorm : 1.f;
                 f.twoEta = (float)(2.0 * M.eta);
                 f.zetaLambda = (float)M.zetaLambda;
                 f.etaL = (float)(M.eta * 1.4426950408889634);  // eta * log2(e)
                 f.sf = M.sf;
                 f.calcExp = M.calcExp;
                 f.sidx = 0;
-                if (G.type != 2) {
+                if (G.type == 3 || G.type == 9) {
                     if (M.rs > 0.0) mm.hasRs = true;
                     int k = 0;
                     while (k < gm.nlam && gm.lam[k] != f.lambda) ++k;
@@ -1184,7 +1184,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                     }
                     f.sidx = k;
                 }
-                sffac[(size_t)e * kMaxSF + M.sf] = (float)(G.type != 2 ? M.factorNorm : M.sfac);
+                sffac[(size_t)e * kMaxSF + M.sf] = (float)((G.type == 3 || G.type == 9) ? M.factorNorm : M.sfac);
             }
             mm.maxS = std::max(mm.maxS, gm.nlam);
             // unused table slots repeat the first lambda (evaluated, never picked)
@@ -1198,7 +1198,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
         int cnt = -1;
         bool same = true;
         for (int g = 0; g < t.ngroups; ++g)
-            if (t.grp[g].type != 2) {
+            if (t.grp[g].type == 3 || t.grp[g].type == 9) {
                 if (cnt < 0) cnt = t.grp[g].nmem;
                 same = same && t.grp[g].nmem == cnt;
             }
@@ -1274,6 +1274,11 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed)
 {
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
+    if (t.has_comp)
+        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact symmetry functions "
+                                          "(types 20-22); use the plain FP64 form"};
+    if (t.max_width > kMaxWidth)
+        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
     if (s.natoms == 0) return;
     const int capL = s.cap;
     const int P = pow2_members(t);
@@ -1338,6 +1343,11 @@ void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_mod
                       const double* u, double* Gdot, int ldG, hipStream_t stream)
 {
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
+    if (t.has_comp)
+        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact symmetry functions "
+                                          "(types 20-22); use the plain FP64 form"};
+    if (t.max_width > kMaxWidth)
+        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
     if (s.natoms == 0) return;
     const int capL = s.cap;
     const int tt = mm.tune.trip_tangent;

@@ -110,6 +110,10 @@ struct SF {
     bool useIntPow;
     int index;
     int slot[kMaxElem];
+    // compact types: radial window (rl, rc), angle window in degrees, subtype
+    double rl, angleLeft, angleRight;
+    std::string subtype;
+    int core, asym;
 };
 
 // Order of SymFncExpRad::operator< / SymFncExpAngn::operator<
@@ -118,6 +122,16 @@ struct SF {
 bool sf_before(const SF& a, const SF& b)
 {
     if (a.type != b.type) return a.type < b.type;
+    if (sf_compact(a.type)) {  // SymFncCompRad.cpp:48-68, SymFncCompAngn/SymFncCompAngw.cpp:56-82
+        if (a.subtype != b.subtype) return a.subtype < b.subtype;
+        if (a.e1 != b.e1) return a.e1 < b.e1;
+        if (a.type != 20 && a.e2 != b.e2) return a.e2 < b.e2;
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.rl != b.rl) return a.rl < b.rl;
+        if (a.type == 20) return false;
+        if (a.angleLeft != b.angleLeft) return a.angleLeft < b.angleLeft;
+        return a.angleRight < b.angleRight;
+    }
     if (a.rc != b.rc) return a.rc < b.rc;
     if (a.eta != b.eta) return a.eta < b.eta;
     if (a.rs != b.rs) return a.rs < b.rs;
@@ -126,6 +140,27 @@ bool sf_before(const SF& a, const SF& b)
     if (a.lambda != b.lambda) return a.lambda < b.lambda;
     if (a.e1 != b.e1) return a.e1 < b.e1;
     return a.e2 < b.e2;
+}
+
+// SymFncBaseComp::setCompactFunction (SymFncBaseComp.cpp:40-88): "p1".."p4"
+// polynomials, a trailing "a" for the asymmetric form, "e" exponential.
+void compact_subtype(SF& f)
+{
+    const std::string& t = f.subtype;
+    if (t.empty() || t.size() > 3) fail(NNPX_ERR_MODEL, "invalid compact function type \"" + t + "\"");
+    f.asym = 0;
+    if (t[0] == 'p') {
+        if (t.size() < 2 || t[1] < '1' || t[1] > '4') fail(NNPX_ERR_MODEL, "invalid polynom type \"" + t + "\"");
+        f.core = t[1] - '0';
+        if (t.size() == 3) {
+            if (t[2] != 'a') fail(NNPX_ERR_MODEL, "invalid polynom specifier \"" + t + "\"");
+            f.asym = 1;
+        }
+    } else if (t[0] == 'e') {
+        f.core = 0;
+    } else {
+        fail(NNPX_ERR_MODEL, "unknown compact SF type \"" + t + "\"");
+    }
 }
 
 }  // namespace
@@ -208,14 +243,40 @@ void load_model(const std::string& dir, HostModel& hm)
             if (f.e1 > f.e2) std::swap(f.e1, f.e2);
             f.zetaInt = (int)std::round(f.zeta);
             f.useIntPow = std::fabs(f.zeta - f.zetaInt) <= std::numeric_limits<double>::min();
+        } else if (f.type == 20) {  // SymFncCompRad.cpp:85-112
+            f.e1 = f.e2 = eidx(w.at(2));
+            f.rl = num(w.at(3));
+            f.rc = num(w.at(4));
+            f.subtype = w.at(5);
+            if (f.rl > f.rc) fail(NNPX_ERR_MODEL, "lower radial boundary >= upper radial boundary");
+            compact_subtype(f);
+        } else if (f.type == 21 || f.type == 22) {  // SymFncBaseCompAng.cpp:47-110
+            f.e1 = eidx(w.at(2));
+            f.e2 = eidx(w.at(3));
+            f.rl = num(w.at(4));
+            f.rc = num(w.at(5));
+            f.angleLeft = num(w.at(6));
+            f.angleRight = num(w.at(7));
+            f.subtype = w.at(8);
+            if (f.e1 > f.e2) std::swap(f.e1, f.e2);
+            if (f.rl > f.rc) fail(NNPX_ERR_MODEL, "lower radial boundary >= upper radial boundary");
+            compact_subtype(f);
+            if (f.angleLeft >= f.angleRight)
+                fail(NNPX_ERR_MODEL, "left angle boundary right of or equal to right angle boundary");
+            const double center = (f.angleLeft + f.angleRight) / 2.0;
+            if ((f.angleLeft < 0.0 && center != 0.0) || (f.angleRight > 180.0 && center != 180.0))
+                fail(NNPX_ERR_MODEL, "angle boundary out of [0,180] and center of angular function /= 0 or /= 180");
+            if (f.angleRight - f.angleLeft > 360.0)
+                fail(NNPX_ERR_MODEL, "periodic symmetry function cannot spread over domain > 360 degrees");
         } else {
             fail(NNPX_ERR_UNSUPPORTED, "symmetry function type " + w.at(1) +
-                                           " not supported (types 2, 3 and 9 are)");
+                                           " not supported (types 2, 3, 9, 20, 21 and 22 are)");
         }
         if (hm.normalize) {  // SymFnc*::changeLengthUnit
             f.eta /= hm.convLength * hm.convLength;
             f.rs *= hm.convLength;
             f.rc *= hm.convLength;
+            f.rl *= hm.convLength;
         }
         sfs[f.ec].push_back(f);
     }
@@ -241,7 +302,7 @@ void load_model(const std::string& dir, HostModel& hm)
             E.tab0[n] = tab + n * kMaxSF;
             E.nsfd[n] = 0;
             for (auto& f : v) {
-                bool rel = f.type == 2 ? f.e1 == n : (f.e1 == n || f.e2 == n);
+                bool rel = sf_radial(f.type) ? f.e1 == n : (f.e1 == n || f.e2 == n);
                 f.slot[n] = -1;
                 if (rel) {
                     f.slot[n] = E.nsfd[n];
@@ -251,15 +312,17 @@ void load_model(const std::string& dir, HostModel& hm)
             t.max_sfd = std::max(t.max_sfd, E.nsfd[n]);
         }
         // Groups (Element::setupSymmetryFunctionGroups, Element.cpp:194-287):
-        // radial groups first, then narrow and wide angular; members in SF order.
+        // radial groups first, then narrow and wide angular; members in SF
+        // order.  Compact groups do not key on rc.
         std::vector<std::vector<int>> groups;
-        for (int pass : {2, 3, 9}) {
+        for (int pass : {2, 20, 3, 9, 21, 22}) {
             for (auto& f : v) {
                 if (f.type != pass) continue;
                 bool added = false;
                 for (auto& g : groups) {
                     const SF& h = v[g[0]];
-                    if (h.type == f.type && h.rc == f.rc && h.e1 == f.e1 && h.e2 == f.e2) {
+                    if (h.type == f.type && (sf_compact(f.type) || h.rc == f.rc) && h.e1 == f.e1 &&
+                        h.e2 == f.e2) {
                         g.push_back(f.index);
                         added = true;
                         break;
@@ -277,21 +340,39 @@ void load_model(const std::string& dir, HostModel& hm)
             if (v[g[0]].rc != v[groups[0][0]].rc) E.ucut = 0;
         for (auto& g : groups) {
             const SF& h = v[g[0]];
-            if ((int)g.size() > kMaxMembers) fail(NNPX_ERR_UNSUPPORTED, "SF group too large");
+            const bool comp = sf_compact(h.type);
+            if ((int)g.size() > (comp || sf_radial(h.type) ? kMaxMembers : kMaxMembersExp))
+                fail(NNPX_ERR_UNSUPPORTED, "SF group too large");
             GroupT& G = t.grp[gcount++];
             G.type = h.type;
             G.e1 = h.e1;
             G.e2 = h.e2;
             G.nmem = (int)g.size();
             G.mem0 = mcount;
-            G.rc = h.rc;
-            G.rc2 = h.rc * h.rc;
-            G.rcinv = 1.0 / h.rc;
-            G.rci = h.rc * t.alpha;
-            G.iw = 1.0 / (h.rc - G.rci);
-            if (h.type == 2) {
+            double rc = h.rc;
+            if (comp) {  // SymGrpCompRad/SymGrpCompAngn/SymGrpCompAngw::addMember
+                G.rmin = 0.0;
+                rc = 0.0;
+                for (int idx : g) {
+                    const SF& f = v[idx];
+                    if (f.rl <= 0.0) G.rmin = 0.0;
+                    else G.rmin = std::min(G.rmin, f.rl);
+                    rc = std::max(rc, f.rc);
+                }
+                t.has_comp = 1;
+            } else {
+                t.has_exp = 1;
+            }
+            G.rc = rc;
+            G.rc2 = rc * rc;
+            G.rcinv = 1.0 / rc;
+            G.rci = rc * t.alpha;
+            G.iw = 1.0 / (rc - G.rci);
+            if (sf_radial(h.type)) {
                 E.nrad++;
-                t.max_mem_rad = std::max(t.max_mem_rad, G.nmem);
+                if (!comp) t.max_mem_rad = std::max(t.max_mem_rad, G.nmem);
+            } else if (comp) {
+                t.max_mem_comp = std::max(t.max_mem_comp, G.nmem);
             } else {
                 t.max_mem_ang = std::max(t.max_mem_ang, G.nmem);
             }
@@ -309,12 +390,40 @@ void load_model(const std::string& dir, HostModel& hm)
                 M.rs = f.rs;
                 M.lambda = f.lambda;
                 M.zeta = f.zeta;
-                if (f.type != 2) {  // SymGrpExpAngn/SymGrpExpAngw::sortMembers (SymGrpExpAngn.cpp:108-148)
+                if (f.type == 3 || f.type == 9) {  // SymGrpExpAngn/SymGrpExpAngw::sortMembers (SymGrpExpAngn.cpp:108-148)
                     M.factorNorm = std::pow(2.0, 1.0 - f.zeta);
                     M.factorDeriv = 2.0 * f.eta / f.zeta / f.lambda;
                     M.zetaLambda = f.zeta * f.lambda;
                 }
                 M.sfac = 1.0;
+                if (comp) {
+                    // CompactFunction::setLeftRight (CompactFunction.cpp:58-77) of
+                    // the radial window and of the angle window in radians
+                    // (SymFncBaseCompAng.cpp:96-100, SymGrpCompAngw.cpp:88-91).
+                    M.calcExp = 0;
+                    M.core = f.core;
+                    M.asym = f.asym;
+                    M.rl = f.rl;
+                    M.rcm = f.rc;
+                    M.cen = (f.rc + f.rl) / 2.0;
+                    M.scl = 1.0 / ((f.rc - f.rl) / 2.0);
+                    M.al = f.angleLeft * M_PI / 180.0;
+                    M.ar = f.angleRight * M_PI / 180.0;
+                    M.acen = (M.ar + M.al) / 2.0;
+                    M.ascl = 1.0 / ((M.ar - M.al) / 2.0);
+                    // Narrow groups re-evaluate the r_jk factor only when
+                    // (rl, rc) change or the previous member was skipped
+                    // (SymGrpCompAngn.cpp:108-126, 288-296).  `walk` marks
+                    // members whose reused factor may be another subtype's.
+                    M.calcComp = (f.type != 21 || !prev || prev->rc != f.rc || prev->rl != f.rl) ? 1 : 0;
+                    M.walk = 0;
+                    for (int r = (int)q - 1; !M.calcComp && r >= 0; --r) {
+                        const SF& p = v[g[r]];
+                        if (p.core != f.core || p.asym != f.asym) M.walk = 1;
+                        if (p.rc != f.rc || p.rl != f.rl || r == 0 || v[g[r - 1]].rc != p.rc || v[g[r - 1]].rl != p.rl)
+                            break;
+                    }
+                }
             }
         }
         sfcount += E.nsf;
@@ -351,9 +460,12 @@ void load_model(const std::string& dir, HostModel& hm)
         N.normalize = normNodes ? 1 : 0;
         for (int l = 0; l < N.nl; ++l) {
             N.n[l] = width[e][l];
-            if (N.n[l] < 1 || N.n[l] > kMaxWidth) fail(NNPX_ERR_UNSUPPORTED, "NN layer width out of range");
+            // the input layer holds the element's SFs (up to kMaxSF)
+            if (N.n[l] < 1 || (l > 0 && N.n[l] > kMaxWidth))
+                fail(NNPX_ERR_UNSUPPORTED, "NN layer width out of range");
             N.act[l] = l == 0 ? AF_IDENTITY : activation(acts[l - 1]);
             t.max_width = std::max(t.max_width, N.n[l]);
+            if (l > 0) t.max_hidden = std::max(t.max_hidden, N.n[l]);
         }
         for (int l = 1; l < N.nl; ++l) {
             N.woff[l] = woff;
@@ -417,7 +529,7 @@ void load_model(const std::string& dir, HostModel& hm)
             for (int q = G.mem0; q < G.mem0 + G.nmem; ++q) {
                 MemberT& M = t.mem[q];
                 M.sfac = t.scale[E.sf0 + M.sf].factor;
-                if (G.type != 2) {
+                if (G.type == 3 || G.type == 9) {
                     M.factorNorm *= M.sfac;
                     M.gnorm = M.factorNorm / M.sfac;
                 }

@@ -12,9 +12,10 @@
 namespace nnpx {
 
 constexpr int kMaxElem = 8;      // elements per model
-constexpr int kMaxSF = 128;      // symmetry functions per element
+constexpr int kMaxSF = 512;      // symmetry functions per element
 constexpr int kMaxGroups = 64;   // SF groups per element
-constexpr int kMaxMembers = 32;  // members per SF group
+constexpr int kMaxMembers = 64;  // members per compact SF group
+constexpr int kMaxMembersExp = 32;  // members per Gaussian (type 3/9) angular group
 constexpr int kMaxLayers = 6;    // NN layers incl. input and output
 constexpr int kMaxWidth = 128;   // NN layer width
 
@@ -24,15 +25,22 @@ enum Activation {
     AF_COS, AF_REVLOGISTIC, AF_EXP, AF_HARMONIC
 };
 
-// One SF group: members share (central element, type, rc, e1[, e2]).
+// SF types: Gaussian radial (2), narrow (3) and wide (9) angular with a
+// cutoff function; compact radial (20), narrow (21) and wide (22) angular.
+__host__ __device__ inline bool sf_compact(int type) { return type >= 20; }
+__host__ __device__ inline bool sf_radial(int type) { return type == 2 || type == 20; }
+
+// One SF group: members share (central element, type, rc, e1[, e2]); compact
+// groups share (central element, type, e1[, e2]) and span (rmin, rc = rmax).
 struct GroupT {
-    int type;        // 2 radial, 3 angular
+    int type;        // 2, 3, 9, 20, 21, 22
     int e1, e2;      // neighbour elements (e1 <= e2)
     int nmem;        // members
     int mem0;        // first member in ModelT::mem
     int pad;
     double rc, rc2;
     double rcinv, rci, iw;  // cutoff constants (CutoffFunction.cpp:95-112)
+    double rmin;     // compact: lower radial bound of the group
 };
 
 // One member (one symmetry function) of a group.
@@ -50,6 +58,16 @@ struct MemberT {
     double zetaLambda;   // angular: zeta * lambda
     double sfac;         // scaling factor (SymFnc::scalingFactor)
     double gnorm;        // angular: factorNorm / sfac
+    // compact members (CompactFunction/CoreFunction): radial function on
+    // (rl, rcm), angular function on (al, ar) radians
+    int core;            // 0 exponential, 1..4 polynomial order
+    int asym;            // asymmetric polynomial (x -> (2 - x) x)
+    int calcComp;        // narrow: r_jk factor evaluated by this member
+    int walk;            // narrow: the reused r_jk factor may come from another subtype
+    double rl, rcm;      // radial window
+    double cen, scl;     // radial centre and 1/half-width
+    double al, ar;       // angular window (radians)
+    double acen, ascl;   // angular centre and 1/half-width
 };
 
 // SymFnc::scale (SymFnc.cpp:169-195): Smin + factor * (v - shift) or v - shift.
@@ -87,9 +105,13 @@ struct ModelT {
     int nmembers;
     int max_sf;             // max SF count over elements
     int max_sfd;            // max derivative slots (over central and neighbour element)
-    int max_mem_ang;        // max members of an angular group
+    int max_mem_ang;        // max members of a Gaussian angular group
     int max_mem_rad;
-    int max_width;          // max NN layer width
+    int max_mem_comp;       // max members of a compact angular group
+    int has_exp;            // any Gaussian (type 2/3/9) group
+    int has_comp;           // any compact (type 20/21/22) group
+    int max_width;          // max NN layer width (input layer included)
+    int max_hidden;         // max width of the hidden and output layers
     int nweights;           // FP64 weights+biases in the weight buffer
     double maxCutoff;
     double alpha;

@@ -9,6 +9,7 @@
 // Weights are read in two layouts: W^T ([in][out]) so lanes read consecutive
 // outputs in the forward sweep, W ([out][in]) for the backward sweep.
 #include "kernels.hpp"
+#include "nnpx.h"
 
 namespace nnpx {
 namespace {
@@ -54,7 +55,7 @@ struct NNArgs {
 };
 
 __global__ __launch_bounds__(kNNThreads) void k_nn64(const ModelT* __restrict__ M, NNArgs a,
-                                                      NNIO io, int maxw)
+                                                      NNIO io, int maxw, int maxh)
 {
     extern __shared__ double sm[];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
@@ -62,11 +63,12 @@ __global__ __launch_bounds__(kNNThreads) void k_nn64(const ModelT* __restrict__ 
     while (e + 1 < a.nelem && (int)blockIdx.x >= a.blk_off[e + 1]) ++e;
     const NNT& N = M->el[e].nn;
     const int nl = N.nl;
-    // per-wave LDS: two activation buffers + f' of every layer
-    double* base = sm + (size_t)wave * maxw * (2 + kMaxLayers);
+    // per-wave LDS: two activation buffers (any layer, input included) and
+    // f' of every hidden/output layer
+    double* base = sm + (size_t)wave * (2 * maxw + kMaxLayers * maxh);
     double* va = base;
     double* vb = base + maxw;
-    double* dfd = base + 2 * maxw;  // [layer][maxw]
+    double* dfd = base + 2 * maxw;  // [layer][maxh]
     const int slot = ((int)blockIdx.x - a.blk_off[e]) * kNNWaves + wave;
     const int pos = a.atom_off[e] + slot;
     if (pos >= a.atom_off[e + 1]) return;
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(kNNThreads) void k_nn64(const ModelT* __restrict__ 
             if (N.normalize) x /= nin;
             double df;
             vout[k] = act_fwd(N.act[l], x, df);
-            dfd[l * maxw + k] = df;
+            dfd[l * maxh + k] = df;
         }
         __builtin_amdgcn_wave_barrier();
         double* t = vin; vin = vout; vout = t;
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(kNNThreads) void k_nn64(const ModelT* __restrict__ 
     double* din = vout;  // free buffer
     double* dout = vin;
     __builtin_amdgcn_wave_barrier();
-    if (lane == 0) din[0] = dfd[(nl - 1) * maxw];
+    if (lane == 0) din[0] = dfd[(nl - 1) * maxh];
     __builtin_amdgcn_wave_barrier();
     for (int l = nl - 1; l >= 1; --l) {
         const int nin = N.n[l - 1], nout = N.n[l];
@@ -108,7 +110,7 @@ __global__ __launch_bounds__(kNNThreads) void k_nn64(const ModelT* __restrict__ 
             double s = 0.0;
             for (int k = 0; k < nout; ++k) s += w[(size_t)k * nin + j] * din[k];
             if (N.normalize) s *= norm;
-            if (l - 1 >= 1) s *= dfd[(l - 1) * maxw + j];
+            if (l - 1 >= 1) s *= dfd[(l - 1) * maxh + j];
             dout[j] = s;
         }
         __builtin_amdgcn_wave_barrier();
@@ -155,9 +157,10 @@ void launch_nn64(const ModelT* m, const ModelT& hm, const double* w, const int* 
     a.blk_off[hm.nelem] = blocks;
     a.atom_off[hm.nelem] = elem_off_host[hm.nelem];
     if (blocks == 0) return;
-    const int maxw = hm.max_width;
-    const size_t lds = sizeof(double) * (size_t)kNNWaves * maxw * (2 + kMaxLayers);
-    hipLaunchKernelGGL(k_nn64, dim3(blocks), dim3(kNNThreads), lds, stream, m, a, io, maxw);
+    const int maxw = hm.max_width, maxh = hm.max_hidden;
+    const size_t lds = sizeof(double) * (size_t)kNNWaves * (2 * maxw + kMaxLayers * maxh);
+    if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "network too wide for the FP64 NN kernel's LDS staging"};
+    hipLaunchKernelGGL(k_nn64, dim3(blocks), dim3(kNNThreads), lds, stream, m, a, io, maxw, maxh);
 }
 
 void launch_energy(const double* atomE, const int* atom_off, int nstruct, double* E,

@@ -432,7 +432,10 @@ void compute_sf64(nnpx_ctx* c, bool deriv)
         o.dGnbr = buf<double>(c, "dGnbr", (size_t)c->N * c->cap * o.S * 3);
     }
     SysView s = sysview(c);
-    timed(c, "sf64", [&] { launch_sf64(c->d_model, t, s, o, deriv, c->cap, c->stream); });
+    timed(c, "sf64", [&] {
+        launch_sf64(c->d_model, t, s, o, deriv, c->cap, c->stream);
+        launch_sfcomp64(c->d_model, t, s, o, deriv, c->cap, c->stream);
+    });
     c->sf_valid = true;
     c->sfd_valid = deriv;
 }
@@ -1282,7 +1285,7 @@ nnpx_status nnpx_autotune(nnpx_ctx* c)
         const int capL = c->cap;
         const ModelT& t = c->hm.t;
         auto fits = [&](int trip, size_t extra) {
-            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembers * 16 + extra <= 65536;
+            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembersExp * 16 + extra <= 65536;
         };
         c->timing = true;
         double best_sf = 1e30, best_f = 1e30, best_t = 1e30;

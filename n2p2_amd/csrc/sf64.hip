@@ -33,14 +33,14 @@ struct Lds {
     int* la;       // [cap] list of neighbours with element e1 inside rc
     int* lb;       // [cap] list with element e2
     double* acc;   // [nmem][3][cap] (neighbour index fastest: conflict-free atomics)
-    double* red;   // [kSFWaves][4*kMaxMembers]
+    double* red;   // [kSFWaves][4*kMaxMembersExp]
 };
 
 __device__ inline double block_sum(double v, double* red, int slot)
 {
     v = wave_sum(v);
     const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) red[w * 4 * kMaxMembers + slot] = v;
+    if ((threadIdx.x & (kWave - 1)) == 0) red[w * 4 * kMaxMembersExp + slot] = v;
     return v;
 }
 
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
     L.fc = L.d + capL;
     L.dfc = L.fc + capL;
     L.red = L.dfc + capL;
-    L.acc = L.red + kSFWaves * 4 * kMaxMembers;
+    L.acc = L.red + kSFWaves * 4 * kMaxMembersExp;
     L.el = reinterpret_cast<int*>(L.acc + (size_t)capL * accM * 3);
     L.la = L.el + capL;
     L.lb = L.la + capL;
@@ -112,6 +112,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
     // ---------------------------------------------------------- radial groups
     for (int gi = 0; gi < E.nrad; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
+        if (G.type != 2) continue;  // compact groups: k_sfcomp64
         for (int k = tid; k < n; k += kSFThreads) {
             double f = 0.0, df = 0.0;
             if (L.el[k] == G.e1 && L.d[k] < G.rc) cutoff_fdf(ctype, G, tanhPre, L.d[k], f, df);
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
             __syncthreads();
             if (tid < 4) {
                 double v = 0.0;
-                for (int w = 0; w < kSFWaves; ++w) v += L.red[w * 4 * kMaxMembers + tid];
+                for (int w = 0; w < kSFWaves; ++w) v += L.red[w * 4 * kMaxMembersExp + tid];
                 if (tid == 0) Gi[mb.sf] = scale_sf(M->scale[E.sf0 + mb.sf], v);
                 else if (deriv) dGi[mb.sf * 3 + tid - 1] = v;
             }
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
     // --------------------------------------------------------- angular groups
     for (int gi = E.nrad; gi < E.ngrp; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
+        if (G.type != 3 && G.type != 9) continue;  // compact groups: k_sfcomp64
         const int nmem = G.nmem;
         const bool same = G.e1 == G.e2;
         // wide angular groups (SymGrpExpAngw.cpp:156-380): no r_jk cutoff and
@@ -293,7 +295,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
         if (tid < nmem) {
             const MemberT& mb = M->mem[G.mem0 + tid];
             double v = 0.0;
-            for (int w = 0; w < kSFWaves; ++w) v += L.red[w * 4 * kMaxMembers + tid];
+            for (int w = 0; w < kSFWaves; ++w) v += L.red[w * 4 * kMaxMembersExp + tid];
             v *= mb.gnorm;
             Gi[mb.sf] = scale_sf(M->scale[E.sf0 + mb.sf], v);
         }
@@ -326,11 +328,11 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
 void launch_sf64(const ModelT* m, const ModelT& hm, const SysView& s, const SF64Out& o,
                  bool deriv, int max_nb, hipStream_t stream)
 {
-    if (s.natoms == 0) return;
+    if (s.natoms == 0 || !hm.has_exp) return;
     const int capL = ((std::max(max_nb, 1) + 31) / 32) * 32;
     const int mam = std::max(hm.max_mem_ang, 1);
     const int MAXM = mam <= 4 ? 4 : mam <= 8 ? 8 : mam <= 16 ? 16 : 32;
-    const size_t lds = sizeof(double) * ((size_t)capL * 6 + kSFWaves * 4 * kMaxMembers +
+    const size_t lds = sizeof(double) * ((size_t)capL * 6 + kSFWaves * 4 * kMaxMembersExp +
                                          (size_t)capL * mam * 3) +
                        sizeof(int) * (size_t)capL * 3;
     if (lds > 65536)

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include "train.hpp"
+#include "nnpx.h"
 
 namespace nnpx {
 namespace {
@@ -273,6 +274,7 @@ void launch_grad(const ModelT* m, const GradArgs& a, int blocks, bool fp32, hipS
 {
     if (blocks == 0) return;
     const size_t lds = grad_lds_bytes(a, fp32);
+    if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "network too wide for the gradient kernel's LDS staging"};
     if (fp32) hipLaunchKernelGGL(k_grad<float>, dim3(blocks), dim3(kWave), lds, stream, m, a);
     else hipLaunchKernelGGL(k_grad<double>, dim3(blocks), dim3(kWave), lds, stream, m, a);
 }

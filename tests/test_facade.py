@@ -78,7 +78,8 @@ def golden_run_dir(tmp_path, gdir=GOLD):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,natoms,z", [("H2O_RPBE-D3", 1080, {1, 8}), ("Cu2S_PBE", 144, {16, 29})])
+@pytest.mark.parametrize("name,natoms,z", [("H2O_RPBE-D3", 1080, {1, 8}), ("Cu2S_PBE", 144, {16, 29}),
+                                           ("Anisole_SCAN", 256, {1, 6, 8})])
 def test_nnpx_predict_golden_plain(tmp_path, name, natoms, z):
     need(PREDICT)
     print("case: nnpx-predict %s plain" % name, flush=True)

@@ -94,7 +94,7 @@ def test_golden_cu2s_mixed():
     pos, elem, box = read_input_data(os.path.join(gdir, "input.data"), ["S", "Cu"])
     m = n2p2_amd.Mode(gdir)
     cl, ce = m.conv_length, m.conv_energy
-    m.set_structures(remap(pos, box) * cl, elem, (box * cl)[None])
+    m.set_structures(remap(pos, box) * cl, elem, None if box is None else (box * cl)[None])
     m.evaluate(n2p2_amd.MIXED, forces=True)
     F = m.forces() * (1.0 / (ce / cl))
     Fg = np.array(json.load(open(os.path.join(gdir, "expected.json")))["forces"])
@@ -109,7 +109,7 @@ def test_golden_h2o_mixed():
     pos, elem, box = read_input_data(os.path.join(GOLD, "input.data"), ["H", "O"])
     m = n2p2_amd.Mode(GOLD)
     cl, ce = m.conv_length, m.conv_energy
-    m.set_structures(remap(pos, box) * cl, elem, (box * cl)[None])
+    m.set_structures(remap(pos, box) * cl, elem, None if box is None else (box * cl)[None])
     m.evaluate(n2p2_amd.MIXED, forces=True)
     o = Oracle(GOLD)
     o.read_structure(os.path.join(GOLD, "input.data"))
@@ -214,3 +214,20 @@ def test_autotune_keeps_results(water_dir):
     assert float(np.abs(m.forces() - F0).max()) <= 1e-4
     with pytest.raises(n2p2_amd.NNPError):
         m.set_tuning(100, 256, 512, 16)
+
+
+def test_mixed_rejects_compact_functions():
+    """The mixed form has no compact-function kernels: a compact model is
+    refused with NNPX_ERR_UNSUPPORTED (and the plain form still runs)."""
+    say("case: mixed form on a compact model (DMABN_SCAN) is refused")
+    gdir = os.path.join(HERE, "golden", "DMABN_SCAN")
+    pos, elem, box = read_input_data(os.path.join(gdir, "input.data"), ["H", "C", "N"])
+    m = n2p2_amd.Mode(gdir)
+    cl = m.conv_length
+    m.set_structures(remap(pos, box) * cl, elem, None if box is None else (box * cl)[None])
+    with pytest.raises(n2p2_amd.NNPError) as ei:
+        m.evaluate(n2p2_amd.MIXED, forces=True)
+    say("  refused:", ei.value)
+    assert ei.value.status == 7  # NNPX_ERR_UNSUPPORTED
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    assert np.isfinite(m.forces()).all()

@@ -12,7 +12,7 @@ import pytest
 
 import n2p2_amd
 from n2p2_amd import read_input_data, remap, synth
-from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle
+from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, compact_model
 
 pytestmark = pytest.mark.gpu
 
@@ -31,7 +31,8 @@ def water_dir(tmp_path_factory):
     return synth.water_model(str(tmp_path_factory.mktemp("water")))
 
 
-GOLDEN = {"H2O_RPBE-D3": ["H", "O"], "Cu2S_PBE": ["S", "Cu"]}
+GOLDEN = {"H2O_RPBE-D3": ["H", "O"], "Cu2S_PBE": ["S", "Cu"], "Anisole_SCAN": ["H", "C", "O"],
+          "DMABN_SCAN": ["H", "C", "N"], "Ethylbenzene_SCAN": ["H", "C"]}
 
 
 def predict_golden(mode, gdir=GOLD, symbols=("H", "O"), precision=n2p2_amd.PLAIN_FP64):
@@ -39,7 +40,7 @@ def predict_golden(mode, gdir=GOLD, symbols=("H", "O"), precision=n2p2_amd.PLAIN
     pos, elem, box = read_input_data(os.path.join(gdir, "input.data"), list(symbols))
     pos = remap(pos, box)
     cl, ce = mode.conv_length, mode.conv_energy
-    mode.set_structures(pos * cl, elem, (box * cl)[None])
+    mode.set_structures(pos * cl, elem, None if box is None else (box * cl)[None])
     mode.evaluate(precision, forces=True)
     n = len(elem)
     E = mode.energies()[0] / ce + n * mode.mean_energy
@@ -51,7 +52,8 @@ def predict_golden(mode, gdir=GOLD, symbols=("H", "O"), precision=n2p2_amd.PLAIN
 @pytest.mark.parametrize("name", sorted(GOLDEN))
 def test_golden_plain_fp64(name):
     """test/cpp/test_Prediction.cpp examples: H2O_RPBE-D3 (SF types 2/3,
-    CT_TANHU) and Cu2S_PBE (types 2/3/9, CT_POLY2, SCALESIGMA, softplus)."""
+    CT_TANHU), Cu2S_PBE (types 2/3/9, CT_POLY2, SCALESIGMA, softplus) and the
+    three SCAN models (compact types 20/22, normalised units)."""
     say("case: %s golden, plain FP64" % name)
     gdir = os.path.join(HERE, "golden", name)
     exp = json.load(open(os.path.join(gdir, "expected.json")))
@@ -84,7 +86,7 @@ def test_neighbor_list_identical(case, water_dir):
         o.set_structure(pos, elem, box)
         m = n2p2_amd.Mode(d)
         cl = m.conv_length
-        m.set_structures(remap(pos, box) * cl, elem, (box * cl)[None])
+        m.set_structures(remap(pos, box) * cl, elem, None if box is None else (box * cl)[None])
         o.build_neighbors(BRUTE)
     else:
         if case == "water192":
@@ -223,3 +225,31 @@ def test_update_positions(water_dir):
     o.predict(PULL, BRUTE)
     assert abs(m.energies()[0] - o.energy()) <= E_REL * abs(o.energy())
     assert np.abs(m.forces() - o.forces()).max() <= F_ABS
+
+
+@pytest.mark.parametrize("n_side", [4, 8])
+def test_compact_model_plain(n_side, tmp_path_factory):
+    """Compact-function water model (types 20/21/22, every subtype, narrow
+    members of different subtypes sharing (rl, rc)): G, dE/dG, energy and
+    forces against the oracle; 192 atoms (brute-force list) and 1,536 atoms
+    (cell list)."""
+    say("case: compact water model, %d atoms, plain FP64" % (3 * n_side ** 3))
+    d = compact_model(str(tmp_path_factory.mktemp("compact") / "m"))
+    pos, elem, box = synth.water_box(n_side)
+    m = n2p2_amd.Mode(d)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE if n_side == 4 else CELL)
+    G, Go = m.G(), o.G()
+    gerr = float((np.abs(G - Go) / np.maximum(np.abs(Go), 1.0)).max())
+    derr = float(np.abs(m.dEdG() - o.dEdG()).max())
+    E, Eo = m.energies()[0], o.energy()
+    dF = float(np.abs(m.forces() - o.forces()).max())
+    say("  G err %.2e  dEdG err %.2e  E rel %.2e  max|dF| %.2e (|F| rms %.3f)"
+        % (gerr, derr, abs(E - Eo) / abs(Eo), dF, float(np.sqrt(np.mean(o.forces() ** 2)))))
+    assert gerr <= 1e-12
+    assert derr <= 1e-10
+    assert abs(E - Eo) <= E_REL * abs(Eo)
+    assert dF <= F_ABS
