@@ -481,10 +481,20 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
             const GroupMixF gm = GM[E.grp0 + gi];
             const MemberMasks R = load_members(S, mf, nmem, nullptr);
+            // FAST: zeta and -eta log2 e in registers for the whole group
+            constexpr int NC = FAST ? MAXM : 1;
+            float cz[NC], ce[NC];
+            if constexpr (FAST) {
+#pragma unroll
+                for (int q = 0; q < MAXM; ++q) {
+                    cz[q] = S.mp[q].x;
+                    ce[q] = -S.mp[q].y;
+                }
+            }
             for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
                 const float4 gj = S.g[j], gk = S.g[k];
                 const float4 cj = S.c[j], ck = S.c[k];
-                const float rjk = sqrtf(r2jk);
+                const float rjk = __builtin_amdgcn_sqrtf(r2jk);
                 float fcjk, dfcjk;
                 fcjk = 1.f;
                 dfcjk = 0.f;
@@ -502,10 +512,9 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 if constexpr (FAST) {
 #pragma unroll
                     for (int q = 0; q < MAXM; ++q) {
-                        const float4 mq = S.mp[q];
-                        const float Ls = mq.w != 0.f ? L[1] : L[0];
-                        const float ec = fmaf(-mq.y, r2sum, lpfc);
-                        kahan(acc[q], cmp[q], fexp2(fmaf(mq.x, Ls, ec)));
+                        const bool s1 = (R.s1 >> q) & 1u;
+                        const float ec = fmaf(ce[q], r2sum, lpfc);
+                        kahan(acc[q], cmp[q], fexp2(fmaf(cz[q], s1 ? L[1] : L[0], ec)));
                     }
                 } else {
                     float ec = 0.f;
@@ -780,15 +789,45 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
         const MemberMasks R = load_members(S, mf, nmem, wi);
-        // contributions to the row atom j are summed in registers over the
-        // lane's run of equal j and flushed when j changes
+        // FAST: the member constants stay in registers for the whole group
+        // (zeta, -eta log2 e, the weights of B and C, and the weight of A
+        // split by lambda index so A needs no per-member select)
+        constexpr int NC = FAST ? MAXM : 1;
+        float cz[NC], ce[NC], cA0[NC], cA1[NC], cB[NC], cC[NC];
+        if constexpr (FAST) {
+#pragma unroll
+            for (int q = 0; q < MAXM; ++q) {
+                const float4 mq = S.mp[q];
+                const float4 wq = S.mp[kMaxMembersExp + q];
+                const bool s1 = (R.s1 >> q) & 1u;
+                cz[q] = mq.x;
+                ce[q] = -mq.y;
+                cB[q] = wq.y;
+                cC[q] = wq.z;
+                cA0[q] = s1 ? 0.f : wq.x;
+                cA1[q] = s1 ? wq.x : 0.f;
+            }
+        }
+        // Contributions to the row atom j and to the atom itself are summed
+        // in FP32 over the lane's run of equal j (a few triplets) and added
+        // in FP64 when j changes.
         int cur = -1;
-        double rx = 0.0, ry = 0.0, rz = 0.0;
+        float rx = 0.f, ry = 0.f, rz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+        auto flush = [&]() {
+            if (cur >= 0) {
+                lds_add(ax + cur, (double)rx);
+                lds_add(ay + cur, (double)ry);
+                lds_add(az + cur, (double)rz);
+                fx -= (double)sx;
+                fy -= (double)sy;
+                fz -= (double)sz;
+            }
+        };
         for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
-            const float rjk = sqrtf(r2jk);
-            const float ijk = 1.f / rjk;
+            const float ijk = __builtin_amdgcn_rsqf(r2jk);
+            const float rjk = r2jk * ijk;
             float fcjk, dfcjk;
             fcjk = 1.f;
             dfcjk = 0.f;
@@ -814,16 +853,17 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             // u_q = w vexp pl^(zeta-1) being the reference's fg times w
             float A = 0.f, B = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, C3 = 0.f;
             if constexpr (FAST) {
+                float A0 = 0.f, A1 = 0.f;
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q) {
-                    const float4 mq = S.mp[q];
-                    const float4 wq = S.mp[kMaxMembersExp + q];
-                    const bool s1 = mq.w != 0.f;
-                    const float t = fexp2(fmaf(mq.x, s1 ? L[1] : L[0], -mq.y * r2sum));
-                    B = fmaf(wq.y, t, B);
-                    C0 = fmaf(wq.z, t, C0);
-                    A = fmaf(wq.x, t * (s1 ? Rp[1] : Rp[0]), A);
+                    const bool s1 = (R.s1 >> q) & 1u;
+                    const float t = fexp2(fmaf(cz[q], s1 ? L[1] : L[0], ce[q] * r2sum));
+                    B = fmaf(cB[q], t, B);
+                    C0 = fmaf(cC[q], t, C0);
+                    A0 = fmaf(cA0[q], t, A0);
+                    A1 = fmaf(cA1[q], t, A1);
                 }
+                A = A0 * Rp[0] + A1 * Rp[1];
             } else {
                 float ec = 0.f, fij = 1.f, fik = 1.f, fjk = 1.f;
                 bool shifted = false;
@@ -869,30 +909,23 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             const float p1x = P1 * gj.x, p1y = P1 * gj.y, p1z = P1 * gj.z;
             const float p2x = P2 * gk.x, p2y = P2 * gk.y, p2z = P2 * gk.z;
             const float p3x = P3 * x3, p3y = P3 * y3, p3z = P3 * z3;
-            fx -= (double)(p1x + p2x);
-            fy -= (double)(p1y + p2y);
-            fz -= (double)(p1z + p2z);
             if (j != cur) {
-                if (cur >= 0) {
-                    lds_add(ax + cur, rx);
-                    lds_add(ay + cur, ry);
-                    lds_add(az + cur, rz);
-                }
+                flush();
                 cur = j;
-                rx = 0.0; ry = 0.0; rz = 0.0;
+                rx = 0.f; ry = 0.f; rz = 0.f;
+                sx = 0.f; sy = 0.f; sz = 0.f;
             }
-            rx += (double)(p1x + p3x);
-            ry += (double)(p1y + p3y);
-            rz += (double)(p1z + p3z);
+            sx += p1x + p2x;
+            sy += p1y + p2y;
+            sz += p1z + p2z;
+            rx += p1x + p3x;
+            ry += p1y + p3y;
+            rz += p1z + p3z;
             lds_add(ax + k, (double)(p2x - p3x));
             lds_add(ay + k, (double)(p2y - p3y));
             lds_add(az + k, (double)(p2z - p3z));
         });
-        if (cur >= 0) {
-            lds_add(ax + cur, rx);
-            lds_add(ay + cur, ry);
-            lds_add(az + cur, rz);
-        }
+        flush();
         __syncthreads();
     }
     fx = wave_sum(fx);
@@ -1007,8 +1040,8 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
             const float4 uj = du[j], uk = du[k];
-            const float rjk = sqrtf(r2jk);
-            const float ijk = 1.f / rjk;
+            const float ijk = __builtin_amdgcn_rsqf(r2jk);
+            const float rjk = r2jk * ijk;
             float fcjk, dfcjk;
             fcjk = 1.f;
             dfcjk = 0.f;

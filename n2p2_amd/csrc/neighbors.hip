@@ -18,6 +18,8 @@ namespace nnpx {
 namespace {
 
 constexpr int kNbMaxLds = 1024;  // max entries sorted in LDS per atom
+constexpr int kCandLds = 2048;   // candidates whose cell is tabulated in LDS
+constexpr int kCandUnroll = 4;   // candidates per lane and step
 
 __device__ inline bool gated_off(const SysView& s) { return s.gate && *s.gate == 0; }
 
@@ -141,6 +143,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
         // The 27 surrounding cells are flattened into one candidate range so
         // the wave tests 64 candidates per step with contiguous loads.
         __shared__ int cs[27], co[28], cim[27];
+        __shared__ unsigned char ccell[kCandLds];
         const int c = s.atom_cell[i] - S.cell0;
         const int c0 = c / (S.nc[1] * S.nc[2]), c1 = (c / S.nc[2]) % S.nc[1], c2 = c % S.nc[2];
         int start = 0, len = 0, img = 0;
@@ -167,35 +170,59 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
         if (lane == 26) co[27] = incl;
         __syncthreads();
         const int total = co[27];
+        // cell of every candidate (one LDS read per candidate in the loop
+        // below instead of a search over the 27 offsets)
+        const bool table = total <= kCandLds;
+        if (table && lane < 27)
+            for (int t = incl - len; t < incl; ++t) ccell[t] = (unsigned char)lane;
+        __syncthreads();
         const int self_img = pack_image(0, 0, 0);
-        for (int b0 = 0; b0 < total; b0 += kWave) {
-            const int x = b0 + lane;
-            bool hit = false;
-            long long key = 0;
-            if (x < total) {
-                int lo = 0, hi = 26;
-                while (lo < hi) {
-                    const int mid = (lo + hi + 1) >> 1;
-                    if (co[mid] <= x) lo = mid;
-                    else hi = mid - 1;
+        // kCandUnroll candidates per lane and step: their loads are
+        // independent, so the wave keeps several in flight; the order of the
+        // collected keys does not matter (they are sorted below).
+        for (int b0 = 0; b0 < total; b0 += kWave * kCandUnroll) {
+            bool hit[kCandUnroll];
+            long long key[kCandUnroll];
+            int nh = 0;
+#pragma unroll
+            for (int u = 0; u < kCandUnroll; ++u) {
+                const int x = b0 + u * kWave + lane;
+                hit[u] = false;
+                key[u] = 0;
+                if (x < total) {
+                    int lo = 0;
+                    if (table) {
+                        lo = ccell[x];
+                    } else {
+                        int hi = 26;
+                        while (lo < hi) {
+                            const int mid = (lo + hi + 1) >> 1;
+                            if (co[mid] <= x) lo = mid;
+                            else hi = mid - 1;
+                        }
+                    }
+                    const int p = cs[lo] + x - co[lo];
+                    const int j = s.cell_atoms[p];
+                    const int im = cim[lo];
+                    if (!(i == j && im == self_img)) {
+                        int q0, q1, q2;
+                        unpack_image(im, q0, q1, q2);
+                        double dr[3];
+                        pair_vector(ri, s.cell_pos + 3 * (size_t)p, S.box, q0, q1, q2, dr);
+                        hit[u] = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2;
+                        key[u] = ((long long)j << 21) | im;
+                    }
                 }
-                const int p = cs[lo] + x - co[lo];
-                const int j = s.cell_atoms[p];
-                const int im = cim[lo];
-                if (!(i == j && im == self_img)) {
-                    int q0, q1, q2;
-                    unpack_image(im, q0, q1, q2);
-                    double dr[3];
-                    pair_vector(ri, s.cell_pos + 3 * (size_t)p, S.box, q0, q1, q2, dr);
-                    hit = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2;
-                    key = ((long long)j << 21) | im;
+                nh += hit[u] ? 1 : 0;
+            }
+            const int inc = wave_incl_scan(nh);
+            int slot = count + inc - nh;
+#pragma unroll
+            for (int u = 0; u < kCandUnroll; ++u)
+                if (hit[u]) {
+                    if (slot < kNbMaxLds) keys[slot] = key[u];
+                    ++slot;
                 }
-            }
-            const int inc = wave_incl_scan(hit ? 1 : 0);
-            if (hit) {
-                const int slot = count + inc - 1;
-                if (slot < kNbMaxLds) keys[slot] = key;
-            }
             count += __shfl(inc, kWave - 1, kWave);
         }
         // Bitonic sort of the collected keys (padded to a power of two).
@@ -223,7 +250,10 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
     if (lane == 0) {
         s.nb_n[i] = min(count, s.cap);
         if (count > s.cap) atomicOr(&s.flags[0], 1);
-        atomicMax(&s.flags[1], count);
+        // every wave would otherwise hit the one address; a plain read first
+        // skips the atomic once the running maximum is at least this count
+        if (count > __hip_atomic_load(&s.flags[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&s.flags[1], count);
     }
 }
 
