@@ -21,7 +21,7 @@ struct StructT {
     int algo;          // 0 brute force, 1 cell list
     int atom0, natoms;
     int cell0;         // first cell of this structure in the global cell arrays
-    int pad;
+    int inside;        // every atom has fractional coordinates in [0, 1)
 };
 
 // Image shift packing: 7 bits per axis, biased by 64, so that the packed

@@ -105,18 +105,35 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
     const double* ri = s.pos + 3 * (size_t)i;
     int count = 0;
     if (S.algo != 1) {
-        const int p0 = S.periodic ? S.pbc[0] : 0;
-        const int p1 = S.periodic ? S.pbc[1] : 0;
-        const int p2 = S.periodic ? S.pbc[2] : 0;
+        // Image ranges: all of -p..p, or, when every atom lies inside the
+        // cell, only the images whose slab along each lattice direction
+        // comes within rc of atom i (slab distance gap * w, w the
+        // perpendicular width), with a margin; the loops keep the
+        // reference's (j, bc0, bc1, bc2) order.
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        if (S.periodic) {
+            const double rc = sqrt(rc2);
+            for (int k = 0; k < 3; ++k) {
+                lo[k] = -S.pbc[k];
+                hi[k] = S.pbc[k];
+                if (S.inside) {
+                    const double a = S.invbox[k], b = S.invbox[3 + k], c = S.invbox[6 + k];
+                    const double f = ri[0] * a + ri[1] * b + ri[2] * c;
+                    const double r = rc * sqrt(a * a + b * b + c * c) * (1.0 + 1e-9) + 1e-9;
+                    lo[k] = max(lo[k], -(int)floor(f + r));
+                    hi[k] = min(hi[k], (int)floor(1.0 - f + r));
+                }
+            }
+        }
         for (int jb = 0; jb < S.natoms; jb += kWave) {
             const int jl = jb + lane;
             const int j = S.atom0 + jl;
             int hits = 0;
             if (jl < S.natoms) {
                 const double* rj = s.pos + 3 * (size_t)j;
-                for (int b0 = -p0; b0 <= p0; ++b0)
-                    for (int b1 = -p1; b1 <= p1; ++b1)
-                        for (int b2 = -p2; b2 <= p2; ++b2) {
+                for (int b0 = lo[0]; b0 <= hi[0]; ++b0)
+                    for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
+                        for (int b2 = lo[2]; b2 <= hi[2]; ++b2) {
                             if (i == j && b0 == 0 && b1 == 0 && b2 == 0) continue;
                             double dr[3];
                             pair_vector(ri, rj, S.box, b0, b1, b2, dr);
@@ -127,9 +144,9 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
             int slot = count + incl - hits;
             if (hits) {
                 const double* rj = s.pos + 3 * (size_t)j;
-                for (int b0 = -p0; b0 <= p0; ++b0)
-                    for (int b1 = -p1; b1 <= p1; ++b1)
-                        for (int b2 = -p2; b2 <= p2; ++b2) {
+                for (int b0 = lo[0]; b0 <= hi[0]; ++b0)
+                    for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
+                        for (int b2 = lo[2]; b2 <= hi[2]; ++b2) {
                             if (i == j && b0 == 0 && b1 == 0 && b2 == 0) continue;
                             double dr[3];
                             pair_vector(ri, rj, S.box, b0, b1, b2, dr);

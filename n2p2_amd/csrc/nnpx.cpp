@@ -310,10 +310,13 @@ void choose_algorithms(nnpx_ctx* c, const double* pos)
         StructT& t = c->st_host[s];
         t.algo = 0;
         t.cell0 = 0;
-        bool cells = t.periodic && t.nc[0] >= 3 && t.nc[1] >= 3 && t.nc[2] >= 3 && c->nbr_algo != 1;
-        if (cells)
-            for (int64_t i = c->atom_off[s]; i < c->atom_off[s + 1] && cells; ++i)
-                cells = inside_cell(t, pos + 3 * i);
+        t.inside = 0;
+        if (t.periodic) {
+            t.inside = 1;
+            for (int64_t i = c->atom_off[s]; i < c->atom_off[s + 1] && t.inside; ++i)
+                t.inside = inside_cell(t, pos + 3 * i) ? 1 : 0;
+        }
+        const bool cells = t.inside && t.nc[0] >= 3 && t.nc[1] >= 3 && t.nc[2] >= 3 && c->nbr_algo != 1;
         if (cells) {
             t.algo = 1;
             t.cell0 = c->ncells;

@@ -108,6 +108,41 @@ def test_neighbor_list_identical(case, water_dir):
     say("  %d entries identical" % n)
 
 
+@pytest.mark.parametrize("case", ["batch24", "triclinic"])
+def test_neighbor_list_images(case, water_dir):
+    """Brute-force lists of cells smaller than 3 rc (images pruned by slab
+    distance): every structure of an uneven batch, and a skewed triclinic
+    cell with several image shells, entry for entry against the oracle."""
+    say("case: neighbour list with images,", case)
+    if case == "batch24":
+        pos, elem, boxes, off = synth.random_batch(n_struct=24, nmin=8, nmax=160)
+        pos = np.vstack([remap(pos[off[s]:off[s + 1]], boxes[s]) for s in range(24)])
+    else:
+        rng = np.random.default_rng(12)
+        boxes = np.array([[[5.0, 0.0, 0.0], [1.7, 4.6, 0.0], [-1.2, 0.9, 4.1]]])
+        f = rng.random((20, 3))
+        pos = remap(f @ boxes[0], boxes[0])
+        elem = np.array([1] * 7 + [0] * 13, np.int32)
+        off = np.array([0, 20])
+    m = n2p2_amd.Mode(water_dir)
+    m.set_structures(pos, elem, boxes, off)
+    m.calculate_neighbor_list()
+    c1, j1, s1 = m.neighbors()
+    e0 = np.concatenate([[0], np.cumsum(c1)])
+    total = 0
+    for s in range(len(off) - 1):
+        a, b = off[s], off[s + 1]
+        o = Oracle(water_dir)
+        o.set_structure(pos[a:b], elem[a:b], boxes[s])
+        o.build_neighbors(BRUTE)
+        c2, j2, s2 = o.neighbors()
+        assert (c1[a:b] == c2).all(), s
+        assert (j1[e0[a]:e0[b]] - a == j2).all(), s
+        assert (s1[e0[a]:e0[b]] == s2).all(), s
+        total += int(c2.sum())
+    say("  %d entries identical over %d structures" % (total, len(off) - 1))
+
+
 def check_against_oracle(m, o, force_mode=PULL):
     o.predict(force_mode, BRUTE)
     G, Go = m.G(), o.G()
