@@ -231,7 +231,8 @@ nnpx_status nnpx_md_energies(nnpx_ctx* ctx, double* epot, double* ekin,
  * SF, force and tangent kernels (LDS per wave against batching) and the
  * atoms per block of the gradient kernel.  Results are not changed by the
  * choice beyond summation order.  values[4] = {trip_sf, trip_force,
- * trip_tangent, grad_atoms}. */
+ * trip_tangent, grad_atoms}; triplet buffers are multiples of 64 in
+ * [64, 4096]. */
 nnpx_status nnpx_autotune(nnpx_ctx* ctx);
 nnpx_status nnpx_get_tuning(const nnpx_ctx* ctx, int* values);
 nnpx_status nnpx_set_tuning(nnpx_ctx* ctx, const int* values);

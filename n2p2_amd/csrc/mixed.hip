@@ -119,8 +119,8 @@ __device__ inline GroupF group_f(const GroupT& g)
     return GroupF{(float)g.rc, (float)g.rc2, (float)g.rci, (float)g.iw, (float)g.rcinv};
 }
 
-constexpr int kTripDefault = 384;  // compacted triplets buffered per wave (tunable)
 constexpr int kRadChunk = 8;  // radial members accumulated per pass
+constexpr int kFill = 1;      // candidate pairs per lane and step of the triplet compaction (more measured slower)
 
 // LDS staging of one atom's neighbours (FP32 geometry) and the compacted
 // triplet buffer.
@@ -280,8 +280,15 @@ __device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
 // divergence (like elements: the folded order (a, a+d mod n), a balanced
 // rectangle; unlike elements: the full product), tested, and the passing
 // ones compacted through LDS so the member loops run with full waves.
-template <typename Work>
-__device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work)
+struct NoFlush {
+    __device__ void operator()() const {}
+};
+
+// `flush` runs after each buffer of triplets (a few per lane), so callers can
+// keep short FP32 partial sums and fold them into their long accumulators.
+template <typename Work, typename Flush = NoFlush>
+__device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work,
+                                        Flush&& flush = Flush{})
 {
     const int lane = threadIdx.x;
     int full, total;
@@ -298,41 +305,51 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
     while (t0 < total) {
         // fill the buffer with passing pairs ...
         int cnt = 0;
-        while (t0 < total && cnt <= S.trip - kWave) {
-            const int t = t0 + lane;
-            bool ok = false;
-            unsigned jk = 0;
-            float r2 = 0.f;
-            if (t < total) {
-                const int u = same && t >= full ? t - full : t;
-                int q = (int)((float)u * inv);
-                int r = u - q * nA;
-                if (r < 0) { --q; r += nA; }
-                else if (r >= nA) { ++q; r -= nA; }
-                int a, b;
-                if (!same) { a = r; b = q; }
-                else if (t < full) { a = r; b = r + q + 1; if (b >= nA) b -= nA; }
-                else { a = u; b = u + nA / 2; }
-                int j = S.la[a], k = lb[b];
-                if (j > k) { const int x = j; j = k; k = x; }
-                const float4 gj = S.g[j], gk = S.g[k];
-                const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
-                r2 = dx * dx + dy * dy + dz * dz;
-                ok = r2 < rc2;
-                jk = (unsigned)j | ((unsigned)k << 16);
+        while (t0 < total && cnt <= S.trip - kFill * kWave) {
+            bool ok[kFill];
+            unsigned jk[kFill];
+            float r2[kFill];
+#pragma unroll
+            for (int f = 0; f < kFill; ++f) {
+                const int t = t0 + f * kWave + lane;
+                ok[f] = false;
+                jk[f] = 0;
+                r2[f] = 0.f;
+                if (t < total) {
+                    const int u = same && t >= full ? t - full : t;
+                    int q = (int)((float)u * inv);
+                    int r = u - q * nA;
+                    if (r < 0) { --q; r += nA; }
+                    else if (r >= nA) { ++q; r -= nA; }
+                    int a, b;
+                    if (!same) { a = r; b = q; }
+                    else if (t < full) { a = r; b = r + q + 1; if (b >= nA) b -= nA; }
+                    else { a = u; b = u + nA / 2; }
+                    int j = S.la[a], k = lb[b];
+                    if (j > k) { const int x = j; j = k; k = x; }
+                    const float4 gj = S.g[j], gk = S.g[k];
+                    const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
+                    r2[f] = dx * dx + dy * dy + dz * dz;
+                    ok[f] = r2[f] < rc2;
+                    jk[f] = (unsigned)j | ((unsigned)k << 16);
+                }
             }
-            const unsigned long long m = __ballot(ok);
-            if (ok) {
-                const int pos = cnt + __popcll(m & __lanemask_lt());
-                S.tjk[pos] = jk;
-                S.tr2[pos] = r2;
+#pragma unroll
+            for (int f = 0; f < kFill; ++f) {
+                const unsigned long long m = __ballot(ok[f]);
+                if (ok[f]) {
+                    const int pos = cnt + __popcll(m & __lanemask_lt());
+                    S.tjk[pos] = jk[f];
+                    S.tr2[pos] = r2[f];
+                }
+                cnt += __popcll(m);
             }
-            cnt += __popcll(m);
-            t0 += kWave;
+            t0 += kFill * kWave;
         }
         __syncthreads();
         // ... then run the member loops with full waves
         for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        flush();
         __syncthreads();
     }
 }
@@ -360,40 +377,49 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
     int t0 = 0;
     while (t0 < total) {
         int cnt = 0;
-        while (t0 < total && cnt <= S.trip - kWave) {
-            const int t = t0 + lane;
-            bool ok = false;
-            unsigned jk = 0;
-            float r2 = 0.f;
-            if (t < total) {
-                int a, b;
-                if (t < full) {
-                    int q = (int)((float)t * inv);
-                    int r = t - q * R;
-                    if (r < 0) { --q; r += R; }
-                    else if (r >= R) { ++q; r -= R; }
-                    a = q;
-                    if (same) { b = a + r + 1; if (b >= nA) b -= nA; }
-                    else b = r;
-                } else {
-                    a = t - full;
-                    b = a + nA / 2;
+        while (t0 < total && cnt <= S.trip - kFill * kWave) {
+            bool ok[kFill];
+            unsigned jk[kFill];
+            float r2[kFill];
+#pragma unroll
+            for (int f = 0; f < kFill; ++f) {
+                const int t = t0 + f * kWave + lane;
+                ok[f] = false;
+                jk[f] = 0;
+                r2[f] = 0.f;
+                if (t < total) {
+                    int a, b;
+                    if (t < full) {
+                        int q = (int)((float)t * inv);
+                        int r = t - q * R;
+                        if (r < 0) { --q; r += R; }
+                        else if (r >= R) { ++q; r -= R; }
+                        a = q;
+                        if (same) { b = a + r + 1; if (b >= nA) b -= nA; }
+                        else b = r;
+                    } else {
+                        a = t - full;
+                        b = a + nA / 2;
+                    }
+                    const int j = S.la[a], k = lb[b];
+                    const float4 gj = S.g[j], gk = S.g[k];
+                    const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
+                    r2[f] = dx * dx + dy * dy + dz * dz;
+                    ok[f] = r2[f] < rc2;
+                    jk[f] = (unsigned)j | ((unsigned)k << 16);
                 }
-                const int j = S.la[a], k = lb[b];
-                const float4 gj = S.g[j], gk = S.g[k];
-                const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
-                r2 = dx * dx + dy * dy + dz * dz;
-                ok = r2 < rc2;
-                jk = (unsigned)j | ((unsigned)k << 16);
             }
-            const unsigned long long m = __ballot(ok);
-            if (ok) {
-                const int pos = cnt + __popcll(m & __lanemask_lt());
-                S.tjk[pos] = jk;
-                S.tr2[pos] = r2;
+#pragma unroll
+            for (int f = 0; f < kFill; ++f) {
+                const unsigned long long m = __ballot(ok[f]);
+                if (ok[f]) {
+                    const int pos = cnt + __popcll(m & __lanemask_lt());
+                    S.tjk[pos] = jk[f];
+                    S.tr2[pos] = r2[f];
+                }
+                cnt += __popcll(m);
             }
-            cnt += __popcll(m);
-            t0 += kWave;
+            t0 += kFill * kWave;
         }
         __syncthreads();
         const int per = (cnt + kWave - 1) / kWave;
@@ -481,14 +507,17 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
             const GroupMixF gm = GM[E.grp0 + gi];
             const MemberMasks R = load_members(S, mf, nmem, nullptr);
-            // FAST: zeta and -eta log2 e in registers for the whole group
+            // FAST: zeta and -eta log2 e in registers for the whole group;
+            // terms summed in FP32 over one triplet buffer (a few per lane),
+            // then folded into the compensated sums
             constexpr int NC = FAST ? MAXM : 1;
-            float cz[NC], ce[NC];
+            float cz[NC], ce[NC], part[NC];
             if constexpr (FAST) {
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q) {
                     cz[q] = S.mp[q].x;
                     ce[q] = -S.mp[q].y;
+                    part[q] = 0.f;
                 }
             }
             for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
@@ -514,7 +543,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                     for (int q = 0; q < MAXM; ++q) {
                         const bool s1 = (R.s1 >> q) & 1u;
                         const float ec = fmaf(ce[q], r2sum, lpfc);
-                        kahan(acc[q], cmp[q], fexp2(fmaf(cz[q], s1 ? L[1] : L[0], ec)));
+                        part[q] += fexp2(fmaf(cz[q], s1 ? L[1] : L[0], ec));
                     }
                 } else {
                     float ec = 0.f;
@@ -536,6 +565,14 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                             const float Ls = mw.w != 0.f ? hi : lo;
                             kahan(acc[q], cmp[q], fexp2(fmaf(mq.x, Ls, ec)));
                         }
+                }
+            }, [&]() {
+                if constexpr (FAST) {
+#pragma unroll
+                    for (int q = 0; q < MAXM; ++q) {
+                        kahan(acc[q], cmp[q], part[q]);
+                        part[q] = 0.f;
+                    }
                 }
             });
         }

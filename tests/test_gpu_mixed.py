@@ -208,12 +208,14 @@ def test_autotune_keeps_results(water_dir):
     d = float(np.abs(m.forces() - F0).max())
     say("  max|dF| after tuning %.2e" % d)
     assert d <= 1e-4  # bf16 operand splits amplify FP32-ulp input changes to ~1e-5 relative
-    m.set_tuning(128, 256, 512, 16)
-    assert m.tuning() == {"trip_sf": 128, "trip_force": 256, "trip_tangent": 512, "grad_atoms": 16}
+    m.set_tuning(128, 320, 512, 16)
+    assert m.tuning() == {"trip_sf": 128, "trip_force": 320, "trip_tangent": 512, "grad_atoms": 16}
     m.evaluate(n2p2_amd.MIXED, forces=True)
     assert float(np.abs(m.forces() - F0).max()) <= 1e-4
     with pytest.raises(n2p2_amd.NNPError):
         m.set_tuning(100, 256, 512, 16)
+    with pytest.raises(n2p2_amd.NNPError):
+        m.set_tuning(0, 256, 512, 16)
 
 
 def test_mixed_rejects_compact_functions():
