@@ -354,12 +354,14 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
     }
 }
 
-// Row-major variant for the force kernel: candidates are enumerated a-major
-// (all partners of la[a] consecutively), compacted the same way, and each lane
-// takes a contiguous chunk of the compacted buffer, so it sees long runs of
-// one first atom j and can keep j's force contribution in registers.
-template <typename Work>
-__device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work)
+// Row-major variant: candidates are enumerated a-major (all partners of
+// la[a] consecutively), compacted the same way, and each lane takes a
+// contiguous chunk of the compacted buffer, so it sees long runs of one first
+// atom j: j's geometry stays in registers over the run and the force kernel
+// keeps j's contribution there too.  `flush` runs after each buffer.
+template <typename Work, typename Flush = NoFlush>
+__device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work,
+                                             Flush&& flush = Flush{})
 {
     const int lane = threadIdx.x;
     int full, total, R;
@@ -425,6 +427,7 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
         const int per = (cnt + kWave - 1) / kWave;
         const int x0 = lane * per, x1 = min(cnt, x0 + per);
         for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        flush();
         __syncthreads();
     }
 }
@@ -520,9 +523,16 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                     part[q] = 0.f;
                 }
             }
-            for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
-                const float4 gj = S.g[j], gk = S.g[k];
-                const float4 cj = S.c[j], ck = S.c[k];
+            int jc = -1;  // j of the cached geometry (runs of one j per lane)
+            float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
+            for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
+                if (j != jc) {
+                    gj = S.g[j];
+                    cj = S.c[j];
+                    jc = j;
+                }
+                const float4 gk = S.g[k];
+                const float4 ck = S.c[k];
                 const float rjk = __builtin_amdgcn_sqrtf(r2jk);
                 float fcjk, dfcjk;
                 fcjk = 1.f;
@@ -860,9 +870,18 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                 fz -= (double)sz;
             }
         };
+        float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;  // geometry of the run's j
         for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
-            const float4 gj = S.g[j], gk = S.g[k];
-            const float4 cj = S.c[j], ck = S.c[k];
+            if (j != cur) {
+                flush();
+                cur = j;
+                rx = 0.f; ry = 0.f; rz = 0.f;
+                sx = 0.f; sy = 0.f; sz = 0.f;
+                gj = S.g[j];
+                cj = S.c[j];
+            }
+            const float4 gk = S.g[k];
+            const float4 ck = S.c[k];
             const float ijk = __builtin_amdgcn_rsqf(r2jk);
             const float rjk = r2jk * ijk;
             float fcjk, dfcjk;
@@ -946,12 +965,6 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             const float p1x = P1 * gj.x, p1y = P1 * gj.y, p1z = P1 * gj.z;
             const float p2x = P2 * gk.x, p2y = P2 * gk.y, p2z = P2 * gk.z;
             const float p3x = P3 * x3, p3y = P3 * y3, p3z = P3 * z3;
-            if (j != cur) {
-                flush();
-                cur = j;
-                rx = 0.f; ry = 0.f; rz = 0.f;
-                sx = 0.f; sy = 0.f; sz = 0.f;
-            }
             sx += p1x + p2x;
             sy += p1y + p2y;
             sz += p1z + p2z;
