@@ -209,6 +209,12 @@ public:
     Structure structure;
 };
 
+/* Column header block of n2p2's output files (createFileHeader +
+ * appendLinesToFile, utility.cpp:111-204): title lines, column table, column
+ * numbers and names right-aligned to the given widths. */
+void writeFileHeader(std::ofstream& o, std::vector<std::string> const& title, std::vector<size_t> const& size,
+                     std::vector<std::string> const& name, std::vector<std::string> const& info);
+
 }  // namespace nnpx
 
 #endif /* NNPX_MODE_HPP */

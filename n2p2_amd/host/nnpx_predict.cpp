@@ -13,9 +13,7 @@
 #include <cstdlib>
 #include <exception>
 #include <fstream>
-#include <iomanip>
 #include <iostream>
-#include <sstream>
 #include <string>
 #include <vector>
 
@@ -32,44 +30,6 @@ std::string strpr(const char* f, ...)
     vsnprintf(buf, sizeof buf, f, ap);
     va_end(ap);
     return buf;
-}
-
-// Column header block of n2p2's output files (utility.cpp:111-204 layout).
-void write_header(std::ofstream& o, std::vector<std::string> const& title, std::vector<size_t> const& size,
-                  std::vector<std::string> const& name, std::vector<std::string> const& info)
-{
-    const size_t std_width = 80;
-    const std::string sep = std::string(std_width, '#') + "\n";
-    o << sep;
-    for (auto const& t : title) o << "# " << t << "\n";
-    o << sep;
-    size_t wn = 0;
-    for (auto const& n : name) wn = std::max(wn, n.size());
-    std::ostringstream s;
-    s << "# " << std::left << std::setw(4) << "Col" << ' ' << std::left << std::setw(wn) << "Name" << ' '
-      << "Description\n";
-    o << s.str() << sep;
-    for (size_t i = 0; i < name.size(); ++i) {
-        std::ostringstream r;
-        r << "# " << std::left << std::setw(4) << i + 1 << ' ' << std::left << std::setw(wn) << name[i] << ' '
-          << info[i] << '\n';
-        o << r.str();
-    }
-    size_t wd = 0;
-    for (size_t c : size) wd += c + 1;
-    wd -= 1;
-    o << (wd > std_width ? std::string(wd, '#') + "\n" : sep);
-    std::ostringstream num, nam;
-    num << '#';
-    nam << '#';
-    for (size_t i = 0; i < size.size(); ++i) {
-        size_t n = size[i] - (i == 0 ? 2 : 0);
-        std::string nm = name[i];
-        if (nm.size() > n) nm.erase(n);
-        num << ' ' << std::right << std::setw(n) << i + 1;
-        nam << ' ' << std::right << std::setw(n) << nm;
-    }
-    o << num.str() << '\n' << nam.str() << '\n' << std::string(wd, '#') << "\n";
 }
 
 int usage(const char* argv0)
@@ -123,7 +83,7 @@ int main(int argc, char* argv[])
         log << "Writing output files...\n";
 
         std::ofstream f("energy.out");
-        write_header(f, {"Energy comparison."}, {10, 10, 24, 24, 24, 24},
+        nnpx::writeFileHeader(f, {"Energy comparison."}, {10, 10, 24, 24, 24, 24},
                      {"conf", "natoms", "Eref", "Ennp", "Ediff", "E_offset"},
                      {"Configuration index (starting with 1).", "Number of atoms in configuration.",
                       "Reference potential energy.", "Potential energy predicted by NNP.",
@@ -134,7 +94,7 @@ int main(int argc, char* argv[])
         f.close();
 
         f.open("nnatoms.out");
-        write_header(f, {"Energy contributions calculated from NNP."}, {10, 10, 3, 24, 24, 24, 24},
+        nnpx::writeFileHeader(f, {"Energy contributions calculated from NNP."}, {10, 10, 3, 24, 24, 24, 24},
                      {"conf", "index", "Z", "Qref", "Qnnp", "Eref_atom", "Ennp_atom"},
                      {"Configuration index (starting with 1).", "Atom index (starting with 1).",
                       "Nuclear charge of atom.", "Reference atomic charge.", "NNP atomic charge.",
@@ -146,7 +106,7 @@ int main(int argc, char* argv[])
         f.close();
 
         f.open("nnforces.out");
-        write_header(f, {"Atomic force comparison (ordered by atom index)."}, {10, 10, 24, 24, 24, 24, 24, 24},
+        nnpx::writeFileHeader(f, {"Atomic force comparison (ordered by atom index)."}, {10, 10, 24, 24, 24, 24, 24, 24},
                      {"conf", "index", "fxRef", "fyRef", "fzRef", "fx", "fy", "fz"},
                      {"Configuration index (starting with 1).", "Atom index (starting with 1).",
                       "Reference force in x direction.", "Reference force in y direction.",
