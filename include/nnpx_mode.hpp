@@ -125,6 +125,8 @@ public:
     void readFromFile(std::string const& fileName = "input.data");
     void readFromLines(std::vector<std::string> const& lines);
     void remap();
+    /* One atom (Structure::remap(Atom&), Structure.cpp:1224-1248). */
+    void remap(Atom& atom);
     void calculateInverseBox();
     void calculateVolume();
     /* Records the cutoff; the list itself is built on the device. */

@@ -209,15 +209,18 @@ void Structure::calculateVolume()
 
 void Structure::remap()
 {
-    for (auto& at : atoms) {
-        Vec3D f = add(add(scale(at.r[0], invbox[0]), scale(at.r[1], invbox[1])), scale(at.r[2], invbox[2]));
-        for (int c = 0; c < 3; ++c) {
-            if (f[c] > 1.0) f[c] -= (int)f[c];
-            if (f[c] < 0.0) f[c] += 1.0 - (int)f[c];
-            if (f[c] == 1.0) f[c] = 0.0;
-        }
-        at.r = add(add(scale(f[0], box[0]), scale(f[1], box[1])), scale(f[2], box[2]));
+    for (auto& at : atoms) remap(at);
+}
+
+void Structure::remap(Atom& at)
+{
+    Vec3D f = add(add(scale(at.r[0], invbox[0]), scale(at.r[1], invbox[1])), scale(at.r[2], invbox[2]));
+    for (int c = 0; c < 3; ++c) {
+        if (f[c] > 1.0) f[c] -= (int)f[c];
+        if (f[c] < 0.0) f[c] += 1.0 - (int)f[c];
+        if (f[c] == 1.0) f[c] = 0.0;
     }
+    at.r = add(add(scale(f[0], box[0]), scale(f[1], box[1])), scale(f[2], box[2]));
 }
 
 void Structure::calculateNeighborList(double cutoffRadius, bool)

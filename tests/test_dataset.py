@@ -190,3 +190,35 @@ def test_dataset_golden_normalised(tmp_path):
     say("  Ennp_phys %.10e  golden %.10e" % (e[0, 3], exp))
     assert abs(e[0, 3] - exp) <= 1e-8 * abs(exp)
     assert not os.path.exists(d / "forces.comp")
+
+
+CHECKF = os.path.join(ROOT, "n2p2_amd", "bin", "nnpx-checkf")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["Cu2S_PBE", "DMABN_SCAN"])
+def test_checkf_goldens(name, tmp_path):
+    """nnpx-checkf (nnp-checkf semantics, delta 1e-4): every structure OK
+    (max |F_analytic - F_numeric| <= 10 delta^2), analytic forces equal the
+    golden forces; Cu2S_PBE periodic, DMABN_SCAN isolated in normalised
+    units."""
+    need(CHECKF)
+    gdir = os.path.join(HERE, "golden", name)
+    d = tmp_path / "run"
+    d.mkdir()
+    for fn in os.listdir(gdir):
+        if fn != "expected.json":
+            shutil.copy(os.path.join(gdir, fn), d / fn)
+    say("case: nnpx-checkf on %s" % name)
+    rc, so, se = run([CHECKF], cwd=str(d))
+    assert rc == 0, se
+    f = read_table(d / "checkf-forces.out", 5)
+    s = read_table(d / "checkf-summary.out", 4)
+    exp = np.array(json.load(open(os.path.join(gdir, "expected.json")))["forces"]).reshape(-1)
+    err = float(np.abs(f[:, 3] - f[:, 4]).max())
+    say("  %d forces, max |analytic - numeric| %.2e, max |analytic - golden| %.2e"
+        % (len(f), err, float(np.abs(f[:, 3] - exp).max())))
+    assert len(f) == exp.size and s.shape == (1, 4)
+    assert np.abs(f[:, 3] - exp).max() <= 1e-8
+    assert err <= 10 * 1e-4 ** 2 and abs(s[0, 3] - err) <= 1e-12
+    assert "WARNING" not in open(d / "nnp-checkf.log.0000").read()
