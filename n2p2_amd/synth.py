@@ -192,6 +192,17 @@ def _parse(line, symbols):
     w = line.split()
     idx = {s: i for i, s in enumerate(symbols)}
     t = int(w[1])
+    if t == 12:
+        return dict(type=12, ec=idx[w[0]], e1=-1, e2=-1, eta=float(w[2]), rs=float(w[3]), rc=float(w[4]),
+                    lam=0.0, zeta=0.0)
+    if t == 13:
+        return dict(type=13, ec=idx[w[0]], e1=-1, e2=-1, eta=float(w[2]), rs=float(w[3]), lam=float(w[4]),
+                    zeta=float(w[5]), rc=float(w[6]))
+    if t == 23:
+        return dict(type=23, ec=idx[w[0]], e1=-1, e2=-1, rl=float(w[2]), rc=float(w[3]), sub=w[4], al=0.0, ar=0.0)
+    if t in (24, 25):
+        return dict(type=t, ec=idx[w[0]], e1=-1, e2=-1, rl=float(w[2]), rc=float(w[3]), al=float(w[4]),
+                    ar=float(w[5]), sub=w[6])
     if t == 20:
         return dict(type=20, ec=idx[w[0]], e1=idx[w[2]], e2=idx[w[2]], rl=float(w[3]), rc=float(w[4]),
                     sub=w[5], al=0.0, ar=0.0)
@@ -208,6 +219,10 @@ def _parse(line, symbols):
 
 
 def _sort_key(s):
+    if s["type"] >= 23:  # weighted compact: subtype, rc, rl, angles
+        return (s["type"], s["sub"], s["rc"], s["rl"], s["al"], s["ar"])
+    if s["type"] in (12, 13):  # weighted Gaussian: rc, eta, rs, zeta, lambda
+        return (s["type"], s["rc"], s["eta"], s["rs"], s["zeta"], s["lam"])
     if s["type"] >= 20:  # SymFncCompRad.cpp:48-68, SymFncCompAngw.cpp:56-82
         return (s["type"], s["sub"], s["e1"], s["e2"], s["rc"], s["rl"], s["al"], s["ar"])
     if s["type"] == 2:
@@ -381,6 +396,38 @@ def compact_sf_lines():
                   "%s 22 O O 1.0 6.0 40.0 140.0 e" % c, "%s 22 H H -6.0 6.0 -90.0 90.0 p4" % c,
                   "%s 22 H H 0.5 5.0 20.0 100.0 p1" % c, "%s 22 H O -5.0 5.0 90.0 270.0 p3a" % c]
     return lines
+
+
+def weighted_sf_lines(rc=6.0):
+    """Water SFs mixing element-resolved Gaussian functions (types 2, 3) with
+    the element-weighted ones (12, 13, 23, 24, 25), so that derivative slots
+    of both kinds interleave."""
+    lines = []
+    for c in ("H", "O"):
+        lines += ["%s 2 H 0.01 0.0 %.1f" % (c, rc), "%s 3 H O 0.005 1.0 2.0 %.1f" % (c, rc)]
+        lines += ["%s 12 %s %s %.1f" % (c, eta, rs, rc) for eta, rs in ((0.001, 0.0), (0.02, 1.5), (0.1, 3.0))]
+        lines += ["%s 13 0.002 0.0 %s %s %.1f" % (c, lam, zeta, rc) for lam in (1.0, -1.0) for zeta in (1.0, 4.0)]
+        lines += ["%s 13 0.01 1.0 1.0 2.5 %.1f" % (c, rc)]
+        lines += ["%s 23 -6.0 6.0 p2" % c, "%s 23 0.5 4.5 p3a" % c, "%s 23 1.0 5.5 e" % c]
+        lines += ["%s 24 -6.0 6.0 0.0 180.0 p2" % c, "%s 24 -6.0 6.0 60.0 150.0 p2" % c,
+                  "%s 24 -6.0 6.0 -90.0 90.0 p2a" % c, "%s 24 0.0 5.0 90.0 270.0 p4" % c]
+        lines += ["%s 25 -6.0 6.0 0.0 180.0 p2" % c, "%s 25 0.5 5.0 40.0 140.0 p1" % c,
+                  "%s 25 -5.0 5.0 90.0 270.0 p3a" % c]
+    return lines
+
+
+def weighted_water_model(directory, rows=None, seed=SEED_WEIGHTS, cutoff_type="2 0.0"):
+    """A water model on weighted_sf_lines() (30-30-1 networks, tanh), scaled
+    and centred with `rows`, unscaled without (used to measure the rows)."""
+    symbols = ("H", "O")
+    lines = weighted_sf_lines()
+    sfs = sorted_sfs(lines, symbols)
+    scaled = rows is not None
+    if not scaled:
+        rows = [(e + 1, k + 1, 0.0, 1.0, 0.0, 1.0) for e in range(2) for k in range(len(sfs[e]))]
+    write_model(directory, symbols, lines, (30, 30), ("t", "t", "l"), rows, seed=seed, scaling=scaled,
+                cutoff_type=cutoff_type)
+    return directory
 
 
 def compact_water_model(directory, rows=None, seed=SEED_WEIGHTS):

@@ -213,7 +213,11 @@ enum Scaling { ST_NONE, ST_SCALE, ST_CENTER, ST_SCALECENTER, ST_SCALESIGMA };
 // built from compact functions (SymFncCompRad, SymFncCompAngn,
 // SymFncCompAngw).
 inline bool is_compact(int type) { return type >= 20; }
-inline bool is_radial(int type) { return type == 2 || type == 20; }
+inline bool is_radial(int type) { return type == 2 || type == 12 || type == 20 || type == 23; }
+// Weighted types (SymFncExpRadWeighted, SymFncExpAngnWeighted,
+// SymFncCompRadWeighted, SymFncCompAngnWeighted, SymFncCompAngwWeighted):
+// every neighbour element contributes, weighted by its atomic number.
+inline bool is_weighted(int type) { return type == 12 || type == 13 || type >= 23; }
 
 struct SF {
     int type = 0;
@@ -234,6 +238,7 @@ struct SF {
 
     bool relevant(int e) const  // SymFnc*::checkRelevantElement
     {
+        if (is_weighted(type)) return true;
         return is_radial(type) ? e == e1 : (e == e1 || e == e2);
     }
     // SymFnc.cpp:169-195
@@ -257,6 +262,24 @@ bool sf_less(const SF& a, const SF& b)
 {
     if (a.ec != b.ec) return a.ec < b.ec;
     if (a.type != b.type) return a.type < b.type;
+    if (a.type >= 23) {
+        // SymFncCompRadWeighted.cpp, SymFncCompAng{n,w}Weighted.cpp operator<
+        if (a.subtype != b.subtype) return a.subtype < b.subtype;
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.rl != b.rl) return a.rl < b.rl;
+        if (a.type == 23) return false;
+        if (a.angleLeft != b.angleLeft) return a.angleLeft < b.angleLeft;
+        return a.angleRight < b.angleRight;
+    }
+    if (a.type == 12 || a.type == 13) {
+        // SymFncExpRadWeighted.cpp, SymFncExpAngnWeighted.cpp operator<
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.eta != b.eta) return a.eta < b.eta;
+        if (a.rs != b.rs) return a.rs < b.rs;
+        if (a.type == 12) return false;
+        if (a.zeta != b.zeta) return a.zeta < b.zeta;
+        return a.lambda < b.lambda;
+    }
     if (is_compact(a.type)) {
         // SymFncCompRad.cpp:48-68, SymFncCompAngn/SymFncCompAngw.cpp:56-82
         if (a.subtype != b.subtype) return a.subtype < b.subtype;
@@ -296,6 +319,7 @@ struct Group {
     double rmin = 0, rmax = 0;
     vector<double> mrl, mrc, mal, mar;  // member ranges (angles in radians)
     vector<char> calcComp;  // narrow: r_jk term recomputed (SymGrpCompAngn.cpp:108-126)
+    vector<double> Z;       // weighted types: atomic number by element index
 };
 
 // ------------------------------------------------------------ neural net --
@@ -545,6 +569,41 @@ void load_model(oracle_ctx& c, const string& dir)
             if (s.e1 > s.e2) std::swap(s.e1, s.e2);
             s.zetaInt = (int)std::round(s.zeta);
             s.useIntPow = fabs(s.zeta - s.zetaInt) <= std::numeric_limits<double>::min();
+        } else if (s.type == 12) {  // SymFncExpRadWeighted.cpp setParameters
+            s.eta = atof(a.at(2).c_str());
+            s.rs = atof(a.at(3).c_str());
+            s.rc = atof(a.at(4).c_str());
+        } else if (s.type == 13) {  // SymFncExpAngnWeighted.cpp setParameters
+            s.eta = atof(a.at(2).c_str());
+            s.rs = atof(a.at(3).c_str());
+            s.lambda = atof(a.at(4).c_str());
+            s.zeta = atof(a.at(5).c_str());
+            s.rc = atof(a.at(6).c_str());
+            s.zetaInt = (int)std::round(s.zeta);
+            s.useIntPow = fabs(s.zeta - s.zetaInt) <= std::numeric_limits<double>::min();
+        } else if (s.type == 23) {  // SymFncCompRadWeighted.cpp setParameters
+            s.rl = atof(a.at(2).c_str());
+            s.rc = atof(a.at(3).c_str());
+            s.subtype = a.at(4);
+            if (s.rl > s.rc) throw std::runtime_error("lower radial boundary >= upper radial boundary");
+            set_compact_subtype(s);
+        } else if (s.type == 24 || s.type == 25) {  // SymFncBaseCompAngWeighted.cpp setParameters
+            s.rl = atof(a.at(2).c_str());
+            s.rc = atof(a.at(3).c_str());
+            s.angleLeft = atof(a.at(4).c_str());
+            s.angleRight = atof(a.at(5).c_str());
+            s.subtype = a.at(6);
+            if (s.rl > s.rc) throw std::runtime_error("lower radial boundary >= upper radial boundary");
+            set_compact_subtype(s);
+            if (s.angleLeft >= s.angleRight)
+                throw std::runtime_error("left angle boundary right of or equal to right angle boundary");
+            double const center = (s.angleLeft + s.angleRight) / 2.0;
+            if ((s.angleLeft < 0.0 && center != 0.0) || (s.angleRight > 180.0 && center != 180.0))
+                throw std::runtime_error("angle boundary out of [0,180] and center of angular function /= 0 or /= 180");
+            if (s.angleRight - s.angleLeft > 360.0)
+                throw std::runtime_error("periodic symmetry function cannot spread over domain > 360 degrees");
+            s.ca.core = s.cr.core;
+            s.ca.set_left_right(s.angleLeft * M_PI / 180.0, s.angleRight * M_PI / 180.0);
         } else if (s.type == 20) {  // SymFncCompRad.cpp:85-112
             s.e1 = eidx(a.at(2));
             s.rl = atof(a.at(3).c_str());
@@ -639,12 +698,16 @@ void load_model(oracle_ctx& c, const string& dir)
                 g.mal.push_back(s.angleLeft * M_PI / 180.0);
                 g.mar.push_back(s.angleRight * M_PI / 180.0);
                 const SF* prev = i ? &e.sfs[g.mem[i - 1]] : nullptr;
-                g.calcComp.push_back(g.type != 21 || !prev || prev->rc != s.rc || prev->rl != s.rl);
+                g.calcComp.push_back((g.type != 21 && g.type != 24) || !prev || prev->rc != s.rc ||
+                                     prev->rl != s.rl);
             }
         }
         // Members are in SF order already (SF sort refines the member sort).
         for (auto& g : e.grps) {
-            if (g.type != 3 && g.type != 9) continue;  // SymGrpExpAngn/SymGrpExpAngw.cpp:108-148 (same member set-up)
+            for (auto& el : c.el) g.Z.push_back((double)el.Z);
+            // SymGrpExpAngn/SymGrpExpAngw/SymGrpExpAngnWeighted::sortMembers
+            // (SymGrpExpAngn.cpp:108-148): the same member set-up
+            if (g.type != 3 && g.type != 9 && g.type != 13) continue;
             for (size_t i = 0; i < g.mem.size(); ++i) {
                 const SF& s = e.sfs[g.mem[i]];
                 g.factorNorm.push_back(pow(2.0, 1.0 - s.zeta));
@@ -719,7 +782,7 @@ void load_model(oracle_ctx& c, const string& dir)
             g.scalingFactors.clear();
             for (size_t i = 0; i < g.mem.size(); ++i) {
                 g.scalingFactors.push_back(e.sfs[g.mem[i]].scalingFactor);
-                if (g.type == 3 || g.type == 9) g.factorNorm[i] *= g.scalingFactors[i];
+                if (g.type == 3 || g.type == 9 || g.type == 13) g.factorNorm[i] *= g.scalingFactors[i];
             }
         }
     }
@@ -990,6 +1053,120 @@ void calc_radial(const Element& e, const Group& g, Atom& a)
     }
 }
 
+// SymGrpExpRadWeighted::calculate (SymGrpExpRadWeighted.cpp:137-231): every
+// neighbour inside rc, each term weighted by its atomic number.
+void calc_radial_weighted(const Element& e, const Group& g, Atom& a)
+{
+    const size_t M = g.mem.size();
+    vector<double> result(M, 0.0);
+    for (auto& n : a.nbr) {
+        if (!(n.d < g.rc)) continue;
+        double const rij = n.d;
+        int const nej = n.element;
+        double pfc, pdfc;
+        g.fc.fdf(rij, pfc, pdfc);
+        for (size_t k = 0; k < M; ++k) {
+            const SF& s = e.sfs[g.mem[k]];
+            double const pexp = g.Z[nej] * exp(-s.eta * (rij - s.rs) * (rij - s.rs));
+            result[k] += pexp * pfc;
+            double const p1 = g.scalingFactors[k] * (pdfc - 2.0 * s.eta * (rij - s.rs) * pfc) * pexp / rij;
+            double const px = p1 * n.dr[0], py = p1 * n.dr[1], pz = p1 * n.dr[2];
+            double* d = &a.dGdr[3 * s.index];
+            d[0] += px; d[1] += py; d[2] += pz;
+            d = &n.dGdr[3 * s.indexPerElement[nej]];
+            d[0] -= px; d[1] -= py; d[2] -= pz;
+        }
+    }
+    for (size_t k = 0; k < M; ++k) {
+        const SF& s = e.sfs[g.mem[k]];
+        a.G[s.index] = s.scale(result[k]);
+    }
+}
+
+double pow_int(double x, int n);
+
+// SymGrpExpAngnWeighted::calculate (SymGrpExpAngnWeighted.cpp:165-395): all
+// neighbour pairs j < k inside rc with r_jk < rc, the cutoff product
+// weighted by Z_j Z_k, the exponent always in shifted form.
+void calc_angular_weighted(const Element& e, const Group& g, Atom& a)
+{
+    const size_t M = g.mem.size();
+    vector<double> result(M, 0.0);
+    double const rc = g.rc, rc2 = rc * rc;
+    size_t nn = a.nbr.size();
+    if (nn == 0) nn = 1;
+    for (size_t j = 0; j + 1 < nn; ++j) {
+        Neighbor& nj = a.nbr[j];
+        int const nej = nj.element;
+        double const rij = nj.d;
+        if (!(rij < rc)) continue;
+        double const r2ij = rij * rij;
+        double pfcij, pdfcij;
+        g.fc.fdf(rij, pfcij, pdfcij);
+        const double* dr1 = nj.dr;
+        for (size_t k = j + 1; k < nn; ++k) {
+            Neighbor& nk = a.nbr[k];
+            int const nek = nk.element;
+            double const rik = nk.d;
+            if (!(rik < rc)) continue;
+            const double* dr2 = nk.dr;
+            double dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1], dr2[2] - dr1[2]};
+            double rjk = dr3[0] * dr3[0] + dr3[1] * dr3[1] + dr3[2] * dr3[2];
+            if (!(rjk < rc2)) continue;
+            double pfcik, pdfcik;
+            g.fc.fdf(rik, pfcik, pdfcik);
+            rjk = sqrt(rjk);
+            double pfcjk, pdfcjk;
+            g.fc.fdf(rjk, pfcjk, pdfcjk);
+            double const rinvijik = 1.0 / rij / rik;
+            double const costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
+            double const z = g.Z[nej] * g.Z[nek];
+            double const pfc = z * pfcij * pfcik * pfcjk;
+            double const r2ik = rik * rik;
+            double const pr1 = z * pfcik * pfcjk * pdfcij / rij;
+            double const pr2 = z * pfcij * pfcjk * pdfcik / rik;
+            double const pr3 = z * pfcij * pfcik * pdfcjk / rjk;
+            double vexp = 0.0, rijs = 0.0, riks = 0.0, rjks = 0.0;
+            for (size_t l = 0; l < M; ++l) {
+                const SF& s = e.sfs[g.mem[l]];
+                if (g.calcExp[l]) {
+                    rijs = rij - s.rs;
+                    riks = rik - s.rs;
+                    rjks = rjk - s.rs;
+                    double const r2sum = rijs * rijs + riks * riks + rjks * rjks;
+                    vexp = exp(-s.eta * r2sum);
+                }
+                double const plambda = 1.0 + s.lambda * costijk;
+                double fg = vexp;
+                if (plambda <= 0.0) fg = 0.0;
+                else if (s.useIntPow) fg *= pow_int(plambda, s.zetaInt - 1);
+                else fg *= pow(plambda, s.zeta - 1.0);
+                result[l] += fg * plambda * pfc;
+                fg *= g.factorNorm[l];
+                double const pfczl = pfc * g.zetaLambda[l];
+                double const p2etapl = plambda * g.factorDeriv[l];
+                double const p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl * rijs / rij) + pr1 * plambda);
+                double const p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl * riks / rik) + pr2 * plambda);
+                double const p3 = fg * (pfczl * (rinvijik + p2etapl * rjks / rjk) - pr3 * plambda);
+                double const p1x = p1 * dr1[0], p1y = p1 * dr1[1], p1z = p1 * dr1[2];
+                double const p2x = p2 * dr2[0], p2y = p2 * dr2[1], p2z = p2 * dr2[2];
+                double const p3x = p3 * dr3[0], p3y = p3 * dr3[1], p3z = p3 * dr3[2];
+                double* d = &a.dGdr[3 * s.index];
+                d[0] += p1x + p2x; d[1] += p1y + p2y; d[2] += p1z + p2z;
+                d = &nj.dGdr[3 * s.indexPerElement[nej]];
+                d[0] -= p1x + p3x; d[1] -= p1y + p3y; d[2] -= p1z + p3z;
+                d = &nk.dGdr[3 * s.indexPerElement[nek]];
+                d[0] -= p2x - p3x; d[1] -= p2y - p3y; d[2] -= p2z - p3z;
+            }
+        }
+    }
+    for (size_t l = 0; l < M; ++l) {
+        const SF& s = e.sfs[g.mem[l]];
+        result[l] *= g.factorNorm[l] / g.scalingFactors[l];
+        a.G[s.index] = s.scale(result[l]);
+    }
+}
+
 // pow_int (utility.cpp:292-316): exponentiation by squaring.
 double pow_int(double x, int n)
 {
@@ -1181,25 +1358,31 @@ void calc_angular_wide(const Element& e, const Group& g, Atom& a)
 }
 
 // SymGrpCompRad::calculate (SymGrpCompRad.cpp:129-199).
-void calc_comp_radial(const Element& e, const Group& g, Atom& a)
+// With `weighted` (type 23), SymGrpCompRadWeighted::calculate
+// (SymGrpCompRadWeighted.cpp:120-200): every neighbour element, each term
+// weighted by the neighbour's atomic number.
+void calc_comp_radial(const Element& e, const Group& g, Atom& a, bool weighted)
 {
     const size_t M = g.mem.size();
     vector<double> result(M, 0.0);
     for (auto& n : a.nbr) {
         double const rij = n.d;
-        if (!(g.e1 == n.element && rij < g.rmax && rij > g.rmin)) continue;
+        if (!((weighted || g.e1 == n.element) && rij < g.rmax && rij > g.rmin)) continue;
+        double const Z = g.Z[n.element];
         for (size_t k = 0; k < M; ++k) {
             if (rij <= g.mrl[k] || rij >= g.mrc[k]) continue;
             const SF& s = e.sfs[g.mem[k]];
             double rad, drad;
             s.cr.fdf(rij, rad, drad);
-            result[k] += rad;
+            if (weighted) result[k] += Z * rad;
+            else result[k] += rad;
             if (drad == 0.0) continue;
-            drad *= g.scalingFactors[k] / rij;
+            if (weighted) drad *= Z * g.scalingFactors[k] / rij;
+            else drad *= g.scalingFactors[k] / rij;
             double const px = drad * n.dr[0], py = drad * n.dr[1], pz = drad * n.dr[2];
             double* d = &a.dGdr[3 * s.index];
             d[0] += px; d[1] += py; d[2] += pz;
-            d = &n.dGdr[3 * s.indexPerElement[g.e1]];
+            d = &n.dGdr[3 * s.indexPerElement[n.element]];
             d[0] -= px; d[1] -= py; d[2] -= pz;
         }
     }
@@ -1216,7 +1399,11 @@ void calc_comp_radial(const Element& e, const Group& g, Atom& a)
 // the r_jk factor is only re-evaluated when a member's (rl, rc) differ from
 // the previous member's or the previous member was skipped for this
 // triplet; otherwise the previous member's value is reused.
-void calc_comp_angular(const Element& e, const Group& g, Atom& a, bool narrow)
+// With `weighted` (types 24, 25), SymGrpCompAngnWeighted::calculate and
+// SymGrpCompAngwWeighted::calculate (SymGrpCompAngnWeighted.cpp:131-390,
+// SymGrpCompAngwWeighted.cpp:116-345): all neighbour pairs, the angular
+// factor weighted by Z_j Z_k.
+void calc_comp_angular(const Element& e, const Group& g, Atom& a, bool narrow, bool weighted = false)
 {
     const size_t M = g.mem.size();
     vector<double> result(M, 0.0), radij(M, 0.0), dradij(M, 0.0);
@@ -1228,7 +1415,7 @@ void calc_comp_angular(const Element& e, const Group& g, Atom& a, bool narrow)
         Neighbor& nj = a.nbr[j];
         int const nej = nj.element;
         double const rij = nj.d;
-        if (!((g.e1 == nej || g.e2 == nej) && rij < rmax && rij > rmin)) continue;
+        if (!((weighted || g.e1 == nej || g.e2 == nej) && rij < rmax && rij > rmin)) continue;
         for (size_t l = 0; l < M; ++l) {
             if (rij > g.mrl[l] && rij < g.mrc[l]) e.sfs[g.mem[l]].cr.fdf(rij, radij[l], dradij[l]);
             else radij[l] = 0.0;
@@ -1237,7 +1424,7 @@ void calc_comp_angular(const Element& e, const Group& g, Atom& a, bool narrow)
         for (size_t k = j + 1; k < nn; ++k) {
             Neighbor& nk = a.nbr[k];
             int const nek = nk.element;
-            if (!((g.e1 == nej && g.e2 == nek) || (g.e2 == nej && g.e1 == nek))) continue;
+            if (!(weighted || (g.e1 == nej && g.e2 == nek) || (g.e2 == nej && g.e1 == nek))) continue;
             double const rik = nk.d;
             if (!(rik < rmax && rik > rmin)) continue;
             const double* dr2 = nk.dr;
@@ -1276,6 +1463,8 @@ void calc_comp_angular(const Element& e, const Group& g, Atom& a, bool narrow)
                     skip = false;
                 }
                 s.ca.fdf(acostijk, ang, dang);
+                double const z = g.Z[nej] * g.Z[nek];
+                if (weighted) ang *= z;
                 double rad = narrow ? radij[l] * radik * radjk : radij[l] * radik;
                 result[l] += rad * ang;
                 dang *= dacostijk;
@@ -1288,13 +1477,16 @@ void calc_comp_angular(const Element& e, const Group& g, Atom& a, bool narrow)
                     chiik = rinvik * radij[l] * dradik * radjk;
                     chijk = -rinvjk * radij[l] * radik * dradjk;
                 } else {
-                    chiij = rinvij * radik * dradij[l];
+                    // the weighted group multiplies in the other order
+                    // (SymGrpCompAngwWeighted.cpp:257)
+                    chiij = weighted ? rinvij * dradij[l] * radik : rinvij * radik * dradij[l];
                     chiik = rinvik * radij[l] * dradik;
                 }
                 double p1, p2, p3;
                 double const sfac = g.scalingFactors[l];
                 if (dang != 0.0 && rad != 0.0) {
-                    rad *= sfac;
+                    if (weighted) rad *= sfac * z;
+                    else rad *= sfac;
                     ang *= sfac;
                     p1 = rad * phiijik + ang * chiij;
                     p2 = rad * phiikij + ang * chiik;
@@ -1342,8 +1534,11 @@ void calc_sf(oracle_ctx& c)
         for (auto& g : e.grps) {
             if (g.type == 2) calc_radial(e, g, a);
             else if (g.type == 9) calc_angular_wide(e, g, a);
-            else if (g.type == 20) calc_comp_radial(e, g, a);
+            else if (g.type == 12) calc_radial_weighted(e, g, a);
+            else if (g.type == 13) calc_angular_weighted(e, g, a);
+            else if (g.type == 20 || g.type == 23) calc_comp_radial(e, g, a, g.type == 23);
             else if (g.type == 21 || g.type == 22) calc_comp_angular(e, g, a, g.type == 21);
+            else if (g.type == 24 || g.type == 25) calc_comp_angular(e, g, a, g.type == 24, true);
             else calc_angular(e, g, a);
         }
     }

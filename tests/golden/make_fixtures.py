@@ -19,7 +19,8 @@ What it writes (data only -- numbers and the example's input files):
   the compact-function models (SF types 20 and 22, polynomial and asymmetric
   polynomial subtypes, normalised units; ``ExamplePrediction.h:1266-1525,
   1526-1550, 1551-1846``).
-* ``sf_kat.json`` -- the type-2/3/9/20/21/22 symmetry-function known answers of
+* ``sf_kat.json`` -- the symmetry-function known answers (types 2, 3, 9, 12,
+  13, 20-25) of
   ``test/cpp/fixtures.h:151-258`` with the cluster geometries of
   ``fixtures.h:11-149`` (values for atom 0, CT_TANHU, no scaling, 1000*eps).
 * ``cutoff_kat.json`` -- f and df of all cutoff types at rc=3, alpha=0.2,
@@ -81,17 +82,18 @@ def main(ref):
            "cutoff_type": 2, "cutoff_alpha": 0.0, "neighbor_cutoff": 10.0,
            "tolerance": "1000*DBL_EPSILON absolute", "sets": {}}
     geoms = {
-        # fixtures.h:11-51 / 95-149 (positions, element indices in ElementMap order)
+        # fixtures.h:11-149 (positions, element indices in ElementMap order)
         "Mono": {"elements": ["H"], "pos": [[0, 0, 0], [1, 2, 1], [0, 3, -1]], "elem": [0, 0, 0]},
+        "Dual": {"elements": ["H", "He"], "pos": [[0, 0, 0], [1, 2, 1], [0, 3, -1]], "elem": [0, 1, 1]},
         "Mixed": {"elements": ["S", "Cu", "U"],
                   "pos": [[0, 0, 0], [1, 2, 1], [0, 3, -1], [-2, 1, -0.5]], "elem": [0, 0, 1, 2]},
     }
-    for name in ("Mono", "Mixed"):
+    for name in ("Mono", "Dual", "Mixed"):
         types = [int(x) for x in re.findall(r"\d+", block(fx, "types" + name))]
         lines = re.findall(r'"([^"]*)"', block(fx, "setupLines" + name))
         vals = floats_in(block(fx, "values" + name))
         assert len(types) == len(lines) == len(vals)
-        rows = [{"type": t, "line": l, "value": v} for t, l, v in zip(types, lines, vals) if t in (2, 3, 9, 20, 21, 22)]
+        rows = [{"type": t, "line": l, "value": v} for t, l, v in zip(types, lines, vals) if t in (2, 3, 9, 12, 13, 20, 21, 22, 23, 24, 25)]
         kat["sets"][name] = dict(geoms[name], functions=rows)
     json.dump(kat, open(os.path.join(HERE, "sf_kat.json"), "w"), indent=1)
 

@@ -220,17 +220,31 @@ class RefLib:
         return [self.lib().ref_phase_time(self.h, p) for p in range(5)]
 
 
-def compact_model(directory):
-    """The compact-function water model (SF types 20/21/22) with scaling
-    statistics measured by the oracle on the seeded 192-atom box."""
+def _measured_model(directory, make):
+    """make(dir, rows=None) writes an unscaled model; the scaling statistics
+    are measured by the oracle on the seeded 192-atom water box and the
+    scaled model written to `directory`."""
     from n2p2_amd import synth
     if os.path.exists(os.path.join(directory, "weights.008.data")):
         return directory
-    raw = synth.compact_water_model(directory + "_raw")
+    raw = make(directory + "_raw")
     pos, elem, box = synth.water_box(4)
     o = Oracle(raw)
     o.set_structure(pos, elem, box)
     o.predict(PULL, BRUTE)
     G = o.G()
     mats = [G[elem == e][:, :o.nsf[e]] for e in range(o.ne)]
-    return synth.compact_water_model(directory, rows=synth.rows_from_values(mats))
+    return make(directory, rows=synth.rows_from_values(mats))
+
+
+def compact_model(directory):
+    """The compact-function water model (SF types 20/21/22)."""
+    from n2p2_amd import synth
+    return _measured_model(directory, synth.compact_water_model)
+
+
+def weighted_model(directory):
+    """The water model mixing types 2/3 with the weighted types 12, 13, 23,
+    24, 25."""
+    from n2p2_amd import synth
+    return _measured_model(directory, synth.weighted_water_model)

@@ -8,7 +8,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, RefLib, compact_model, cutoff_fdf, ref_available
+from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, RefLib, compact_model, cutoff_fdf, ref_available, weighted_model
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -81,9 +81,9 @@ def _z(sym):
     return _ZTAB[sym]
 
 
-@pytest.mark.parametrize("name", ["Mono", "Mixed"])
+@pytest.mark.parametrize("name", ["Mono", "Dual", "Mixed"])
 def test_sf_known_answers(tmp_path, name):
-    """test/cpp/fixtures.h:151-258 values of atom 0 for types 2, 3, 9, 20, 21, 22 at 1000*eps
+    """test/cpp/fixtures.h:151-258 values of atom 0 for types 2, 3, 9, 12, 13, 20-25 at 1000*eps
     (test/cpp/test_SymmetryFunctions.cpp:36, 162-190)."""
     from n2p2_amd.synth import sorted_sfs
     k = json.load(open(os.path.join(GOLD, "sf_kat.json")))["sets"][name]
@@ -188,7 +188,8 @@ def _water_dir():
 
 
 @pytest.mark.skipif(not ref_available(), reason="reference compute classes not built (no reference tree)")
-@pytest.mark.parametrize("case", ["golden", "water192", "small_cell", "fcc108", "cu2s", "dmabn", "compact"])
+@pytest.mark.parametrize("case", ["golden", "water192", "small_cell", "fcc108", "cu2s", "dmabn", "compact",
+                                  "weighted"])
 def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path):
     """The oracle reproduces n2p2's own SymGrp/NeuralNetwork/Atom code bit for bit."""
     from n2p2_amd import read_input_data, synth
@@ -208,6 +209,10 @@ def test_oracle_bitwise_equals_reference_classes(case, water_model_dir, tmp_path
         # every compact subtype, and narrow members of different subtypes
         # sharing (rl, rc), whose r_jk factor the reference reuses
         d = compact_model(str(tmp_path / "compact"))
+        pos, elem, box = synth.water_box(4)
+    elif case == "weighted":
+        # element-weighted types 12/13/23/24/25 interleaved with types 2/3
+        d = weighted_model(str(tmp_path / "weighted"))
         pos, elem, box = synth.water_box(4)
     elif case == "fcc108":
         d = synth.fcc_model(str(tmp_path / "fcc"))
