@@ -97,7 +97,7 @@ __global__ __launch_bounds__(64) void k_work(const ModelT* __restrict__ M, SysVi
         if (sd[k] < M->maxCutoff) ++P;
         for (int g = 0; g < E.nrad; ++g) {
             const GroupT& G = M->grp[E.grp0 + g];
-            if (se[k] == G.e1 && sd[k] < G.rc) Pm += G.nmem;
+            if ((G.e1 < 0 || se[k] == G.e1) && sd[k] < G.rc) Pm += G.nmem;
         }
     }
     for (int g = E.nrad; g < E.ngrp; ++g) {
@@ -105,12 +105,13 @@ __global__ __launch_bounds__(64) void k_work(const ModelT* __restrict__ M, SysVi
         int nexp = 0;
         for (int q = 0; q < G.nmem; ++q) nexp += M->mem[G.mem0 + q].calcExp;
         for (int j = lane; j < n; j += kWave) {
-            if (!((se[j] == G.e1 || se[j] == G.e2) && sd[j] < G.rc)) continue;
+            const bool any = G.e1 < 0;  // element-weighted group
+            if (!((any || se[j] == G.e1 || se[j] == G.e2) && sd[j] < G.rc)) continue;
             for (int k = j + 1; k < n; ++k) {
-                if (!((se[j] == G.e1 && se[k] == G.e2) || (se[j] == G.e2 && se[k] == G.e1))) continue;
+                if (!(any || (se[j] == G.e1 && se[k] == G.e2) || (se[j] == G.e2 && se[k] == G.e1))) continue;
                 if (!(sd[k] < G.rc)) continue;
                 const double a = sx[0][k] - sx[0][j], b = sx[1][k] - sx[1][j], c = sx[2][k] - sx[2][j];
-                if (G.type == 9 || G.type == 22 || a * a + b * b + c * c < G.rc2) {
+                if (G.type == 9 || G.type == 22 || G.type == 25 || a * a + b * b + c * c < G.rc2) {
                     ++T;
                     Tm += G.nmem;
                     Te += nexp;

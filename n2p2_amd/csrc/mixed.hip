@@ -1357,9 +1357,9 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed)
 {
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
-    if (t.has_comp)
-        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact symmetry functions "
-                                          "(types 20-22); use the plain FP64 form"};
+    if (t.has_comp || t.has_weighted)
+        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact or element-weighted "
+                                          "symmetry functions (types 12, 13, 20-25); use the plain FP64 form"};
     if (t.max_width > kMaxWidth)
         throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
     if (s.natoms == 0) return;
@@ -1426,9 +1426,9 @@ void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_mod
                       const double* u, double* Gdot, int ldG, hipStream_t stream)
 {
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
-    if (t.has_comp)
-        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact symmetry functions "
-                                          "(types 20-22); use the plain FP64 form"};
+    if (t.has_comp || t.has_weighted)
+        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact or element-weighted "
+                                          "symmetry functions (types 12, 13, 20-25); use the plain FP64 form"};
     if (t.max_width > kMaxWidth)
         throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
     if (s.natoms == 0) return;

@@ -122,6 +122,22 @@ struct SF {
 bool sf_before(const SF& a, const SF& b)
 {
     if (a.type != b.type) return a.type < b.type;
+    if (a.type >= 23) {  // SymFncCompRadWeighted / SymFncCompAng{n,w}Weighted operator<
+        if (a.subtype != b.subtype) return a.subtype < b.subtype;
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.rl != b.rl) return a.rl < b.rl;
+        if (a.type == 23) return false;
+        if (a.angleLeft != b.angleLeft) return a.angleLeft < b.angleLeft;
+        return a.angleRight < b.angleRight;
+    }
+    if (a.type == 12 || a.type == 13) {  // SymFncExpRadWeighted / SymFncExpAngnWeighted operator<
+        if (a.rc != b.rc) return a.rc < b.rc;
+        if (a.eta != b.eta) return a.eta < b.eta;
+        if (a.rs != b.rs) return a.rs < b.rs;
+        if (a.type == 12) return false;
+        if (a.zeta != b.zeta) return a.zeta < b.zeta;
+        return a.lambda < b.lambda;
+    }
     if (sf_compact(a.type)) {  // SymFncCompRad.cpp:48-68, SymFncCompAngn/SymFncCompAngw.cpp:56-82
         if (a.subtype != b.subtype) return a.subtype < b.subtype;
         if (a.e1 != b.e1) return a.e1 < b.e1;
@@ -198,6 +214,7 @@ void load_model(const std::string& dir, HostModel& hm)
     hm.symbols = syms;
     hm.Z.clear();
     for (auto& x : syms) hm.Z.push_back(atomic_number(x));
+    for (size_t e = 0; e < syms.size() && e < (size_t)kMaxElem; ++e) t.Zel[e] = (double)hm.Z[e];
     auto eidx = [&](const std::string& x) {
         for (int i = 0; i < ne; ++i) if (syms[i] == x) return i;
         fail(NNPX_ERR_MODEL, "element \"" + x + "\" not in element map");
@@ -243,6 +260,43 @@ void load_model(const std::string& dir, HostModel& hm)
             if (f.e1 > f.e2) std::swap(f.e1, f.e2);
             f.zetaInt = (int)std::round(f.zeta);
             f.useIntPow = std::fabs(f.zeta - f.zetaInt) <= std::numeric_limits<double>::min();
+        } else if (f.type == 12) {  // SymFncExpRadWeighted::setParameters
+            f.e1 = f.e2 = -1;
+            f.eta = num(w.at(2));
+            f.rs = num(w.at(3));
+            f.rc = num(w.at(4));
+        } else if (f.type == 13) {  // SymFncExpAngnWeighted::setParameters
+            f.e1 = f.e2 = -1;
+            f.eta = num(w.at(2));
+            f.rs = num(w.at(3));
+            f.lambda = num(w.at(4));
+            f.zeta = num(w.at(5));
+            f.rc = num(w.at(6));
+            f.zetaInt = (int)std::round(f.zeta);
+            f.useIntPow = std::fabs(f.zeta - f.zetaInt) <= std::numeric_limits<double>::min();
+        } else if (f.type == 23) {  // SymFncCompRadWeighted::setParameters
+            f.e1 = f.e2 = -1;
+            f.rl = num(w.at(2));
+            f.rc = num(w.at(3));
+            f.subtype = w.at(4);
+            if (f.rl >= f.rc) fail(NNPX_ERR_MODEL, "lower radial boundary >= upper radial boundary");
+            compact_subtype(f);
+        } else if (f.type == 24 || f.type == 25) {  // SymFncBaseCompAngWeighted::setParameters
+            f.e1 = f.e2 = -1;
+            f.rl = num(w.at(2));
+            f.rc = num(w.at(3));
+            f.angleLeft = num(w.at(4));
+            f.angleRight = num(w.at(5));
+            f.subtype = w.at(6);
+            if (f.rl >= f.rc) fail(NNPX_ERR_MODEL, "lower radial boundary >= upper radial boundary");
+            compact_subtype(f);
+            if (f.angleLeft >= f.angleRight)
+                fail(NNPX_ERR_MODEL, "left angle boundary right of or equal to right angle boundary");
+            const double center = (f.angleLeft + f.angleRight) / 2.0;
+            if ((f.angleLeft < 0.0 && center != 0.0) || (f.angleRight > 180.0 && center != 180.0))
+                fail(NNPX_ERR_MODEL, "angle boundary out of [0,180] and center of angular function /= 0 or /= 180");
+            if (f.angleRight - f.angleLeft > 360.0)
+                fail(NNPX_ERR_MODEL, "periodic symmetry function cannot spread over domain > 360 degrees");
         } else if (f.type == 20) {  // SymFncCompRad.cpp:85-112
             f.e1 = f.e2 = eidx(w.at(2));
             f.rl = num(w.at(3));
@@ -269,8 +323,7 @@ void load_model(const std::string& dir, HostModel& hm)
             if (f.angleRight - f.angleLeft > 360.0)
                 fail(NNPX_ERR_MODEL, "periodic symmetry function cannot spread over domain > 360 degrees");
         } else {
-            fail(NNPX_ERR_UNSUPPORTED, "symmetry function type " + w.at(1) +
-                                           " not supported (types 2, 3, 9, 20, 21 and 22 are)");
+            fail(NNPX_ERR_UNSUPPORTED, "symmetry function type " + w.at(1) + " not supported");
         }
         if (hm.normalize) {  // SymFnc*::changeLengthUnit
             f.eta /= hm.convLength * hm.convLength;
@@ -302,7 +355,7 @@ void load_model(const std::string& dir, HostModel& hm)
             E.tab0[n] = tab + n * kMaxSF;
             E.nsfd[n] = 0;
             for (auto& f : v) {
-                bool rel = sf_radial(f.type) ? f.e1 == n : (f.e1 == n || f.e2 == n);
+                bool rel = sf_weighted(f.type) || (sf_radial(f.type) ? f.e1 == n : (f.e1 == n || f.e2 == n));
                 f.slot[n] = -1;
                 if (rel) {
                     f.slot[n] = E.nsfd[n];
@@ -315,7 +368,7 @@ void load_model(const std::string& dir, HostModel& hm)
         // radial groups first, then narrow and wide angular; members in SF
         // order.  Compact groups do not key on rc.
         std::vector<std::vector<int>> groups;
-        for (int pass : {2, 20, 3, 9, 21, 22}) {
+        for (int pass : {2, 12, 20, 23, 3, 9, 13, 21, 22, 24, 25}) {
             for (auto& f : v) {
                 if (f.type != pass) continue;
                 bool added = false;
@@ -380,8 +433,10 @@ void load_model(const std::string& dir, HostModel& hm)
                 const SF& f = v[g[q]];
                 MemberT& M = t.mem[mcount++];
                 M.sf = f.index;
-                M.slot1 = f.slot[f.e1];
-                M.slot2 = f.slot[f.e2];
+                M.slot1 = f.e1 >= 0 ? f.slot[f.e1] : -1;
+                M.slot2 = f.e2 >= 0 ? f.slot[f.e2] : -1;
+                for (int n = 0; n < kMaxElem; ++n) M.slotE[n] = n < ne ? f.slot[n] : -1;
+                if (sf_weighted(f.type)) t.has_weighted = 1;
                 const SF* prev = q ? &v[g[q - 1]] : nullptr;
                 M.calcExp = (!prev || prev->eta != f.eta || prev->rs != f.rs) ? 1 : 0;
                 M.useIntPow = f.useIntPow ? 1 : 0;
@@ -390,7 +445,7 @@ void load_model(const std::string& dir, HostModel& hm)
                 M.rs = f.rs;
                 M.lambda = f.lambda;
                 M.zeta = f.zeta;
-                if (f.type == 3 || f.type == 9) {  // SymGrpExpAngn/SymGrpExpAngw::sortMembers (SymGrpExpAngn.cpp:108-148)
+                if (f.type == 3 || f.type == 9 || f.type == 13) {  // SymGrpExpAngn/-Angw/-AngnWeighted::sortMembers (SymGrpExpAngn.cpp:108-148)
                     M.factorNorm = std::pow(2.0, 1.0 - f.zeta);
                     M.factorDeriv = 2.0 * f.eta / f.zeta / f.lambda;
                     M.zetaLambda = f.zeta * f.lambda;
@@ -415,7 +470,8 @@ void load_model(const std::string& dir, HostModel& hm)
                     // (rl, rc) change or the previous member was skipped
                     // (SymGrpCompAngn.cpp:108-126, 288-296).  `walk` marks
                     // members whose reused factor may be another subtype's.
-                    M.calcComp = (f.type != 21 || !prev || prev->rc != f.rc || prev->rl != f.rl) ? 1 : 0;
+                    M.calcComp =
+                        ((f.type != 21 && f.type != 24) || !prev || prev->rc != f.rc || prev->rl != f.rl) ? 1 : 0;
                     M.walk = 0;
                     for (int r = (int)q - 1; !M.calcComp && r >= 0; --r) {
                         const SF& p = v[g[r]];
@@ -529,7 +585,7 @@ void load_model(const std::string& dir, HostModel& hm)
             for (int q = G.mem0; q < G.mem0 + G.nmem; ++q) {
                 MemberT& M = t.mem[q];
                 M.sfac = t.scale[E.sf0 + M.sf].factor;
-                if (G.type == 3 || G.type == 9) {
+                if (G.type == 3 || G.type == 9 || G.type == 13) {
                     M.factorNorm *= M.sfac;
                     M.gnorm = M.factorNorm / M.sfac;
                 }

@@ -28,7 +28,10 @@ enum Activation {
 // SF types: Gaussian radial (2), narrow (3) and wide (9) angular with a
 // cutoff function; compact radial (20), narrow (21) and wide (22) angular.
 __host__ __device__ inline bool sf_compact(int type) { return type >= 20; }
-__host__ __device__ inline bool sf_radial(int type) { return type == 2 || type == 20; }
+__host__ __device__ inline bool sf_radial(int type) { return type == 2 || type == 12 || type == 20 || type == 23; }
+// Element-weighted types (12, 13, 23, 24, 25): every neighbour element
+// contributes, weighted by its atomic number; e1 = e2 = -1 in their groups.
+__host__ __device__ inline bool sf_weighted(int type) { return type == 12 || type == 13 || type >= 23; }
 
 // One SF group: members share (central element, type, rc, e1[, e2]); compact
 // groups share (central element, type, e1[, e2]) and span (rmin, rc = rmax).
@@ -48,6 +51,7 @@ struct MemberT {
     int sf;          // SF index of the central element (row of G)
     int slot1;       // derivative slot for neighbour element e1
     int slot2;       // derivative slot for neighbour element e2 (angular)
+    int slotE[kMaxElem];  // derivative slot per neighbour element (weighted types)
     int calcExp;     // recompute exp (eta or rs differ from previous member)
     int useIntPow;
     int zetaInt;
@@ -109,7 +113,9 @@ struct ModelT {
     int max_mem_rad;
     int max_mem_comp;       // max members of a compact angular group
     int has_exp;            // any Gaussian (type 2/3/9) group
-    int has_comp;           // any compact (type 20/21/22) group
+    int has_comp;           // any compact (type 20-25) group
+    int has_weighted;       // any element-weighted (type 12/13/23/24/25) group
+    double Zel[kMaxElem];   // atomic number by element index
     int max_width;          // max NN layer width (input layer included)
     int max_hidden;         // max width of the hidden and output layers
     int nweights;           // FP64 weights+biases in the weight buffer

@@ -44,8 +44,9 @@ __device__ inline double block_sum(double v, double* red, int slot)
     return v;
 }
 
-// Ordered compaction of neighbours into a list (single wave does it so the
-// list keeps neighbour order).
+// Ordered compaction of neighbours of element `elem` (any element if
+// negative) inside rc into a list (a single wave does it so the list keeps
+// neighbour order).
 __device__ inline int build_list(const Lds& L, int n, int elem, double rc, int* out)
 {
     __shared__ int cnt_sh;
@@ -54,7 +55,7 @@ __device__ inline int build_list(const Lds& L, int n, int elem, double rc, int* 
         int cnt = 0;
         for (int b = 0; b < n; b += kWave) {
             const int k = b + lane;
-            const bool ok = k < n && L.el[k] == elem && L.d[k] < rc;
+            const bool ok = k < n && (elem < 0 || L.el[k] == elem) && L.d[k] < rc;
             const int incl = wave_incl_scan(ok ? 1 : 0);
             if (ok) out[cnt + incl - 1] = k;
             cnt += __shfl(incl, kWave - 1, kWave);
@@ -112,10 +113,13 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
     // ---------------------------------------------------------- radial groups
     for (int gi = 0; gi < E.nrad; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
-        if (G.type != 2) continue;  // compact groups: k_sfcomp64
+        if (G.type != 2 && G.type != 12) continue;  // compact groups: k_sfcomp64
+        // weighted (SymGrpExpRadWeighted.cpp:137-231): every neighbour
+        // element, each term times its atomic number
+        const bool wt = G.type == 12;
         for (int k = tid; k < n; k += kSFThreads) {
             double f = 0.0, df = 0.0;
-            if (L.el[k] == G.e1 && L.d[k] < G.rc) cutoff_fdf(ctype, G, tanhPre, L.d[k], f, df);
+            if ((wt || L.el[k] == G.e1) && L.d[k] < G.rc) cutoff_fdf(ctype, G, tanhPre, L.d[k], f, df);
             L.fc[k] = f;
             L.dfc[k] = df;
         }
@@ -124,12 +128,13 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
             const MemberT& mb = M->mem[G.mem0 + q];
             double gs = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
             for (int k = tid; k < n; k += kSFThreads) {
-                if (L.el[k] != G.e1) continue;
+                if (!wt && L.el[k] != G.e1) continue;
                 double px = 0.0, py = 0.0, pz = 0.0;
                 const double rij = L.d[k];
                 if (rij < G.rc) {
                     const double pfc = L.fc[k], pdfc = L.dfc[k];
-                    const double pexp = exp(-mb.eta * (rij - mb.rs) * (rij - mb.rs));
+                    double pexp = exp(-mb.eta * (rij - mb.rs) * (rij - mb.rs));
+                    if (wt) pexp = M->Zel[L.el[k]] * pexp;
                     gs += pexp * pfc;
                     const double p1 = mb.sfac * (pdfc - 2.0 * mb.eta * (rij - mb.rs) * pfc) * pexp / rij;
                     px = p1 * L.dr[k];
@@ -138,7 +143,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                     sx += px; sy += py; sz += pz;
                 }
                 if (deriv) {
-                    double* d = dGn + ((size_t)k * o.S + mb.slot1) * 3;
+                    double* d = dGn + ((size_t)k * o.S + (wt ? mb.slotE[L.el[k]] : mb.slot1)) * 3;
                     d[0] = -px; d[1] = -py; d[2] = -pz;
                 }
             }
@@ -160,18 +165,21 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
     // --------------------------------------------------------- angular groups
     for (int gi = E.nrad; gi < E.ngrp; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
-        if (G.type != 3 && G.type != 9) continue;  // compact groups: k_sfcomp64
+        if (G.type != 3 && G.type != 9 && G.type != 13) continue;  // compact groups: k_sfcomp64
         const int nmem = G.nmem;
         const bool same = G.e1 == G.e2;
         // wide angular groups (SymGrpExpAngw.cpp:156-380): no r_jk cutoff and
         // no r_jk term in the exponent
         const bool wide = G.type == 9;
+        // weighted (SymGrpExpAngnWeighted.cpp:165-395): all neighbour pairs,
+        // cutoff product times Z_j Z_k, exponent always in shifted form
+        const bool wt = G.type == 13;
         const int nA = build_list(L, n, G.e1, G.rc, L.la);
         const int nB = same ? nA : build_list(L, n, G.e2, G.rc, L.lb);
         const int* lb = same ? L.la : L.lb;
         for (int k = tid; k < n; k += kSFThreads) {
             double f = 0.0, df = 0.0;
-            if ((L.el[k] == G.e1 || L.el[k] == G.e2) && L.d[k] < G.rc)
+            if ((wt || L.el[k] == G.e1 || L.el[k] == G.e2) && L.d[k] < G.rc)
                 cutoff_fdf(ctype, G, tanhPre, L.d[k], f, df);
             L.fc[k] = f;
             L.dfc[k] = df;
@@ -231,11 +239,19 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                 rjk = sqrt(rjk);
                 double pfcjk, pdfcjk;
                 cutoff_fdf(ctype, G, tanhPre, rjk, pfcjk, pdfcjk);
-                pfc = pfcij * pfcik * pfcjk;
                 r2sum = r2ij + r2ik + rjk * rjk;
-                pr1 = pfcik * pfcjk * pdfcij / rij;
-                pr2 = pfcij * pfcjk * pdfcik / rik;
-                pr3 = pfcij * pfcik * pdfcjk / rjk;
+                if (wt) {
+                    const double z = M->Zel[L.el[j]] * M->Zel[L.el[k]];
+                    pfc = z * pfcij * pfcik * pfcjk;
+                    pr1 = z * pfcik * pfcjk * pdfcij / rij;
+                    pr2 = z * pfcij * pfcjk * pdfcik / rik;
+                    pr3 = z * pfcij * pfcik * pdfcjk / rjk;
+                } else {
+                    pfc = pfcij * pfcik * pfcjk;
+                    pr1 = pfcik * pfcjk * pdfcij / rij;
+                    pr2 = pfcij * pfcjk * pdfcik / rik;
+                    pr3 = pfcij * pfcik * pdfcjk / rjk;
+                }
             }
             double vexp = 0.0, rijs = 0.0, riks = 0.0, rjks = 0.0;
 #pragma unroll
@@ -243,7 +259,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                 if (q < nmem) {
                 const MemberT& mb = M->mem[G.mem0 + q];
                 if (mb.calcExp) {
-                    if (mb.rs > 0.0) {
+                    if (mb.rs > 0.0 || wt) {
                         rijs = rij - mb.rs;
                         riks = rik - mb.rs;
                         if (wide) {
@@ -267,7 +283,7 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
                 const double pfczl = pfc * mb.zetaLambda;
                 const double p2etapl = plambda * mb.factorDeriv;
                 double p1, p2, p3;
-                if (mb.rs > 0.0) {
+                if (mb.rs > 0.0 || wt) {
                     p1 = fg * (pfczl * (rinvijik - costijk / r2ij - p2etapl * rijs / rij) + pr1 * plambda);
                     p2 = fg * (pfczl * (rinvijik - costijk / r2ik - p2etapl * riks / rik) + pr2 * plambda);
                     p3 = wide ? fg * pfczl * rinvijik
@@ -304,9 +320,9 @@ __global__ __launch_bounds__(kSFThreads) void k_sf64(const ModelT* __restrict__ 
             for (int x = tid; x < n * nmem; x += kSFThreads) {
                 const int k = x / nmem, q = x % nmem;
                 const int ek = L.el[k];
-                if (ek != G.e1 && ek != G.e2) continue;
+                if (!wt && ek != G.e1 && ek != G.e2) continue;
                 const MemberT& mb = M->mem[G.mem0 + q];
-                const int slot = ek == G.e1 ? mb.slot1 : mb.slot2;
+                const int slot = wt ? mb.slotE[ek] : ek == G.e1 ? mb.slot1 : mb.slot2;
                 double* d = dGn + ((size_t)k * o.S + slot) * 3;
                 const double* a = L.acc + (size_t)q * 3 * capL + k;
                 d[0] = a[0]; d[1] = a[capL]; d[2] = a[2 * capL];

@@ -5,7 +5,9 @@
 // (SymGrpCompAngn.cpp:128-390) and SymGrpCompAngw::calculate
 // (SymGrpCompAngw.cpp:117-341) compute: scaled G, the atom's own dG/dr and
 // every neighbour entry's dG/dr in the compact slot order, for the groups of
-// types 20, 21 and 22 (the Gaussian groups are the FP64 SF kernel's).
+// types 20, 21 and 22, and of their element-weighted forms 23, 24 and 25
+// (SymGrpCompRadWeighted, SymGrpCompAng{n,w}Weighted); the Gaussian groups
+// are the FP64 SF kernel's.
 //
 // Mapping: one 256-thread workgroup per atom with the neighbour geometry
 // staged in LDS.  Radial groups: threads over neighbours, members in
@@ -48,8 +50,9 @@ __device__ inline double red_total(const double* red, int slot)
     return v;
 }
 
-// Ordered compaction of the neighbours of element `elem` with
-// rmin < |dr| < rmax (one wave, so the list keeps neighbour order).
+// Ordered compaction of the neighbours of element `elem` (any element if
+// negative) with rmin < |dr| < rmax (one wave, so the list keeps neighbour
+// order).
 __device__ inline int build_list(const CLds& L, int n, int elem, double rmin, double rmax, int* out)
 {
     __shared__ int cnt_sh;
@@ -58,7 +61,7 @@ __device__ inline int build_list(const CLds& L, int n, int elem, double rmin, do
         int cnt = 0;
         for (int b = 0; b < n; b += kWave) {
             const int k = b + lane;
-            const bool ok = k < n && L.el[k] == elem && L.d[k] < rmax && L.d[k] > rmin;
+            const bool ok = k < n && (elem < 0 || L.el[k] == elem) && L.d[k] < rmax && L.d[k] > rmin;
             const int incl = wave_incl_scan(ok ? 1 : 0);
             if (ok) out[cnt + incl - 1] = k;
             cnt += __shfl(incl, kWave - 1, kWave);
@@ -134,20 +137,26 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
     // ---------------------------------------------------------- radial groups
     for (int gi = 0; gi < E.nrad; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
-        if (G.type != 20) continue;
+        if (G.type != 20 && G.type != 23) continue;
+        // weighted (SymGrpCompRadWeighted.cpp:120-200): every neighbour
+        // element, each term times its atomic number
+        const bool wt = G.type == 23;
         for (int q = 0; q < G.nmem; ++q) {
             const MemberT& mb = M->mem[G.mem0 + q];
             double gs = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
             for (int k = tid; k < n; k += kCThreads) {
-                if (L.el[k] != G.e1) continue;
+                if (!wt && L.el[k] != G.e1) continue;
                 double px = 0.0, py = 0.0, pz = 0.0;
                 const double rij = L.d[k];
                 if (rij < G.rc && rij > G.rmin && rij > mb.rl && rij < mb.rcm) {
                     double rad, drad;
                     radial_fdf(mb, rij, rad, drad);
-                    gs += rad;
+                    const double Z = M->Zel[L.el[k]];
+                    if (wt) gs += Z * rad;
+                    else gs += rad;
                     if (drad != 0.0) {
-                        drad *= mb.sfac / rij;
+                        if (wt) drad *= Z * mb.sfac / rij;
+                        else drad *= mb.sfac / rij;
                         px = drad * L.dr[k];
                         py = drad * L.dr[capL + k];
                         pz = drad * L.dr[2 * capL + k];
@@ -155,7 +164,7 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                     }
                 }
                 if (deriv) {
-                    double* d = dGn + ((size_t)k * o.S + mb.slot1) * 3;
+                    double* d = dGn + ((size_t)k * o.S + (wt ? mb.slotE[L.el[k]] : mb.slot1)) * 3;
                     d[0] = -px; d[1] = -py; d[2] = -pz;
                 }
             }
@@ -176,8 +185,11 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
     // --------------------------------------------------------- angular groups
     for (int gi = E.nrad; gi < E.ngrp; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
-        if (G.type != 21 && G.type != 22) continue;
-        const bool narrow = G.type == 21;
+        if (G.type < 21 || G.type == 23) continue;
+        const bool narrow = G.type == 21 || G.type == 24;
+        // weighted (types 24, 25; SymGrpCompAng{n,w}Weighted.cpp): all
+        // neighbour pairs, the angular factor times Z_j Z_k
+        const bool wt = G.type >= 24;
         const bool same = G.e1 == G.e2;
         const int nA = build_list(L, n, G.e1, G.rmin, G.rc, L.la);
         const int nB = same ? nA : build_list(L, n, G.e2, G.rmin, G.rc, L.lb);
@@ -264,6 +276,11 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                         radial_fdf(mem[src], rjk, radjk, dradjk);
                     }
                     compact_fdf(mb.core, 0, mb.acen, mb.ascl, th, ang, dang);
+                    double z = 1.0;
+                    if (wt) {
+                        z = M->Zel[L.el[j]] * M->Zel[L.el[k]];
+                        ang *= z;
+                    }
                     double rad = narrow ? radij * radik * radjk : radij * radik;
                     gacc[q] += rad * ang;
                     if (!deriv) continue;
@@ -277,12 +294,15 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                         chiik = rinvik * radij * dradik * radjk;
                         chijk = -rinvjk * radij * radik * dradjk;
                     } else {
-                        chiij = rinvij * radik * dradij;
+                        // the weighted group multiplies in the other order
+                        // (SymGrpCompAngwWeighted.cpp:257)
+                        chiij = wt ? rinvij * dradij * radik : rinvij * radik * dradij;
                         chiik = rinvik * radij * dradik;
                     }
                     double p1, p2, p3;
                     if (dang != 0.0 && rad != 0.0) {
-                        rad *= mb.sfac;
+                        if (wt) rad *= mb.sfac * z;
+                        else rad *= mb.sfac;
                         ang *= mb.sfac;
                         p1 = rad * phiijik + ang * chiij;
                         p2 = rad * phiikij + ang * chiik;
@@ -317,9 +337,9 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                 for (int x = tid; x < n * nq; x += kCThreads) {
                     const int k = x / nq, q = x % nq;
                     const int ek = L.el[k];
-                    if (ek != G.e1 && ek != G.e2) continue;
+                    if (!wt && ek != G.e1 && ek != G.e2) continue;
                     const MemberT& mb = mem[q0 + q];
-                    const int slot = ek == G.e1 ? mb.slot1 : mb.slot2;
+                    const int slot = wt ? mb.slotE[ek] : ek == G.e1 ? mb.slot1 : mb.slot2;
                     double* d = dGn + ((size_t)k * o.S + slot) * 3;
                     const double* a = L.acc + (size_t)q * 3 * capL + k;
                     d[0] = a[0]; d[1] = a[capL]; d[2] = a[2 * capL];

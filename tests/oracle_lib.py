@@ -248,3 +248,38 @@ def weighted_model(directory):
     24, 25."""
     from n2p2_amd import synth
     return _measured_model(directory, synth.weighted_water_model)
+
+
+def kat_model(tmp, elements, lines):
+    """Model directory holding the known-answer functions of
+    test/cpp/fixtures.h (CT_TANHU, no scaling); every
+    element gets at least one function so every atom has NN inputs."""
+    os.makedirs(tmp, exist_ok=True)
+    allines = list(lines)
+    for e in elements:
+        if not any(l.split()[0] == e for l in lines):
+            allines.append("%s 2 %s 0.001 0.0 10.0" % (e, elements[0]))
+    with open(os.path.join(tmp, "input.nn"), "w") as f:
+        f.write("number_of_elements %d\nelements %s\ncutoff_type 2\n" % (len(elements), " ".join(elements)))
+        f.write("global_hidden_layers_short 1\nglobal_nodes_short 2\nglobal_activation_short t l\n")
+        for l in allines:
+            f.write("symfunction_short %s\n" % l)
+    counts = {e: sum(1 for l in allines if l.split()[0] == e) for e in elements}
+    order = sorted(elements, key=_z)
+    with open(os.path.join(tmp, "scaling.data"), "w") as f:
+        for ie, e in enumerate(order):
+            for k in range(counts[e]):
+                f.write("%d %d 0 1 0 1\n" % (ie + 1, k + 1))
+    for e in order:
+        n = counts[e]
+        with open(os.path.join(tmp, "weights.%03d.data" % _z(e)), "w") as f:
+            for _ in range(n * 2 + 2 + 2 + 1):
+                f.write("0.1\n")
+    return allines, order
+
+
+_ZTAB = {"H": 1, "He": 2, "S": 16, "Cu": 29, "U": 92}
+
+
+def _z(sym):
+    return _ZTAB[sym]

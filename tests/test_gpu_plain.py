@@ -12,7 +12,7 @@ import pytest
 
 import n2p2_amd
 from n2p2_amd import read_input_data, remap, synth
-from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, compact_model
+from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, compact_model, kat_model, weighted_model
 
 pytestmark = pytest.mark.gpu
 
@@ -260,6 +260,58 @@ def test_update_positions(water_dir):
     o.predict(PULL, BRUTE)
     assert abs(m.energies()[0] - o.energy()) <= E_REL * abs(o.energy())
     assert np.abs(m.forces() - o.forces()).max() <= F_ABS
+
+
+@pytest.mark.parametrize("name", ["Mono", "Dual", "Mixed"])
+def test_sf_known_answers_gpu(name, tmp_path):
+    """test/cpp/fixtures.h known answers of every SF type (2, 3, 9, 12, 13,
+    20-25) on the fixture clusters through the plain GPU form: G of atom 0 at
+    1000*eps (test_SymmetryFunctions.cpp:36), forces equal to the oracle's."""
+    k = json.load(open(os.path.join(HERE, "golden", "sf_kat.json")))["sets"][name]
+    say("case: SF known answers on the GPU, %s set (%d functions)" % (name, len(k["functions"])))
+    worst = 0.0
+    for q, row in enumerate(k["functions"]):
+        d = str(tmp_path / ("m%d" % q))
+        kat_model(d, k["elements"], [row["line"]])
+        pos = np.array(k["pos"], float)
+        elem = np.array(k["elem"], np.int32)
+        m = n2p2_amd.Mode(d)
+        m.set_structures(pos, elem, None)
+        m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+        g = m.G()[0, 0]
+        o = Oracle(d)
+        o.set_structure(pos, elem, None)
+        o.predict(PULL, BRUTE)
+        dF = float(np.abs(m.forces() - o.forces()).max())
+        worst = max(worst, abs(g - row["value"]))
+        assert abs(g - row["value"]) <= 1000 * np.finfo(float).eps, (row["line"], g, row["value"])
+        assert dF <= F_ABS, (row["line"], dF)
+    say("  max |G - KAT| %.2e" % worst)
+
+
+@pytest.mark.parametrize("n_side", [4, 8])
+def test_weighted_model_plain(n_side, tmp_path_factory):
+    """Water model mixing element-resolved types 2/3 with the element-weighted
+    types 12, 13, 23, 24, 25: G, dE/dG, energy and forces against the oracle;
+    192 atoms (brute-force list) and 1,536 atoms (cell list)."""
+    say("case: weighted water model, %d atoms, plain FP64" % (3 * n_side ** 3))
+    d = weighted_model(str(tmp_path_factory.mktemp("weighted") / "m"))
+    pos, elem, box = synth.water_box(n_side)
+    m = n2p2_amd.Mode(d)
+    m.set_structures(remap(pos, box), elem, box[None])
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE if n_side == 4 else CELL)
+    G, Go = m.G(), o.G()
+    gerr = float((np.abs(G - Go) / np.maximum(np.abs(Go), 1.0)).max())
+    E, Eo = m.energies()[0], o.energy()
+    dF = float(np.abs(m.forces() - o.forces()).max())
+    say("  G err %.2e  E rel %.2e  max|dF| %.2e (|F| rms %.3f)"
+        % (gerr, abs(E - Eo) / abs(Eo), dF, float(np.sqrt(np.mean(o.forces() ** 2)))))
+    assert gerr <= 1e-12
+    assert abs(E - Eo) <= E_REL * abs(Eo)
+    assert dF <= F_ABS
 
 
 @pytest.mark.parametrize("n_side", [4, 8])

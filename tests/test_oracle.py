@@ -8,7 +8,8 @@ import os
 import numpy as np
 import pytest
 
-from oracle_lib import BRUTE, CELL, PULL, PUSH, Oracle, RefLib, compact_model, cutoff_fdf, ref_available, weighted_model
+from oracle_lib import (BRUTE, CELL, PULL, PUSH, Oracle, RefLib, compact_model, cutoff_fdf, kat_model,
+                        ref_available, weighted_model)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -47,40 +48,6 @@ def test_cutoff_known_answers():
         assert abs(dfc + np.pi / 6 * np.sin(np.pi * r / 3)) < 1e-14
 
 
-def _kat_model(tmp, elements, lines):
-    """Model directory holding the KAT functions (CT_TANHU, no scaling); every
-    element gets at least one function so every atom has NN inputs."""
-    os.makedirs(tmp, exist_ok=True)
-    allines = list(lines)
-    for e in elements:
-        if not any(l.split()[0] == e for l in lines):
-            allines.append("%s 2 %s 0.001 0.0 10.0" % (e, elements[0]))
-    with open(os.path.join(tmp, "input.nn"), "w") as f:
-        f.write("number_of_elements %d\nelements %s\ncutoff_type 2\n" % (len(elements), " ".join(elements)))
-        f.write("global_hidden_layers_short 1\nglobal_nodes_short 2\nglobal_activation_short t l\n")
-        for l in allines:
-            f.write("symfunction_short %s\n" % l)
-    counts = {e: sum(1 for l in allines if l.split()[0] == e) for e in elements}
-    order = sorted(elements, key=_z)
-    with open(os.path.join(tmp, "scaling.data"), "w") as f:
-        for ie, e in enumerate(order):
-            for k in range(counts[e]):
-                f.write("%d %d 0 1 0 1\n" % (ie + 1, k + 1))
-    for e in order:
-        n = counts[e]
-        with open(os.path.join(tmp, "weights.%03d.data" % _z(e)), "w") as f:
-            for _ in range(n * 2 + 2 + 2 + 1):
-                f.write("0.1\n")
-    return allines, order
-
-
-_ZTAB = {"H": 1, "He": 2, "S": 16, "Cu": 29, "U": 92}
-
-
-def _z(sym):
-    return _ZTAB[sym]
-
-
 @pytest.mark.parametrize("name", ["Mono", "Dual", "Mixed"])
 def test_sf_known_answers(tmp_path, name):
     """test/cpp/fixtures.h:151-258 values of atom 0 for types 2, 3, 9, 12, 13, 20-25 at 1000*eps
@@ -89,7 +56,7 @@ def test_sf_known_answers(tmp_path, name):
     k = json.load(open(os.path.join(GOLD, "sf_kat.json")))["sets"][name]
     for row in k["functions"]:
         d = str(tmp_path / ("m%d" % abs(hash(row["line"]))))
-        lines, order = _kat_model(d, k["elements"], [row["line"]])
+        lines, order = kat_model(d, k["elements"], [row["line"]])
         o = Oracle(d)
         # element indices of the fixture are in ElementMap (Z) order already
         o.set_structure(np.array(k["pos"], float), np.array(k["elem"]), None)
