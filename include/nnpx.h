@@ -199,6 +199,19 @@ nnpx_status nnpx_set_weights(nnpx_ctx* ctx, const double* w);
 nnpx_status nnpx_loss_gradient(nnpx_ctx* ctx, int precision, const double* E_ref,
                                const double* F_ref, double beta, double* loss,
                                double* grad);
+/* Weight derivatives of energies and forces, one gradient per structure
+ * (replaces Training::dPdc, Training.cpp:3319-3389, for the "energy" and
+ * "force" properties).  With seeds vE[n_struct] and vF[3N], structure s
+ * receives
+ *   grad[s * n_conn + c] = vE[s] dE_s/dw_c + sum_{a in s, k} vF[3a+k] dF_ak/dw_c
+ * (n_conn = nnpx_num_connections(ctx, -1, ...), layout of nnpx_get_weights).
+ * vE = 1, vF = 0 gives the "energy" row; a unit vF on (atom a, component k)
+ * gives the "force" row of that component, so packing copies of a structure
+ * with different seeds yields the whole Jacobian in one call.  Evaluates the
+ * structures first (forces included).  Model units; normalize_nodes is not
+ * supported. */
+nnpx_status nnpx_weight_vjp(nnpx_ctx* ctx, int precision, const double* vE,
+                            const double* vF, double* grad);
 
 /* ---- molecular dynamics ----------------------------------------------- */
 /* No counterpart in nnp::Mode: the velocity-Verlet loop a driver runs around

@@ -217,6 +217,30 @@ public:
 void writeFileHeader(std::ofstream& o, std::vector<std::string> const& title, std::vector<size_t> const& size,
                      std::vector<std::string> const& name, std::vector<std::string> const& info);
 
+/* printf into a std::string (n2p2's strpr, utility.cpp:247-264). */
+std::string strpr(const char* format, ...) __attribute__((format(printf, 1, 2)));
+
+/* All configurations of an input.data file as line blocks begin..end (the
+ * reading loop of Dataset::loadStructuresFromFile); throws when the file
+ * cannot be opened. */
+std::vector<std::vector<std::string>> readStructureBlocks(std::string const& fileName);
+
+/* Keyword present in a settings file (input.nn), comments after '#'
+ * ignored (Settings::keywordExists on the file's contents). */
+bool settingsFileHasKeyword(std::string const& fileName, std::string const& keyword);
+
+/* Positions (scaled by a length factor), element indices, boxes and
+ * periodicity of a list of structures, in the packed layout of
+ * nnpx_set_structures. */
+struct PackedStructures {
+    std::vector<int64_t> offset{0};
+    std::vector<double> position, box;
+    std::vector<int> element, periodic;
+    void add(Structure const& s, double convLength);
+    int count() const { return (int)periodic.size(); }
+    void clear();
+};
+
 }  // namespace nnpx
 
 #endif /* NNPX_MODE_HPP */

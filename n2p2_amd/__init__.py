@@ -89,6 +89,7 @@ _SIGS = {
     "nnpx_set_weights": [ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_loss_gradient": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                            ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_weight_vjp": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_md_state": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_md_energies": [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, _c_i64_p, _c_i64_p],
 }
@@ -347,6 +348,17 @@ class Mode:
                                               _ptr(g)))
         return loss, g
 
+    def weight_vjp(self, vE, vF, precision=PLAIN_FP64):
+        """Per-structure weight derivatives [n_struct, n_conn]:
+        vE_s dE_s/dw + sum_{a in s} vF_a . dF_a/dw (Training::dPdc rows)."""
+        E = np.ascontiguousarray(vE, dtype=np.float64).reshape(-1)
+        F = np.ascontiguousarray(vF, dtype=np.float64).reshape(-1)
+        if len(E) != self.n_struct or len(F) != 3 * self.n_atoms:
+            raise NNPError(1, "seeds have the wrong size")
+        g = np.zeros((self.n_struct, self.num_connections()))
+        self._ck(self._lib.nnpx_weight_vjp(self._h, precision, _ptr(E), _ptr(F), _ptr(g)))
+        return g
+
     # ------------------------------------------------------------ MD --
     def md_init(self, masses, velocities, dt, skin, precision=MIXED):
         """Velocity Verlet set-up: masses per element (amu), velocities
@@ -383,20 +395,21 @@ class Mode:
 
 
 # ---------------------------------------------------------------- files --
-def read_input_data(path, symbols):
-    """First structure of an n2p2 input.data file (Structure::readFromLines,
-    Structure.cpp:142-271) without the remap: (pos, elem, box or None)."""
-    pos, el, box = [], [], []
+def read_structures(path, symbols):
+    """All structures of an n2p2 input.data file (Structure::readFromLines,
+    Structure.cpp:142-271) without the remap: a list of (pos, elem, box or
+    None)."""
+    out = []
     index = {s: i for i, s in enumerate(symbols)}
+    pos = el = box = None
     with open(path) as f:
-        begun = False
         for line in f:
             w = line.split()
             if not w:
                 continue
             if w[0] == "begin":
-                begun = True
-            elif not begun:
+                pos, el, box = [], [], []
+            elif pos is None:
                 raise ValueError("expected 'begin'")
             elif w[0] == "lattice":
                 box.append([float(x) for x in w[1:4]])
@@ -404,10 +417,19 @@ def read_input_data(path, symbols):
                 pos.append([float(x) for x in w[1:4]])
                 el.append(index[w[4]])
             elif w[0] == "end":
-                break
-    if box and len(box) != 3:
-        raise ValueError("strange number of box vectors")
-    return np.array(pos), np.array(el, np.int32), (np.array(box) if box else None)
+                if box and len(box) != 3:
+                    raise ValueError("strange number of box vectors")
+                out.append((np.array(pos), np.array(el, np.int32), (np.array(box) if box else None)))
+                pos = None
+    return out
+
+
+def read_input_data(path, symbols):
+    """First structure of an n2p2 input.data file: (pos, elem, box or None)."""
+    s = read_structures(path, symbols)
+    if not s:
+        raise ValueError("no structure in %s" % path)
+    return s[0]
 
 
 def inverse_box(box):

@@ -1183,20 +1183,94 @@ nnpx_status nnpx_set_weights(nnpx_ctx* c, const double* w)
     });
 }
 
+namespace {
+
+// Weight vector-Jacobian product of the evaluated structures: with the seeds
+// u (3N, device) on the forces and cE (S, device) on the energies,
+//   grad += sum_s cE_s dE_s/dc + sum_{a,c} u_ac dF_ac/dc,
+// the tangent Gdot = sum_ac u_ac dG/dx_ac (Training::collectDGdxia,
+// Training.cpp:3586-3625) followed by the network's forward tangent and one
+// reverse pass (NeuralNetwork::calculateDEdc / calculateDFdc,
+// NeuralNetwork.cpp:444-527).  ldgrad = 0 sums over structures; otherwise
+// structure s adds into gd + s * ldgrad.  gd must be cleared by the caller.
+void weight_vjp(nnpx_ctx* c, int precision, const double* u, const double* cE, double* gd, int ldgrad)
+{
+    const ModelT& t = c->hm.t;
+    const int N = c->N, ldG = t.max_sf;
+    double* Gdot = buf<double>(c, "t_Gdot", (size_t)N * ldG);
+    hip_check(hipMemsetAsync(Gdot, 0, sizeof(double) * (size_t)N * ldG, c->stream), "hipMemsetAsync");
+    SysView s = sysview(c);
+    if (precision == NNPX_PLAIN_FP64) {
+        timed(c, "tangent64", [&] {
+            launch_tangent64(c->d_model, s, existing<double>(c, "dGself"), existing<double>(c, "dGnbr"),
+                             std::max(t.max_sfd, 1), existing<int>(c, "islot"), u, Gdot, ldG, c->stream);
+        });
+    } else {
+        timed(c, "tangent32", [&] { launch_tangent32(c->mixed, t, c->d_model, s, u, Gdot, ldG, c->stream); });
+    }
+    GradArgs a{};
+    int total = 0;
+    for (int e = 0; e < t.nelem; ++e) {
+        a.coff[e] = total;
+        a.nconn[e] = num_connections(t.el[e].nn);
+        a.nconn_max = std::max(a.nconn_max, a.nconn[e]);
+        total += a.nconn[e];
+    }
+    a.w = c->d_w;
+    a.nweights = t.nweights;
+    a.G = existing<double>(c, "G");
+    a.Gdot = Gdot;
+    a.ldG = ldG;
+    a.cE = cE;
+    a.sid = existing<int>(c, "sid");
+    a.atoms = existing<int>(c, "elem_atoms");
+    a.nelem = t.nelem;
+    a.maxw = std::max(t.max_width, t.max_sf);
+    a.per_block = c->mixed.tune.grad_atoms;
+    a.grad = gd;
+    a.ldgrad = ldgrad;
+    int blocks = 0;
+    for (int e = 0; e < t.nelem; ++e) {
+        a.blk_off[e] = blocks;
+        a.atom_off[e] = c->elem_off[e];
+        blocks += (c->elem_count[e] + a.per_block - 1) / a.per_block;
+    }
+    a.blk_off[t.nelem] = blocks;
+    a.atom_off[t.nelem] = c->elem_off[t.nelem];
+    if (grad_lds_bytes(a, precision == NNPX_MIXED) > 65536)
+        fail(NNPX_ERR_UNSUPPORTED, "network too large for the gradient kernel's LDS");
+    timed(c, precision == NNPX_MIXED ? "grad32" : "grad64",
+          [&] { launch_grad(c->d_model, a, blocks, precision == NNPX_MIXED, c->stream); });
+}
+
+int total_connections(const ModelT& t)
+{
+    int total = 0;
+    for (int e = 0; e < t.nelem; ++e) total += num_connections(t.el[e].nn);
+    return total;
+}
+
+void gradient_checks(nnpx_ctx* c, int precision)
+{
+    need_structures(c);
+    if (precision != NNPX_PLAIN_FP64 && precision != NNPX_MIXED) fail(NNPX_ERR_ARGUMENT, "bad precision");
+    const ModelT& t = c->hm.t;
+    for (int e = 0; e < t.nelem; ++e)
+        if (t.el[e].nn.normalize) fail(NNPX_ERR_UNSUPPORTED, "weight derivatives with normalize_nodes");
+}
+
+}  // namespace
+
 nnpx_status nnpx_loss_gradient(nnpx_ctx* c, int precision, const double* E_ref, const double* F_ref, double beta,
                                double* loss, double* grad)
 {
     return guard(c, [&] {
-        need_structures(c);
-        if (precision != NNPX_PLAIN_FP64 && precision != NNPX_MIXED) fail(NNPX_ERR_ARGUMENT, "bad precision");
+        gradient_checks(c, precision);
         if (!E_ref || !F_ref || !loss || !grad) fail(NNPX_ERR_ARGUMENT, "null argument");
         if (!(beta >= 0.0)) fail(NNPX_ERR_ARGUMENT, "beta must be >= 0");
-        const ModelT& t = c->hm.t;
-        for (int e = 0; e < t.nelem; ++e)
-            if (t.el[e].nn.normalize) fail(NNPX_ERR_UNSUPPORTED, "loss gradient with normalize_nodes");
         evaluate(c, precision, NNPX_EVAL_FORCES, true);
         settle(c);
-        const int N = c->N, S = c->S, ldG = t.max_sf;
+        const int N = c->N, S = c->S;
         double* Eref = buf<double>(c, "t_Eref", S);
         double* Fref = buf<double>(c, "t_Fref", (size_t)3 * N);
         hip_check(hipMemcpyAsync(Eref, E_ref, sizeof(double) * S, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
@@ -1205,62 +1279,42 @@ nnpx_status nnpx_loss_gradient(nnpx_ctx* c, int precision, const double* E_ref, 
         double* u = buf<double>(c, "t_u", (size_t)3 * N);
         double* cE = buf<double>(c, "t_cE", S);
         double* lossd = buf<double>(c, "t_loss", 4);
-        double* Gdot = buf<double>(c, "t_Gdot", (size_t)N * ldG);
-        int total = 0;
-        GradArgs a{};
-        for (int e = 0; e < t.nelem; ++e) {
-            a.coff[e] = total;
-            a.nconn[e] = num_connections(t.el[e].nn);
-            a.nconn_max = std::max(a.nconn_max, a.nconn[e]);
-            total += a.nconn[e];
-        }
+        const int total = total_connections(c->hm.t);
         double* gd = buf<double>(c, "t_grad", total);
         hip_check(hipMemsetAsync(lossd, 0, 4 * sizeof(double), c->stream), "hipMemsetAsync");
         hip_check(hipMemsetAsync(gd, 0, sizeof(double) * total, c->stream), "hipMemsetAsync");
-        hip_check(hipMemsetAsync(Gdot, 0, sizeof(double) * (size_t)N * ldG, c->stream), "hipMemsetAsync");
         timed(c, "residual", [&] {
             launch_residual(existing<double>(c, "F"), Fref, existing<double>(c, "E"), Eref, existing<int>(c, "atom_off"),
                             N, S, beta / (3.0 * N), u, cE, lossd, c->stream);
         });
-        SysView s = sysview(c);
-        if (precision == NNPX_PLAIN_FP64) {
-            timed(c, "tangent64", [&] {
-                launch_tangent64(c->d_model, s, existing<double>(c, "dGself"), existing<double>(c, "dGnbr"),
-                                 std::max(t.max_sfd, 1), existing<int>(c, "islot"), u, Gdot, ldG, c->stream);
-            });
-        } else {
-            timed(c, "tangent32", [&] { launch_tangent32(c->mixed, t, c->d_model, s, u, Gdot, ldG, c->stream); });
-        }
-        a.w = c->d_w;
-        a.nweights = t.nweights;
-        a.G = existing<double>(c, "G");
-        a.Gdot = Gdot;
-        a.ldG = ldG;
-        a.cE = cE;
-        a.sid = existing<int>(c, "sid");
-        a.atoms = existing<int>(c, "elem_atoms");
-        a.nelem = t.nelem;
-        a.maxw = std::max(t.max_width, t.max_sf);
-        a.per_block = c->mixed.tune.grad_atoms;
-        a.grad = gd;
-        int blocks = 0;
-        for (int e = 0; e < t.nelem; ++e) {
-            a.blk_off[e] = blocks;
-            a.atom_off[e] = c->elem_off[e];
-            blocks += (c->elem_count[e] + a.per_block - 1) / a.per_block;
-        }
-        a.blk_off[t.nelem] = blocks;
-        a.atom_off[t.nelem] = c->elem_off[t.nelem];
-        if (grad_lds_bytes(a, precision == NNPX_MIXED) > 65536)
-            fail(NNPX_ERR_UNSUPPORTED, "network too large for the gradient kernel's LDS");
-        timed(c, precision == NNPX_MIXED ? "grad32" : "grad64",
-              [&] { launch_grad(c->d_model, a, blocks, precision == NNPX_MIXED, c->stream); });
+        weight_vjp(c, precision, u, cE, gd, 0);
         double l[4];
         copy_out(c, l, lossd, sizeof l);
         loss[1] = l[1];
         loss[2] = l[2];
         loss[0] = l[1] + l[2];
         copy_out(c, grad, gd, sizeof(double) * total);
+        collect_times(c);
+    });
+}
+
+nnpx_status nnpx_weight_vjp(nnpx_ctx* c, int precision, const double* vE, const double* vF, double* grad)
+{
+    return guard(c, [&] {
+        gradient_checks(c, precision);
+        if (!vE || !vF || !grad) fail(NNPX_ERR_ARGUMENT, "null argument");
+        evaluate(c, precision, NNPX_EVAL_FORCES, true);
+        settle(c);
+        const int N = c->N, S = c->S;
+        const int total = total_connections(c->hm.t);
+        double* u = buf<double>(c, "t_u", (size_t)3 * N);
+        double* cE = buf<double>(c, "t_cE", S);
+        double* gd = buf<double>(c, "t_gradS", (size_t)S * total);
+        hip_check(hipMemcpyAsync(cE, vE, sizeof(double) * S, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(u, vF, sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+        hip_check(hipMemsetAsync(gd, 0, sizeof(double) * (size_t)S * total, c->stream), "hipMemsetAsync");
+        weight_vjp(c, precision, u, cE, gd, total);
+        copy_out(c, grad, gd, sizeof(double) * (size_t)S * total);
         collect_times(c);
     });
 }

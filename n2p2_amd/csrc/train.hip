@@ -166,9 +166,23 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
     const int first = a.atom_off[e] + ((int)blockIdx.x - a.blk_off[e]) * a.per_block;
     const int last = min(first + a.per_block, a.atom_off[e + 1]);
     __syncthreads();
+    // per-structure sums: flushed when the structure changes (the block's
+    // atoms are in index order, so each structure's atoms are contiguous)
+    int cur = -1;
     for (int p = first; p < last; ++p) {
         const int i = a.atoms[p];
-        const T cE = (T)a.cE[a.sid[i]];
+        const int si = a.sid[i];
+        if (a.ldgrad && si != cur) {
+            if (cur >= 0) {
+                double* g = a.grad + (size_t)cur * a.ldgrad + a.coff[e];
+                for (int x = lane; x < nconn; x += kWave) {
+                    atomicAdd(g + x, acc[x]);
+                    acc[x] = 0.0;
+                }
+            }
+            cur = si;
+        }
+        const T cE = (T)a.cE[si];
         for (int k = lane; k < N.n[0]; k += kWave) {
             H[k] = (T)a.G[(size_t)i * a.ldG + k];
             Hd[k] = (T)a.Gdot[(size_t)i * a.ldG + k];
@@ -242,7 +256,8 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
         }
     }
     __syncthreads();
-    double* g = a.grad + a.coff[e];
+    if (a.ldgrad && cur < 0) return;
+    double* g = a.grad + (a.ldgrad ? (size_t)cur * a.ldgrad : 0) + a.coff[e];
     for (int x = lane; x < nconn; x += kWave) atomicAdd(g + x, acc[x]);
 }
 

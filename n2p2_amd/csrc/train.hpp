@@ -29,6 +29,7 @@ struct GradArgs {
     int maxw;              // max layer width
     int per_block;         // atoms of one element per block
     double* grad;          // [sum nconn], accumulated
+    int ldgrad;            // 0: one sum; else grad + sid * ldgrad per structure
 };
 
 void launch_residual(const double* F, const double* Fref, const double* E, const double* Eref, const int* atom_off,

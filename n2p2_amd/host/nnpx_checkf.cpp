@@ -27,16 +27,7 @@
 
 namespace {
 
-std::string strpr(const char* f, ...) __attribute__((format(printf, 1, 2)));
-std::string strpr(const char* f, ...)
-{
-    char buf[1024];
-    va_list ap;
-    va_start(ap, f);
-    vsnprintf(buf, sizeof buf, f, ap);
-    va_end(ap);
-    return buf;
-}
+using nnpx::strpr;
 
 int usage(const char* argv0)
 {
@@ -50,55 +41,10 @@ int usage(const char* argv0)
     return 1;
 }
 
-std::vector<std::vector<std::string>> read_blocks(std::string const& file)
-{
-    std::ifstream in(file);
-    if (!in) throw std::runtime_error("ERROR: Could not open file: \"" + file + "\".\n");
-    std::vector<std::vector<std::string>> blocks;
-    std::vector<std::string> cur;
-    std::string line;
-    bool open = false;
-    while (std::getline(in, line)) {
-        std::istringstream w(line);
-        std::string k;
-        if (!(w >> k)) continue;
-        if (k == "begin") {
-            cur.clear();
-            open = true;
-        }
-        if (open) cur.push_back(line);
-        if (k == "end" && open) {
-            blocks.push_back(cur);
-            open = false;
-        }
-    }
-    return blocks;
-}
-
 void check(nnpx_ctx* ctx, nnpx_status st)
 {
     if (st != NNPX_OK) throw std::runtime_error(std::string("ERROR: ") + nnpx_last_error(ctx) + "\n");
 }
-
-// Positions (model units), elements, box and periodicity of a list of
-// structures, packed for nnpx_set_structures.
-struct Packed {
-    std::vector<int64_t> off{0};
-    std::vector<double> pos, box;
-    std::vector<int> elem, per;
-    void add(nnpx::Structure const& s, double convLength)
-    {
-        for (auto const& a : s.atoms) {
-            for (int c = 0; c < 3; ++c) pos.push_back(a.r[c] * convLength);
-            elem.push_back((int)a.element);
-        }
-        for (int r = 0; r < 3; ++r)
-            for (int c = 0; c < 3; ++c) box.push_back(s.box[r][c] * convLength);
-        per.push_back(s.isPeriodic ? 1 : 0);
-        off.push_back(off.back() + (int64_t)s.numAtoms);
-    }
-    int count() const { return (int)per.size(); }
-};
 
 }  // namespace
 
@@ -149,7 +95,7 @@ int main(int argc, char* argv[])
              "Mean over all absolute differences between analytic and numeric forces in this structure.",
              "Maximum over all absolute differences between analytic and numeric forces in this structure."});
 
-        auto blocks = read_blocks("input.data");
+        auto blocks = nnpx::readStructureBlocks("input.data");
         log << strpr("Found %zu configurations in data file: %s.\n", blocks.size(), "input.data");
         log << strpr("Starting loop over %zu configurations...\n", blocks.size());
         log << "\n";
@@ -163,10 +109,10 @@ int main(int argc, char* argv[])
             const size_t N = original.numAtoms;
 
             // analytic forces of the original (physical units)
-            Packed p0;
+            nnpx::PackedStructures p0;
             p0.add(original, cL);
-            check(ctx, nnpx_set_structures(ctx, 1, p0.off.data(), p0.pos.data(), p0.elem.data(), p0.box.data(),
-                                           p0.per.data()));
+            check(ctx, nnpx_set_structures(ctx, 1, p0.offset.data(), p0.position.data(), p0.element.data(),
+                                           p0.box.data(), p0.periodic.data()));
             check(ctx, nnpx_evaluate(ctx, NNPX_PLAIN_FP64, NNPX_EVAL_FORCES));
             std::vector<double> Fa(3 * N);
             check(ctx, nnpx_get_forces(ctx, Fa.data()));
@@ -179,7 +125,7 @@ int main(int argc, char* argv[])
             const size_t perBatch = std::max<size_t>(1, (size_t)batchAtoms / std::max<size_t>(N, 1));
             for (size_t c0 = 0; c0 < ncopy; c0 += perBatch) {
                 const size_t c1 = std::min(ncopy, c0 + perBatch);
-                Packed pk;
+                nnpx::PackedStructures pk;
                 for (size_t c = c0; c < c1; ++c) {
                     nnpx::Structure s = original;
                     const size_t ia = c / 6, ix = c % 3;
@@ -188,8 +134,8 @@ int main(int argc, char* argv[])
                     if (s.isPeriodic) s.remap(s.atoms[ia]);
                     pk.add(s, cL);
                 }
-                check(ctx, nnpx_set_structures(ctx, pk.count(), pk.off.data(), pk.pos.data(), pk.elem.data(),
-                                               pk.box.data(), pk.per.data()));
+                check(ctx, nnpx_set_structures(ctx, pk.count(), pk.offset.data(), pk.position.data(), pk.element.data(),
+                                               pk.box.data(), pk.periodic.data()));
                 check(ctx, nnpx_evaluate(ctx, NNPX_PLAIN_FP64, 0u));
                 std::vector<double> E(pk.count());
                 check(ctx, nnpx_get_energy(ctx, E.data()));

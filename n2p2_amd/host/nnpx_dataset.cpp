@@ -30,16 +30,7 @@
 
 namespace {
 
-std::string strpr(const char* f, ...) __attribute__((format(printf, 1, 2)));
-std::string strpr(const char* f, ...)
-{
-    char buf[1024];
-    va_list ap;
-    va_start(ap, f);
-    vsnprintf(buf, sizeof buf, f, ap);
-    va_end(ap);
-    return buf;
-}
+using nnpx::strpr;
 
 int usage(const char* argv0)
 {
@@ -53,47 +44,6 @@ int usage(const char* argv0)
               << "       - scaling.data (symmetry function scaling data)\n"
               << "       - \"weights.%03d.data\" (weights files)\n";
     return 1;
-}
-
-// Keyword present in input.nn (comments after '#' ignored).
-bool has_keyword(std::string const& file, std::string const& key)
-{
-    std::ifstream in(file);
-    std::string line;
-    while (std::getline(in, line)) {
-        const size_t h = line.find('#');
-        if (h != std::string::npos) line.erase(h);
-        std::istringstream w(line);
-        std::string k;
-        if (w >> k && k == key) return true;
-    }
-    return false;
-}
-
-// All configurations of an input.data file, as line blocks begin..end.
-std::vector<std::vector<std::string>> read_blocks(std::string const& file)
-{
-    std::ifstream in(file);
-    if (!in) throw std::runtime_error("ERROR: Could not open file: \"" + file + "\".\n");
-    std::vector<std::vector<std::string>> blocks;
-    std::vector<std::string> cur;
-    std::string line;
-    bool open = false;
-    while (std::getline(in, line)) {
-        std::istringstream w(line);
-        std::string k;
-        if (!(w >> k)) continue;
-        if (k == "begin") {
-            cur.clear();
-            open = true;
-        }
-        if (open) cur.push_back(line);
-        if (k == "end" && open) {
-            blocks.push_back(cur);
-            open = false;
-        }
-    }
-    return blocks;
 }
 
 // Structure::updateError / Atom::updateError (Structure.cpp:1387-1415,
@@ -166,7 +116,7 @@ int main(int argc, char* argv[])
         mode.setupFromDirectory(".");
         nnpx_ctx* ctx = mode.context();
         const bool normalize = mode.useNormalization();
-        const bool useForces = has_keyword("input.nn", "use_short_forces");
+        const bool useForces = nnpx::settingsFileHasKeyword("input.nn", "use_short_forces");
         const double cE = mode.getConvEnergy(), cL = mode.getConvLength(), mE = mode.getMeanEnergy();
         const size_t ne = mode.getNumElements();
         int numElements = 0, maxSF = 0;
@@ -182,7 +132,7 @@ int main(int argc, char* argv[])
         // the atomic energy offsets of the reference energy; then
         // Dataset::toNormalizedUnits.
         std::vector<nnpx::Structure> structures;
-        for (auto const& lines : read_blocks(dataFileName)) {
+        for (auto const& lines : nnpx::readStructureBlocks(dataFileName)) {
             structures.emplace_back();
             nnpx::Structure& s = structures.back();
             s.setElementMap(mode.elementMap);
@@ -208,21 +158,10 @@ int main(int argc, char* argv[])
             while (b1 < structures.size() && (b1 == b0 || atoms + (long)structures[b1].numAtoms <= batchAtoms))
                 atoms += (long)structures[b1++].numAtoms;
             const int ns = (int)(b1 - b0);
-            std::vector<int64_t> off(ns + 1, 0);
-            std::vector<double> pos(3 * atoms), box(9 * ns, 0.0);
-            std::vector<int> elem(atoms), per(ns);
-            for (int q = 0; q < ns; ++q) {
-                nnpx::Structure const& s = structures[b0 + q];
-                off[q + 1] = off[q] + (int64_t)s.numAtoms;
-                for (size_t i = 0; i < s.numAtoms; ++i) {
-                    for (int c = 0; c < 3; ++c) pos[3 * (off[q] + i) + c] = s.atoms[i].r[c];
-                    elem[off[q] + i] = (int)s.atoms[i].element;
-                }
-                for (int r = 0; r < 3; ++r)
-                    for (int c = 0; c < 3; ++c) box[9 * q + 3 * r + c] = s.box[r][c];
-                per[q] = s.isPeriodic ? 1 : 0;
-            }
-            check(ctx, nnpx_set_structures(ctx, ns, off.data(), pos.data(), elem.data(), box.data(), per.data()));
+            nnpx::PackedStructures pk;
+            for (int q = 0; q < ns; ++q) pk.add(structures[b0 + q], 1.0);
+            check(ctx, nnpx_set_structures(ctx, ns, pk.offset.data(), pk.position.data(), pk.element.data(),
+                                           pk.box.data(), pk.periodic.data()));
             // dE/dG is always computed (the sensitivity needs it, as
             // nnp-dataset.cpp:210-215 asks for derivatives in any case)
             check(ctx, nnpx_evaluate(ctx, mixed ? NNPX_MIXED : NNPX_PLAIN_FP64, useForces ? NNPX_EVAL_FORCES : 0u));
@@ -236,7 +175,7 @@ int main(int argc, char* argv[])
                 s.energy = E[q];
                 for (size_t i = 0; i < s.numAtoms; ++i) {
                     nnpx::Atom& a = s.atoms[i];
-                    const size_t g = (size_t)off[q] + i;
+                    const size_t g = (size_t)pk.offset[q] + i;
                     a.energy = Ea[g];
                     if (useForces)
                         for (int c = 0; c < 3; ++c) a.f[c] = F[3 * g + c];

@@ -21,16 +21,7 @@
 
 namespace {
 
-std::string strpr(const char* f, ...) __attribute__((format(printf, 1, 2)));
-std::string strpr(const char* f, ...)
-{
-    char buf[1024];
-    va_list ap;
-    va_start(ap, f);
-    vsnprintf(buf, sizeof buf, f, ap);
-    va_end(ap);
-    return buf;
-}
+using nnpx::strpr;
 
 int usage(const char* argv0)
 {

@@ -1,6 +1,10 @@
 // Host utilities shared by the command-line tools.
+#include <cstdarg>
+#include <cstdio>
+#include <fstream>
 #include <iomanip>
 #include <sstream>
+#include <stdexcept>
 
 #include "nnpx_mode.hpp"
 
@@ -44,5 +48,75 @@ void writeFileHeader(std::ofstream& o, std::vector<std::string> const& title, st
     o << num.str() << '\n' << nam.str() << '\n' << std::string(wd, '#') << "\n";
 }
 
+
+std::string strpr(const char* format, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, format);
+    vsnprintf(buf, sizeof buf, format, ap);
+    va_end(ap);
+    return buf;
+}
+
+std::vector<std::vector<std::string>> readStructureBlocks(std::string const& fileName)
+{
+    std::ifstream in(fileName);
+    if (!in) throw std::runtime_error("ERROR: Could not open file: \"" + fileName + "\".\n");
+    std::vector<std::vector<std::string>> blocks;
+    std::vector<std::string> cur;
+    std::string line;
+    bool open = false;
+    while (std::getline(in, line)) {
+        std::istringstream w(line);
+        std::string k;
+        if (!(w >> k)) continue;
+        if (k == "begin") {
+            cur.clear();
+            open = true;
+        }
+        if (open) cur.push_back(line);
+        if (k == "end" && open) {
+            blocks.push_back(cur);
+            open = false;
+        }
+    }
+    return blocks;
+}
+
+bool settingsFileHasKeyword(std::string const& fileName, std::string const& keyword)
+{
+    std::ifstream in(fileName);
+    std::string line;
+    while (std::getline(in, line)) {
+        const size_t h = line.find('#');
+        if (h != std::string::npos) line.erase(h);
+        std::istringstream w(line);
+        std::string k;
+        if (w >> k && k == keyword) return true;
+    }
+    return false;
+}
+
+void PackedStructures::add(Structure const& s, double convLength)
+{
+    for (auto const& a : s.atoms) {
+        for (int c = 0; c < 3; ++c) position.push_back(a.r[c] * convLength);
+        element.push_back((int)a.element);
+    }
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) box.push_back(s.box[r][c] * convLength);
+    periodic.push_back(s.isPeriodic ? 1 : 0);
+    offset.push_back(offset.back() + (int64_t)s.numAtoms);
+}
+
+void PackedStructures::clear()
+{
+    offset.assign(1, 0);
+    position.clear();
+    box.clear();
+    element.clear();
+    periodic.clear();
+}
 
 }  // namespace nnpx

@@ -2102,6 +2102,65 @@ int oracle_set_weights(oracle_ctx* c, const double* w)
     return 0;
 }
 
+// Training::dPdc (Training.cpp:3319-3389) after oracle_predict:
+// property 0 ("energy"): out[c] = sum_a dE_a/dw_c (NeuralNetwork::calculateDEdc);
+// property 1 ("force"): out[(3 ia + k) * n_conn + c] = sum_a dF_{ia,k}/dw_c,
+// NeuralNetwork::calculateDFdc with dG_a/dx_{ia,k} from
+// Training::collectDGdxia (Training.cpp:3586-3625): the entries of atom a's
+// neighbours that are atom ia (self images included) plus, for a == ia, the
+// atom's own derivative.  Connections of all elements in element order.
+int oracle_dPdc(oracle_ctx* c, int property, double* out)
+{
+    ORACLE_TRY({
+        if (c->normalize) throw std::runtime_error("dPdc: normalised models are not supported");
+        if (property != 0 && property != 1) throw std::runtime_error("dPdc: property must be 0 or 1");
+        const size_t N = c->atoms.size();
+        for (auto const& a : c->atoms)
+            if (a.dEdG.empty() || a.dGdr.empty()) throw std::runtime_error("dPdc: run oracle_predict first");
+        vector<size_t> off(c->el.size() + 1, 0);
+        for (size_t e = 0; e < c->el.size(); ++e) off[e + 1] = off[e] + c->el[e].nn.numConnections();
+        const size_t nc = off.back();
+        const size_t rows = property == 0 ? 1 : 3 * N;
+        for (size_t q = 0; q < rows * nc; ++q) out[q] = 0.0;
+        vector<NNState> st(N);
+        for (size_t i = 0; i < N; ++i) nn_forward2(c->el[c->atoms[i].element].nn, c->atoms[i].G, st[i]);
+        if (property == 0) {
+            for (size_t i = 0; i < N; ++i) {
+                const NN& nn = c->el[c->atoms[i].element].nn;
+                const ConnOffsets o = conn_offsets(nn);
+                vector<double> tmp(o.n);
+                nn_dEdc(nn, st[i], o, tmp.data());
+                double* g = out + off[c->atoms[i].element];
+                for (size_t q = 0; q < o.n; ++q) g[q] += tmp[q];
+            }
+            return 0;
+        }
+        for (size_t ia = 0; ia < N; ++ia)
+            for (int k = 0; k < 3; ++k) {
+                double* row = out + (3 * ia + k) * nc;
+                for (size_t i = 0; i < N; ++i) {
+                    const Atom& a = c->atoms[i];
+                    const size_t ns = a.G.size();
+                    vector<double> dGdxia(ns, 0.0);
+                    for (const Neighbor& n : a.nbr) {
+                        if ((size_t)n.j != ia) continue;
+                        const vector<int>& t = c->el[a.element].table[n.element];
+                        for (size_t q = 0; q < t.size(); ++q) dGdxia[t[q]] += n.dGdr[3 * q + k];
+                    }
+                    if (i == ia)
+                        for (size_t q = 0; q < ns; ++q) dGdxia[q] += a.dGdr[3 * q + k];
+                    const NN& nn = c->el[a.element].nn;
+                    const ConnOffsets o = conn_offsets(nn);
+                    vector<double> tmp(o.n);
+                    nn_dFdc(nn, st[i], o, dGdxia.data(), tmp.data());
+                    double* g = row + off[a.element];
+                    for (size_t q = 0; q < o.n; ++q) g[q] += tmp[q];
+                }
+            }
+    });
+    return 0;
+}
+
 int oracle_loss_gradient(oracle_ctx* c, double E_ref, const double* F_ref, double wE, double wF, double* loss,
                          double* grad)
 {

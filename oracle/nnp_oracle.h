@@ -108,6 +108,10 @@ int oracle_set_weights(oracle_ctx* c, const double* w);
  * with respect to all connections (layout of oracle_get_weights). */
 int oracle_loss_gradient(oracle_ctx* c, double E_ref, const double* F_ref, double wE, double wF,
                          double* loss, double* grad);
+/* After oracle_predict (forces, pull form) of a non-normalised model:
+ * Training::dPdc rows.  property 0: out[n_conn] = dE/dw; property 1:
+ * out[3N][n_conn], row 3a+k = dF_ak/dw. */
+int oracle_dPdc(oracle_ctx* c, int property, double* out);
 
 #ifdef __cplusplus
 }

@@ -381,4 +381,52 @@ int ref_get_atomic_energies(const ref_ctx* c, double* e)
 }
 double ref_phase_time(const ref_ctx* c, int p) { return (p >= 0 && p < 5) ? c->tphase[p] : 0.0; }
 
+// Training::dPdc (Training.cpp:3319-3389) with the reference's
+// NeuralNetwork::calculateDEdc / calculateDFdc; the dG/dx_{ia,k} gather of
+// Training::collectDGdxia (Training.cpp:3586-3625, not part of the compiled
+// subset) is restated here.  After ref_predict of a non-normalised model.
+// property 0: out[n_conn]; property 1: out[3N][n_conn].
+int ref_dPdc(ref_ctx* c, int property, double* out)
+{
+    try {
+        if (c->normalize) throw std::runtime_error("dPdc: normalised models are not supported");
+        vector<size_t> off(c->elements.size() + 1, 0);
+        for (size_t e = 0; e < c->elements.size(); ++e)
+            off[e + 1] = off[e] + c->elements[e].neuralNetworks.at("short").getNumConnections();
+        const size_t nc = off.back(), N = c->atoms.size();
+        const size_t rows = property == 0 ? 1 : 3 * N;
+        for (size_t q = 0; q < rows * nc; ++q) out[q] = 0.0;
+        for (size_t r = 0; r < rows; ++r) {
+            const size_t ia = r / 3, k = r % 3;
+            for (auto& a : c->atoms) {
+                NeuralNetwork& nn = c->elements.at(a.element).neuralNetworks.at("short");
+                vector<double> tmp(nn.getNumConnections(), 0.0);
+                nn.setInput(a.G.data());
+                nn.propagate();
+                if (property == 0) {
+                    nn.calculateDEdc(tmp.data());
+                } else {
+                    vector<double> dGdxia(a.numSymmetryFunctions, 0.0);
+                    vector<vector<size_t>> const& tableFull = c->elements.at(a.element).getSymmetryFunctionTable();
+                    for (size_t i = 0; i < a.numNeighbors; ++i) {
+                        Atom::Neighbor const& n = a.neighbors[i];
+                        if (n.index != ia) continue;
+                        vector<size_t> const& table = tableFull.at(n.element);
+                        for (size_t j = 0; j < n.dGdr.size(); ++j) dGdxia[table.at(j)] += n.dGdr[j][k];
+                    }
+                    if (a.index == ia)
+                        for (size_t i = 0; i < a.numSymmetryFunctions; ++i) dGdxia[i] += a.dGdr[i][k];
+                    nn.calculateDEdG(a.dEdG.data());
+                    nn.calculateDFdc(tmp.data(), dGdxia.data());
+                }
+                for (size_t j = 0; j < tmp.size(); ++j) out[r * nc + off[a.element] + j] += tmp[j];
+            }
+        }
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -19,6 +19,9 @@ What it writes (data only -- numbers and the example's input files):
   the compact-function models (SF types 20 and 22, polynomial and asymmetric
   polynomial subtypes, normalised units; ``ExamplePrediction.h:1266-1525,
   1526-1550, 1551-1846``).
+* ``LJ_checkdw/`` -- the input files of ``examples/nnp-checkdw/LJ`` (101 Ne
+  dimers, radial functions, a 5-3-1 network), the example the reference's
+  nnp-checkdw tool test runs (``test/cpp/Example_nnp_checkdw.h:21-24``).
 * ``sf_kat.json`` -- the symmetry-function known answers (types 2, 3, 9, 12,
   13, 20-25) of
   ``test/cpp/fixtures.h:151-258`` with the cluster geometries of
@@ -70,12 +73,24 @@ def example(ref, name, count):
               open(os.path.join(out, "expected.json"), "w"), indent=0)
 
 
+def checkdw_example(ref, name):
+    """Copy an nnp-checkdw example's input files (no expected values: the
+    reference's tool test only checks that the output files exist)."""
+    ex = os.path.join(ref, "examples", "nnp-checkdw", name)
+    out = os.path.join(HERE, name + "_checkdw")
+    os.makedirs(out, exist_ok=True)
+    for fn in sorted(os.listdir(ex)):
+        if fn in ("input.nn", "scaling.data", "input.data") or re.match(r"weights\.\d{3}\.data$", fn):
+            shutil.copyfile(os.path.join(ex, fn), os.path.join(out, fn))
+
+
 def main(ref):
     example(ref, "H2O_RPBE-D3", 1080)
     example(ref, "Cu2S_PBE", 144)
     example(ref, "Anisole_SCAN", 256)
     example(ref, "DMABN_SCAN", 21)
     example(ref, "Ethylbenzene_SCAN", 288)
+    checkdw_example(ref, "LJ")
 
     fx = open(os.path.join(ref, "test", "cpp", "fixtures.h")).read()
     kat = {"source": "test/cpp/fixtures.h:11-258; test/cpp/test_SymmetryFunctions.cpp:36,68-74",

@@ -146,6 +146,16 @@ class Oracle:
         self._ck(L.oracle_loss_gradient(self.h, float(E_ref), _vp(F), float(wE), float(wF), _vp(loss), _vp(g)))
         return loss, g
 
+    def dPdc(self, prop):
+        """After predict(PULL): Training::dPdc rows, 'energy' -> [n_conn],
+        'force' -> [3N, n_conn]."""
+        L = self.lib()
+        L.oracle_dPdc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        n = self.num_connections()
+        out = np.zeros(n if prop == "energy" else (3 * len(self.positions()), n))
+        self._ck(L.oracle_dPdc(self.h, 0 if prop == "energy" else 1, _vp(out)))
+        return out
+
     def _ck(self, rc):
         if rc != 0:
             raise RuntimeError(self.lib().oracle_last_error().decode())
@@ -218,6 +228,14 @@ class RefLib:
 
     def phase_times(self):
         return [self.lib().ref_phase_time(self.h, p) for p in range(5)]
+
+    def dPdc(self, prop, n_conn):
+        """After predict(): Training::dPdc rows through the reference's
+        NeuralNetwork::calculateDEdc / calculateDFdc."""
+        out = np.zeros(n_conn if prop == "energy" else (3 * self.n, n_conn))
+        if self.lib().ref_dPdc(self.h, 0 if prop == "energy" else 1, _vp(out)):
+            raise RuntimeError(self.lib().ref_last_error().decode())
+        return out
 
 
 def _measured_model(directory, make):

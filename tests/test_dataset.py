@@ -14,6 +14,7 @@ its force bar.  The H2O_RPBE-D3 golden (normalised units, energies only)
 reproduces the golden energy."""
 import json
 import os
+import re
 import shutil
 import subprocess
 
@@ -222,3 +223,74 @@ def test_checkf_goldens(name, tmp_path):
     assert np.abs(f[:, 3] - exp).max() <= 1e-8
     assert err <= 10 * 1e-4 ** 2 and abs(s[0, 3] - err) <= 1e-12
     assert "WARNING" not in open(d / "nnp-checkf.log.0000").read()
+
+
+CHECKDW = os.path.join(ROOT, "n2p2_amd", "bin", "nnpx-checkdw")
+
+
+def test_checkdw_usage_without_arguments():
+    need(CHECKDW)
+    rc, so, _ = run([CHECKDW])
+    assert rc == 1 and "USAGE" in so
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("batch_atoms", [1000000, 5])
+def test_checkdw_lj_example(batch_atoms, tmp_path):
+    """nnpx-checkdw on the reference's nnp-checkdw LJ example (101 Ne dimers,
+    22 connections, use_short_forces): the output files of the reference's
+    tool test (test_nnp-checkdw.cpp:17-21), every property of every structure
+    OK (max |analytic - numeric| <= 100 delta^2), analytic rows equal to the
+    oracle's Training::dPdc rows, numeric rows to the oracle's central
+    differences; batch-atoms 5 splits the numeric batches and the seeded
+    copies."""
+    need(CHECKDW)
+    gdir = os.path.join(HERE, "golden", "LJ_checkdw")
+    d = tmp_path / "run"
+    shutil.copytree(gdir, d)
+    say("case: nnpx-checkdw on the LJ example, batch-atoms %d" % batch_atoms)
+    rc, so, se = run([CHECKDW, "0", "--batch-atoms", str(batch_atoms)], cwd=str(d))
+    assert rc == 0, se
+    for fn in ("nnp-checkdw.log.0000", "checkdw-summary.out", "checkdw-weights.energy.out",
+               "checkdw-weights.force.out"):
+        assert os.path.exists(d / fn), fn
+    we = read_table(d / "checkdw-weights.energy.out", 5)
+    wf = read_table(d / "checkdw-weights.force.out", 5)
+    summ = read_table(d / "checkdw-summary.out", 6)
+    assert we.shape == (101 * 22, 5) and wf.shape == (101 * 6 * 22, 5) and summ.shape == (101, 6)
+    log = open(d / "nnp-checkdw.log.0000").read()
+    assert len(re.findall(r"^Configuration .* OK .* OK", log, re.M)) == 101 and "WARN" not in log
+    errE, errF = np.abs(we[:, 3] - we[:, 4]).max(), np.abs(wf[:, 3] - wf[:, 4]).max()
+    say("  max |analytic - numeric|: energy %.2e  force %.2e" % (errE, errF))
+    assert max(errE, errF) <= 100 * 1e-4 ** 2
+    from n2p2_amd import read_structures
+    structs = read_structures(os.path.join(gdir, "input.data"), ["Ne"])
+    o = Oracle(str(d))
+    w0 = o.weights()
+    for s in (0, 50, 100):
+        pos, elem, box = structs[s]
+        o.set_weights(w0)
+        o.set_structure(pos, elem, box)
+        o.predict(PULL, BRUTE)
+        aE, aF = o.dPdc("energy"), o.dPdc("force")
+        rE = we[we[:, 0] == s + 1]
+        rF = wf[wf[:, 0] == s + 1]
+        assert (rE[:, 1] == 1).all() and (rE[:, 2] == np.arange(1, 23)).all()
+        scale = max(np.abs(aF).max(), np.abs(aE).max())
+        assert np.abs(rE[:, 3] - aE).max() <= 1e-10 * scale
+        assert np.abs(rF[:, 3] - aF.reshape(-1)).max() <= 1e-10 * scale
+        # numeric: w + delta, then (w + delta) - 2 delta, as Training::dPdcN
+        k = 7
+        res = []
+        for step in (1e-4, -2e-4):
+            w = w0.copy() if step > 0 else w
+            w[k] += step
+            o.set_weights(w)
+            o.set_structure(pos, elem, box)
+            o.predict(PULL, BRUTE)
+            res.append((o.energy(), o.forces().reshape(-1)))
+        nE = (res[0][0] - res[1][0]) / 2e-4
+        nF = (res[0][1] - res[1][1]) / 2e-4
+        assert abs(rE[k, 4] - nE) <= 1e-9 * max(abs(nE), 1.0)
+        assert np.abs(rF[k::22, 4] - nF).max() <= 1e-9 * max(np.abs(nF).max(), 1.0)
+    o.set_weights(w0)

@@ -109,3 +109,38 @@ def test_gradient_step_reduces_loss(models):
     m.set_weights(w)
     l2, _ = m.loss_gradient(E_ref, F_ref, BETA, n2p2_amd.PLAIN_FP64)
     assert abs(l2[0] - l0[0]) <= 1e-3 * l0[0]
+
+
+@pytest.mark.parametrize("precision", [n2p2_amd.PLAIN_FP64, n2p2_amd.MIXED])
+def test_weight_vjp_vs_oracle_dPdc(models, precision):
+    """nnpx_weight_vjp: the per-structure rows vE_s dE_s/dw + sum vF . dF/dw
+    of three structures in one call -- a 24-atom water box whose 6.21 A cell
+    is smaller than the cutoff (self images) and two 66-78 atom boxes --
+    against the oracle's Training::dPdc rows contracted with the same random
+    seeds."""
+    say("case: weight VJP, 3 structures (24 atoms self-image + 2 water boxes), precision", precision)
+    student, _ = models
+    p0, e0, b0 = synth.water_box(2)
+    p1, e1, b1, o1 = synth.water_set(n_struct=2, mmin=22, mmax=26, seed=synth.SEED_WATERSET + 2)
+    pos = np.vstack([p0, p1])
+    elem = np.concatenate([e0, e1]).astype(np.int32)
+    boxes = [b0] + list(b1)
+    off = np.concatenate([[0], len(e0) + np.asarray(o1)])
+    rng = np.random.default_rng(21)
+    vE = rng.normal(size=3)
+    vF = rng.normal(size=3 * len(elem))
+    m = n2p2_amd.Mode(student)
+    m.set_structures(pos, elem, boxes, off)
+    g = m.weight_vjp(vE, vF, precision)
+    assert g.shape == (3, m.num_connections())
+    worst = 0.0
+    for s in range(3):
+        a, b = off[s], off[s + 1]
+        o = Oracle(student)
+        o.set_structure(pos[a:b], elem[a:b], boxes[s])
+        o.predict(PULL, BRUTE)
+        ref = vE[s] * o.dPdc("energy") + vF[3 * a:3 * b] @ o.dPdc("force")
+        rel = float(np.abs(g[s] - ref).max() / np.abs(ref).max())
+        say("  structure %d (%d atoms): max |row| %.3e  max rel %.2e" % (s, b - a, np.abs(ref).max(), rel))
+        worst = max(worst, rel)
+    assert worst <= (1e-9 if precision == n2p2_amd.PLAIN_FP64 else 2e-3)

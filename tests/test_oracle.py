@@ -247,3 +247,76 @@ def test_loss_gradient_matches_finite_differences(tmp_path, part):
     o.set_weights(w0)
     scale = np.abs(g).max()
     assert worst <= 1e-6 * scale, (worst, scale)
+
+
+def _dPdc_case(case, tmp_path):
+    """(model dir, structures) of the weight-derivative pins: three dimers of
+    the nnp-checkdw LJ example, or a 24-atom water box whose 6.21 A cell is
+    smaller than the cutoff (self images in collectDGdxia)."""
+    from n2p2_amd import read_structures, synth
+    if case == "lj":
+        d = os.path.join(GOLD, "LJ_checkdw")
+        s = read_structures(os.path.join(d, "input.data"), ["Ne"])
+        return d, [s[0], s[50], s[100]]
+    d = synth.water_model(str(tmp_path / "water"))
+    return d, [synth.water_box(2)]
+
+
+@pytest.mark.parametrize("case", ["lj", "water24"])
+def test_dPdc_matches_finite_differences(case, tmp_path):
+    """Training::dPdc restated (oracle_dPdc): the energy row and every force
+    row against central differences of the oracle's energy and forces,
+    h = 1e-5 (Training::dPdcN, Training.cpp:3391-3471); every connection of
+    the LJ network, 40 connections over both water networks.  collectDGdxia
+    keeps the entries of an atom's own images, so in the self-image cell the
+    rows are derivatives of the push-form forces (InterfaceLammps::getForces
+    semantics), not of Mode::calculateForces, which skips them; the pull form
+    differs there by O(1e-4) and is not the function differentiated."""
+    d, structs = _dPdc_case(case, tmp_path)
+    o = Oracle(d)
+    w0 = o.weights()
+    rng = np.random.default_rng(11)
+    idx = np.arange(len(w0)) if len(w0) <= 64 else np.unique(np.concatenate(
+        [rng.choice(len(w0), 36, replace=False), [0, len(w0) // 2, len(w0) - 26, len(w0) - 1]]))
+    h = 1e-5
+    for pos, elem, box in structs:
+        o.set_weights(w0)
+        o.set_structure(pos, elem, box)
+        o.predict(PULL, BRUTE)
+        dE = o.dPdc("energy")
+        dF = o.dPdc("force")
+        assert dF.shape == (3 * len(elem), len(w0))
+        worstE = worstF = 0.0
+        for k in idx:
+            res = []
+            for sgn in (1.0, -1.0):
+                w = w0.copy()
+                w[k] += sgn * h
+                o.set_weights(w)
+                o.set_structure(pos, elem, box)
+                o.predict(PUSH, BRUTE)
+                res.append((o.energy(), o.forces().reshape(-1)))
+            worstE = max(worstE, abs((res[0][0] - res[1][0]) / (2 * h) - dE[k]))
+            worstF = max(worstF, float(np.abs((res[0][1] - res[1][1]) / (2 * h) - dF[:, k]).max()))
+        assert worstE <= 1e-7 * max(np.abs(dE).max(), 1.0), (worstE, np.abs(dE).max())
+        assert worstF <= 1e-7 * max(np.abs(dF).max(), 1.0), (worstF, np.abs(dF).max())
+    o.set_weights(w0)
+
+
+@pytest.mark.skipif(not ref_available(), reason="reference compute classes not built (no reference tree)")
+@pytest.mark.parametrize("case", ["lj", "water24"])
+def test_dPdc_equals_reference_classes(case, tmp_path):
+    """oracle_dPdc against Training::dPdc driven through the reference's own
+    NeuralNetwork::calculateDEdc / calculateDFdc and Atom/Element code."""
+    d, structs = _dPdc_case(case, tmp_path)
+    o = Oracle(d)
+    r = RefLib(d)
+    n = o.num_connections()
+    for pos, elem, box in structs:
+        o.set_structure(pos, elem, box)
+        r.set_structure(pos, elem, box)
+        o.predict(PULL, BRUTE)
+        r.predict()
+        for prop in ("energy", "force"):
+            a, b = o.dPdc(prop), r.dPdc(prop, n)
+            assert np.abs(a - b).max() <= 1e3 * EPS * max(np.abs(b).max(), 1.0), (prop, np.abs(a - b).max())
