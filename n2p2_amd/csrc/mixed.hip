@@ -13,6 +13,7 @@
 //            Mode.cpp:1922-1984) without storing dG/dr; FP64 accumulation
 //            in LDS per neighbour entry, pushed with FP64 atomics.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <type_traits>
 #include <cstring>
@@ -354,11 +355,86 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
     }
 }
 
-// Row-major variant: candidates are enumerated a-major (all partners of
-// la[a] consecutively), compacted the same way, and each lane takes a
-// contiguous chunk of the compacted buffer, so it sees long runs of one first
-// atom j: j's geometry stays in registers over the run and the force kernel
-// keeps j's contribution there too.  `flush` runs after each buffer.
+// Block variant (the SF-value kernels): each row a of list A pairs with its columns c -- for
+// one list b = (a + c + 1) mod nA with c < (nA - 1) / 2, and for even nA one
+// more column (b = a + nA / 2) on the rows a < nA / 2, so every unordered
+// pair appears once; for two lists b = c, c < nB.  A step tests a block
+// of rows x columns (lane = row * width + column, fixed per group), so a
+// candidate's indices cost a few adds.
+// Passing pairs are compacted through LDS in block order and each lane takes
+// a contiguous chunk of the buffer, so it sees runs of one first atom j: j's
+// geometry stays in registers over the run and the force kernel keeps j's
+// contribution there too.  `flush` runs after each buffer.
+template <typename Work, typename Flush = NoFlush>
+__device__ inline void for_each_triplet_blocks(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work,
+                                               Flush&& flush = Flush{})
+{
+    const int lane = threadIdx.x;
+    const int R = same ? (nA - 1) / 2 : nB;          // regular columns per row
+    const int Rx = same && nA % 2 == 0 ? R + 1 : R;  // with the half-way column (rows a < nA / 2 only)
+    const int half = nA / 2;
+    if (nA == 0 || Rx == 0) return;
+    // a row's columns split into nearly equal blocks of at most 16, so
+    // the passing partners of one row stay long runs in the buffer
+    const int nblk = (Rx + 15) >> 4;
+    const int cw = (Rx + nblk - 1) / nblk, rh = kWave / cw;
+    const int lr = lane / cw, lc = lane - lr * cw;  // lanes with lr >= rh stay idle
+    const unsigned short* lb = same ? S.la : S.lb;
+    int a0 = 0, c0 = 0;
+    // the lane's row atom and its geometry, reloaded when the block moves down
+    int ja = S.la[lr < min(nA, rh) ? lr : 0];
+    float4 ga = S.g[ja];
+    while (a0 < nA) {
+        int cnt = 0;
+        while (a0 < nA && cnt <= S.trip - kWave) {
+            const int a = a0 + lr, c = c0 + lc;
+            bool ok = false;
+            unsigned jk = 0;
+            float r2 = 0.f;
+            if (lr < rh && a < nA && c < Rx && (c < R || a < half)) {
+                int b = same ? a + c + 1 : c;
+                if (same && b >= nA) b -= nA;
+                const int k = lb[b];
+                const float4 gk = S.g[k];
+                const float dx = gk.x - ga.x, dy = gk.y - ga.y, dz = gk.z - ga.z;
+                r2 = dx * dx + dy * dy + dz * dz;
+                ok = r2 < rc2;
+                jk = (unsigned)ja | ((unsigned)k << 16);
+            }
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                S.tjk[pos] = jk;
+                S.tr2[pos] = r2;
+            }
+            cnt += __popcll(m);
+            c0 += cw;
+            if (c0 >= Rx) {
+                c0 = 0;
+                a0 += rh;
+                const int an = a0 + lr;
+                ja = S.la[an < nA ? an : 0];
+                ga = S.g[ja];
+            }
+        }
+        __syncthreads();
+        const int per = (cnt + kWave - 1) / kWave;
+        const int x0 = lane * per, x1 = min(cnt, x0 + per);
+        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        flush();
+        __syncthreads();
+    }
+}
+
+// Row-major variant (the force kernel): candidates are enumerated a-major
+// from a linear index (all partners of la[a] consecutively), compacted the
+// same way, and each lane takes a contiguous chunk of the compacted buffer,
+// so it sees long runs of one first atom j: j's geometry stays in registers
+// over the run and the force kernel keeps j's contribution there too.  Its
+// partner atoms k differ more across the lanes of one step than in the block
+// variant, which keeps the force kernel's LDS atomics on k from colliding.
+// `flush` runs after each buffer.
 template <typename Work, typename Flush = NoFlush>
 __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work,
                                              Flush&& flush = Flush{})
@@ -415,7 +491,8 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
             for (int f = 0; f < kFill; ++f) {
                 const unsigned long long m = __ballot(ok[f]);
                 if (ok[f]) {
-                    const int pos = cnt + __popcll(m & __lanemask_lt());
+                    const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                     S.tjk[pos] = jk[f];
                     S.tr2[pos] = r2[f];
                 }
@@ -432,6 +509,113 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
     }
 }
 
+// ------------------------------------------------------------ power path --
+// Angular groups whose members are exp(-eta_h sum r^2) (1 + lambda_s cos)^zeta
+// with zeta = 2^b (b <= 4), at most two distinct eta and two lambda, rs = 0:
+// per triplet one exponential per eta and the powers p, p^2, p^4, ... of the
+// two bases by repeated squaring (the reference's integer-zeta pow_int,
+// utility.h, also multiplies) stand in for one exp2/log2 per member.  Terms
+// are indexed beta = s NB0 + b (first eta) and 2 NB0 + s NB1 + b (second).
+
+// SF values of one power-path group: FP32 partial sums per term over a
+// triplet buffer folded into compensated sums, one FP64 wave reduction per
+// used term, then each member takes its term (times 2^(1-zeta)) and is scaled.
+template <int NB0, int NB1>
+__device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stage& S,
+                                        const MemberF* __restrict__ mf, const GroupMixF& gm, const GroupT& GT,
+                                        const GroupF& g, int n, int ctype, float tanhPre, int sf0,
+                                        double* __restrict__ Gi, float* __restrict__ Ii)
+{
+    constexpr int NBAS = 2 * (NB0 + NB1);
+    constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
+    const int lane = threadIdx.x;
+    const bool same = GT.e1 == GT.e2;
+    const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
+    const int nA = make_list(S, n, GT.e1, g.rc, S.la);
+    const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+    float acc[NBAS], cmp[NBAS], part[NBAS];
+#pragma unroll
+    for (int b = 0; b < NBAS; ++b) { acc[b] = 0.f; cmp[b] = 0.f; part[b] = 0.f; }
+    const float ce0 = -gm.etaL[0], ce1 = -gm.etaL[1];
+    const float lam0 = gm.lam[0], lam1 = gm.lam[1];
+    int jc = -1;  // j of the cached geometry (runs of one j per lane)
+    float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
+    for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
+        if (j != jc) {
+            gj = S.g[j];
+            cj = S.c[j];
+            jc = j;
+        }
+        const float4 gk = S.g[k];
+        const float4 ck = S.c[k];
+        float fcjk = 1.f, dfcjk = 0.f;
+        if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, __builtin_amdgcn_sqrtf(r2jk), fcjk, dfcjk);
+        const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
+        const float pfc = cj.y * ck.y * fcjk;
+        const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
+        const float E0 = fexp2(ce0 * r2sum) * pfc;
+        const float E1 = NB1 > 0 ? fexp2(ce1 * r2sum) * pfc : 0.f;
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+            const float pl = fmaf(sl ? lam1 : lam0, cost, 1.f);
+            float p = pl > 0.f ? pl : 0.f;  // the reference's zero term for 1 + lambda cos <= 0
+#pragma unroll
+            for (int b = 0; b < NBM; ++b) {
+                if (b < NB0) part[sl * NB0 + b] = fmaf(E0, p, part[sl * NB0 + b]);
+                if (b < NB1) part[2 * NB0 + sl * NB1 + b] = fmaf(E1, p, part[2 * NB0 + sl * NB1 + b]);
+                p *= p;
+            }
+        }
+    }, [&]() {
+#pragma unroll
+        for (int b = 0; b < NBAS; ++b) {
+            kahan(acc[b], cmp[b], part[b]);
+            part[b] = 0.f;
+        }
+    });
+    double* red = reinterpret_cast<double*>(S.mp);
+#pragma unroll
+    for (int b = 0; b < NBAS; ++b)
+        if ((gm.used >> b) & 1) {
+            const double v = wave_sum((double)acc[b] + (double)cmp[b]);
+            if (lane == 0) red[b] = v;
+        }
+    __syncthreads();
+    if (lane < GT.nmem) {
+        const MemberT& mt = M->mem[GT.mem0 + lane];
+        const double gs = scale_sf(M->scale[sf0 + mt.sf], red[mf[lane].beta] * mt.gnorm);
+        Gi[mt.sf] = gs;
+        Ii[mt.sf] = (float)gs;
+    }
+    __syncthreads();
+}
+
+// Force weights of one power-path group: wb[beta] = sum of the members'
+// dE/dG weights w on term beta, wa[beta] = sum of zeta lambda w.
+template <int NBAS>
+__device__ inline void power_weights(const Stage& S, const MemberF* __restrict__ mf, int nmem,
+                                     const float* __restrict__ w, float* wb, float* wa)
+{
+    const int lane = threadIdx.x;
+    if (lane < NBAS) {
+        float sb = 0.f, sa = 0.f;
+        for (int q = 0; q < nmem; ++q)
+            if (mf[q].beta == lane) {
+                const float wq = w[mf[q].sf];
+                sb += wq;
+                sa += wq * mf[q].zetaLambda;
+            }
+        S.mp[lane] = make_float4(sb, sa, 0.f, 0.f);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < NBAS; ++b) {
+        const float4 v = S.mp[b];
+        wb[b] = v.x;
+        wa[b] = v.y;
+    }
+}
+
 // ------------------------------------------------------------- SF values --
 // Angular terms: vexp * pfc * pl^zeta evaluated as one exp2 of
 // zeta * log2(pl) + log2(pfc) - eta log2(e) * sum(r - rs)^2, with log2(pl)
@@ -440,7 +624,7 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
 // FAST: every angular group has exactly MAXM members, at most two distinct
 // lambda and rs = 0 -- the member loop is then branch-free.  Otherwise the
 // general loop handles shifted Gaussians and up to four lambda values.
-template <int MAXM, bool FAST>
+template <int MAXM, bool FAST, int NB0, int NB1>
 __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                              const GroupMixF* __restrict__ GM, SysView s, double* __restrict__ G,
                                              float* __restrict__ Gin, int ldG, int ldK, int capL, int trip)
@@ -500,12 +684,16 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             __syncthreads();
             continue;
         }
+        if constexpr (NB0 > 0) {
+            sf32_power_group<NB0, NB1>(M, S, mf, GM[E.grp0 + gi], GT, g, n, ctype, tanhPre, E.sf0, Gi, Ii);
+            continue;
+        }
         float acc[MAXM], cmp[MAXM];
 #pragma unroll
         for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
         {
             const bool same = GT.e1 == GT.e2;
-        const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
+            const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
             const GroupMixF gm = GM[E.grp0 + gi];
@@ -525,7 +713,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             }
             int jc = -1;  // j of the cached geometry (runs of one j per lane)
             float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-            for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
+            for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
                 if (j != jc) {
                     gj = S.g[j];
                     cj = S.c[j];
@@ -776,7 +964,7 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
 // (FP64) and pushed to the force array with FP64 atomics; contributions to
 // the atom's own periodic images are dropped in the pull form
 // (Mode::calculateForces skips them, Mode.cpp:1957) and kept in the push form.
-template <int MAXM, bool FAST>
+template <int MAXM, bool FAST, int NB0, int NB1>
 __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                                 const GroupMixF* __restrict__ GM, SysView s,
                                                 const float* __restrict__ fw, int ldG, double* __restrict__ F,
@@ -835,13 +1023,19 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const int nA = make_list(S, n, GT.e1, g.rc, S.la);
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
-        const MemberMasks R = load_members(S, mf, nmem, wi);
+        constexpr bool POW = NB0 > 0;
+        constexpr int NBAS = POW ? 2 * (NB0 + NB1) : 1;
+        constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
+        float pwb[NBAS], pwa[NBAS];
+        MemberMasks R{0u, 0u, 0u};
+        if constexpr (POW) power_weights<NBAS>(S, mf, nmem, wi, pwb, pwa);
+        else R = load_members(S, mf, nmem, wi);
         // FAST: the member constants stay in registers for the whole group
         // (zeta, -eta log2 e, the weights of B and C, and the weight of A
         // split by lambda index so A needs no per-member select)
-        constexpr int NC = FAST ? MAXM : 1;
+        constexpr int NC = FAST && !POW ? MAXM : 1;
         float cz[NC], ce[NC], cA0[NC], cA1[NC], cB[NC], cC[NC];
-        if constexpr (FAST) {
+        if constexpr (FAST && !POW) {
 #pragma unroll
             for (int q = 0; q < MAXM; ++q) {
                 const float4 mq = S.mp[q];
@@ -908,7 +1102,35 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             // t_q = vexp * pl^zeta: u_q * pl = w t_q and u_q = w t_q / pl,
             // u_q = w vexp pl^(zeta-1) being the reference's fg times w
             float A = 0.f, B = 0.f, C0 = 0.f, C1 = 0.f, C2 = 0.f, C3 = 0.f;
-            if constexpr (FAST) {
+            if constexpr (POW) {
+                // B_h = E_h sum wb P, A_s = sum_h E_h sum wa P (powers of lambda s)
+                const float E0 = fexp2(-gm.etaL[0] * r2sum);
+                const float E1 = NB1 > 0 ? fexp2(-gm.etaL[1] * r2sum) : 0.f;
+                float Q0 = 0.f, Q1 = 0.f, As[2];
+#pragma unroll
+                for (int sl = 0; sl < 2; ++sl) {
+                    const float pl = fmaf(gm.lam[sl], cost, 1.f);
+                    float p = pl > 0.f ? pl : 0.f;
+                    float qa0 = 0.f, qa1 = 0.f;
+#pragma unroll
+                    for (int b = 0; b < NBM; ++b) {
+                        if (b < NB0) {
+                            Q0 = fmaf(pwb[sl * NB0 + b], p, Q0);
+                            qa0 = fmaf(pwa[sl * NB0 + b], p, qa0);
+                        }
+                        if (b < NB1) {
+                            Q1 = fmaf(pwb[2 * NB0 + sl * NB1 + b], p, Q1);
+                            qa1 = fmaf(pwa[2 * NB0 + sl * NB1 + b], p, qa1);
+                        }
+                        p *= p;
+                    }
+                    As[sl] = E0 * qa0 + E1 * qa1;
+                }
+                const float B0 = E0 * Q0, B1 = E1 * Q1;
+                B = B0 + B1;
+                C0 = gm.twoEta[0] * B0 + gm.twoEta[1] * B1;
+                A = As[0] * Rp[0] + As[1] * Rp[1];
+            } else if constexpr (FAST) {
                 float A0 = 0.f, A1 = 0.f;
 #pragma unroll
                 for (int q = 0; q < MAXM; ++q) {
@@ -1220,6 +1442,44 @@ void dispatch_members(int exact, int maxm, F&& f)
     else f(integral_constant<int, 32>{}, std::false_type{});
 }
 
+// Power-path shapes (NB0, NB1) with kernels built for them: terms per
+// lambda of the first and second eta.
+constexpr int kPowShapes[][2] = {{3, 0}, {3, 3}, {5, 0}, {5, 1}, {5, 5}};
+constexpr int kNumPowShapes = sizeof(kPowShapes) / sizeof(kPowShapes[0]);
+
+// b with zeta == 2^b, 0 <= b <= 4; -1 otherwise.
+int pow2_exponent(double zeta)
+{
+    for (int b = 0; b <= 4; ++b)
+        if (zeta == (double)(1 << b)) return b;
+    return -1;
+}
+
+// Calls f(MAXM, FAST, NB0, NB1) for the SF/force kernel instance of the
+// model: the power path when it applies, else as dispatch_members.
+template <typename F>
+void dispatch_angular(const MixedModel& mm, int maxm, F&& f)
+{
+    using std::integral_constant;
+    using I0 = integral_constant<int, 0>;
+    using I1 = integral_constant<int, 1>;
+    using I3 = integral_constant<int, 3>;
+    using I4 = integral_constant<int, 4>;
+    using I5 = integral_constant<int, 5>;
+    if (mm.powNB[0] > 0) {
+        const std::true_type fast{};
+        switch (mm.powNB[0] * 10 + mm.powNB[1]) {
+        case 30: f(I4{}, fast, I3{}, I0{}); return;
+        case 33: f(I4{}, fast, I3{}, I3{}); return;
+        case 50: f(I4{}, fast, I5{}, I0{}); return;
+        case 51: f(I4{}, fast, I5{}, I1{}); return;
+        case 55: f(I4{}, fast, I5{}, I5{}); return;
+        default: throw Error{NNPX_ERR_STATE, "no kernel built for this power-path shape"};
+        }
+    }
+    dispatch_members(mm.fastM, maxm, [&](auto m, auto fs) { f(m, fs, I0{}, I0{}); });
+}
+
 int pow2_members(const ModelT& t)
 {
     const int m = std::max(std::max(t.max_mem_ang, t.max_mem_rad), 1);
@@ -1257,6 +1517,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 f.sf = M.sf;
                 f.calcExp = M.calcExp;
                 f.sidx = 0;
+                f.beta = -1;
                 if (G.type == 3 || G.type == 9) {
                     if (M.rs > 0.0) mm.hasRs = true;
                     int k = 0;
@@ -1286,6 +1547,71 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 same = same && t.grp[g].nmem == cnt;
             }
         if (same && (cnt == 4 || cnt == 8 || cnt == 10 || cnt == 16)) mm.fastM = cnt;
+    }
+    // power path: every angular group has zeta in {1, 2, 4, 8, 16}, at most
+    // two distinct eta and two lambda, rs = 0
+    mm.powNB[0] = mm.powNB[1] = 0;
+    if (mm.maxS == 2 && !mm.hasRs) {
+        bool ok = true;
+        int need[2] = {0, 0};
+        std::vector<std::array<double, 2>> heta(std::max(t.ngroups, 1));
+        std::vector<std::array<int, 2>> hnb(std::max(t.ngroups, 1));
+        for (int g = 0; g < t.ngroups && ok; ++g) {
+            const GroupT& G = t.grp[g];
+            if (G.type != 3 && G.type != 9) continue;
+            int nh = 0;
+            int nb[2] = {0, 0};
+            double eta[2] = {0.0, 0.0};
+            for (int q = G.mem0; q < G.mem0 + G.nmem && ok; ++q) {
+                const MemberT& M = t.mem[q];
+                int h = 0;
+                while (h < nh && eta[h] != M.eta) ++h;
+                if (h == nh) {
+                    if (nh == 2) { ok = false; break; }
+                    eta[nh++] = M.eta;
+                }
+                const int b = pow2_exponent(M.zeta);
+                if (b < 0) { ok = false; break; }
+                nb[h] = std::max(nb[h], b + 1);
+            }
+            if (!ok) break;
+            // the eta with the higher powers takes the larger block
+            if (nb[1] > nb[0]) {
+                std::swap(nb[0], nb[1]);
+                std::swap(eta[0], eta[1]);
+            }
+            heta[g] = {eta[0], eta[1]};
+            hnb[g] = {nb[0], nb[1]};
+            need[0] = std::max(need[0], nb[0]);
+            need[1] = std::max(need[1], nb[1]);
+        }
+        int best = -1;
+        for (int k = 0; k < kNumPowShapes; ++k)
+            if (kPowShapes[k][0] >= need[0] && kPowShapes[k][1] >= need[1] &&
+                (best < 0 || kPowShapes[k][0] + kPowShapes[k][1] < kPowShapes[best][0] + kPowShapes[best][1]))
+                best = k;
+        if (ok && need[0] > 0 && best >= 0) {
+            const int NB0 = kPowShapes[best][0], NB1 = kPowShapes[best][1];
+            mm.powNB[0] = NB0;
+            mm.powNB[1] = NB1;
+            for (int g = 0; g < t.ngroups; ++g) {
+                const GroupT& G = t.grp[g];
+                if (G.type != 3 && G.type != 9) continue;
+                GroupMixF& gm = gmix[g];
+                for (int h = 0; h < 2; ++h) {
+                    gm.etaL[h] = (float)(heta[g][h] * 1.4426950408889634);
+                    gm.twoEta[h] = (float)(2.0 * heta[g][h]);
+                }
+                gm.used = 0;
+                for (int q = G.mem0; q < G.mem0 + G.nmem; ++q) {
+                    const MemberT& M = t.mem[q];
+                    const int h = M.eta == heta[g][0] ? 0 : 1;
+                    const int b = pow2_exponent(M.zeta);
+                    mf[q].beta = h == 0 ? mf[q].sidx * NB0 + b : 2 * NB0 + mf[q].sidx * NB1 + b;
+                    gm.used |= 1 << mf[q].beta;
+                }
+            }
+        }
     }
     // bf16 weights, padded to 16: W [out][in] and WT [in][out], hi then lo.
     std::vector<uint16_t> w;
@@ -1371,8 +1697,9 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     if (lds_f > 65536 || lds_sf > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     timed("sf32", [&] {
-        dispatch_members(mm.fastM, P, [&](auto m, auto fast) {
-            hipLaunchKernelGGL((k_sf32<decltype(m)::value, decltype(fast)::value>),
+        dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
+            hipLaunchKernelGGL((k_sf32<decltype(m)::value, decltype(fast)::value, decltype(nb0)::value,
+                                       decltype(nb1)::value>),
                                dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, mm.d_gmix, s, b.G,
                                b.Gin, b.ldG, mm.ldK, capL, tsf);
         });
@@ -1414,8 +1741,9 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
         hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0, stream,
                            b.F, n3);
         const int ps = push ? 1 : 0;
-        dispatch_members(mm.fastM, P, [&](auto m, auto fast) {
-            hipLaunchKernelGGL((k_force32<decltype(m)::value, decltype(fast)::value>),
+        dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
+            hipLaunchKernelGGL((k_force32<decltype(m)::value, decltype(fast)::value, decltype(nb0)::value,
+                                          decltype(nb1)::value>),
                                dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, mm.d_gmix, s, b.dEdGf,
                                b.ldG, b.F, ps, capL, tf);
         });

@@ -28,14 +28,22 @@ struct MemberF {
     float twoEta;      // 2 eta
     float zetaLambda;  // zeta * lambda
     float etaL;        // eta * log2(e): exponent factor for exp2
-    int sf, calcExp, sidx, pad;  // sidx: index of lambda in the group's lambda table
+    int sf, calcExp, sidx;  // sidx: index of lambda in the group's lambda table
+    int beta;               // power path: index of the member's (eta, lambda, zeta) term, else -1
 };
 
 // Per-group table of the distinct lambda values (members index it by sidx).
+// Power path (angular groups whose zeta are powers of two up to 16, with at
+// most two distinct eta and two lambda): the group's terms are
+// exp(-eta_h sum r^2) * (1 + lambda_s cos)^(2^b), indexed
+// beta = s * NB0 + b for h = 0 and 2 NB0 + s * NB1 + b for h = 1.
 struct GroupMixF {
     float lam[4];
     int nlam;
-    int pad[3];
+    int used;          // power path: bit beta set for every term a member uses
+    float etaL[2];     // power path: eta_h * log2(e)
+    float twoEta[2];   // power path: 2 eta_h
+    int pad[2];
 };
 
 // Launch shapes of the mixed-form kernels, set by the autotuner
@@ -56,6 +64,7 @@ struct MixedModel {
     int maxS = 2;               // max distinct lambda values per angular group (2 or 4)
     bool hasRs = false;         // some angular member has rs > 0
     int fastM = 0;              // exact angular member count of the branch-free path (0: general path)
+    int powNB[2] = {0, 0};      // power path shape (NB0, NB1); {0, 0}: not used
     float* d_sffac = nullptr;   // [nelem*kMaxSF] force weight factor per SF
     uint16_t* d_w = nullptr;    // bf16 weights: per element/layer W hi, W lo, WT hi, WT lo
     float* d_b = nullptr;       // FP32 biases, padded
