@@ -121,20 +121,32 @@ __global__ __launch_bounds__(kNNThreads) void k_nn64(const ModelT* __restrict__ 
 }
 
 // One block per structure, fixed-order tree sum.
-__global__ __launch_bounds__(256) void k_energy(const double* atomE, const int* off, double* E)
+// Per-structure sum of the atomic energies, one block per structure: four
+// independent strided partial sums per thread keep several loads in flight,
+// then a fixed-order tree in LDS (deterministic from run to run).
+constexpr int kEnergyThreads = 1024;
+__global__ __launch_bounds__(kEnergyThreads) void k_energy(const double* atomE, const int* off, double* E)
 {
-    __shared__ double red[256];
+    __shared__ double red[kEnergyThreads];
     const int st = blockIdx.x;
     const int b = off[st], e = off[st + 1];
-    double s = 0.0;
-    for (int q = b + (int)threadIdx.x; q < e; q += 256) s += atomE[q];
-    red[threadIdx.x] = s;
+    const int t = (int)threadIdx.x;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int q = b + t;
+    for (; q + 3 * kEnergyThreads < e; q += 4 * kEnergyThreads) {
+        s0 += atomE[q];
+        s1 += atomE[q + kEnergyThreads];
+        s2 += atomE[q + 2 * kEnergyThreads];
+        s3 += atomE[q + 3 * kEnergyThreads];
+    }
+    for (; q < e; q += kEnergyThreads) s0 += atomE[q];
+    red[t] = (s0 + s1) + (s2 + s3);
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    for (int o = kEnergyThreads / 2; o > 0; o >>= 1) {
+        if (t < o) red[t] += red[t + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) E[st] = red[0];
+    if (t == 0) E[st] = red[0];
 }
 
 }  // namespace
@@ -167,7 +179,7 @@ void launch_energy(const double* atomE, const int* atom_off, int nstruct, double
                    hipStream_t stream)
 {
     if (nstruct == 0) return;
-    hipLaunchKernelGGL(k_energy, dim3(nstruct), dim3(256), 0, stream, atomE, atom_off, E);
+    hipLaunchKernelGGL(k_energy, dim3(nstruct), dim3(kEnergyThreads), 0, stream, atomE, atom_off, E);
 }
 
 }  // namespace nnpx
