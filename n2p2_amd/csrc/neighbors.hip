@@ -19,6 +19,15 @@ namespace {
 
 constexpr int kNbMaxLds = 1024;  // max entries sorted in LDS per atom
 constexpr int kCandLds = 2048;   // candidates whose cell is tabulated in LDS
+
+// Sort buffer of the cell-list builder: the power of two at or above the
+// list capacity (the builder's LDS, and so the waves per CU, follow it).
+inline int sort_slots(int cap)
+{
+    int np = 64;
+    while (np < cap && np < kNbMaxLds) np <<= 1;
+    return np;
+}
 constexpr int kCandUnroll = 4;   // candidates per lane and step
 
 __device__ inline bool gated_off(const SysView& s) { return s.gate && *s.gate == 0; }
@@ -94,11 +103,12 @@ __device__ inline void emit(const SysView& s, int i, int slot, int j, int img)
     }
 }
 
-// One wave per atom.
-__global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
+// One wave per atom.  Dynamic LDS (cell-list structures only): the sort
+// buffer of `slots` keys, then the candidate-cell table.
+__global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
 {
     if (gated_off(s)) return;
-    __shared__ long long keys[kNbMaxLds];
+    extern __shared__ long long keys[];
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const StructT& S = s.st[s.sid[i]];
@@ -160,7 +170,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
         // The 27 surrounding cells are flattened into one candidate range so
         // the wave tests 64 candidates per step with contiguous loads.
         __shared__ int cs[27], co[28], cim[27];
-        __shared__ unsigned char ccell[kCandLds];
+        unsigned char* ccell = reinterpret_cast<unsigned char*>(keys + slots);
         const int c = s.atom_cell[i] - S.cell0;
         const int c0 = c / (S.nc[1] * S.nc[2]), c1 = (c / S.nc[2]) % S.nc[1], c2 = c % S.nc[2];
         int start = 0, len = 0, img = 0;
@@ -237,13 +247,13 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
 #pragma unroll
             for (int u = 0; u < kCandUnroll; ++u)
                 if (hit[u]) {
-                    if (slot < kNbMaxLds) keys[slot] = key[u];
+                    if (slot < slots) keys[slot] = key[u];
                     ++slot;
                 }
             count += __shfl(inc, kWave - 1, kWave);
         }
         // Bitonic sort of the collected keys (padded to a power of two).
-        const int n = min(count, kNbMaxLds);
+        const int n = min(count, slots);
         int np = 1;
         while (np < n) np <<= 1;
         for (int q = n + lane; q < np; q += kWave) keys[q] = 0x7fffffffffffffffLL;
@@ -262,7 +272,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2)
             }
         for (int q = lane; q < n; q += kWave)
             emit(s, i, q, (int)(keys[q] >> 21), (int)(keys[q] & ((1 << 21) - 1)));
-        if (count > kNbMaxLds && lane == 0) atomicOr(&s.flags[0], 1);
+        if (count > slots && lane == 0) atomicOr(&s.flags[0], 1);
     }
     if (lane == 0) {
         s.nb_n[i] = min(count, s.cap);
@@ -327,7 +337,9 @@ void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t st
         hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, s);
         hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(256), 0, stream, s);
     }
-    hipLaunchKernelGGL(k_build, dim3(s.natoms), dim3(kWave), 0, stream, s, rc * rc);
+    const int slots = sort_slots(s.cap);
+    const size_t lds = any_cell ? sizeof(long long) * (size_t)slots + kCandLds : 0;
+    hipLaunchKernelGGL(k_build, dim3(s.natoms), dim3(kWave), lds, stream, s, rc * rc, slots);
 }
 
 void launch_reverse(const SysView& s, hipStream_t stream)
