@@ -19,6 +19,7 @@ struct SysView {
     const StructT* st;     // [nstruct]
     // neighbour list, fixed capacity per atom, entries sorted by (j, image)
     int cap;
+    int stage_cap;         // LDS staging slots of the mixed kernels (<= cap)
     int* nb_n;             // [N]
     int* nb_j;             // [N*cap]
     int* nb_img;           // [N*cap]

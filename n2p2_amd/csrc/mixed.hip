@@ -130,31 +130,44 @@ struct Stage {
     float4* c;       // 1/|dr|, fc, dfc (cutoff of the current group), unused
     unsigned char* el;  // neighbour element
     unsigned short* la; unsigned short* lb;  // group lists
-    unsigned* tjk;   // compacted pair (j | k << 16)
-    float* tr2;      // its r_jk^2
-    float4* mp;      // [2*kMaxMembersExp] member constants of the current group
+    unsigned* tjk;   // compacted pair (j | k << 16); r_jk^2 is recomputed from g
+    float4* mp;      // [mpn] per-group constants (member constants or power-path terms)
     int trip;        // capacity of the triplet buffer
 };
 
-__device__ inline Stage carve(float* base, int capL, int trip)
+// float4 slots of the per-group constant area: member constants (two per
+// member) on the member paths, one per term on the power path.
+__host__ __device__ constexpr int stage_mp(int nb0, int nb1)
+{
+    return nb0 > 0 ? 2 * (nb0 + nb1) : 2 * kMaxMembersExp;
+}
+
+__device__ inline Stage carve(float* base, int capL, int trip, int mpn)
 {
     Stage S;
     S.trip = trip;
     S.g = reinterpret_cast<float4*>(base);
     S.c = S.g + capL;
     S.tjk = reinterpret_cast<unsigned*>(S.c + capL);
-    S.tr2 = reinterpret_cast<float*>(S.tjk + trip);
-    S.mp = reinterpret_cast<float4*>(S.tr2 + trip);
-    S.la = reinterpret_cast<unsigned short*>(S.mp + 2 * kMaxMembersExp);
+    S.mp = reinterpret_cast<float4*>(S.tjk + trip);
+    S.la = reinterpret_cast<unsigned short*>(S.mp + mpn);
     S.lb = S.la + capL;
     S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
     return S;
 }
-// bytes of the staging area (capL is a multiple of 32, so the byte arrays end
-// 8-byte aligned)
-__host__ __device__ constexpr size_t stage_bytes(int capL, int trip)
+// bytes of the staging area (capL is a multiple of 32 and trip of 64, so the
+// byte arrays end 8-byte aligned)
+__host__ __device__ constexpr size_t stage_bytes(int capL, int trip, int mpn)
 {
-    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembersExp * 16;
+    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)mpn * 16;
+}
+
+// |b - a|^2 of two staged positions, one fixed operation order for the
+// candidate test and the terms
+__device__ inline float dist2(const float4& a, const float4& b)
+{
+    const float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
 __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
@@ -162,6 +175,7 @@ __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage&
     const int lane = threadIdx.x;
     const StructT& St = s.st[s.sid[i]];
     const int n = min(s.nb_n[i], capL);
+    if (s.nb_n[i] > capL && lane == 0) atomicOr(&s.flags[0], 4);  // the host repeats at full capacity
     const size_t base = (size_t)i * s.cap;
     for (int k = lane; k < n; k += kWave) {
         const int j = s.nb_j[base + k];
@@ -328,9 +342,7 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
                     else { a = u; b = u + nA / 2; }
                     int j = S.la[a], k = lb[b];
                     if (j > k) { const int x = j; j = k; k = x; }
-                    const float4 gj = S.g[j], gk = S.g[k];
-                    const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
-                    r2[f] = dx * dx + dy * dy + dz * dz;
+                    r2[f] = dist2(S.g[j], S.g[k]);
                     ok[f] = r2[f] < rc2;
                     jk[f] = (unsigned)j | ((unsigned)k << 16);
                 }
@@ -341,7 +353,6 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
                 if (ok[f]) {
                     const int pos = cnt + __popcll(m & __lanemask_lt());
                     S.tjk[pos] = jk[f];
-                    S.tr2[pos] = r2[f];
                 }
                 cnt += __popcll(m);
             }
@@ -349,7 +360,7 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
         }
         __syncthreads();
         // ... then run the member loops with full waves
-        for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        for (int x = lane; x < cnt; x += kWave) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16);
         flush();
         __syncthreads();
     }
@@ -395,9 +406,7 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, int nA, int nB, b
                 int b = same ? a + c + 1 : c;
                 if (same && b >= nA) b -= nA;
                 const int k = lb[b];
-                const float4 gk = S.g[k];
-                const float dx = gk.x - ga.x, dy = gk.y - ga.y, dz = gk.z - ga.z;
-                r2 = dx * dx + dy * dy + dz * dz;
+                r2 = dist2(ga, S.g[k]);
                 ok = r2 < rc2;
                 jk = (unsigned)ja | ((unsigned)k << 16);
             }
@@ -406,7 +415,6 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, int nA, int nB, b
                 const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                 S.tjk[pos] = jk;
-                S.tr2[pos] = r2;
             }
             cnt += __popcll(m);
             c0 += cw;
@@ -421,7 +429,7 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, int nA, int nB, b
         __syncthreads();
         const int per = (cnt + kWave - 1) / kWave;
         const int x0 = lane * per, x1 = min(cnt, x0 + per);
-        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16);
         flush();
         __syncthreads();
     }
@@ -480,9 +488,7 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
                         b = a + nA / 2;
                     }
                     const int j = S.la[a], k = lb[b];
-                    const float4 gj = S.g[j], gk = S.g[k];
-                    const float dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
-                    r2[f] = dx * dx + dy * dy + dz * dz;
+                    r2[f] = dist2(S.g[j], S.g[k]);
                     ok[f] = r2[f] < rc2;
                     jk[f] = (unsigned)j | ((unsigned)k << 16);
                 }
@@ -494,7 +500,6 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
                     const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                                         __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
                     S.tjk[pos] = jk[f];
-                    S.tr2[pos] = r2[f];
                 }
                 cnt += __popcll(m);
             }
@@ -503,7 +508,7 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
         __syncthreads();
         const int per = (cnt + kWave - 1) / kWave;
         const int x0 = lane * per, x1 = min(cnt, x0 + per);
-        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16, S.tr2[x]);
+        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16);
         flush();
         __syncthreads();
     }
@@ -540,7 +545,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const float lam0 = gm.lam[0], lam1 = gm.lam[1];
     int jc = -1;  // j of the cached geometry (runs of one j per lane)
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
+    for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
         if (j != jc) {
             gj = S.g[j];
             cj = S.c[j];
@@ -548,6 +553,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
         }
         const float4 gk = S.g[k];
         const float4 ck = S.c[k];
+        const float r2jk = dist2(gj, gk);
         float fcjk = 1.f, dfcjk = 0.f;
         if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, __builtin_amdgcn_sqrtf(r2jk), fcjk, dfcjk);
         const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
@@ -630,7 +636,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                                              float* __restrict__ Gin, int ldG, int ldK, int capL, int trip)
 {
     extern __shared__ float sm[];
-    const Stage S = carve(sm, capL, trip);
+    const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
@@ -713,7 +719,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             }
             int jc = -1;  // j of the cached geometry (runs of one j per lane)
             float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-            for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
+            for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
                 if (j != jc) {
                     gj = S.g[j];
                     cj = S.c[j];
@@ -721,6 +727,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 }
                 const float4 gk = S.g[k];
                 const float4 ck = S.c[k];
+                const float r2jk = dist2(gj, gk);
                 const float rjk = __builtin_amdgcn_sqrtf(r2jk);
                 float fcjk, dfcjk;
                 fcjk = 1.f;
@@ -971,8 +978,8 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                                                 int push, int capL, int trip)
 {
     extern __shared__ float sm[];
-    const Stage S = carve(sm, capL, trip);
-    double* ax = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip));
+    const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
+    double* ax = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip, stage_mp(NB0, NB1)));
     double* ay = ax + capL;
     double* az = ay + capL;
     const int i = blockIdx.x;
@@ -1065,7 +1072,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             }
         };
         float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;  // geometry of the run's j
-        for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
+        for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
             if (j != cur) {
                 flush();
                 cur = j;
@@ -1076,6 +1083,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             }
             const float4 gk = S.g[k];
             const float4 ck = S.c[k];
+            const float r2jk = dist2(gj, gk);
             const float ijk = __builtin_amdgcn_rsqf(r2jk);
             const float rjk = r2jk * ijk;
             float fcjk, dfcjk;
@@ -1234,8 +1242,8 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
                                                   double* __restrict__ Gdot, int ldG, int capL, int trip)
 {
     extern __shared__ float sm[];
-    const Stage S = carve(sm, capL, trip);
-    float4* du = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip));
+    const Stage S = carve(sm, capL, trip, stage_mp(0, 0));
+    float4* du = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip, stage_mp(0, 0)));
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
@@ -1308,9 +1316,10 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
         const MemberMasks R = load_members(S, mf, nmem, nullptr);
-        for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k, float r2jk) {
+        for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
+            const float r2jk = dist2(gj, gk);
             const float4 uj = du[j], uk = du[k];
             const float ijk = __builtin_amdgcn_rsqf(r2jk);
             const float rjk = r2jk * ijk;
@@ -1689,15 +1698,17 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     if (t.max_width > kMaxWidth)
         throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
     if (s.natoms == 0) return;
-    const int capL = s.cap;
+    const int capL = s.stage_cap;
     const int P = pow2_members(t);
     const int tsf = mm.tune.trip_sf, tf = mm.tune.trip_force;
-    const size_t lds_sf = stage_bytes(capL, tsf);
-    const size_t lds_f = stage_bytes(capL, tf) + sizeof(double) * 3 * (size_t)capL;
-    if (lds_f > 65536 || lds_sf > 65536)
+    // LDS of the largest kernel instances (the member paths' constant area)
+    const size_t lds_sf_max = stage_bytes(capL, tsf, stage_mp(0, 0));
+    const size_t lds_f_max = stage_bytes(capL, tf, stage_mp(0, 0)) + sizeof(double) * 3 * (size_t)capL;
+    if (lds_f_max > 65536 || lds_sf_max > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     timed("sf32", [&] {
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
+            const size_t lds_sf = stage_bytes(capL, tsf, stage_mp(decltype(nb0)::value, decltype(nb1)::value));
             hipLaunchKernelGGL((k_sf32<decltype(m)::value, decltype(fast)::value, decltype(nb0)::value,
                                        decltype(nb1)::value>),
                                dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, mm.d_gmix, s, b.G,
@@ -1742,6 +1753,8 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                            b.F, n3);
         const int ps = push ? 1 : 0;
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
+            const size_t lds_f = stage_bytes(capL, tf, stage_mp(decltype(nb0)::value, decltype(nb1)::value)) +
+                                 sizeof(double) * 3 * (size_t)capL;
             hipLaunchKernelGGL((k_force32<decltype(m)::value, decltype(fast)::value, decltype(nb0)::value,
                                           decltype(nb1)::value>),
                                dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, mm.d_gmix, s, b.dEdGf,
@@ -1760,9 +1773,9 @@ void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_mod
     if (t.max_width > kMaxWidth)
         throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
     if (s.natoms == 0) return;
-    const int capL = s.cap;
+    const int capL = s.stage_cap;
     const int tt = mm.tune.trip_tangent;
-    const size_t lds = stage_bytes(capL, tt) + sizeof(float4) * (size_t)capL;
+    const size_t lds = stage_bytes(capL, tt, stage_mp(0, 0)) + sizeof(float4) * (size_t)capL;
     if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     dispatch_members(mm.fastM, pow2_members(t), [&](auto m, auto fast) {
         hipLaunchKernelGGL((k_tangent32<decltype(m)::value, decltype(fast)::value>), dim3(s.natoms), dim3(kWave), lds,

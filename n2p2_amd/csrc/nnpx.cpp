@@ -62,6 +62,10 @@ struct nnpx_ctx {
     // neighbour list state
     int cap = 0;
     int max_nb = 0;
+    // LDS staging capacity of the mixed kernels: the last checked build's
+    // maximum count (0: the list capacity); a larger count at run time is
+    // flagged (bit 4) and the evaluation repeated at full capacity
+    int stage_cap = 0;
     bool nbr_valid = false;
     bool rev_valid = false;
     // results state
@@ -206,6 +210,7 @@ SysView sysview(nnpx_ctx* c)
     s.sid = existing<int>(c, "sid");
     s.st = existing<StructT>(c, "st");
     s.cap = c->cap;
+    s.stage_cap = c->stage_cap > 0 && c->stage_cap < c->cap && !c->md_ready ? c->stage_cap : c->cap;
     s.nb_n = existing<int>(c, "nb_n");
     s.nb_j = existing<int>(c, "nb_j");
     s.nb_img = existing<int>(c, "nb_img");
@@ -387,6 +392,7 @@ void build_neighbors(nnpx_ctx* c, bool check)
         int f[2];
         read_flags(c, f);
         c->max_nb = f[1];
+        c->stage_cap = ((f[1] + 31) / 32) * 32;
         if (!(f[0] & 1)) break;
         if (f[1] > 1024) fail(NNPX_ERR_UNSUPPORTED, "more than 1024 neighbours per atom");
         c->cap = ((f[1] + f[1] / 4 + 31) / 32) * 32;
@@ -533,6 +539,10 @@ void settle(nnpx_ctx* c)
     if (f[0] & 2) fail(NNPX_ERR_HIP, "neighbour list is not symmetric");
     if (f[0] & 1) {
         c->cap = ((f[1] + f[1] / 4 + 31) / 32) * 32;
+        evaluate(c, c->last_precision, c->f_valid ? NNPX_EVAL_FORCES : 0u, true);
+    } else if (f[0] & 4) {
+        // an atom had more neighbours than the mixed kernels staged: the
+        // checked rebuild refits the staging capacity to this list
         evaluate(c, c->last_precision, c->f_valid ? NNPX_EVAL_FORCES : 0u, true);
     }
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
@@ -924,12 +934,25 @@ nnpx_status nnpx_update_positions(nnpx_ctx* c, const double* pos)
         if (!pos) fail(NNPX_ERR_ARGUMENT, "null positions");
         if (c->external) fail(NNPX_ERR_STATE, "positions are not used with an external neighbour list");
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        bool changed = false;
         for (int s = 0; s < c->S; ++s) {
             StructT& t = c->st_host[s];
-            if (t.algo == 1)
-                for (int64_t i = c->atom_off[s]; i < c->atom_off[s + 1]; ++i)
-                    if (!inside_cell(t, pos + 3 * i)) fail(NNPX_ERR_ARGUMENT, "position outside the cell (remap first)");
+            if (!t.periodic) continue;
+            int inside = 1;
+            for (int64_t i = c->atom_off[s]; i < c->atom_off[s + 1] && inside; ++i)
+                inside = inside_cell(t, pos + 3 * i) ? 1 : 0;
+            if (t.algo == 1 && !inside) fail(NNPX_ERR_ARGUMENT, "position outside the cell (remap first)");
+            // the brute-force image ranges are pruned only while every atom
+            // is inside the cell
+            if (inside != t.inside) {
+                t.inside = inside;
+                changed = true;
+            }
         }
+        if (changed)
+            hip_check(hipMemcpy(existing<StructT>(c, "st"), c->st_host.data(), sizeof(StructT) * c->S,
+                                hipMemcpyHostToDevice),
+                      "hipMemcpy");
         hip_check(hipMemcpy(existing<double>(c, "pos"), pos, sizeof(double) * 3 * c->N, hipMemcpyHostToDevice),
                   "hipMemcpy");
         c->nbr_valid = false;
@@ -1342,7 +1365,7 @@ nnpx_status nnpx_autotune(nnpx_ctx* c)
         const int capL = c->cap;
         const ModelT& t = c->hm.t;
         auto fits = [&](int trip, size_t extra) {
-            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 8 + (size_t)2 * kMaxMembersExp * 16 + extra <= 65536;
+            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)2 * kMaxMembersExp * 16 + extra <= 65536;
         };
         c->timing = true;
         double best_sf = 1e30, best_f = 1e30, best_t = 1e30;

@@ -233,3 +233,64 @@ def test_mixed_rejects_compact_functions():
     assert ei.value.status == 7  # NNPX_ERR_UNSUPPORTED
     m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
     assert np.isfinite(m.forces()).all()
+
+
+@pytest.mark.parametrize("remapped", [True, False])
+def test_position_update_regrows_lists(water_dir, remapped):
+    """nnpx_update_positions followed by an evaluation must give the results
+    of a fresh context on the new positions, (1) when one atom gains more
+    neighbours than the largest list of the last checked build (the mixed
+    kernels stage that many slots; the overflow is flagged and the
+    evaluation repeated) and (2) when atoms leave the cell of a small
+    periodic structure (the brute-force image ranges are pruned only while
+    every atom is inside).  192-atom water cell (12.42 A < 2 rc, brute
+    force); both forms."""
+    say("case: position update, water-192, remapped", remapped)
+    pos, elem, box = synth.water_box(4)
+    pos = remap(pos, box)
+    o = Oracle(water_dir)
+    o.set_structure(pos, elem, box)
+    o.build_neighbors(BRUTE)
+    counts = o.neighbors()[0]
+    a = int(np.argmax(counts))
+    rng = np.random.default_rng(17)
+    new = pos.copy()
+    L = box[0, 0]
+    # move 24 atoms that are not neighbours of atom a into a 5.5 A shell around it
+    d = new - new[a]
+    d -= L * np.round(d / L)
+    far = np.where(np.linalg.norm(d, axis=1) > 6.5)[0][:24]
+    for q in far:
+        for _ in range(1000):
+            v = rng.normal(size=3)
+            v *= rng.uniform(2.0, 5.5) / np.linalg.norm(v)
+            p = new[a] + v
+            dd = new - p
+            dd -= L * np.round(dd / L)
+            if np.linalg.norm(dd, axis=1).min() > 1.2:
+                new[q] = p
+                break
+    if remapped:
+        new = remap(new, box)
+    else:
+        new[::7] += box[0]  # a seventh of the atoms one cell over
+    o.set_structure(new, elem, box)
+    o.build_neighbors(BRUTE)
+    grown = int(o.neighbors()[0].max())
+    say("  largest list %d -> %d" % (counts.max(), grown))
+    assert grown >= ((counts.max() + 31) // 32) * 32
+    for precision in (n2p2_amd.PLAIN_FP64, n2p2_amd.MIXED):
+        m = n2p2_amd.Mode(water_dir)
+        m.set_structures(pos, elem, box[None])
+        m.evaluate(precision, forces=True)
+        m.update_positions(new)
+        m.evaluate(precision, forces=True)
+        F1, E1 = m.forces(), m.energies()[0]
+        f = n2p2_amd.Mode(water_dir)
+        f.set_structures(new, elem, box[None])
+        f.evaluate(precision, forces=True)
+        F2, E2 = f.forces(), f.energies()[0]
+        dF = float(np.abs(F1 - F2).max())
+        say("  precision %d: max |dF| %.2e  dE %.2e" % (precision, dF, abs(E1 - E2)))
+        assert dF <= 1e-9 * max(np.abs(F2).max(), 1.0)
+        assert abs(E1 - E2) <= 1e-9 * abs(E2)
