@@ -21,7 +21,12 @@ What it writes (data only -- numbers and the example's input files):
   1526-1550, 1551-1846``).
 * ``LJ_checkdw/`` -- the input files of ``examples/nnp-checkdw/LJ`` (101 Ne
   dimers, radial functions, a 5-3-1 network), the example the reference's
-  nnp-checkdw tool test runs (``test/cpp/Example_nnp_checkdw.h:21-24``).
+  nnp-checkdw tool test runs (``test/cpp/Example_nnp_checkdw.h:21-24``);
+  ``examples/nnp-checkf/LJ`` holds the same files.
+* ``H2O_RPBE-D3_checkf/`` -- the input files of
+  ``examples/nnp-checkf/H2O_RPBE-D3`` (three 48-atom periodic water cells,
+  normalised units), run by the reference's nnp-checkf tool test
+  (``test/cpp/Example_nnp_checkf.h:18-20``).
 * ``sf_kat.json`` -- the symmetry-function known answers (types 2, 3, 9, 12,
   13, 20-25) of
   ``test/cpp/fixtures.h:151-258`` with the cluster geometries of
@@ -73,11 +78,12 @@ def example(ref, name, count):
               open(os.path.join(out, "expected.json"), "w"), indent=0)
 
 
-def checkdw_example(ref, name):
-    """Copy an nnp-checkdw example's input files (no expected values: the
-    reference's tool test only checks that the output files exist)."""
-    ex = os.path.join(ref, "examples", "nnp-checkdw", name)
-    out = os.path.join(HERE, name + "_checkdw")
+def tool_example(ref, tool, name):
+    """Copy an nnp-checkdw / nnp-checkf example's input files (no expected
+    values: the reference's tool tests only check that the output files
+    exist)."""
+    ex = os.path.join(ref, "examples", "nnp-" + tool, name)
+    out = os.path.join(HERE, name + "_" + tool)
     os.makedirs(out, exist_ok=True)
     for fn in sorted(os.listdir(ex)):
         if fn in ("input.nn", "scaling.data", "input.data") or re.match(r"weights\.\d{3}\.data$", fn):
@@ -90,7 +96,8 @@ def main(ref):
     example(ref, "Anisole_SCAN", 256)
     example(ref, "DMABN_SCAN", 21)
     example(ref, "Ethylbenzene_SCAN", 288)
-    checkdw_example(ref, "LJ")
+    tool_example(ref, "checkdw", "LJ")
+    tool_example(ref, "checkf", "H2O_RPBE-D3")
 
     fx = open(os.path.join(ref, "test", "cpp", "fixtures.h")).read()
     kat = {"source": "test/cpp/fixtures.h:11-258; test/cpp/test_SymmetryFunctions.cpp:36,68-74",

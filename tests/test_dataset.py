@@ -294,3 +294,60 @@ def test_checkdw_lj_example(batch_atoms, tmp_path):
         assert abs(rE[k, 4] - nE) <= 1e-9 * max(abs(nE), 1.0)
         assert np.abs(rF[k::22, 4] - nF).max() <= 1e-9 * max(np.abs(nF).max(), 1.0)
     o.set_weights(w0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,symbols", [("LJ_checkdw", ["Ne"]), ("H2O_RPBE-D3_checkf", ["H", "O"])])
+def test_checkf_reference_examples(name, symbols, tmp_path):
+    """nnpx-checkf on the examples the reference's nnp-checkf tool test runs
+    (test_nnp-checkf.cpp:17-21, Example_nnp_checkf.h:18-24: the LJ dimers --
+    the same files as the nnp-checkdw LJ example -- and three normalised
+    48-atom H2O_RPBE-D3 cells).  The reference test only asks for the output
+    files; here in addition every structure's verdict follows the rule
+    (WARNING iff max |F_analytic - F_numeric| > 10 delta^2; the steepest LJ
+    dimers miss it, as they would in the reference), the analytic column
+    equals the oracle's forces and the numeric column the oracle's central
+    differences (a few structures, a few coordinates each)."""
+    need(CHECKF)
+    from n2p2_amd import read_structures
+    gdir = os.path.join(HERE, "golden", name)
+    d = tmp_path / "run"
+    shutil.copytree(gdir, d)
+    say("case: nnpx-checkf on the %s example" % name)
+    rc, so, se = run([CHECKF], cwd=str(d))
+    assert rc == 0, se
+    for fn in ("nnp-checkf.log.0000", "checkf-forces.out", "checkf-summary.out"):
+        assert os.path.exists(d / fn), fn
+    structs = read_structures(os.path.join(gdir, "input.data"), symbols)
+    f = read_table(d / "checkf-forces.out", 5)
+    summ = read_table(d / "checkf-summary.out", 4)
+    assert len(f) == sum(3 * len(s[1]) for s in structs) and len(summ) == len(structs)
+    verdicts = re.findall(r"^Configuration\s+(\d+):.*?(OK\.|WARNING!)", open(d / "nnp-checkf.log.0000").read(), re.M)
+    assert len(verdicts) == len(structs)
+    for (idx, v), row in zip(verdicts, summ):
+        assert int(idx) == int(row[0]) and (v == "WARNING!") == (row[3] > 10 * 1e-4 ** 2)
+    o = Oracle(str(d))
+    wa = wn = 0.0
+    for s in range(0, len(structs), max(1, len(structs) // 4)):
+        pos, elem, box = structs[s]
+        o.set_structure(pos, elem, box)
+        o.predict(PULL, BRUTE)
+        rows = f[f[:, 0] == s + 1]
+        wa = max(wa, float(np.abs(rows[:, 3] - o.forces().reshape(-1)).max()))
+        for q in (0, len(rows) // 2, len(rows) - 1):
+            a, c = q // 3, q % 3
+            E = []
+            for sign in (-1.0, 1.0):
+                p = pos.copy()
+                p[a, c] += sign * 1e-4
+                o.set_structure(p, elem, box)
+                o.predict(PULL, BRUTE)
+                E.append(o.energy())
+            fn = -(E[1] - E[0]) / 2e-4
+            wn = max(wn, abs(rows[q, 4] - fn) / max(abs(fn), 1.0))
+    ok = sum(v == "OK." for _, v in verdicts)
+    say("  %d structures (%d OK), max |analytic - oracle| %.2e, numeric vs oracle rel %.2e" % (len(structs), ok, wa, wn))
+    assert wa <= 1e-8
+    assert wn <= 1e-6
+    if name == "H2O_RPBE-D3_checkf":
+        assert ok == len(structs)
