@@ -140,6 +140,46 @@ __device__ inline void compact_fdf(int core, int asym, double cen, double scl, d
     dfa *= copysign(scl, a);
 }
 
+// FP32 forms of core_fdf / compact_fdf for the mixed form's compact terms.
+__device__ inline void core_fdf(int core, float x, float& f, float& df)
+{
+    if (core == 0) {
+        float const t = 1.f / (x * x - 1.f);
+        float const t2 = __expf(t);
+        f = (float)M_E * t2;
+        df = -2.f * (float)M_E * x * t * t * t2;
+    } else if (core == 1) {
+        f = (2.f * x - 3.f) * x * x + 1.f;
+        df = x * (6.f * x - 6.f);
+    } else if (core == 2) {
+        float const x2 = x * x;
+        f = ((15.f - 6.f * x) * x - 10.f) * x * x2 + 1.f;
+        df = x2 * ((60.f - 30.f * x) * x - 30.f);
+    } else if (core == 3) {
+        float const x2 = x * x;
+        f = (x * (x * (20.f * x - 70.f) + 84.f) - 35.f) * x2 * x2 + 1.f;
+        df = x2 * x * (x * (x * (140.f * x - 420.f) + 420.f) - 140.f);
+    } else {
+        float x4 = x * x;
+        x4 *= x4;
+        f = (x * (x * ((315.f - 70.f * x) * x - 540.f) + 420.f) - 126.f) * x * x4 + 1.f;
+        df = x4 * (x * (x * ((2520.f - 630.f * x) * x - 3780.f) + 2520.f) - 630.f);
+    }
+}
+
+__device__ inline void compact_fdf(int core, int asym, float cen, float scl, float a, float& fa, float& dfa)
+{
+    a = (a - cen) * scl;
+    float const x = fabsf(a);
+    if (asym) {
+        core_fdf(core, (2.f - x) * x, fa, dfa);
+        dfa *= 2.f - 2.f * x;
+    } else {
+        core_fdf(core, x, fa, dfa);
+    }
+    dfa *= copysignf(scl, a);
+}
+
 // Exponentiation by squaring (utility.cpp:292-316).
 template <typename T>
 __device__ inline T pow_int(T x, int n)

@@ -79,6 +79,12 @@ void launch_sf64(const ModelT* m, const ModelT& hm, const SysView& s, const SF64
 // The compact-function groups (types 20, 21, 22) into the same outputs.
 void launch_sfcomp64(const ModelT* m, const ModelT& hm, const SysView& s, const SF64Out& o,
                      bool deriv, int max_nb, hipStream_t stream);
+// The compact groups in the mixed form (FP32 terms): values (scaled G and its
+// FP32 copy) or, with forces, the terms' derivatives contracted with the
+// weights fw = dE/dG times the scaling factor and added to F.  Stages
+// s.stage_cap neighbours (a longer list sets flags[0] bit 4).
+void launch_sfcomp32(const ModelT* m, const ModelT& hm, const SysView& s, double* G, float* Gin, int ldG, int ldK,
+                     const float* fw, double* F, bool push, bool forces, hipStream_t stream);
 
 // FP64 atomic NNs over per-element atom lists; dE/dG by the adjoint.
 struct NNIO {

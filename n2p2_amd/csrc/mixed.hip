@@ -658,6 +658,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
         } else {
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
+        if (sf_compact(GT.type)) continue;  // the compact kernel's
         if (GT.type == 2) {
             // radial members in chunks of kRadChunk (any member count)
             for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
@@ -807,7 +808,9 @@ struct NNMix {
     int atom_off[kMaxElem + 1];
     int wW[kMaxElem][kMaxLayers], wT[kMaxElem][kMaxLayers], bo[kMaxElem][kMaxLayers];
     int pin[kMaxElem][kMaxLayers], pout[kMaxElem][kMaxLayers];
-    int nelem, ldK, ldG, stride, terms;
+    int nelem, ldK, ldG, terms;
+    int stride_in;  // LDS row length of the input tile (padded input width + 4)
+    int stride;     // ... of the hidden/output tiles (padded layer width + 4)
 };
 
 __device__ inline float act32(int a, float x, float& d)
@@ -882,7 +885,8 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
     const NNT& N = M->el[e].nn;
     const int nl = N.nl;
     const int st = a.stride;
-    float* bufA = sm;
+    float* bufI = sm;                           // [16][stride_in] network input
+    float* bufA = bufI + kNNRows * a.stride_in;  // hidden/output tiles [16][st]
     float* bufB = bufA + kNNRows * st;
     float* dfd = bufB + kNNRows * st;  // [layer][16][st]
     const int tile = (int)blockIdx.x - a.blk_off[e];
@@ -893,11 +897,12 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
     const int pin0 = a.pin[e][1];
     for (int x = lane; x < kNNRows * pin0; x += kWave) {
         const int r = x / pin0, k = x % pin0;
-        bufA[r * st + k] = r < cnt ? Gin[(size_t)a.atoms[first + r] * a.ldK + k] : 0.f;
+        bufI[r * a.stride_in + k] = r < cnt ? Gin[(size_t)a.atoms[first + r] * a.ldK + k] : 0.f;
     }
     __syncthreads();
-    float* in = bufA;
-    float* out = bufB;
+    float* in = bufI;
+    float* out = bufA;
+    int sin = a.stride_in;
     for (int l = 1; l < nl; ++l) {
         const int pi = a.pin[e][l], po = a.pout[e][l];
         const uint16_t* W = a.w + a.wW[e][l];
@@ -907,7 +912,7 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
             f4 c = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < pi; k0 += 16) {
                 s4 ah, al, bh, bl;
-                a_frag(in, st, k0, ah, al);
+                a_frag(in, sin, k0, ah, al);
                 b_frag(W, pi, po * pi, n0, k0, bh, bl);
                 c = mma(c, ah, al, bh, bl, a.terms);
             }
@@ -923,7 +928,10 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
             }
         }
         __syncthreads();
-        float* t = in; in = out; out = t;
+        // the hidden and output tiles alternate between bufA and bufB
+        in = out;
+        out = in == bufA ? bufB : bufA;
+        sin = st;
     }
     if (lane < cnt) Eatom[a.atoms[first + lane]] = (double)in[lane * st];
     // adjoint: delta of the output layer is f'_out for the energy neuron
@@ -1004,6 +1012,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         } else {
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
+        if (sf_compact(GT.type)) continue;  // the compact kernel's
         if (GT.type == 2) {
             // radial: each lane owns its neighbours, no conflicts
             for (int k = lane; k < n; k += kWave) {
@@ -1276,6 +1285,7 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
         } else {
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
+        if (sf_compact(GT.type)) continue;  // the compact kernel's
         if (GT.type == 2) {
             for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
                 float acc[kRadChunk], cmp[kRadChunk];
@@ -1489,6 +1499,13 @@ void dispatch_angular(const MixedModel& mm, int maxm, F&& f)
     dispatch_members(mm.fastM, maxm, [&](auto m, auto fs) { f(m, fs, I0{}, I0{}); });
 }
 
+bool has_weighted_gaussian(const ModelT& t)
+{
+    for (int g = 0; g < t.ngroups; ++g)
+        if (t.grp[g].type == 12 || t.grp[g].type == 13) return true;
+    return false;
+}
+
 int pow2_members(const ModelT& t)
 {
     const int m = std::max(std::max(t.max_mem_ang, t.max_mem_rad), 1);
@@ -1627,6 +1644,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
     std::vector<float> b;
     auto pad16 = [](int x) { return (x + 15) / 16 * 16; };
     mm.maxpad = 16;
+    mm.maxpadh = 16;
     mm.ldK = 16;
     for (int e = 0; e < t.nelem; ++e) {
         const NNT& N = t.el[e].nn;
@@ -1637,6 +1655,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
             mm.pin[e][l] = pi;
             mm.pout[e][l] = po;
             mm.maxpad = std::max(mm.maxpad, std::max(pi, po));
+            mm.maxpadh = std::max(mm.maxpadh, po);
             std::vector<float> W((size_t)po * pi, 0.f);
             for (int k = 0; k < nout; ++k)
                 for (int j = 0; j < nin; ++j) W[(size_t)k * pi + j] = (float)hm.weights[N.woff[l] + (size_t)k * nin + j];
@@ -1692,11 +1711,9 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed)
 {
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
-    if (t.has_comp || t.has_weighted)
-        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact or element-weighted "
-                                          "symmetry functions (types 12, 13, 20-25); use the plain FP64 form"};
-    if (t.max_width > kMaxWidth)
-        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
+    if (has_weighted_gaussian(t))
+        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate element-weighted Gaussian "
+                                          "symmetry functions (types 12, 13); use the plain FP64 form"};
     if (s.natoms == 0) return;
     const int capL = s.stage_cap;
     const int P = pow2_members(t);
@@ -1714,6 +1731,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                                dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, mm.d_gmix, s, b.G,
                                b.Gin, b.ldG, mm.ldK, capL, tsf);
         });
+        launch_sfcomp32(d_model, t, s, b.G, b.Gin, b.ldG, mm.ldK, nullptr, nullptr, false, false, stream);
     });
     NNMix a{};
     a.w = mm.d_w;
@@ -1723,7 +1741,8 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     a.nelem = t.nelem;
     a.ldK = mm.ldK;
     a.ldG = b.ldG;
-    a.stride = mm.maxpad + 4;
+    a.stride = mm.maxpadh + 4;
+    a.stride_in = mm.ldK + 4;
     a.terms = mm.terms;
     int blocks = 0;
     for (int e = 0; e < t.nelem; ++e) {
@@ -1740,7 +1759,8 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     }
     a.blk_off[t.nelem] = blocks;
     a.atom_off[t.nelem] = b.elem_off[t.nelem];
-    const size_t lds_nn = sizeof(float) * (size_t)kNNRows * a.stride * (2 + kMaxLayers);
+    const size_t lds_nn = sizeof(float) * (size_t)kNNRows * (a.stride_in + (size_t)a.stride * (2 + kMaxLayers));
+    if (lds_nn > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "network too wide for the mixed NN kernel's LDS"};
     timed("nnbf16", [&] {
         hipLaunchKernelGGL(k_nnbf16, dim3(blocks), dim3(kWave), lds_nn, stream, d_model, a, b.Gin, b.Eatom, b.dEdG,
                            b.dEdGf);
@@ -1760,6 +1780,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                                dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, mm.d_gmix, s, b.dEdGf,
                                b.ldG, b.F, ps, capL, tf);
         });
+        launch_sfcomp32(d_model, t, s, nullptr, nullptr, b.ldG, mm.ldK, b.dEdGf, b.F, push, true, stream);
     });
 }
 

@@ -58,6 +58,7 @@ struct Tuning {
 struct MixedModel {
     int ldK = 0;                // padded NN input width (multiple of 16)
     int maxpad = 0;             // max padded layer width
+    int maxpadh = 0;            // max padded width of the hidden and output layers
     int terms = 3;              // bf16 product terms (1 or 3)
     MemberF* d_mem = nullptr;   // [nmembers], same order as ModelT::mem
     GroupMixF* d_gmix = nullptr; // [ngroups]

@@ -1,22 +1,28 @@
-// FP64 compact symmetry-function groups with stored dG/dr (the plain form).
+// Compact symmetry-function groups (types 20-25).
 //
 // Computes, per central atom, what SymGrpCompRad::calculate
 // (SymGrpCompRad.cpp:129-199), SymGrpCompAngn::calculate
 // (SymGrpCompAngn.cpp:128-390) and SymGrpCompAngw::calculate
-// (SymGrpCompAngw.cpp:117-341) compute: scaled G, the atom's own dG/dr and
-// every neighbour entry's dG/dr in the compact slot order, for the groups of
-// types 20, 21 and 22, and of their element-weighted forms 23, 24 and 25
-// (SymGrpCompRadWeighted, SymGrpCompAng{n,w}Weighted); the Gaussian groups
-// are the FP64 SF kernel's.
+// (SymGrpCompAngw.cpp:117-341) compute, for the groups of types 20, 21 and 22
+// and of their element-weighted forms 23, 24 and 25 (SymGrpCompRadWeighted,
+// SymGrpCompAng{n,w}Weighted); the Gaussian groups are the other kernels'.
+// Three modes share the term code:
+//  * plain (FP64 terms): scaled G, the atom's own dG/dr and every neighbour
+//    entry's dG/dr in the compact slot order (the stored-derivative form);
+//  * values (FP32 terms, FP64 sums): scaled G and its FP32 copy, the mixed
+//    form's network input;
+//  * force (FP32 terms): the terms' derivatives contracted with dE/dG on the
+//    fly (members summed per triplet in FP32, neighbours accumulated in LDS
+//    FP64) and added to the forces, the mixed form's fused backward.
 //
 // Mapping: one 256-thread workgroup per atom with the neighbour geometry
 // staged in LDS.  Radial groups: threads over neighbours, members in
 // sequence.  Angular groups: threads over the unordered neighbour pairs of
-// the group's element pair; the members are taken in chunks of CH, each
-// chunk re-walking the pairs with its dG/dr contributions accumulated by LDS
-// FP64 atomics ([CH][3][cap]) and written once.  Compact groups are large
-// (tens of members with disjoint radial and angular windows), so a chunk
-// keeps the LDS accumulator bounded whatever the group size.
+// the group's element pair.  Plain and values modes take the members in
+// chunks of CH, each chunk re-walking the pairs (plain: dG/dr contributions
+// accumulated by LDS FP64 atomics [CH][3][cap] and written once); compact
+// groups are large (tens of members with disjoint radial and angular
+// windows), so a chunk keeps the accumulators bounded whatever the group size.
 #include "kernels.hpp"
 #include "nnpx.h"
 
@@ -25,7 +31,7 @@ namespace {
 
 constexpr int kCThreads = 256;
 constexpr int kCWaves = kCThreads / kWave;
-constexpr int kRedSlots = 8;
+constexpr int kRedSlots = 16;  // per-wave reduction slots: members of a chunk (and 4 radial sums)
 
 struct CLds {
     double* dr;   // [3][cap]
@@ -74,39 +80,58 @@ __device__ inline int build_list(const CLds& L, int n, int elem, double rmin, do
     return c;
 }
 
-__device__ inline void radial_fdf(const MemberT& m, double r, double& f, double& df)
+enum CompMode { CM_PLAIN = 0, CM_VALUES = 1, CM_FORCE = 2 };
+
+// Member windows and centres in the term precision.
+template <typename T>
+__device__ inline void radial_fdf(const MemberT& m, T r, T& f, T& df)
 {
-    compact_fdf(m.core, m.asym, m.cen, m.scl, r, f, df);
+    compact_fdf(m.core, m.asym, (T)m.cen, (T)m.scl, r, f, df);
 }
 
 // The r_ij factor of member m (zero outside its window), as the reference's
 // per-j precomputation radij[l] holds it.
-__device__ inline double radial_ij(const MemberT& m, double rij, double& dradij)
+template <typename T>
+__device__ inline T radial_ij(const MemberT& m, T rij, T& dradij)
 {
-    double f = 0.0;
-    dradij = 0.0;
-    if (rij > m.rl && rij < m.rcm) radial_fdf(m, rij, f, dradij);
+    T f = T(0);
+    dradij = T(0);
+    if (rij > (T)m.rl && rij < (T)m.rcm) radial_fdf(m, rij, f, dradij);
     return f;
 }
 
 // The reference's break condition of member m for one triplet.
-__device__ inline bool member_skipped(const MemberT& m, double radij, double rik, double rjk, bool narrow, double th)
+template <typename T>
+__device__ inline bool member_skipped(const MemberT& m, T radij, T rik, T rjk, bool narrow, T th)
 {
-    return radij == 0.0 || rik <= m.rl || rik >= m.rcm || (narrow && (rjk <= m.rl || rjk >= m.rcm)) ||
-           th <= m.al || th >= m.ar;
+    return radij == T(0) || rik <= (T)m.rl || rik >= (T)m.rcm || (narrow && (rjk <= (T)m.rl || rjk >= (T)m.rcm)) ||
+           th <= (T)m.al || th >= (T)m.ar;
 }
 
-template <int CH>
-__global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict__ M, SysView s, SF64Out o,
-                                                         int deriv, int capL)
+// Outputs of the mixed modes.
+struct CompMix {
+    double* G;        // [N*ldG] scaled values (values mode)
+    float* Gin;       // [N*ldK] their FP32 copy, the network input
+    int ldG, ldK;
+    const float* fw;  // [N*ldG] dE/dG times the scaling factor (force mode)
+    double* F;        // forces (force mode)
+    int push;         // keep contributions to the atom's own images
+};
+
+template <typename T, int MODE, int CH>
+__global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__ M, SysView s, SF64Out o,
+                                                       CompMix c, int deriv, int capL)
 {
+    constexpr bool PLAIN = MODE == CM_PLAIN, FORCE = MODE == CM_FORCE;
+    static_assert(CH <= kRedSlots, "a member chunk needs one reduction slot per member");
+    constexpr int ACC = PLAIN ? CH : FORCE ? 1 : 0;  // [3][cap] accumulator sets in LDS
     extern __shared__ double smem[];
     CLds L;
     L.dr = smem;
     L.d = L.dr + 3 * capL;
     L.red = L.d + capL;
     L.acc = L.red + kCWaves * kRedSlots;
-    L.el = reinterpret_cast<int*>(L.acc + (size_t)CH * 3 * capL);
+    L.el = reinterpret_cast<int*>(L.acc + (size_t)ACC * 3 * capL);
     L.la = L.el + capL;
     L.lb = L.la + capL;
 
@@ -116,6 +141,7 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
     const ElementT& E = M->el[e];
     const StructT& S = s.st[s.sid[i]];
     const int n = min(s.nb_n[i], capL);
+    if (!PLAIN && tid == 0 && s.nb_n[i] > capL) atomicOr(&s.flags[0], 4);  // the host repeats at full capacity
     const size_t ebase = (size_t)i * s.cap;
 
     for (int k = tid; k < n; k += kCThreads) {
@@ -127,12 +153,27 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
         L.dr[2 * capL + k] = dr[2];
         L.d[k] = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
         L.el[k] = s.elem[j];
+        if constexpr (FORCE) {
+            L.acc[k] = 0.0;
+            L.acc[capL + k] = 0.0;
+            L.acc[2 * capL + k] = 0.0;
+        }
     }
     __syncthreads();
 
-    double* Gi = o.G + (size_t)i * o.ldG;
+    double* Gi = PLAIN ? o.G + (size_t)i * o.ldG : c.G + (size_t)i * c.ldG;
     double* dGi = o.dGself + (size_t)i * o.ldG * 3;
     double* dGn = o.dGnbr + ebase * o.S * 3;
+    float* Ii = c.Gin + (size_t)i * c.ldK;
+    const float* wi = c.fw + (size_t)i * c.ldG;
+    double fself[3] = {0.0, 0.0, 0.0};  // force mode: this thread's part of the atom's own force
+
+    // Writes member mb's value v (values and plain modes).
+    auto put = [&](const MemberT& mb, double v) {
+        const double gs = scale_sf(M->scale[E.sf0 + mb.sf], v);
+        Gi[mb.sf] = gs;
+        if (MODE == CM_VALUES) Ii[mb.sf] = (float)gs;
+    };
 
     // ---------------------------------------------------------- radial groups
     for (int gi = 0; gi < E.nrad; ++gi) {
@@ -141,20 +182,44 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
         // weighted (SymGrpCompRadWeighted.cpp:120-200): every neighbour
         // element, each term times its atomic number
         const bool wt = G.type == 23;
+        if constexpr (FORCE) {
+            // threads own neighbours: sum_q w_q dG_q/dr over the members
+            for (int k = tid; k < n; k += kCThreads) {
+                if (!wt && L.el[k] != G.e1) continue;
+                const T rij = (T)L.d[k];
+                if (!(rij < (T)G.rc && rij > (T)G.rmin)) continue;
+                const T Z = wt ? (T)M->Zel[L.el[k]] : T(1);
+                T P = T(0);
+                for (int q = 0; q < G.nmem; ++q) {
+                    const MemberT& mb = M->mem[G.mem0 + q];
+                    if (!(rij > (T)mb.rl && rij < (T)mb.rcm)) continue;
+                    T rad, drad;
+                    radial_fdf(mb, rij, rad, drad);
+                    P += wi[mb.sf] * drad;
+                }
+                P *= Z / rij;
+                for (int cc = 0; cc < 3; ++cc) {
+                    const double v = (double)(P * (T)L.dr[cc * capL + k]);
+                    fself[cc] -= v;
+                    L.acc[cc * capL + k] += v;
+                }
+            }
+            continue;
+        }
         for (int q = 0; q < G.nmem; ++q) {
             const MemberT& mb = M->mem[G.mem0 + q];
             double gs = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
             for (int k = tid; k < n; k += kCThreads) {
                 if (!wt && L.el[k] != G.e1) continue;
                 double px = 0.0, py = 0.0, pz = 0.0;
-                const double rij = L.d[k];
-                if (rij < G.rc && rij > G.rmin && rij > mb.rl && rij < mb.rcm) {
-                    double rad, drad;
+                const T rij = (T)L.d[k];
+                if (rij < (T)G.rc && rij > (T)G.rmin && rij > (T)mb.rl && rij < (T)mb.rcm) {
+                    T rad, drad;
                     radial_fdf(mb, rij, rad, drad);
-                    const double Z = M->Zel[L.el[k]];
-                    if (wt) gs += Z * rad;
-                    else gs += rad;
-                    if (drad != 0.0) {
+                    const T Z = (T)M->Zel[L.el[k]];
+                    if (wt) gs += (double)(Z * rad);
+                    else gs += (double)rad;
+                    if (PLAIN && drad != T(0)) {
                         if (wt) drad *= Z * mb.sfac / rij;
                         else drad *= mb.sfac / rij;
                         px = drad * L.dr[k];
@@ -163,20 +228,22 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                         sx += px; sy += py; sz += pz;
                     }
                 }
-                if (deriv) {
+                if (PLAIN && deriv) {
                     double* d = dGn + ((size_t)k * o.S + (wt ? mb.slotE[L.el[k]] : mb.slot1)) * 3;
                     d[0] = -px; d[1] = -py; d[2] = -pz;
                 }
             }
             block_sum_slot(gs, L.red, 0);
-            block_sum_slot(sx, L.red, 1);
-            block_sum_slot(sy, L.red, 2);
-            block_sum_slot(sz, L.red, 3);
+            if (PLAIN) {
+                block_sum_slot(sx, L.red, 1);
+                block_sum_slot(sy, L.red, 2);
+                block_sum_slot(sz, L.red, 3);
+            }
             __syncthreads();
             if (tid < 4) {
                 const double v = red_total(L.red, tid);
-                if (tid == 0) Gi[mb.sf] = scale_sf(M->scale[E.sf0 + mb.sf], v);
-                else if (deriv) dGi[mb.sf * 3 + tid - 1] = v;
+                if (tid == 0) put(mb, v);
+                else if (PLAIN && deriv) dGi[mb.sf * 3 + tid - 1] = v;
             }
             __syncthreads();
         }
@@ -202,17 +269,19 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
         } else {
             total = nA * nB;
         }
-        const double r2min = G.rmin * G.rmin, r2max = G.rc2;
+        const T r2min = (T)(G.rmin * G.rmin), r2max = (T)G.rc2;
         const MemberT* mem = M->mem + G.mem0;
+        // force mode: one pass over all members; plain/values: chunks of CH
+        const int step = FORCE ? G.nmem : CH;
 
-        for (int q0 = 0; q0 < G.nmem; q0 += CH) {
-            const int nq = min(CH, G.nmem - q0);
-            if (deriv)
+        for (int q0 = 0; q0 < G.nmem; q0 += step) {
+            const int nq = min(step, G.nmem - q0);
+            if (PLAIN && deriv)
                 for (int x = tid; x < nq * 3 * capL; x += kCThreads) L.acc[x] = 0.0;
             __syncthreads();
-            double gacc[CH];
+            double gacc[FORCE ? 1 : CH];
 #pragma unroll
-            for (int q = 0; q < CH; ++q) gacc[q] = 0.0;
+            for (int q = 0; q < (FORCE ? 1 : CH); ++q) gacc[q] = 0.0;
 
             for (int t = tid; t < total; t += kCThreads) {
                 int a, b;
@@ -230,36 +299,48 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                 }
                 int j = L.la[a], k = lb[b];
                 if (j > k) { const int x = j; j = k; k = x; }
-                const double dr1[3] = {L.dr[j], L.dr[capL + j], L.dr[2 * capL + j]};
-                const double dr2[3] = {L.dr[k], L.dr[capL + k], L.dr[2 * capL + k]};
-                const double dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1], dr2[2] - dr1[2]};
-                double rjk = 0.0;
+                const T dr1[3] = {(T)L.dr[j], (T)L.dr[capL + j], (T)L.dr[2 * capL + j]};
+                const T dr2[3] = {(T)L.dr[k], (T)L.dr[capL + k], (T)L.dr[2 * capL + k]};
+                const T dr3[3] = {dr2[0] - dr1[0], dr2[1] - dr1[1], dr2[2] - dr1[2]};
+                T rjk = T(0);
                 if (narrow) {
                     rjk = dr3[0] * dr3[0] + dr3[1] * dr3[1] + dr3[2] * dr3[2];
                     if (rjk >= r2max || rjk <= r2min) continue;
                     rjk = sqrt(rjk);
                 }
-                const double rij = L.d[j], rik = L.d[k];
-                const double rinvijik = 1.0 / (rij * rik);
-                const double costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
-                // the angular polynomial vanishes at 0 and 180 degrees
-                if (costijk <= -1.0 || costijk >= 1.0) continue;
-                const double th = acos(costijk);
-                const double rinvij = rinvijik * rik;
-                const double rinvik = rinvijik * rij;
-                const double rinvjk = narrow ? 1.0 / rjk : 0.0;
-                const double phiijik0 = rinvij * (rinvik - rinvij * costijk);
-                const double phiikij0 = rinvik * (rinvij - rinvik * costijk);
-                const double dacos = -1.0 / sqrt(1.0 - costijk * costijk);
-#pragma unroll
-                for (int q = 0; q < CH; ++q) {
-                    if (q >= nq) break;
+                const T rij = (T)L.d[j], rik = (T)L.d[k];
+                const T rinvijik = T(1) / (rij * rik);
+                const T costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
+                T th, dacos;
+                if constexpr (PLAIN) {
+                    // the angular polynomial vanishes at 0 and 180 degrees
+                    if (costijk <= T(-1) || costijk >= T(1)) continue;
+                    th = acos(costijk);
+                    dacos = T(-1) / sqrt(T(1) - costijk * costijk);
+                } else {
+                    // FP32: the angle from the sine (cross product) and cosine,
+                    // accurate at both ends where acos(cos) is not
+                    const T cx = dr1[1] * dr2[2] - dr1[2] * dr2[1];
+                    const T cy = dr1[2] * dr2[0] - dr1[0] * dr2[2];
+                    const T cz = dr1[0] * dr2[1] - dr1[1] * dr2[0];
+                    const T sint = sqrt(cx * cx + cy * cy + cz * cz) * rinvijik;
+                    if (!(sint > T(0))) continue;
+                    th = atan2f(sint, costijk);
+                    dacos = T(-1) / sint;
+                }
+                const T rinvij = rinvijik * rik;
+                const T rinvik = rinvijik * rij;
+                const T rinvjk = narrow ? T(1) / rjk : T(0);
+                const T phiijik0 = rinvij * (rinvik - rinvij * costijk);
+                const T phiikij0 = rinvik * (rinvij - rinvik * costijk);
+                T P1 = T(0), P2 = T(0), P3 = T(0);  // force mode: sum_q w_q p_q
+                auto member = [&](int q) {
                     const int l = q0 + q;
                     const MemberT& mb = mem[l];
-                    double dradij;
-                    const double radij = radial_ij(mb, rij, dradij);
-                    if (member_skipped(mb, radij, rik, rjk, narrow, th)) continue;
-                    double radik, dradik, radjk = 1.0, dradjk = 0.0, ang, dang;
+                    T dradij;
+                    const T radij = radial_ij(mb, rij, dradij);
+                    if (member_skipped(mb, radij, rik, rjk, narrow, th)) return;
+                    T radik, dradik, radjk = T(1), dradjk = T(0), ang, dang;
                     radial_fdf(mb, rik, radik, dradik);
                     if (narrow) {
                         // r_jk factor: this member's, or the one the
@@ -267,28 +348,30 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                         int src = l;
                         if (mb.walk) {
                             while (!mem[src].calcComp) {
-                                double dp;
-                                const double rp = radial_ij(mem[src - 1], rij, dp);
+                                T dp;
+                                const T rp = radial_ij(mem[src - 1], rij, dp);
                                 if (member_skipped(mem[src - 1], rp, rik, rjk, true, th)) break;
                                 --src;
                             }
                         }
                         radial_fdf(mem[src], rjk, radjk, dradjk);
                     }
-                    compact_fdf(mb.core, 0, mb.acen, mb.ascl, th, ang, dang);
-                    double z = 1.0;
+                    compact_fdf(mb.core, 0, (T)mb.acen, (T)mb.ascl, th, ang, dang);
+                    T z = T(1);
                     if (wt) {
-                        z = M->Zel[L.el[j]] * M->Zel[L.el[k]];
+                        z = (T)(M->Zel[L.el[j]] * M->Zel[L.el[k]]);
                         ang *= z;
                     }
-                    double rad = narrow ? radij * radik * radjk : radij * radik;
-                    gacc[q] += rad * ang;
-                    if (!deriv) continue;
+                    T rad = narrow ? radij * radik * radjk : radij * radik;
+                    if constexpr (!FORCE) {
+                        gacc[q] += (double)(rad * ang);
+                        if (!PLAIN || !deriv) return;
+                    }
                     dang *= dacos;
-                    const double phiijik = phiijik0 * dang;
-                    const double phiikij = phiikij0 * dang;
-                    const double psiijik = rinvijik * dang;
-                    double chiij, chiik, chijk = 0.0;
+                    const T phiijik = phiijik0 * dang;
+                    const T phiikij = phiikij0 * dang;
+                    const T psiijik = rinvijik * dang;
+                    T chiij, chiik, chijk = T(0);
                     if (narrow) {
                         chiij = rinvij * dradij * radik * radjk;
                         chiik = rinvik * radij * dradik * radjk;
@@ -299,40 +382,68 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                         chiij = wt ? rinvij * dradij * radik : rinvij * radik * dradij;
                         chiik = rinvik * radij * dradik;
                     }
-                    double p1, p2, p3;
-                    if (dang != 0.0 && rad != 0.0) {
-                        if (wt) rad *= mb.sfac * z;
-                        else rad *= mb.sfac;
-                        ang *= mb.sfac;
+                    // plain: times the scaling factor here; force mode: it
+                    // is part of the weight w_q
+                    const T sf = PLAIN ? (T)mb.sfac : T(1);
+                    T p1, p2, p3;
+                    if (dang != T(0) && rad != T(0)) {
+                        if (wt) rad *= sf * z;
+                        else rad *= sf;
+                        ang *= sf;
                         p1 = rad * phiijik + ang * chiij;
                         p2 = rad * phiikij + ang * chiik;
                         p3 = narrow ? rad * psiijik + ang * chijk : rad * psiijik;
-                    } else if (ang != 0.0) {
-                        ang *= mb.sfac;
+                    } else if (ang != T(0)) {
+                        ang *= sf;
                         p1 = ang * chiij;
                         p2 = ang * chiik;
-                        p3 = narrow ? ang * chijk : 0.0;
+                        p3 = narrow ? ang * chijk : T(0);
                     } else {
-                        continue;
+                        return;
                     }
-                    for (int c = 0; c < 3; ++c) {
-                        const double p1c = p1 * dr1[c], p2c = p2 * dr2[c], p3c = p3 * dr3[c];
-                        double* acc = L.acc + (size_t)(q * 3 + c) * capL;
-                        lds_add(acc + j, -(p1c + p3c));
-                        lds_add(acc + k, -(p2c - p3c));
+                    if constexpr (FORCE) {
+                        const T w = wi[mb.sf];
+                        P1 += w * p1;
+                        P2 += w * p2;
+                        P3 += w * p3;
+                    } else {
+                        for (int cc = 0; cc < 3; ++cc) {
+                            const double p1c = p1 * dr1[cc], p2c = p2 * dr2[cc], p3c = p3 * dr3[cc];
+                            double* acc = L.acc + (size_t)(q * 3 + cc) * capL;
+                            lds_add(acc + j, -(p1c + p3c));
+                            lds_add(acc + k, -(p2c - p3c));
+                        }
+                    }
+                };
+                if constexpr (FORCE) {
+                    for (int q = 0; q < nq; ++q) member(q);
+                } else {
+#pragma unroll
+                    for (int q = 0; q < CH; ++q)
+                        if (q < nq) member(q);
+                }
+                if constexpr (FORCE) {
+                    // dG/dr contracted with dE/dG: entry j gets -(P1 dr1 + P3 dr3),
+                    // entry k -(P2 dr2 - P3 dr3), the atom minus their sum;
+                    // forces are minus those
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const T gj = -(P1 * dr1[cc] + P3 * dr3[cc]);
+                        const T gk = -(P2 * dr2[cc] - P3 * dr3[cc]);
+                        lds_add(L.acc + cc * capL + j, -(double)gj);
+                        lds_add(L.acc + cc * capL + k, -(double)gk);
+                        fself[cc] += (double)(gj + gk);
                     }
                 }
             }
-            // G of the chunk's members.
+            if constexpr (!FORCE) {
+                // G of the chunk's members.
 #pragma unroll
-            for (int q = 0; q < CH; ++q)
-                if (q < nq) block_sum_slot(gacc[q], L.red, q);
-            __syncthreads();
-            if (tid < nq) {
-                const MemberT& mb = mem[q0 + tid];
-                Gi[mb.sf] = scale_sf(M->scale[E.sf0 + mb.sf], red_total(L.red, tid));
+                for (int q = 0; q < CH; ++q)
+                    if (q < nq) block_sum_slot(gacc[q], L.red, q);
+                __syncthreads();
+                if (tid < nq) put(mem[q0 + tid], red_total(L.red, tid));
             }
-            if (deriv) {
+            if (PLAIN && deriv) {
                 // Neighbour entries of the chunk's slots (zero where no triplet).
                 for (int x = tid; x < n * nq; x += kCThreads) {
                     const int k = x / nq, q = x % nq;
@@ -346,20 +457,35 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp64(const ModelT* __restrict
                 }
                 // The atom's own dG/dr: minus the sum over its neighbour entries.
                 if (tid < nq * 3) {
-                    const int q = tid / 3, c = tid % 3;
+                    const int q = tid / 3, cc = tid % 3;
                     double v = 0.0;
-                    for (int k = 0; k < n; ++k) v += L.acc[(size_t)(q * 3 + c) * capL + k];
-                    dGi[mem[q0 + q].sf * 3 + c] = -v;
+                    for (int k = 0; k < n; ++k) v += L.acc[(size_t)(q * 3 + cc) * capL + k];
+                    dGi[mem[q0 + q].sf * 3 + cc] = -v;
                 }
             }
             __syncthreads();
         }
     }
+
+    if constexpr (FORCE) {
+        // the atom's own force, then its neighbours' (the pull form drops
+        // the atom's own periodic images, Mode.cpp:1957)
+        for (int cc = 0; cc < 3; ++cc) block_sum_slot(fself[cc], L.red, cc);
+        __syncthreads();
+        if (tid < 3) atomicAdd(c.F + 3 * (size_t)i + tid, red_total(L.red, tid));
+        for (int k = tid; k < n; k += kCThreads) {
+            const int j = s.nb_j[ebase + k];
+            if (j == i && !c.push) continue;
+            atomicAdd(c.F + 3 * (size_t)j, L.acc[k]);
+            atomicAdd(c.F + 3 * (size_t)j + 1, L.acc[capL + k]);
+            atomicAdd(c.F + 3 * (size_t)j + 2, L.acc[2 * capL + k]);
+        }
+    }
 }
 
-size_t comp_lds(int capL, int ch)
+size_t comp_lds(int capL, int acc)
 {
-    return sizeof(double) * ((size_t)capL * 4 + kCWaves * kRedSlots + (size_t)ch * 3 * capL) +
+    return sizeof(double) * ((size_t)capL * 4 + kCWaves * kRedSlots + (size_t)acc * 3 * capL) +
            sizeof(int) * (size_t)capL * 3;
 }
 
@@ -377,12 +503,28 @@ void launch_sfcomp64(const ModelT* m, const ModelT& hm, const SysView& s, const 
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the FP64 compact SF kernel's LDS staging"};
     const dim3 grid(s.natoms), block(kCThreads);
     const int d = deriv ? 1 : 0;
+    const CompMix c{};
     switch (ch) {
-    case 8: hipLaunchKernelGGL(k_sfcomp64<8>, grid, block, lds, stream, m, s, o, d, capL); break;
-    case 4: hipLaunchKernelGGL(k_sfcomp64<4>, grid, block, lds, stream, m, s, o, d, capL); break;
-    case 2: hipLaunchKernelGGL(k_sfcomp64<2>, grid, block, lds, stream, m, s, o, d, capL); break;
-    default: hipLaunchKernelGGL(k_sfcomp64<1>, grid, block, lds, stream, m, s, o, d, capL); break;
+    case 8: hipLaunchKernelGGL((k_sfcomp<double, CM_PLAIN, 8>), grid, block, lds, stream, m, s, o, c, d, capL); break;
+    case 4: hipLaunchKernelGGL((k_sfcomp<double, CM_PLAIN, 4>), grid, block, lds, stream, m, s, o, c, d, capL); break;
+    case 2: hipLaunchKernelGGL((k_sfcomp<double, CM_PLAIN, 2>), grid, block, lds, stream, m, s, o, c, d, capL); break;
+    default: hipLaunchKernelGGL((k_sfcomp<double, CM_PLAIN, 1>), grid, block, lds, stream, m, s, o, c, d, capL); break;
     }
+}
+
+void launch_sfcomp32(const ModelT* m, const ModelT& hm, const SysView& s, double* G, float* Gin, int ldG, int ldK,
+                     const float* fw, double* F, bool push, bool forces, hipStream_t stream)
+{
+    if (s.natoms == 0 || !hm.has_comp) return;
+    const int capL = s.stage_cap;
+    const size_t lds = comp_lds(capL, forces ? 1 : 0);
+    if (lds > 65536)
+        throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed compact SF kernel's LDS staging"};
+    const dim3 grid(s.natoms), block(kCThreads);
+    const SF64Out o{};
+    const CompMix c{G, Gin, ldG, ldK, fw, F, push ? 1 : 0};
+    if (forces) hipLaunchKernelGGL((k_sfcomp<float, CM_FORCE, 1>), grid, block, lds, stream, m, s, o, c, 0, capL);
+    else hipLaunchKernelGGL((k_sfcomp<float, CM_VALUES, 16>), grid, block, lds, stream, m, s, o, c, 0, capL);
 }
 
 }  // namespace nnpx

@@ -218,15 +218,75 @@ def test_autotune_keeps_results(water_dir):
         m.set_tuning(0, 256, 512, 16)
 
 
-def test_mixed_rejects_compact_functions():
-    """The mixed form has no compact-function kernels: a compact model is
-    refused with NNPX_ERR_UNSUPPORTED (and the plain form still runs)."""
-    say("case: mixed form on a compact model (DMABN_SCAN) is refused")
-    gdir = os.path.join(HERE, "golden", "DMABN_SCAN")
-    pos, elem, box = read_input_data(os.path.join(gdir, "input.data"), ["H", "C", "N"])
+@pytest.mark.parametrize("name,symbols", [("Anisole_SCAN", ["H", "C", "O"]), ("DMABN_SCAN", ["H", "C", "N"]),
+                                          ("Ethylbenzene_SCAN", ["H", "C"])])
+def test_golden_scan_mixed(name, symbols):
+    """The compact-function goldens (types 20 and 22, ~350 SFs per element,
+    polynomial cores, normalised units) in the mixed form: FP32 compact terms
+    (values kernel, then the contracted force kernel), the network input
+    wider than 128; forces against the reference's golden forces."""
+    import json
+    say("case: %s mixed" % name)
+    gdir = os.path.join(HERE, "golden", name)
+    pos, elem, box = read_input_data(os.path.join(gdir, "input.data"), symbols)
     m = n2p2_amd.Mode(gdir)
-    cl = m.conv_length
+    cl, ce = m.conv_length, m.conv_energy
     m.set_structures(remap(pos, box) * cl, elem, None if box is None else (box * cl)[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    F = m.forces() * (1.0 / (ce / cl))
+    exp = json.load(open(os.path.join(gdir, "expected.json")))
+    Fg = np.array(exp["forces"])
+    rms = float(np.sqrt(np.mean((F - Fg) ** 2)))
+    E1 = m.energies()[0]
+    m.evaluate(n2p2_amd.PLAIN_FP64, forces=False)
+    dE = abs(E1 - m.energies()[0]) / ce / len(elem)  # physical units per atom
+    say("  F rms %.2e max %.2e (|F| rms %.3e); |dE| per atom vs plain %.2e"
+        % (rms, np.abs(F - Fg).max(), np.sqrt(np.mean(Fg ** 2)), dE))
+    assert rms <= F_RMS
+    assert dE <= 1e-4
+    # the values path alone (no forces) gives the same energy
+    m.evaluate(n2p2_amd.MIXED, forces=False)
+    assert m.energies()[0] == E1
+
+
+@pytest.mark.parametrize("nbox", [4, 5])
+def test_compact_model_mixed(nbox, tmp_path):
+    """A synthetic compact model with every subtype (types 20, 21, 22;
+    polynomial, asymmetric and exponential cores; narrow members sharing
+    (rl, rc)) on periodic water boxes, mixed form against the FP64 oracle,
+    pull and push forces."""
+    from oracle_lib import compact_model
+    d = compact_model(str(tmp_path / "compact"))
+    pos, elem, box = synth.water_box(nbox)
+    pos = remap(pos, box)
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    for mode, name in ((PULL, "pull"), (PUSH, "push")):
+        o.predict(mode, BRUTE)
+        Fo, Eo = o.forces(), o.energy()
+        m = n2p2_amd.Mode(d)
+        m.set_option(n2p2_amd.OPT_FORCE_MODE, mode)
+        m.set_structures(pos, elem, box[None])
+        m.evaluate(n2p2_amd.MIXED, forces=True)
+        F, E = m.forces(), m.energies()[0]
+        rms = float(np.sqrt(np.mean((F - Fo) ** 2)))
+        dE = abs(E - Eo) / len(elem)
+        say("  %d atoms, %s: E %.6e, |dE| per atom %.2e  F rms %.2e max %.2e (|F| rms %.3e)"
+            % (len(elem), name, Eo, dE, rms, np.abs(F - Fo).max(), np.sqrt(np.mean(Fo ** 2))))
+        assert rms <= F_RMS
+        assert dE <= 1e-4
+
+
+def test_mixed_rejects_weighted_gaussian(tmp_path):
+    """Element-weighted Gaussian groups (types 12, 13) have no mixed kernels:
+    such a model is refused with NNPX_ERR_UNSUPPORTED (and the plain form
+    still runs)."""
+    from oracle_lib import weighted_model
+    say("case: mixed form on a model with types 12/13 is refused")
+    d = weighted_model(str(tmp_path / "weighted"))
+    pos, elem, box = synth.water_box(4)
+    m = n2p2_amd.Mode(d)
+    m.set_structures(remap(pos, box), elem, box[None])
     with pytest.raises(n2p2_amd.NNPError) as ei:
         m.evaluate(n2p2_amd.MIXED, forces=True)
     say("  refused:", ei.value)
