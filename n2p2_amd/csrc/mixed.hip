@@ -170,7 +170,7 @@ __device__ inline float dist2(const float4& a, const float4& b)
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
-__device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage& S)
+__device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s, int i, int capL, const Stage& S)
 {
     const int lane = threadIdx.x;
     const StructT& St = s.st[s.sid[i]];
@@ -183,7 +183,8 @@ __device__ inline int stage_atom(const SysView& s, int i, int capL, const Stage&
         entry_vector(s, i, base + k, St, dr);
         const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
         S.g[k] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
-        S.c[k] = make_float4((float)(1.0 / d), 0.f, 0.f, 0.f);
+        // .w: the neighbour's atomic number, the weight of types 12 and 13
+        S.c[k] = make_float4((float)(1.0 / d), 0.f, 0.f, (float)M->Zel[s.elem[j]]);
         S.el[k] = (unsigned char)s.elem[j];
     }
     __syncthreads();
@@ -197,7 +198,7 @@ __device__ inline int make_list(const Stage& S, int n, int elem, float rc, unsig
     int cnt = 0;
     for (int b = 0; b < n; b += kWave) {
         const int k = b + lane;
-        const bool ok = k < n && S.el[k] == elem && S.g[k].w < rc;
+        const bool ok = k < n && (elem < 0 || S.el[k] == elem) && S.g[k].w < rc;
         const int incl = wave_incl_scan(ok ? 1 : 0);
         if (ok) out[cnt + incl - 1] = (unsigned short)k;
         cnt += __shfl(incl, kWave - 1, kWave);
@@ -224,7 +225,8 @@ __device__ inline void group_cutoffs(const Stage& S, int n, int e1, int e2, int 
 {
     for (int k = threadIdx.x; k < n; k += kWave) {
         float f = 0.f, df = 0.f;
-        if (S.el[k] == e1 || S.el[k] == e2) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, S.g[k].w, f, df);
+        if (e1 < 0 || S.el[k] == e1 || S.el[k] == e2)
+            cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, S.g[k].w, f, df);
         S.c[k].y = f;
         S.c[k].z = df;
     }
@@ -536,6 +538,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const int lane = threadIdx.x;
     const bool same = GT.e1 == GT.e2;
     const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
+    const bool wt = GT.type == 13;
     const int nA = make_list(S, n, GT.e1, g.rc, S.la);
     const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
     float acc[NBAS], cmp[NBAS], part[NBAS];
@@ -557,7 +560,8 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
         float fcjk = 1.f, dfcjk = 0.f;
         if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, __builtin_amdgcn_sqrtf(r2jk), fcjk, dfcjk);
         const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
-        const float pfc = cj.y * ck.y * fcjk;
+        // type 13: times Z_j Z_k (SymGrpExpAngnWeighted.cpp)
+        const float pfc = cj.y * ck.y * fcjk * (wt ? cj.w * ck.w : 1.f);
         const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
         const float E0 = fexp2(ce0 * r2sum) * pfc;
         const float E1 = NB1 > 0 ? fexp2(ce1 * r2sum) * pfc : 0.f;
@@ -643,7 +647,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
-    const int n = stage_atom(s, i, capL, S);
+    const int n = stage_atom(M, s, i, capL, S);
     double* Gi = G + (size_t)i * ldG;
     float* Ii = Gin + (size_t)i * ldK;
     for (int q = E.nsf + lane; q < ldK; q += kWave) Ii[q] = 0.f;
@@ -659,16 +663,19 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
         if (sf_compact(GT.type)) continue;  // the compact kernel's
-        if (GT.type == 2) {
-            // radial members in chunks of kRadChunk (any member count)
+        if (GT.type == 2 || GT.type == 12) {
+            // radial members in chunks of kRadChunk (any member count);
+            // type 12: every neighbour, each term times its atomic number
+            // (SymGrpExpRadWeighted.cpp)
+            const bool wt = GT.type == 12;
             for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
                 float acc[kRadChunk], cmp[kRadChunk];
 #pragma unroll
                 for (int q = 0; q < kRadChunk; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
                 for (int k = lane; k < n; k += kWave) {
                     const float r = S.g[k].w;
-                    if (S.el[k] != GT.e1 || !(r < g.rc)) continue;
-                    const float fc = S.c[k].y;
+                    if ((!wt && S.el[k] != GT.e1) || !(r < g.rc)) continue;
+                    const float fc = wt ? S.c[k].y * S.c[k].w : S.c[k].y;
 #pragma unroll
                     for (int q = 0; q < kRadChunk; ++q)
                         if (q0 + q < nmem) {
@@ -701,6 +708,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
         {
             const bool same = GT.e1 == GT.e2;
             const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
+            const bool wt = GT.type == 13;
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
             const GroupMixF gm = GM[E.grp0 + gi];
@@ -735,7 +743,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 dfcjk = 0.f;
                 if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
                 const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
-                const float lpfc = flog2(cj.y * ck.y * fcjk);
+                const float lpfc = flog2(cj.y * ck.y * fcjk * (wt ? cj.w * ck.w : 1.f));  // type 13: Z_j Z_k
                 const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
                 constexpr int MAXS = FAST ? 2 : 4;
                 float L[MAXS];
@@ -996,7 +1004,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
-    const int n = stage_atom(s, i, capL, S);
+    const int n = stage_atom(M, s, i, capL, S);
     for (int k = lane; k < n; k += kWave) { ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0; }
     const float* wi = fw + (size_t)i * ldG;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -1013,11 +1021,13 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
         if (sf_compact(GT.type)) continue;  // the compact kernel's
-        if (GT.type == 2) {
-            // radial: each lane owns its neighbours, no conflicts
+        if (GT.type == 2 || GT.type == 12) {
+            // radial: each lane owns its neighbours, no conflicts; type 12
+            // weights every neighbour by its atomic number
+            const bool wt = GT.type == 12;
             for (int k = lane; k < n; k += kWave) {
                 const float4 gk = S.g[k];
-                if (S.el[k] != GT.e1 || !(gk.w < g.rc)) continue;
+                if ((!wt && S.el[k] != GT.e1) || !(gk.w < g.rc)) continue;
                 const float4 ck = S.c[k];
                 float P = 0.f;
 #pragma unroll 2
@@ -1026,7 +1036,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                     const float ex = __expf(-mf[q].eta * dr * dr);
                     P += wi[mf[q].sf] * (ck.z - mf[q].twoEta * dr * ck.y) * ex;
                 }
-                P *= ck.x;
+                P *= wt ? ck.x * ck.w : ck.x;
                 const double vx = (double)(P * gk.x), vy = (double)(P * gk.y), vz = (double)(P * gk.z);
                 fx -= vx; fy -= vy; fz -= vz;
                 ax[k] += vx; ay[k] += vy; az[k] += vz;
@@ -1036,6 +1046,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         }
         const bool same = GT.e1 == GT.e2;
         const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
+        const bool wt = GT.type == 13;    // Z_j Z_k weights (SymGrpExpAngnWeighted.cpp)
         const int nA = make_list(S, n, GT.e1, g.rc, S.la);
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
@@ -1197,9 +1208,10 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             }
             C1 += C0; C2 += C0;
             if (!wide) C3 += C0;
-            const float P1 = pfc * (A * (rinvijik - cost * cj.x * cj.x) - C1) + B * pr1;
-            const float P2 = pfc * (A * (rinvijik - cost * ck.x * ck.x) - C2) + B * pr2;
-            const float P3 = pfc * (A * rinvijik + C3) - B * pr3;
+            const float zw = wt ? cj.w * ck.w : 1.f;
+            const float P1 = zw * (pfc * (A * (rinvijik - cost * cj.x * cj.x) - C1) + B * pr1);
+            const float P2 = zw * (pfc * (A * (rinvijik - cost * ck.x * ck.x) - C2) + B * pr2);
+            const float P3 = zw * (pfc * (A * rinvijik + C3) - B * pr3);
             const float x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
             const float p1x = P1 * gj.x, p1y = P1 * gj.y, p1z = P1 * gj.z;
             const float p2x = P2 * gk.x, p2y = P2 * gk.y, p2z = P2 * gk.z;
@@ -1259,7 +1271,7 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
-    const int n = stage_atom(s, i, capL, S);
+    const int n = stage_atom(M, s, i, capL, S);
     {
         const double ux = u[3 * (size_t)i], uy = u[3 * (size_t)i + 1], uz = u[3 * (size_t)i + 2];
         const size_t base = (size_t)i * s.cap;
@@ -1499,13 +1511,6 @@ void dispatch_angular(const MixedModel& mm, int maxm, F&& f)
     dispatch_members(mm.fastM, maxm, [&](auto m, auto fs) { f(m, fs, I0{}, I0{}); });
 }
 
-bool has_weighted_gaussian(const ModelT& t)
-{
-    for (int g = 0; g < t.ngroups; ++g)
-        if (t.grp[g].type == 12 || t.grp[g].type == 13) return true;
-    return false;
-}
-
 int pow2_members(const ModelT& t)
 {
     const int m = std::max(std::max(t.max_mem_ang, t.max_mem_rad), 1);
@@ -1536,7 +1541,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 f.rs = (float)M.rs;
                 f.lambda = (float)M.lambda;
                 f.zeta = (float)M.zeta;
-                f.gnorm = (G.type == 3 || G.type == 9) ? (float)M.gnorm : 1.f;
+                f.gnorm = (G.type == 3 || G.type == 9 || G.type == 13) ? (float)M.gnorm : 1.f;
                 f.twoEta = (float)(2.0 * M.eta);
                 f.zetaLambda = (float)M.zetaLambda;
                 f.etaL = (float)(M.eta * 1.4426950408889634);  // eta * log2(e)
@@ -1544,7 +1549,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 f.calcExp = M.calcExp;
                 f.sidx = 0;
                 f.beta = -1;
-                if (G.type == 3 || G.type == 9) {
+                if (G.type == 3 || G.type == 9 || G.type == 13) {
                     if (M.rs > 0.0) mm.hasRs = true;
                     int k = 0;
                     while (k < gm.nlam && gm.lam[k] != f.lambda) ++k;
@@ -1554,7 +1559,8 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                     }
                     f.sidx = k;
                 }
-                sffac[(size_t)e * kMaxSF + M.sf] = (float)((G.type == 3 || G.type == 9) ? M.factorNorm : M.sfac);
+                sffac[(size_t)e * kMaxSF + M.sf] =
+                    (float)((G.type == 3 || G.type == 9 || G.type == 13) ? M.factorNorm : M.sfac);
             }
             mm.maxS = std::max(mm.maxS, gm.nlam);
             // unused table slots repeat the first lambda (evaluated, never picked)
@@ -1568,7 +1574,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
         int cnt = -1;
         bool same = true;
         for (int g = 0; g < t.ngroups; ++g)
-            if (t.grp[g].type == 3 || t.grp[g].type == 9) {
+            if (t.grp[g].type == 3 || t.grp[g].type == 9 || t.grp[g].type == 13) {
                 if (cnt < 0) cnt = t.grp[g].nmem;
                 same = same && t.grp[g].nmem == cnt;
             }
@@ -1584,7 +1590,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
         std::vector<std::array<int, 2>> hnb(std::max(t.ngroups, 1));
         for (int g = 0; g < t.ngroups && ok; ++g) {
             const GroupT& G = t.grp[g];
-            if (G.type != 3 && G.type != 9) continue;
+            if (G.type != 3 && G.type != 9 && G.type != 13) continue;
             int nh = 0;
             int nb[2] = {0, 0};
             double eta[2] = {0.0, 0.0};
@@ -1622,7 +1628,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
             mm.powNB[1] = NB1;
             for (int g = 0; g < t.ngroups; ++g) {
                 const GroupT& G = t.grp[g];
-                if (G.type != 3 && G.type != 9) continue;
+                if (G.type != 3 && G.type != 9 && G.type != 13) continue;
                 GroupMixF& gm = gmix[g];
                 for (int h = 0; h < 2; ++h) {
                     gm.etaL[h] = (float)(heta[g][h] * 1.4426950408889634);
@@ -1711,9 +1717,6 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed)
 {
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
-    if (has_weighted_gaussian(t))
-        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate element-weighted Gaussian "
-                                          "symmetry functions (types 12, 13); use the plain FP64 form"};
     if (s.natoms == 0) return;
     const int capL = s.stage_cap;
     const int P = pow2_members(t);

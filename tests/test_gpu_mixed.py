@@ -277,22 +277,36 @@ def test_compact_model_mixed(nbox, tmp_path):
         assert dE <= 1e-4
 
 
-def test_mixed_rejects_weighted_gaussian(tmp_path):
-    """Element-weighted Gaussian groups (types 12, 13) have no mixed kernels:
-    such a model is refused with NNPX_ERR_UNSUPPORTED (and the plain form
-    still runs)."""
+@pytest.mark.parametrize("nbox", [4, 5])
+def test_weighted_model_mixed(nbox, tmp_path):
+    """Element-weighted types 12 and 13 (Gaussian, every neighbour element,
+    terms times Z_j and Z_j Z_k) interleaved with types 2/3 and the weighted
+    compact types 23/24/25, mixed form against the FP64 oracle, pull and
+    push forces.  The mixed training tangent still refuses weighted and
+    compact types (NNPX_ERR_UNSUPPORTED)."""
     from oracle_lib import weighted_model
-    say("case: mixed form on a model with types 12/13 is refused")
     d = weighted_model(str(tmp_path / "weighted"))
-    pos, elem, box = synth.water_box(4)
-    m = n2p2_amd.Mode(d)
-    m.set_structures(remap(pos, box), elem, box[None])
-    with pytest.raises(n2p2_amd.NNPError) as ei:
+    pos, elem, box = synth.water_box(nbox)
+    pos = remap(pos, box)
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    for mode, name in ((PULL, "pull"), (PUSH, "push")):
+        o.predict(mode, BRUTE)
+        Fo, Eo = o.forces(), o.energy()
+        m = n2p2_amd.Mode(d)
+        m.set_option(n2p2_amd.OPT_FORCE_MODE, mode)
+        m.set_structures(pos, elem, box[None])
         m.evaluate(n2p2_amd.MIXED, forces=True)
-    say("  refused:", ei.value)
+        F, E = m.forces(), m.energies()[0]
+        rms = float(np.sqrt(np.mean((F - Fo) ** 2)))
+        dE = abs(E - Eo) / len(elem)
+        say("  %d atoms, %s: E %.6e, |dE| per atom %.2e  F rms %.2e max %.2e (|F| rms %.3e)"
+            % (len(elem), name, Eo, dE, rms, np.abs(F - Fo).max(), np.sqrt(np.mean(Fo ** 2))))
+        assert rms <= F_RMS
+        assert dE <= 1e-4
+    with pytest.raises(n2p2_amd.NNPError) as ei:
+        m.loss_gradient([E], F, 0.5, n2p2_amd.MIXED)
     assert ei.value.status == 7  # NNPX_ERR_UNSUPPORTED
-    m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
-    assert np.isfinite(m.forces()).all()
 
 
 @pytest.mark.parametrize("remapped", [True, False])
@@ -354,3 +368,35 @@ def test_position_update_regrows_lists(water_dir, remapped):
         say("  precision %d: max |dF| %.2e  dE %.2e" % (precision, dF, abs(E1 - E2)))
         assert dF <= 1e-9 * max(np.abs(F2).max(), 1.0)
         assert abs(E1 - E2) <= 1e-9 * abs(E2)
+
+
+@pytest.mark.parametrize("name", ["Mono", "Dual", "Mixed"])
+def test_sf_known_answers_mixed(name, tmp_path):
+    """test/cpp/fixtures.h known answers of every SF type (2, 3, 9, 12, 13,
+    20-25) on the fixture clusters through the mixed form: G of atom 0 to
+    FP32 accuracy (relative 1e-5 of max(|G|, 1)), forces within the force
+    bar of the oracle's."""
+    import json
+    from oracle_lib import kat_model
+    k = json.load(open(os.path.join(HERE, "golden", "sf_kat.json")))["sets"][name]
+    say("case: SF known answers, mixed form, %s set (%d functions)" % (name, len(k["functions"])))
+    worst = worstF = 0.0
+    for q, row in enumerate(k["functions"]):
+        d = str(tmp_path / ("m%d" % q))
+        kat_model(d, k["elements"], [row["line"]])
+        pos = np.array(k["pos"], float)
+        elem = np.array(k["elem"], np.int32)
+        m = n2p2_amd.Mode(d)
+        m.set_structures(pos, elem, None)
+        m.evaluate(n2p2_amd.MIXED, forces=True)
+        g = m.G()[0, 0]
+        o = Oracle(d)
+        o.set_structure(pos, elem, None)
+        o.predict(PULL, BRUTE)
+        dF = float(np.abs(m.forces() - o.forces()).max())
+        err = abs(g - row["value"]) / max(abs(row["value"]), 1.0)
+        worst = max(worst, err)
+        worstF = max(worstF, dF)
+        assert err <= 1e-5, (row["line"], g, row["value"])
+        assert dF <= F_RMS, (row["line"], dF)
+    say("  max rel |G - KAT| %.2e, max |dF| %.2e" % (worst, worstF))
