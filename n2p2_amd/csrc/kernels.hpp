@@ -85,6 +85,10 @@ void launch_sfcomp64(const ModelT* m, const ModelT& hm, const SysView& s, const 
 // s.stage_cap neighbours (a longer list sets flags[0] bit 4).
 void launch_sfcomp32(const ModelT* m, const ModelT& hm, const SysView& s, double* G, float* Gin, int ldG, int ldK,
                      const float* fw, double* F, bool push, bool forces, hipStream_t stream);
+// Tangent Gdot = sum_a dG/dx_a . u_a of the compact groups (FP32 terms,
+// FP64 sums; the scaling factor included), written to Gdot [N][ldG].
+void launch_tangentcomp32(const ModelT* m, const ModelT& hm, const SysView& s, const double* u, double* Gdot,
+                          int ldG, hipStream_t stream);
 
 // FP64 atomic NNs over per-element atom lists; dE/dG by the adjoint.
 struct NNIO {

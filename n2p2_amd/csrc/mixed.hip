@@ -1298,16 +1298,17 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
         if (sf_compact(GT.type)) continue;  // the compact kernel's
-        if (GT.type == 2) {
+        if (GT.type == 2 || GT.type == 12) {
+            const bool wt = GT.type == 12;  // every neighbour, times its atomic number
             for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
                 float acc[kRadChunk], cmp[kRadChunk];
 #pragma unroll
                 for (int q = 0; q < kRadChunk; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
                 for (int k = lane; k < n; k += kWave) {
                     const float4 gk = S.g[k];
-                    if (S.el[k] != GT.e1 || !(gk.w < g.rc)) continue;
+                    if ((!wt && S.el[k] != GT.e1) || !(gk.w < g.rc)) continue;
                     const float4 ck = S.c[k];
-                    const float sk = du[k].w * ck.x;  // (r_ij / r).(u_i - u_j)
+                    const float sk = du[k].w * (wt ? ck.x * ck.w : ck.x);  // (r_ij / r).(u_i - u_j)
 #pragma unroll
                     for (int q = 0; q < kRadChunk; ++q)
                         if (q0 + q < nmem) {
@@ -1334,6 +1335,7 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
         for (int q = 0; q < MAXM; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
         const bool same = GT.e1 == GT.e2;
         const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
+        const bool wt = GT.type == 13;    // Z_j Z_k weights (SymGrpExpAngnWeighted.cpp)
         const int nA = make_list(S, n, GT.e1, g.rc, S.la);
         const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
@@ -1351,11 +1353,13 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
             if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
             const float rinvijik = cj.x * ck.x;
             const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * rinvijik;
-            const float pfc = cj.y * ck.y * fcjk;
+            // type 13: every term and derivative times Z_j Z_k
+            const float zw = wt ? cj.w * ck.w : 1.f;
+            const float pfc = cj.y * ck.y * fcjk * zw;
             const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
-            const float pr1 = ck.y * fcjk * cj.z * cj.x;
-            const float pr2 = cj.y * fcjk * ck.z * ck.x;
-            const float pr3 = cj.y * ck.y * dfcjk * ijk;
+            const float pr1 = ck.y * fcjk * cj.z * cj.x * zw;
+            const float pr2 = cj.y * fcjk * ck.z * ck.x * zw;
+            const float pr3 = cj.y * ck.y * dfcjk * ijk * zw;
             // geometric scalars of the field
             const float s1 = uj.w, s2 = uk.w;
             const float s3 = (gk.x - gj.x) * (uj.x - uk.x) + (gk.y - gj.y) * (uj.y - uk.y) +
@@ -1791,11 +1795,6 @@ void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_mod
                       const double* u, double* Gdot, int ldG, hipStream_t stream)
 {
     if (!mm.ready) throw Error{NNPX_ERR_STATE, "mixed-precision tables not set up"};
-    if (t.has_comp || t.has_weighted)
-        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form does not evaluate compact or element-weighted "
-                                          "symmetry functions (types 12, 13, 20-25); use the plain FP64 form"};
-    if (t.max_width > kMaxWidth)
-        throw Error{NNPX_ERR_UNSUPPORTED, "the mixed form takes at most 128 symmetry functions per element"};
     if (s.natoms == 0) return;
     const int capL = s.stage_cap;
     const int tt = mm.tune.trip_tangent;
@@ -1805,6 +1804,7 @@ void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_mod
         hipLaunchKernelGGL((k_tangent32<decltype(m)::value, decltype(fast)::value>), dim3(s.natoms), dim3(kWave), lds,
                            stream, d_model, mm.d_mem, mm.d_gmix, s, u, mm.d_sffac, Gdot, ldG, capL, tt);
     });
+    launch_tangentcomp32(d_model, t, s, u, Gdot, ldG, stream);
 }
 
 double mfma_selftest()

@@ -13,7 +13,10 @@
 //    form's network input;
 //  * force (FP32 terms): the terms' derivatives contracted with dE/dG on the
 //    fly (members summed per triplet in FP32, neighbours accumulated in LDS
-//    FP64) and added to the forces, the mixed form's fused backward.
+//    FP64) and added to the forces, the mixed form's fused backward;
+//  * tangent (FP32 terms, FP64 sums): Gdot = sum_a dG/dx_a . u_a for a field
+//    u, the mixed training gradient's tangent (Training::collectDGdxia
+//    contracted with u, Training.cpp:3586-3625).
 //
 // Mapping: one 256-thread workgroup per atom with the neighbour geometry
 // staged in LDS.  Radial groups: threads over neighbours, members in
@@ -80,7 +83,7 @@ __device__ inline int build_list(const CLds& L, int n, int elem, double rmin, do
     return c;
 }
 
-enum CompMode { CM_PLAIN = 0, CM_VALUES = 1, CM_FORCE = 2 };
+enum CompMode { CM_PLAIN = 0, CM_VALUES = 1, CM_FORCE = 2, CM_TANGENT = 3 };
 
 // Member windows and centres in the term precision.
 template <typename T>
@@ -116,15 +119,18 @@ struct CompMix {
     const float* fw;  // [N*ldG] dE/dG times the scaling factor (force mode)
     double* F;        // forces (force mode)
     int push;         // keep contributions to the atom's own images
+    const double* u;  // [3N] field of the tangent mode (its output goes to G)
 };
 
 template <typename T, int MODE, int CH>
 __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__ M, SysView s, SF64Out o,
                                                        CompMix c, int deriv, int capL)
 {
-    constexpr bool PLAIN = MODE == CM_PLAIN, FORCE = MODE == CM_FORCE;
+    constexpr bool PLAIN = MODE == CM_PLAIN, FORCE = MODE == CM_FORCE, TANGENT = MODE == CM_TANGENT;
+    constexpr bool DER = PLAIN || TANGENT;  // derivatives times the scaling factor
     static_assert(CH <= kRedSlots, "a member chunk needs one reduction slot per member");
-    constexpr int ACC = PLAIN ? CH : FORCE ? 1 : 0;  // [3][cap] accumulator sets in LDS
+    // [3][cap] sets in LDS: plain dG/dr accumulators, force sums, tangent field
+    constexpr int ACC = PLAIN ? CH : (FORCE || TANGENT) ? 1 : 0;
     extern __shared__ double smem[];
     CLds L;
     L.dr = smem;
@@ -158,6 +164,9 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
             L.acc[capL + k] = 0.0;
             L.acc[2 * capL + k] = 0.0;
         }
+        if constexpr (TANGENT) {  // u of the neighbour minus u of the atom
+            for (int cc = 0; cc < 3; ++cc) L.acc[cc * capL + k] = c.u[3 * (size_t)j + cc] - c.u[3 * (size_t)i + cc];
+        }
     }
     __syncthreads();
 
@@ -168,8 +177,12 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
     const float* wi = c.fw + (size_t)i * c.ldG;
     double fself[3] = {0.0, 0.0, 0.0};  // force mode: this thread's part of the atom's own force
 
-    // Writes member mb's value v (values and plain modes).
+    // Writes member mb's value v (values and plain modes) or tangent.
     auto put = [&](const MemberT& mb, double v) {
+        if (TANGENT) {
+            Gi[mb.sf] = v;
+            return;
+        }
         const double gs = scale_sf(M->scale[E.sf0 + mb.sf], v);
         Gi[mb.sf] = gs;
         if (MODE == CM_VALUES) Ii[mb.sf] = (float)gs;
@@ -217,15 +230,19 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                     T rad, drad;
                     radial_fdf(mb, rij, rad, drad);
                     const T Z = (T)M->Zel[L.el[k]];
-                    if (wt) gs += (double)(Z * rad);
-                    else gs += (double)rad;
-                    if (PLAIN && drad != T(0)) {
+                    if (!TANGENT) {
+                        if (wt) gs += (double)(Z * rad);
+                        else gs += (double)rad;
+                    }
+                    if (DER && drad != T(0)) {
                         if (wt) drad *= Z * mb.sfac / rij;
                         else drad *= mb.sfac / rij;
                         px = drad * L.dr[k];
                         py = drad * L.dr[capL + k];
                         pz = drad * L.dr[2 * capL + k];
                         sx += px; sy += py; sz += pz;
+                        // tangent: entry k's dG/dr is -p, times u_k - u_i
+                        if (TANGENT) gs -= px * L.acc[k] + py * L.acc[capL + k] + pz * L.acc[2 * capL + k];
                     }
                 }
                 if (PLAIN && deriv) {
@@ -334,6 +351,16 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                 const T phiijik0 = rinvij * (rinvik - rinvij * costijk);
                 const T phiikij0 = rinvik * (rinvij - rinvik * costijk);
                 T P1 = T(0), P2 = T(0), P3 = T(0);  // force mode: sum_q w_q p_q
+                // tangent mode: dr1.w_j, dr2.w_k, dr3.(w_j - w_k) with w = u_nbr - u_i
+                T a1 = T(0), a2 = T(0), a3 = T(0);
+                if constexpr (TANGENT) {
+                    for (int cc = 0; cc < 3; ++cc) {
+                        const T wj = (T)L.acc[cc * capL + j], wk = (T)L.acc[cc * capL + k];
+                        a1 += dr1[cc] * wj;
+                        a2 += dr2[cc] * wk;
+                        a3 += dr3[cc] * (wj - wk);
+                    }
+                }
                 auto member = [&](int q) {
                     const int l = q0 + q;
                     const MemberT& mb = mem[l];
@@ -363,7 +390,7 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                         ang *= z;
                     }
                     T rad = narrow ? radij * radik * radjk : radij * radik;
-                    if constexpr (!FORCE) {
+                    if constexpr (!FORCE && !TANGENT) {
                         gacc[q] += (double)(rad * ang);
                         if (!PLAIN || !deriv) return;
                     }
@@ -384,7 +411,7 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                     }
                     // plain: times the scaling factor here; force mode: it
                     // is part of the weight w_q
-                    const T sf = PLAIN ? (T)mb.sfac : T(1);
+                    const T sf = DER ? (T)mb.sfac : T(1);
                     T p1, p2, p3;
                     if (dang != T(0) && rad != T(0)) {
                         if (wt) rad *= sf * z;
@@ -406,6 +433,9 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                         P1 += w * p1;
                         P2 += w * p2;
                         P3 += w * p3;
+                    } else if constexpr (TANGENT) {
+                        // entries j, k hold -(p1 dr1 + p3 dr3), -(p2 dr2 - p3 dr3)
+                        gacc[q] -= (double)(p1 * a1 + p2 * a2 + p3 * a3);
                     } else {
                         for (int cc = 0; cc < 3; ++cc) {
                             const double p1c = p1 * dr1[cc], p2c = p2 * dr2[cc], p3c = p3 * dr3[cc];
@@ -512,6 +542,20 @@ void launch_sfcomp64(const ModelT* m, const ModelT& hm, const SysView& s, const 
     }
 }
 
+void launch_tangentcomp32(const ModelT* m, const ModelT& hm, const SysView& s, const double* u, double* Gdot,
+                          int ldG, hipStream_t stream)
+{
+    if (s.natoms == 0 || !hm.has_comp) return;
+    const int capL = s.stage_cap;
+    const size_t lds = comp_lds(capL, 1);
+    if (lds > 65536)
+        throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed compact SF kernel's LDS staging"};
+    const SF64Out o{};
+    const CompMix c{Gdot, nullptr, ldG, 0, nullptr, nullptr, 0, u};
+    hipLaunchKernelGGL((k_sfcomp<float, CM_TANGENT, 16>), dim3(s.natoms), dim3(kCThreads), lds, stream, m, s, o, c, 0,
+                       capL);
+}
+
 void launch_sfcomp32(const ModelT* m, const ModelT& hm, const SysView& s, double* G, float* Gin, int ldG, int ldK,
                      const float* fw, double* F, bool push, bool forces, hipStream_t stream)
 {
@@ -522,7 +566,7 @@ void launch_sfcomp32(const ModelT* m, const ModelT& hm, const SysView& s, double
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed compact SF kernel's LDS staging"};
     const dim3 grid(s.natoms), block(kCThreads);
     const SF64Out o{};
-    const CompMix c{G, Gin, ldG, ldK, fw, F, push ? 1 : 0};
+    const CompMix c{G, Gin, ldG, ldK, fw, F, push ? 1 : 0, nullptr};
     if (forces) hipLaunchKernelGGL((k_sfcomp<float, CM_FORCE, 1>), grid, block, lds, stream, m, s, o, c, 0, capL);
     else hipLaunchKernelGGL((k_sfcomp<float, CM_VALUES, 16>), grid, block, lds, stream, m, s, o, c, 0, capL);
 }

@@ -282,8 +282,7 @@ def test_weighted_model_mixed(nbox, tmp_path):
     """Element-weighted types 12 and 13 (Gaussian, every neighbour element,
     terms times Z_j and Z_j Z_k) interleaved with types 2/3 and the weighted
     compact types 23/24/25, mixed form against the FP64 oracle, pull and
-    push forces.  The mixed training tangent still refuses weighted and
-    compact types (NNPX_ERR_UNSUPPORTED)."""
+    push forces."""
     from oracle_lib import weighted_model
     d = weighted_model(str(tmp_path / "weighted"))
     pos, elem, box = synth.water_box(nbox)
@@ -304,9 +303,6 @@ def test_weighted_model_mixed(nbox, tmp_path):
             % (len(elem), name, Eo, dE, rms, np.abs(F - Fo).max(), np.sqrt(np.mean(Fo ** 2))))
         assert rms <= F_RMS
         assert dE <= 1e-4
-    with pytest.raises(n2p2_amd.NNPError) as ei:
-        m.loss_gradient([E], F, 0.5, n2p2_amd.MIXED)
-    assert ei.value.status == 7  # NNPX_ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("remapped", [True, False])
