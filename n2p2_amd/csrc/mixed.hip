@@ -378,9 +378,11 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
 // a contiguous chunk of the buffer, so it sees runs of one first atom j: j's
 // geometry stays in registers over the run and the force kernel keeps j's
 // contribution there too.  `flush` runs after each buffer.
+// gB: the partners' geometry in list order (gB[b] = S.g[lb[b]]), so a
+// candidate's index and position load together.
 template <typename Work, typename Flush = NoFlush>
-__device__ inline void for_each_triplet_blocks(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work,
-                                               Flush&& flush = Flush{})
+__device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB, int nA, int nB, bool same, float rc2,
+                                               Work&& work, Flush&& flush = Flush{})
 {
     const int lane = threadIdx.x;
     const int R = same ? (nA - 1) / 2 : nB;          // regular columns per row
@@ -408,7 +410,7 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, int nA, int nB, b
                 int b = same ? a + c + 1 : c;
                 if (same && b >= nA) b -= nA;
                 const int k = lb[b];
-                r2 = dist2(ga, S.g[k]);
+                r2 = dist2(ga, gB[b]);
                 ok = r2 < rc2;
                 jk = (unsigned)ja | ((unsigned)k << 16);
             }
@@ -516,6 +518,13 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
     }
 }
 
+// Copies the partner list's geometry into list order for the block walk.
+__device__ inline void gather_partners(const Stage& S, const unsigned short* lb, int nB, float4* gB)
+{
+    for (int b = threadIdx.x; b < nB; b += kWave) gB[b] = S.g[lb[b]];
+    __syncthreads();
+}
+
 // ------------------------------------------------------------ power path --
 // Angular groups whose members are exp(-eta_h sum r^2) (1 + lambda_s cos)^zeta
 // with zeta = 2^b (b <= 4), at most two distinct eta and two lambda, rs = 0:
@@ -528,7 +537,7 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
 // triplet buffer folded into compensated sums, one FP64 wave reduction per
 // used term, then each member takes its term (times 2^(1-zeta)) and is scaled.
 template <int NB0, int NB1>
-__device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stage& S,
+__device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stage& S, float4* gB,
                                         const MemberF* __restrict__ mf, const GroupMixF& gm, const GroupT& GT,
                                         const GroupF& g, int n, int ctype, float tanhPre, int sf0,
                                         double* __restrict__ Gi, float* __restrict__ Ii)
@@ -541,6 +550,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const bool wt = GT.type == 13;
     const int nA = make_list(S, n, GT.e1, g.rc, S.la);
     const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+    gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
     float acc[NBAS], cmp[NBAS], part[NBAS];
 #pragma unroll
     for (int b = 0; b < NBAS; ++b) { acc[b] = 0.f; cmp[b] = 0.f; part[b] = 0.f; }
@@ -548,7 +558,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const float lam0 = gm.lam[0], lam1 = gm.lam[1];
     int jc = -1;  // j of the cached geometry (runs of one j per lane)
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    for_each_triplet_blocks(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
         if (j != jc) {
             gj = S.g[j];
             cj = S.c[j];
@@ -641,6 +651,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
+    float4* gB = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip, stage_mp(NB0, NB1)));
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
@@ -699,7 +710,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             continue;
         }
         if constexpr (NB0 > 0) {
-            sf32_power_group<NB0, NB1>(M, S, mf, GM[E.grp0 + gi], GT, g, n, ctype, tanhPre, E.sf0, Gi, Ii);
+            sf32_power_group<NB0, NB1>(M, S, gB, mf, GM[E.grp0 + gi], GT, g, n, ctype, tanhPre, E.sf0, Gi, Ii);
             continue;
         }
         float acc[MAXM], cmp[MAXM];
@@ -711,6 +722,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             const bool wt = GT.type == 13;
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+            gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
             const GroupMixF gm = GM[E.grp0 + gi];
             const MemberMasks R = load_members(S, mf, nmem, nullptr);
             // FAST: zeta and -eta log2 e in registers for the whole group;
@@ -728,7 +740,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             }
             int jc = -1;  // j of the cached geometry (runs of one j per lane)
             float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-            for_each_triplet_blocks(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+            for_each_triplet_blocks(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
                 if (j != jc) {
                     gj = S.g[j];
                     cj = S.c[j];
@@ -1726,13 +1738,14 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     const int P = pow2_members(t);
     const int tsf = mm.tune.trip_sf, tf = mm.tune.trip_force;
     // LDS of the largest kernel instances (the member paths' constant area)
-    const size_t lds_sf_max = stage_bytes(capL, tsf, stage_mp(0, 0));
+    const size_t lds_sf_max = stage_bytes(capL, tsf, stage_mp(0, 0)) + sizeof(float4) * (size_t)capL;
     const size_t lds_f_max = stage_bytes(capL, tf, stage_mp(0, 0)) + sizeof(double) * 3 * (size_t)capL;
     if (lds_f_max > 65536 || lds_sf_max > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     timed("sf32", [&] {
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
-            const size_t lds_sf = stage_bytes(capL, tsf, stage_mp(decltype(nb0)::value, decltype(nb1)::value));
+            const size_t lds_sf = stage_bytes(capL, tsf, stage_mp(decltype(nb0)::value, decltype(nb1)::value)) +
+                                  sizeof(float4) * (size_t)capL;
             hipLaunchKernelGGL((k_sf32<decltype(m)::value, decltype(fast)::value, decltype(nb0)::value,
                                        decltype(nb1)::value>),
                                dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, mm.d_gmix, s, b.G,
