@@ -1260,8 +1260,8 @@ void weight_vjp(nnpx_ctx* c, int precision, const double* u, const double* cE, d
     }
     a.blk_off[t.nelem] = blocks;
     a.atom_off[t.nelem] = c->elem_off[t.nelem];
-    if (grad_lds_bytes(a, precision == NNPX_MIXED) > 65536)
-        fail(NNPX_ERR_UNSUPPORTED, "network too large for the gradient kernel's LDS");
+    if (grad_act_bytes(a, precision == NNPX_MIXED) > 65536)
+        fail(NNPX_ERR_UNSUPPORTED, "network too wide for the gradient kernel's LDS staging");
     timed(c, precision == NNPX_MIXED ? "grad32" : "grad64",
           [&] { launch_grad(c->d_model, a, blocks, precision == NNPX_MIXED, c->stream); });
 }

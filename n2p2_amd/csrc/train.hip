@@ -122,8 +122,10 @@ __global__ __launch_bounds__(64) void k_tangent64(const ModelT* __restrict__ M, 
 // Network forward with the tangent and one reverse pass for
 // Phi_i = cE_s E_i - dE_i/dG . Gdot_i; dPhi/dw accumulated per block in LDS
 // (connection order) and added to grad.  One wave per block, a.per_block
-// atoms of one element per block.
-template <typename T>
+// atoms of one element per block.  GATOM: networks whose connection gradient
+// does not fit in LDS add every atom's dPhi/dw straight into its structure's
+// row with FP64 global atomics (only the activations are staged in LDS).
+template <typename T, bool GATOM>
 __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradArgs a)
 {
     extern __shared__ double gsm[];
@@ -137,7 +139,7 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
     // f', f'', adot; then the two adjoint buffers of the current layer and
     // the two of the layer below.
     double* acc = gsm;
-    T* act = reinterpret_cast<T*>(acc + a.nconn_max);
+    T* act = reinterpret_cast<T*>(GATOM ? gsm : acc + a.nconn_max);
     T* H = act;                       // [nl][W]
     T* Hd = H + kMaxLayers * W;       // tangent activations
     T* D1 = Hd + kMaxLayers * W;
@@ -150,7 +152,8 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
     T* ab = hd2 + W;                  // d Phi / d a_l
     T* ad = ab + W;                   // d Phi / d adot_l
     const int nconn = a.nconn[e];
-    for (int x = lane; x < nconn; x += kWave) acc[x] = 0.0;
+    if (!GATOM)
+        for (int x = lane; x < nconn; x += kWave) acc[x] = 0.0;
     // connection-order offsets (NeuralNetwork::setConnections, NeuralNetwork.cpp:147-169)
     int cw[kMaxLayers], cb[kMaxLayers];
     {
@@ -172,7 +175,8 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
     for (int p = first; p < last; ++p) {
         const int i = a.atoms[p];
         const int si = a.sid[i];
-        if (a.ldgrad && si != cur) {
+        double* grow = a.grad + (a.ldgrad ? (size_t)si * a.ldgrad : 0) + a.coff[e];
+        if (!GATOM && a.ldgrad && si != cur) {
             if (cur >= 0) {
                 double* g = a.grad + (size_t)cur * a.ldgrad + a.coff[e];
                 for (int x = lane; x < nconn; x += kWave) {
@@ -229,9 +233,14 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
             const T* hdin = Hd + (l - 1) * W;
             for (int x = lane; x < nin * nout; x += kWave) {
                 const int j = x / nout, k = x - j * nout;
-                acc[cw[l] + x] += (double)(ab[k] * hin[j] + ad[k] * hdin[j]);
+                const double v = (double)(ab[k] * hin[j] + ad[k] * hdin[j]);
+                if (GATOM) atomicAdd(grow + cw[l] + x, v);
+                else acc[cw[l] + x] += v;
             }
-            for (int k = lane; k < nout; k += kWave) acc[cb[l] + k] += (double)ab[k];
+            for (int k = lane; k < nout; k += kWave) {
+                if (GATOM) atomicAdd(grow + cb[l] + k, (double)ab[k]);
+                else acc[cb[l] + k] += (double)ab[k];
+            }
             if (l > 1) {
                 const double* w = a.w + N.woff[l];  // [out][in]
                 for (int j = lane; j < nin; j += kWave) {
@@ -255,6 +264,7 @@ __global__ __launch_bounds__(64) void k_grad(const ModelT* __restrict__ M, GradA
             }
         }
     }
+    if (GATOM) return;
     __syncthreads();
     if (a.ldgrad && cur < 0) return;
     double* g = a.grad + (a.ldgrad ? (size_t)cur * a.ldgrad : 0) + a.coff[e];
@@ -279,19 +289,31 @@ void launch_tangent64(const ModelT* m, const SysView& s, const double* dGself, c
                        ldG);
 }
 
-size_t grad_lds_bytes(const GradArgs& a, bool fp32)
+size_t grad_act_bytes(const GradArgs& a, bool fp32)
 {
     const size_t t = fp32 ? sizeof(float) : sizeof(double);
-    return sizeof(double) * (size_t)a.nconn_max + t * ((size_t)5 * kMaxLayers * a.maxw + 6 * (size_t)a.maxw);
+    return t * ((size_t)5 * kMaxLayers * a.maxw + 6 * (size_t)a.maxw);
+}
+
+size_t grad_lds_bytes(const GradArgs& a, bool fp32)
+{
+    return sizeof(double) * (size_t)a.nconn_max + grad_act_bytes(a, fp32);
 }
 
 void launch_grad(const ModelT* m, const GradArgs& a, int blocks, bool fp32, hipStream_t stream)
 {
     if (blocks == 0) return;
-    const size_t lds = grad_lds_bytes(a, fp32);
+    size_t lds = grad_lds_bytes(a, fp32);
+    const bool gatom = lds > 65536;
+    if (gatom) lds = grad_act_bytes(a, fp32);
     if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "network too wide for the gradient kernel's LDS staging"};
-    if (fp32) hipLaunchKernelGGL(k_grad<float>, dim3(blocks), dim3(kWave), lds, stream, m, a);
-    else hipLaunchKernelGGL(k_grad<double>, dim3(blocks), dim3(kWave), lds, stream, m, a);
+    if (fp32) {
+        if (gatom) hipLaunchKernelGGL((k_grad<float, true>), dim3(blocks), dim3(kWave), lds, stream, m, a);
+        else hipLaunchKernelGGL((k_grad<float, false>), dim3(blocks), dim3(kWave), lds, stream, m, a);
+    } else {
+        if (gatom) hipLaunchKernelGGL((k_grad<double, true>), dim3(blocks), dim3(kWave), lds, stream, m, a);
+        else hipLaunchKernelGGL((k_grad<double, false>), dim3(blocks), dim3(kWave), lds, stream, m, a);
+    }
 }
 
 }  // namespace nnpx

@@ -36,7 +36,8 @@ void launch_residual(const double* F, const double* Fref, const double* E, const
                      int natoms, int nstruct, double wF, double* u, double* cE, double* loss, hipStream_t stream);
 void launch_tangent64(const ModelT* m, const SysView& s, const double* dGself, const double* dGnbr, int S,
                       const int* islot, const double* u, double* Gdot, int ldG, hipStream_t stream);
-size_t grad_lds_bytes(const GradArgs& a, bool fp32);
+size_t grad_lds_bytes(const GradArgs& a, bool fp32);  // connection gradient + activations
+size_t grad_act_bytes(const GradArgs& a, bool fp32);  // activations only (global-atomic variant)
 void launch_grad(const ModelT* m, const GradArgs& a, int blocks, bool fp32, hipStream_t stream);
 
 }  // namespace nnpx

@@ -363,17 +363,18 @@ def write_model(directory, symbols, sf_lines, hidden, activations, rows, seed=SE
     return all_w
 
 
-def water_model(directory, seed=SEED_WEIGHTS, cutoff_type="1 0.0", activations=("t", "t", "l")):
-    """The 50-25-25-1 water model, scaling statistics from the seeded 192-atom
-    box (computed with the cosine cutoff whatever ``cutoff_type`` says: they
-    are only the model's scaling constants)."""
+def water_model(directory, seed=SEED_WEIGHTS, cutoff_type="1 0.0", activations=("t", "t", "l"), hidden=(25, 25)):
+    """The 50-25-25-1 water model (hidden layer widths ``hidden``), scaling
+    statistics from the seeded 192-atom box (computed with the cosine cutoff
+    whatever ``cutoff_type`` says: they are only the model's scaling
+    constants)."""
     symbols = ("H", "O")
     lines = water_sf_lines()
     if os.path.exists(os.path.join(directory, "weights.008.data")):
         return directory
     pos, elem, box = water_box(4)
     rows = scaling_rows(sorted_sfs(lines, symbols), pos, elem, box)
-    write_model(directory, symbols, lines, (25, 25), activations, rows, seed=seed, cutoff_type=cutoff_type)
+    write_model(directory, symbols, lines, tuple(hidden), activations, rows, seed=seed, cutoff_type=cutoff_type)
     return directory
 
 

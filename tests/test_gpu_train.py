@@ -180,3 +180,49 @@ def test_weight_vjp_all_sf_types(kind, precision, tmp_path):
         say("  structure %d (%d atoms): max rel %.2e" % (s, b - a, rel))
         worst = max(worst, rel)
     assert worst <= (1e-9 if precision == n2p2_amd.PLAIN_FP64 else 2e-3)
+
+
+
+@pytest.fixture(scope="module")
+def wide_case(tmp_path_factory):
+    """A 50-100-100-1 network on two 24-atom water boxes (6.21 A cells, self
+    images) and the oracle's Training::dPdc rows of both."""
+    d = synth.water_model(str(tmp_path_factory.mktemp("wide")), hidden=(100, 100))
+    p0, e0, b0 = synth.water_box(2)
+    p1, e1, b1 = synth.water_box(2, seed=synth.SEED_WATER + 7)
+    pos = np.vstack([remap(p0, b0), remap(p1, b1)])
+    elem = np.concatenate([e0, e1]).astype(np.int32)
+    off = np.array([0, len(e0), len(e0) + len(e1)])
+    boxes = [b0, b1]
+    refs = []
+    for s in range(2):
+        a, b = off[s], off[s + 1]
+        o = Oracle(d)
+        o.set_structure(pos[a:b], elem[a:b], boxes[s])
+        o.predict(PULL, BRUTE)
+        refs.append((o.dPdc("energy"), o.dPdc("force")))
+    return d, pos, elem, boxes, off, refs
+
+
+@pytest.mark.parametrize("precision", [n2p2_amd.PLAIN_FP64, n2p2_amd.MIXED])
+def test_weight_vjp_wide_network(wide_case, precision):
+    """15301 connections per element, more than the 64 KB of LDS holds as an
+    FP64 gradient: the gradient kernel's global-atomic variant, per-structure
+    rows vE dE/dw + vF . dF/dw against the oracle's Training::dPdc rows."""
+    d, pos, elem, boxes, off, refs = wide_case
+    say("case: weight VJP, 50-100-100-1 network, 2 x 24 atoms, precision %d" % precision)
+    rng = np.random.default_rng(41)
+    vE = rng.normal(size=2)
+    vF = rng.normal(size=3 * len(elem))
+    m = n2p2_amd.Mode(d)
+    m.set_structures(pos, elem, boxes, off)
+    assert m.num_connections() == 2 * 15301
+    g = m.weight_vjp(vE, vF, precision)
+    worst = 0.0
+    for s in range(2):
+        a, b = off[s], off[s + 1]
+        ref = vE[s] * refs[s][0] + vF[3 * a:3 * b] @ refs[s][1]
+        rel = float(np.abs(g[s] - ref).max() / np.abs(ref).max())
+        say("  structure %d (%d atoms): max rel %.2e" % (s, b - a, rel))
+        worst = max(worst, rel)
+    assert worst <= (1e-9 if precision == n2p2_amd.PLAIN_FP64 else 2e-3)
