@@ -378,9 +378,9 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
 // a contiguous chunk of the buffer, so it sees runs of one first atom j: j's
 // geometry stays in registers over the run and the force kernel keeps j's
 // contribution there too.  `flush` runs after each buffer.
-// gB: the partners' geometry in list order (gB[b] = S.g[lb[b]]), so a
-// candidate's index and position load together.
-template <typename Work, typename Flush = NoFlush>
+// GATHER: gB holds the partners' geometry in list order (gB[b] =
+// S.g[lb[b]]), so a candidate's index and position load together.
+template <bool GATHER, typename Work, typename Flush = NoFlush>
 __device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB, int nA, int nB, bool same, float rc2,
                                                Work&& work, Flush&& flush = Flush{})
 {
@@ -410,7 +410,7 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB,
                 int b = same ? a + c + 1 : c;
                 if (same && b >= nA) b -= nA;
                 const int k = lb[b];
-                r2 = dist2(ga, gB[b]);
+                r2 = dist2(ga, GATHER ? gB[b] : S.g[k]);
                 ok = r2 < rc2;
                 jk = (unsigned)ja | ((unsigned)k << 16);
             }
@@ -536,7 +536,7 @@ __device__ inline void gather_partners(const Stage& S, const unsigned short* lb,
 // SF values of one power-path group: FP32 partial sums per term over a
 // triplet buffer folded into compensated sums, one FP64 wave reduction per
 // used term, then each member takes its term (times 2^(1-zeta)) and is scaled.
-template <int NB0, int NB1>
+template <int NB0, int NB1, bool GATHER>
 __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stage& S, float4* gB,
                                         const MemberF* __restrict__ mf, const GroupMixF& gm, const GroupT& GT,
                                         const GroupF& g, int n, int ctype, float tanhPre, int sf0,
@@ -550,7 +550,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const bool wt = GT.type == 13;
     const int nA = make_list(S, n, GT.e1, g.rc, S.la);
     const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
-    gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
+    if constexpr (GATHER) gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
     float acc[NBAS], cmp[NBAS], part[NBAS];
 #pragma unroll
     for (int b = 0; b < NBAS; ++b) { acc[b] = 0.f; cmp[b] = 0.f; part[b] = 0.f; }
@@ -558,7 +558,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const float lam0 = gm.lam[0], lam1 = gm.lam[1];
     int jc = -1;  // j of the cached geometry (runs of one j per lane)
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    for_each_triplet_blocks(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    for_each_triplet_blocks<GATHER>(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
         if (j != jc) {
             gj = S.g[j];
             cj = S.c[j];
@@ -644,7 +644,7 @@ __device__ inline void power_weights(const Stage& S, const MemberF* __restrict__
 // FAST: every angular group has exactly MAXM members, at most two distinct
 // lambda and rs = 0 -- the member loop is then branch-free.  Otherwise the
 // general loop handles shifted Gaussians and up to four lambda values.
-template <int MAXM, bool FAST, int NB0, int NB1>
+template <int MAXM, bool FAST, int NB0, int NB1, bool GATHER>
 __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                              const GroupMixF* __restrict__ GM, SysView s, double* __restrict__ G,
                                              float* __restrict__ Gin, int ldG, int ldK, int capL, int trip)
@@ -710,7 +710,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             continue;
         }
         if constexpr (NB0 > 0) {
-            sf32_power_group<NB0, NB1>(M, S, gB, mf, GM[E.grp0 + gi], GT, g, n, ctype, tanhPre, E.sf0, Gi, Ii);
+            sf32_power_group<NB0, NB1, GATHER>(M, S, gB, mf, GM[E.grp0 + gi], GT, g, n, ctype, tanhPre, E.sf0, Gi, Ii);
             continue;
         }
         float acc[MAXM], cmp[MAXM];
@@ -722,7 +722,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             const bool wt = GT.type == 13;
             const int nA = make_list(S, n, GT.e1, g.rc, S.la);
             const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
-            gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
+            if constexpr (GATHER) gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
             const GroupMixF gm = GM[E.grp0 + gi];
             const MemberMasks R = load_members(S, mf, nmem, nullptr);
             // FAST: zeta and -eta log2 e in registers for the whole group;
@@ -740,7 +740,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             }
             int jc = -1;  // j of the cached geometry (runs of one j per lane)
             float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-            for_each_triplet_blocks(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+            for_each_triplet_blocks<GATHER>(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
                 if (j != jc) {
                     gj = S.g[j];
                     cj = S.c[j];
@@ -1738,18 +1738,34 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     const int P = pow2_members(t);
     const int tsf = mm.tune.trip_sf, tf = mm.tune.trip_force;
     // LDS of the largest kernel instances (the member paths' constant area)
-    const size_t lds_sf_max = stage_bytes(capL, tsf, stage_mp(0, 0)) + sizeof(float4) * (size_t)capL;
+    const size_t lds_sf_max = stage_bytes(capL, tsf, stage_mp(0, 0));
     const size_t lds_f_max = stage_bytes(capL, tf, stage_mp(0, 0)) + sizeof(double) * 3 * (size_t)capL;
     if (lds_f_max > 65536 || lds_sf_max > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     timed("sf32", [&] {
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
-            const size_t lds_sf = stage_bytes(capL, tsf, stage_mp(decltype(nb0)::value, decltype(nb1)::value)) +
-                                  sizeof(float4) * (size_t)capL;
-            hipLaunchKernelGGL((k_sf32<decltype(m)::value, decltype(fast)::value, decltype(nb0)::value,
-                                       decltype(nb1)::value>),
-                               dim3(s.natoms), dim3(kWave), lds_sf, stream, d_model, mm.d_mem, mm.d_gmix, s, b.G,
-                               b.Gin, b.ldG, mm.ldK, capL, tsf);
+            constexpr int cm = decltype(m)::value, c0 = decltype(nb0)::value, c1 = decltype(nb1)::value;
+            constexpr bool cf = decltype(fast)::value;
+            // the partner gather costs 16 B of LDS per slot: taken only when
+            // it leaves the blocks per CU unchanged
+            const size_t lds0 = stage_bytes(capL, tsf, stage_mp(c0, c1));
+            const size_t lds1 = lds0 + sizeof(float4) * (size_t)capL;
+            int occ0 = 0, occ1 = 0;
+            if (lds1 <= 65536) {
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ0, reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, false>), kWave, lds0) !=
+                        hipSuccess ||
+                    hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &occ1, reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, true>), kWave, lds1) !=
+                        hipSuccess)
+                    throw Error{NNPX_ERR_HIP, "hipOccupancyMaxActiveBlocksPerMultiprocessor"};
+            }
+            if (occ1 > 0 && occ1 >= occ0)
+                hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, true>), dim3(s.natoms), dim3(kWave), lds1, stream, d_model,
+                                   mm.d_mem, mm.d_gmix, s, b.G, b.Gin, b.ldG, mm.ldK, capL, tsf);
+            else
+                hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, false>), dim3(s.natoms), dim3(kWave), lds0, stream, d_model,
+                                   mm.d_mem, mm.d_gmix, s, b.G, b.Gin, b.ldG, mm.ldK, capL, tsf);
         });
         launch_sfcomp32(d_model, t, s, b.G, b.Gin, b.ldG, mm.ldK, nullptr, nullptr, false, false, stream);
     });
