@@ -58,9 +58,14 @@ typedef enum {
     NNPX_OPT_TIMING = 1,      /* 1: record HIP events around every kernel   */
     NNPX_OPT_NEIGHBOR_ALGO = 2, /* 0 auto (cell list where every axis holds
                                   >= 3 cells), 1 brute force, 2 cell list   */
-    NNPX_OPT_BF16_TERMS = 3     /* mixed form NN: 1 = plain bf16 operands,
+    NNPX_OPT_BF16_TERMS = 3,    /* mixed form NN: 1 = plain bf16 operands,
                                   3 = hi/lo split, three bf16 MFMA products
                                   (default 3)                              */
+    NNPX_OPT_SF_GATHER = 4      /* mixed SF kernel: copy each group's partner
+                                  geometry into list order; -1 auto (when
+                                  it costs no occupancy, default), 0 off,
+                                  1 on (where it fits in LDS); results are
+                                  identical either way                     */
 } nnpx_option;
 
 /* Flags for nnpx_evaluate. */

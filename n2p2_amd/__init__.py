@@ -24,6 +24,7 @@ OPT_FORCE_MODE = 0
 OPT_TIMING = 1
 OPT_NEIGHBOR_ALGO = 2
 OPT_BF16_TERMS = 3
+OPT_SF_GATHER = 4
 EVAL_FORCES = 1
 
 _STATUS = {0: "OK", 1: "ARGUMENT", 2: "FILE", 3: "MODEL", 4: "RANGE", 5: "STATE", 6: "HIP", 7: "UNSUPPORTED"}

@@ -1751,7 +1751,8 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
             const size_t lds0 = stage_bytes(capL, tsf, stage_mp(c0, c1));
             const size_t lds1 = lds0 + sizeof(float4) * (size_t)capL;
             int occ0 = 0, occ1 = 0;
-            if (lds1 <= 65536) {
+            const int want = mm.tune.sf_gather;
+            if (lds1 <= 65536 && want != 0) {
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
                         &occ0, reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, false>), kWave, lds0) !=
                         hipSuccess ||
@@ -1760,7 +1761,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                         hipSuccess)
                     throw Error{NNPX_ERR_HIP, "hipOccupancyMaxActiveBlocksPerMultiprocessor"};
             }
-            if (occ1 > 0 && occ1 >= occ0)
+            if (occ1 > 0 && (occ1 >= occ0 || want == 1))
                 hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, true>), dim3(s.natoms), dim3(kWave), lds1, stream, d_model,
                                    mm.d_mem, mm.d_gmix, s, b.G, b.Gin, b.ldG, mm.ldK, capL, tsf);
             else

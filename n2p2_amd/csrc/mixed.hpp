@@ -53,6 +53,7 @@ struct Tuning {
     int trip_force = 384;    // ... of the force kernel
     int trip_tangent = 384;  // ... of the tangent kernel
     int grad_atoms = 32;     // atoms per block of the gradient kernel
+    int sf_gather = -1;      // SF kernel partner gather: -1 auto, 0 off, 1 on
 };
 
 struct MixedModel {

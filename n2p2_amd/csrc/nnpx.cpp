@@ -642,6 +642,10 @@ nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
             c->mixed.terms = (int)value;
             invalidate(c);
             break;
+        case NNPX_OPT_SF_GATHER:
+            if (value < -1 || value > 1) fail(NNPX_ERR_ARGUMENT, "sf gather must be -1, 0 or 1");
+            c->mixed.tune.sf_gather = (int)value;
+            break;
         default: fail(NNPX_ERR_ARGUMENT, "unknown option");
         }
     });

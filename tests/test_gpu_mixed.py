@@ -396,3 +396,27 @@ def test_sf_known_answers_mixed(name, tmp_path):
         assert err <= 1e-5, (row["line"], g, row["value"])
         assert dF <= F_RMS, (row["line"], dF)
     say("  max rel |G - KAT| %.2e, max |dF| %.2e" % (worst, worstF))
+
+
+@pytest.mark.parametrize("kind", ["water", "weighted"])
+def test_sf_gather_identical(kind, water_dir, tmp_path):
+    """The SF kernel with and without the per-group copy of the partner
+    geometry in list order (forced on, forced off, automatic): the same
+    values loaded another way, so G agrees bit for bit; energies and forces
+    differ at most by the summation order of the force kernel's FP64 device
+    atomics (any two evaluations do)."""
+    from oracle_lib import weighted_model
+    d = water_dir if kind == "water" else weighted_model(str(tmp_path / "weighted"))
+    say("case: sf gather on/off/auto, %s model, water-192" % kind)
+    pos, elem, box = synth.water_box(4)
+    m = n2p2_amd.Mode(d)
+    m.set_structures(remap(pos, box), elem, box[None])
+    out = []
+    for g in (0, 1, -1):
+        m.set_option(n2p2_amd.OPT_SF_GATHER, g)
+        m.evaluate(n2p2_amd.MIXED, forces=True)
+        out.append((m.G().copy(), m.energies().copy(), m.forces().copy()))
+    for g, o in zip((1, -1), out[1:]):
+        assert np.array_equal(out[0][0], o[0]), "gather %d: G differs" % g
+        assert np.allclose(out[0][1], o[1], rtol=1e-13, atol=0.0), "gather %d: energies differ" % g
+        assert np.abs(out[0][2] - o[2]).max() <= 1e-13 * np.abs(o[2]).max(), "gather %d: forces differ" % g
