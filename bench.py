@@ -27,6 +27,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W]
 import argparse
 import json
 import os
+import subprocess
 import sys
 import tempfile
 import time
@@ -39,7 +40,7 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s (E+F) mixed vs FP64 GPU form at force RMS err <=1e-3 eV/A"
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X FP64 vector
 PEAK_FP32 = 157.3   # TFLOP/s, MI355X FP32 vector
-KERNELS = ("nbr", "rev", "sf64", "nn64", "energy", "force64", "sf32", "nnbf16", "force32")
+KERNELS = ("nbr", "rev", "sf64", "nn64", "energy", "force64", "force64f", "sf32", "nnbf16", "force32")
 NN_MACS = {"water": 50 * 25 + 25 * 25 + 25 * 1, "fcc": 80 * 30 + 30 * 30 + 30 * 1}
 
 
@@ -108,6 +109,33 @@ def time_form(m, form, steps, warmup):
     return dt, times
 
 
+def time_plain(m, n, steps, warmup):
+    """The plain FP64 form both ways: fused (derivatives recomputed in the
+    force kernel, nnpx_evaluate's default) and stored (dG/dr written in the
+    reference layout and contracted).  Returns the two rates (atom-steps/s),
+    the fused path's forces and energies and its kernel times."""
+    import n2p2_amd
+    m.set_option(n2p2_amd.OPT_PLAIN_FORCES, 1)
+    dt_s, _ = time_form(m, n2p2_amd.PLAIN_FP64, steps, warmup)
+    m.set_option(n2p2_amd.OPT_PLAIN_FORCES, 0)
+    dt_f, times = time_form(m, n2p2_amd.PLAIN_FP64, steps, warmup)
+    return n * steps / dt_f, n * steps / dt_s, m.forces(), m.energies(), {k: v[0] / v[1] for k, v in times.items()}
+
+
+def tune(m):
+    """Launch shapes of the mixed kernels from the autotuner, timed on the
+    resident structures before the measured region."""
+    return m.autotune()
+
+
+def plain_fields(out, value, rate_f, rate_s, times_p):
+    out["plain_fp64_atom_steps_per_s"] = rate_f
+    out["plain_fp64_stored_atom_steps_per_s"] = rate_s
+    out["form_over_plain"] = value / rate_f
+    out["form_over_plain_stored"] = value / rate_s
+    out["plain_kernel_ms_per_step"] = times_p
+
+
 def roofline(times, w, nn_macs, natoms, workload):
     """Roofline object of the dominant kernel: algorithmic FLOPs per launch
     over its average HIP-event duration.  The fused force kernel recomputes
@@ -128,7 +156,26 @@ def roofline(times, w, nn_macs, natoms, workload):
 
 
 def cpu_threads():
-    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    """OpenMP threads of the CPU baseline: OMP_NUM_THREADS where the job
+    environment sets it (the CPU share of a one-GPU job on the GPU machine),
+    else every CPU this process may run on."""
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+
+
+def host_cpu():
+    """Host CPU as lscpu reports it (model, sockets x cores per socket x
+    threads per core) and the CPUs this process may run on."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"), "threads_per_core": info.get("Thread(s) per core"),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 def cpu_checker(model_dir, cell=False):
@@ -136,52 +183,74 @@ def cpu_checker(model_dir, cell=False):
     groups, NN, force contraction compiled from n2p2's sources) when built,
     else the from-scratch FP64 oracle.  `cell` selects the oracle's
     set-identical cell list (for systems where the O(N^2) list is
-    infeasible)."""
+    infeasible).  Returns the checker and run(threads)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import BRUTE, CELL, Oracle, RefLib, ref_available
-    threads = cpu_threads()
-    os.environ["OMP_NUM_THREADS"] = str(threads)
     if ref_available() and not cell:
         c = RefLib(model_dir)
-        return c, c.predict, "reference", threads
+
+        def run(threads):
+            c.set_threads(threads)
+            c.predict()
+        return c, run, "reference"
     c = Oracle(model_dir)
-    return c, (lambda: c.predict(0, CELL if cell else BRUTE, threads)), "port", threads
+    return c, (lambda threads: c.predict(0, CELL if cell else BRUTE, threads)), "port"
 
 
-def cpu_baseline_single(model_dir, pos, elem, box, budget_s, cell=False, max_reps=20):
-    c, run, kind, threads = cpu_checker(model_dir, cell)
+def one_thread_rate(model_dir, pos, elem, box, cell):
+    """Atom-steps/s of one evaluation on one thread (after one untimed)."""
+    c, run, _ = cpu_checker(model_dir, cell)
+    c.set_structure(pos, elem, box)
+    run(1)
+    t0 = time.perf_counter()
+    run(1)
+    return len(elem) / (time.perf_counter() - t0)
+
+
+def cpu_baseline_single(model_dir, pos, elem, box, budget_s, cell=False, max_reps=20, one_thread=None):
+    """The multi-thread rate on the structure, and a one-thread rate on
+    `one_thread` = (pos, elem, box, label) or on the structure itself."""
+    threads = cpu_threads()
+    c, run, kind = cpu_checker(model_dir, cell)
     c.set_structure(pos, elem, box)
     t0 = time.perf_counter()
-    run()
+    run(threads)
     first = time.perf_counter() - t0
     F, E = c.forces(), c.energy()
     reps, tot, phases = 0, 0.0, np.zeros(5)
     while tot < budget_s and reps < max_reps:
         t0 = time.perf_counter()
-        run()
+        run(threads)
         tot += time.perf_counter() - t0
         phases += np.array(c.phase_times())
         reps += 1
     per, ph, n = tot / reps, phases / reps, len(elem)
+    p1, e1, b1, label = one_thread if one_thread else (pos, elem, box, "the same structure")
+    r1 = one_thread_rate(model_dir, p1, e1, b1, cell)
+    host = host_cpu()
     sample = ("%d evaluations of the %d-atom structure (first %.2f s untimed), %d OpenMP threads, %s "
               "neighbour list; per evaluation %.3f s: neighbour list %.3f, SF %.3f, NN %.3f, forces %.3f; "
-              "without the neighbour list %.1f atom-steps/s"
+              "without the neighbour list %.1f atom-steps/s; one thread (%s): %.1f atom-steps/s; host %s, "
+              "%s sockets x %s cores x %s threads, %d CPUs in this process's affinity"
               % (reps, n, first, threads, "cell" if cell else "O(N^2)", per, ph[0], ph[1], ph[2], ph[4],
-                 n / max(per - ph[0], 1e-12)))
-    return {"value": n / per, "unit": "atom-steps/s", "cores": threads, "kind": kind, "sample": sample}, E, F
+                 n / max(per - ph[0], 1e-12), label, r1, host["model"], host["sockets"], host["cores_per_socket"],
+                 host["threads_per_core"], host["affinity_cpus"]))
+    return {"value": n / per, "unit": "atom-steps/s", "cores": threads, "kind": kind, "sample": sample,
+            "one_thread": r1, "host": host}, E, F
 
 
 def cpu_baseline_batch(model_dir, pos, elem, boxes, off, budget_s):
     """Loop over structures as nnp-dataset does (nnp-dataset.cpp:210-215),
     until the budget is spent; returns the rate and the energies/forces of
     the structures it covered."""
-    c, run, kind, threads = cpu_checker(model_dir)
+    threads = cpu_threads()
+    c, run, kind = cpu_checker(model_dir)
     atoms, tot, done, E, F = 0, 0.0, 0, [], []
     for s in range(len(off) - 1):
         a, b = off[s], off[s + 1]
         c.set_structure(pos[a:b], elem[a:b], boxes[s])
         t0 = time.perf_counter()
-        run()
+        run(threads)
         tot += time.perf_counter() - t0
         atoms += b - a
         done += 1
@@ -189,10 +258,25 @@ def cpu_baseline_batch(model_dir, pos, elem, boxes, off, budget_s):
         F.append(c.forces())
         if tot >= budget_s:
             break
-    sample = ("first %d of %d structures (%d atoms), one evaluation each, %d OpenMP threads, %.2f s"
-              % (done, len(off) - 1, atoms, threads, tot))
-    return {"value": atoms / tot, "unit": "atom-steps/s", "cores": threads, "kind": kind, "sample": sample}, \
-        np.array(E), np.vstack(F), done
+    # one thread on the first structures, about a fifth of the budget
+    a1, t1 = 0, 0.0
+    for s in range(done):
+        a, b = off[s], off[s + 1]
+        c.set_structure(pos[a:b], elem[a:b], boxes[s])
+        t0 = time.perf_counter()
+        run(1)
+        t1 += time.perf_counter() - t0
+        a1 += b - a
+        if t1 >= budget_s / 5:
+            break
+    host = host_cpu()
+    sample = ("first %d of %d structures (%d atoms), one evaluation each, %d OpenMP threads, %.2f s; one thread: "
+              "%.1f atom-steps/s over the first %d atoms; host %s, %s sockets x %s cores x %s threads, %d CPUs in "
+              "this process's affinity" % (done, len(off) - 1, atoms, threads, tot, a1 / t1, a1, host["model"],
+                                           host["sockets"], host["cores_per_socket"], host["threads_per_core"],
+                                           host["affinity_cpus"]))
+    return {"value": atoms / tot, "unit": "atom-steps/s", "cores": threads, "kind": kind, "sample": sample,
+            "one_thread": a1 / t1, "host": host}, np.array(E), np.vstack(F), done
 
 
 def base_record(value, steps, warmup, ms_per_step, dtype, config, data):
@@ -215,19 +299,19 @@ def run_water(args):
     w = m.work_counts()
     form = n2p2_amd.MIXED if args.form == "mixed" else n2p2_amd.PLAIN_FP64
     plain_steps = max(5, args.steps // 5)
-    dt_plain, _ = time_form(m, n2p2_amd.PLAIN_FP64, plain_steps, 2)
-    F_plain, E_plain = m.forces(), m.energies()[0]
+    rate_f, rate_s, F_plain, E_plain, times_p = time_plain(m, n, plain_steps, 2)
+    E_plain = E_plain[0]
+    tuning = tune(m)
     dt, times = time_form(m, form, args.steps, args.warmup)
     F_form, E_form = m.forces(), m.energies()[0]
     value = n * args.steps / dt
-    plain_rate = n * plain_steps / dt_plain
     out = base_record(value, args.steps, args.warmup, dt / args.steps * 1e3, DTYPE[args.form],
                       {"workload": "water-12288: 4096 H2O, 12288 atoms, 49.68 A periodic cell, rc 6.35 A, "
                                    "50 SF/element, NN 50-25-25-1", "form": args.form, "atoms": n,
                        "pairs": w["pairs"], "triplets": w["triplets"]},
                       "synthetic (seeded water box, random-init weights)")
-    out["plain_fp64_atom_steps_per_s"] = plain_rate
-    out["form_over_plain"] = value / plain_rate
+    plain_fields(out, value, rate_f, rate_s, times_p)
+    out["tuning"] = tuning
     out["kernel_ms_per_step"] = {k: v[0] / v[1] for k, v in times.items()}
     out["roofline"] = roofline(times, w, NN_MACS["water"], n, "water-12288")
     if not args.no_cpu:
@@ -253,20 +337,19 @@ def run_batch(args):
     w = m.work_counts()
     form = n2p2_amd.MIXED if args.form == "mixed" else n2p2_amd.PLAIN_FP64
     plain_steps = max(3, args.steps // 5)
-    dt_plain, _ = time_form(m, n2p2_amd.PLAIN_FP64, plain_steps, 1)
-    F_plain, E_plain = m.forces(), m.energies()
+    rate_f, rate_s, F_plain, E_plain, times_p = time_plain(m, n, plain_steps, 1)
+    tuning = tune(m)
     dt, times = time_form(m, form, args.steps, args.warmup)
     F_form, E_form = m.forces(), m.energies()
     value = n * args.steps / dt
-    plain_rate = n * plain_steps / dt_plain
     out = base_record(value, args.steps, args.warmup, dt / args.steps * 1e3, DTYPE[args.form],
                       {"workload": "batch-512: 512 periodic structures, 16-512 atoms each (141,122 atoms), "
                                    "2 elements 1:2, density 0.1/A^3, rc 6.35 A, 50 SF/element, NN 50-25-25-1",
                        "form": args.form, "atoms": n, "structures": len(off) - 1, "pairs": w["pairs"],
                        "triplets": w["triplets"]},
                       "synthetic (seeded random structures, random-init weights)")
-    out["plain_fp64_atom_steps_per_s"] = plain_rate
-    out["form_over_plain"] = value / plain_rate
+    plain_fields(out, value, rate_f, rate_s, times_p)
+    out["tuning"] = tuning
     out["kernel_ms_per_step"] = {k: v[0] / v[1] for k, v in times.items()}
     out["roofline"] = roofline(times, w, NN_MACS["water"], n, "batch-512")
     if not args.no_cpu:
@@ -297,8 +380,12 @@ def run_md(args):
     m = n2p2_amd.Mode(mdir)
     m.set_structures(remap(pos, box), elem, box[None])
     w = m.work_counts()
+    # the plain form's full evaluation (fused and stored) of the initial positions
+    _, _, F_plain, E_plain, times_p = time_plain(m, n, 3, 1)
+    tuning = tune(m)
     # per-kernel durations of the same kernels in a full evaluation
     _, times = time_form(m, form, 5, 1)
+    F_form, E_form = m.forces(), m.energies()[0]
     m.md_init(mass, vel, 1.0, 1.0, form)
     m.md_run(graph, graph_steps=graph)   # capture and one warm-up replay
     m.synchronize()
@@ -309,25 +396,31 @@ def run_md(args):
     ep, ek, rebuilds = m.md_energies()
     et = ep + ek
     value = n * steps / dt
-    # the plain FP64 form: the same MD, eager, a few steps
-    mp = n2p2_amd.Mode(mdir)
-    mp.set_structures(remap(pos, box), elem, box[None])
-    mp.md_init(mass, vel, 1.0, 1.0, n2p2_amd.PLAIN_FP64)
-    mp.md_run(1)
-    mp.synchronize()
-    pst = max(3, steps // 20)
-    t0 = time.perf_counter()
-    mp.md_run(pst)
-    mp.synchronize()
-    plain_rate = n * pst / (time.perf_counter() - t0)
+    # the plain FP64 form: the same MD in the same HIP graphs, fused forces;
+    # then a few eager steps with the stored dG/dr
+    rates = []
+    for stored in (0, 1):
+        mp = n2p2_amd.Mode(mdir)
+        mp.set_option(n2p2_amd.OPT_PLAIN_FORCES, stored)
+        mp.set_structures(remap(pos, box), elem, box[None])
+        mp.md_init(mass, vel, 1.0, 1.0, n2p2_amd.PLAIN_FP64)
+        gs = 0 if stored else graph
+        pst = max(3, steps // 20) if stored else graph * max(1, steps // (5 * graph))
+        mp.md_run(gs or 1, graph_steps=gs)
+        mp.synchronize()
+        t0 = time.perf_counter()
+        mp.md_run(pst, graph_steps=gs)
+        mp.synchronize()
+        rates.append(n * pst / (time.perf_counter() - t0))
+        del mp
     out = base_record(value, steps, graph, dt / steps * 1e3, DTYPE[args.form],
                       {"workload": "fcc-md: 131072-atom 4-species FCC (32^3 cells, a 3.6 A), rc 6.0 A, 80 "
                                    "SF/element, NN 80-30-30-1; velocity Verlet dt 1 fs, 300 K, skin 1 A, "
                                    "HIP graphs of 10 steps", "form": args.form, "atoms": n,
                        "pairs": w["pairs"], "triplets": w["triplets"]},
                       "synthetic (seeded FCC lattice and velocities, random-init weights)")
-    out["plain_fp64_atom_steps_per_s"] = plain_rate
-    out["form_over_plain"] = value / plain_rate
+    plain_fields(out, value, rates[0], rates[1], times_p)
+    out["tuning"] = tuning
     out["md"] = {"rebuilds": int(rebuilds), "steps_total": int(len(ep)),
                  "etot_drift_eV": float(np.abs(et - et[0]).max()), "ekin_first": float(ek[0]),
                  "ekin_last": float(ek[-1])}
@@ -336,11 +429,62 @@ def run_md(args):
     if not args.no_cpu:
         # The reference's O(N^2) list (~4.6e11 pair tests) is infeasible at this
         # size: the oracle with its set-identical cell list, one evaluation per step.
-        cb, _, _ = cpu_baseline_single(mdir, pos, elem, box, args.cpu_budget, cell=True, max_reps=3)
+        # One thread: a 10^3-cell piece of the same crystal (4,000 atoms,
+        # identical per-atom work).
+        p1, e1, b1 = synth.fcc_lattice(n_cells=10)
+        cb, E_cpu, F_cpu = cpu_baseline_single(mdir, pos, elem, box, args.cpu_budget, cell=True, max_reps=3,
+                                               one_thread=(p1, e1, b1, "a 4,000-atom 10^3-cell piece"))
         out["cpu_baseline"] = cb
+        out["force_rms_err_vs_fp64_cpu"] = float(np.sqrt(np.mean((F_form - F_cpu) ** 2)))
+        out["energy_rel_err_vs_fp64_cpu"] = float(abs(E_form - E_cpu) / abs(E_cpu))
+        out["plain_force_max_err_vs_fp64_cpu"] = float(np.abs(F_plain - F_cpu).max())
+        out["plain_energy_rel_err_vs_fp64_cpu"] = float(abs(E_plain[0] - E_cpu) / abs(E_cpu))
     else:
         out["cpu_baseline"] = None
     return out
+
+
+def run_profile(args):
+    """Only the timed work of one form on one workload (after the autotuner
+    for the mixed form), for kernel-trace and counter runs: no plain/mixed
+    comparison and no CPU leg.  fcc-md runs its MD steps (HIP graphs of 10)."""
+    import n2p2_amd
+    from n2p2_amd import remap, synth
+    form = n2p2_amd.MIXED if args.form == "mixed" else n2p2_amd.PLAIN_FP64
+    if args.workload == "fcc-md":
+        mdir = synth.fcc_model(os.path.join(tempfile.gettempdir(), "n2p2_amd_fcc_model"))
+        pos, elem, box = synth.fcc_lattice(n_cells=32)
+        m = n2p2_amd.Mode(mdir)
+        m.set_option(n2p2_amd.OPT_PLAIN_FORCES, args.plain_forces)
+        m.set_structures(remap(pos, box), elem, box[None])
+        if form == n2p2_amd.MIXED:
+            m.autotune()
+        mass = np.array([synth.MASS[s] for s in synth.FCC_SYMBOLS])
+        m.md_init(mass, synth.maxwell_velocities(mass[elem], 300.0), 1.0, 1.0, form)
+        m.md_run(10, graph_steps=10)
+        m.synchronize()
+        t0 = time.perf_counter()
+        m.md_run(args.steps, graph_steps=10)
+        m.synchronize()
+        dt = time.perf_counter() - t0
+        n = len(elem)
+    else:
+        mdir = synth.water_model(os.path.join(tempfile.gettempdir(), "n2p2_amd_water_model"))
+        if args.workload == "water-12288":
+            pos, elem, box = synth.water_box(16)
+            boxes, off = box[None], None
+            pos = remap(pos, box)
+        else:
+            pos, elem, boxes, off = synth.random_batch()
+        m = n2p2_amd.Mode(mdir)
+        m.set_option(n2p2_amd.OPT_PLAIN_FORCES, args.plain_forces)
+        m.set_structures(pos, elem, boxes, off)
+        if form == n2p2_amd.MIXED:
+            m.autotune()
+        dt, _ = time_form(m, form, args.steps, args.warmup)
+        n = len(elem)
+    return {"profile": args.workload, "form": args.form, "plain_forces": args.plain_forces, "steps": args.steps,
+            "atom_steps_per_s": n * args.steps / dt}
 
 
 def main():
@@ -354,6 +498,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain copy of MIB MiB (counter calibration)")
+    ap.add_argument("--profile", action="store_true",
+                    help="only the timed work of --form on --workload (kernel traces, counters)")
+    ap.add_argument("--plain-forces", type=int, default=0, choices=[0, 1],
+                    help="plain form forces in --profile runs: 0 fused recompute, 1 stored dG/dr")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -365,6 +513,9 @@ def main():
     args.steps_given = args.steps is not None
     if args.steps is None:
         args.steps = 50 if args.workload == "water-12288" else 10
+    if args.profile:
+        print(json.dumps(run_profile(args)))
+        return
     run = {"water-12288": run_water, "batch-512": run_batch, "fcc-md": run_md}[args.workload]
     print(json.dumps(run(args)))
 

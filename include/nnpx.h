@@ -61,11 +61,19 @@ typedef enum {
     NNPX_OPT_BF16_TERMS = 3,    /* mixed form NN: 1 = plain bf16 operands,
                                   3 = hi/lo split, three bf16 MFMA products
                                   (default 3)                              */
-    NNPX_OPT_SF_GATHER = 4      /* mixed SF kernel: copy each group's partner
+    NNPX_OPT_SF_GATHER = 4,     /* mixed SF kernel: copy each group's partner
                                   geometry into list order; -1 auto (when
                                   it costs no occupancy, default), 0 off,
                                   1 on (where it fits in LDS); results are
                                   identical either way                     */
+    NNPX_OPT_PLAIN_FORCES = 5   /* forces of nnpx_evaluate in the plain form:
+                                  0 (default) recompute every SF derivative
+                                  in FP64 and contract it with dE/dG on the
+                                  fly (no dG/dr stored; lists too long for
+                                  the kernel's LDS fall back to 1); 1 store
+                                  dG/dr in the reference layout and contract
+                                  it (Mode::calculateForces).  The step calls
+                                  (nnpx_calculate_*) always store dG/dr.   */
 } nnpx_option;
 
 /* Flags for nnpx_evaluate. */
@@ -159,7 +167,9 @@ nnpx_status nnpx_calculate_energy(nnpx_ctx* ctx);
 nnpx_status nnpx_calculate_forces(nnpx_ctx* ctx);
 /* Mode::evaluateNNP (Mode.cpp:2029-2077) in either precision form:
  * neighbour list, SF, NN, energy and (flag NNPX_EVAL_FORCES) forces.
- * Asynchronous on the context's stream; getters synchronise. */
+ * The plain form's forces follow NNPX_OPT_PLAIN_FORCES (nnpx_get_dGdr needs
+ * the stored form).  Asynchronous on the context's stream; getters
+ * synchronise. */
 nnpx_status nnpx_evaluate(nnpx_ctx* ctx, int precision, unsigned flags);
 
 /* ---- results ---------------------------------------------------------- */
@@ -182,6 +192,29 @@ nnpx_status nnpx_get_neighbors(nnpx_ctx* ctx, int64_t* total, int* counts,
  * Either pointer may be NULL. */
 nnpx_status nnpx_get_dGdr(nnpx_ctx* ctx, double* self, int ld_self,
                           double* nbr, int ld_nbr);
+
+/* ---- checks of the SF driver ------------------------------------------ */
+/* Extrapolation warnings of the last evaluation's symmetry functions
+ * (Element::updateSymmetryFunctionStatistics, Element.cpp:462-541, run by
+ * Mode::calculateSymmetryFunctionGroups when enabled, Mode.cpp:1694-1702, as
+ * Prediction::setup and InterfaceLammps::initialize enable it): every
+ * (atom, SF) whose unscaled value (SymFnc::unscale) lies below Gmin - 1000 eps
+ * or above Gmax + 1000 eps of scaling.data.  *count receives their number;
+ * the first min(max, count) in (atom, SF) order are written: atom index,
+ * SF index (0-based, the element's sorted order), SF type, unscaled value,
+ * Gmin and Gmax (model units).  max = 0 counts only; the optional arrays
+ * may be NULL. */
+nnpx_status nnpx_extrapolation_warnings(nnpx_ctx* ctx, int64_t max, int64_t* count,
+                                        int* atom, int* sf, int* type,
+                                        double* value, double* gmin, double* gmax);
+/* Atoms with fewer neighbours within their element's smallest SF cutoff than
+ * their SFs need (1 radial, 2 angular: Element::getMinNeighbors /
+ * getMinCutoffRadius, Element.cpp:384-411) -- the check of
+ * Mode::calculateSymmetryFunctionGroups (Mode.cpp:1667-1679).  Same
+ * count/max convention; records are (atom index, neighbour count), in atom
+ * order. */
+nnpx_status nnpx_low_neighbor_atoms(nnpx_ctx* ctx, int64_t max, int64_t* count,
+                                    int* atom, int* num_neighbors);
 
 /* ---- training gradient ------------------------------------------------ */
 /* Connections (weights and biases) of element e's network, or of all
@@ -217,6 +250,37 @@ nnpx_status nnpx_loss_gradient(nnpx_ctx* ctx, int precision, const double* E_ref
  * supported. */
 nnpx_status nnpx_weight_vjp(nnpx_ctx* ctx, int precision, const double* vE,
                             const double* vF, double* grad);
+
+/* ---- training update (Kalman filter) ---------------------------------- */
+/* The weight updaters of Training with updater_type 1 (Training.cpp:
+ * 1040-1175; KalmanFilter constructor and setParametersStandard /
+ * setParametersFadingMemory, KalmanFilter.cpp:27-250): one filter over all
+ * connections (update_strategy 0) or one per element (1), P = I / epsilon
+ * in FP64 on the device.  type 0 (standard): par = {epsilon, q0, qtau,
+ * qmin, eta0, etatau, etamax}; type 1 (fading memory): {epsilon, q0, qtau,
+ * qmin, lambda, nu, unused}.  qtau and etatau are per update (the reference
+ * divides the input.nn values by the projected updates per epoch). */
+nnpx_status nnpx_kalman_init(nnpx_ctx* ctx, int type, int per_element, const double* par);
+/* One Training::update (Training.cpp:2256-2697) of the weights for the
+ * property "energy" (0) or "force" (1), for a batch of update candidates of
+ * the current structures chosen by the caller (the reference draws them by
+ * its selection mode): structure[b]; for forces also atom[b] (index in the
+ * packed structures) and component[b].  E_ref[n_struct] / F_ref[3N] in
+ * model units.  jacobian_full = 0 sums the batch into one observation
+ * (jacobian_mode 0, and 1 on one process), 1 keeps one per candidate
+ * (jacobian_mode 2).  Per-element filters scale each candidate's error by
+ * the element's share (atoms of the structure, or neighbours of the atom).
+ * Forces: error and Jacobian times force_weight.  The Jacobian rows come from
+ * the weight-derivative kernels in `precision`; the filter runs in FP64.
+ * Afterwards the model holds the new weights; error[b] (optional) receives
+ * each candidate's error before the update. */
+nnpx_status nnpx_kalman_update(nnpx_ctx* ctx, int precision, int property, int batch,
+                               const int* structure, const int* atom, const int* component,
+                               const double* E_ref, const double* F_ref, double force_weight,
+                               int jacobian_full, double* error);
+/* State of updater u: P[n*n] (optional) and scalars[7] = {n, first
+ * connection, eta, lambda, gamma, q, updates}. */
+nnpx_status nnpx_kalman_state(nnpx_ctx* ctx, int updater, double* P, double* scalars);
 
 /* ---- molecular dynamics ----------------------------------------------- */
 /* No counterpart in nnp::Mode: the velocity-Verlet loop a driver runs around

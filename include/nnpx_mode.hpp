@@ -32,6 +32,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <fstream>
+#include <ostream>
 #include <string>
 #include <vector>
 
@@ -156,6 +157,23 @@ public:
     void setForceMode(ForceMode m);
     void setCopyNeighbors(bool on) { copyNeighbors_ = on; }
     void setCopyDerivatives(bool on) { copyDerivatives_ = on; }
+    /* Mode::setupSymmetryFunctionStatistics (Mode.cpp:1172-1205): after each
+     * calculateSymmetryFunctionGroups, extrapolation warnings
+     * (Element::updateSymmetryFunctionStatistics, Element.cpp:462-541) are
+     * collected, written to stderr in the reference's format, or raised as
+     * std::out_of_range.  Collecting min/max/mean statistics is not provided
+     * (collectStatistics must be false). */
+    void setupSymmetryFunctionStatistics(bool collectStatistics, bool collectExtrapolationWarnings,
+                                         bool writeExtrapolationWarnings, bool stopOnExtrapolationWarnings);
+    /* Collected extrapolation warnings: count, the reference's summary lines
+     * (SymFncStatistics::getExtrapolationWarningLines format, one per
+     * warning) and reset (SymFncStatistics::resetExtrapolationWarnings). */
+    size_t getNumExtrapolationWarnings() const { return ewLines_.size(); }
+    std::vector<std::string> const& getExtrapolationWarningLines() const { return ewLines_; }
+    void resetExtrapolationWarnings() { ewLines_.clear(); }
+    /* Where log lines go (the low-neighbour warnings of
+     * calculateSymmetryFunctionGroups, Mode.cpp:1667-1679); none by default. */
+    void setLogStream(std::ostream* log) { log_ = log; }
 
     void calculateSymmetryFunctionGroups(Structure& structure, bool const derivatives);
     void calculateAtomicNeuralNetworks(Structure& structure, bool const derivatives, std::string id = "");
@@ -184,10 +202,12 @@ protected:
     double convLength_ = 1.0;
     double maxCutoffRadius_ = 0.0;
     std::vector<double> atomicEnergyOffset_;
+    /* Raises the exception n2p2 throws for a non-OK status. */
+    void check(nnpx_status st) const;
+    std::ostream* logStream() const { return log_; }
 
 private:
     void upload(Structure& s);
-    void check(nnpx_status st) const;
     void requireResident(Structure const& s) const;
 
     nnpx_ctx* ctx_ = nullptr;
@@ -198,6 +218,10 @@ private:
     Structure const* resident_ = nullptr;
     size_t residentAtoms_ = 0;
     bool mixedForces_ = false;
+    bool collectEW_ = false, writeEW_ = false, stopEW_ = false;
+    std::vector<std::string> ewLines_;
+    std::ostream* log_ = nullptr;
+    void checkStructure(Structure const& s);
 };
 
 class Prediction : public Mode {

@@ -25,6 +25,7 @@ OPT_TIMING = 1
 OPT_NEIGHBOR_ALGO = 2
 OPT_BF16_TERMS = 3
 OPT_SF_GATHER = 4
+OPT_PLAIN_FORCES = 5
 EVAL_FORCES = 1
 
 _STATUS = {0: "OK", 1: "ARGUMENT", 2: "FILE", 3: "MODEL", 4: "RANGE", 5: "STATE", 6: "HIP", 7: "UNSUPPORTED"}
@@ -87,6 +88,14 @@ _SIGS = {
     "nnpx_set_neighbors_external": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_get_weights": [ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_extrapolation_warnings": [ctypes.c_void_p, ctypes.c_int64, _c_i64_p, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_low_neighbor_atoms": [ctypes.c_void_p, ctypes.c_int64, _c_i64_p, ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_kalman_init": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p],
+    "nnpx_kalman_state": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p],
+    "nnpx_kalman_update": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                           ctypes.c_int, ctypes.c_void_p],
     "nnpx_set_weights": [ctypes.c_void_p, ctypes.c_void_p],
     "nnpx_loss_gradient": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                            ctypes.c_void_p, ctypes.c_void_p],
@@ -290,6 +299,71 @@ class Mode:
         nb = np.zeros((int(counts.sum()), ld_nbr, 3))
         self._ck(self._lib.nnpx_get_dGdr(self._h, _ptr(s), self.max_sf, _ptr(nb), ld_nbr))
         return s, nb
+
+    def extrapolation_warnings(self):
+        """Extrapolation warnings of the last evaluation
+        (Element::updateSymmetryFunctionStatistics): a structured array of
+        (atom, sf, type, value, gmin, gmax) in (atom, SF) order."""
+        n = ctypes.c_int64()
+        self._ck(self._lib.nnpx_extrapolation_warnings(self._h, 0, ctypes.byref(n), None, None, None, None, None,
+                                                         None))
+        k = n.value
+        a, q, t = (np.zeros(k, np.int32) for _ in range(3))
+        v, lo, hi = (np.zeros(k) for _ in range(3))
+        if k:
+            self._ck(self._lib.nnpx_extrapolation_warnings(self._h, k, ctypes.byref(n), _ptr(a), _ptr(q), _ptr(t),
+                                                             _ptr(v), _ptr(lo), _ptr(hi)))
+        out = np.zeros(k, dtype=[("atom", np.int32), ("sf", np.int32), ("type", np.int32), ("value", float),
+                                 ("gmin", float), ("gmax", float)])
+        out["atom"], out["sf"], out["type"], out["value"], out["gmin"], out["gmax"] = a, q, t, v, lo, hi
+        return out
+
+    def low_neighbor_atoms(self):
+        """(atom indices, neighbour counts) of the atoms with fewer neighbours
+        within their element's smallest SF cutoff than their SFs need."""
+        n = ctypes.c_int64()
+        self._ck(self._lib.nnpx_low_neighbor_atoms(self._h, 0, ctypes.byref(n), None, None))
+        a = np.zeros(n.value, np.int32)
+        c = np.zeros(n.value, np.int32)
+        if n.value:
+            self._ck(self._lib.nnpx_low_neighbor_atoms(self._h, n.value, ctypes.byref(n), _ptr(a), _ptr(c)))
+        return a, c
+
+    # -- Kalman filter training updates (Training::update) ------------------
+    def kalman_init(self, kalman_type=0, per_element=False, epsilon=1e-2, q0=0.0, qtau=0.0, qmin=0.0,
+                    eta=1e-2, etatau=0.0, etamax=1e-2, lam=0.99, nu=0.995):
+        """Kalman filter updaters: type 0 standard (eta, etatau, etamax), 1
+        fading memory (lambda, nu); one filter or one per element."""
+        par = np.array([epsilon, q0, qtau, qmin] + ([eta, etatau, etamax] if kalman_type == 0 else [lam, nu, 0.0]),
+                       np.float64)
+        self._ck(self._lib.nnpx_kalman_init(self._h, kalman_type, 1 if per_element else 0, _ptr(par)))
+
+    def kalman_state(self, updater=0, with_P=True):
+        sc = np.zeros(7)
+        self._ck(self._lib.nnpx_kalman_state(self._h, updater, None, _ptr(sc)))
+        n = int(sc[0])
+        P = None
+        if with_P:
+            P = np.zeros((n, n))
+            self._ck(self._lib.nnpx_kalman_state(self._h, updater, _ptr(P), _ptr(sc)))
+        return P, {"n": n, "offset": int(sc[1]), "eta": sc[2], "lambda": sc[3], "gamma": sc[4], "q": sc[5],
+                   "updates": int(sc[6])}
+
+    def kalman_update(self, prop, structures, E_ref=None, F_ref=None, atoms=None, components=None,
+                      force_weight=1.0, full=False, precision=PLAIN_FP64):
+        """One Training::update for "energy" or "force" with the given batch
+        of candidates; returns the candidates' errors before the update."""
+        b = len(structures)
+        st = np.ascontiguousarray(structures, np.int32)
+        at = None if atoms is None else np.ascontiguousarray(atoms, np.int32)
+        co = None if components is None else np.ascontiguousarray(components, np.int32)
+        er = None if E_ref is None else np.ascontiguousarray(E_ref, np.float64)
+        fr = None if F_ref is None else np.ascontiguousarray(F_ref, np.float64).reshape(-1)
+        err = np.zeros(b)
+        self._ck(self._lib.nnpx_kalman_update(self._h, precision, 0 if prop == "energy" else 1, b, _ptr(st), _ptr(at),
+                                              _ptr(co), _ptr(er), _ptr(fr), float(force_weight), 1 if full else 0,
+                                              _ptr(err)))
+        return err
 
     def kernel_time(self, name):
         ms, n = ctypes.c_double(), ctypes.c_int64()

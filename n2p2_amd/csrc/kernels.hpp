@@ -116,6 +116,36 @@ void launch_force64(const ModelT* m, const SysView& s, const double* dEdG, int l
                     const double* dGself, const double* dGnbr, int S, int push,
                     double* f, hipStream_t stream);
 
+// Plain-form forces without stored dG/dr: every SF term recomputed in FP64
+// and contracted with w = dE/dG times fac (the member's derivative scale per
+// element and SF, [kMaxElem*kMaxSF]); w [N*ldG] is scratch.  F [3*nforce] is
+// overwritten.  The compact groups go through their kernel's FP64 force mode.
+void launch_force64_fused(const ModelT* m, const ModelT& hm, const SysView& s, const double* dEdG,
+                          const double* fac, double* w, int ldG, double* F, bool push, hipStream_t stream);
+void launch_sfcomp64_force(const ModelT* m, const ModelT& hm, const SysView& s, const double* w, int ldG, double* F,
+                           bool push, hipStream_t stream);
+// LDS bytes of the fused FP64 force kernel at a list capacity (<= 65536 to run).
+size_t fused64_lds(int cap);
+
+// Extrapolation warnings (Element::updateSymmetryFunctionStatistics): every
+// (atom, SF) whose unscaled G lies outside [Gmin, Gmax] by more than 1000 eps
+// increments *count; the first cap are written (atom, SF, unscaled value).
+void launch_extrapolation(const ModelT* m, int natoms, const int* elem, const double* G, int ldG,
+                          unsigned long long* count, long long cap, int* atom, int* sf, double* val,
+                          hipStream_t stream);
+// Low-neighbour atoms (Mode.cpp:1667-1679): fewer than nmin[e] entries with
+// |dr| <= rmin[e]; the first cap are written (atom, count).
+void launch_low_neighbors(const ModelT* m, const SysView& s, const double* rmin, const int* nmin,
+                          unsigned long long* count, long long cap, int* atom, int* num, hipStream_t stream);
+
+// Kalman filter pieces (KalmanFilter::update): P = diag I; X = P H and
+// A = H^T X; then K = X Ai, P = (P - K X^T) scale + q I (symmetric, from the
+// lower triangle) and dw = K xi.  P [n][n], H/X/K [n][m], A/Ai [m][m].
+void kf_init(double* P, int n, double diag, hipStream_t stream);
+void kf_xa(const double* P, const double* H, int n, int m, double* X, double* A, hipStream_t stream);
+void kf_finish(double* P, const double* X, const double* Ai, const double* xi, int n, int m, double scale, double q,
+               double* K, double* dw, hipStream_t stream);
+
 // Algorithmic work counters (pairs, triplets, member evaluations).
 void launch_work_counts(const ModelT* m, const SysView& s, unsigned long long* out5,
                         hipStream_t stream);

@@ -17,6 +17,8 @@
 #include <cmath>
 #include <type_traits>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "mixed.hpp"
@@ -1729,6 +1731,22 @@ void free_mixed(MixedModel& mm)
     mm.tune = tune;
 }
 
+// Resident blocks per CU of a kernel instance at a dynamic LDS size, from the
+// HIP occupancy calculator, cached per (instance, size).
+static int blocks_per_cu(const void* kernel, size_t lds)
+{
+    static std::mutex mu;
+    static std::map<std::pair<const void*, size_t>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find({kernel, lds});
+    if (it != cache.end()) return it->second;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kWave, lds) != hipSuccess)
+        throw Error{NNPX_ERR_HIP, "hipOccupancyMaxActiveBlocksPerMultiprocessor"};
+    cache[{kernel, lds}] = occ;
+    return occ;
+}
+
 void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
                const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed)
 {
@@ -1753,13 +1771,8 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
             int occ0 = 0, occ1 = 0;
             const int want = mm.tune.sf_gather;
             if (lds1 <= 65536 && want != 0) {
-                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &occ0, reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, false>), kWave, lds0) !=
-                        hipSuccess ||
-                    hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                        &occ1, reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, true>), kWave, lds1) !=
-                        hipSuccess)
-                    throw Error{NNPX_ERR_HIP, "hipOccupancyMaxActiveBlocksPerMultiprocessor"};
+                occ0 = blocks_per_cu(reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, false>), lds0);
+                occ1 = blocks_per_cu(reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, true>), lds1);
             }
             if (occ1 > 0 && (occ1 >= occ0 || want == 1))
                 hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, true>), dim3(s.natoms), dim3(kWave), lds1, stream, d_model,

@@ -566,7 +566,8 @@ void load_model(const std::string& dir, HostModel& hm)
                 S.type = st;
                 S.Smin = Smin;
                 S.Gmin = num(w[2]);
-                double Gmax = num(w[3]);
+                const double Gmax = num(w[3]);
+                S.Gmax = Gmax;
                 S.Gmean = num(w[4]);
                 double Gsigma = w.size() > 5 ? num(w[5]) : 0.0;
                 if (st == ST_SCALE || st == ST_SCALECENTER) S.factor = (Smax - Smin) / (Gmax - S.Gmin);

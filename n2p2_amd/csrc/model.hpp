@@ -79,6 +79,7 @@ struct ScaleT {
     int type;
     int pad;
     double Smin, Gmin, Gmean, factor;
+    double Gmax;     // scaling.data maximum (extrapolation warnings)
 };
 
 struct NNT {

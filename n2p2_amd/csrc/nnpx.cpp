@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,7 +23,6 @@ using namespace nnpx;
 namespace {
 
 thread_local std::string g_create_error;
-
 struct Timer {
     std::string name;
     hipEvent_t a, b;
@@ -72,6 +72,8 @@ struct nnpx_ctx {
     bool sf_valid = false, sfd_valid = false, nn_valid = false, dedg_valid = false;
     bool e_valid = false, f_valid = false;
     int last_precision = NNPX_PLAIN_FP64;
+    bool last_stored = false;    // the last plain evaluation stored dG/dr
+    int plain_forces = 0;        // NNPX_OPT_PLAIN_FORCES: 0 fused recompute, 1 stored dG/dr
     // MD (velocity Verlet with a skin-buffered list)
     double skin = 0.0;           // extra list cutoff, model length units
     bool md_ready = false;
@@ -83,6 +85,15 @@ struct nnpx_ctx {
     // caller-provided neighbour lists (LAMMPS path): forces on ntotal atoms
     bool external = false;
     int ntotal = 0;
+    // Kalman filter weight updaters (one over all connections, or one per
+    // element): state size, first connection, scalars; P on the device
+    struct KF {
+        int n = 0, off = 0;
+        double eta = 0, etatau = 0, etamax = 0, lambda = 0, nu = 0, gamma = 1, q = 0, qtau = 0, qmin = 0;
+        int64_t updates = 0;
+    };
+    int kf_type = -1;  // -1: not set up
+    std::vector<KF> kf;
     // timing
     std::vector<Timer> pending;
     std::map<std::string, std::pair<double, int64_t>> ktime;
@@ -137,8 +148,8 @@ void drop_graph(nnpx_ctx* c)
 template <typename T>
 T* buf(nnpx_ctx* c, const std::string& name, size_t count)
 {
-    DevBuf& b = c->bufs[name];
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    DevBuf& b = c->bufs[name];
     if (b.bytes < bytes) {
         drop_graph(c);
         if (b.p) hip_check(hipFree(b.p), "hipFree");
@@ -148,6 +159,16 @@ T* buf(nnpx_ctx* c, const std::string& name, size_t count)
         b.bytes = bytes;
     }
     return static_cast<T*>(b.p);
+}
+
+// Model-level tables (set up with the model, read by the kernels of every
+// evaluation): always the context's own buffers.
+template <typename T>
+T* model_table(nnpx_ctx* c, const std::string& name)
+{
+    auto it = c->bufs.find(name);
+    if (it == c->bufs.end() || !it->second.p) fail(NNPX_ERR_STATE, "model table " + name + " missing");
+    return static_cast<T*>(it->second.p);
 }
 
 template <typename T>
@@ -170,16 +191,21 @@ hipEvent_t get_event(nnpx_ctx* c)
 }
 
 // Brackets one launcher call with events when timing is enabled.
+// Every launch group is checked: a rejected launch (a dynamic LDS size the
+// host guards missed, say) fails the call instead of leaving a previous
+// evaluation's results in the buffers.
 template <typename F>
 void timed(nnpx_ctx* c, const char* name, F&& f)
 {
     if (!c->timing) {
         f();
+        hip_check(hipGetLastError(), name);
         return;
     }
     Timer t{name, get_event(c), get_event(c)};
     hip_check(hipEventRecord(t.a, c->stream), "hipEventRecord");
     f();
+    hip_check(hipGetLastError(), name);
     hip_check(hipEventRecord(t.b, c->stream), "hipEventRecord");
     c->pending.push_back(t);
 }
@@ -465,6 +491,24 @@ void compute_nn64(nnpx_ctx* c, bool deriv)
     c->dedg_valid = deriv;
 }
 
+// Plain-form forces without stored dG/dr (the fused FP64 kernel); needs the
+// values G (compute_sf64 without derivatives) and dE/dG.
+void compute_force64_fused(nnpx_ctx* c)
+{
+    const ModelT& t = c->hm.t;
+    double* f = buf<double>(c, "F", (size_t)3 * force_entries(c));
+    double* w = buf<double>(c, "w64", (size_t)c->N * t.max_sf);
+    SysView s = sysview(c);
+    timed(c, "force64f", [&] {
+        launch_force64_fused(c->d_model, t, s, existing<double>(c, "dEdG"), model_table<double>(c, "fac64"), w, t.max_sf, f,
+                             c->external || c->force_mode == NNPX_FORCES_PUSH, c->stream);
+    });
+    c->f_valid = true;
+}
+
+// The fused FP64 force kernel stages the whole list in LDS.
+bool fused_fits(const nnpx_ctx* c) { return c->plain_forces == 0 && fused64_lds(c->cap) <= 65536; }
+
 void compute_energy(nnpx_ctx* c)
 {
     double* E = buf<double>(c, "E", c->S);
@@ -497,7 +541,9 @@ void compute_force64(nnpx_ctx* c)
     c->f_valid = true;
 }
 
-void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check)
+// store: the plain form keeps the stored dG/dr (the gradient kernels read it)
+// instead of the fused force kernel.
+void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check, bool store = false)
 {
     const bool forces = flags & NNPX_EVAL_FORCES;
     invalidate(c);
@@ -511,6 +557,9 @@ void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check)
         md_set_gate(gate, 0, c->stream);
     }
     build_neighbors(c, check);
+    // an external list has no build kernel to clear the flags the mixed
+    // kernels set
+    if (c->external) fill_int(existing<int>(c, "flags"), 0, 4, c->stream);
     if (precision == NNPX_MIXED) {
         MixedBuffers mb = mixed_buffers(c);
         run_mixed(c->mixed, c->hm.t, c->d_model, sysview(c), mb, forces,
@@ -522,10 +571,15 @@ void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check)
         c->e_valid = true;
         c->f_valid = forces;
     } else {
-        compute_sf64(c, forces);
+        const bool fused = forces && !store && fused_fits(c);
+        compute_sf64(c, forces && !fused);
         compute_nn64(c, true);
         compute_energy(c);
-        if (forces) compute_force64(c);
+        if (forces) {
+            if (fused) compute_force64_fused(c);
+            else compute_force64(c);
+        }
+        c->last_stored = forces && !fused;
     }
     c->last_precision = precision;
 }
@@ -539,13 +593,89 @@ void settle(nnpx_ctx* c)
     if (f[0] & 2) fail(NNPX_ERR_HIP, "neighbour list is not symmetric");
     if (f[0] & 1) {
         c->cap = ((f[1] + f[1] / 4 + 31) / 32) * 32;
-        evaluate(c, c->last_precision, c->f_valid ? NNPX_EVAL_FORCES : 0u, true);
+        evaluate(c, c->last_precision, c->f_valid ? NNPX_EVAL_FORCES : 0u, true, c->last_stored);
     } else if (f[0] & 4) {
         // an atom had more neighbours than the mixed kernels staged: the
         // checked rebuild refits the staging capacity to this list
-        evaluate(c, c->last_precision, c->f_valid ? NNPX_EVAL_FORCES : 0u, true);
+        evaluate(c, c->last_precision, c->f_valid ? NNPX_EVAL_FORCES : 0u, true, c->last_stored);
+    } else {
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        return;
     }
+    read_flags(c, f);
+    if (f[0] & 7) fail(NNPX_ERR_HIP, "neighbour list or staging overflow persists after a refit");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+// Structures of nnpx_set_structures (copied to device buffers; the first
+// neighbour build sizes the list capacity).
+void set_structures(nnpx_ctx* c, int ns, const int64_t* off, const double* pos, const int* elem, const double* box,
+                    const int* periodic)
+{
+    need_model(c);
+    if (ns < 1 || !off || !pos || !elem || !periodic) fail(NNPX_ERR_ARGUMENT, "null or empty structure input");
+    if (off[0] != 0) fail(NNPX_ERR_ARGUMENT, "atom_offsets[0] must be 0");
+    for (int s = 0; s < ns; ++s)
+        if (off[s + 1] < off[s]) fail(NNPX_ERR_ARGUMENT, "atom_offsets must be non-decreasing");
+    const int64_t N = off[ns];
+    if (N <= 0 || N > (int64_t)1 << 30) fail(NNPX_ERR_ARGUMENT, "bad atom count");
+    // a new structure set ends any MD set-up (skin, graph) and any
+    // caller-provided neighbour list
+    c->external = false;
+    c->ntotal = 0;
+    c->skin = 0.0;
+    c->md_ready = false;
+    drop_graph(c);
+    for (int64_t i = 0; i < N; ++i)
+        if (elem[i] < 0 || elem[i] >= c->hm.t.nelem) fail(NNPX_ERR_RANGE, "element index out of range");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    c->N = (int)N;
+    c->S = ns;
+    c->atom_off.assign(off, off + ns + 1);
+    c->elem_host.assign(elem, elem + N);
+    c->st_host.assign(ns, StructT{});
+    for (int s = 0; s < ns; ++s) {
+        if (periodic[s] && !box) fail(NNPX_ERR_ARGUMENT, "periodic structure without box");
+        setup_struct(c->st_host[s], periodic[s] ? box + 9 * s : nullptr, periodic[s] != 0, list_cutoff(c));
+        c->st_host[s].atom0 = (int)off[s];
+        c->st_host[s].natoms = (int)(off[s + 1] - off[s]);
+    }
+    choose_algorithms(c, pos);
+    std::vector<int> sid(N);
+    for (int s = 0; s < ns; ++s)
+        for (int64_t i = off[s]; i < off[s + 1]; ++i) sid[i] = s;
+    const int ne = c->hm.t.nelem;
+    c->elem_count.assign(ne, 0);
+    c->elem_off.assign(ne + 1, 0);
+    for (int64_t i = 0; i < N; ++i) c->elem_count[elem[i]]++;
+    for (int e = 0; e < ne; ++e) c->elem_off[e + 1] = c->elem_off[e] + c->elem_count[e];
+    std::vector<int> atoms(N);
+    {
+        std::vector<int> fillp(c->elem_off.begin(), c->elem_off.end() - 1);
+        for (int64_t i = 0; i < N; ++i) atoms[fillp[elem[i]]++] = (int)i;
+    }
+    std::vector<int> off32(ns + 1);
+    for (int s = 0; s <= ns; ++s) off32[s] = (int)off[s];
+    auto up = [&](const std::string& name, const void* src, size_t bytes) {
+        void* d = buf<char>(c, name, bytes);
+        hip_check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    };
+    up("pos", pos, sizeof(double) * 3 * N);
+    up("elem", elem, sizeof(int) * N);
+    up("sid", sid.data(), sizeof(int) * N);
+    up("st", c->st_host.data(), sizeof(StructT) * ns);
+    up("elem_atoms", atoms.data(), sizeof(int) * N);
+    up("atom_off", off32.data(), sizeof(int) * (ns + 1));
+    c->cap = estimate_cap(c);
+    c->nbr_valid = false;
+    invalidate(c);
+    // First build sizes the capacity to the actual maximum.
+    build_neighbors(c, true);
+    const int want = ((c->max_nb + c->max_nb / 8 + 8 + 31) / 32) * 32;
+    if (want < c->cap) {
+        c->cap = std::max(want, 32);
+        build_neighbors(c, true);
+    }
 }
 
 void copy_out(nnpx_ctx* c, void* dst, const void* src, size_t bytes)
@@ -629,8 +759,10 @@ nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
         switch (option) {
         case NNPX_OPT_FORCE_MODE:
             if (value != NNPX_FORCES_PULL && value != NNPX_FORCES_PUSH) fail(NNPX_ERR_ARGUMENT, "bad force mode");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
             c->force_mode = (int)value;
             c->f_valid = false;
+            drop_graph(c);
             break;
         case NNPX_OPT_TIMING: c->timing = value != 0; break;
         case NNPX_OPT_NEIGHBOR_ALGO:
@@ -639,12 +771,23 @@ nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
             break;
         case NNPX_OPT_BF16_TERMS:
             if (value != 1 && value != 3) fail(NNPX_ERR_ARGUMENT, "bf16 terms must be 1 or 3");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
             c->mixed.terms = (int)value;
+            drop_graph(c);  // captured MD steps hold the old value
+            invalidate(c);
+            break;
+        case NNPX_OPT_PLAIN_FORCES:
+            if (value != 0 && value != 1) fail(NNPX_ERR_ARGUMENT, "plain forces must be 0 or 1");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            c->plain_forces = (int)value;
+            drop_graph(c);
             invalidate(c);
             break;
         case NNPX_OPT_SF_GATHER:
             if (value < -1 || value > 1) fail(NNPX_ERR_ARGUMENT, "sf gather must be -1, 0 or 1");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
             c->mixed.tune.sf_gather = (int)value;
+            drop_graph(c);
             break;
         default: fail(NNPX_ERR_ARGUMENT, "unknown option");
         }
@@ -680,6 +823,19 @@ nnpx_status nnpx_load_model(nnpx_ctx* c, const char* dir)
                     islot[((size_t)e * kMaxElem + n) * kMaxSF + t.table[t.el[e].tab0[n] + q]] = q;
         int* di = buf<int>(c, "islot", islot.size());
         hip_check(hipMemcpy(di, islot.data(), sizeof(int) * islot.size(), hipMemcpyHostToDevice), "hipMemcpy");
+        // derivative scale of every SF for the fused FP64 forces: Gaussian
+        // angular members 2^(1-zeta) times the scaling factor, all others
+        // the scaling factor
+        std::vector<double> fac((size_t)kMaxElem * kMaxSF, 0.0);
+        for (int e = 0; e < t.nelem; ++e)
+            for (int g = t.el[e].grp0; g < t.el[e].grp0 + t.el[e].ngrp; ++g) {
+                const GroupT& G = t.grp[g];
+                const bool ang = G.type == 3 || G.type == 9 || G.type == 13;
+                for (int q = G.mem0; q < G.mem0 + G.nmem; ++q)
+                    fac[(size_t)e * kMaxSF + t.mem[q].sf] = ang ? t.mem[q].factorNorm : t.mem[q].sfac;
+            }
+        double* dfac = buf<double>(c, "fac64", fac.size());
+        hip_check(hipMemcpy(dfac, fac.data(), sizeof(double) * fac.size(), hipMemcpyHostToDevice), "hipMemcpy");
         c->has_model = true;
         c->N = c->S = 0;
         c->nbr_valid = false;
@@ -771,72 +927,7 @@ nnpx_status nnpx_model_units(const nnpx_ctx* cc, int* normalize, double* mean, d
 nnpx_status nnpx_set_structures(nnpx_ctx* c, int ns, const int64_t* off, const double* pos, const int* elem,
                                 const double* box, const int* periodic)
 {
-    return guard(c, [&] {
-        need_model(c);
-        if (ns < 1 || !off || !pos || !elem || !periodic) fail(NNPX_ERR_ARGUMENT, "null or empty structure input");
-        if (off[0] != 0) fail(NNPX_ERR_ARGUMENT, "atom_offsets[0] must be 0");
-        for (int s = 0; s < ns; ++s)
-            if (off[s + 1] < off[s]) fail(NNPX_ERR_ARGUMENT, "atom_offsets must be non-decreasing");
-        const int64_t N = off[ns];
-        if (N <= 0 || N > (int64_t)1 << 30) fail(NNPX_ERR_ARGUMENT, "bad atom count");
-        // a new structure set ends any MD set-up (skin, graph) and any
-        // caller-provided neighbour list
-        c->external = false;
-        c->ntotal = 0;
-        c->skin = 0.0;
-        c->md_ready = false;
-        drop_graph(c);
-        for (int64_t i = 0; i < N; ++i)
-            if (elem[i] < 0 || elem[i] >= c->hm.t.nelem) fail(NNPX_ERR_RANGE, "element index out of range");
-        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        c->N = (int)N;
-        c->S = ns;
-        c->atom_off.assign(off, off + ns + 1);
-        c->elem_host.assign(elem, elem + N);
-        c->st_host.assign(ns, StructT{});
-        for (int s = 0; s < ns; ++s) {
-            if (periodic[s] && !box) fail(NNPX_ERR_ARGUMENT, "periodic structure without box");
-            setup_struct(c->st_host[s], periodic[s] ? box + 9 * s : nullptr, periodic[s] != 0, list_cutoff(c));
-            c->st_host[s].atom0 = (int)off[s];
-            c->st_host[s].natoms = (int)(off[s + 1] - off[s]);
-        }
-        choose_algorithms(c, pos);
-        std::vector<int> sid(N);
-        for (int s = 0; s < ns; ++s)
-            for (int64_t i = off[s]; i < off[s + 1]; ++i) sid[i] = s;
-        const int ne = c->hm.t.nelem;
-        c->elem_count.assign(ne, 0);
-        c->elem_off.assign(ne + 1, 0);
-        for (int64_t i = 0; i < N; ++i) c->elem_count[elem[i]]++;
-        for (int e = 0; e < ne; ++e) c->elem_off[e + 1] = c->elem_off[e] + c->elem_count[e];
-        std::vector<int> atoms(N);
-        {
-            std::vector<int> fillp(c->elem_off.begin(), c->elem_off.end() - 1);
-            for (int64_t i = 0; i < N; ++i) atoms[fillp[elem[i]]++] = (int)i;
-        }
-        std::vector<int> off32(ns + 1);
-        for (int s = 0; s <= ns; ++s) off32[s] = (int)off[s];
-        auto up = [&](const std::string& name, const void* src, size_t bytes) {
-            void* d = buf<char>(c, name, bytes);
-            hip_check(hipMemcpy(d, src, bytes, hipMemcpyHostToDevice), "hipMemcpy");
-        };
-        up("pos", pos, sizeof(double) * 3 * N);
-        up("elem", elem, sizeof(int) * N);
-        up("sid", sid.data(), sizeof(int) * N);
-        up("st", c->st_host.data(), sizeof(StructT) * ns);
-        up("elem_atoms", atoms.data(), sizeof(int) * N);
-        up("atom_off", off32.data(), sizeof(int) * (ns + 1));
-        c->cap = estimate_cap(c);
-        c->nbr_valid = false;
-        invalidate(c);
-        // First build sizes the capacity to the actual maximum.
-        build_neighbors(c, true);
-        const int want = ((c->max_nb + c->max_nb / 8 + 8 + 31) / 32) * 32;
-        if (want < c->cap) {
-            c->cap = std::max(want, 32);
-            build_neighbors(c, true);
-        }
-    });
+    return guard(c, [&] { set_structures(c, ns, off, pos, elem, box, periodic); });
 }
 
 nnpx_status nnpx_set_neighbors_external(nnpx_ctx* c, int nlocal, const int* elem, int ntotal,
@@ -896,6 +987,7 @@ nnpx_status nnpx_set_neighbors_external(nnpx_ctx* c, int nlocal, const int* elem
         }
         c->cap = std::max(32, ((maxn + 31) / 32) * 32);
         c->max_nb = maxn;
+        c->stage_cap = 0;  // stage the whole list (a fit from earlier structures does not apply)
         const size_t slots = (size_t)nlocal * c->cap;
         std::vector<int> nn(nlocal), jj(slots, 0), im(slots, pack_image(0, 0, 0));
         std::vector<double> dr(3 * slots, 0.0);
@@ -1158,6 +1250,131 @@ nnpx_status nnpx_get_dGdr(nnpx_ctx* c, double* self, int ld_self, double* nbr, i
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// Two passes of an append kernel: the first counts, the second fills buffers
+// of that size; returns the count.
+template <typename Launch>
+int64_t append_records(nnpx_ctx* c, Launch&& launch, int64_t want)
+{
+    unsigned long long* cnt = buf<unsigned long long>(c, "rec_count", 1);
+    hip_check(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream), "hipMemsetAsync");
+    launch(cnt, 0ll);
+    unsigned long long n = 0;
+    copy_out(c, &n, cnt, sizeof n);
+    if (n > 0 && want > 0) {
+        hip_check(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), c->stream), "hipMemsetAsync");
+        launch(cnt, (long long)n);
+        hip_check(hipGetLastError(), "append kernel");
+    }
+    return (int64_t)n;
+}
+
+}  // namespace
+
+extern "C" {
+
+nnpx_status nnpx_extrapolation_warnings(nnpx_ctx* c, int64_t max, int64_t* count, int* atom, int* sf, int* type,
+                                        double* value, double* gmin, double* gmax)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (!c->sf_valid) fail(NNPX_ERR_STATE, "symmetry functions not calculated");
+        if (max < 0 || (max > 0 && !atom)) fail(NNPX_ERR_ARGUMENT, "bad record buffers");
+        settle(c);
+        const ModelT& t = c->hm.t;
+        const size_t total = (size_t)c->N * t.max_sf;
+        int* ra = buf<int>(c, "rec_atom", total);
+        int* rs = buf<int>(c, "rec_sf", total);
+        double* rv = buf<double>(c, "rec_val", total);
+        const int64_t n = append_records(c, [&](unsigned long long* cnt, long long cap) {
+            launch_extrapolation(c->d_model, c->N, existing<int>(c, "elem"), existing<double>(c, "G"), t.max_sf, cnt,
+                                 cap, ra, rs, rv, c->stream);
+        }, max);
+        if (count) *count = n;
+        if (n == 0 || max == 0) return;
+        std::vector<int> a(n), q(n);
+        std::vector<double> v(n);
+        copy_out(c, a.data(), ra, sizeof(int) * n);
+        copy_out(c, q.data(), rs, sizeof(int) * n);
+        copy_out(c, v.data(), rv, sizeof(double) * n);
+        std::vector<int64_t> order(n);
+        for (int64_t k = 0; k < n; ++k) order[k] = k;
+        std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+            return a[x] != a[y] ? a[x] < a[y] : q[x] < q[y];
+        });
+        // SF type of every (element, SF)
+        std::vector<int> sft((size_t)kMaxElem * kMaxSF, 0);
+        for (int e = 0; e < t.nelem; ++e)
+            for (int g = t.el[e].grp0; g < t.el[e].grp0 + t.el[e].ngrp; ++g)
+                for (int m = t.grp[g].mem0; m < t.grp[g].mem0 + t.grp[g].nmem; ++m)
+                    sft[(size_t)e * kMaxSF + t.mem[m].sf] = t.grp[g].type;
+        const int64_t k = std::min<int64_t>(max, n);
+        for (int64_t r = 0; r < k; ++r) {
+            const int64_t x = order[r];
+            const int e = c->elem_host[a[x]];
+            const ScaleT& S = t.scale[t.el[e].sf0 + q[x]];
+            atom[r] = a[x];
+            if (sf) sf[r] = q[x];
+            if (type) type[r] = sft[(size_t)e * kMaxSF + q[x]];
+            if (value) value[r] = v[x];
+            if (gmin) gmin[r] = S.Gmin;
+            if (gmax) gmax[r] = S.Gmax;
+        }
+    });
+}
+
+nnpx_status nnpx_low_neighbor_atoms(nnpx_ctx* c, int64_t max, int64_t* count, int* atom, int* num_neighbors)
+{
+    return guard(c, [&] {
+        need_structures(c);
+        if (max < 0 || (max > 0 && !atom)) fail(NNPX_ERR_ARGUMENT, "bad record buffers");
+        if (!c->nbr_valid) build_neighbors(c, true);
+        settle(c);
+        const ModelT& t = c->hm.t;
+        // Element::getMinCutoffRadius / getMinNeighbors (Element.cpp:384-411)
+        std::vector<double> rmin(kMaxElem, 0.0);
+        std::vector<int> nmin(kMaxElem, 0);
+        for (int e = 0; e < t.nelem; ++e) {
+            double r = 1e300;
+            for (int g = t.el[e].grp0; g < t.el[e].grp0 + t.el[e].ngrp; ++g) {
+                const int ty = t.grp[g].type;
+                const bool radial = ty == 2 || ty == 12 || ty == 20 || ty == 23;
+                nmin[e] = std::max(nmin[e], radial ? 1 : 2);
+                r = std::min(r, t.grp[g].rc);
+            }
+            rmin[e] = t.el[e].ngrp ? r : 0.0;
+        }
+        double* drmin = buf<double>(c, "low_rmin", kMaxElem);
+        int* dnmin = buf<int>(c, "low_nmin", kMaxElem);
+        hip_check(hipMemcpyAsync(drmin, rmin.data(), sizeof(double) * kMaxElem, hipMemcpyHostToDevice, c->stream),
+                  "hipMemcpyAsync");
+        hip_check(hipMemcpyAsync(dnmin, nmin.data(), sizeof(int) * kMaxElem, hipMemcpyHostToDevice, c->stream),
+                  "hipMemcpyAsync");
+        int* ra = buf<int>(c, "rec_atom", c->N);
+        int* rn = buf<int>(c, "rec_sf", c->N);
+        SysView s = sysview(c);
+        const int64_t n = append_records(c, [&](unsigned long long* cnt, long long cap) {
+            launch_low_neighbors(c->d_model, s, drmin, dnmin, cnt, cap, ra, rn, c->stream);
+        }, max);
+        if (count) *count = n;
+        if (n == 0 || max == 0) return;
+        std::vector<int> a(n), m(n);
+        copy_out(c, a.data(), ra, sizeof(int) * n);
+        copy_out(c, m.data(), rn, sizeof(int) * n);
+        std::vector<int64_t> order(n);
+        for (int64_t k = 0; k < n; ++k) order[k] = k;
+        std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return a[x] < a[y]; });
+        const int64_t k = std::min<int64_t>(max, n);
+        for (int64_t r = 0; r < k; ++r) {
+            atom[r] = a[order[r]];
+            if (num_neighbors) num_neighbors[r] = m[order[r]];
+        }
+    });
+}
+
 nnpx_status nnpx_num_connections(const nnpx_ctx* cc, int element, int* n)
 {
     nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
@@ -1171,21 +1388,53 @@ nnpx_status nnpx_num_connections(const nnpx_ctx* cc, int element, int* n)
     });
 }
 
+}  // extern "C"
+
+namespace {
+
+// The connections of all networks in NeuralNetwork::setConnections order
+// (NeuralNetwork.cpp:147-169), elements in order.
+void get_connections(const nnpx_ctx* c, double* w)
+{
+    size_t p = 0;
+    for (int e = 0; e < c->hm.t.nelem; ++e) {
+        const NNT& N = c->hm.t.el[e].nn;
+        for (int l = 1; l < N.nl; ++l) {
+            for (int j = 0; j < N.n[l - 1]; ++j)
+                for (int k = 0; k < N.n[l]; ++k) w[p++] = c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j];
+            for (int k = 0; k < N.n[l]; ++k) w[p++] = c->hm.weights[N.boff[l] + k];
+        }
+    }
+}
+
+void set_connections(nnpx_ctx* c, const double* w)
+{
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    size_t p = 0;
+    for (int e = 0; e < c->hm.t.nelem; ++e) {
+        const NNT& N = c->hm.t.el[e].nn;
+        for (int l = 1; l < N.nl; ++l) {
+            for (int j = 0; j < N.n[l - 1]; ++j)
+                for (int k = 0; k < N.n[l]; ++k) c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j] = w[p++];
+            for (int k = 0; k < N.n[l]; ++k) c->hm.weights[N.boff[l] + k] = w[p++];
+        }
+    }
+    upload_weights(c);
+    drop_graph(c);
+    invalidate(c);
+}
+
+}  // namespace
+
+extern "C" {
+
 nnpx_status nnpx_get_weights(const nnpx_ctx* cc, double* w)
 {
     nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
     return guard(c, [&] {
         need_model(c);
         if (!w) fail(NNPX_ERR_ARGUMENT, "null output");
-        size_t p = 0;
-        for (int e = 0; e < c->hm.t.nelem; ++e) {
-            const NNT& N = c->hm.t.el[e].nn;
-            for (int l = 1; l < N.nl; ++l) {
-                for (int j = 0; j < N.n[l - 1]; ++j)
-                    for (int k = 0; k < N.n[l]; ++k) w[p++] = c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j];
-                for (int k = 0; k < N.n[l]; ++k) w[p++] = c->hm.weights[N.boff[l] + k];
-            }
-        }
+        get_connections(c, w);
     });
 }
 
@@ -1194,19 +1443,7 @@ nnpx_status nnpx_set_weights(nnpx_ctx* c, const double* w)
     return guard(c, [&] {
         need_model(c);
         if (!w) fail(NNPX_ERR_ARGUMENT, "null weights");
-        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        size_t p = 0;
-        for (int e = 0; e < c->hm.t.nelem; ++e) {
-            const NNT& N = c->hm.t.el[e].nn;
-            for (int l = 1; l < N.nl; ++l) {
-                for (int j = 0; j < N.n[l - 1]; ++j)
-                    for (int k = 0; k < N.n[l]; ++k) c->hm.weights[N.woff[l] + (size_t)k * N.n[l - 1] + j] = w[p++];
-                for (int k = 0; k < N.n[l]; ++k) c->hm.weights[N.boff[l] + k] = w[p++];
-            }
-        }
-        upload_weights(c);
-        drop_graph(c);
-        invalidate(c);
+        set_connections(c, w);
     });
 }
 
@@ -1230,7 +1467,7 @@ void weight_vjp(nnpx_ctx* c, int precision, const double* u, const double* cE, d
     if (precision == NNPX_PLAIN_FP64) {
         timed(c, "tangent64", [&] {
             launch_tangent64(c->d_model, s, existing<double>(c, "dGself"), existing<double>(c, "dGnbr"),
-                             std::max(t.max_sfd, 1), existing<int>(c, "islot"), u, Gdot, ldG, c->stream);
+                             std::max(t.max_sfd, 1), model_table<int>(c, "islot"), u, Gdot, ldG, c->stream);
         });
     } else {
         timed(c, "tangent32", [&] { launch_tangent32(c->mixed, t, c->d_model, s, u, Gdot, ldG, c->stream); });
@@ -1295,7 +1532,7 @@ nnpx_status nnpx_loss_gradient(nnpx_ctx* c, int precision, const double* E_ref, 
         gradient_checks(c, precision);
         if (!E_ref || !F_ref || !loss || !grad) fail(NNPX_ERR_ARGUMENT, "null argument");
         if (!(beta >= 0.0)) fail(NNPX_ERR_ARGUMENT, "beta must be >= 0");
-        evaluate(c, precision, NNPX_EVAL_FORCES, true);
+        evaluate(c, precision, NNPX_EVAL_FORCES, true, precision == NNPX_PLAIN_FP64);
         settle(c);
         const int N = c->N, S = c->S;
         double* Eref = buf<double>(c, "t_Eref", S);
@@ -1330,7 +1567,7 @@ nnpx_status nnpx_weight_vjp(nnpx_ctx* c, int precision, const double* vE, const 
     return guard(c, [&] {
         gradient_checks(c, precision);
         if (!vE || !vF || !grad) fail(NNPX_ERR_ARGUMENT, "null argument");
-        evaluate(c, precision, NNPX_EVAL_FORCES, true);
+        evaluate(c, precision, NNPX_EVAL_FORCES, true, precision == NNPX_PLAIN_FP64);
         settle(c);
         const int N = c->N, S = c->S;
         const int total = total_connections(c->hm.t);
@@ -1342,6 +1579,245 @@ nnpx_status nnpx_weight_vjp(nnpx_ctx* c, int precision, const double* vE, const 
         hip_check(hipMemsetAsync(gd, 0, sizeof(double) * (size_t)S * total, c->stream), "hipMemsetAsync");
         weight_vjp(c, precision, u, cE, gd, total);
         copy_out(c, grad, gd, sizeof(double) * (size_t)S * total);
+        collect_times(c);
+    });
+}
+
+nnpx_status nnpx_kalman_init(nnpx_ctx* c, int type, int per_element, const double* par)
+{
+    return guard(c, [&] {
+        need_model(c);
+        if (type != 0 && type != 1) fail(NNPX_ERR_ARGUMENT, "kalman type must be 0 (standard) or 1 (fading memory)");
+        if (!par) fail(NNPX_ERR_ARGUMENT, "null parameters");
+        if (!(par[0] > 0.0)) fail(NNPX_ERR_ARGUMENT, "kalman epsilon must be > 0");
+        const ModelT& t = c->hm.t;
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        c->kf.clear();
+        int off = 0;
+        for (int e = 0; e < t.nelem; ++e) {
+            const int n = num_connections(t.el[e].nn);
+            if (per_element || e == 0) {
+                nnpx_ctx::KF k;
+                k.off = off;
+                c->kf.push_back(k);
+            }
+            c->kf.back().n += n;
+            off += n;
+        }
+        for (size_t u = 0; u < c->kf.size(); ++u) {
+            nnpx_ctx::KF& k = c->kf[u];
+            // KalmanFilter::setParametersStandard / setParametersFadingMemory
+            k.q = par[1];
+            k.qtau = par[2];
+            k.qmin = par[3];
+            if (type == 0) {
+                k.eta = par[4];
+                k.etatau = par[5];
+                k.etamax = par[6];
+            } else {
+                k.lambda = par[4];
+                k.nu = par[5];
+                k.gamma = 1.0;
+            }
+            double* P = buf<double>(c, "kf_P" + std::to_string(u), (size_t)k.n * k.n);
+            kf_init(P, k.n, 1.0 / par[0], c->stream);  // P = I / epsilon
+        }
+        hip_check(hipGetLastError(), "kf_init");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        c->kf_type = type;
+    });
+}
+
+nnpx_status nnpx_kalman_state(nnpx_ctx* c, int updater, double* P, double* scalars)
+{
+    return guard(c, [&] {
+        if (c->kf_type < 0) fail(NNPX_ERR_STATE, "Kalman filter not set up (nnpx_kalman_init)");
+        if (updater < 0 || updater >= (int)c->kf.size()) fail(NNPX_ERR_ARGUMENT, "bad updater index");
+        const nnpx_ctx::KF& k = c->kf[updater];
+        if (P) copy_out(c, P, existing<double>(c, "kf_P" + std::to_string(updater)), sizeof(double) * k.n * k.n);
+        if (scalars) {
+            scalars[0] = k.n;
+            scalars[1] = k.off;
+            scalars[2] = k.eta;
+            scalars[3] = k.lambda;
+            scalars[4] = k.gamma;
+            scalars[5] = k.q;
+            scalars[6] = (double)k.updates;
+        }
+    });
+}
+
+nnpx_status nnpx_kalman_update(nnpx_ctx* c, int precision, int property, int batch, const int* structure,
+                               const int* atom, const int* component, const double* E_ref, const double* F_ref,
+                               double force_weight, int jacobian_full, double* error)
+{
+    return guard(c, [&] {
+        gradient_checks(c, precision);
+        if (c->kf_type < 0) fail(NNPX_ERR_STATE, "Kalman filter not set up (nnpx_kalman_init)");
+        if (property != 0 && property != 1) fail(NNPX_ERR_ARGUMENT, "property must be 0 (energy) or 1 (force)");
+        if (batch < 1 || !structure || (property == 0 && !E_ref) || (property == 1 && (!F_ref || !atom || !component)))
+            fail(NNPX_ERR_ARGUMENT, "bad update candidates");
+        const int N = c->N, S = c->S;
+        for (int b = 0; b < batch; ++b) {
+            if (structure[b] < 0 || structure[b] >= S) fail(NNPX_ERR_RANGE, "candidate structure out of range");
+            if (property == 1) {
+                if (atom[b] < c->atom_off[structure[b]] || atom[b] >= c->atom_off[structure[b] + 1])
+                    fail(NNPX_ERR_RANGE, "candidate atom not in its structure");
+                if (component[b] < 0 || component[b] > 2) fail(NNPX_ERR_RANGE, "candidate component out of range");
+            }
+        }
+        const ModelT& t = c->hm.t;
+        const int total = total_connections(t);
+        const int m = jacobian_full ? batch : 1;
+        const bool force = property == 1;
+        const double fw = force ? force_weight : 1.0;
+        // Errors of the current weights (Training::update PART 2)
+        evaluate(c, precision, NNPX_EVAL_FORCES, true, precision == NNPX_PLAIN_FP64);
+        settle(c);
+        std::vector<double> E(S), F((size_t)3 * N);
+        copy_out(c, E.data(), existing<double>(c, "E"), sizeof(double) * S);
+        if (force) copy_out(c, F.data(), existing<double>(c, "F"), sizeof(double) * 3 * N);
+        std::vector<double> err(batch);
+        for (int b = 0; b < batch; ++b)
+            err[b] = force ? F_ref[3 * atom[b] + component[b]] - F[3 * (size_t)atom[b] + component[b]]
+                           : E_ref[structure[b]] - E[structure[b]];
+        if (error)
+            for (int b = 0; b < batch; ++b) error[b] = err[b];
+        // Per-element error fractions of US_ELEMENT: energy N_e / N of the
+        // structure, force the atom's neighbours of element e over all of
+        // them (Training.cpp:2881-2906)
+        const int nu = (int)c->kf.size();
+        std::vector<double> frac((size_t)batch * nu, 1.0);
+        if (nu > 1) {
+            std::vector<int> cnt, jj;
+            if (force) {
+                cnt.resize(N);
+                copy_out(c, cnt.data(), existing<int>(c, "nb_n"), sizeof(int) * N);
+                jj.resize((size_t)N * c->cap);
+                copy_out(c, jj.data(), existing<int>(c, "nb_j"), sizeof(int) * jj.size());
+            }
+            for (int b = 0; b < batch; ++b) {
+                std::vector<double> ne(t.nelem, 0.0);
+                double all = 0.0;
+                if (force) {
+                    const int a = atom[b];
+                    for (int k = 0; k < cnt[a]; ++k) ne[c->elem_host[jj[(size_t)a * c->cap + k]]] += 1.0;
+                    all = cnt[a];
+                } else {
+                    for (int64_t i = c->atom_off[structure[b]]; i < c->atom_off[structure[b] + 1]; ++i)
+                        ne[c->elem_host[i]] += 1.0;
+                    all = (double)(c->atom_off[structure[b] + 1] - c->atom_off[structure[b]]);
+                }
+                for (int u = 0; u < nu; ++u) frac[(size_t)b * nu + u] = all > 0.0 ? ne[u] / all : 0.0;
+            }
+        }
+        // Jacobian rows (Training::dPdc contracted with unit seeds): one
+        // summed row, or one row per candidate in rounds of candidates on
+        // different structures
+        std::vector<double> rows((size_t)m * total, 0.0);
+        double* u = buf<double>(c, "t_u", (size_t)3 * N);
+        double* cE = buf<double>(c, "t_cE", S);
+        double* gd = buf<double>(c, "t_gradS", (size_t)S * total);
+        std::vector<int> done(batch, 0);
+        int left = batch;
+        while (left > 0) {
+            std::vector<double> vE(S, 0.0), vF((size_t)3 * N, 0.0);
+            std::vector<int> take, used(S, 0);
+            for (int b = 0; b < batch; ++b) {
+                if (done[b] || (jacobian_full && used[structure[b]])) continue;
+                used[structure[b]] = 1;
+                take.push_back(b);
+                if (force) vF[3 * (size_t)atom[b] + component[b]] += 1.0;
+                else vE[structure[b]] += 1.0;
+            }
+            hip_check(hipMemcpyAsync(cE, vE.data(), sizeof(double) * S, hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync");
+            hip_check(hipMemcpyAsync(u, vF.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync");
+            const int ld = jacobian_full ? total : 0;
+            hip_check(hipMemsetAsync(gd, 0, sizeof(double) * (jacobian_full ? (size_t)S * total : total), c->stream),
+                      "hipMemsetAsync");
+            weight_vjp(c, precision, u, cE, gd, ld);
+            if (jacobian_full) {
+                std::vector<double> r((size_t)S * total);
+                copy_out(c, r.data(), gd, sizeof(double) * r.size());
+                for (int b : take)
+                    std::copy(r.begin() + (size_t)structure[b] * total, r.begin() + (size_t)(structure[b] + 1) * total,
+                              rows.begin() + (size_t)b * total);
+            } else {
+                copy_out(c, rows.data(), gd, sizeof(double) * total);
+            }
+            for (int b : take) done[b] = 1;
+            left -= (int)take.size();
+        }
+        // The filters (Training::update PART 4, KalmanFilter::update)
+        std::vector<double> w(total);
+        get_connections(c, w.data());
+        for (int ui = 0; ui < nu; ++ui) {
+            nnpx_ctx::KF& k = c->kf[ui];
+            const int n = k.n;
+            std::vector<double> H((size_t)n * m), xi(m, 0.0);
+            for (int j = 0; j < n; ++j)
+                for (int col = 0; col < m; ++col) H[(size_t)j * m + col] = rows[(size_t)col * total + k.off + j] * fw;
+            for (int b = 0; b < batch; ++b) xi[jacobian_full ? b : 0] += err[b] * fw * frac[(size_t)b * nu + ui];
+            double* dH = buf<double>(c, "kf_H", (size_t)n * m);
+            double* dX = buf<double>(c, "kf_X", (size_t)n * m);
+            double* dK = buf<double>(c, "kf_K", (size_t)n * m);
+            double* dA = buf<double>(c, "kf_A", (size_t)m * m);
+            double* dxi = buf<double>(c, "kf_xi", m);
+            double* ddw = buf<double>(c, "kf_dw", n);
+            double* P = existing<double>(c, "kf_P" + std::to_string(ui));
+            hip_check(hipMemcpyAsync(dH, H.data(), sizeof(double) * H.size(), hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync");
+            hip_check(hipMemcpyAsync(dxi, xi.data(), sizeof(double) * m, hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync");
+            timed(c, "kalman", [&] { kf_xa(P, dH, n, m, dX, dA, c->stream); });
+            std::vector<double> A((size_t)m * m), Ai((size_t)m * m, 0.0);
+            copy_out(c, A.data(), dA, sizeof(double) * A.size());
+            if (c->kf_type == 0 && k.eta < k.etamax) k.eta *= std::exp(k.etatau);
+            for (int a = 0; a < m; ++a) A[(size_t)a * m + a] += c->kf_type == 0 ? 1.0 / k.eta : k.lambda;
+            // A^-1 (Gauss-Jordan, partial pivoting)
+            for (int a = 0; a < m; ++a) Ai[(size_t)a * m + a] = 1.0;
+            for (int col = 0; col < m; ++col) {
+                int piv = col;
+                for (int r = col + 1; r < m; ++r)
+                    if (std::fabs(A[(size_t)r * m + col]) > std::fabs(A[(size_t)piv * m + col])) piv = r;
+                if (A[(size_t)piv * m + col] == 0.0) fail(NNPX_ERR_RANGE, "Kalman filter: singular innovation matrix");
+                if (piv != col)
+                    for (int q = 0; q < m; ++q) {
+                        std::swap(A[(size_t)piv * m + q], A[(size_t)col * m + q]);
+                        std::swap(Ai[(size_t)piv * m + q], Ai[(size_t)col * m + q]);
+                    }
+                const double d = A[(size_t)col * m + col];
+                for (int q = 0; q < m; ++q) {
+                    A[(size_t)col * m + q] /= d;
+                    Ai[(size_t)col * m + q] /= d;
+                }
+                for (int r = 0; r < m; ++r)
+                    if (r != col) {
+                        const double f = A[(size_t)r * m + col];
+                        for (int q = 0; q < m; ++q) {
+                            A[(size_t)r * m + q] -= f * A[(size_t)col * m + q];
+                            Ai[(size_t)r * m + q] -= f * Ai[(size_t)col * m + q];
+                        }
+                    }
+            }
+            double* dAi = buf<double>(c, "kf_Ai", (size_t)m * m);
+            hip_check(hipMemcpyAsync(dAi, Ai.data(), sizeof(double) * Ai.size(), hipMemcpyHostToDevice, c->stream),
+                      "hipMemcpyAsync");
+            const double scale = c->kf_type == 1 ? 1.0 / k.lambda : 1.0;
+            timed(c, "kalman", [&] { kf_finish(P, dX, dAi, dxi, n, m, scale, k.q, dK, ddw, c->stream); });
+            std::vector<double> dw(n);
+            copy_out(c, dw.data(), ddw, sizeof(double) * n);
+            for (int j = 0; j < n; ++j) w[k.off + j] += dw[j];
+            if (k.q > k.qmin) k.q *= std::exp(-k.qtau);
+            if (c->kf_type == 1) {
+                k.lambda = k.nu * k.lambda + 1.0 - k.nu;
+                k.gamma = 1.0 / (1.0 + k.lambda / k.gamma);
+            }
+            k.updates++;
+        }
+        set_connections(c, w.data());
         collect_times(c);
     });
 }
@@ -1505,6 +1981,11 @@ void md_step(nnpx_ctx* c)
         MixedBuffers mb = mixed_buffers(c);
         run_mixed(c->mixed, c->hm.t, c->d_model, sysview(c), mb, true, c->force_mode == NNPX_FORCES_PUSH, c->stream,
                   [&](const char* name, const std::function<void()>& f) { timed(c, name, f); });
+    } else if (fused_fits(c)) {
+        compute_sf64(c, false);
+        compute_nn64(c, true);
+        compute_energy(c);
+        compute_force64_fused(c);
     } else {
         launch_reverse(s, c->stream);
         c->rev_valid = true;
@@ -1516,6 +1997,7 @@ void md_step(nnpx_ctx* c)
     md_kick(vel, existing<double>(c, "F"), elem, c->mdc, c->N, counters, existing<double>(c, "md_ekin"), c->stream);
     md_step_end(gate, counters, existing<double>(c, "E"), existing<double>(c, "md_epot"),
                 existing<double>(c, "md_ekin"), c->stream);
+    hip_check(hipGetLastError(), "MD step launch");
 }
 
 void need_md(nnpx_ctx* c)
@@ -1640,7 +2122,8 @@ nnpx_status nnpx_md_run(nnpx_ctx* c, int nsteps, int graph_steps)
         read_flags(c, f);
         if (f[0] & 1) fail(NNPX_ERR_RANGE, "neighbour capacity exceeded during MD (raise the skin or re-initialise)");
         c->sf_valid = c->nn_valid = c->dedg_valid = c->e_valid = c->f_valid = true;
-        c->sfd_valid = c->md_precision == NNPX_PLAIN_FP64;
+        c->sfd_valid = c->md_precision == NNPX_PLAIN_FP64 && !fused_fits(c);
+        c->last_stored = c->sfd_valid;
         c->last_precision = c->md_precision;
     });
 }

@@ -348,18 +348,20 @@ void launch_sf64(const ModelT* m, const ModelT& hm, const SysView& s, const SF64
     const int capL = ((std::max(max_nb, 1) + 31) / 32) * 32;
     const int mam = std::max(hm.max_mem_ang, 1);
     const int MAXM = mam <= 4 ? 4 : mam <= 8 ? 8 : mam <= 16 ? 16 : 32;
+    // the dG/dr accumulators only when derivatives are stored
+    const int accM = deriv ? mam : 0;
     const size_t lds = sizeof(double) * ((size_t)capL * 6 + kSFWaves * 4 * kMaxMembersExp +
-                                         (size_t)capL * mam * 3) +
+                                         (size_t)capL * accM * 3) +
                        sizeof(int) * (size_t)capL * 3;
     if (lds > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the FP64 SF kernel's LDS staging"};
     const dim3 grid(s.natoms), block(kSFThreads);
     const int d = deriv ? 1 : 0;
     switch (MAXM) {
-    case 4: hipLaunchKernelGGL(k_sf64<4>, grid, block, lds, stream, m, s, o, d, capL, mam); break;
-    case 8: hipLaunchKernelGGL(k_sf64<8>, grid, block, lds, stream, m, s, o, d, capL, mam); break;
-    case 16: hipLaunchKernelGGL(k_sf64<16>, grid, block, lds, stream, m, s, o, d, capL, mam); break;
-    default: hipLaunchKernelGGL(k_sf64<32>, grid, block, lds, stream, m, s, o, d, capL, mam); break;
+    case 4: hipLaunchKernelGGL(k_sf64<4>, grid, block, lds, stream, m, s, o, d, capL, accM); break;
+    case 8: hipLaunchKernelGGL(k_sf64<8>, grid, block, lds, stream, m, s, o, d, capL, accM); break;
+    case 16: hipLaunchKernelGGL(k_sf64<16>, grid, block, lds, stream, m, s, o, d, capL, accM); break;
+    default: hipLaunchKernelGGL(k_sf64<32>, grid, block, lds, stream, m, s, o, d, capL, accM); break;
     }
 }
 

@@ -26,6 +26,8 @@
 // accumulated by LDS FP64 atomics [CH][3][cap] and written once); compact
 // groups are large (tens of members with disjoint radial and angular
 // windows), so a chunk keeps the accumulators bounded whatever the group size.
+#include <type_traits>
+
 #include "kernels.hpp"
 #include "nnpx.h"
 
@@ -120,6 +122,7 @@ struct CompMix {
     double* F;        // forces (force mode)
     int push;         // keep contributions to the atom's own images
     const double* u;  // [3N] field of the tangent mode (its output goes to G)
+    const double* fw64;  // [N*ldG] FP64 weights of the FP64 force mode (plain form)
 };
 
 template <typename T, int MODE, int CH>
@@ -174,7 +177,11 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
     double* dGi = o.dGself + (size_t)i * o.ldG * 3;
     double* dGn = o.dGnbr + ebase * o.S * 3;
     float* Ii = c.Gin + (size_t)i * c.ldK;
-    const float* wi = c.fw + (size_t)i * c.ldG;
+    // force weights: FP32 in the mixed form, FP64 in the plain form's fused force
+    auto weight = [&](int sf) -> T {
+        if constexpr (std::is_same<T, double>::value) return c.fw64[(size_t)i * c.ldG + sf];
+        else return (T)c.fw[(size_t)i * c.ldG + sf];
+    };
     double fself[3] = {0.0, 0.0, 0.0};  // force mode: this thread's part of the atom's own force
 
     // Writes member mb's value v (values and plain modes) or tangent.
@@ -208,7 +215,7 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                     if (!(rij > (T)mb.rl && rij < (T)mb.rcm)) continue;
                     T rad, drad;
                     radial_fdf(mb, rij, rad, drad);
-                    P += wi[mb.sf] * drad;
+                    P += weight(mb.sf) * drad;
                 }
                 P *= Z / rij;
                 for (int cc = 0; cc < 3; ++cc) {
@@ -329,9 +336,12 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                 const T rinvijik = T(1) / (rij * rik);
                 const T costijk = (dr1[0] * dr2[0] + dr1[1] * dr2[1] + dr1[2] * dr2[2]) * rinvijik;
                 T th, dacos;
-                if constexpr (PLAIN) {
+                if constexpr (PLAIN || std::is_same<T, double>::value) {
+                    // FP64 (the plain form's values and forces): the
+                    // reference's acos (SymGrpCompAngn.cpp)
                     // the angular polynomial vanishes at 0 and 180 degrees
                     if (costijk <= T(-1) || costijk >= T(1)) continue;
+                    // (FP32 below: the angle from the sine and cosine instead)
                     th = acos(costijk);
                     dacos = T(-1) / sqrt(T(1) - costijk * costijk);
                 } else {
@@ -429,7 +439,7 @@ __global__ __launch_bounds__(kCThreads) void k_sfcomp(const ModelT* __restrict__
                         return;
                     }
                     if constexpr (FORCE) {
-                        const T w = wi[mb.sf];
+                        const T w = weight(mb.sf);
                         P1 += w * p1;
                         P2 += w * p2;
                         P3 += w * p3;
@@ -540,6 +550,24 @@ void launch_sfcomp64(const ModelT* m, const ModelT& hm, const SysView& s, const 
     case 2: hipLaunchKernelGGL((k_sfcomp<double, CM_PLAIN, 2>), grid, block, lds, stream, m, s, o, c, d, capL); break;
     default: hipLaunchKernelGGL((k_sfcomp<double, CM_PLAIN, 1>), grid, block, lds, stream, m, s, o, c, d, capL); break;
     }
+}
+
+void launch_sfcomp64_force(const ModelT* m, const ModelT& hm, const SysView& s, const double* w, int ldG, double* F,
+                           bool push, hipStream_t stream)
+{
+    if (s.natoms == 0 || !hm.has_comp) return;
+    const int capL = ((std::max(s.cap, 1) + 31) / 32) * 32;
+    const size_t lds = comp_lds(capL, 1);
+    if (lds > 65536)
+        throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the FP64 compact force kernel's LDS staging"};
+    const SF64Out o{};
+    CompMix c{};
+    c.ldG = ldG;
+    c.F = F;
+    c.push = push ? 1 : 0;
+    c.fw64 = w;
+    hipLaunchKernelGGL((k_sfcomp<double, CM_FORCE, 1>), dim3(s.natoms), dim3(kCThreads), lds, stream, m, s, o, c, 0,
+                       capL);
 }
 
 void launch_tangentcomp32(const ModelT* m, const ModelT& hm, const SysView& s, const double* u, double* Gdot,

@@ -3,7 +3,9 @@
 // The device holds one structure at a time; a Structure object becomes
 // resident when calculateSymmetryFunctionGroups is called on it, and the
 // later calls of the same evaluation copy their results back into it.
+#include <cinttypes>
 #include <cmath>
+#include <iostream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -60,6 +62,55 @@ void Mode::setupFromDirectory(std::string const& directory)
 }
 
 void Mode::setForceMode(ForceMode m) { check(nnpx_set_option(ctx_, NNPX_OPT_FORCE_MODE, (int64_t)m)); }
+
+void Mode::setupSymmetryFunctionStatistics(bool collectStatistics, bool collectExtrapolationWarnings,
+                                           bool writeExtrapolationWarnings, bool stopOnExtrapolationWarnings)
+{
+    if (collectStatistics)
+        throw std::runtime_error("ERROR: symmetry function statistics (min/max/mean) are not collected by this "
+                                 "library.\n");
+    collectEW_ = collectExtrapolationWarnings;
+    writeEW_ = writeExtrapolationWarnings;
+    stopEW_ = stopOnExtrapolationWarnings;
+}
+
+// The checks of Mode::calculateSymmetryFunctionGroups on the resident
+// structure: low-neighbour atoms to the log (Mode.cpp:1667-1679) and, when
+// enabled, extrapolation warnings (Element.cpp:462-541, Mode.cpp:1694-1702).
+void Mode::checkStructure(Structure const& s)
+{
+    if (log_) {
+        int64_t n = 0;
+        check(nnpx_low_neighbor_atoms(ctx_, 0, &n, nullptr, nullptr));
+        if (n > 0) {
+            std::vector<int> a(n), c(n);
+            check(nnpx_low_neighbor_atoms(ctx_, n, &n, a.data(), c.data()));
+            for (int64_t k = 0; k < n; ++k)
+                *log_ << strpr("WARNING: Structure %6zu Atom %6zu : %zu neighbors.\n", s.index,
+                               s.atoms.at(a[k]).index, (size_t)c[k]);
+        }
+    }
+    if (!collectEW_ && !writeEW_ && !stopEW_) return;
+    int64_t n = 0;
+    check(nnpx_extrapolation_warnings(ctx_, 0, &n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+    if (n == 0) return;
+    std::vector<int> a(n), q(n), t(n);
+    std::vector<double> v(n), lo(n), hi(n);
+    check(nnpx_extrapolation_warnings(ctx_, n, &n, a.data(), q.data(), t.data(), v.data(), lo.data(), hi.data()));
+    const double cl = normalize_ ? convLength_ : 1.0;
+    (void)cl;
+    for (int64_t k = 0; k < n; ++k) {
+        Atom const& at = s.atoms.at(a[k]);
+        const std::string line =
+            strpr("### NNP EXTRAPOLATION WARNING ### STRUCTURE: %6zu ATOM: %9" PRId64 " ELEMENT: %2s SYMFUNC: %4zu "
+                  "TYPE: %2zu VALUE: %10.3E MIN: %10.3E MAX: %10.3E\n",
+                  at.indexStructure, at.tag, elementMap[at.element].c_str(), (size_t)q[k] + 1, (size_t)t[k], v[k],
+                  lo[k], hi[k]);
+        if (collectEW_) ewLines_.push_back(line);
+        if (writeEW_) std::cerr << line;
+        if (stopEW_) throw std::out_of_range(line + "ERROR: Symmetry function value out of range.\n");
+    }
+}
 
 void Mode::upload(Structure& s)
 {
@@ -173,6 +224,7 @@ void Mode::calculateSymmetryFunctionGroups(Structure& s, bool const derivatives)
     s.hasNeighborList = true;
     s.hasSymmetryFunctions = true;
     if (derivatives) s.hasSymmetryFunctionDerivatives = true;
+    checkStructure(s);
 }
 
 void Mode::calculateAtomicNeuralNetworks(Structure& s, bool const derivatives, std::string)
@@ -251,7 +303,13 @@ double Mode::getEnergyOffset(Structure const& s) const
     return r;
 }
 
-void Prediction::setup() { setupFromDirectory(directory); }
+// Prediction::setup (Prediction.cpp:36-48): extrapolation warnings written
+// to stderr as they occur.
+void Prediction::setup()
+{
+    setupFromDirectory(directory);
+    setupSymmetryFunctionStatistics(false, false, true, false);
+}
 
 void Prediction::readStructureFromFile(std::string const& fileName)
 {

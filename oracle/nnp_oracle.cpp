@@ -253,6 +253,18 @@ struct SF {
         }
         return 0.0;
     }
+    // SymFnc::unscale (SymFnc.cpp:197-223)
+    double unscale(double v) const
+    {
+        switch (scalingType) {
+        case ST_NONE: return v;
+        case ST_SCALE: return (v - Smin) / scalingFactor + Gmin;
+        case ST_CENTER: return v + Gmean;
+        case ST_SCALECENTER: return (v - Smin) / scalingFactor + Gmean;
+        case ST_SCALESIGMA: return (v - Smin) / scalingFactor + Gmean;
+        }
+        return 0.0;
+    }
 };
 
 // Sort key of SymFncExpRad::operator< (SymFncExpRad.cpp:57-75) and
@@ -2038,6 +2050,196 @@ int oracle_get_dEdG(const oracle_ctx* c, double* d, int ld)
 {
     for (size_t i = 0; i < c->atoms.size(); ++i)
         for (size_t s = 0; s < c->atoms[i].dEdG.size() && (int)s < ld; ++s) d[i * ld + s] = c->atoms[i].dEdG[s];
+    return 0;
+}
+
+// Atom::dGdr and Atom::Neighbor::dGdr (Atom.h:35-131) of the last predict:
+// self[N][ld_self][3]; nbr[entry][ld_nbr][3] for every neighbour entry in
+// list order, slot q of an entry with neighbour element e being SF
+// table[e][q] of the centre element (Element::setupSymmetryFunctionMemory,
+// Element.cpp:289-311).  Unused slots are zero; either pointer may be NULL.
+int oracle_get_dGdr(const oracle_ctx* c, double* self, int ld_self, double* nbr, int ld_nbr)
+{
+    size_t p = 0;
+    for (size_t i = 0; i < c->atoms.size(); ++i) {
+        const Atom& a = c->atoms[i];
+        if (a.dGdr.empty()) {
+            g_err = "dGdr: run oracle_predict first";
+            return 1;
+        }
+        if (self)
+            for (int q = 0; q < ld_self; ++q)
+                for (int k = 0; k < 3; ++k)
+                    self[(i * ld_self + q) * 3 + k] = 3 * (size_t)q < a.dGdr.size() ? a.dGdr[3 * q + k] : 0.0;
+        for (const Neighbor& n : a.nbr) {
+            if (nbr)
+                for (int q = 0; q < ld_nbr; ++q)
+                    for (int k = 0; k < 3; ++k)
+                        nbr[(p * ld_nbr + q) * 3 + k] = 3 * (size_t)q < n.dGdr.size() ? n.dGdr[3 * q + k] : 0.0;
+            ++p;
+        }
+    }
+    return 0;
+}
+
+// Element::updateSymmetryFunctionStatistics (Element.cpp:462-541) over the
+// atoms in order: an SF whose unscaled value lies below Gmin - 1000 eps or
+// above Gmax + 1000 eps.  Records (atom, SF, type, value, Gmin, Gmax) in
+// (atom, SF) order; returns their number, writes at most max.
+long oracle_extrapolation_warnings(const oracle_ctx* c, long max, int* atom, int* sf, int* type, double* value,
+                                   double* gmin, double* gmax)
+{
+    const double eps = 1000.0 * std::numeric_limits<double>::epsilon();
+    long n = 0;
+    for (size_t i = 0; i < c->atoms.size(); ++i) {
+        const Atom& a = c->atoms[i];
+        const Element& e = c->el[a.element];
+        for (size_t q = 0; q < a.G.size(); ++q) {
+            const SF& f = e.sfs[q];
+            const double v = f.unscale(a.G[q]);
+            if (!(v + eps < f.Gmin || v - eps > f.Gmax)) continue;
+            if (n < max) {
+                atom[n] = (int)i;
+                sf[n] = (int)q;
+                type[n] = f.type;
+                value[n] = v;
+                gmin[n] = f.Gmin;
+                gmax[n] = f.Gmax;
+            }
+            ++n;
+        }
+    }
+    return n;
+}
+
+// The low-neighbour check of Mode::calculateSymmetryFunctionGroups
+// (Mode.cpp:1667-1679): Atom::calculateNumNeighbors(Element::
+// getMinCutoffRadius()) (entries with d <= rmin, Atom.cpp:307-327) below
+// Element::getMinNeighbors() (Element.cpp:384-411: 1 for radial, 2 for
+// angular SFs).  Records (atom, count); returns their number.
+long oracle_low_neighbor_atoms(const oracle_ctx* c, long max, int* atom, int* num)
+{
+    long n = 0;
+    for (size_t i = 0; i < c->atoms.size(); ++i) {
+        const Atom& a = c->atoms[i];
+        const Element& e = c->el[a.element];
+        double rmin = std::numeric_limits<double>::max();
+        size_t need = 0;
+        for (const SF& f : e.sfs) {
+            rmin = std::min(rmin, f.rc);
+            need = std::max<size_t>(need, is_radial(f.type) ? 1 : 2);
+        }
+        size_t cnt = 0;
+        for (const Neighbor& nb : a.nbr)
+            if (nb.d <= rmin) ++cnt;
+        if (cnt < need) {
+            if (n < max) {
+                atom[n] = (int)i;
+                num[n] = (int)cnt;
+            }
+            ++n;
+        }
+    }
+    return n;
+}
+
+// KalmanFilter::update (KalmanFilter.cpp:119-185) on caller arrays, FP64
+// loops: P [n][n] row-major (only its lower triangle is read, as
+// selfAdjointView<Lower> does; the whole matrix is written), w [n],
+// H [n][m] (column q = Jacobian of observation q), xi [m]; par (in/out) =
+// {type (0 standard, 1 fading memory), eta, etatau, etamax, lambda, nu,
+// gamma, q, qtau, qmin}.  The m x m system is solved by Gauss-Jordan
+// elimination with partial pivoting (A^-1, then K = X A^-1).
+int oracle_kalman_update(int n, int m, double* P, double* w, const double* H, const double* xi, double* par)
+{
+    const int type = (int)par[0];
+    double& eta = par[1];
+    const double etatau = par[2], etamax = par[3];
+    double& lambda = par[4];
+    const double nu = par[5];
+    double& gamma = par[6];
+    double& q = par[7];
+    const double qtau = par[8], qmin = par[9];
+    auto Pl = [&](int i, int j) { return i >= j ? P[(size_t)i * n + j] : P[(size_t)j * n + i]; };
+    // X = P H
+    vector<double> X((size_t)n * m, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int c = 0; c < m; ++c) {
+            double v = 0.0;
+            for (int j = 0; j < n; ++j) v += Pl(i, j) * H[(size_t)j * m + c];
+            X[(size_t)i * m + c] = v;
+        }
+    // A = H^T X (+ R)
+    vector<double> A((size_t)m * m, 0.0);
+    for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) {
+            double v = 0.0;
+            for (int j = 0; j < n; ++j) v += H[(size_t)j * m + a] * X[(size_t)j * m + b];
+            A[(size_t)a * m + b] = v;
+        }
+    if (type == 0 && eta < etamax) eta *= std::exp(etatau);
+    for (int a = 0; a < m; ++a) A[(size_t)a * m + a] += type == 0 ? 1.0 / eta : lambda;
+    // A^-1
+    vector<double> Ai((size_t)m * m, 0.0);
+    for (int a = 0; a < m; ++a) Ai[(size_t)a * m + a] = 1.0;
+    for (int col = 0; col < m; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < m; ++r)
+            if (std::fabs(A[(size_t)r * m + col]) > std::fabs(A[(size_t)piv * m + col])) piv = r;
+        if (A[(size_t)piv * m + col] == 0.0) {
+            g_err = "kalman: singular innovation matrix";
+            return 1;
+        }
+        if (piv != col)
+            for (int k = 0; k < m; ++k) {
+                std::swap(A[(size_t)piv * m + k], A[(size_t)col * m + k]);
+                std::swap(Ai[(size_t)piv * m + k], Ai[(size_t)col * m + k]);
+            }
+        const double d = A[(size_t)col * m + col];
+        for (int k = 0; k < m; ++k) {
+            A[(size_t)col * m + k] /= d;
+            Ai[(size_t)col * m + k] /= d;
+        }
+        for (int r = 0; r < m; ++r)
+            if (r != col) {
+                const double f = A[(size_t)r * m + col];
+                for (int k = 0; k < m; ++k) {
+                    A[(size_t)r * m + k] -= f * A[(size_t)col * m + k];
+                    Ai[(size_t)r * m + k] -= f * Ai[(size_t)col * m + k];
+                }
+            }
+    }
+    // K = X A^-1
+    vector<double> K((size_t)n * m, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int b = 0; b < m; ++b) {
+            double v = 0.0;
+            for (int a = 0; a < m; ++a) v += X[(size_t)i * m + a] * Ai[(size_t)a * m + b];
+            K[(size_t)i * m + b] = v;
+        }
+    // P = (P - K X^T) (/ lambda) + q I, symmetric from the lower triangle
+    const double inv = type == 1 ? 1.0 / lambda : 1.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j <= i; ++j) {
+            double v = 0.0;
+            for (int b = 0; b < m; ++b) v += K[(size_t)i * m + b] * X[(size_t)j * m + b];
+            double p = P[(size_t)i * n + j] - v;
+            if (type == 1) p *= inv;
+            if (i == j) p += q;
+            P[(size_t)i * n + j] = p;
+            P[(size_t)j * n + i] = p;
+        }
+    // w = w + K xi
+    for (int i = 0; i < n; ++i) {
+        double v = 0.0;
+        for (int b = 0; b < m; ++b) v += K[(size_t)i * m + b] * xi[b];
+        w[i] += v;
+    }
+    if (q > qmin) q *= std::exp(-qtau);
+    if (type == 1) {
+        lambda = nu * lambda + 1.0 - nu;
+        gamma = 1.0 / (1.0 + lambda / gamma);
+    }
     return 0;
 }
 

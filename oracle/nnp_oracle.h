@@ -79,6 +79,30 @@ int    oracle_get_positions(const oracle_ctx* c, double* r);       /* 3N, after 
 int    oracle_get_G(const oracle_ctx* c, double* G, int ld);
 int    oracle_get_dEdG(const oracle_ctx* c, double* dEdG, int ld);
 
+/* Stored derivatives of the last oracle_predict (normalised units):
+ * self[N][ld_self][3] = Atom::dGdr; nbr[entry][ld_nbr][3] = Neighbor::dGdr of
+ * every neighbour entry in list order, slots in Element::symmetryFunctionTable
+ * order (Element.cpp:289-311), unused slots zero.  Either may be NULL. */
+int    oracle_get_dGdr(const oracle_ctx* c, double* self, int ld_self,
+                       double* nbr, int ld_nbr);
+
+/* Checks of the SF driver after oracle_predict: extrapolation warnings
+ * (Element::updateSymmetryFunctionStatistics, Element.cpp:462-541) as
+ * (atom, SF, type, unscaled value, Gmin, Gmax) records in (atom, SF) order,
+ * and low-neighbour atoms (Mode.cpp:1667-1679) as (atom, count) records.
+ * Both return the number of records and write at most max. */
+long   oracle_extrapolation_warnings(const oracle_ctx* c, long max, int* atom, int* sf,
+                                     int* type, double* value, double* gmin,
+                                     double* gmax);
+long   oracle_low_neighbor_atoms(const oracle_ctx* c, long max, int* atom, int* num);
+
+/* KalmanFilter::update (KalmanFilter.cpp:119-185) on caller arrays: P [n][n]
+ * (lower triangle read, symmetric result written), w [n], H [n][m],
+ * xi [m]; par (in/out) = {type (0 standard, 1 fading memory), eta, etatau,
+ * etamax, lambda, nu, gamma, q, qtau, qmin}. */
+int    oracle_kalman_update(int n, int m, double* P, double* w, const double* H,
+                            const double* xi, double* par);
+
 /* Neighbour list of atom i: count, then indices and image shifts
  * (bc0,bc1,bc2 per entry) in list order. */
 int    oracle_num_neighbors(const oracle_ctx* c, int i);

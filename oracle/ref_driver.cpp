@@ -16,6 +16,7 @@
 #include <cmath>
 #include <fstream>
 #include <map>
+#include <omp.h>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -235,6 +236,13 @@ extern "C" {
 
 const char* ref_last_error(void) { return g_err.c_str(); }
 
+// OpenMP threads of the parallel loops (the reference's OpenMP sites); the
+// CPU baseline times both the full core count and one thread.
+void ref_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+}
+
 ref_ctx* ref_create(const char* dir)
 {
     ref_ctx* c = new ref_ctx();
@@ -374,6 +382,94 @@ int ref_get_dEdG(const ref_ctx* c, double* d, int ld)
         for (size_t s = 0; s < c->atoms[i].dEdG.size() && (int)s < ld; ++s) d[i * ld + s] = c->atoms[i].dEdG[s];
     return 0;
 }
+// Atom::dGdr and Atom::Neighbor::dGdr of the reference's Atom objects after
+// ref_predict, in the layout of oracle_get_dGdr.
+int ref_get_dGdr(const ref_ctx* c, double* self, int ld_self, double* nbr, int ld_nbr)
+{
+    size_t p = 0;
+    for (size_t i = 0; i < c->atoms.size(); ++i) {
+        const Atom& a = c->atoms[i];
+        if (self)
+            for (int q = 0; q < ld_self; ++q)
+                for (int k = 0; k < 3; ++k)
+                    self[(i * ld_self + q) * 3 + k] = (size_t)q < a.dGdr.size() ? a.dGdr[q][k] : 0.0;
+        for (const Atom::Neighbor& n : a.neighbors) {
+            if (nbr)
+                for (int q = 0; q < ld_nbr; ++q)
+                    for (int k = 0; k < 3; ++k)
+                        nbr[(p * ld_nbr + q) * 3 + k] = (size_t)q < n.dGdr.size() ? n.dGdr[q][k] : 0.0;
+            ++p;
+        }
+    }
+    return 0;
+}
+// Extrapolation warnings through the reference's own
+// Element::updateSymmetryFunctionStatistics (Element.cpp:462-541) with
+// collectExtrapolationWarnings on, atoms in order (tags = atom indices);
+// records gathered from SymFncStatistics and put in (atom, SF) order.
+long ref_extrapolation_warnings(ref_ctx* c, long max, int* atom, int* sf, int* type, double* value, double* gmin,
+                                double* gmax)
+{
+    try {
+        for (auto& e : c->elements) {
+            e.statistics.collectStatistics = false;
+            e.statistics.collectExtrapolationWarnings = true;
+            e.statistics.writeExtrapolationWarnings = false;
+            e.statistics.stopOnExtrapolationWarnings = false;
+            e.statistics.resetExtrapolationWarnings();
+        }
+        for (size_t i = 0; i < c->atoms.size(); ++i) {
+            Atom& a = c->atoms[i];
+            a.tag = (int64_t)i;
+            c->elements.at(a.element).updateSymmetryFunctionStatistics(a);
+        }
+        struct Rec { int atom, sf, type; double value, gmin, gmax; };
+        vector<Rec> rec;
+        for (auto const& e : c->elements)
+            for (auto const& kv : e.statistics.data) {
+                auto const& d = kv.second;
+                for (size_t k = 0; k < d.valueEW.size(); ++k)
+                    rec.push_back(Rec{(int)d.indexAtomEW[k], (int)kv.first, (int)d.type, d.valueEW[k], d.Gmin, d.Gmax});
+            }
+        std::sort(rec.begin(), rec.end(), [](Rec const& x, Rec const& y) {
+            return x.atom != y.atom ? x.atom < y.atom : x.sf < y.sf;
+        });
+        for (size_t r = 0; r < rec.size() && (long)r < max; ++r) {
+            atom[r] = rec[r].atom;
+            sf[r] = rec[r].sf;
+            type[r] = rec[r].type;
+            value[r] = rec[r].value;
+            gmin[r] = rec[r].gmin;
+            gmax[r] = rec[r].gmax;
+        }
+        return (long)rec.size();
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+// The low-neighbour check (Mode.cpp:1667-1679) with the reference's
+// Atom::calculateNumNeighbors and Element::getMinCutoffRadius /
+// getMinNeighbors.
+long ref_low_neighbor_atoms(const ref_ctx* c, long max, int* atom, int* num)
+{
+    long n = 0;
+    for (size_t i = 0; i < c->atoms.size(); ++i) {
+        Atom const& a = c->atoms[i];
+        Element const& e = c->elements.at(a.element);
+        const size_t cnt = a.calculateNumNeighbors(e.getMinCutoffRadius());
+        if (cnt < e.getMinNeighbors()) {
+            if (n < max) {
+                atom[n] = (int)i;
+                num[n] = (int)cnt;
+            }
+            ++n;
+        }
+    }
+    return n;
+}
+
 int ref_get_atomic_energies(const ref_ctx* c, double* e)
 {
     for (size_t i = 0; i < c->atoms.size(); ++i) e[i] = c->atoms[i].energy;

@@ -6,11 +6,18 @@
 //       CPU only: parse, remap and normalise a structure, write it back.
 //   test_facade steps <model_dir> <input.data> [mixed]
 //       GPU: neighbour list, SF groups, NNs, energy and forces step by step.
+//   test_facade dgdr <model_dir> <input.data> <out.bin>
+//       GPU: SF groups with setCopyNeighbors/setCopyDerivatives(true); writes
+//       (model units) for every atom its Atom::dGdr rows, then for every
+//       neighbour entry in list order its index and Neighbor::dGdr rows, as
+//       int64 counts followed by doubles (layout in the code below).
 #include <cstdio>
 #include <cstring>
 #include <exception>
 #include <fstream>
+#include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "nnpx_mode.hpp"
 
@@ -90,6 +97,41 @@ int main(int argc, char** argv)
             p.addEnergyOffset(s, false);
             p.addEnergyOffset(s, true);
             print_structure(s, true);
+            return 0;
+        }
+        if (argc >= 5 && std::strcmp(argv[1], "dgdr") == 0) {
+            nnpx::Prediction p;
+            p.directory = argv[2];
+            p.setup();
+            p.setCopyNeighbors(true);
+            p.setCopyDerivatives(true);
+            p.readStructureFromFile(argv[3]);
+            nnpx::Structure& s = p.structure;
+            s.calculateNeighborList(p.getMaxCutoffRadius());
+            p.calculateSymmetryFunctionGroups(s, true);
+            // per atom: int64 nsf, nsf*3 doubles, int64 numNeighbors; per
+            // neighbour: int64 index, int64 n, n*3 doubles
+            std::FILE* f = std::fopen(argv[4], "wb");
+            if (!f) throw std::runtime_error("cannot write output");
+            auto put_i = [&](int64_t v) { std::fwrite(&v, sizeof v, 1, f); };
+            auto put_v = [&](std::vector<nnpx::Vec3D> const& v) {
+                put_i((int64_t)v.size());
+                for (auto const& x : v) {
+                    double const w[3] = {x[0], x[1], x[2]};
+                    std::fwrite(w, sizeof(double), 3, f);
+                }
+            };
+            for (auto const& a : s.atoms) {
+                put_v(a.dGdr);
+                put_i((int64_t)a.neighbors.size());
+                for (auto const& n : a.neighbors) {
+                    put_i((int64_t)n.index);
+                    put_v(n.dGdr);
+                }
+            }
+            std::fclose(f);
+            std::printf("{\"numAtoms\": %zu, \"flags\": [%d, %d]}\n", s.numAtoms, (int)s.hasSymmetryFunctions,
+                        (int)s.hasSymmetryFunctionDerivatives);
             return 0;
         }
     } catch (std::exception const& e) {

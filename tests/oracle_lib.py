@@ -121,6 +121,25 @@ class Oracle:
     def phase_times(self):
         return [self.lib().oracle_phase_time(self.h, p) for p in range(5)]
 
+    def dGdr(self, ld_nbr):
+        """Stored dG/dr of the last predict: (self [N, max_sf, 3], neighbour
+        entries in list order [P, ld_nbr, 3], slots in symmetryFunctionTable
+        order, unused slots zero)."""
+        L = self.lib()
+        total = sum(L.oracle_num_neighbors(self.h, i) for i in range(self.n))
+        s = np.zeros((self.n, self.max_sf, 3))
+        nb = np.zeros((total, ld_nbr, 3))
+        self._ck(L.oracle_get_dGdr(self.h, _vp(s), self.max_sf, _vp(nb), ld_nbr))
+        return s, nb
+
+    def extrapolation_warnings(self):
+        """(atom, sf, type, value, gmin, gmax) records of the last predict,
+        in (atom, SF) order (Element::updateSymmetryFunctionStatistics)."""
+        return _records(self.lib().oracle_extrapolation_warnings, self.h)
+
+    def low_neighbor_atoms(self):
+        return _low_records(self.lib().oracle_low_neighbor_atoms, self.h)
+
     # training gradient (NeuralNetwork::calculateDEdc / calculateDFdc restated)
     def num_connections(self):
         return sum(self.lib().oracle_num_connections(self.h, e) for e in range(self.ne))
@@ -159,6 +178,48 @@ class Oracle:
     def _ck(self, rc):
         if rc != 0:
             raise RuntimeError(self.lib().oracle_last_error().decode())
+
+
+_EW_DTYPE = [("atom", np.int32), ("sf", np.int32), ("type", np.int32), ("value", float), ("gmin", float),
+             ("gmax", float)]
+
+
+def _records(fn, h):
+    fn.restype = ctypes.c_long
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_long] + [ctypes.c_void_p] * 6
+    n = fn(h, 0, None, None, None, None, None, None)
+    if n < 0:
+        raise RuntimeError("extrapolation warnings failed")
+    a, q, t = (np.zeros(n, np.int32) for _ in range(3))
+    v, lo, hi = (np.zeros(n) for _ in range(3))
+    fn(h, n, _vp(a), _vp(q), _vp(t), _vp(v), _vp(lo), _vp(hi))
+    out = np.zeros(n, dtype=_EW_DTYPE)
+    out["atom"], out["sf"], out["type"], out["value"], out["gmin"], out["gmax"] = a, q, t, v, lo, hi
+    return out
+
+
+def _low_records(fn, h):
+    fn.restype = ctypes.c_long
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_void_p]
+    n = fn(h, 0, None, None)
+    a = np.zeros(n, np.int32)
+    c = np.zeros(n, np.int32)
+    fn(h, n, _vp(a), _vp(c))
+    return a, c
+
+
+def kalman_update(P, w, H, xi, par):
+    """The oracle's KalmanFilter::update on copies: returns (P, w, par)."""
+    L = Oracle.lib()
+    L.oracle_kalman_update.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 5
+    P = np.array(P, dtype=np.float64, order="C", copy=True)
+    w = np.array(w, dtype=np.float64, copy=True)
+    H = np.ascontiguousarray(H, dtype=np.float64).reshape(len(w), -1)
+    xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1)
+    par = np.array(par, dtype=np.float64, copy=True)
+    if L.oracle_kalman_update(len(w), H.shape[1], _vp(P), _vp(w), _vp(H), _vp(xi), _vp(par)):
+        raise RuntimeError(L.oracle_last_error().decode())
+    return P, w, par
 
 
 def cutoff_fdf(ctype, rc, alpha, r):
@@ -204,6 +265,9 @@ class RefLib:
             raise RuntimeError(self.lib().ref_last_error().decode())
         self.n = len(e)
 
+    def set_threads(self, n):
+        self.lib().ref_set_threads(int(n))
+
     def predict(self):
         if self.lib().ref_predict(self.h):
             raise RuntimeError(self.lib().ref_last_error().decode())
@@ -228,6 +292,23 @@ class RefLib:
 
     def phase_times(self):
         return [self.lib().ref_phase_time(self.h, p) for p in range(5)]
+
+    def extrapolation_warnings(self):
+        """The reference's Element::updateSymmetryFunctionStatistics records,
+        (atom, SF) order."""
+        return _records(self.lib().ref_extrapolation_warnings, self.h)
+
+    def low_neighbor_atoms(self):
+        return _low_records(self.lib().ref_low_neighbor_atoms, self.h)
+
+    def dGdr(self, ld_self, ld_nbr, total):
+        """Atom::dGdr / Atom::Neighbor::dGdr of the reference's Atom objects
+        (layout of Oracle.dGdr; total = number of neighbour entries)."""
+        s = np.zeros((self.n, ld_self, 3))
+        nb = np.zeros((total, ld_nbr, 3))
+        if self.lib().ref_get_dGdr(self.h, _vp(s), ld_self, _vp(nb), ld_nbr):
+            raise RuntimeError(self.lib().ref_last_error().decode())
+        return s, nb
 
     def dPdc(self, prop, n_conn):
         """After predict(): Training::dPdc rows through the reference's

@@ -219,7 +219,8 @@ def test_step_api_equals_evaluate(water_dir):
     m.calculate_atomic_neural_networks(True)
     m.calculate_energy()
     m.calculate_forces()
-    # dG/dr pair terms are summed with LDS atomics: equal up to summation order
+    # evaluate recomputes the derivatives (fused), the step calls store them:
+    # equal up to summation order
     assert np.abs(m.energies() - E1).max() <= 1e-14 * np.abs(E1).max()
     assert np.abs(m.forces() - F1).max() <= 1e-12
 
@@ -244,6 +245,38 @@ def test_batched_small_cells(force_mode, water_dir):
         o.predict(force_mode, BRUTE)
         assert abs(E[s] - o.energy()) <= E_REL * abs(o.energy()), s
         assert np.abs(F[a:b] - o.forces()).max() <= F_ABS, s
+
+
+@pytest.mark.parametrize("path", [0, 1])
+@pytest.mark.parametrize("force_mode", [PULL, PUSH])
+def test_plain_force_paths(path, force_mode, water_dir, tmp_path_factory):
+    """Both force paths of the plain form (NNPX_OPT_PLAIN_FORCES: 0 the fused
+    FP64 recompute, 1 the stored dG/dr) against the oracle: a batch of
+    uneven cells including ones smaller than rc (self images), and the
+    weighted (types 12/13 with 2/3 and 23-25) and compact (20-22) models."""
+    say("case: plain force path %d, force mode %d" % (path, force_mode))
+    pos, elem, boxes, off = synth.random_batch(n_struct=8, nmin=16, nmax=90, seed=synth.SEED_BATCH + 9)
+    models = [("water", water_dir),
+              ("weighted", weighted_model(str(tmp_path_factory.mktemp("w") / "m"))),
+              ("compact", compact_model(str(tmp_path_factory.mktemp("c") / "m")))]
+    for name, d in models:
+        m = n2p2_amd.Mode(d)
+        m.set_option(n2p2_amd.OPT_PLAIN_FORCES, path)
+        m.set_option(n2p2_amd.OPT_FORCE_MODE, force_mode)
+        m.set_structures(pos, elem, boxes, off)
+        m.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+        E, F = m.energies(), m.forces()
+        worst_e = worst_f = 0.0
+        o = Oracle(d)
+        for s in range(len(off) - 1):
+            a, b = off[s], off[s + 1]
+            o.set_structure(pos[a:b], elem[a:b], boxes[s])
+            o.predict(force_mode, BRUTE)
+            worst_e = max(worst_e, abs(E[s] - o.energy()) / abs(o.energy()))
+            worst_f = max(worst_f, float(np.abs(F[a:b] - o.forces()).max()))
+        say("  %s: max E rel %.2e  max |dF| %.2e" % (name, worst_e, worst_f))
+        assert worst_e <= E_REL
+        assert worst_f <= F_ABS
 
 
 def test_update_positions(water_dir):

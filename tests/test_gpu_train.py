@@ -226,3 +226,29 @@ def test_weight_vjp_wide_network(wide_case, precision):
         say("  structure %d (%d atoms): max rel %.2e" % (s, b - a, rel))
         worst = max(worst, rel)
     assert worst <= (1e-9 if precision == n2p2_amd.PLAIN_FP64 else 2e-3)
+
+
+@pytest.mark.parametrize("precision", [n2p2_amd.PLAIN_FP64, n2p2_amd.MIXED])
+def test_loss_gradient_wide_network(precision, tmp_path):
+    """nnpx_loss_gradient (one summed gradient) with the 50-100-100-1 network
+    whose FP64 connection gradient exceeds the LDS accumulator: every atom's
+    connections go into one row through the global-atomic variant; three
+    66-78 atom water structures against the oracle's loss gradient."""
+    d = synth.water_model(str(tmp_path / "wide"), hidden=(100, 100))
+    t = synth.water_model(str(tmp_path / "wide_teacher"), hidden=(100, 100), seed=synth.SEED_TEACHER)
+    say("case: loss gradient, 50-100-100-1 network, 3 water structures, precision %d" % precision)
+    pos, elem, boxes, off = synth.water_set(n_struct=3, mmin=22, mmax=26, seed=synth.SEED_WATERSET + 5)
+    E_ref, F_ref = targets(t, pos, elem, boxes, off)
+    lo, go = oracle_gradient(d, pos, elem, boxes, off, E_ref, F_ref)
+    m = n2p2_amd.Mode(d)
+    m.set_structures(pos, elem, boxes, off)
+    loss, g = m.loss_gradient(E_ref, F_ref, BETA, precision)
+    rel = float(np.abs(g - go).max() / np.abs(go).max())
+    lrel = np.abs(loss - lo) / np.abs(lo)
+    say("  loss rel %s  grad max rel %.2e" % (lrel, rel))
+    if precision == n2p2_amd.PLAIN_FP64:
+        assert lrel.max() <= 1e-10
+        assert rel <= 1e-9
+    else:
+        assert lrel.max() <= 1e-4
+        assert rel <= 2e-3
