@@ -319,6 +319,26 @@ nnpx_status nnpx_autotune(nnpx_ctx* ctx);
 nnpx_status nnpx_get_tuning(const nnpx_ctx* ctx, int* values);
 nnpx_status nnpx_set_tuning(nnpx_ctx* ctx, const int* values);
 
+/* ---- evaluation without device allocation ----------------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nnpx_set_structures + nnpx_evaluate + nnpx_get_energy
+ * (+ nnpx_get_forces) in one call, with every intermediate buffer (neighbour
+ * and cell lists, symmetry functions, dE/dG, energies, forces) carved from a
+ * file-scope device array of 64 MiB that each GPU holds until the process
+ * ends; the call itself allocates no device memory.  Same kernels and results
+ * as the allocating calls.  Inputs and outputs are host arrays as in those
+ * calls: energy[n_struct], forces[3N] (with NNPX_EVAL_FORCES).  An input whose
+ * buffers exceed the array fails with NNPX_ERR_RANGE; a context set up for MD
+ * is refused (NNPX_ERR_STATE).  Afterwards the context holds no structures.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (from several contexts or threads) race. */
+nnpx_status nnpx_predict_noalloc(nnpx_ctx* ctx, int precision, unsigned flags,
+                                 int n_struct, const int64_t* atom_offsets,
+                                 const double* pos, const int* elem,
+                                 const double* box, const int* periodic,
+                                 double* energy, double* forces);
+
 /* ---- measurement ------------------------------------------------------ */
 /* No counterpart in the reference: multiplies two known 16x16 matrices with
  * the bf16 MFMA operand layout the NN kernels use and returns the max abs

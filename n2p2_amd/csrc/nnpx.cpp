@@ -20,9 +20,39 @@
 
 using namespace nnpx;
 
+// Device memory of nnpx_predict_noalloc: every GPU the library runs on
+// holds its own copy of this array until the process ends.  Host code never
+// passes its name to a kernel or a copy: it uses the device address
+// hipGetSymbolAddress returns for the current device (scratch_address).
+// kScratchBytes is the most device memory one such call may use.
+constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_create_error;
+
+// Device address of `scratch` on each device, looked up once per device
+// (with that device current) when the first context on it is created.
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;
+
+hipError_t scratch_address(int device, unsigned char** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it != g_scratch_ptr.end()) {
+        *out = it->second;
+        return hipSuccess;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    *out = static_cast<unsigned char*>(p);
+    return hipSuccess;
+}
+
 struct Timer {
     std::string name;
     hipEvent_t a, b;
@@ -85,6 +115,12 @@ struct nnpx_ctx {
     // caller-provided neighbour lists (LAMMPS path): forces on ntotal atoms
     bool external = false;
     int ntotal = 0;
+    // no-allocation calls: named buffers carved from the device's file-scope
+    // scratch array instead of hipMalloc (see nnpx_predict_noalloc)
+    unsigned char* scratch = nullptr;  // this device's copy of the array
+    bool arena = false;
+    size_t arena_used = 0;
+    std::map<std::string, DevBuf> abufs;
     // Kalman filter weight updaters (one over all connections, or one per
     // element): state size, first connection, scalars; P on the device
     struct KF {
@@ -149,6 +185,23 @@ template <typename T>
 T* buf(nnpx_ctx* c, const std::string& name, size_t count)
 {
     const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    if (c->arena) {
+        // bump allocation in the scratch array; a buffer that grows within a
+        // call moves to fresh space (the old one is not reused before the
+        // next call)
+        DevBuf& a = c->abufs[name];
+        if (a.bytes < bytes) {
+            const size_t off = (c->arena_used + 255) & ~(size_t)255;
+            if (off > kScratchBytes || bytes > kScratchBytes - off)
+                fail(NNPX_ERR_RANGE, "input too large for the no-allocation scratch buffer (" +
+                                         std::to_string(kScratchBytes >> 20) + " MiB): buffer " + name + " needs " +
+                                         std::to_string(bytes) + " bytes at offset " + std::to_string(off));
+            a.p = c->scratch + off;
+            a.bytes = bytes;
+            c->arena_used = off + bytes;
+        }
+        return static_cast<T*>(a.p);
+    }
     DevBuf& b = c->bufs[name];
     if (b.bytes < bytes) {
         drop_graph(c);
@@ -174,8 +227,9 @@ T* model_table(nnpx_ctx* c, const std::string& name)
 template <typename T>
 T* existing(nnpx_ctx* c, const std::string& name)
 {
-    auto it = c->bufs.find(name);
-    return it == c->bufs.end() ? nullptr : static_cast<T*>(it->second.p);
+    auto& m = c->arena ? c->abufs : c->bufs;
+    auto it = m.find(name);
+    return it == m.end() ? nullptr : static_cast<T*>(it->second.p);
 }
 
 hipEvent_t get_event(nnpx_ctx* c)
@@ -726,6 +780,13 @@ nnpx_status nnpx_create(nnpx_ctx** ctx, int device)
     c->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         g_create_error = "could not create a HIP stream";
+        delete c;
+        return NNPX_ERR_HIP;
+    }
+    const hipError_t se = scratch_address(device, &c->scratch);
+    if (se != hipSuccess) {
+        g_create_error = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(se);
+        (void)hipStreamDestroy(c->stream);
         delete c;
         return NNPX_ERR_HIP;
     }
@@ -2154,6 +2215,48 @@ nnpx_status nnpx_md_energies(nnpx_ctx* c, double* epot, double* ekin, int64_t ma
         if (n > 0 && cnt[0] > kMDHistory) fail(NNPX_ERR_RANGE, "more steps than the kept energy history");
         if (epot && n > 0) copy_out(c, epot, existing<double>(c, "md_epot"), sizeof(double) * n);
         if (ekin && n > 0) copy_out(c, ekin, existing<double>(c, "md_ekin"), sizeof(double) * n);
+    });
+}
+
+nnpx_status nnpx_predict_noalloc(nnpx_ctx* c, int precision, unsigned flags, int n_struct,
+                                 const int64_t* atom_offsets, const double* pos, const int* elem,
+                                 const double* box, const int* periodic, double* energy, double* forces)
+{
+    return guard(c, [&] {
+        need_model(c);
+        if (precision != NNPX_PLAIN_FP64 && precision != NNPX_MIXED) fail(NNPX_ERR_ARGUMENT, "bad precision");
+        if (!energy || ((flags & NNPX_EVAL_FORCES) && !forces)) fail(NNPX_ERR_ARGUMENT, "null output");
+        if (c->md_ready || c->gexec) fail(NNPX_ERR_STATE, "no-allocation call on a context set up for MD");
+        if (!c->scratch) fail(NNPX_ERR_STATE, "scratch buffer address unknown");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        // The call works in the scratch array; afterwards the context holds
+        // no structures (its own buffers were not touched).
+        const bool timing = c->timing;
+        struct Scope {
+            nnpx_ctx* c;
+            bool timing;
+            ~Scope()
+            {
+                (void)hipStreamSynchronize(c->stream);
+                c->arena = false;
+                c->abufs.clear();
+                c->timing = timing;
+                c->N = c->S = 0;
+                c->external = false;
+                c->nbr_valid = c->rev_valid = false;
+                c->stage_cap = 0;
+                invalidate(c);
+            }
+        } scope{c, timing};
+        c->timing = false;  // no event objects are created
+        c->arena = true;
+        c->arena_used = 0;
+        c->abufs.clear();
+        set_structures(c, n_struct, atom_offsets, pos, elem, box, periodic);
+        evaluate(c, precision, flags, false);
+        settle(c);
+        copy_out(c, energy, existing<double>(c, "E"), sizeof(double) * c->S);
+        if (flags & NNPX_EVAL_FORCES) copy_out(c, forces, existing<double>(c, "F"), sizeof(double) * 3 * c->N);
     });
 }
 
