@@ -40,7 +40,7 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s (E+F) mixed vs FP64 GPU form at force RMS err <=1e-3 eV/A"
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X FP64 vector
 PEAK_FP32 = 157.3   # TFLOP/s, MI355X FP32 vector
-KERNELS = ("nbr", "rev", "sf64", "nn64", "energy", "force64", "force64f", "sf32", "nnbf16", "force32")
+KERNELS = ("nbr", "rev", "sf64", "sf64v", "nn64", "energy", "force64", "force64f", "sf32", "nnbf16", "force32")
 NN_MACS = {"water": 50 * 25 + 25 * 25 + 25 * 1, "fcc": 80 * 30 + 30 * 30 + 30 * 1}
 
 

@@ -30,6 +30,7 @@
 // Neighbour sums are pushed to the force array with FP64 device atomics;
 // the pull form drops the atom's own periodic images (Mode.cpp:1957).
 #include "kernels.hpp"
+#include "mixed.hpp"
 #include "nnpx.h"
 
 namespace nnpx {
@@ -48,19 +49,23 @@ struct Stage64 {
     unsigned short* la;  // group lists
     unsigned short* lb;
     unsigned char* el;   // neighbour element
+    double2* pw;         // [kPw64] power-path term weights (force) or term sums (values)
     int trip;
 };
 
+constexpr int kPw64 = 32;
+
 __host__ __device__ constexpr size_t stage64_bytes(int capL, int trip)
 {
-    return (size_t)capL * (32 + 32 + 24 + 2 + 2 + 1) + (size_t)trip * 4;
+    return (size_t)kPw64 * 16 + (size_t)capL * (32 + 32 + 24 + 2 + 2 + 1) + (size_t)trip * 4;
 }
 
 __device__ inline Stage64 carve64(double4* base, int capL, int trip)
 {
     Stage64 S;
     S.trip = trip;
-    S.g = base;
+    S.pw = reinterpret_cast<double2*>(base);
+    S.g = base + kPw64 / 2;
     S.c = S.g + capL;
     S.ax = reinterpret_cast<double*>(S.c + capL);
     S.ay = S.ax + capL;
@@ -177,8 +182,15 @@ __device__ inline void for_each_triplet64(const Stage64& S, int nA, int nB, bool
     }
 }
 
-template <int MAXM>
-__global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, SysView s,
+// NB0 > 0: the power path (angular groups whose members are
+// exp(-eta_h sum r^2) (1 + lambda_s cos)^(2^b), b < NB0 resp. NB1, rs = 0):
+// per triplet one exponential per eta and the powers by repeated squaring;
+// the members' weights are folded per term (wb = sum w, wa = sum zeta lambda
+// w), and (1 + lambda cos)^(2^b - 1) is carried beside the power so A needs
+// no division.
+template <int MAXM, int NB0, int NB1>
+__global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
+                                                  const GroupPow64* __restrict__ GP, SysView s,
                                                   const double* __restrict__ fw, int ldG, double* __restrict__ F,
                                                   int push, int capL, int trip)
 {
@@ -249,6 +261,22 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, S
         const bool wide = type == 9;
         const int nA = make_list64(S, n, G.e1, G.rc, S.la);
         const int nB = same ? nA : make_list64(S, n, G.e2, G.rc, S.lb);
+        GroupPow64 gp{};
+        if constexpr (NB0 > 0) {
+            gp = GP[E.grp0 + gi];
+            // term weights: wb = sum of the members' w, wa = sum of zeta lambda w
+            if (lane < 2 * (NB0 + NB1)) {
+                double sb = 0.0, sa = 0.0;
+                for (int q = 0; q < nmem; ++q)
+                    if (MF[G.mem0 + q].beta == lane) {
+                        const double wq = wi[mem[q].sf];
+                        sb += wq;
+                        sa += wq * mem[q].zetaLambda;
+                    }
+                S.pw[lane] = make_double2(sb, sa);
+            }
+            __syncthreads();
+        }
         int cur = -1;
         double rx = 0.0, ry = 0.0, rz = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
         double4 gj = make_double4(0.0, 0.0, 0.0, 0.0), cj = gj;
@@ -294,6 +322,39 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, S
             double A = 0.0, B = 0.0, C0 = 0.0, C1 = 0.0, C2 = 0.0, C3 = 0.0;
             double vexp = 0.0, fij = 0.0, fik = 0.0, fjk = 0.0;
             bool shifted = false;
+            if constexpr (NB0 > 0) {
+                constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
+                const double E0 = exp(-gp.eta[0] * r2sum);
+                const double E1 = NB1 > 0 ? exp(-gp.eta[1] * r2sum) : 0.0;
+                double Q0 = 0.0, Q1 = 0.0;
+#pragma unroll
+                for (int sl = 0; sl < 2; ++sl) {
+                    const double pl = 1.0 + gp.lam[sl] * cost;
+                    const bool pos = pl > 0.0;
+                    double pp = pos ? pl : 0.0;   // pl^(2^b)
+                    double qq = pos ? 1.0 : 0.0;  // pl^(2^b - 1)
+                    double qa0 = 0.0, qa1 = 0.0;
+#pragma unroll
+                    for (int b = 0; b < NBM; ++b) {
+                        if (b < NB0) {
+                            const double2 wv = S.pw[sl * NB0 + b];
+                            Q0 += wv.x * pp;
+                            qa0 += wv.y * qq;
+                        }
+                        if (b < NB1) {
+                            const double2 wv = S.pw[2 * NB0 + sl * NB1 + b];
+                            Q1 += wv.x * pp;
+                            qa1 += wv.y * qq;
+                        }
+                        qq *= pp;
+                        pp *= pp;
+                    }
+                    A += E0 * qa0 + E1 * qa1;
+                }
+                const double B0 = E0 * Q0, B1 = E1 * Q1;
+                B = B0 + B1;
+                C0 = gp.twoEta[0] * B0 + gp.twoEta[1] * B1;
+            } else {
 #pragma unroll
             for (int q = 0; q < MAXM; ++q) {
                 if (q < nmem) {
@@ -327,6 +388,7 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, S
                         }
                     }
                 }
+            }
             }
             C1 += C0;
             C2 += C0;
@@ -367,6 +429,192 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, S
     }
 }
 
+// Scaled SF values of the plain form (what k_sf64 computes without
+// derivatives) on the same staging and triplet walk as the force kernel:
+// radial members summed by the lanes owning the neighbours, angular members
+// per lane over its chunk of compacted triplets (no idle lanes on rejected
+// candidates), one wave reduction per member.  Terms follow the
+// reference's expressions (SymGrpExpRad.cpp:190-196, SymGrpExpAngn.cpp:
+// 260-300): exp(-eta (r - rs)^2) fc, and vexp (1 + lambda cos)^(zeta - 1)
+// (1 + lambda cos) fc_ij fc_ik fc_jk with the exponential shared by members
+// of equal (eta, rs).
+template <int MAXM, int NB0, int NB1>
+__global__ __launch_bounds__(64) void k_values64(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
+                                                  const GroupPow64* __restrict__ GP, SysView s,
+                                                  double* __restrict__ G, int ldG, int capL, int trip)
+{
+    extern __shared__ double4 sm64[];
+    const Stage64 S = carve64(sm64, capL, trip);
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = s.elem[i];
+    const ElementT& E = M->el[e];
+    const int ctype = M->cutoffType;
+    const double tanhPre = M->tanhPre;
+    const StructT& St = s.st[s.sid[i]];
+    const int n = min(s.nb_n[i], capL);
+    const size_t base = (size_t)i * s.cap;
+    for (int k = lane; k < n; k += kWave) {
+        const int j = s.nb_j[base + k];
+        double dr[3];
+        entry_vector(s, i, base + k, St, dr);
+        const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+        S.g[k] = make_double4(dr[0], dr[1], dr[2], d);
+        S.c[k] = make_double4(1.0 / d, 0.0, 0.0, M->Zel[s.elem[j]]);
+        S.el[k] = (unsigned char)s.elem[j];
+    }
+    double* Gi = G + (size_t)i * ldG;
+    __syncthreads();
+
+    for (int gi = 0; gi < E.ngrp; ++gi) {
+        const GroupT& GR = M->grp[E.grp0 + gi];
+        const int type = GR.type;
+        if (type != 2 && type != 12 && type != 3 && type != 9 && type != 13) continue;  // compact: k_sfcomp
+        const bool radial = type == 2 || type == 12;
+        const bool wt = type == 12 || type == 13;
+        for (int k = lane; k < n; k += kWave) {
+            double f = 0.0, df = 0.0;
+            const int ek = S.el[k];
+            if ((wt || ek == GR.e1 || ek == GR.e2) && S.g[k].w < GR.rc) cutoff_fdf(ctype, GR, tanhPre, S.g[k].w, f, df);
+            S.c[k].y = f;
+        }
+        __syncthreads();
+        const int nmem = GR.nmem;
+        const MemberT* mem = M->mem + GR.mem0;
+        if (radial) {
+            constexpr int CH = 8;
+            for (int q0 = 0; q0 < nmem; q0 += CH) {
+                double acc[CH];
+#pragma unroll
+                for (int q = 0; q < CH; ++q) acc[q] = 0.0;
+                for (int k = lane; k < n; k += kWave) {
+                    const double r = S.g[k].w;
+                    if ((!wt && S.el[k] != GR.e1) || !(r < GR.rc)) continue;
+                    const double fc = S.c[k].y;
+#pragma unroll
+                    for (int q = 0; q < CH; ++q)
+                        if (q0 + q < nmem) {
+                            const MemberT& mb = mem[q0 + q];
+                            double pexp = exp(-mb.eta * (r - mb.rs) * (r - mb.rs));
+                            if (wt) pexp = S.c[k].w * pexp;
+                            acc[q] += pexp * fc;
+                        }
+                }
+#pragma unroll
+                for (int q = 0; q < CH; ++q)
+                    if (q0 + q < nmem) {
+                        const double v = wave_sum(acc[q]);
+                        if (lane == 0) {
+                            const MemberT& mb = mem[q0 + q];
+                            Gi[mb.sf] = scale_sf(M->scale[E.sf0 + mb.sf], v);
+                        }
+                    }
+            }
+            __syncthreads();
+            continue;
+        }
+        const bool same = GR.e1 == GR.e2;
+        const bool wide = type == 9;
+        const int nA = make_list64(S, n, GR.e1, GR.rc, S.la);
+        const int nB = same ? nA : make_list64(S, n, GR.e2, GR.rc, S.lb);
+        if constexpr (NB0 > 0) {
+            // power path: term sums pfc E_h (1 + lambda_s cos)^(2^b) per lane,
+            // one wave reduction per used term, members take their term
+            constexpr int NBAS = 2 * (NB0 + NB1);
+            constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
+            const GroupPow64 gp = GP[E.grp0 + gi];
+            double part[NBAS];
+#pragma unroll
+            for (int b = 0; b < NBAS; ++b) part[b] = 0.0;
+            for_each_triplet64(S, nA, nB, same, wide ? INFINITY : GR.rc2, [&](int j, int k) {
+                const double4 gj = S.g[j], gk = S.g[k];
+                const double4 cj = S.c[j], ck = S.c[k];
+                const double x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
+                const double r2jk = x3 * x3 + y3 * y3 + z3 * z3;
+                double fcjk = 1.0, dfcjk = 0.0;
+                if (!wide) cutoff_fdf(ctype, GR, tanhPre, sqrt(r2jk), fcjk, dfcjk);
+                const double costijk = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * (cj.x * ck.x);
+                const double pfc = (wt ? cj.w * ck.w : 1.0) * cj.y * ck.y * fcjk;
+                const double r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.0 : r2jk);
+                const double E0 = exp(-gp.eta[0] * r2sum) * pfc;
+                const double E1 = NB1 > 0 ? exp(-gp.eta[1] * r2sum) * pfc : 0.0;
+#pragma unroll
+                for (int sl = 0; sl < 2; ++sl) {
+                    const double pl = 1.0 + gp.lam[sl] * costijk;
+                    double p = pl > 0.0 ? pl : 0.0;
+#pragma unroll
+                    for (int b = 0; b < NBM; ++b) {
+                        if (b < NB0) part[sl * NB0 + b] += E0 * p;
+                        if (b < NB1) part[2 * NB0 + sl * NB1 + b] += E1 * p;
+                        p *= p;
+                    }
+                }
+            }, [] {});
+#pragma unroll
+            for (int b = 0; b < NBAS; ++b)
+                if ((gp.used >> b) & 1) {
+                    const double v = wave_sum(part[b]);
+                    if (lane == 0) S.pw[b].x = v;
+                }
+            __syncthreads();
+            for (int q = lane; q < nmem; q += kWave) {
+                const MemberT& mb = mem[q];
+                Gi[mb.sf] = scale_sf(M->scale[E.sf0 + mb.sf], S.pw[MF[GR.mem0 + q].beta].x * mb.gnorm);
+            }
+            __syncthreads();
+            continue;
+        }
+        double acc[MAXM];
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q) acc[q] = 0.0;
+        for_each_triplet64(S, nA, nB, same, wide ? INFINITY : GR.rc2, [&](int j, int k) {
+            const double4 gj = S.g[j], gk = S.g[k];
+            const double4 cj = S.c[j], ck = S.c[k];
+            const double x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
+            const double r2jk = x3 * x3 + y3 * y3 + z3 * z3;
+            const double rij = gj.w, rik = gk.w;
+            double rjk = 0.0, fcjk = 1.0, dfcjk = 0.0;
+            if (!wide) {
+                rjk = sqrt(r2jk);
+                cutoff_fdf(ctype, GR, tanhPre, rjk, fcjk, dfcjk);
+            }
+            const double costijk = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * (cj.x * ck.x);
+            const double pfc = (wt ? cj.w * ck.w : 1.0) * cj.y * ck.y * fcjk;
+            const double r2sum = rij * rij + rik * rik + (wide ? 0.0 : r2jk);
+            double vexp = 0.0;
+#pragma unroll
+            for (int q = 0; q < MAXM; ++q)
+                if (q < nmem) {
+                    const MemberT& mb = mem[q];
+                    if (mb.calcExp) {
+                        if (mb.rs > 0.0 || wt) {
+                            const double a = rij - mb.rs, b = rik - mb.rs, c = rjk - mb.rs;
+                            vexp = exp(-mb.eta * (a * a + b * b + (wide ? 0.0 : c * c)));
+                        } else {
+                            vexp = exp(-mb.eta * r2sum);
+                        }
+                    }
+                    const double plambda = 1.0 + mb.lambda * costijk;
+                    if (plambda > 0.0) {
+                        const double fg =
+                            vexp * (mb.useIntPow ? pow_int(plambda, mb.zetaInt - 1) : pow(plambda, mb.zeta - 1.0));
+                        acc[q] += fg * plambda * pfc;
+                    }
+                }
+        }, [] {});
+#pragma unroll
+        for (int q = 0; q < MAXM; ++q)
+            if (q < nmem) {
+                const double v = wave_sum(acc[q]);
+                if (lane == 0) {
+                    const MemberT& mb = mem[q];
+                    Gi[mb.sf] = scale_sf(M->scale[E.sf0 + mb.sf], v * mb.gnorm);
+                }
+            }
+        __syncthreads();
+    }
+}
+
 // w[i][q] = dE/dG[i][q] times the member's derivative scale.
 __global__ void k_weights64(const int* __restrict__ elem, const double* __restrict__ dEdG,
                             const double* __restrict__ fac, int ldG, size_t total, double* __restrict__ w)
@@ -386,10 +634,56 @@ __global__ void k_zero_f64(double* p, size_t n)
 
 }  // namespace
 
+// Kernel instance by the model's angular shape: the power path (NB0, NB1)
+// when every angular group qualifies (MixedModel::powNB), else the member
+// loop unrolled to the largest group.
+template <typename F>
+void dispatch64(const ModelT& hm, const MixedModel& mm, F&& f)
+{
+    using std::integral_constant;
+    using I0 = integral_constant<int, 0>;
+    using I1 = integral_constant<int, 1>;
+    using I3 = integral_constant<int, 3>;
+    using I4 = integral_constant<int, 4>;
+    using I5 = integral_constant<int, 5>;
+    if (mm.powNB[0] > 0) {
+        switch (mm.powNB[0] * 10 + mm.powNB[1]) {
+        case 30: f(I4{}, I3{}, I0{}); return;
+        case 33: f(I4{}, I3{}, I3{}); return;
+        case 50: f(I4{}, I5{}, I0{}); return;
+        case 51: f(I4{}, I5{}, I1{}); return;
+        case 55: f(I4{}, I5{}, I5{}); return;
+        default: break;
+        }
+    }
+    const int mam = std::max(hm.max_mem_ang, 1);
+    if (mam <= 4) f(integral_constant<int, 4>{}, I0{}, I0{});
+    else if (mam <= 8) f(integral_constant<int, 8>{}, I0{}, I0{});
+    else if (mam <= 16) f(integral_constant<int, 16>{}, I0{}, I0{});
+    else f(integral_constant<int, 32>{}, I0{}, I0{});
+}
+
+void launch_values64(const ModelT* m, const ModelT& hm, const MixedModel& mm, const SysView& s, double* G, int ldG,
+                     hipStream_t stream)
+{
+    if (s.natoms == 0) return;
+    if (!G || !s.nb_n || !s.nb_j) throw Error{NNPX_ERR_STATE, "FP64 SF values: a buffer is missing"};
+    const int capL = ((std::max(s.cap, 1) + 31) / 32) * 32;
+    const size_t lds = stage64_bytes(capL, kTrip64);
+    if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the FP64 SF value kernel's LDS"};
+    if (!hm.has_exp) return;
+    dispatch64(hm, mm, [&](auto mx, auto n0, auto n1) {
+        hipLaunchKernelGGL((k_values64<decltype(mx)::value, decltype(n0)::value, decltype(n1)::value>),
+                           dim3(s.natoms), dim3(kWave), lds, stream, m, mm.d_mem, mm.d_gpow64, s, G, ldG, capL,
+                           kTrip64);
+    });
+}
+
 size_t fused64_lds(int cap) { return stage64_bytes(((std::max(cap, 1) + 31) / 32) * 32, kTrip64); }
 
-void launch_force64_fused(const ModelT* m, const ModelT& hm, const SysView& s, const double* dEdG,
-                          const double* fac, double* w, int ldG, double* F, bool push, hipStream_t stream)
+void launch_force64_fused(const ModelT* m, const ModelT& hm, const MixedModel& mm, const SysView& s,
+                          const double* dEdG, const double* fac, double* w, int ldG, double* F, bool push,
+                          hipStream_t stream)
 {
     if (s.natoms == 0) return;
     if (!dEdG || !fac || !w || !F || !s.nb_n || !s.nb_j)
@@ -405,18 +699,12 @@ void launch_force64_fused(const ModelT* m, const ModelT& hm, const SysView& s, c
     hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0, stream, F,
                        n3);
     const int ps = push ? 1 : 0;
-    if (hm.has_exp) {
-        const int mam = std::max(hm.max_mem_ang, 1);
-        const dim3 grid(s.natoms), block(kWave);
-        if (mam <= 4)
-            hipLaunchKernelGGL(k_force64f<4>, grid, block, lds, stream, m, s, w, ldG, F, ps, capL, kTrip64);
-        else if (mam <= 8)
-            hipLaunchKernelGGL(k_force64f<8>, grid, block, lds, stream, m, s, w, ldG, F, ps, capL, kTrip64);
-        else if (mam <= 16)
-            hipLaunchKernelGGL(k_force64f<16>, grid, block, lds, stream, m, s, w, ldG, F, ps, capL, kTrip64);
-        else
-            hipLaunchKernelGGL(k_force64f<32>, grid, block, lds, stream, m, s, w, ldG, F, ps, capL, kTrip64);
-    }
+    if (hm.has_exp)
+        dispatch64(hm, mm, [&](auto mx, auto n0, auto n1) {
+            hipLaunchKernelGGL((k_force64f<decltype(mx)::value, decltype(n0)::value, decltype(n1)::value>),
+                               dim3(s.natoms), dim3(kWave), lds, stream, m, mm.d_mem, mm.d_gpow64, s, w, ldG, F, ps,
+                               capL, kTrip64);
+        });
     launch_sfcomp64_force(m, hm, s, w, ldG, F, push, stream);
 }
 

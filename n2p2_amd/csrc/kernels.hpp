@@ -120,10 +120,17 @@ void launch_force64(const ModelT* m, const SysView& s, const double* dEdG, int l
 // and contracted with w = dE/dG times fac (the member's derivative scale per
 // element and SF, [kMaxElem*kMaxSF]); w [N*ldG] is scratch.  F [3*nforce] is
 // overwritten.  The compact groups go through their kernel's FP64 force mode.
-void launch_force64_fused(const ModelT* m, const ModelT& hm, const SysView& s, const double* dEdG,
-                          const double* fac, double* w, int ldG, double* F, bool push, hipStream_t stream);
+// mm supplies the power-path tables (MixedModel::powNB, d_mem, d_gpow64).
+struct MixedModel;
+void launch_force64_fused(const ModelT* m, const ModelT& hm, const MixedModel& mm, const SysView& s,
+                          const double* dEdG, const double* fac, double* w, int ldG, double* F, bool push,
+                          hipStream_t stream);
 void launch_sfcomp64_force(const ModelT* m, const ModelT& hm, const SysView& s, const double* w, int ldG, double* F,
                            bool push, hipStream_t stream);
+// Scaled SF values of the Gaussian groups in FP64 (no derivatives) on the
+// fused kernels' staging and compacted triplet walk.
+void launch_values64(const ModelT* m, const ModelT& hm, const MixedModel& mm, const SysView& s, double* G, int ldG,
+                     hipStream_t stream);
 // LDS bytes of the fused FP64 force kernel at a list capacity (<= 65536 to run).
 size_t fused64_lds(int cap);
 

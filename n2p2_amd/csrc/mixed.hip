@@ -1543,6 +1543,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
     const ModelT& t = hm.t;
     std::vector<MemberF> mf(std::max(t.nmembers, 1));
     std::vector<GroupMixF> gmix(std::max(t.ngroups, 1));
+    std::vector<GroupPow64> gp64(std::max(t.ngroups, 1), GroupPow64{});
     std::vector<float> sffac((size_t)kMaxElem * kMaxSF, 0.f);
     mm.maxS = 1;
     mm.hasRs = false;
@@ -1648,9 +1649,12 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                 const GroupT& G = t.grp[g];
                 if (G.type != 3 && G.type != 9 && G.type != 13) continue;
                 GroupMixF& gm = gmix[g];
+                GroupPow64& gp = gp64[g];
                 for (int h = 0; h < 2; ++h) {
                     gm.etaL[h] = (float)(heta[g][h] * 1.4426950408889634);
                     gm.twoEta[h] = (float)(2.0 * heta[g][h]);
+                    gp.eta[h] = heta[g][h];
+                    gp.twoEta[h] = 2.0 * heta[g][h];
                 }
                 gm.used = 0;
                 for (int q = G.mem0; q < G.mem0 + G.nmem; ++q) {
@@ -1659,7 +1663,9 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
                     const int b = pow2_exponent(M.zeta);
                     mf[q].beta = h == 0 ? mf[q].sidx * NB0 + b : 2 * NB0 + mf[q].sidx * NB1 + b;
                     gm.used |= 1 << mf[q].beta;
+                    gp.lam[mf[q].sidx] = M.lambda;
                 }
+                gp.used = gm.used;
             }
         }
     }
@@ -1711,6 +1717,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
     };
     up((void**)&mm.d_mem, mf.data(), sizeof(MemberF) * mf.size());
     up((void**)&mm.d_gmix, gmix.data(), sizeof(GroupMixF) * gmix.size());
+    up((void**)&mm.d_gpow64, gp64.data(), sizeof(GroupPow64) * gp64.size());
     up((void**)&mm.d_sffac, sffac.data(), sizeof(float) * sffac.size());
     up((void**)&mm.d_w, w.data(), sizeof(uint16_t) * w.size());
     up((void**)&mm.d_b, b.data(), sizeof(float) * b.size());
@@ -1721,6 +1728,7 @@ void free_mixed(MixedModel& mm)
 {
     if (mm.d_mem) (void)hipFree(mm.d_mem);
     if (mm.d_gmix) (void)hipFree(mm.d_gmix);
+    if (mm.d_gpow64) (void)hipFree(mm.d_gpow64);
     if (mm.d_sffac) (void)hipFree(mm.d_sffac);
     if (mm.d_w) (void)hipFree(mm.d_w);
     if (mm.d_b) (void)hipFree(mm.d_b);

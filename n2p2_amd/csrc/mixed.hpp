@@ -46,6 +46,16 @@ struct GroupMixF {
     int pad[2];
 };
 
+// FP64 constants of a power-path angular group for the plain form's fused
+// kernels (same term indexing as GroupMixF).
+struct GroupPow64 {
+    double eta[2];     // eta_h
+    double twoEta[2];  // 2 eta_h
+    double lam[2];     // lambda_s
+    int used;          // bit beta set for every term a member uses
+    int pad;
+};
+
 // Launch shapes of the mixed-form kernels, set by the autotuner
 // (nnpx_autotune) or left at these defaults.
 struct Tuning {
@@ -63,6 +73,7 @@ struct MixedModel {
     int terms = 3;              // bf16 product terms (1 or 3)
     MemberF* d_mem = nullptr;   // [nmembers], same order as ModelT::mem
     GroupMixF* d_gmix = nullptr; // [ngroups]
+    GroupPow64* d_gpow64 = nullptr;  // [ngroups] FP64 power-path constants
     int maxS = 2;               // max distinct lambda values per angular group (2 or 4)
     bool hasRs = false;         // some angular member has rs > 0
     int fastM = 0;              // exact angular member count of the branch-free path (0: general path)

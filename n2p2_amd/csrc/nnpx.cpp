@@ -521,10 +521,18 @@ void compute_sf64(nnpx_ctx* c, bool deriv)
         o.dGnbr = buf<double>(c, "dGnbr", (size_t)c->N * c->cap * o.S * 3);
     }
     SysView s = sysview(c);
-    timed(c, "sf64", [&] {
-        launch_sf64(c->d_model, t, s, o, deriv, c->cap, c->stream);
-        launch_sfcomp64(c->d_model, t, s, o, deriv, c->cap, c->stream);
-    });
+    if (!deriv && c->plain_forces == 0 && fused64_lds(c->cap) <= 65536) {
+        // values only: the compacted triplet walk of the fused kernels
+        timed(c, "sf64v", [&] {
+            launch_values64(c->d_model, t, c->mixed, s, o.G, o.ldG, c->stream);
+            launch_sfcomp64(c->d_model, t, s, o, false, c->cap, c->stream);
+        });
+    } else {
+        timed(c, "sf64", [&] {
+            launch_sf64(c->d_model, t, s, o, deriv, c->cap, c->stream);
+            launch_sfcomp64(c->d_model, t, s, o, deriv, c->cap, c->stream);
+        });
+    }
     c->sf_valid = true;
     c->sfd_valid = deriv;
 }
@@ -554,7 +562,7 @@ void compute_force64_fused(nnpx_ctx* c)
     double* w = buf<double>(c, "w64", (size_t)c->N * t.max_sf);
     SysView s = sysview(c);
     timed(c, "force64f", [&] {
-        launch_force64_fused(c->d_model, t, s, existing<double>(c, "dEdG"), model_table<double>(c, "fac64"), w, t.max_sf, f,
+        launch_force64_fused(c->d_model, t, c->mixed, s, existing<double>(c, "dEdG"), model_table<double>(c, "fac64"), w, t.max_sf, f,
                              c->external || c->force_mode == NNPX_FORCES_PUSH, c->stream);
     });
     c->f_valid = true;
