@@ -66,7 +66,7 @@ typedef enum {
                                   it costs no occupancy, default), 0 off,
                                   1 on (where it fits in LDS); results are
                                   identical either way                     */
-    NNPX_OPT_PLAIN_FORCES = 5   /* forces of nnpx_evaluate in the plain form:
+    NNPX_OPT_PLAIN_FORCES = 5,  /* forces of nnpx_evaluate in the plain form:
                                   0 (default) recompute every SF derivative
                                   in FP64 and contract it with dE/dG on the
                                   fly (no dG/dr stored; lists too long for
@@ -74,6 +74,14 @@ typedef enum {
                                   dG/dr in the reference layout and contract
                                   it (Mode::calculateForces).  The step calls
                                   (nnpx_calculate_*) always store dG/dr.   */
+    NNPX_OPT_ANGULAR_MERGE_SF = 6,  /* mixed SF kernel: walk all angular
+                                  groups of an element in one triplet pass
+                                  (models whose groups share type, cutoff
+                                  and term constants); -1 auto (where the
+                                  elements average >= 4 angular groups,
+                                  default), 0 off, 1 on; results agree to
+                                  FP32 summation order                     */
+    NNPX_OPT_ANGULAR_MERGE_FORCE = 7 /* the same for the mixed force kernel */
 } nnpx_option;
 
 /* Flags for nnpx_evaluate. */
@@ -86,6 +94,8 @@ nnpx_status nnpx_destroy(nnpx_ctx* ctx);
 /* Last error message of this context (or of the last failed create). */
 const char* nnpx_last_error(const nnpx_ctx* ctx);
 nnpx_status nnpx_set_option(nnpx_ctx* ctx, int option, int64_t value);
+/* Current value of an option (as set, or as the autotuner chose it). */
+nnpx_status nnpx_get_option(const nnpx_ctx* ctx, int option, int64_t* value);
 nnpx_status nnpx_synchronize(nnpx_ctx* ctx);
 
 /* ---- model ------------------------------------------------------------ */

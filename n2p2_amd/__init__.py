@@ -26,6 +26,8 @@ OPT_NEIGHBOR_ALGO = 2
 OPT_BF16_TERMS = 3
 OPT_SF_GATHER = 4
 OPT_PLAIN_FORCES = 5
+OPT_ANGULAR_MERGE_SF = 6
+OPT_ANGULAR_MERGE_FORCE = 7
 EVAL_FORCES = 1
 
 _STATUS = {0: "OK", 1: "ARGUMENT", 2: "FILE", 3: "MODEL", 4: "RANGE", 5: "STATE", 6: "HIP", 7: "UNSUPPORTED"}
@@ -49,6 +51,7 @@ _SIGS = {
     "nnpx_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int],
     "nnpx_destroy": [ctypes.c_void_p],
     "nnpx_set_option": [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64],
+    "nnpx_get_option": [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p],
     "nnpx_synchronize": [ctypes.c_void_p],
     "nnpx_load_model": [ctypes.c_void_p, ctypes.c_char_p],
     "nnpx_model_info": [ctypes.c_void_p, _c_int_p, _c_int_p, _c_dbl_p],
@@ -191,6 +194,11 @@ class Mode:
 
     def set_option(self, option, value):
         self._ck(self._lib.nnpx_set_option(self._h, option, int(value)))
+
+    def get_option(self, option):
+        v = np.zeros(1, np.int64)
+        self._ck(self._lib.nnpx_get_option(self._h, option, _ptr(v)))
+        return int(v[0])
 
     # -- structures ----------------------------------------------------
     def set_structures(self, positions, elements, boxes=None, atom_offsets=None, periodic=None):
@@ -386,9 +394,14 @@ class Mode:
         return self.tuning()
 
     def tuning(self):
+        """Launch shapes in use: the triplet buffers, the gradient kernel's
+        atoms per block and the merged angular walk settings (-1 auto)."""
         v = np.zeros(4, np.int32)
         self._ck(self._lib.nnpx_get_tuning(self._h, _ptr(v)))
-        return dict(zip(("trip_sf", "trip_force", "trip_tangent", "grad_atoms"), (int(x) for x in v)))
+        out = dict(zip(("trip_sf", "trip_force", "trip_tangent", "grad_atoms"), (int(x) for x in v)))
+        out["merge_sf"] = self.get_option(OPT_ANGULAR_MERGE_SF)
+        out["merge_force"] = self.get_option(OPT_ANGULAR_MERGE_FORCE)
+        return out
 
     def set_tuning(self, trip_sf, trip_force, trip_tangent, grad_atoms):
         v = np.array([trip_sf, trip_force, trip_tangent, grad_atoms], np.int32)

@@ -209,6 +209,16 @@ __device__ inline int make_list(const Stage& S, int n, int elem, float rc, unsig
     return cnt;
 }
 
+// In-cutoff neighbours of all elements, grouped by element (neighbour order
+// within an element): a merged walk's pairs then change group only at
+// element boundaries along a row.
+__device__ inline int make_sorted_list(const Stage& S, int n, int nelem, float rc, unsigned short* out)
+{
+    int cnt = 0;
+    for (int el = 0; el < nelem; ++el) cnt += make_list(S, n, el, rc, out + cnt);
+    return cnt;
+}
+
 // Cutoff values of every staged neighbour for one radius (elements whose
 // groups all share it compute them once per atom).
 __device__ inline void all_cutoffs(const Stage& S, int n, int ctype, const GroupF& g, float tanhPre)
@@ -638,6 +648,104 @@ __device__ inline void power_weights(const Stage& S, const MemberF* __restrict__
     }
 }
 
+// LDS of a merged walk: the pair map, then per (merged group, term) one
+// 8-byte entry (the FP64 term sum, or the FP32 weights wb, wa).
+__host__ __device__ constexpr size_t merge_bytes(int ng, int nbas)
+{
+    return (size_t)kMaxElem * kMaxElem + (size_t)ng * nbas * 8;
+}
+
+// SF values of all power-path angular groups of an element in one triplet
+// walk over the element-ordered in-cutoff list: FP32 partial sums per term
+// over a lane's run of one group, folded into FP64 LDS sums per (group,
+// term) when the group changes and after each buffer; then every member
+// takes its group's term (times 2^(1-zeta)) and is scaled.
+template <int NB0, int NB1, bool GATHER>
+__device__ inline void sf32_power_merged(const ModelT* __restrict__ M, const Stage& S, float4* gB,
+                                         const MemberF* __restrict__ MF, const GroupMixF* __restrict__ GM,
+                                         const AngMergeF& am, const ElementT& E, int n, int ctype, float tanhPre,
+                                         char* mbase, double* __restrict__ Gi, float* __restrict__ Ii)
+{
+    constexpr int NBAS = 2 * (NB0 + NB1);
+    constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
+    const int lane = threadIdx.x;
+    const GroupT& G0 = M->grp[am.grp[0]];
+    const GroupF g = group_f(G0);
+    const GroupMixF gm = GM[am.grp[0]];
+    const bool wide = G0.type == 9;
+    const int ng = am.ng, used = am.used;
+    signed char* pmap = reinterpret_cast<signed char*>(mbase);
+    double* red = reinterpret_cast<double*>(mbase + kMaxElem * kMaxElem);
+    for (int x = lane; x < kMaxElem * kMaxElem; x += kWave) pmap[x] = am.pair[x];
+    for (int x = lane; x < ng * NBAS; x += kWave) red[x] = 0.0;
+    const int nA = make_sorted_list(S, n, M->nelem, g.rc, S.la);  // its barrier orders the fills above
+    if constexpr (GATHER) gather_partners(S, S.la, nA, gB);
+    float part[NBAS];
+#pragma unroll
+    for (int b = 0; b < NBAS; ++b) part[b] = 0.f;
+    int gc = -1;  // group of the partial sums
+    auto flush = [&]() {
+        if (gc >= 0) {
+#pragma unroll
+            for (int b = 0; b < NBAS; ++b)
+                if ((used >> b) & 1) lds_add(red + gc * NBAS + b, (double)part[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < NBAS; ++b) part[b] = 0.f;
+        gc = -1;
+    };
+    const float ce0 = -gm.etaL[0], ce1 = -gm.etaL[1];
+    const float lam0 = gm.lam[0], lam1 = gm.lam[1];
+    int jc = -1, rowj = 0;
+    float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
+    for_each_triplet_blocks<GATHER>(S, gB, nA, nA, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
+        if (j != jc) {
+            gj = S.g[j];
+            cj = S.c[j];
+            rowj = S.el[j] * kMaxElem;
+            jc = j;
+        }
+        const int q = pmap[rowj + S.el[k]];
+        if (q != gc) {
+            flush();
+            gc = q;
+        }
+        if (q < 0) return;
+        const float4 gk = S.g[k];
+        const float4 ck = S.c[k];
+        const float r2jk = dist2(gj, gk);
+        float fcjk = 1.f, dfcjk = 0.f;
+        if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, __builtin_amdgcn_sqrtf(r2jk), fcjk, dfcjk);
+        const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * cj.x * ck.x;
+        const float pfc = cj.y * ck.y * fcjk;
+        const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
+        const float E0 = fexp2(ce0 * r2sum) * pfc;
+        const float E1 = NB1 > 0 ? fexp2(ce1 * r2sum) * pfc : 0.f;
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+            const float pl = fmaf(sl ? lam1 : lam0, cost, 1.f);
+            float p = pl > 0.f ? pl : 0.f;
+#pragma unroll
+            for (int b = 0; b < NBM; ++b) {
+                if (b < NB0) part[sl * NB0 + b] = fmaf(E0, p, part[sl * NB0 + b]);
+                if (b < NB1) part[2 * NB0 + sl * NB1 + b] = fmaf(E1, p, part[2 * NB0 + sl * NB1 + b]);
+                p *= p;
+            }
+        }
+    }, flush);
+    __syncthreads();
+    for (int q = 0; q < ng; ++q) {
+        const GroupT& GT = M->grp[am.grp[q]];
+        for (int x = lane; x < GT.nmem; x += kWave) {
+            const MemberT& mt = M->mem[GT.mem0 + x];
+            const double gs = scale_sf(M->scale[E.sf0 + mt.sf], red[q * NBAS + MF[GT.mem0 + x].beta] * mt.gnorm);
+            Gi[mt.sf] = gs;
+            Ii[mt.sf] = (float)gs;
+        }
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------------------------- SF values --
 // Angular terms: vexp * pfc * pl^zeta evaluated as one exp2 of
 // zeta * log2(pl) + log2(pfc) - eta log2(e) * sum(r - rs)^2, with log2(pl)
@@ -646,14 +754,18 @@ __device__ inline void power_weights(const Stage& S, const MemberF* __restrict__
 // FAST: every angular group has exactly MAXM members, at most two distinct
 // lambda and rs = 0 -- the member loop is then branch-free.  Otherwise the
 // general loop handles shifted Gaussians and up to four lambda values.
-template <int MAXM, bool FAST, int NB0, int NB1, bool GATHER>
+// MERGE: the element's angular groups go through one merged walk when it
+// has one (AngMergeF::ng > 0).
+template <int MAXM, bool FAST, int NB0, int NB1, bool GATHER, bool MERGE>
 __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
-                                             const GroupMixF* __restrict__ GM, SysView s, double* __restrict__ G,
-                                             float* __restrict__ Gin, int ldG, int ldK, int capL, int trip)
+                                             const GroupMixF* __restrict__ GM, const AngMergeF* __restrict__ AM,
+                                             SysView s, double* __restrict__ G, float* __restrict__ Gin, int ldG,
+                                             int ldK, int capL, int trip)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
     float4* gB = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip, stage_mp(NB0, NB1)));
+    char* mbase = reinterpret_cast<char*>(gB) + (GATHER ? sizeof(float4) * (size_t)capL : 0);
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
@@ -664,6 +776,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
     double* Gi = G + (size_t)i * ldG;
     float* Ii = Gin + (size_t)i * ldK;
     for (int q = E.nsf + lane; q < ldK; q += kWave) Ii[q] = 0.f;
+    const bool merged = MERGE && AM[e].ng > 0;
 
     for (int gi = 0; gi < E.ngrp; ++gi) {
         const GroupT& GT = M->grp[E.grp0 + gi];
@@ -676,6 +789,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
         if (sf_compact(GT.type)) continue;  // the compact kernel's
+        if (merged && GT.type != 2 && GT.type != 12) continue;  // the merged walk's
         if (GT.type == 2 || GT.type == 12) {
             // radial members in chunks of kRadChunk (any member count);
             // type 12: every neighbour, each term times its atomic number
@@ -817,6 +931,9 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 }
             }
         __syncthreads();
+    }
+    if constexpr (MERGE && NB0 > 0) {
+        if (merged) sf32_power_merged<NB0, NB1, GATHER>(M, S, gB, MF, GM, AM[e], E, n, ctype, tanhPre, mbase, Gi, Ii);
     }
 }
 
@@ -997,21 +1114,150 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
 }
 
 // --------------------------------------------------- fused force backward --
+// Forces of all power-path angular groups of an element in one triplet walk
+// over the element-ordered in-cutoff list: each triplet takes the term
+// weights (wb, wa) of its pair's group from LDS; the geometry and the
+// accumulation are those of the per-group walk in k_force32.
+template <int NB0, int NB1>
+__device__ inline void force32_power_merged(const ModelT* __restrict__ M, const Stage& S,
+                                            const MemberF* __restrict__ MF, const GroupMixF* __restrict__ GM,
+                                            const AngMergeF& am, int n, int ctype, float tanhPre, char* mbase,
+                                            const float* __restrict__ wi, double* ax, double* ay, double* az,
+                                            double& fx, double& fy, double& fz)
+{
+    constexpr int NBAS = 2 * (NB0 + NB1);
+    constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
+    const int lane = threadIdx.x;
+    const GroupT& G0 = M->grp[am.grp[0]];
+    const GroupF g = group_f(G0);
+    const GroupMixF gm = GM[am.grp[0]];
+    const bool wide = G0.type == 9;
+    const int ng = am.ng;
+    signed char* pmap = reinterpret_cast<signed char*>(mbase);
+    float2* wtab = reinterpret_cast<float2*>(mbase + kMaxElem * kMaxElem);
+    for (int x = lane; x < kMaxElem * kMaxElem; x += kWave) pmap[x] = am.pair[x];
+    // wb = sum of the members' w on term b, wa = sum of zeta lambda w
+    for (int x = lane; x < ng * NBAS; x += kWave) {
+        const int q = x / NBAS, b = x - q * NBAS;
+        const GroupT& GT = M->grp[am.grp[q]];
+        const MemberF* mf = MF + GT.mem0;
+        float sb = 0.f, sa = 0.f;
+        for (int m = 0; m < GT.nmem; ++m)
+            if (mf[m].beta == b) {
+                const float wq = wi[mf[m].sf];
+                sb += wq;
+                sa += wq * mf[m].zetaLambda;
+            }
+        wtab[x] = make_float2(sb, sa);
+    }
+    const int nA = make_sorted_list(S, n, M->nelem, g.rc, S.la);  // its barrier orders the tables above
+    int cur = -1, rowj = 0;
+    float rx = 0.f, ry = 0.f, rz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    auto flush = [&]() {
+        if (cur >= 0) {
+            lds_add(ax + cur, (double)rx);
+            lds_add(ay + cur, (double)ry);
+            lds_add(az + cur, (double)rz);
+            fx -= (double)sx;
+            fy -= (double)sy;
+            fz -= (double)sz;
+        }
+        cur = -1;
+    };
+    float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
+    for_each_triplet_rows(S, nA, nA, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
+        if (j != cur) {
+            flush();
+            cur = j;
+            rx = 0.f; ry = 0.f; rz = 0.f;
+            sx = 0.f; sy = 0.f; sz = 0.f;
+            gj = S.g[j];
+            cj = S.c[j];
+            rowj = S.el[j] * kMaxElem;
+        }
+        const int q = pmap[rowj + S.el[k]];
+        if (q < 0) return;
+        const float2* wq = wtab + q * NBAS;
+        const float4 gk = S.g[k];
+        const float4 ck = S.c[k];
+        const float r2jk = dist2(gj, gk);
+        const float ijk = __builtin_amdgcn_rsqf(r2jk);
+        const float rjk = r2jk * ijk;
+        float fcjk = 1.f, dfcjk = 0.f;
+        if (!wide) cutoff32(ctype, g.rc, g.rci, g.iw, g.rcinv, tanhPre, rjk, fcjk, dfcjk);
+        const float rinvijik = cj.x * ck.x;
+        const float cost = (gj.x * gk.x + gj.y * gk.y + gj.z * gk.z) * rinvijik;
+        const float pfc = cj.y * ck.y * fcjk;
+        const float r2sum = gj.w * gj.w + gk.w * gk.w + (wide ? 0.f : r2jk);
+        const float pr1 = ck.y * fcjk * cj.z * cj.x;
+        const float pr2 = cj.y * fcjk * ck.z * ck.x;
+        const float pr3 = cj.y * ck.y * dfcjk * ijk;
+        const float E0 = fexp2(-gm.etaL[0] * r2sum);
+        const float E1 = NB1 > 0 ? fexp2(-gm.etaL[1] * r2sum) : 0.f;
+        float Q0 = 0.f, Q1 = 0.f, A = 0.f;
+#pragma unroll
+        for (int sl = 0; sl < 2; ++sl) {
+            const float pl = fmaf(gm.lam[sl], cost, 1.f);
+            const bool pos = pl > 0.f;
+            float p = pos ? pl : 0.f;
+            float qa0 = 0.f, qa1 = 0.f;
+#pragma unroll
+            for (int b = 0; b < NBM; ++b) {
+                if (b < NB0) {
+                    const float2 w = wq[sl * NB0 + b];
+                    Q0 = fmaf(w.x, p, Q0);
+                    qa0 = fmaf(w.y, p, qa0);
+                }
+                if (b < NB1) {
+                    const float2 w = wq[2 * NB0 + sl * NB1 + b];
+                    Q1 = fmaf(w.x, p, Q1);
+                    qa1 = fmaf(w.y, p, qa1);
+                }
+                p *= p;
+            }
+            A += (E0 * qa0 + E1 * qa1) * (pos ? __builtin_amdgcn_rcpf(pl) : 0.f);
+        }
+        const float B0 = E0 * Q0, B1 = E1 * Q1;
+        const float B = B0 + B1;
+        const float C0 = gm.twoEta[0] * B0 + gm.twoEta[1] * B1;
+        const float C3 = wide ? 0.f : C0;
+        const float P1 = pfc * (A * (rinvijik - cost * cj.x * cj.x) - C0) + B * pr1;
+        const float P2 = pfc * (A * (rinvijik - cost * ck.x * ck.x) - C0) + B * pr2;
+        const float P3 = pfc * (A * rinvijik + C3) - B * pr3;
+        const float x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
+        const float p1x = P1 * gj.x, p1y = P1 * gj.y, p1z = P1 * gj.z;
+        const float p2x = P2 * gk.x, p2y = P2 * gk.y, p2z = P2 * gk.z;
+        const float p3x = P3 * x3, p3y = P3 * y3, p3z = P3 * z3;
+        sx += p1x + p2x;
+        sy += p1y + p2y;
+        sz += p1z + p2z;
+        rx += p1x + p3x;
+        ry += p1y + p3y;
+        rz += p1z + p3z;
+        lds_add(ax + k, (double)(p2x - p3x));
+        lds_add(ay + k, (double)(p2y - p3y));
+        lds_add(az + k, (double)(p2z - p3z));
+    }, flush);
+    flush();
+    __syncthreads();
+}
+
 // Forces on the atom's neighbours are accumulated per neighbour entry in LDS
 // (FP64) and pushed to the force array with FP64 atomics; contributions to
 // the atom's own periodic images are dropped in the pull form
 // (Mode::calculateForces skips them, Mode.cpp:1957) and kept in the push form.
-template <int MAXM, bool FAST, int NB0, int NB1>
+template <int MAXM, bool FAST, int NB0, int NB1, bool MERGE>
 __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
-                                                const GroupMixF* __restrict__ GM, SysView s,
-                                                const float* __restrict__ fw, int ldG, double* __restrict__ F,
-                                                int push, int capL, int trip)
+                                                const GroupMixF* __restrict__ GM, const AngMergeF* __restrict__ AM,
+                                                SysView s, const float* __restrict__ fw, int ldG,
+                                                double* __restrict__ F, int push, int capL, int trip)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
     double* ax = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip, stage_mp(NB0, NB1)));
     double* ay = ax + capL;
     double* az = ay + capL;
+    char* mbase = reinterpret_cast<char*>(az + capL);
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
@@ -1022,6 +1268,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     for (int k = lane; k < n; k += kWave) { ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0; }
     const float* wi = fw + (size_t)i * ldG;
     double fx = 0.0, fy = 0.0, fz = 0.0;
+    const bool merged = MERGE && AM[e].ng > 0;
     __syncthreads();
 
     for (int gi = 0; gi < E.ngrp; ++gi) {
@@ -1035,6 +1282,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         }
         if (sf_compact(GT.type)) continue;  // the compact kernel's
+        if (merged && GT.type != 2 && GT.type != 12) continue;  // the merged walk's
         if (GT.type == 2 || GT.type == 12) {
             // radial: each lane owns its neighbours, no conflicts; type 12
             // weights every neighbour by its atomic number
@@ -1242,6 +1490,10 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         });
         flush();
         __syncthreads();
+    }
+    if constexpr (MERGE && NB0 > 0) {
+        if (merged)
+            force32_power_merged<NB0, NB1>(M, S, MF, GM, AM[e], n, ctype, tanhPre, mbase, wi, ax, ay, az, fx, fy, fz);
     }
     fx = wave_sum(fx);
     fy = wave_sum(fy);
@@ -1669,6 +1921,51 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
             }
         }
     }
+    // merged angular walks: per element, every angular group on the power
+    // path, one type (3 or 9), the element's single cutoff radius, identical
+    // term constants and distinct neighbour element pairs
+    std::vector<AngMergeF> amerge(std::max(t.nelem, 1));
+    mm.merge_ng = 0;
+    mm.merge_avg = 0.0;
+    int merged_elements = 0;
+    for (int e = 0; e < t.nelem && mm.powNB[0] > 0; ++e) {
+        AngMergeF& am = amerge[e];
+        std::memset(&am, 0, sizeof(am));
+        std::memset(am.pair, -1, sizeof(am.pair));
+        const ElementT& E = t.el[e];
+        std::vector<int> ang;
+        for (int gi = 0; gi < E.ngrp; ++gi) {
+            const int g = E.grp0 + gi;
+            const int ty = t.grp[g].type;
+            if (ty == 3 || ty == 9 || ty == 13) ang.push_back(g);
+        }
+        bool ok = E.ucut && ang.size() >= 2 && (int)ang.size() <= kMaxMerge;
+        for (size_t q = 0; q < ang.size() && ok; ++q) {
+            const GroupT& G = t.grp[ang[q]];
+            const GroupT& G0 = t.grp[ang[0]];
+            const GroupMixF& a = gmix[ang[q]];
+            const GroupMixF& a0 = gmix[ang[0]];
+            ok = G.type != 13 && G.type == G0.type && G.rc == G0.rc && a.nlam == a0.nlam && a.lam[0] == a0.lam[0] &&
+                 a.lam[1] == a0.lam[1] && a.etaL[0] == a0.etaL[0] && a.etaL[1] == a0.etaL[1] &&
+                 a.twoEta[0] == a0.twoEta[0] && a.twoEta[1] == a0.twoEta[1] &&
+                 am.pair[G.e1 * kMaxElem + G.e2] < 0;
+            if (!ok) break;
+            am.grp[q] = ang[q];
+            am.used |= a.used;
+            am.pair[G.e1 * kMaxElem + G.e2] = (signed char)q;
+            am.pair[G.e2 * kMaxElem + G.e1] = (signed char)q;
+        }
+        if (!ok) {
+            std::memset(&am, 0, sizeof(am));
+            std::memset(am.pair, -1, sizeof(am.pair));
+            continue;
+        }
+        am.ng = (int)ang.size();
+        mm.merge_ng = std::max(mm.merge_ng, am.ng);
+        mm.merge_avg += am.ng;
+        ++merged_elements;
+    }
+    if (merged_elements) mm.merge_avg /= merged_elements;
     // bf16 weights, padded to 16: W [out][in] and WT [in][out], hi then lo.
     std::vector<uint16_t> w;
     std::vector<float> b;
@@ -1718,6 +2015,7 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
     up((void**)&mm.d_mem, mf.data(), sizeof(MemberF) * mf.size());
     up((void**)&mm.d_gmix, gmix.data(), sizeof(GroupMixF) * gmix.size());
     up((void**)&mm.d_gpow64, gp64.data(), sizeof(GroupPow64) * gp64.size());
+    up((void**)&mm.d_amerge, amerge.data(), sizeof(AngMergeF) * amerge.size());
     up((void**)&mm.d_sffac, sffac.data(), sizeof(float) * sffac.size());
     up((void**)&mm.d_w, w.data(), sizeof(uint16_t) * w.size());
     up((void**)&mm.d_b, b.data(), sizeof(float) * b.size());
@@ -1729,6 +2027,7 @@ void free_mixed(MixedModel& mm)
     if (mm.d_mem) (void)hipFree(mm.d_mem);
     if (mm.d_gmix) (void)hipFree(mm.d_gmix);
     if (mm.d_gpow64) (void)hipFree(mm.d_gpow64);
+    if (mm.d_amerge) (void)hipFree(mm.d_amerge);
     if (mm.d_sffac) (void)hipFree(mm.d_sffac);
     if (mm.d_w) (void)hipFree(mm.d_w);
     if (mm.d_b) (void)hipFree(mm.d_b);
@@ -1755,6 +2054,16 @@ static int blocks_per_cu(const void* kernel, size_t lds)
     return occ;
 }
 
+// Merged angular walk on or off: a tuning value of 0 or 1 decides, -1 takes
+// it where the model merges at least four groups per element on average
+// (fewer groups have enough triplets each to fill the per-group walks).
+static bool use_merge(const MixedModel& mm, int want)
+{
+    if (mm.merge_ng == 0 || mm.powNB[0] == 0) return false;
+    if (want >= 0) return want == 1;
+    return mm.merge_avg >= 4.0;
+}
+
 void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, const SysView& s,
                const MixedBuffers& b, bool forces, bool push, hipStream_t stream, const TimedFn& timed)
 {
@@ -1768,26 +2077,40 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     const size_t lds_f_max = stage_bytes(capL, tf, stage_mp(0, 0)) + sizeof(double) * 3 * (size_t)capL;
     if (lds_f_max > 65536 || lds_sf_max > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
+    const bool merge_sf = use_merge(mm, mm.tune.merge_sf), merge_f = use_merge(mm, mm.tune.merge_force);
     timed("sf32", [&] {
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
             constexpr int cm = decltype(m)::value, c0 = decltype(nb0)::value, c1 = decltype(nb1)::value;
             constexpr bool cf = decltype(fast)::value;
+            constexpr bool cmg = c0 > 0;  // merged instances exist on the power path only
+            const bool mg = cmg && merge_sf;
+            const size_t extra = mg ? merge_bytes(mm.merge_ng, 2 * (c0 + c1)) : 0;
             // the partner gather costs 16 B of LDS per slot: taken only when
             // it leaves the blocks per CU unchanged
-            const size_t lds0 = stage_bytes(capL, tsf, stage_mp(c0, c1));
+            const size_t lds0 = stage_bytes(capL, tsf, stage_mp(c0, c1)) + extra;
             const size_t lds1 = lds0 + sizeof(float4) * (size_t)capL;
-            int occ0 = 0, occ1 = 0;
-            const int want = mm.tune.sf_gather;
-            if (lds1 <= 65536 && want != 0) {
-                occ0 = blocks_per_cu(reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, false>), lds0);
-                occ1 = blocks_per_cu(reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, true>), lds1);
-            }
-            if (occ1 > 0 && (occ1 >= occ0 || want == 1))
-                hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, true>), dim3(s.natoms), dim3(kWave), lds1, stream, d_model,
-                                   mm.d_mem, mm.d_gmix, s, b.G, b.Gin, b.ldG, mm.ldK, capL, tsf);
-            else
-                hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, false>), dim3(s.natoms), dim3(kWave), lds0, stream, d_model,
-                                   mm.d_mem, mm.d_gmix, s, b.G, b.Gin, b.ldG, mm.ldK, capL, tsf);
+            if (lds0 > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
+            auto launch = [&](auto gather, auto merge) {
+                constexpr bool cg = decltype(gather)::value, cmm = decltype(merge)::value;
+                hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, cg, cmm>), dim3(s.natoms), dim3(kWave), cg ? lds1 : lds0,
+                                   stream, d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.G, b.Gin, b.ldG, mm.ldK,
+                                   capL, tsf);
+            };
+            auto pick = [&](auto merge) {
+                constexpr bool cmm = decltype(merge)::value;
+                int occ0 = 0, occ1 = 0;
+                const int want = mm.tune.sf_gather;
+                if (lds1 <= 65536 && want != 0) {
+                    occ0 = blocks_per_cu(reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, false, cmm>), lds0);
+                    occ1 = blocks_per_cu(reinterpret_cast<const void*>(&k_sf32<cm, cf, c0, c1, true, cmm>), lds1);
+                }
+                if (occ1 > 0 && (occ1 >= occ0 || want == 1))
+                    launch(std::true_type{}, merge);
+                else
+                    launch(std::false_type{}, merge);
+            };
+            if (mg) pick(std::integral_constant<bool, cmg>{});
+            else pick(std::false_type{});
         });
         launch_sfcomp32(d_model, t, s, b.G, b.Gin, b.ldG, mm.ldK, nullptr, nullptr, false, false, stream);
     });
@@ -1831,12 +2154,18 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                            b.F, n3);
         const int ps = push ? 1 : 0;
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
-            const size_t lds_f = stage_bytes(capL, tf, stage_mp(decltype(nb0)::value, decltype(nb1)::value)) +
-                                 sizeof(double) * 3 * (size_t)capL;
-            hipLaunchKernelGGL((k_force32<decltype(m)::value, decltype(fast)::value, decltype(nb0)::value,
-                                          decltype(nb1)::value>),
-                               dim3(s.natoms), dim3(kWave), lds_f, stream, d_model, mm.d_mem, mm.d_gmix, s, b.dEdGf,
-                               b.ldG, b.F, ps, capL, tf);
+            constexpr int cm = decltype(m)::value, c0 = decltype(nb0)::value, c1 = decltype(nb1)::value;
+            constexpr bool cf = decltype(fast)::value;
+            const bool mg = c0 > 0 && merge_f;
+            const size_t lds_f = stage_bytes(capL, tf, stage_mp(c0, c1)) + sizeof(double) * 3 * (size_t)capL +
+                                 (mg ? merge_bytes(mm.merge_ng, 2 * (c0 + c1)) : 0);
+            if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
+            if (mg)
+                hipLaunchKernelGGL((k_force32<cm, cf, c0, c1, (c0 > 0)>), dim3(s.natoms), dim3(kWave), lds_f, stream,
+                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf);
+            else
+                hipLaunchKernelGGL((k_force32<cm, cf, c0, c1, false>), dim3(s.natoms), dim3(kWave), lds_f, stream,
+                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf);
         });
         launch_sfcomp32(d_model, t, s, nullptr, nullptr, b.ldG, mm.ldK, b.dEdGf, b.F, push, true, stream);
     });

@@ -56,6 +56,19 @@ struct GroupPow64 {
     int pad;
 };
 
+// Angular groups of one element walked together (power path): the groups
+// share type, cutoff and term constants and differ only in their neighbour
+// element pair and members, so one triplet walk over all in-cutoff
+// neighbours serves them all, each triplet's terms going to the group of
+// its pair.  ng = 0: the element's groups are walked one by one.
+constexpr int kMaxMerge = 16;
+struct AngMergeF {
+    int ng;                                // merged groups
+    int used;                              // union of the groups' term masks
+    int grp[kMaxMerge];                    // their ModelT::grp indices
+    signed char pair[kMaxElem * kMaxElem]; // (element j, element k) -> merged group, -1: none
+};
+
 // Launch shapes of the mixed-form kernels, set by the autotuner
 // (nnpx_autotune) or left at these defaults.
 struct Tuning {
@@ -64,6 +77,8 @@ struct Tuning {
     int trip_tangent = 384;  // ... of the tangent kernel
     int grad_atoms = 32;     // atoms per block of the gradient kernel
     int sf_gather = -1;      // SF kernel partner gather: -1 auto, 0 off, 1 on
+    int merge_sf = -1;       // merged angular walk in the SF kernel: -1 auto, 0 off, 1 on
+    int merge_force = -1;    // ... in the force kernel
 };
 
 struct MixedModel {
@@ -78,6 +93,9 @@ struct MixedModel {
     bool hasRs = false;         // some angular member has rs > 0
     int fastM = 0;              // exact angular member count of the branch-free path (0: general path)
     int powNB[2] = {0, 0};      // power path shape (NB0, NB1); {0, 0}: not used
+    AngMergeF* d_amerge = nullptr;  // [nelem] merged angular walks (power path only)
+    int merge_ng = 0;           // most merged groups of an element (0: no element merges)
+    double merge_avg = 0.0;     // mean angular groups per merged element
     float* d_sffac = nullptr;   // [nelem*kMaxSF] force weight factor per SF
     uint16_t* d_w = nullptr;    // bf16 weights: per element/layer W hi, W lo, WT hi, WT lo
     float* d_b = nullptr;       // FP32 biases, padded

@@ -858,6 +858,32 @@ nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
             c->mixed.tune.sf_gather = (int)value;
             drop_graph(c);
             break;
+        case NNPX_OPT_ANGULAR_MERGE_SF:
+        case NNPX_OPT_ANGULAR_MERGE_FORCE:
+            if (value < -1 || value > 1) fail(NNPX_ERR_ARGUMENT, "angular merge must be -1, 0 or 1");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            (option == NNPX_OPT_ANGULAR_MERGE_SF ? c->mixed.tune.merge_sf : c->mixed.tune.merge_force) = (int)value;
+            drop_graph(c);
+            break;
+        default: fail(NNPX_ERR_ARGUMENT, "unknown option");
+        }
+    });
+}
+
+nnpx_status nnpx_get_option(const nnpx_ctx* cc, int option, int64_t* value)
+{
+    nnpx_ctx* c = const_cast<nnpx_ctx*>(cc);
+    return guard(c, [&] {
+        if (!value) fail(NNPX_ERR_ARGUMENT, "null output");
+        switch (option) {
+        case NNPX_OPT_FORCE_MODE: *value = c->force_mode; break;
+        case NNPX_OPT_TIMING: *value = c->timing ? 1 : 0; break;
+        case NNPX_OPT_NEIGHBOR_ALGO: *value = c->nbr_algo; break;
+        case NNPX_OPT_BF16_TERMS: *value = c->mixed.terms; break;
+        case NNPX_OPT_SF_GATHER: *value = c->mixed.tune.sf_gather; break;
+        case NNPX_OPT_PLAIN_FORCES: *value = c->plain_forces; break;
+        case NNPX_OPT_ANGULAR_MERGE_SF: *value = c->mixed.tune.merge_sf; break;
+        case NNPX_OPT_ANGULAR_MERGE_FORCE: *value = c->mixed.tune.merge_force; break;
         default: fail(NNPX_ERR_ARGUMENT, "unknown option");
         }
     });
@@ -1940,6 +1966,21 @@ nnpx_status nnpx_autotune(nnpx_ctx* c)
                 if (d < best_t) { best_t = d; best.trip_tangent = trip; }
             }
             tu = best;
+            // merged angular walks (models that have them), per kernel
+            if (c->mixed.merge_ng > 0) {
+                double t_sf[2], t_f[2];
+                for (int mg = 0; mg < 2; ++mg) {
+                    tu.merge_sf = tu.merge_force = mg;
+                    evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, true);
+                    settle(c);
+                    c->ktime.clear();
+                    for (int r = 0; r < 3; ++r) evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, false);
+                    t_sf[mg] = mean_time(c, "sf32");
+                    t_f[mg] = mean_time(c, "force32");
+                }
+                tu.merge_sf = t_sf[1] < t_sf[0] ? 1 : 0;
+                tu.merge_force = t_f[1] < t_f[0] ? 1 : 0;
+            }
             // atoms per block of the gradient kernel (FP32 network pass on the
             // current G with a zero tangent)
             GradArgs a{};

@@ -80,3 +80,37 @@ def test_external_list_equals_periodic_push(precision):
         assert erel <= 1e-12 and dF <= 1e-10
     else:
         assert erel <= 1e-5 and rms <= 1e-3
+
+
+def test_external_list_after_periodic_structure():
+    """A context that first evaluated a sparse periodic box (small fitted
+    neighbour staging) and is then handed a denser external list: the
+    staging is refitted to the external list and the mixed result equals
+    the oracle's push form, as on a fresh context."""
+    say("case: set_structures (sparse box) then external list (water-192), mixed")
+    import tempfile
+    d = synth.water_model(tempfile.mkdtemp())
+    pos, elem, box = synth.water_box(4)
+    pos = remap(pos, box)
+    m = n2p2_amd.Mode(d)
+    sparse = 1.6 * box
+    m.set_structures(remap(1.6 * pos, sparse), elem, sparse[None])
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    offsets, index, nb_elem, dr, owner = lammps_like(pos, elem, box, d)
+    n = len(elem)
+    m.set_neighbors_external(elem, n + len(owner), offsets, index, nb_elem, dr)
+    m.evaluate(n2p2_amd.MIXED, forces=True)
+    E = m.energies()[0]
+    F = m.forces()
+    Ff = F[:n].copy()
+    np.add.at(Ff, owner, F[n:])
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PUSH, BRUTE)
+    Eo, Fo = o.energy(), o.forces()
+    erel = abs(E - Eo) / abs(Eo)
+    rms = float(np.sqrt(np.mean((Ff - Fo) ** 2)))
+    say("  dense list: max neighbours %d  E rel %.2e  F rms %.2e" % (int(np.diff(offsets).max()), erel, rms))
+    assert erel <= 1e-5 and rms <= 1e-5
+    # a second read returns the same evaluation
+    assert m.energies()[0] == E

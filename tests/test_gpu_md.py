@@ -90,6 +90,35 @@ def test_md_graph_replays_equal_eager(fcc_dir, precision):
         b.energies()  # eager host work on the graph context between replays
 
 
+def test_md_graph_dropped_on_option_change(fcc_dir):
+    """Changing the BF16 term count between graph-captured runs re-captures
+    the step graph: graph runs before and after the change equal the same
+    eager runs (the one-term forces differ from the three-term ones by far
+    more than the tolerance, so a stale graph would show)."""
+    say("case: MD graph, BF16 terms 3 -> 1 between runs, FCC 500")
+    a = setup_md(fcc_dir, 5, n2p2_amd.MIXED, skin=0.4)[0]
+    b = setup_md(fcc_dir, 5, n2p2_amd.MIXED, skin=0.4)[0]
+    c = setup_md(fcc_dir, 5, n2p2_amd.MIXED, skin=0.4)[0]
+    for m in (a, b, c):
+        m.set_option(n2p2_amd.OPT_BF16_TERMS, 3)
+    a.md_run(5, graph_steps=5)
+    b.md_run(5)
+    c.md_run(5)
+    a.set_option(n2p2_amd.OPT_BF16_TERMS, 1)
+    b.set_option(n2p2_amd.OPT_BF16_TERMS, 1)
+    a.md_run(5, graph_steps=5)
+    b.md_run(5)
+    c.md_run(5)
+    xa, _ = a.md_state()
+    xb, _ = b.md_state()
+    xc, _ = c.md_state()
+    dab = float(np.abs(xa - xb).max())
+    dbc = float(np.abs(xb - xc).max())
+    say("  graph vs eager max|dx| %.2e; one vs three terms max|dx| %.2e" % (dab, dbc))
+    assert dab <= 1e-9
+    assert dbc > 100 * max(dab, 1e-12)
+
+
 def test_md_energy_conservation_and_rebuilds(fcc_dir):
     """Mixed form, 500 atoms, 200 steps at 600 K with a 0.3 A skin: the list
     is rebuilt on the device, total energy is conserved, and the final forces

@@ -242,7 +242,9 @@ def test_autotune_keeps_results(water_dir):
     say("  max|dF| after tuning %.2e" % d)
     assert d <= 1e-4  # bf16 operand splits amplify FP32-ulp input changes to ~1e-5 relative
     m.set_tuning(128, 320, 512, 16)
-    assert m.tuning() == {"trip_sf": 128, "trip_force": 320, "trip_tangent": 512, "grad_atoms": 16}
+    tu2 = m.tuning()
+    assert {k: tu2[k] for k in ("trip_sf", "trip_force", "trip_tangent", "grad_atoms")} == \
+        {"trip_sf": 128, "trip_force": 320, "trip_tangent": 512, "grad_atoms": 16}
     m.evaluate(n2p2_amd.MIXED, forces=True)
     assert float(np.abs(m.forces() - F0).max()) <= 1e-4
     with pytest.raises(n2p2_amd.NNPError):
@@ -453,3 +455,48 @@ def test_sf_gather_identical(kind, water_dir, tmp_path):
         assert np.array_equal(out[0][0], o[0]), "gather %d: G differs" % g
         assert np.allclose(out[0][1], o[1], rtol=1e-13, atol=0.0), "gather %d: energies differ" % g
         assert np.abs(out[0][2] - o[2]).max() <= 1e-13 * np.abs(o[2]).max(), "gather %d: forces differ" % g
+
+
+@pytest.mark.parametrize("kind", ["fcc", "water"])
+def test_angular_merge(kind, water_dir, tmp_path):
+    """All angular groups of an element in one triplet walk (merged) against
+    the per-group walks and the oracle, with and without the partner gather:
+    the same terms summed in another order, so G, energies and forces agree
+    to FP32 summation order and both meet the oracle bounds.  FCC: 10
+    four-member groups per element (merged by default); water: 3 ten-member
+    groups (per-group walks by default)."""
+    if kind == "fcc":
+        d = synth.fcc_model(str(tmp_path / "fcc"))
+        pos, elem, box = synth.fcc_lattice(n_cells=5)
+    else:
+        d = water_dir
+        pos, elem, box = synth.water_box(4)
+    say("case: angular merge on/off, %s, %d atoms" % (kind, len(elem)))
+    m = n2p2_amd.Mode(d)
+    m.set_structures(remap(pos, box), elem, box[None])
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE)
+    Go = o.G()
+    out = {}
+    for mg in (0, 1):
+        for gather in (0, 1):
+            m.set_option(n2p2_amd.OPT_ANGULAR_MERGE_SF, mg)
+            m.set_option(n2p2_amd.OPT_ANGULAR_MERGE_FORCE, mg)
+            m.set_option(n2p2_amd.OPT_SF_GATHER, gather)
+            m.evaluate(n2p2_amd.MIXED, forces=True)
+            G = m.G().copy()
+            gerr = float((np.abs(G - Go) / np.maximum(np.abs(Go), 1.0)).max())
+            rms, mx, erel = errors(m, o)
+            say("  merge %d gather %d: G err %.2e  E rel %.2e  F rms %.2e max %.2e" % (mg, gather, gerr, erel, rms, mx))
+            assert gerr <= 1e-5 and erel <= 1e-5 and rms <= 1e-5
+            out[(mg, gather)] = (G, m.energies().copy(), m.forces().copy())
+    G0, E0, F0 = out[(0, 0)]
+    for key, (G, E, F) in out.items():
+        dg = float((np.abs(G - G0) / np.maximum(np.abs(G0), 1.0)).max())
+        df = float(np.abs(F - F0).max() / np.abs(F0).max())
+        say("  %s vs per-group walk: G %.2e  F %.2e" % (key, dg, df))
+        assert dg <= 2e-6 and df <= 2e-5
+    m.set_option(n2p2_amd.OPT_ANGULAR_MERGE_SF, -1)
+    m.set_option(n2p2_amd.OPT_ANGULAR_MERGE_FORCE, -1)
+    assert m.get_option(n2p2_amd.OPT_ANGULAR_MERGE_SF) == -1
