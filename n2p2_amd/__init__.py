@@ -28,6 +28,7 @@ OPT_SF_GATHER = 4
 OPT_PLAIN_FORCES = 5
 OPT_ANGULAR_MERGE_SF = 6
 OPT_ANGULAR_MERGE_FORCE = 7
+OPT_MD_STAGING = 8
 EVAL_FORCES = 1
 
 _STATUS = {0: "OK", 1: "ARGUMENT", 2: "FILE", 3: "MODEL", 4: "RANGE", 5: "STATE", 6: "HIP", 7: "UNSUPPORTED"}

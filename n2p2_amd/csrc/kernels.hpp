@@ -24,7 +24,7 @@ struct SysView {
     int* nb_j;             // [N*cap]
     int* nb_img;           // [N*cap]
     int* nb_rev;           // [N*cap] entry (j*cap + k) holding (i, -image)
-    int* flags;            // [0] overflow, [1] max count
+    int* flags;            // [0] overflow bits, [1] max list count, [2] max staged count
     // cell list
     int ncells;
     int* atom_cell;        // [N]

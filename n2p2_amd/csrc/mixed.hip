@@ -172,25 +172,51 @@ __device__ inline float dist2(const float4& a, const float4& b)
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
-__device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s, int i, int capL, const Stage& S)
+// Stages the atom's list entries closer than the model's largest cutoff, in
+// list order (entries beyond it contribute nothing; lists built with a skin
+// carry them).  idx, when given, receives each staged slot's list entry.
+// More such entries than capL: the first capL are staged, flag bit 4 is set
+// (the caller repeats with the full capacity) and flags[2] keeps the
+// largest count met, from which callers size capL.
+__device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s, int i, int capL, const Stage& S,
+                                 unsigned short* idx = nullptr)
 {
     const int lane = threadIdx.x;
     const StructT& St = s.st[s.sid[i]];
-    const int n = min(s.nb_n[i], capL);
-    if (s.nb_n[i] > capL && lane == 0) atomicOr(&s.flags[0], 4);  // the host repeats at full capacity
+    const int nl = s.nb_n[i];
     const size_t base = (size_t)i * s.cap;
-    for (int k = lane; k < n; k += kWave) {
-        const int j = s.nb_j[base + k];
-        double dr[3];
-        entry_vector(s, i, base + k, St, dr);
-        const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-        S.g[k] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
-        // .w: the neighbour's atomic number, the weight of types 12 and 13
-        S.c[k] = make_float4((float)(1.0 / d), 0.f, 0.f, (float)M->Zel[s.elem[j]]);
-        S.el[k] = (unsigned char)s.elem[j];
+    const double rmax = M->maxCutoff;
+    int cnt = 0;
+    for (int b = 0; b < nl; b += kWave) {
+        const int k = b + lane;
+        double dr[3] = {0.0, 0.0, 0.0}, d = 0.0;
+        int j = 0;
+        bool ok = k < nl;
+        if (ok) {
+            j = s.nb_j[base + k];
+            entry_vector(s, i, base + k, St, dr);
+            d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+            ok = d < rmax;
+        }
+        const unsigned long long m = __ballot(ok);
+        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (ok && pos < capL) {
+            S.g[pos] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
+            // .w: the neighbour's atomic number, the weight of types 12 and 13
+            S.c[pos] = make_float4((float)(1.0 / d), 0.f, 0.f, (float)M->Zel[s.elem[j]]);
+            S.el[pos] = (unsigned char)s.elem[j];
+            if (idx) idx[pos] = (unsigned short)k;
+        }
+        cnt += __popcll(m);
+    }
+    if (lane == 0) {
+        if (cnt > capL) atomicOr(&s.flags[0], 4);
+        if (cnt > __hip_atomic_load(&s.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&s.flags[2], cnt);
     }
     __syncthreads();
-    return n;
+    return min(cnt, capL);
 }
 
 // Ordered list of neighbours with element `elem` inside rc.
@@ -1257,14 +1283,15 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     double* ax = reinterpret_cast<double*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip, stage_mp(NB0, NB1)));
     double* ay = ax + capL;
     double* az = ay + capL;
-    char* mbase = reinterpret_cast<char*>(az + capL);
+    unsigned short* idx = reinterpret_cast<unsigned short*>(az + capL);
+    char* mbase = reinterpret_cast<char*>(idx + capL);
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
-    const int n = stage_atom(M, s, i, capL, S);
+    const int n = stage_atom(M, s, i, capL, S, idx);
     for (int k = lane; k < n; k += kWave) { ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0; }
     const float* wi = fw + (size_t)i * ldG;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -1505,7 +1532,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     }
     const size_t base = (size_t)i * s.cap;
     for (int k = lane; k < n; k += kWave) {
-        const int j = s.nb_j[base + k];
+        const int j = s.nb_j[base + idx[k]];
         if (j == i && !push) continue;
         atomicAdd(F + 3 * (size_t)j, ax[k]);
         atomicAdd(F + 3 * (size_t)j + 1, ay[k]);
@@ -1531,18 +1558,19 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(0, 0));
     float4* du = reinterpret_cast<float4*>(reinterpret_cast<char*>(sm) + stage_bytes(capL, trip, stage_mp(0, 0)));
+    unsigned short* idx = reinterpret_cast<unsigned short*>(du + capL);
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.elem[i];
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
-    const int n = stage_atom(M, s, i, capL, S);
+    const int n = stage_atom(M, s, i, capL, S, idx);
     {
         const double ux = u[3 * (size_t)i], uy = u[3 * (size_t)i + 1], uz = u[3 * (size_t)i + 2];
         const size_t base = (size_t)i * s.cap;
         for (int k = lane; k < n; k += kWave) {
-            const int j = s.nb_j[base + k];
+            const int j = s.nb_j[base + idx[k]];
             const float dx = (float)(ux - u[3 * (size_t)j]);
             const float dy = (float)(uy - u[3 * (size_t)j + 1]);
             const float dz = (float)(uz - u[3 * (size_t)j + 2]);
@@ -2074,7 +2102,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     const int tsf = mm.tune.trip_sf, tf = mm.tune.trip_force;
     // LDS of the largest kernel instances (the member paths' constant area)
     const size_t lds_sf_max = stage_bytes(capL, tsf, stage_mp(0, 0));
-    const size_t lds_f_max = stage_bytes(capL, tf, stage_mp(0, 0)) + sizeof(double) * 3 * (size_t)capL;
+    const size_t lds_f_max = stage_bytes(capL, tf, stage_mp(0, 0)) + (sizeof(double) * 3 + 2) * (size_t)capL;
     if (lds_f_max > 65536 || lds_sf_max > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     const bool merge_sf = use_merge(mm, mm.tune.merge_sf), merge_f = use_merge(mm, mm.tune.merge_force);
@@ -2157,7 +2185,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
             constexpr int cm = decltype(m)::value, c0 = decltype(nb0)::value, c1 = decltype(nb1)::value;
             constexpr bool cf = decltype(fast)::value;
             const bool mg = c0 > 0 && merge_f;
-            const size_t lds_f = stage_bytes(capL, tf, stage_mp(c0, c1)) + sizeof(double) * 3 * (size_t)capL +
+            const size_t lds_f = stage_bytes(capL, tf, stage_mp(c0, c1)) + (sizeof(double) * 3 + 2) * (size_t)capL +
                                  (mg ? merge_bytes(mm.merge_ng, 2 * (c0 + c1)) : 0);
             if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
             if (mg)
@@ -2178,7 +2206,7 @@ void launch_tangent32(const MixedModel& mm, const ModelT& t, const ModelT* d_mod
     if (s.natoms == 0) return;
     const int capL = s.stage_cap;
     const int tt = mm.tune.trip_tangent;
-    const size_t lds = stage_bytes(capL, tt, stage_mp(0, 0)) + sizeof(float4) * (size_t)capL;
+    const size_t lds = stage_bytes(capL, tt, stage_mp(0, 0)) + (sizeof(float4) + 2) * (size_t)capL;
     if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     dispatch_members(mm.fastM, pow2_members(t), [&](auto m, auto fast) {
         hipLaunchKernelGGL((k_tangent32<decltype(m)::value, decltype(fast)::value>), dim3(s.natoms), dim3(kWave), lds,

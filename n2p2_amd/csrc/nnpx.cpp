@@ -96,6 +96,8 @@ struct nnpx_ctx {
     // maximum count (0: the list capacity); a larger count at run time is
     // flagged (bit 4) and the evaluation repeated at full capacity
     int stage_cap = 0;
+    int md_stage_cap = 0;        // mixed MD: staging fitted to the in-cutoff entries (0: the list capacity)
+    int md_stage_req = -1;       // NNPX_OPT_MD_STAGING
     bool nbr_valid = false;
     bool rev_valid = false;
     // results state
@@ -290,7 +292,10 @@ SysView sysview(nnpx_ctx* c)
     s.sid = existing<int>(c, "sid");
     s.st = existing<StructT>(c, "st");
     s.cap = c->cap;
-    s.stage_cap = c->stage_cap > 0 && c->stage_cap < c->cap && !c->md_ready ? c->stage_cap : c->cap;
+    // MD lists carry a skin: the mixed kernels stage only the entries inside
+    // the cutoff, in a capacity fitted to them at nnpx_md_init
+    const int fit = c->md_ready ? c->md_stage_cap : c->stage_cap;
+    s.stage_cap = fit > 0 && fit < c->cap ? fit : c->cap;
     s.nb_n = existing<int>(c, "nb_n");
     s.nb_j = existing<int>(c, "nb_j");
     s.nb_img = existing<int>(c, "nb_img");
@@ -858,6 +863,10 @@ nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
             c->mixed.tune.sf_gather = (int)value;
             drop_graph(c);
             break;
+        case NNPX_OPT_MD_STAGING:
+            if (value < -1 || value > 65535) fail(NNPX_ERR_ARGUMENT, "MD staging must be -1, 0 or a slot count");
+            c->md_stage_req = (int)value;
+            break;
         case NNPX_OPT_ANGULAR_MERGE_SF:
         case NNPX_OPT_ANGULAR_MERGE_FORCE:
             if (value < -1 || value > 1) fail(NNPX_ERR_ARGUMENT, "angular merge must be -1, 0 or 1");
@@ -884,6 +893,7 @@ nnpx_status nnpx_get_option(const nnpx_ctx* cc, int option, int64_t* value)
         case NNPX_OPT_PLAIN_FORCES: *value = c->plain_forces; break;
         case NNPX_OPT_ANGULAR_MERGE_SF: *value = c->mixed.tune.merge_sf; break;
         case NNPX_OPT_ANGULAR_MERGE_FORCE: *value = c->mixed.tune.merge_force; break;
+        case NNPX_OPT_MD_STAGING: *value = c->md_stage_req; break;
         default: fail(NNPX_ERR_ARGUMENT, "unknown option");
         }
     });
@@ -1951,7 +1961,8 @@ nnpx_status nnpx_autotune(nnpx_ctx* c)
         hip_check(hipMemsetAsync(u, 0, sizeof(double) * 3 * c->N, c->stream), "hipMemsetAsync");
         try {
             for (int trip : {128, 256, 384, 512, 768, 1024}) {
-                if (!fits(trip, sizeof(double) * 3 * (size_t)capL)) continue;
+                if (!fits(trip, (sizeof(double) * 3 + 2) * (size_t)capL + kMaxElem * kMaxElem + (size_t)kMaxMerge * 20 * 8))
+                continue;
                 tu.trip_sf = tu.trip_force = tu.trip_tangent = trip;
                 evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, true);  // warm-up
                 settle(c);
@@ -2177,20 +2188,101 @@ nnpx_status nnpx_md_init(nnpx_ctx* c, int precision, const double* masses, const
         buf<double>(c, "md_epot", kMDHistory);
         double* ekin = buf<double>(c, "md_ekin", kMDHistory);
         c->md_precision = precision;
+        c->md_stage_cap = 0;
         // forces of the initial positions (allocates every buffer a step uses)
         evaluate(c, precision, NNPX_EVAL_FORCES, true);
         settle(c);
+        if (precision == NNPX_MIXED && c->md_stage_req != 0) {
+            // staging for the most in-cutoff entries met plus an eighth (at
+            // least 16), or as requested: a run that meets more repeats at
+            // the list capacity
+            int f[3];
+            copy_out(c, f, existing<int>(c, "flags"), sizeof f);
+            const int m = c->md_stage_req > 0 ? c->md_stage_req : f[2] + std::max(16, f[2] / 8);
+            const int fitted = ((m + 31) / 32) * 32;
+            c->md_stage_cap = m > 0 && fitted < c->cap ? fitted : 0;
+        }
         md_reset(gate, counters, ekin, c->stream);
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         c->md_ready = true;
     });
 }
 
+namespace {
+
+// State a run of MD steps changes, copied before a run with fitted staging
+// so a run that overflowed it can be repeated at the list capacity.
+const char* const kMDSaved[] = {"pos", "md_vel", "md_ref", "F", "md_epot", "md_ekin"};
+
+void md_backup(nnpx_ctx* c, bool restore)
+{
+    for (const char* name : kMDSaved) {
+        const DevBuf& b = c->bufs.at(name);
+        const std::string bak = std::string("bak_") + name;
+        void* d = restore ? b.p : buf<char>(c, bak, b.bytes);
+        const void* src = restore ? existing<char>(c, bak) : b.p;
+        hip_check(hipMemcpyAsync(d, src, b.bytes, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+    }
+    int* cnt = existing<int>(c, "md_counters");
+    int* bak = restore ? cnt : buf<int>(c, "bak_md_counters", 16);
+    hip_check(hipMemcpyAsync(bak, restore ? existing<int>(c, "bak_md_counters") : cnt, 16 * sizeof(int),
+                             hipMemcpyDeviceToDevice, c->stream),
+              "hipMemcpyAsync");
+}
+
+void md_steps(nnpx_ctx* c, int nsteps, int graph_steps);
+
+}  // namespace
+
 nnpx_status nnpx_md_run(nnpx_ctx* c, int nsteps, int graph_steps)
 {
     return guard(c, [&] {
         need_md(c);
         if (nsteps < 0 || graph_steps < 0) fail(NNPX_ERR_ARGUMENT, "negative step count");
+        const bool fitted = c->md_precision == NNPX_MIXED && c->md_stage_cap > 0;
+        int* flags = existing<int>(c, "flags");
+        if (fitted) {
+            int f[3];
+            copy_out(c, f, flags, sizeof f);
+            if (f[0] & 4) {
+                f[0] &= ~4;
+                hip_check(hipMemcpy(flags, f, sizeof(int), hipMemcpyHostToDevice), "hipMemcpy");
+            }
+            md_backup(c, false);
+        }
+        md_steps(c, nsteps, graph_steps);
+        if (fitted) {
+            int f[3];
+            copy_out(c, f, flags, sizeof f);
+            if (f[0] & 4) {
+                // an atom met more in-cutoff entries than the fitted staging:
+                // back to the state before the run, a fresh list, and the run
+                // again with the whole list capacity staged
+                md_backup(c, true);
+                int* gate = existing<int>(c, "md_gate");
+                md_set_gate(gate, 1, c->stream);
+                md_wrap(existing<double>(c, "pos"), existing<double>(c, "md_ref"), existing<int>(c, "sid"),
+                        existing<StructT>(c, "st"), c->N, gate, c->stream);
+                SysView s = sysview(c);
+                s.gate = gate;
+                launch_neighbors(s, list_cutoff(c), c->any_cell, c->stream, false);
+                md_set_gate(gate, 0, c->stream);
+                f[0] &= ~4;
+                hip_check(hipMemcpyAsync(flags, f, sizeof(int), hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+                hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+                c->md_stage_cap = 0;
+                drop_graph(c);
+                md_steps(c, nsteps, graph_steps);
+            }
+        }
+    });
+}
+
+namespace {
+
+void md_steps(nnpx_ctx* c, int nsteps, int graph_steps)
+{
+    {
         const bool timing = c->timing;
         if (graph_steps > 0 && (!c->gexec || c->graph_steps != graph_steps)) {
             drop_graph(c);
@@ -2235,8 +2327,10 @@ nnpx_status nnpx_md_run(nnpx_ctx* c, int nsteps, int graph_steps)
         c->sfd_valid = c->md_precision == NNPX_PLAIN_FP64 && !fused_fits(c);
         c->last_stored = c->sfd_valid;
         c->last_precision = c->md_precision;
-    });
+    }
 }
+
+}  // namespace
 
 nnpx_status nnpx_md_state(nnpx_ctx* c, double* pos, double* vel)
 {

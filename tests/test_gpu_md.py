@@ -119,6 +119,39 @@ def test_md_graph_dropped_on_option_change(fcc_dir):
     assert dbc > 100 * max(dab, 1e-12)
 
 
+@pytest.mark.parametrize("staging", [-1, 32])
+def test_md_fitted_staging(fcc_dir, staging):
+    """Mixed MD with the in-cutoff staging fitted at nnpx_md_init (-1) and
+    with a capacity below the 78 in-cutoff neighbours of the lattice (32):
+    the too-small run overflows, is repeated from its start with the whole
+    list staged, and both end where the run with the whole list staged (0)
+    ends; later runs of the repeated context keep the whole list."""
+    say("case: MD staging %d vs whole list, FCC 500, 2 x 10 steps in graphs of 5" % staging)
+
+    def run(opt):
+        pos, elem, box = synth.fcc_lattice(n_cells=5)
+        m = n2p2_amd.Mode(fcc_dir)
+        m.set_option(n2p2_amd.OPT_MD_STAGING, opt)
+        m.set_structures(remap(pos, box), elem, box[None])
+        mass = masses_fcc()
+        m.md_init(mass, synth.maxwell_velocities(mass[elem], 600.0, synth.SEED_VELOCITY), 1.0, 1.0, n2p2_amd.MIXED)
+        out = []
+        for _ in range(2):
+            m.md_run(10, graph_steps=5)
+            x, v = m.md_state()
+            ep, ek, nr = m.md_energies()
+            out.append((x, v, ep.copy(), ek.copy(), nr))
+        return out
+    ref, got = run(0), run(staging)
+    for k, (a, b) in enumerate(zip(ref, got)):
+        dx = float(np.abs(a[0] - b[0]).max())
+        dv = float(np.abs(a[1] - b[1]).max())
+        de = float(np.abs(a[2] - b[2]).max())
+        say("  run %d: max|dx| %.2e  max|dv| %.2e  max|dEpot| %.2e  steps %d/%d" % (k, dx, dv, de, len(a[2]), len(b[2])))
+        assert len(a[2]) == len(b[2]) == 10 * (k + 1)
+        assert dx <= 1e-9 and dv <= 1e-9 and de <= 1e-8 * max(1.0, float(np.abs(a[2]).max()))
+
+
 def test_md_energy_conservation_and_rebuilds(fcc_dir):
     """Mixed form, 500 atoms, 200 steps at 600 K with a 0.3 A skin: the list
     is rebuilt on the device, total energy is conserved, and the final forces
