@@ -123,7 +123,6 @@ __device__ inline GroupF group_f(const GroupT& g)
 }
 
 constexpr int kRadChunk = 8;  // radial members accumulated per pass
-constexpr int kFill = 1;      // candidate pairs per lane and step of the triplet compaction (more measured slower)
 
 // LDS staging of one atom's neighbours (FP32 geometry) and the compacted
 // triplet buffer.
@@ -134,8 +133,29 @@ struct Stage {
     unsigned short* la; unsigned short* lb;  // group lists
     unsigned* tjk;   // compacted pair (j | k << 16); r_jk^2 is recomputed from g
     float4* mp;      // [mpn] per-group constants (member constants or power-path terms)
+    int* eoff;       // [kMaxElem + 1] first staged slot of each neighbour element
     int trip;        // capacity of the triplet buffer
 };
+
+// A group's neighbour list: the staged slots base .. base + n - 1 (contig:
+// every staged neighbour of the element lies inside the group's cutoff) or
+// the n slots listed in S.la / S.lb.
+struct List {
+    int base, n;
+    bool contig;
+};
+
+// A wave-uniform value held in a scalar register (values read from LDS or
+// shuffled are uniform but not known to be by the compiler).
+__device__ inline int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Slot of list entry a (CONTIG: base + a, else l[a]).
+template <bool CONTIG>
+__device__ inline int slot(const unsigned short* l, int base, int a)
+{
+    if constexpr (CONTIG) return base + a;
+    else return l[a];
+}
 
 // float4 slots of the per-group constant area: member constants (two per
 // member) on the member paths, one per term on the power path.
@@ -155,13 +175,14 @@ __device__ inline Stage carve(float* base, int capL, int trip, int mpn)
     S.la = reinterpret_cast<unsigned short*>(S.mp + mpn);
     S.lb = S.la + capL;
     S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
+    S.eoff = reinterpret_cast<int*>(S.el + capL);
     return S;
 }
 // bytes of the staging area (capL is a multiple of 32 and trip of 64, so the
-// byte arrays end 8-byte aligned)
+// byte arrays end 16-byte aligned, as does the element offset table)
 __host__ __device__ constexpr size_t stage_bytes(int capL, int trip, int mpn)
 {
-    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)mpn * 16;
+    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)mpn * 16 + 48;
 }
 
 // |b - a|^2 of two staged positions, one fixed operation order for the
@@ -172,78 +193,107 @@ __device__ inline float dist2(const float4& a, const float4& b)
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
 }
 
-// Stages the atom's list entries closer than the model's largest cutoff, in
-// list order (entries beyond it contribute nothing; lists built with a skin
-// carry them).  idx, when given, receives each staged slot's list entry.
-// More such entries than capL: the first capL are staged, flag bit 4 is set
-// (the caller repeats with the full capacity) and flags[2] keeps the
-// largest count met, from which callers size capL.
+// Stages the atom's list entries closer than the model's largest cutoff
+// (entries beyond it contribute nothing; lists built with a skin carry
+// them), grouped by neighbour element and in list order within an element,
+// so an element's neighbours are the slots eoff[e] .. eoff[e + 1] - 1.  idx,
+// when given, receives each staged slot's list entry.  More such entries than
+// capL: only slots below capL are staged, flag bit 4 is set (the caller
+// repeats with the full capacity) and flags[2] keeps the largest count met,
+// from which callers size capL.
 __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s, int i, int capL, const Stage& S,
                                  unsigned short* idx = nullptr)
 {
     const int lane = threadIdx.x;
     const StructT& St = s.st[s.sid[i]];
     const int nl = s.nb_n[i];
+    const int ne = M->nelem;
     const size_t base = (size_t)i * s.cap;
     const double rmax = M->maxCutoff;
-    int cnt = 0;
-    for (int b = 0; b < nl; b += kWave) {
-        const int k = b + lane;
-        double dr[3] = {0.0, 0.0, 0.0}, d = 0.0;
-        int j = 0;
-        bool ok = k < nl;
-        if (ok) {
-            j = s.nb_j[base + k];
+    // the entry of this lane in pass b: its vector, distance and element
+    // (-1: none or beyond the cutoff)
+    auto entry = [&](int k, double* dr, double& d) {
+        int el = -1;
+        d = 0.0;
+        dr[0] = dr[1] = dr[2] = 0.0;
+        if (k < nl) {
             entry_vector(s, i, base + k, St, dr);
             d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-            ok = d < rmax;
+            if (d < rmax) el = s.elem[s.nb_j[base + k]];
         }
-        const unsigned long long m = __ballot(ok);
-        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        if (ok && pos < capL) {
+        return el;
+    };
+    // pass 1: in-cutoff entries per element (lane e counts element e)
+    int cnt = 0;
+    for (int b = 0; b < nl; b += kWave) {
+        double dr[3], d;
+        const int el = entry(b + lane, dr, d);
+        for (int e = 0; e < ne; ++e) {
+            const int c = __popcll(__ballot(el == e));
+            if (lane == e) cnt += c;
+        }
+    }
+    const int incl = wave_incl_scan(lane < ne ? cnt : 0);
+    const int total = __shfl(incl, kWave - 1, kWave);
+    int next = incl - (lane < ne ? cnt : 0);  // lane e: next slot of element e
+    if (lane <= ne) S.eoff[lane] = min(lane < ne ? next : total, capL);
+    // pass 2: each entry to its element's next slot
+    for (int b = 0; b < nl; b += kWave) {
+        const int k = b + lane;
+        double dr[3], d;
+        const int el = entry(k, dr, d);
+        int pos = -1;
+        for (int e = 0; e < ne; ++e) {
+            const unsigned long long m = __ballot(el == e);
+            const int at = __shfl(next, e, kWave);
+            if (el == e)
+                pos = at + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (lane == e) next += __popcll(m);
+        }
+        if (pos >= 0 && pos < capL) {
             S.g[pos] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
             // .w: the neighbour's atomic number, the weight of types 12 and 13
-            S.c[pos] = make_float4((float)(1.0 / d), 0.f, 0.f, (float)M->Zel[s.elem[j]]);
-            S.el[pos] = (unsigned char)s.elem[j];
+            S.c[pos] = make_float4((float)(1.0 / d), 0.f, 0.f, (float)M->Zel[el]);
+            S.el[pos] = (unsigned char)el;
             if (idx) idx[pos] = (unsigned short)k;
         }
-        cnt += __popcll(m);
     }
     if (lane == 0) {
-        if (cnt > capL) atomicOr(&s.flags[0], 4);
-        if (cnt > __hip_atomic_load(&s.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(&s.flags[2], cnt);
+        if (total > capL) atomicOr(&s.flags[0], 4);
+        if (total > __hip_atomic_load(&s.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&s.flags[2], total);
     }
     __syncthreads();
-    return min(cnt, capL);
+    return uni(min(total, capL));
 }
 
-// Ordered list of neighbours with element `elem` inside rc.
-__device__ inline int make_list(const Stage& S, int n, int elem, float rc, unsigned short* out)
+// Neighbours of element `elem` (all elements: elem < 0) inside the group's
+// cutoff rc.  full: rc is the model's largest cutoff, so every staged
+// neighbour of the element qualifies and the list is its slot range;
+// otherwise the qualifying slots are listed in `out`.  Ends with a barrier.
+__device__ inline List make_list(const Stage& S, int n, int elem, float rc, bool full, unsigned short* out)
 {
     const int lane = threadIdx.x;
-    int cnt = 0;
-    for (int b = 0; b < n; b += kWave) {
-        const int k = b + lane;
-        const bool ok = k < n && (elem < 0 || S.el[k] == elem) && S.g[k].w < rc;
-        const int incl = wave_incl_scan(ok ? 1 : 0);
-        if (ok) out[cnt + incl - 1] = (unsigned short)k;
-        cnt += __shfl(incl, kWave - 1, kWave);
+    const int lo = uni(elem < 0 ? 0 : S.eoff[elem]), hi = uni(elem < 0 ? n : S.eoff[elem + 1]);
+    List L{lo, hi - lo, true};
+    if (!full) {
+        int cnt = 0;
+        for (int b = lo; b < hi; b += kWave) {
+            const int k = b + lane;
+            const bool ok = k < hi && S.g[k].w < rc;
+            const int incl = wave_incl_scan(ok ? 1 : 0);
+            if (ok) out[cnt + incl - 1] = (unsigned short)k;
+            cnt += __shfl(incl, kWave - 1, kWave);
+        }
+        L = List{0, uni(cnt), false};
     }
     __syncthreads();
-    return cnt;
+    return L;
 }
 
-// In-cutoff neighbours of all elements, grouped by element (neighbour order
-// within an element): a merged walk's pairs then change group only at
-// element boundaries along a row.
-__device__ inline int make_sorted_list(const Stage& S, int n, int nelem, float rc, unsigned short* out)
-{
-    int cnt = 0;
-    for (int el = 0; el < nelem; ++el) cnt += make_list(S, n, el, rc, out + cnt);
-    return cnt;
-}
+// Whether a group's cutoff covers every staged neighbour.
+__device__ inline bool full_cutoff(const ModelT* __restrict__ M, const GroupT& G) { return G.rc >= M->maxCutoff; }
 
 // Cutoff values of every staged neighbour for one radius (elements whose
 // groups all share it compute them once per atom).
@@ -341,11 +391,12 @@ struct NoFlush {
 
 // `flush` runs after each buffer of triplets (a few per lane), so callers can
 // keep short FP32 partial sums and fold them into their long accumulators.
-template <typename Work, typename Flush = NoFlush>
-__device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work,
-                                        Flush&& flush = Flush{})
+template <bool CONTIG, typename Work, typename Flush = NoFlush>
+__device__ inline void for_each_triplet(const Stage& S, const List& A, const List& B, bool same, float rc2,
+                                        Work&& work, Flush&& flush = Flush{})
 {
     const int lane = threadIdx.x;
+    const int nA = A.n, nB = B.n;
     int full, total;
     if (same) {
         full = nA * ((nA - 1) / 2);
@@ -355,48 +406,39 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
         total = nA * nB;
     }
     const unsigned short* lb = same ? S.la : S.lb;
+    const int bB = same ? A.base : B.base;
     const float inv = nA > 0 ? 1.f / (float)nA : 0.f;
     int t0 = 0;
     while (t0 < total) {
         // fill the buffer with passing pairs ...
         int cnt = 0;
-        while (t0 < total && cnt <= S.trip - kFill * kWave) {
-            bool ok[kFill];
-            unsigned jk[kFill];
-            float r2[kFill];
-#pragma unroll
-            for (int f = 0; f < kFill; ++f) {
-                const int t = t0 + f * kWave + lane;
-                ok[f] = false;
-                jk[f] = 0;
-                r2[f] = 0.f;
-                if (t < total) {
-                    const int u = same && t >= full ? t - full : t;
-                    int q = (int)((float)u * inv);
-                    int r = u - q * nA;
-                    if (r < 0) { --q; r += nA; }
-                    else if (r >= nA) { ++q; r -= nA; }
-                    int a, b;
-                    if (!same) { a = r; b = q; }
-                    else if (t < full) { a = r; b = r + q + 1; if (b >= nA) b -= nA; }
-                    else { a = u; b = u + nA / 2; }
-                    int j = S.la[a], k = lb[b];
-                    if (j > k) { const int x = j; j = k; k = x; }
-                    r2[f] = dist2(S.g[j], S.g[k]);
-                    ok[f] = r2[f] < rc2;
-                    jk[f] = (unsigned)j | ((unsigned)k << 16);
-                }
+        while (t0 < total && cnt <= S.trip - kWave) {
+            const int t = t0 + lane;
+            bool ok = false;
+            unsigned jk = 0;
+            if (t < total) {
+                const int u = same && t >= full ? t - full : t;
+                int q = (int)((float)u * inv);
+                int r = u - q * nA;
+                if (r < 0) { --q; r += nA; }
+                else if (r >= nA) { ++q; r -= nA; }
+                int a, b;
+                if (!same) { a = r; b = q; }
+                else if (t < full) { a = r; b = r + q + 1; if (b >= nA) b -= nA; }
+                else { a = u; b = u + nA / 2; }
+                int j = slot<CONTIG>(S.la, A.base, a), k = slot<CONTIG>(lb, bB, b);
+                if (j > k) { const int x = j; j = k; k = x; }
+                ok = dist2(S.g[j], S.g[k]) < rc2;
+                jk = (unsigned)j | ((unsigned)k << 16);
             }
-#pragma unroll
-            for (int f = 0; f < kFill; ++f) {
-                const unsigned long long m = __ballot(ok[f]);
-                if (ok[f]) {
-                    const int pos = cnt + __popcll(m & __lanemask_lt());
-                    S.tjk[pos] = jk[f];
-                }
-                cnt += __popcll(m);
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                S.tjk[pos] = jk;
             }
-            t0 += kFill * kWave;
+            cnt += __popcll(m);
+            t0 += kWave;
         }
         __syncthreads();
         // ... then run the member loops with full waves
@@ -418,11 +460,12 @@ __device__ inline void for_each_triplet(const Stage& S, int nA, int nB, bool sam
 // contribution there too.  `flush` runs after each buffer.
 // GATHER: gB holds the partners' geometry in list order (gB[b] =
 // S.g[lb[b]]), so a candidate's index and position load together.
-template <bool GATHER, typename Work, typename Flush = NoFlush>
-__device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB, int nA, int nB, bool same, float rc2,
-                                               Work&& work, Flush&& flush = Flush{})
+template <bool CONTIG, bool GATHER, typename Work, typename Flush = NoFlush>
+__device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB, const List& A, const List& B,
+                                               bool same, float rc2, Work&& work, Flush&& flush = Flush{})
 {
     const int lane = threadIdx.x;
+    const int nA = A.n, nB = B.n;
     const int R = same ? (nA - 1) / 2 : nB;          // regular columns per row
     const int Rx = same && nA % 2 == 0 ? R + 1 : R;  // with the half-way column (rows a < nA / 2 only)
     const int half = nA / 2;
@@ -433,9 +476,12 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB,
     const int cw = (Rx + nblk - 1) / nblk, rh = kWave / cw;
     const int lr = lane / cw, lc = lane - lr * cw;  // lanes with lr >= rh stay idle
     const unsigned short* lb = same ? S.la : S.lb;
+    const int bB = same ? A.base : B.base;
+    // contiguous partners are already in list order
+    const float4* gp = CONTIG ? S.g + bB : gB;
     int a0 = 0, c0 = 0;
     // the lane's row atom and its geometry, reloaded when the block moves down
-    int ja = S.la[lr < min(nA, rh) ? lr : 0];
+    int ja = slot<CONTIG>(S.la, A.base, lr < min(nA, rh) ? lr : 0);
     float4 ga = S.g[ja];
     while (a0 < nA) {
         int cnt = 0;
@@ -443,13 +489,11 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB,
             const int a = a0 + lr, c = c0 + lc;
             bool ok = false;
             unsigned jk = 0;
-            float r2 = 0.f;
             if (lr < rh && a < nA && c < Rx && (c < R || a < half)) {
                 int b = same ? a + c + 1 : c;
                 if (same && b >= nA) b -= nA;
-                const int k = lb[b];
-                r2 = dist2(ga, GATHER ? gB[b] : S.g[k]);
-                ok = r2 < rc2;
+                const int k = slot<CONTIG>(lb, bB, b);
+                ok = dist2(ga, (GATHER || CONTIG) ? gp[b] : S.g[k]) < rc2;
                 jk = (unsigned)ja | ((unsigned)k << 16);
             }
             const unsigned long long m = __ballot(ok);
@@ -464,7 +508,7 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB,
                 c0 = 0;
                 a0 += rh;
                 const int an = a0 + lr;
-                ja = S.la[an < nA ? an : 0];
+                ja = slot<CONTIG>(S.la, A.base, an < nA ? an : 0);
                 ga = S.g[ja];
             }
         }
@@ -485,11 +529,12 @@ __device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB,
 // partner atoms k differ more across the lanes of one step than in the block
 // variant, which keeps the force kernel's LDS atomics on k from colliding.
 // `flush` runs after each buffer.
-template <typename Work, typename Flush = NoFlush>
-__device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, bool same, float rc2, Work&& work,
-                                             Flush&& flush = Flush{})
+template <bool CONTIG, typename Work, typename Flush = NoFlush>
+__device__ inline void for_each_triplet_rows(const Stage& S, const List& A, const List& B, bool same, float rc2,
+                                             Work&& work, Flush&& flush = Flush{})
 {
     const int lane = threadIdx.x;
+    const int nA = A.n, nB = B.n;
     int full, total, R;
     if (same) {
         R = (nA - 1) / 2;
@@ -500,52 +545,49 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
         full = nA * nB;
         total = full;
     }
+    if (total == 0) return;
     const unsigned short* lb = same ? S.la : S.lb;
-    const float inv = R > 0 ? 1.f / (float)R : 0.f;
+    const int bB = same ? A.base : B.base;
+    // the lane's linear index t = q R + r, advanced by kWave = dq R + dr per
+    // step (decoded once per buffer, so only t0 lives across the work)
+    const int dq = R > 0 ? kWave / R : 0, dr = R > 0 ? kWave - dq * R : kWave;
     int t0 = 0;
     while (t0 < total) {
         int cnt = 0;
-        while (t0 < total && cnt <= S.trip - kFill * kWave) {
-            bool ok[kFill];
-            unsigned jk[kFill];
-            float r2[kFill];
-#pragma unroll
-            for (int f = 0; f < kFill; ++f) {
-                const int t = t0 + f * kWave + lane;
-                ok[f] = false;
-                jk[f] = 0;
-                r2[f] = 0.f;
-                if (t < total) {
-                    int a, b;
-                    if (t < full) {
-                        int q = (int)((float)t * inv);
-                        int r = t - q * R;
-                        if (r < 0) { --q; r += R; }
-                        else if (r >= R) { ++q; r -= R; }
-                        a = q;
-                        if (same) { b = a + r + 1; if (b >= nA) b -= nA; }
-                        else b = r;
-                    } else {
-                        a = t - full;
-                        b = a + nA / 2;
-                    }
-                    const int j = S.la[a], k = lb[b];
-                    r2[f] = dist2(S.g[j], S.g[k]);
-                    ok[f] = r2[f] < rc2;
-                    jk[f] = (unsigned)j | ((unsigned)k << 16);
+        int t = t0 + lane, q = 0, r = t;
+        if (R > 0) {
+            q = t / R;
+            r = t - q * R;
+        }
+        while (t0 < total && cnt <= S.trip - kWave) {
+            bool ok = false;
+            unsigned jk = 0;
+            if (t < total) {
+                int a, b;
+                if (t < full) {
+                    a = q;
+                    if (same) { b = a + r + 1; if (b >= nA) b -= nA; }
+                    else b = r;
+                } else {
+                    a = t - full;
+                    b = a + nA / 2;
                 }
+                const int j = slot<CONTIG>(S.la, A.base, a), k = slot<CONTIG>(lb, bB, b);
+                ok = dist2(S.g[j], S.g[k]) < rc2;
+                jk = (unsigned)j | ((unsigned)k << 16);
             }
-#pragma unroll
-            for (int f = 0; f < kFill; ++f) {
-                const unsigned long long m = __ballot(ok[f]);
-                if (ok[f]) {
-                    const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                        __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    S.tjk[pos] = jk[f];
-                }
-                cnt += __popcll(m);
+            const unsigned long long m = __ballot(ok);
+            if (ok) {
+                const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                S.tjk[pos] = jk;
             }
-            t0 += kFill * kWave;
+            cnt += __popcll(m);
+            t0 += kWave;
+            t += kWave;
+            q += dq;
+            r += dr;
+            if (r >= R) { r -= R; ++q; }
         }
         __syncthreads();
         const int per = (cnt + kWave - 1) / kWave;
@@ -556,11 +598,40 @@ __device__ inline void for_each_triplet_rows(const Stage& S, int nA, int nB, boo
     }
 }
 
-// Copies the partner list's geometry into list order for the block walk.
-__device__ inline void gather_partners(const Stage& S, const unsigned short* lb, int nB, float4* gB)
+// Copies the partner list's geometry into list order for the block walk
+// (listed partners only: a slot range is in list order already).
+__device__ inline void gather_partners(const Stage& S, const List& A, const List& B, bool same, float4* gB)
 {
+    if (A.contig) return;
+    const unsigned short* lb = same ? S.la : S.lb;
+    const int nB = same ? A.n : B.n;
     for (int b = threadIdx.x; b < nB; b += kWave) gB[b] = S.g[lb[b]];
     __syncthreads();
+}
+
+// The walks for slot-range lists and for listed slots.
+template <bool GATHER, typename Work, typename Flush = NoFlush>
+__device__ inline void walk_blocks(const Stage& S, const float4* gB, const List& A, const List& B, bool same,
+                                   float rc2, Work&& work, Flush&& flush = Flush{})
+{
+    if (A.contig) for_each_triplet_blocks<true, GATHER>(S, gB, A, B, same, rc2, work, flush);
+    else for_each_triplet_blocks<false, GATHER>(S, gB, A, B, same, rc2, work, flush);
+}
+
+template <typename Work, typename Flush = NoFlush>
+__device__ inline void walk_rows(const Stage& S, const List& A, const List& B, bool same, float rc2, Work&& work,
+                                 Flush&& flush = Flush{})
+{
+    if (A.contig) for_each_triplet_rows<true>(S, A, B, same, rc2, work, flush);
+    else for_each_triplet_rows<false>(S, A, B, same, rc2, work, flush);
+}
+
+template <typename Work, typename Flush = NoFlush>
+__device__ inline void walk_all(const Stage& S, const List& A, const List& B, bool same, float rc2, Work&& work,
+                                Flush&& flush = Flush{})
+{
+    if (A.contig) for_each_triplet<true>(S, A, B, same, rc2, work, flush);
+    else for_each_triplet<false>(S, A, B, same, rc2, work, flush);
 }
 
 // ------------------------------------------------------------ power path --
@@ -586,9 +657,10 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const bool same = GT.e1 == GT.e2;
     const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
     const bool wt = GT.type == 13;
-    const int nA = make_list(S, n, GT.e1, g.rc, S.la);
-    const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
-    if constexpr (GATHER) gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
+    const bool full = full_cutoff(M, GT);
+    const List A = make_list(S, n, GT.e1, g.rc, full, S.la);
+    const List B = same ? A : make_list(S, n, GT.e2, g.rc, full, S.lb);
+    if constexpr (GATHER) gather_partners(S, A, B, same, gB);
     float acc[NBAS], cmp[NBAS], part[NBAS];
 #pragma unroll
     for (int b = 0; b < NBAS; ++b) { acc[b] = 0.f; cmp[b] = 0.f; part[b] = 0.f; }
@@ -596,7 +668,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const float lam0 = gm.lam[0], lam1 = gm.lam[1];
     int jc = -1;  // j of the cached geometry (runs of one j per lane)
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    for_each_triplet_blocks<GATHER>(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    walk_blocks<GATHER>(S, gB, A, B, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
         if (j != jc) {
             gj = S.g[j];
             cj = S.c[j];
@@ -704,8 +776,9 @@ __device__ inline void sf32_power_merged(const ModelT* __restrict__ M, const Sta
     double* red = reinterpret_cast<double*>(mbase + kMaxElem * kMaxElem);
     for (int x = lane; x < kMaxElem * kMaxElem; x += kWave) pmap[x] = am.pair[x];
     for (int x = lane; x < ng * NBAS; x += kWave) red[x] = 0.0;
-    const int nA = make_sorted_list(S, n, M->nelem, g.rc, S.la);  // its barrier orders the fills above
-    if constexpr (GATHER) gather_partners(S, S.la, nA, gB);
+    // all elements' in-cutoff neighbours, element by element (staging order)
+    const List A = make_list(S, n, -1, g.rc, full_cutoff(M, G0), S.la);  // its barrier orders the fills above
+    if constexpr (GATHER) gather_partners(S, A, A, true, gB);
     float part[NBAS];
 #pragma unroll
     for (int b = 0; b < NBAS; ++b) part[b] = 0.f;
@@ -724,7 +797,7 @@ __device__ inline void sf32_power_merged(const ModelT* __restrict__ M, const Sta
     const float lam0 = gm.lam[0], lam1 = gm.lam[1];
     int jc = -1, rowj = 0;
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    for_each_triplet_blocks<GATHER>(S, gB, nA, nA, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    walk_blocks<GATHER>(S, gB, A, A, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
         if (j != jc) {
             gj = S.g[j];
             cj = S.c[j];
@@ -825,9 +898,11 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 float acc[kRadChunk], cmp[kRadChunk];
 #pragma unroll
                 for (int q = 0; q < kRadChunk; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
-                for (int k = lane; k < n; k += kWave) {
+                // type 2: the slots of neighbour element e1; type 12: every slot
+                const int lo = wt ? 0 : S.eoff[GT.e1], hi = wt ? n : S.eoff[GT.e1 + 1];
+                for (int k = lo + lane; k < hi; k += kWave) {
                     const float r = S.g[k].w;
-                    if ((!wt && S.el[k] != GT.e1) || !(r < g.rc)) continue;
+                    if (!(r < g.rc)) continue;
                     const float fc = wt ? S.c[k].y * S.c[k].w : S.c[k].y;
 #pragma unroll
                     for (int q = 0; q < kRadChunk; ++q)
@@ -862,9 +937,10 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             const bool same = GT.e1 == GT.e2;
             const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
             const bool wt = GT.type == 13;
-            const int nA = make_list(S, n, GT.e1, g.rc, S.la);
-            const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
-            if constexpr (GATHER) gather_partners(S, same ? S.la : S.lb, same ? nA : nB, gB);
+            const bool full = full_cutoff(M, GT);
+            const List A = make_list(S, n, GT.e1, g.rc, full, S.la);
+            const List B = same ? A : make_list(S, n, GT.e2, g.rc, full, S.lb);
+            if constexpr (GATHER) gather_partners(S, A, B, same, gB);
             const GroupMixF gm = GM[E.grp0 + gi];
             const MemberMasks R = load_members(S, mf, nmem, nullptr);
             // FAST: zeta and -eta log2 e in registers for the whole group;
@@ -882,7 +958,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             }
             int jc = -1;  // j of the cached geometry (runs of one j per lane)
             float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-            for_each_triplet_blocks<GATHER>(S, gB, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+            walk_blocks<GATHER>(S, gB, A, B, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
                 if (j != jc) {
                     gj = S.g[j];
                     cj = S.c[j];
@@ -1176,7 +1252,8 @@ __device__ inline void force32_power_merged(const ModelT* __restrict__ M, const 
             }
         wtab[x] = make_float2(sb, sa);
     }
-    const int nA = make_sorted_list(S, n, M->nelem, g.rc, S.la);  // its barrier orders the tables above
+    // all elements' in-cutoff neighbours, element by element (staging order)
+    const List A = make_list(S, n, -1, g.rc, full_cutoff(M, G0), S.la);  // its barrier orders the tables above
     int cur = -1, rowj = 0;
     float rx = 0.f, ry = 0.f, rz = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
     auto flush = [&]() {
@@ -1191,7 +1268,7 @@ __device__ inline void force32_power_merged(const ModelT* __restrict__ M, const 
         cur = -1;
     };
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    for_each_triplet_rows(S, nA, nA, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    walk_rows(S, A, A, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
         if (j != cur) {
             flush();
             cur = j;
@@ -1314,9 +1391,11 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             // radial: each lane owns its neighbours, no conflicts; type 12
             // weights every neighbour by its atomic number
             const bool wt = GT.type == 12;
-            for (int k = lane; k < n; k += kWave) {
+            // type 2: the slots of neighbour element e1; type 12: every slot
+            const int lo = wt ? 0 : S.eoff[GT.e1], hi = wt ? n : S.eoff[GT.e1 + 1];
+            for (int k = lo + lane; k < hi; k += kWave) {
                 const float4 gk = S.g[k];
-                if ((!wt && S.el[k] != GT.e1) || !(gk.w < g.rc)) continue;
+                if (!(gk.w < g.rc)) continue;
                 const float4 ck = S.c[k];
                 float P = 0.f;
 #pragma unroll 2
@@ -1336,8 +1415,9 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
         const bool same = GT.e1 == GT.e2;
         const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
         const bool wt = GT.type == 13;    // Z_j Z_k weights (SymGrpExpAngnWeighted.cpp)
-        const int nA = make_list(S, n, GT.e1, g.rc, S.la);
-        const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+        const bool full = full_cutoff(M, GT);
+        const List A = make_list(S, n, GT.e1, g.rc, full, S.la);
+        const List B = same ? A : make_list(S, n, GT.e2, g.rc, full, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
         constexpr bool POW = NB0 > 0;
         constexpr int NBAS = POW ? 2 * (NB0 + NB1) : 1;
@@ -1381,7 +1461,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             }
         };
         float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;  // geometry of the run's j
-        for_each_triplet_rows(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+        walk_rows(S, A, B, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
             if (j != cur) {
                 flush();
                 cur = j;
@@ -1598,9 +1678,11 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
                 float acc[kRadChunk], cmp[kRadChunk];
 #pragma unroll
                 for (int q = 0; q < kRadChunk; ++q) { acc[q] = 0.f; cmp[q] = 0.f; }
-                for (int k = lane; k < n; k += kWave) {
+                // type 2: the slots of neighbour element e1; type 12: every slot
+                const int lo = wt ? 0 : S.eoff[GT.e1], hi = wt ? n : S.eoff[GT.e1 + 1];
+                for (int k = lo + lane; k < hi; k += kWave) {
                     const float4 gk = S.g[k];
-                    if ((!wt && S.el[k] != GT.e1) || !(gk.w < g.rc)) continue;
+                    if (!(gk.w < g.rc)) continue;
                     const float4 ck = S.c[k];
                     const float sk = du[k].w * (wt ? ck.x * ck.w : ck.x);  // (r_ij / r).(u_i - u_j)
 #pragma unroll
@@ -1630,11 +1712,12 @@ __global__ __launch_bounds__(64) void k_tangent32(const ModelT* __restrict__ M, 
         const bool same = GT.e1 == GT.e2;
         const bool wide = GT.type == 9;  // no r_jk cutoff or exponent term (SymGrpExpAngw.cpp:156-380)
         const bool wt = GT.type == 13;    // Z_j Z_k weights (SymGrpExpAngnWeighted.cpp)
-        const int nA = make_list(S, n, GT.e1, g.rc, S.la);
-        const int nB = same ? nA : make_list(S, n, GT.e2, g.rc, S.lb);
+        const bool full = full_cutoff(M, GT);
+        const List A = make_list(S, n, GT.e1, g.rc, full, S.la);
+        const List B = same ? A : make_list(S, n, GT.e2, g.rc, full, S.lb);
         const GroupMixF gm = GM[E.grp0 + gi];
         const MemberMasks R = load_members(S, mf, nmem, nullptr);
-        for_each_triplet(S, nA, nB, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+        walk_all(S, A, B, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
             const float4 gj = S.g[j], gk = S.g[k];
             const float4 cj = S.c[j], ck = S.c[k];
             const float r2jk = dist2(gj, gk);

@@ -1950,7 +1950,8 @@ nnpx_status nnpx_autotune(nnpx_ctx* c)
         const int capL = c->cap;
         const ModelT& t = c->hm.t;
         auto fits = [&](int trip, size_t extra) {
-            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)2 * kMaxMembersExp * 16 + extra <= 65536;
+            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)2 * kMaxMembersExp * 16 + 48 + extra <=
+                   65536;
         };
         c->timing = true;
         double best_sf = 1e30, best_f = 1e30, best_t = 1e30;
