@@ -82,14 +82,15 @@ typedef enum {
                                   default), 0 off, 1 on; results agree to
                                   FP32 summation order                     */
     NNPX_OPT_ANGULAR_MERGE_FORCE = 7, /* the same for the mixed force kernel */
-    NNPX_OPT_MD_STAGING = 8     /* mixed-form MD (read by nnpx_md_init): LDS
-                                  slots for the in-cutoff neighbours of an
-                                  atom; -1 (default) the most met at
-                                  nnpx_md_init plus an eighth (at least 16),
-                                  0 the whole skin-extended list, n > 0 that
-                                  many (rounded up to 32).  A run of steps
-                                  that meets more is repeated from its start
-                                  with the whole list staged.              */
+    NNPX_OPT_MD_STAGING = 8     /* MD (read by nnpx_md_init): LDS slots of
+                                  the fused kernels for the in-cutoff
+                                  neighbours of an atom; -1 (default) the
+                                  most met at nnpx_md_init plus an eighth
+                                  (at least 16), 0 the whole skin-extended
+                                  list, n > 0 that many (rounded up to 32).
+                                  A run of steps that meets more is
+                                  repeated from its start with the whole
+                                  list staged.                             */
 } nnpx_option;
 
 /* Flags for nnpx_evaluate. */

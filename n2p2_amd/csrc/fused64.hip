@@ -49,6 +49,7 @@ struct Stage64 {
     unsigned short* la;  // group lists
     unsigned short* lb;
     unsigned char* el;   // neighbour element
+    unsigned short* idx; // list entry of each staged slot
     double2* pw;         // [kPw64] power-path term weights (force) or term sums (values)
     int trip;
 };
@@ -57,7 +58,7 @@ constexpr int kPw64 = 32;
 
 __host__ __device__ constexpr size_t stage64_bytes(int capL, int trip)
 {
-    return (size_t)kPw64 * 16 + (size_t)capL * (32 + 32 + 24 + 2 + 2 + 1) + (size_t)trip * 4;
+    return (size_t)kPw64 * 16 + (size_t)capL * (32 + 32 + 24 + 2 + 2 + 1 + 2) + (size_t)trip * 4;
 }
 
 __device__ inline Stage64 carve64(double4* base, int capL, int trip)
@@ -74,7 +75,58 @@ __device__ inline Stage64 carve64(double4* base, int capL, int trip)
     S.la = reinterpret_cast<unsigned short*>(S.tjk + trip);
     S.lb = S.la + capL;
     S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
+    S.idx = reinterpret_cast<unsigned short*>(S.el + capL);
     return S;
+}
+
+// Stages the atom's list entries closer than the model's largest cutoff, in
+// list order (entries beyond it contribute nothing; MD lists built with a
+// skin carry them), with each slot's list entry in idx and, for the force
+// kernel, zeroed neighbour force sums.  More such entries than capL: only
+// the first capL are staged, flag bit 4 is set (the caller repeats with the
+// full capacity) and flags[2] keeps the largest count met.
+__device__ inline int stage64(const ModelT* __restrict__ M, const SysView& s, int i, int capL, const Stage64& S,
+                              bool zero_sums)
+{
+    const int lane = threadIdx.x;
+    const StructT& St = s.st[s.sid[i]];
+    const int nl = s.nb_n[i];
+    const size_t base = (size_t)i * s.cap;
+    const double rmax = M->maxCutoff;
+    int cnt = 0;
+    for (int b = 0; b < nl; b += kWave) {
+        const int k = b + lane;
+        double dr[3] = {0.0, 0.0, 0.0}, d = 0.0;
+        bool ok = k < nl;
+        if (ok) {
+            entry_vector(s, i, base + k, St, dr);
+            d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
+            ok = d < rmax;
+        }
+        const unsigned long long m = __ballot(ok);
+        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+        if (ok && pos < capL) {
+            const int el = s.elem[s.nb_j[base + k]];
+            S.g[pos] = make_double4(dr[0], dr[1], dr[2], d);
+            S.c[pos] = make_double4(1.0 / d, 0.0, 0.0, M->Zel[el]);
+            S.el[pos] = (unsigned char)el;
+            S.idx[pos] = (unsigned short)k;
+            if (zero_sums) {
+                S.ax[pos] = 0.0;
+                S.ay[pos] = 0.0;
+                S.az[pos] = 0.0;
+            }
+        }
+        cnt += __popcll(m);
+    }
+    if (lane == 0) {
+        if (cnt > capL) atomicOr(&s.flags[0], 4);
+        if (cnt > __hip_atomic_load(&s.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&s.flags[2], cnt);
+    }
+    __syncthreads();
+    return __builtin_amdgcn_readfirstlane(min(cnt, capL));
 }
 
 // Cutoff value and derivative; the cosine cutoff through sincospi (exact
@@ -202,24 +254,10 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, c
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const double tanhPre = M->tanhPre;
-    const StructT& St = s.st[s.sid[i]];
-    const int n = min(s.nb_n[i], capL);
+    const int n = stage64(M, s, i, capL, S, true);
     const size_t base = (size_t)i * s.cap;
-    for (int k = lane; k < n; k += kWave) {
-        const int j = s.nb_j[base + k];
-        double dr[3];
-        entry_vector(s, i, base + k, St, dr);
-        const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-        S.g[k] = make_double4(dr[0], dr[1], dr[2], d);
-        S.c[k] = make_double4(1.0 / d, 0.0, 0.0, M->Zel[s.elem[j]]);
-        S.el[k] = (unsigned char)s.elem[j];
-        S.ax[k] = 0.0;
-        S.ay[k] = 0.0;
-        S.az[k] = 0.0;
-    }
     const double* wi = fw + (size_t)i * ldG;
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    __syncthreads();
 
     for (int gi = 0; gi < E.ngrp; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
@@ -421,7 +459,7 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, c
         atomicAdd(F + 3 * (size_t)i + 2, fz);
     }
     for (int k = lane; k < n; k += kWave) {
-        const int j = s.nb_j[base + k];
+        const int j = s.nb_j[base + S.idx[k]];
         if (j == i && !push) continue;
         atomicAdd(F + 3 * (size_t)j, S.ax[k]);
         atomicAdd(F + 3 * (size_t)j + 1, S.ay[k]);
@@ -451,20 +489,8 @@ __global__ __launch_bounds__(64) void k_values64(const ModelT* __restrict__ M, c
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const double tanhPre = M->tanhPre;
-    const StructT& St = s.st[s.sid[i]];
-    const int n = min(s.nb_n[i], capL);
-    const size_t base = (size_t)i * s.cap;
-    for (int k = lane; k < n; k += kWave) {
-        const int j = s.nb_j[base + k];
-        double dr[3];
-        entry_vector(s, i, base + k, St, dr);
-        const double d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-        S.g[k] = make_double4(dr[0], dr[1], dr[2], d);
-        S.c[k] = make_double4(1.0 / d, 0.0, 0.0, M->Zel[s.elem[j]]);
-        S.el[k] = (unsigned char)s.elem[j];
-    }
+    const int n = stage64(M, s, i, capL, S, false);
     double* Gi = G + (size_t)i * ldG;
-    __syncthreads();
 
     for (int gi = 0; gi < E.ngrp; ++gi) {
         const GroupT& GR = M->grp[E.grp0 + gi];
@@ -668,7 +694,7 @@ void launch_values64(const ModelT* m, const ModelT& hm, const MixedModel& mm, co
 {
     if (s.natoms == 0) return;
     if (!G || !s.nb_n || !s.nb_j) throw Error{NNPX_ERR_STATE, "FP64 SF values: a buffer is missing"};
-    const int capL = ((std::max(s.cap, 1) + 31) / 32) * 32;
+    const int capL = ((std::max(s.stage_cap, 1) + 31) / 32) * 32;
     const size_t lds = stage64_bytes(capL, kTrip64);
     if (lds > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the FP64 SF value kernel's LDS"};
     if (!hm.has_exp) return;
@@ -688,7 +714,7 @@ void launch_force64_fused(const ModelT* m, const ModelT& hm, const MixedModel& m
     if (s.natoms == 0) return;
     if (!dEdG || !fac || !w || !F || !s.nb_n || !s.nb_j)
         throw Error{NNPX_ERR_STATE, "fused FP64 forces: a buffer is missing"};
-    const int capL = ((std::max(s.cap, 1) + 31) / 32) * 32;
+    const int capL = ((std::max(s.stage_cap, 1) + 31) / 32) * 32;
     const size_t lds = stage64_bytes(capL, kTrip64);
     if (lds > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the FP64 fused force kernel's LDS"};

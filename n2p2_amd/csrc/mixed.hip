@@ -1216,6 +1216,26 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
 }
 
 // --------------------------------------------------- fused force backward --
+// One triplet's force terms, P1 r_ij + P2 r_ik + P3 (r_ik - r_ij) in the
+// form (P1 - P3) r_ij + P3 r_ik for j and (P2 - P3) r_ik + P3 r_ij for k:
+// the atom's own and j's parts summed in FP32 over the lane's run of j, k's
+// part added to its FP64 LDS sum.
+__device__ inline void force_terms(float P1, float P2, float P3, const float4& gj, const float4& gk, float& sx,
+                                   float& sy, float& sz, float& rx, float& ry, float& rz, double* kx, double* ky,
+                                   double* kz)
+{
+    const float d1 = P1 - P3, d2 = P2 - P3;
+    sx = fmaf(P1, gj.x, fmaf(P2, gk.x, sx));
+    sy = fmaf(P1, gj.y, fmaf(P2, gk.y, sy));
+    sz = fmaf(P1, gj.z, fmaf(P2, gk.z, sz));
+    rx = fmaf(d1, gj.x, fmaf(P3, gk.x, rx));
+    ry = fmaf(d1, gj.y, fmaf(P3, gk.y, ry));
+    rz = fmaf(d1, gj.z, fmaf(P3, gk.z, rz));
+    lds_add(kx, (double)fmaf(d2, gk.x, P3 * gj.x));
+    lds_add(ky, (double)fmaf(d2, gk.y, P3 * gj.y));
+    lds_add(kz, (double)fmaf(d2, gk.z, P3 * gj.z));
+}
+
 // Forces of all power-path angular groups of an element in one triplet walk
 // over the element-ordered in-cutoff list: each triplet takes the term
 // weights (wb, wa) of its pair's group from LDS; the geometry and the
@@ -1297,29 +1317,30 @@ __device__ inline void force32_power_merged(const ModelT* __restrict__ M, const 
         const float pr3 = cj.y * ck.y * dfcjk * ijk;
         const float E0 = fexp2(-gm.etaL[0] * r2sum);
         const float E1 = NB1 > 0 ? fexp2(-gm.etaL[1] * r2sum) : 0.f;
-        float Q0 = 0.f, Q1 = 0.f, A = 0.f;
+        // (1 + lambda cos)^(2^b - 1) carried beside the power: no division
+        float Q0 = 0.f, Q1 = 0.f, qa0 = 0.f, qa1 = 0.f;
 #pragma unroll
         for (int sl = 0; sl < 2; ++sl) {
             const float pl = fmaf(gm.lam[sl], cost, 1.f);
             const bool pos = pl > 0.f;
-            float p = pos ? pl : 0.f;
-            float qa0 = 0.f, qa1 = 0.f;
+            float p = pos ? pl : 0.f, pq = pos ? 1.f : 0.f;
 #pragma unroll
             for (int b = 0; b < NBM; ++b) {
                 if (b < NB0) {
                     const float2 w = wq[sl * NB0 + b];
                     Q0 = fmaf(w.x, p, Q0);
-                    qa0 = fmaf(w.y, p, qa0);
+                    qa0 = fmaf(w.y, pq, qa0);
                 }
                 if (b < NB1) {
                     const float2 w = wq[2 * NB0 + sl * NB1 + b];
                     Q1 = fmaf(w.x, p, Q1);
-                    qa1 = fmaf(w.y, p, qa1);
+                    qa1 = fmaf(w.y, pq, qa1);
                 }
+                pq *= p;
                 p *= p;
             }
-            A += (E0 * qa0 + E1 * qa1) * (pos ? __builtin_amdgcn_rcpf(pl) : 0.f);
         }
+        const float A = E0 * qa0 + E1 * qa1;
         const float B0 = E0 * Q0, B1 = E1 * Q1;
         const float B = B0 + B1;
         const float C0 = gm.twoEta[0] * B0 + gm.twoEta[1] * B1;
@@ -1327,19 +1348,7 @@ __device__ inline void force32_power_merged(const ModelT* __restrict__ M, const 
         const float P1 = pfc * (A * (rinvijik - cost * cj.x * cj.x) - C0) + B * pr1;
         const float P2 = pfc * (A * (rinvijik - cost * ck.x * ck.x) - C0) + B * pr2;
         const float P3 = pfc * (A * rinvijik + C3) - B * pr3;
-        const float x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
-        const float p1x = P1 * gj.x, p1y = P1 * gj.y, p1z = P1 * gj.z;
-        const float p2x = P2 * gk.x, p2y = P2 * gk.y, p2z = P2 * gk.z;
-        const float p3x = P3 * x3, p3y = P3 * y3, p3z = P3 * z3;
-        sx += p1x + p2x;
-        sy += p1y + p2y;
-        sz += p1z + p2z;
-        rx += p1x + p3x;
-        ry += p1y + p3y;
-        rz += p1z + p3z;
-        lds_add(ax + k, (double)(p2x - p3x));
-        lds_add(ay + k, (double)(p2y - p3y));
-        lds_add(az + k, (double)(p2z - p3z));
+        force_terms(P1, P2, P3, gj, gk, sx, sy, sz, rx, ry, rz, ax + k, ay + k, az + k);
     }, flush);
     flush();
     __syncthreads();
@@ -1503,30 +1512,32 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
                 // B_h = E_h sum wb P, A_s = sum_h E_h sum wa P (powers of lambda s)
                 const float E0 = fexp2(-gm.etaL[0] * r2sum);
                 const float E1 = NB1 > 0 ? fexp2(-gm.etaL[1] * r2sum) : 0.f;
-                float Q0 = 0.f, Q1 = 0.f, As[2];
+                // (1 + lambda cos)^(2^b - 1) is carried beside the power, so
+                // A needs no division
+                float Q0 = 0.f, Q1 = 0.f, qa0 = 0.f, qa1 = 0.f;
 #pragma unroll
                 for (int sl = 0; sl < 2; ++sl) {
                     const float pl = fmaf(gm.lam[sl], cost, 1.f);
-                    float p = pl > 0.f ? pl : 0.f;
-                    float qa0 = 0.f, qa1 = 0.f;
+                    const bool pos = pl > 0.f;
+                    float p = pos ? pl : 0.f, pq = pos ? 1.f : 0.f;
 #pragma unroll
                     for (int b = 0; b < NBM; ++b) {
                         if (b < NB0) {
                             Q0 = fmaf(pwb[sl * NB0 + b], p, Q0);
-                            qa0 = fmaf(pwa[sl * NB0 + b], p, qa0);
+                            qa0 = fmaf(pwa[sl * NB0 + b], pq, qa0);
                         }
                         if (b < NB1) {
                             Q1 = fmaf(pwb[2 * NB0 + sl * NB1 + b], p, Q1);
-                            qa1 = fmaf(pwa[2 * NB0 + sl * NB1 + b], p, qa1);
+                            qa1 = fmaf(pwa[2 * NB0 + sl * NB1 + b], pq, qa1);
                         }
+                        pq *= p;
                         p *= p;
                     }
-                    As[sl] = E0 * qa0 + E1 * qa1;
                 }
                 const float B0 = E0 * Q0, B1 = E1 * Q1;
                 B = B0 + B1;
                 C0 = gm.twoEta[0] * B0 + gm.twoEta[1] * B1;
-                A = As[0] * Rp[0] + As[1] * Rp[1];
+                A = E0 * qa0 + E1 * qa1;
             } else if constexpr (FAST) {
                 float A0 = 0.f, A1 = 0.f;
 #pragma unroll
@@ -1581,19 +1592,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             const float P1 = zw * (pfc * (A * (rinvijik - cost * cj.x * cj.x) - C1) + B * pr1);
             const float P2 = zw * (pfc * (A * (rinvijik - cost * ck.x * ck.x) - C2) + B * pr2);
             const float P3 = zw * (pfc * (A * rinvijik + C3) - B * pr3);
-            const float x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
-            const float p1x = P1 * gj.x, p1y = P1 * gj.y, p1z = P1 * gj.z;
-            const float p2x = P2 * gk.x, p2y = P2 * gk.y, p2z = P2 * gk.z;
-            const float p3x = P3 * x3, p3y = P3 * y3, p3z = P3 * z3;
-            sx += p1x + p2x;
-            sy += p1y + p2y;
-            sz += p1z + p2z;
-            rx += p1x + p3x;
-            ry += p1y + p3y;
-            rz += p1z + p3z;
-            lds_add(ax + k, (double)(p2x - p3x));
-            lds_add(ay + k, (double)(p2y - p3y));
-            lds_add(az + k, (double)(p2z - p3z));
+            force_terms(P1, P2, P3, gj, gk, sx, sy, sz, rx, ry, rz, ax + k, ay + k, az + k);
         });
         flush();
         __syncthreads();

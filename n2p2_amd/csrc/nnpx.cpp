@@ -96,7 +96,7 @@ struct nnpx_ctx {
     // maximum count (0: the list capacity); a larger count at run time is
     // flagged (bit 4) and the evaluation repeated at full capacity
     int stage_cap = 0;
-    int md_stage_cap = 0;        // mixed MD: staging fitted to the in-cutoff entries (0: the list capacity)
+    int md_stage_cap = 0;        // MD: staging fitted to the in-cutoff entries (0: the list capacity)
     int md_stage_req = -1;       // NNPX_OPT_MD_STAGING
     bool nbr_valid = false;
     bool rev_valid = false;
@@ -2193,13 +2193,13 @@ nnpx_status nnpx_md_init(nnpx_ctx* c, int precision, const double* masses, const
         // forces of the initial positions (allocates every buffer a step uses)
         evaluate(c, precision, NNPX_EVAL_FORCES, true);
         settle(c);
-        if (precision == NNPX_MIXED && c->md_stage_req != 0) {
+        if (c->md_stage_req != 0) {
             // staging for the most in-cutoff entries met plus an eighth (at
             // least 16), or as requested: a run that meets more repeats at
             // the list capacity
             int f[3];
             copy_out(c, f, existing<int>(c, "flags"), sizeof f);
-            const int m = c->md_stage_req > 0 ? c->md_stage_req : f[2] + std::max(16, f[2] / 8);
+            const int m = c->md_stage_req > 0 ? c->md_stage_req : f[2] > 0 ? f[2] + std::max(16, f[2] / 8) : 0;
             const int fitted = ((m + 31) / 32) * 32;
             c->md_stage_cap = m > 0 && fitted < c->cap ? fitted : 0;
         }
@@ -2240,7 +2240,7 @@ nnpx_status nnpx_md_run(nnpx_ctx* c, int nsteps, int graph_steps)
     return guard(c, [&] {
         need_md(c);
         if (nsteps < 0 || graph_steps < 0) fail(NNPX_ERR_ARGUMENT, "negative step count");
-        const bool fitted = c->md_precision == NNPX_MIXED && c->md_stage_cap > 0;
+        const bool fitted = c->md_stage_cap > 0;
         int* flags = existing<int>(c, "flags");
         if (fitted) {
             int f[3];

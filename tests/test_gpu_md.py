@@ -119,14 +119,16 @@ def test_md_graph_dropped_on_option_change(fcc_dir):
     assert dbc > 100 * max(dab, 1e-12)
 
 
+@pytest.mark.parametrize("precision", [n2p2_amd.MIXED, n2p2_amd.PLAIN_FP64])
 @pytest.mark.parametrize("staging", [-1, 32])
-def test_md_fitted_staging(fcc_dir, staging):
-    """Mixed MD with the in-cutoff staging fitted at nnpx_md_init (-1) and
-    with a capacity below the 78 in-cutoff neighbours of the lattice (32):
-    the too-small run overflows, is repeated from its start with the whole
-    list staged, and both end where the run with the whole list staged (0)
-    ends; later runs of the repeated context keep the whole list."""
-    say("case: MD staging %d vs whole list, FCC 500, 2 x 10 steps in graphs of 5" % staging)
+def test_md_fitted_staging(fcc_dir, staging, precision):
+    """MD with the in-cutoff staging fitted at nnpx_md_init (-1) and with a
+    capacity below the 78 in-cutoff neighbours of the lattice (32): the
+    too-small run overflows, is repeated from its start with the whole list
+    staged, and both end where the run with the whole list staged (0) ends;
+    later runs of the repeated context keep the whole list."""
+    say("case: MD staging %d vs whole list, FCC 500, 2 x 10 steps in graphs of 5, precision %d"
+        % (staging, precision))
 
     def run(opt):
         pos, elem, box = synth.fcc_lattice(n_cells=5)
@@ -134,7 +136,7 @@ def test_md_fitted_staging(fcc_dir, staging):
         m.set_option(n2p2_amd.OPT_MD_STAGING, opt)
         m.set_structures(remap(pos, box), elem, box[None])
         mass = masses_fcc()
-        m.md_init(mass, synth.maxwell_velocities(mass[elem], 600.0, synth.SEED_VELOCITY), 1.0, 1.0, n2p2_amd.MIXED)
+        m.md_init(mass, synth.maxwell_velocities(mass[elem], 600.0, synth.SEED_VELOCITY), 1.0, 1.0, precision)
         out = []
         for _ in range(2):
             m.md_run(10, graph_steps=5)
