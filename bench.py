@@ -40,7 +40,8 @@ sys.path.insert(0, ROOT)
 METRIC = "atom-steps/s (E+F) mixed vs FP64 GPU form at force RMS err <=1e-3 eV/A"
 PEAK_FP64 = 78.6    # TFLOP/s, MI355X FP64 vector
 PEAK_FP32 = 157.3   # TFLOP/s, MI355X FP32 vector
-KERNELS = ("nbr", "rev", "sf64", "sf64v", "nn64", "energy", "force64", "force64f", "sf32", "nnbf16", "force32")
+KERNELS = ("nbr", "rev", "sf64", "sf64v", "nn64", "energy", "force64", "force64f", "sf32", "nnbf16", "force32",
+           "residual", "tangent64", "tangent32", "grad64", "grad32")
 NN_MACS = {"water": 50 * 25 + 25 * 25 + 25 * 1, "fcc": 80 * 30 + 30 * 30 + 30 * 1}
 
 
@@ -147,6 +148,9 @@ def roofline(times, w, nn_macs, natoms, workload):
     avg_s = ms / launches / 1e3
     if dom in ("nn64", "nnbf16"):
         flop, peak, bound = nn_flop, PEAK_FP32, "mfma"
+    elif dom in ("grad64", "grad32"):
+        # forward pass with the tangent and the reverse pass: 8 C per atom
+        flop, peak, bound = nn_flop * 8 / 6, (PEAK_FP64 if dom.endswith("64") else PEAK_FP32), "valu"
     else:
         flop, peak, bound = sf_flop, (PEAK_FP64 if dom.endswith("64") else PEAK_FP32), "valu"
     achieved = flop / avg_s / 1e12
@@ -288,11 +292,16 @@ def base_record(value, steps, warmup, ms_per_step, dtype, config, data):
 DTYPE = {"mixed": "bf16+fp32 (fp64 reductions)", "plain": "f64"}
 
 
+WATER = {"water-12288": (16, "water-12288: 4096 H2O, 12288 atoms, 49.68 A periodic cell"),
+         "water-192": (4, "water-192: 64 H2O, 192 atoms, 12.42 A periodic cell")}
+
+
 def run_water(args):
     import n2p2_amd
     from n2p2_amd import remap, synth
     mdir = synth.water_model(os.path.join(tempfile.gettempdir(), "n2p2_amd_water_model"))
-    pos, elem, box = synth.water_box(16)
+    n_side, label = WATER[args.workload]
+    pos, elem, box = synth.water_box(n_side)
     n = len(elem)
     m = n2p2_amd.Mode(mdir)
     m.set_structures(remap(pos, box), elem, box[None])
@@ -306,14 +315,13 @@ def run_water(args):
     F_form, E_form = m.forces(), m.energies()[0]
     value = n * args.steps / dt
     out = base_record(value, args.steps, args.warmup, dt / args.steps * 1e3, DTYPE[args.form],
-                      {"workload": "water-12288: 4096 H2O, 12288 atoms, 49.68 A periodic cell, rc 6.35 A, "
-                                   "50 SF/element, NN 50-25-25-1", "form": args.form, "atoms": n,
-                       "pairs": w["pairs"], "triplets": w["triplets"]},
+                      {"workload": label + ", rc 6.35 A, 50 SF/element, NN 50-25-25-1", "form": args.form,
+                       "atoms": n, "pairs": w["pairs"], "triplets": w["triplets"]},
                       "synthetic (seeded water box, random-init weights)")
     plain_fields(out, value, rate_f, rate_s, times_p)
     out["tuning"] = tuning
     out["kernel_ms_per_step"] = {k: v[0] / v[1] for k, v in times.items()}
-    out["roofline"] = roofline(times, w, NN_MACS["water"], n, "water-12288")
+    out["roofline"] = roofline(times, w, NN_MACS["water"], n, args.workload)
     if not args.no_cpu:
         cb, E_cpu, F_cpu = cpu_baseline_single(mdir, pos, elem, box, args.cpu_budget)
         out["cpu_baseline"] = cb
@@ -444,6 +452,99 @@ def run_md(args):
     return out
 
 
+def run_train(args):
+    """Loss gradient of 256 water structures (64-256 atoms) with energy and
+    force targets from a teacher network (Training's error function,
+    beta = 0.7), both forms; a step is one gradient of the whole batch."""
+    import n2p2_amd
+    from n2p2_amd import synth
+    base = tempfile.gettempdir()
+    student = synth.water_model(os.path.join(base, "n2p2_amd_train_student"))
+    teacher = synth.water_model(os.path.join(base, "n2p2_amd_train_teacher"), seed=synth.SEED_TEACHER)
+    pos, elem, boxes, off = synth.water_set()
+    n, ns, beta = len(elem), len(off) - 1, 0.7
+    t = n2p2_amd.Mode(teacher)
+    t.set_structures(pos, elem, boxes, off)
+    t.evaluate(n2p2_amd.PLAIN_FP64, forces=True)
+    E_ref, F_ref = t.energies(), t.forces()
+    del t
+    m = n2p2_amd.Mode(student)
+    m.set_structures(pos, elem, boxes, off)
+    w = m.work_counts()
+
+    def time_grad(prec, steps, warmup):
+        for _ in range(warmup):
+            m.loss_gradient(E_ref, F_ref, beta, prec)
+        m.synchronize()
+        m.set_option(n2p2_amd.OPT_TIMING, 1)
+        m.reset_kernel_times()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            loss, g = m.loss_gradient(E_ref, F_ref, beta, prec)
+        m.synchronize()
+        dt = time.perf_counter() - t0
+        times = {}
+        for k in KERNELS:
+            ms, cnt = m.kernel_time(k)
+            if cnt:
+                times[k] = (ms, cnt)
+        m.set_option(n2p2_amd.OPT_TIMING, 0)
+        return dt, times, loss, g
+    dt_p, times_p, loss_p, g_p = time_grad(n2p2_amd.PLAIN_FP64, max(3, args.steps // 5), 1)
+    tuning = tune(m)
+    form = n2p2_amd.MIXED if args.form == "mixed" else n2p2_amd.PLAIN_FP64
+    dt, times, loss, g = time_grad(form, args.steps, args.warmup)
+    value = n * args.steps / dt
+    rate_p = n * max(3, args.steps // 5) / dt_p
+    out = base_record(value, args.steps, args.warmup, dt / args.steps * 1e3, DTYPE[args.form],
+                      {"workload": "train-256: loss gradient (energy + force terms, beta 0.7) of 256 water "
+                                   "structures, 66-255 atoms (%d atoms), rc 6.35 A, 50 SF/element, NN 50-25-25-1"
+                                   % n, "form": args.form, "atoms": n, "structures": ns, "pairs": w["pairs"],
+                       "triplets": w["triplets"]},
+                      "synthetic (seeded water structures, random-init student and teacher weights)")
+    out["plain_fp64_atom_steps_per_s"] = rate_p
+    out["form_over_plain"] = value / rate_p
+    out["plain_kernel_ms_per_step"] = {k: v[0] / v[1] for k, v in times_p.items()}
+    out["gradient_max_rel_err_vs_plain"] = float(np.abs(g - g_p).max() / np.abs(g_p).max())
+    out["loss_rel_err_vs_plain"] = [float(x) for x in np.abs(loss - loss_p) / np.maximum(np.abs(loss_p), 1e-300)]
+    out["tuning"] = tuning
+    out["kernel_ms_per_step"] = {k: v[0] / v[1] for k, v in times.items()}
+    out["roofline"] = roofline(times, w, NN_MACS["water"], n, args.workload)
+    if not args.no_cpu:
+        # the oracle's Training-style gradient (calculateDEdc / calculateDFdc
+        # with the contracted dG/dx Jacobian), structure by structure until
+        # the budget is spent
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import BRUTE, PULL, Oracle
+        o = Oracle(student)
+        threads = cpu_threads()
+        atoms, tot, done = 0, 0.0, 0
+        g_cpu = None
+        for s_ in range(ns):
+            a, b = off[s_], off[s_ + 1]
+            o.set_structure(pos[a:b], elem[a:b], boxes[s_])
+            t0 = time.perf_counter()
+            o.predict(PULL, BRUTE, threads)
+            _, gs = o.loss_gradient(E_ref[s_], F_ref[a:b], 1.0 / (b - a), beta / (3.0 * n))
+            tot += time.perf_counter() - t0
+            g_cpu = gs if g_cpu is None else g_cpu + gs
+            atoms += b - a
+            done += 1
+            if tot >= args.cpu_budget:
+                break
+        host = host_cpu()
+        out["cpu_baseline"] = {"value": atoms / tot, "unit": "atom-steps/s", "cores": threads, "kind": "port",
+                               "sample": "loss gradient of the first %d of %d structures (%d atoms), %.2f s, "
+                                         "%d OpenMP threads (evaluation; the gradient pass is serial); host %s, "
+                                         "%s sockets x %s cores x %s threads"
+                                         % (done, ns, atoms, tot, threads, host["model"], host["sockets"],
+                                            host["cores_per_socket"], host["threads_per_core"]),
+                               "host": host}
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
 def run_profile(args):
     """Only the timed work of one form on one workload (after the autotuner
     for the mixed form), for kernel-trace and counter runs: no plain/mixed
@@ -492,7 +593,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=["water-12288", "batch-512", "fcc-md"], default="water-12288")
+    ap.add_argument("--workload", choices=["water-12288", "water-192", "batch-512", "fcc-md", "train-256"],
+                    default="water-12288")
     ap.add_argument("--form", choices=["mixed", "plain"], default="mixed")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -504,8 +606,8 @@ def main():
                     help="plain form forces in --profile runs: 0 fused recompute, 1 stored dG/dr")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU only: this build has no multi-GPU path; run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -516,7 +618,8 @@ def main():
     if args.profile:
         print(json.dumps(run_profile(args)))
         return
-    run = {"water-12288": run_water, "batch-512": run_batch, "fcc-md": run_md}[args.workload]
+    run = {"water-12288": run_water, "water-192": run_water, "batch-512": run_batch, "fcc-md": run_md,
+           "train-256": run_train}[args.workload]
     print(json.dumps(run(args)))
 
 
