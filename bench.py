@@ -137,15 +137,26 @@ def plain_fields(out, value, rate_f, rate_s, times_p):
     out["plain_kernel_ms_per_step"] = times_p
 
 
+# the kernel computing SF values only, paired with the fused force kernel of its form
+VALUE_PAIR = {"sf32": "force32", "sf64v": "force64f"}
+
+
 def roofline(times, w, nn_macs, natoms, workload):
     """Roofline object of the dominant kernel: algorithmic FLOPs per launch
     over its average HIP-event duration.  The fused force kernel recomputes
     every SF term with its derivative, so its algorithmic work is the SF
-    part of FLOP_EF; the NN kernel's is 6C per atom."""
+    part of FLOP_EF; the NN kernel's is 6C per atom.  FLOP_EF has no split
+    into values and derivatives, so where the SF value kernel dominates it
+    is reported together with its force kernel: the SF part over the two
+    kernels' summed durations."""
     sf_flop, nn_flop = flop_counts(w, nn_macs, natoms)
     dom = max((k for k in times if k not in ("nbr", "rev")), key=lambda k: times[k][0])
     ms, launches = times[dom]
     avg_s = ms / launches / 1e3
+    if dom in VALUE_PAIR and VALUE_PAIR[dom] in times:
+        f = VALUE_PAIR[dom]
+        avg_s += times[f][0] / times[f][1] / 1e3
+        dom = dom + "+" + f
     if dom in ("nn64", "nnbf16"):
         flop, peak, bound = nn_flop, PEAK_FP32, "mfma"
     elif dom in ("grad64", "grad32"):
@@ -154,9 +165,14 @@ def roofline(times, w, nn_macs, natoms, workload):
     else:
         flop, peak, bound = sf_flop, (PEAK_FP64 if dom.endswith("64") else PEAK_FP32), "valu"
     achieved = flop / avg_s / 1e12
+    traffic = None
+    if "+" not in dom:
+        traffic = traffic_from_profile("k_" + {"force64f": "force64f", "sf64v": "values64"}.get(dom, dom), workload)
+    else:
+        parts = [traffic_from_profile("k_" + {"sf64v": "values64"}.get(k, k), workload) for k in dom.split("+")]
+        traffic = sum(parts) if all(x is not None for x in parts) else None
     return {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": traffic_from_profile("k_" + dom, workload),
-            "flop_per_launch": flop, "avg_launch_ms": avg_s * 1e3}
+            "frac": achieved / peak, "traffic": traffic, "flop_per_launch": flop, "avg_launch_ms": avg_s * 1e3}
 
 
 def cpu_threads():

@@ -207,23 +207,71 @@ __device__ inline double scale_sf(const ScaleT& s, double v)
     }
 }
 
+// Cross-lane moves through DPP (data-parallel primitives: operand swizzles
+// inside rows of 16 lanes, no LDS traffic) and readlane, for 4- and 8-byte
+// values (8-byte values move as two 32-bit halves).  Every lane of the wave
+// must be active.
+constexpr int kDppXor1 = 0xB1;        // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4E;        // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141; // lane i <-> 7 - i within each 8
+constexpr int kDppMirror = 0x140;     // lane i <-> 15 - i within each 16
+constexpr int kDppShr = 0x110;        // row_shr:n = kDppShr + n (zero where no source)
+constexpr int kDppBcast15 = 0x142;    // lane 15 of each row to the next row
+constexpr int kDppBcast31 = 0x143;    // lane 31 to rows 2 and 3
+
+template <int CTRL, int ROWS = 0xF, bool ZERO = false, typename T>
+__device__ inline T dpp_move(T v)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+    if constexpr (sizeof(T) == 4) {
+        const int r = __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, ROWS, 0xF, ZERO);
+        return __builtin_bit_cast(T, r);
+    } else {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROWS, 0xF, ZERO);
+        const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWS, 0xF, ZERO);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
+// Value of lane l (l uniform), in a scalar register.
+template <typename T>
+__device__ inline T lane_value(T v, int l)
+{
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+    } else {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+        return __builtin_bit_cast(T, ((unsigned long long)hi << 32) | lo);
+    }
+}
+
+// Sum over the wave, received by every lane: pairwise sums inside rows of 16
+// (quad swaps, half-row and row mirrors), then the four row sums added in
+// row order.
 template <typename T>
 __device__ inline T wave_sum(T v)
 {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += dpp_move<kDppXor1>(v);
+    v += dpp_move<kDppXor2>(v);
+    v += dpp_move<kDppHalfMirror>(v);
+    v += dpp_move<kDppMirror>(v);
+    return ((lane_value(v, 0) + lane_value(v, 16)) + lane_value(v, 32)) + lane_value(v, 48);
 }
 
-// Inclusive prefix sum over the wave.
+// Inclusive prefix sum over the wave: shifts by 1, 2, 4, 8 inside each row,
+// then lane 15 of a row added to the next and lane 31 to the upper half.
 __device__ inline int wave_incl_scan(int v)
 {
-    const int lane = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        int t = __shfl_up(v, o, kWave);
-        if (lane >= o) v += t;
-    }
+    v += dpp_move<kDppShr + 1, 0xF, true>(v);
+    v += dpp_move<kDppShr + 2, 0xF, true>(v);
+    v += dpp_move<kDppShr + 4, 0xF, true>(v);
+    v += dpp_move<kDppShr + 8, 0xF, true>(v);
+    v += dpp_move<kDppBcast15, 0xA>(v);
+    v += dpp_move<kDppBcast31, 0xC>(v);
     return v;
 }
 

@@ -153,7 +153,7 @@ __device__ inline int make_list64(const Stage64& S, int n, int elem, double rc, 
         const bool ok = k < n && (elem < 0 || S.el[k] == elem) && S.g[k].w < rc;
         const int incl = wave_incl_scan(ok ? 1 : 0);
         if (ok) out[cnt + incl - 1] = (unsigned short)k;
-        cnt += __shfl(incl, kWave - 1, kWave);
+        cnt += lane_value(incl, kWave - 1);
     }
     __syncthreads();
     return cnt;

@@ -234,7 +234,7 @@ __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s,
         }
     }
     const int incl = wave_incl_scan(lane < ne ? cnt : 0);
-    const int total = __shfl(incl, kWave - 1, kWave);
+    const int total = lane_value(incl, kWave - 1);
     int next = incl - (lane < ne ? cnt : 0);  // lane e: next slot of element e
     if (lane <= ne) S.eoff[lane] = min(lane < ne ? next : total, capL);
     // pass 2: each entry to its element's next slot
@@ -245,7 +245,7 @@ __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s,
         int pos = -1;
         for (int e = 0; e < ne; ++e) {
             const unsigned long long m = __ballot(el == e);
-            const int at = __shfl(next, e, kWave);
+            const int at = lane_value(next, e);
             if (el == e)
                 pos = at + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
@@ -284,7 +284,7 @@ __device__ inline List make_list(const Stage& S, int n, int elem, float rc, bool
             const bool ok = k < hi && S.g[k].w < rc;
             const int incl = wave_incl_scan(ok ? 1 : 0);
             if (ok) out[cnt + incl - 1] = (unsigned short)k;
-            cnt += __shfl(incl, kWave - 1, kWave);
+            cnt += lane_value(incl, kWave - 1);
         }
         L = List{0, uni(cnt), false};
     }

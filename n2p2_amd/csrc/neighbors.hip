@@ -164,7 +164,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
                                 emit(s, i, slot++, j, pack_image(b0, b1, b2));
                         }
             }
-            count += __shfl(incl, kWave - 1, kWave);
+            count += lane_value(incl, kWave - 1);
         }
     } else {
         // The 27 surrounding cells are flattened into one candidate range so
@@ -250,7 +250,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
                     if (slot < slots) keys[slot] = key[u];
                     ++slot;
                 }
-            count += __shfl(inc, kWave - 1, kWave);
+            count += lane_value(inc, kWave - 1);
         }
         // Bitonic sort of the collected keys (padded to a power of two).
         const int n = min(count, slots);

@@ -58,7 +58,7 @@ __device__ inline int build_list(const Lds& L, int n, int elem, double rc, int* 
             const bool ok = k < n && (elem < 0 || L.el[k] == elem) && L.d[k] < rc;
             const int incl = wave_incl_scan(ok ? 1 : 0);
             if (ok) out[cnt + incl - 1] = k;
-            cnt += __shfl(incl, kWave - 1, kWave);
+            cnt += lane_value(incl, kWave - 1);
         }
         if (lane == 0) cnt_sh = cnt;
     }
