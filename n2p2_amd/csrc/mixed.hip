@@ -30,7 +30,8 @@ namespace {
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kNNRows = 16;  // atoms per wave in the NN kernel
+constexpr int kNNRows = 16;  // atoms per tile of the NN kernel
+constexpr int kNNWaves = 2;  // waves per tile
 
 // float -> bf16, round to nearest even
 __host__ __device__ inline uint16_t to_bf16(float f)
@@ -1083,7 +1084,7 @@ __device__ inline float act32(int a, float x, float& d)
 // LDS row-major [16][stride] FP32 tile, split into bf16 hi and lo.
 __device__ inline void a_frag(const float* tile, int stride, int k0, s4& hi, s4& lo)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
     const float* p = tile + (lane & 15) * stride + k0 + 4 * (lane >> 4);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -1097,7 +1098,7 @@ __device__ inline void a_frag(const float* tile, int stride, int k0, s4& hi, s4&
 // + 0..3, from a row-major bf16 matrix with row length ld (hi; lo at +lo_off).
 __device__ inline void b_frag(const uint16_t* __restrict__ w, int ld, int lo_off, int n0, int k0, s4& hi, s4& lo)
 {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
     const uint16_t* p = w + (size_t)(n0 + (lane & 15)) * ld + k0 + 4 * (lane >> 4);
     const uint2 h = *reinterpret_cast<const uint2*>(p);
     const uint2 l = *reinterpret_cast<const uint2*>(p + lo_off);
@@ -1115,12 +1116,18 @@ __device__ inline f4 mma(f4 c, const s4& ah, const s4& al, const s4& bh, const s
     return c;
 }
 
-__global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNMix a, const float* __restrict__ Gin,
-                                               double* __restrict__ Eatom, double* __restrict__ dEdG,
-                                               float* __restrict__ dEdGf)
+// kNNWaves waves share one tile of 16 atoms: each takes every kNNWaves-th
+// 16-column block of a layer, so the chain of dependent MFMA steps per wave
+// is that much shorter (the network kernel is latency-bound: a few waves
+// per CU).
+__global__ __launch_bounds__(kWave * kNNWaves) void k_nnbf16(const ModelT* __restrict__ M, NNMix a,
+                                                            const float* __restrict__ Gin,
+                                                            double* __restrict__ Eatom, double* __restrict__ dEdG,
+                                                            float* __restrict__ dEdGf)
 {
     extern __shared__ float sm[];
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
     int e = 0;
     while (e + 1 < a.nelem && (int)blockIdx.x >= a.blk_off[e + 1]) ++e;
     const NNT& N = M->el[e].nn;
@@ -1136,7 +1143,7 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
 
     // layer-0 input rows
     const int pin0 = a.pin[e][1];
-    for (int x = lane; x < kNNRows * pin0; x += kWave) {
+    for (int x = threadIdx.x; x < kNNRows * pin0; x += kWave * kNNWaves) {
         const int r = x / pin0, k = x % pin0;
         bufI[r * a.stride_in + k] = r < cnt ? Gin[(size_t)a.atoms[first + r] * a.ldK + k] : 0.f;
     }
@@ -1149,7 +1156,7 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
         const uint16_t* W = a.w + a.wW[e][l];
         const float* bias = a.b + a.bo[e][l];
         const float norm = N.normalize ? 1.f / N.n[l - 1] : 1.f;
-        for (int n0 = 0; n0 < po; n0 += 16) {
+        for (int n0 = 16 * wv; n0 < po; n0 += 16 * kNNWaves) {
             f4 c = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < pi; k0 += 16) {
                 s4 ah, al, bh, bl;
@@ -1174,11 +1181,11 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
         out = in == bufA ? bufB : bufA;
         sin = st;
     }
-    if (lane < cnt) Eatom[a.atoms[first + lane]] = (double)in[lane * st];
+    if (threadIdx.x < cnt) Eatom[a.atoms[first + threadIdx.x]] = (double)in[threadIdx.x * st];
     // adjoint: delta of the output layer is f'_out for the energy neuron
     float* dl = out;
     const int pl = a.pout[e][nl - 1];
-    for (int x = lane; x < kNNRows * pl; x += kWave) {
+    for (int x = threadIdx.x; x < kNNRows * pl; x += kWave * kNNWaves) {
         const int r = x / pl, n = x % pl;
         dl[r * st + n] = n == 0 ? dfd[((nl - 1) * kNNRows + r) * st] : 0.f;
     }
@@ -1188,7 +1195,7 @@ __global__ __launch_bounds__(64) void k_nnbf16(const ModelT* __restrict__ M, NNM
         const int pi = a.pin[e][l], po = a.pout[e][l];
         const uint16_t* WT = a.w + a.wT[e][l];
         const float norm = N.normalize ? 1.f / N.n[l - 1] : 1.f;
-        for (int m0 = 0; m0 < pi; m0 += 16) {
+        for (int m0 = 16 * wv; m0 < pi; m0 += 16 * kNNWaves) {
             f4 c = {0.f, 0.f, 0.f, 0.f};
             for (int k0 = 0; k0 < po; k0 += 16) {
                 s4 ah, al, bh, bl;
@@ -2253,8 +2260,8 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     const size_t lds_nn = sizeof(float) * (size_t)kNNRows * (a.stride_in + (size_t)a.stride * (2 + kMaxLayers));
     if (lds_nn > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "network too wide for the mixed NN kernel's LDS"};
     timed("nnbf16", [&] {
-        hipLaunchKernelGGL(k_nnbf16, dim3(blocks), dim3(kWave), lds_nn, stream, d_model, a, b.Gin, b.Eatom, b.dEdG,
-                           b.dEdGf);
+        hipLaunchKernelGGL(k_nnbf16, dim3(blocks), dim3(kWave * kNNWaves), lds_nn, stream, d_model, a, b.Gin, b.Eatom,
+                           b.dEdG, b.dEdGf);
     });
     timed("energy", [&] { launch_energy(b.Eatom, b.atom_off, b.nstruct, b.E, stream); });
     if (!forces) return;
