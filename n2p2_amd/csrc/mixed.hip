@@ -31,7 +31,7 @@ typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kNNRows = 16;  // atoms per tile of the NN kernel
-constexpr int kNNWaves = 2;  // waves per tile
+constexpr int kNNWaves = 4;  // waves per tile
 
 // float -> bf16, round to nearest even
 __host__ __device__ inline uint16_t to_bf16(float f)
