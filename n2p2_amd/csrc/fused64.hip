@@ -50,15 +50,31 @@ struct Stage64 {
     unsigned short* lb;
     unsigned char* el;   // neighbour element
     unsigned short* idx; // list entry of each staged slot
+    int* eoff;           // [kMaxElem + 1] first staged slot of each neighbour element
     double2* pw;         // [kPw64] power-path term weights (force) or term sums (values)
     int trip;
 };
+
+// A group's neighbour list: staged slots base .. base + n - 1 (contig: the
+// group's cutoff covers every staged neighbour of its element) or the n
+// slots listed in S.la / S.lb.
+struct List64 {
+    int base, n;
+    bool contig;
+};
+
+template <bool CONTIG>
+__device__ inline int slot64(const unsigned short* l, int base, int a)
+{
+    if constexpr (CONTIG) return base + a;
+    else return l[a];
+}
 
 constexpr int kPw64 = 32;
 
 __host__ __device__ constexpr size_t stage64_bytes(int capL, int trip)
 {
-    return (size_t)kPw64 * 16 + (size_t)capL * (32 + 32 + 24 + 2 + 2 + 1 + 2) + (size_t)trip * 4;
+    return (size_t)kPw64 * 16 + (size_t)capL * (32 + 32 + 24 + 2 + 2 + 1 + 2) + (size_t)trip * 4 + 48;
 }
 
 __device__ inline Stage64 carve64(double4* base, int capL, int trip)
@@ -76,38 +92,67 @@ __device__ inline Stage64 carve64(double4* base, int capL, int trip)
     S.lb = S.la + capL;
     S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
     S.idx = reinterpret_cast<unsigned short*>(S.el + capL);
+    S.eoff = reinterpret_cast<int*>(S.idx + capL);
     return S;
 }
 
-// Stages the atom's list entries closer than the model's largest cutoff, in
-// list order (entries beyond it contribute nothing; MD lists built with a
-// skin carry them), with each slot's list entry in idx and, for the force
-// kernel, zeroed neighbour force sums.  More such entries than capL: only
-// the first capL are staged, flag bit 4 is set (the caller repeats with the
-// full capacity) and flags[2] keeps the largest count met.
+// Stages the atom's list entries closer than the model's largest cutoff
+// (entries beyond it contribute nothing; MD lists built with a skin carry
+// them), grouped by neighbour element and in list order within an element,
+// so element e's neighbours are the slots eoff[e] .. eoff[e + 1] - 1; each
+// slot's list entry in idx and, for the force kernel, zeroed neighbour force
+// sums.  More such entries than capL: only slots below capL are staged, flag
+// bit 4 is set (the caller repeats with the full capacity) and flags[2]
+// keeps the largest count met.
 __device__ inline int stage64(const ModelT* __restrict__ M, const SysView& s, int i, int capL, const Stage64& S,
                               bool zero_sums)
 {
     const int lane = threadIdx.x;
     const StructT& St = s.st[s.sid[i]];
     const int nl = s.nb_n[i];
+    const int ne = M->nelem;
     const size_t base = (size_t)i * s.cap;
     const double rmax = M->maxCutoff;
-    int cnt = 0;
-    for (int b = 0; b < nl; b += kWave) {
-        const int k = b + lane;
-        double dr[3] = {0.0, 0.0, 0.0}, d = 0.0;
-        bool ok = k < nl;
-        if (ok) {
+    auto entry = [&](int k, double* dr, double& d) {
+        int el = -1;
+        d = 0.0;
+        dr[0] = dr[1] = dr[2] = 0.0;
+        if (k < nl) {
             entry_vector(s, i, base + k, St, dr);
             d = sqrt(dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2]);
-            ok = d < rmax;
+            if (d < rmax) el = s.elem[s.nb_j[base + k]];
         }
-        const unsigned long long m = __ballot(ok);
-        const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-        if (ok && pos < capL) {
-            const int el = s.elem[s.nb_j[base + k]];
+        return el;
+    };
+    // pass 1: in-cutoff entries per element (lane e counts element e)
+    int cnt = 0;
+    for (int b = 0; b < nl; b += kWave) {
+        double dr[3], d;
+        const int el = entry(b + lane, dr, d);
+        for (int e = 0; e < ne; ++e) {
+            const int c = __popcll(__ballot(el == e));
+            if (lane == e) cnt += c;
+        }
+    }
+    const int incl = wave_incl_scan(lane < ne ? cnt : 0);
+    const int total = lane_value(incl, kWave - 1);
+    int next = incl - (lane < ne ? cnt : 0);  // lane e: next slot of element e
+    if (lane <= ne) S.eoff[lane] = min(lane < ne ? next : total, capL);
+    // pass 2: each entry to its element's next slot
+    for (int b = 0; b < nl; b += kWave) {
+        const int k = b + lane;
+        double dr[3], d;
+        const int el = entry(k, dr, d);
+        int pos = -1;
+        for (int e = 0; e < ne; ++e) {
+            const unsigned long long m = __ballot(el == e);
+            const int at = lane_value(next, e);
+            if (el == e)
+                pos = at + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (lane == e) next += __popcll(m);
+        }
+        if (pos >= 0 && pos < capL) {
             S.g[pos] = make_double4(dr[0], dr[1], dr[2], d);
             S.c[pos] = make_double4(1.0 / d, 0.0, 0.0, M->Zel[el]);
             S.el[pos] = (unsigned char)el;
@@ -118,15 +163,14 @@ __device__ inline int stage64(const ModelT* __restrict__ M, const SysView& s, in
                 S.az[pos] = 0.0;
             }
         }
-        cnt += __popcll(m);
     }
     if (lane == 0) {
-        if (cnt > capL) atomicOr(&s.flags[0], 4);
-        if (cnt > __hip_atomic_load(&s.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-            atomicMax(&s.flags[2], cnt);
+        if (total > capL) atomicOr(&s.flags[0], 4);
+        if (total > __hip_atomic_load(&s.flags[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+            atomicMax(&s.flags[2], total);
     }
     __syncthreads();
-    return __builtin_amdgcn_readfirstlane(min(cnt, capL));
+    return __builtin_amdgcn_readfirstlane(min(total, capL));
 }
 
 // Cutoff value and derivative; the cosine cutoff through sincospi (exact
@@ -144,19 +188,29 @@ __device__ inline void cutoff_fdf64(int type, const GroupT& g, double tanhPre, d
     cutoff_fdf(type, g, tanhPre, r, fc, dfc);
 }
 
-__device__ inline int make_list64(const Stage64& S, int n, int elem, double rc, unsigned short* out)
+// Neighbours of element `elem` (all elements: elem < 0) inside the group's
+// cutoff: the element's slot range when the group's cutoff is the model's
+// largest, else the qualifying slots listed in `out`.  Ends with a barrier.
+__device__ inline List64 make_list64(const ModelT* __restrict__ M, const Stage64& S, int n, const GroupT& G, int elem,
+                                     unsigned short* out)
 {
     const int lane = threadIdx.x;
-    int cnt = 0;
-    for (int b = 0; b < n; b += kWave) {
-        const int k = b + lane;
-        const bool ok = k < n && (elem < 0 || S.el[k] == elem) && S.g[k].w < rc;
-        const int incl = wave_incl_scan(ok ? 1 : 0);
-        if (ok) out[cnt + incl - 1] = (unsigned short)k;
-        cnt += lane_value(incl, kWave - 1);
+    const int lo = __builtin_amdgcn_readfirstlane(elem < 0 ? 0 : S.eoff[elem]);
+    const int hi = __builtin_amdgcn_readfirstlane(elem < 0 ? n : S.eoff[elem + 1]);
+    List64 L{lo, hi - lo, true};
+    if (!(G.rc >= M->maxCutoff)) {
+        int cnt = 0;
+        for (int b = lo; b < hi; b += kWave) {
+            const int k = b + lane;
+            const bool ok = k < hi && S.g[k].w < G.rc;
+            const int incl = wave_incl_scan(ok ? 1 : 0);
+            if (ok) out[cnt + incl - 1] = (unsigned short)k;
+            cnt += lane_value(incl, kWave - 1);
+        }
+        L = List64{0, __builtin_amdgcn_readfirstlane(cnt), false};
     }
     __syncthreads();
-    return cnt;
+    return L;
 }
 
 // Calls work(j, k) for every unordered pair of the group's lists with
@@ -164,11 +218,13 @@ __device__ inline int make_list64(const Stage64& S, int n, int elem, double rc, 
 // c < (nA - 1) / 2, plus b = a + nA / 2 for even nA on rows a < nA / 2;
 // unlike elements: the full product), rows consecutive in the compacted
 // buffer; flush() after each buffer.
-template <typename Work, typename Flush>
-__device__ inline void for_each_triplet64(const Stage64& S, int nA, int nB, bool same, double rc2, Work&& work,
-                                          Flush&& flush)
+template <bool CONTIG, typename Work, typename Flush>
+__device__ inline void triplets64(const Stage64& S, const List64& A, const List64& B, bool same, double rc2,
+                                 Work&& work, Flush&& flush)
 {
     const int lane = threadIdx.x;
+    const int nA = A.n, nB = B.n;
+    const int bB = same ? A.base : B.base;
     int full, total, R;
     if (same) {
         R = (nA - 1) / 2;
@@ -206,7 +262,7 @@ __device__ inline void for_each_triplet64(const Stage64& S, int nA, int nB, bool
                         a = t - full;
                         b = a + nA / 2;
                     }
-                    const int j = S.la[a], k = lb[b];
+                    const int j = slot64<CONTIG>(S.la, A.base, a), k = slot64<CONTIG>(lb, bB, b);
                     const double4 gj = S.g[j], gk = S.g[k];
                     const double dx = gk.x - gj.x, dy = gk.y - gj.y, dz = gk.z - gj.z;
                     ok[f] = dx * dx + dy * dy + dz * dz < rc2;
@@ -234,6 +290,14 @@ __device__ inline void for_each_triplet64(const Stage64& S, int nA, int nB, bool
     }
 }
 
+template <typename Work, typename Flush>
+__device__ inline void for_each_triplet64(const Stage64& S, const List64& A, const List64& B, bool same, double rc2,
+                                          Work&& work, Flush&& flush)
+{
+    if (A.contig) triplets64<true>(S, A, B, same, rc2, work, flush);
+    else triplets64<false>(S, A, B, same, rc2, work, flush);
+}
+
 // NB0 > 0: the power path (angular groups whose members are
 // exp(-eta_h sum r^2) (1 + lambda_s cos)^(2^b), b < NB0 resp. NB1, rs = 0):
 // per triplet one exponential per eta and the powers by repeated squaring;
@@ -258,6 +322,7 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, c
     const size_t base = (size_t)i * s.cap;
     const double* wi = fw + (size_t)i * ldG;
     double fx = 0.0, fy = 0.0, fz = 0.0;
+    bool cut_done = false;  // the cutoffs of a uniform-cutoff element are staged
 
     for (int gi = 0; gi < E.ngrp; ++gi) {
         const GroupT& G = M->grp[E.grp0 + gi];
@@ -265,21 +330,29 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, c
         if (type != 2 && type != 12 && type != 3 && type != 9 && type != 13) continue;  // compact: k_sfcomp
         const bool radial = type == 2 || type == 12;
         const bool wt = type == 12 || type == 13;
-        for (int k = lane; k < n; k += kWave) {
-            double f = 0.0, df = 0.0;
-            const int ek = S.el[k];
-            if ((wt || ek == G.e1 || ek == G.e2) && S.g[k].w < G.rc) cutoff_fdf64(ctype, G, tanhPre, S.g[k].w, f, df);
-            S.c[k].y = f;
-            S.c[k].z = df;
+        // cutoffs of the group's neighbours (every slot once per atom when
+        // all the element's groups share one radius)
+        if (!E.ucut || !cut_done) {
+            for (int k = lane; k < n; k += kWave) {
+                double f = 0.0, df = 0.0;
+                const int ek = S.el[k];
+                if ((E.ucut || wt || ek == G.e1 || ek == G.e2) && S.g[k].w < G.rc)
+                    cutoff_fdf64(ctype, G, tanhPre, S.g[k].w, f, df);
+                S.c[k].y = f;
+                S.c[k].z = df;
+            }
+            __syncthreads();
+            cut_done = true;
         }
-        __syncthreads();
         const int nmem = G.nmem;
         const MemberT* mem = M->mem + G.mem0;
         if (radial) {
-            // lanes own neighbours: sum_q w_q sfac_q (dfc - 2 eta (r - rs) fc) exp(-eta (r - rs)^2)
-            for (int k = lane; k < n; k += kWave) {
+            // lanes own neighbours: sum_q w_q sfac_q (dfc - 2 eta (r - rs) fc) exp(-eta (r - rs)^2);
+            // type 2: the slots of element e1, type 12: every slot
+            const int lo = wt ? 0 : S.eoff[G.e1], hi = wt ? n : S.eoff[G.e1 + 1];
+            for (int k = lo + lane; k < hi; k += kWave) {
                 const double4 gk = S.g[k];
-                if ((!wt && S.el[k] != G.e1) || !(gk.w < G.rc)) continue;
+                if (!(gk.w < G.rc)) continue;
                 const double4 ck = S.c[k];
                 double P = 0.0;
                 for (int q = 0; q < nmem; ++q) {
@@ -297,8 +370,8 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, c
         }
         const bool same = G.e1 == G.e2;
         const bool wide = type == 9;
-        const int nA = make_list64(S, n, G.e1, G.rc, S.la);
-        const int nB = same ? nA : make_list64(S, n, G.e2, G.rc, S.lb);
+        const List64 A = make_list64(M, S, n, G, G.e1, S.la);
+        const List64 B = same ? A : make_list64(M, S, n, G, G.e2, S.lb);
         GroupPow64 gp{};
         if constexpr (NB0 > 0) {
             gp = GP[E.grp0 + gi];
@@ -329,7 +402,7 @@ __global__ __launch_bounds__(64) void k_force64f(const ModelT* __restrict__ M, c
                 cur = -1;
             }
         };
-        for_each_triplet64(S, nA, nB, same, wide ? INFINITY : G.rc2, [&](int j, int k) {
+        for_each_triplet64(S, A, B, same, wide ? INFINITY : G.rc2, [&](int j, int k) {
             if (j != cur) {
                 flush();
                 cur = j;
@@ -491,6 +564,7 @@ __global__ __launch_bounds__(64) void k_values64(const ModelT* __restrict__ M, c
     const double tanhPre = M->tanhPre;
     const int n = stage64(M, s, i, capL, S, false);
     double* Gi = G + (size_t)i * ldG;
+    bool cut_done = false;  // the cutoffs of a uniform-cutoff element are staged
 
     for (int gi = 0; gi < E.ngrp; ++gi) {
         const GroupT& GR = M->grp[E.grp0 + gi];
@@ -498,24 +572,29 @@ __global__ __launch_bounds__(64) void k_values64(const ModelT* __restrict__ M, c
         if (type != 2 && type != 12 && type != 3 && type != 9 && type != 13) continue;  // compact: k_sfcomp
         const bool radial = type == 2 || type == 12;
         const bool wt = type == 12 || type == 13;
-        for (int k = lane; k < n; k += kWave) {
-            double f = 0.0, df = 0.0;
-            const int ek = S.el[k];
-            if ((wt || ek == GR.e1 || ek == GR.e2) && S.g[k].w < GR.rc) cutoff_fdf(ctype, GR, tanhPre, S.g[k].w, f, df);
-            S.c[k].y = f;
+        if (!E.ucut || !cut_done) {
+            for (int k = lane; k < n; k += kWave) {
+                double f = 0.0, df = 0.0;
+                const int ek = S.el[k];
+                if ((E.ucut || wt || ek == GR.e1 || ek == GR.e2) && S.g[k].w < GR.rc)
+                    cutoff_fdf(ctype, GR, tanhPre, S.g[k].w, f, df);
+                S.c[k].y = f;
+            }
+            __syncthreads();
+            cut_done = true;
         }
-        __syncthreads();
         const int nmem = GR.nmem;
         const MemberT* mem = M->mem + GR.mem0;
         if (radial) {
             constexpr int CH = 8;
+            const int lo = wt ? 0 : S.eoff[GR.e1], hi = wt ? n : S.eoff[GR.e1 + 1];
             for (int q0 = 0; q0 < nmem; q0 += CH) {
                 double acc[CH];
 #pragma unroll
                 for (int q = 0; q < CH; ++q) acc[q] = 0.0;
-                for (int k = lane; k < n; k += kWave) {
+                for (int k = lo + lane; k < hi; k += kWave) {
                     const double r = S.g[k].w;
-                    if ((!wt && S.el[k] != GR.e1) || !(r < GR.rc)) continue;
+                    if (!(r < GR.rc)) continue;
                     const double fc = S.c[k].y;
 #pragma unroll
                     for (int q = 0; q < CH; ++q)
@@ -541,8 +620,8 @@ __global__ __launch_bounds__(64) void k_values64(const ModelT* __restrict__ M, c
         }
         const bool same = GR.e1 == GR.e2;
         const bool wide = type == 9;
-        const int nA = make_list64(S, n, GR.e1, GR.rc, S.la);
-        const int nB = same ? nA : make_list64(S, n, GR.e2, GR.rc, S.lb);
+        const List64 A = make_list64(M, S, n, GR, GR.e1, S.la);
+        const List64 B = same ? A : make_list64(M, S, n, GR, GR.e2, S.lb);
         if constexpr (NB0 > 0) {
             // power path: term sums pfc E_h (1 + lambda_s cos)^(2^b) per lane,
             // one wave reduction per used term, members take their term
@@ -552,7 +631,7 @@ __global__ __launch_bounds__(64) void k_values64(const ModelT* __restrict__ M, c
             double part[NBAS];
 #pragma unroll
             for (int b = 0; b < NBAS; ++b) part[b] = 0.0;
-            for_each_triplet64(S, nA, nB, same, wide ? INFINITY : GR.rc2, [&](int j, int k) {
+            for_each_triplet64(S, A, B, same, wide ? INFINITY : GR.rc2, [&](int j, int k) {
                 const double4 gj = S.g[j], gk = S.g[k];
                 const double4 cj = S.c[j], ck = S.c[k];
                 const double x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
@@ -593,7 +672,7 @@ __global__ __launch_bounds__(64) void k_values64(const ModelT* __restrict__ M, c
         double acc[MAXM];
 #pragma unroll
         for (int q = 0; q < MAXM; ++q) acc[q] = 0.0;
-        for_each_triplet64(S, nA, nB, same, wide ? INFINITY : GR.rc2, [&](int j, int k) {
+        for_each_triplet64(S, A, B, same, wide ? INFINITY : GR.rc2, [&](int j, int k) {
             const double4 gj = S.g[j], gk = S.g[k];
             const double4 cj = S.c[j], ck = S.c[k];
             const double x3 = gk.x - gj.x, y3 = gk.y - gj.y, z3 = gk.z - gj.z;
