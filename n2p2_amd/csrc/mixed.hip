@@ -124,6 +124,7 @@ __device__ inline GroupF group_f(const GroupT& g)
 }
 
 constexpr int kRadChunk = 8;  // radial members accumulated per pass
+constexpr int kRadBatch = 32;  // radial sums scaled per batch (fits the smallest triplet buffer)
 
 // LDS staging of one atom's neighbours (FP32 geometry) and the compacted
 // triplet buffer.
@@ -705,12 +706,13 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
         }
     });
     double* red = reinterpret_cast<double*>(S.mp);
+    double v[NBAS];
 #pragma unroll
-    for (int b = 0; b < NBAS; ++b)
-        if ((gm.used >> b) & 1) {
-            const double v = wave_sum((double)acc[b] + (double)cmp[b]);
-            if (lane == 0) red[b] = v;
-        }
+    for (int b = 0; b < NBAS; ++b) v[b] = (gm.used >> b) & 1 ? wave_sum((double)acc[b] + (double)cmp[b]) : 0.0;
+    if (lane == 0) {
+#pragma unroll
+        for (int b = 0; b < NBAS; ++b) red[b] = v[b];
+    }
     __syncthreads();
     if (lane < GT.nmem) {
         const MemberT& mt = M->mem[GT.mem0 + lane];
@@ -728,16 +730,27 @@ __device__ inline void power_weights(const Stage& S, const MemberF* __restrict__
                                      const float* __restrict__ w, float* wb, float* wa)
 {
     const int lane = threadIdx.x;
-    if (lane < NBAS) {
-        float sb = 0.f, sa = 0.f;
-        for (int q = 0; q < nmem; ++q)
-            if (mf[q].beta == lane) {
-                const float wq = w[mf[q].sf];
-                sb += wq;
-                sa += wq * mf[q].zetaLambda;
-            }
-        S.mp[lane] = make_float4(sb, sa, 0.f, 0.f);
+    // one member per lane (nmem <= kMaxMembersExp < kWave), so its constants
+    // and its weight are loaded once, all members at the same time
+    int beta = -1;
+    float wq = 0.f, wz = 0.f;
+    if (lane < nmem) {
+        const MemberF m = mf[lane];
+        beta = m.beta;
+        wq = w[m.sf];
+        wz = wq * m.zetaLambda;
     }
+    // lane b sums the weights of term b in member order
+    float sb = 0.f, sa = 0.f;
+    for (int q = 0; q < nmem; ++q) {
+        const int bq = lane_value(beta, q);
+        const float x = lane_value(wq, q), y = lane_value(wz, q);
+        if (lane == bq) {
+            sb += x;
+            sa += y;
+        }
+    }
+    if (lane < NBAS) S.mp[lane] = make_float4(sb, sa, 0.f, 0.f);
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < NBAS; ++b) {
@@ -895,6 +908,10 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             // type 12: every neighbour, each term times its atomic number
             // (SymGrpExpRadWeighted.cpp)
             const bool wt = GT.type == 12;
+            // the wave sums go through LDS (the triplet buffer, >= kRadBatch
+            // doubles) so a batch of members is scaled and stored by one lane
+            // each, not one after another by lane 0
+            double* red = reinterpret_cast<double*>(S.tjk);
             for (int q0 = 0; q0 < nmem; q0 += kRadChunk) {
                 float acc[kRadChunk], cmp[kRadChunk];
 #pragma unroll
@@ -912,19 +929,27 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                             kahan(acc[q], cmp[q], __expf(-mf[q0 + q].eta * dr * dr) * fc);
                         }
                 }
+                double v[kRadChunk];
 #pragma unroll
                 for (int q = 0; q < kRadChunk; ++q)
-                    if (q0 + q < nmem) {
-                        const double v = wave_sum((double)acc[q] + (double)cmp[q]);
-                        if (lane == 0) {
-                            const MemberT& mt = M->mem[GT.mem0 + q0 + q];
-                            const double gs = scale_sf(M->scale[E.sf0 + mt.sf], v);
-                            Gi[mt.sf] = gs;
-                            Ii[mt.sf] = (float)gs;
-                        }
+                    v[q] = q0 + q < nmem ? wave_sum((double)acc[q] + (double)cmp[q]) : 0.0;
+                if (lane == 0) {
+#pragma unroll
+                    for (int q = 0; q < kRadChunk; ++q) red[(q0 + q) % kRadBatch] = v[q];
+                }
+                const int q1 = q0 + kRadChunk;
+                if (q1 % kRadBatch == 0 || q1 >= nmem) {
+                    __syncthreads();
+                    const int qm = q0 - q0 % kRadBatch + lane;  // this batch's members
+                    if (lane < kRadBatch && qm < min(q1, nmem)) {
+                        const MemberT& mt = M->mem[GT.mem0 + qm];
+                        const double gs = scale_sf(M->scale[E.sf0 + mt.sf], red[qm % kRadBatch]);
+                        Gi[mt.sf] = gs;
+                        Ii[mt.sf] = (float)gs;
                     }
+                    __syncthreads();
+                }
             }
-            __syncthreads();
             continue;
         }
         if constexpr (NB0 > 0) {
@@ -1021,18 +1046,23 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
                 }
             });
         }
+        // wave sums through LDS, then one lane per member scales and stores
+        double* red = reinterpret_cast<double*>(S.tjk);
+        static_assert(MAXM <= 32, "member sums must fit the smallest triplet buffer");
+        double v[MAXM];
 #pragma unroll
-        for (int q = 0; q < MAXM; ++q)
-            if (q < nmem) {
-                double v = wave_sum((double)acc[q] + (double)cmp[q]);
-                if (lane == 0) {
-                    const MemberT& mt = M->mem[GT.mem0 + q];
-                    v *= mt.gnorm;
-                    const double gs = scale_sf(M->scale[E.sf0 + mt.sf], v);
-                    Gi[mt.sf] = gs;
-                    Ii[mt.sf] = (float)gs;
-                }
-            }
+        for (int q = 0; q < MAXM; ++q) v[q] = q < nmem ? wave_sum((double)acc[q] + (double)cmp[q]) : 0.0;
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < MAXM; ++q) red[q] = v[q];
+        }
+        __syncthreads();
+        if (lane < nmem) {
+            const MemberT& mt = M->mem[GT.mem0 + lane];
+            const double gs = scale_sf(M->scale[E.sf0 + mt.sf], red[lane] * mt.gnorm);
+            Gi[mt.sf] = gs;
+            Ii[mt.sf] = (float)gs;
+        }
         __syncthreads();
     }
     if constexpr (MERGE && NB0 > 0) {

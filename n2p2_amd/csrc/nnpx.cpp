@@ -241,8 +241,11 @@ hipEvent_t get_event(nnpx_ctx* c)
         c->event_pool.pop_back();
         return e;
     }
+    // timing-only markers: no system-scope fence (cache write-back and
+    // invalidate) when they complete, which delays the next dispatch; the
+    // times are read after a stream synchronization
     hipEvent_t e;
-    hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventCreateWithFlags(&e, hipEventDisableSystemFence), "hipEventCreateWithFlags");
     return e;
 }
 
