@@ -82,7 +82,7 @@ typedef enum {
                                   default), 0 off, 1 on; results agree to
                                   FP32 summation order                     */
     NNPX_OPT_ANGULAR_MERGE_FORCE = 7, /* the same for the mixed force kernel */
-    NNPX_OPT_MD_STAGING = 8     /* MD (read by nnpx_md_init): LDS slots of
+    NNPX_OPT_MD_STAGING = 8,    /* MD (read by nnpx_md_init): LDS slots of
                                   the fused kernels for the in-cutoff
                                   neighbours of an atom; -1 (default) the
                                   most met at nnpx_md_init plus an eighth
@@ -91,6 +91,15 @@ typedef enum {
                                   A run of steps that meets more is
                                   repeated from its start with the whole
                                   list staged.                             */
+    NNPX_OPT_PAIR_LISTS = 9     /* mixed form: the SF kernel keeps each
+                                  atom's angular pairs in device memory
+                                  and the force kernel reads them instead
+                                  of testing the candidates again (both
+                                  walk row x column blocks); off, the
+                                  force kernel walks rows.  -1 auto (off
+                                  until nnpx_autotune keeps the faster,
+                                  default), 0 off, 1 on; results agree to
+                                  FP32 summation order                     */
 } nnpx_option;
 
 /* Flags for nnpx_evaluate. */

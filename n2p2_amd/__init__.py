@@ -29,6 +29,7 @@ OPT_PLAIN_FORCES = 5
 OPT_ANGULAR_MERGE_SF = 6
 OPT_ANGULAR_MERGE_FORCE = 7
 OPT_MD_STAGING = 8
+OPT_PAIR_LISTS = 9
 EVAL_FORCES = 1
 
 _STATUS = {0: "OK", 1: "ARGUMENT", 2: "FILE", 3: "MODEL", 4: "RANGE", 5: "STATE", 6: "HIP", 7: "UNSUPPORTED"}
@@ -396,12 +397,14 @@ class Mode:
 
     def tuning(self):
         """Launch shapes in use: the triplet buffers, the gradient kernel's
-        atoms per block and the merged angular walk settings (-1 auto)."""
+        atoms per block, the merged angular walk and pair list settings
+        (-1 auto)."""
         v = np.zeros(4, np.int32)
         self._ck(self._lib.nnpx_get_tuning(self._h, _ptr(v)))
         out = dict(zip(("trip_sf", "trip_force", "trip_tangent", "grad_atoms"), (int(x) for x in v)))
         out["merge_sf"] = self.get_option(OPT_ANGULAR_MERGE_SF)
         out["merge_force"] = self.get_option(OPT_ANGULAR_MERGE_FORCE)
+        out["pair_lists"] = self.get_option(OPT_PAIR_LISTS)
         return out
 
     def set_tuning(self, trip_sf, trip_force, trip_tangent, grad_atoms):

@@ -133,7 +133,8 @@ struct Stage {
     float4* c;       // 1/|dr|, fc, dfc (cutoff of the current group), unused
     unsigned char* el;  // neighbour element
     unsigned short* la; unsigned short* lb;  // group lists
-    unsigned* tjk;   // compacted pair (j | k << 16); r_jk^2 is recomputed from g
+    unsigned* tjk;   // [trip + kWave] compacted pairs (j | k << 16) and up to
+                     // a wave's worth past them; r_jk^2 is recomputed from g
     float4* mp;      // [mpn] per-group constants (member constants or power-path terms)
     int* eoff;       // [kMaxElem + 1] first staged slot of each neighbour element
     int trip;        // capacity of the triplet buffer
@@ -159,12 +160,6 @@ __device__ inline int slot(const unsigned short* l, int base, int a)
     else return l[a];
 }
 
-// float4 slots of the per-group constant area: member constants (two per
-// member) on the member paths, one per term on the power path.
-__host__ __device__ constexpr int stage_mp(int nb0, int nb1)
-{
-    return nb0 > 0 ? 2 * (nb0 + nb1) : 2 * kMaxMembersExp;
-}
 
 __device__ inline Stage carve(float* base, int capL, int trip, int mpn)
 {
@@ -173,18 +168,12 @@ __device__ inline Stage carve(float* base, int capL, int trip, int mpn)
     S.g = reinterpret_cast<float4*>(base);
     S.c = S.g + capL;
     S.tjk = reinterpret_cast<unsigned*>(S.c + capL);
-    S.mp = reinterpret_cast<float4*>(S.tjk + trip);
+    S.mp = reinterpret_cast<float4*>(S.tjk + trip + kWave);
     S.la = reinterpret_cast<unsigned short*>(S.mp + mpn);
     S.lb = S.la + capL;
     S.el = reinterpret_cast<unsigned char*>(S.lb + capL);
     S.eoff = reinterpret_cast<int*>(S.el + capL);
     return S;
-}
-// bytes of the staging area (capL is a multiple of 32 and trip of 64, so the
-// byte arrays end 16-byte aligned, as does the element offset table)
-__host__ __device__ constexpr size_t stage_bytes(int capL, int trip, int mpn)
-{
-    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)mpn * 16 + 48;
 }
 
 // |b - a|^2 of two staged positions, one fixed operation order for the
@@ -450,87 +439,115 @@ __device__ inline void for_each_triplet(const Stage& S, const List& A, const Lis
     }
 }
 
-// Block variant (the SF-value kernels): each row a of list A pairs with its columns c -- for
-// one list b = (a + c + 1) mod nA with c < (nA - 1) / 2, and for even nA one
+// Runs `work` over the first cnt buffered pairs, each lane a contiguous
+// chunk, so a lane sees runs of one first atom j (j's geometry stays in
+// registers over the run and the force kernel keeps j's contribution there).
+template <typename Work>
+__device__ inline void run_buffer(const Stage& S, int cnt, Work&& work)
+{
+    const int per = (cnt + kWave - 1) / kWave;
+    const int x0 = threadIdx.x * per, x1 = min(cnt, x0 + per);
+    for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16);
+}
+
+// The angular walk: each row a of list A pairs with its columns c -- for one
+// list b = (a + c + 1) mod nA with c < (nA - 1) / 2, and for even nA one
 // more column (b = a + nA / 2) on the rows a < nA / 2, so every unordered
-// pair appears once; for two lists b = c, c < nB.  A step tests a block
-// of rows x columns (lane = row * width + column, fixed per group), so a
-// candidate's indices cost a few adds.
-// Passing pairs are compacted through LDS in block order and each lane takes
-// a contiguous chunk of the buffer, so it sees runs of one first atom j: j's
-// geometry stays in registers over the run and the force kernel keeps j's
-// contribution there too.  `flush` runs after each buffer.
+// pair appears once; for two lists b = c, c < nB.  A step tests a block of
+// rows x columns (lane = row * width + column, fixed per group), so a
+// candidate's indices cost a few adds, and the row's geometry stays in
+// registers.  Passing pairs are compacted through LDS in block order; the
+// pairs run in buffers of exactly S.trip (the last one shorter), the pairs
+// a step adds past a full buffer being carried to the next, so the buffers
+// depend only on the pair sequence.  `flush` runs after each buffer.
 // GATHER: gB holds the partners' geometry in list order (gB[b] =
 // S.g[lb[b]]), so a candidate's index and position load together.
+// src: the walk's nsrc pairs as stored by an earlier walk (same sequence,
+// same buffers), read instead of tested.  dst: the pairs are stored there
+// too while they fit in `room` entries.  Returns the walk's pair count.
 template <bool CONTIG, bool GATHER, typename Work, typename Flush = NoFlush>
-__device__ inline void for_each_triplet_blocks(const Stage& S, const float4* gB, const List& A, const List& B,
-                                               bool same, float rc2, Work&& work, Flush&& flush = Flush{})
+__device__ inline int for_each_triplet_blocks(const Stage& S, const float4* gB, const List& A, const List& B,
+                                              bool same, float rc2, const unsigned* src, int nsrc, unsigned* dst,
+                                              int room, Work&& work, Flush&& flush = Flush{})
 {
     const int lane = threadIdx.x;
     const int nA = A.n, nB = B.n;
     const int R = same ? (nA - 1) / 2 : nB;          // regular columns per row
     const int Rx = same && nA % 2 == 0 ? R + 1 : R;  // with the half-way column (rows a < nA / 2 only)
     const int half = nA / 2;
-    if (nA == 0 || Rx == 0) return;
     // a row's columns split into nearly equal blocks of at most 16, so
     // the passing partners of one row stay long runs in the buffer
     const int nblk = (Rx + 15) >> 4;
-    const int cw = (Rx + nblk - 1) / nblk, rh = kWave / cw;
+    const int cw = nblk > 0 ? (Rx + nblk - 1) / nblk : 1, rh = kWave / cw;
     const int lr = lane / cw, lc = lane - lr * cw;  // lanes with lr >= rh stay idle
+    const bool live = lr < rh;
     const unsigned short* lb = same ? S.la : S.lb;
     const int bB = same ? A.base : B.base;
     // contiguous partners are already in list order
     const float4* gp = CONTIG ? S.g + bB : gB;
-    int a0 = 0, c0 = 0;
+    // no candidates: the fill loop never runs
+    int a0 = src || nA == 0 || Rx == 0 ? nA : 0, c0 = 0;
     // the lane's row atom and its geometry, reloaded when the block moves down
     int ja = slot<CONTIG>(S.la, A.base, lr < min(nA, rh) ? lr : 0);
-    float4 ga = S.g[ja];
-    while (a0 < nA) {
-        int cnt = 0;
-        while (a0 < nA && cnt <= S.trip - kWave) {
-            const int a = a0 + lr, c = c0 + lc;
-            bool ok = false;
-            unsigned jk = 0;
-            if (lr < rh && a < nA && c < Rx && (c < R || a < half)) {
+    float4 ga = a0 < nA ? S.g[ja] : make_float4(0.f, 0.f, 0.f, 0.f);
+    int cnt = 0, done = 0;  // pairs in the buffer, pairs run so far
+    for (;;) {
+        if (src) {
+            cnt = min(S.trip, nsrc - done);
+            for (int x = lane; x < cnt; x += kWave) S.tjk[x] = src[done + x];
+        } else {
+            while (a0 < nA && cnt < S.trip) {
+                // branch-free: lanes outside the block test column 0 and fail
+                const int a = a0 + lr, c = c0 + lc;
+                const bool valid = live && a < nA && c < Rx && (c < R || a < half);
                 int b = same ? a + c + 1 : c;
-                if (same && b >= nA) b -= nA;
+                b = same && b >= nA ? b - nA : b;
+                b = valid ? b : 0;
                 const int k = slot<CONTIG>(lb, bB, b);
-                ok = dist2(ga, (GATHER || CONTIG) ? gp[b] : S.g[k]) < rc2;
-                jk = (unsigned)ja | ((unsigned)k << 16);
-            }
-            const unsigned long long m = __ballot(ok);
-            if (ok) {
-                const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                S.tjk[pos] = jk;
-            }
-            cnt += __popcll(m);
-            c0 += cw;
-            if (c0 >= Rx) {
-                c0 = 0;
-                a0 += rh;
-                const int an = a0 + lr;
-                ja = slot<CONTIG>(S.la, A.base, an < nA ? an : 0);
-                ga = S.g[ja];
+                const float d2 = dist2(ga, (GATHER || CONTIG) ? gp[b] : S.g[k]);  // in bounds for every lane
+                const bool ok = valid & (d2 < rc2);
+                const unsigned long long m = __ballot(ok);
+                // passing pairs to the next entries (at most a wave past the buffer)
+                if (ok)
+                    S.tjk[cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                        (unsigned)ja | ((unsigned)k << 16);
+                cnt += __popcll(m);
+                c0 += cw;
+                if (c0 >= Rx) {
+                    c0 = 0;
+                    a0 += rh;
+                    const int an = a0 + lr;
+                    ja = slot<CONTIG>(S.la, A.base, an < nA ? an : 0);
+                    ga = S.g[ja];
+                }
             }
         }
+        const int m = min(cnt, S.trip);
+        if (m <= 0) break;
         __syncthreads();
-        const int per = (cnt + kWave - 1) / kWave;
-        const int x0 = lane * per, x1 = min(cnt, x0 + per);
-        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16);
+        if (dst && done + m <= room)
+            for (int x = lane; x < m; x += kWave) dst[done + x] = S.tjk[x];
+        run_buffer(S, m, work);
         flush();
+        done += m;
+        __syncthreads();
+        // pairs past the buffer move to its front (disjoint ranges)
+        const int rest = cnt - m;
+        if (lane < rest) S.tjk[lane] = S.tjk[S.trip + lane];
+        cnt = rest;
         __syncthreads();
     }
+    return done;
 }
 
-// Row-major variant (the force kernel): candidates are enumerated a-major
-// from a linear index (all partners of la[a] consecutively), compacted the
-// same way, and each lane takes a contiguous chunk of the compacted buffer,
-// so it sees long runs of one first atom j: j's geometry stays in registers
-// over the run and the force kernel keeps j's contribution there too.  Its
-// partner atoms k differ more across the lanes of one step than in the block
-// variant, which keeps the force kernel's LDS atomics on k from colliding.
-// `flush` runs after each buffer.
+// Row-major walk (the force kernel without pair lists): candidates are
+// enumerated a-major from a linear index (all partners of la[a]
+// consecutively), compacted the same way, and each lane takes a contiguous
+// chunk of the compacted buffer.  Its partner atoms k differ more across the
+// lanes of one step than in the block walk, which keeps the force kernel's
+// LDS atomics on k from colliding; the block walk, whose lists the SF kernel
+// can keep, tests candidates more cheaply.  `flush` runs after each buffer.
 template <bool CONTIG, typename Work, typename Flush = NoFlush>
 __device__ inline void for_each_triplet_rows(const Stage& S, const List& A, const List& B, bool same, float rc2,
                                              Work&& work, Flush&& flush = Flush{})
@@ -562,28 +579,21 @@ __device__ inline void for_each_triplet_rows(const Stage& S, const List& A, cons
             r = t - q * R;
         }
         while (t0 < total && cnt <= S.trip - kWave) {
-            bool ok = false;
-            unsigned jk = 0;
-            if (t < total) {
-                int a, b;
-                if (t < full) {
-                    a = q;
-                    if (same) { b = a + r + 1; if (b >= nA) b -= nA; }
-                    else b = r;
-                } else {
-                    a = t - full;
-                    b = a + nA / 2;
-                }
-                const int j = slot<CONTIG>(S.la, A.base, a), k = slot<CONTIG>(lb, bB, b);
-                ok = dist2(S.g[j], S.g[k]) < rc2;
-                jk = (unsigned)j | ((unsigned)k << 16);
-            }
+            // branch-free: rows q (t < full) or the half-way column of row
+            // t - full; lanes past the end test slot 0 and fail
+            const bool valid = t < total, tail = t >= full;
+            int b = same ? q + r + 1 : r;
+            b = same && b >= nA ? b - nA : b;
+            const int a = valid ? (tail ? t - full : q) : 0;
+            b = valid ? (tail ? a + (nA >> 1) : b) : 0;
+            const int j = slot<CONTIG>(S.la, A.base, a), k = slot<CONTIG>(lb, bB, b);
+            const float d2 = dist2(S.g[j], S.g[k]);  // in bounds for every lane
+            const bool ok = valid & (d2 < rc2);
             const unsigned long long m = __ballot(ok);
-            if (ok) {
-                const int pos = cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                                    __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                S.tjk[pos] = jk;
-            }
+            if (ok)
+                S.tjk[cnt + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                    (unsigned)j | ((unsigned)k << 16);
             cnt += __popcll(m);
             t0 += kWave;
             t += kWave;
@@ -592,9 +602,7 @@ __device__ inline void for_each_triplet_rows(const Stage& S, const List& A, cons
             if (r >= R) { r -= R; ++q; }
         }
         __syncthreads();
-        const int per = (cnt + kWave - 1) / kWave;
-        const int x0 = lane * per, x1 = min(cnt, x0 + per);
-        for (int x = x0; x < x1; ++x) work(S.tjk[x] & 0xffffu, S.tjk[x] >> 16);
+        run_buffer(S, cnt, work);
         flush();
         __syncthreads();
     }
@@ -611,21 +619,69 @@ __device__ inline void gather_partners(const Stage& S, const List& A, const List
     __syncthreads();
 }
 
-// The walks for slot-range lists and for listed slots.
-template <bool GATHER, typename Work, typename Flush = NoFlush>
-__device__ inline void walk_blocks(const Stage& S, const float4* gB, const List& A, const List& B, bool same,
-                                   float rc2, Work&& work, Flush&& flush = Flush{})
-{
-    if (A.contig) for_each_triplet_blocks<true, GATHER>(S, gB, A, B, same, rc2, work, flush);
-    else for_each_triplet_blocks<false, GATHER>(S, gB, A, B, same, rc2, work, flush);
-}
-
 template <typename Work, typename Flush = NoFlush>
 __device__ inline void walk_rows(const Stage& S, const List& A, const List& B, bool same, float rc2, Work&& work,
                                  Flush&& flush = Flush{})
 {
     if (A.contig) for_each_triplet_rows<true>(S, A, B, same, rc2, work, flush);
     else for_each_triplet_rows<false>(S, A, B, same, rc2, work, flush);
+}
+
+// Per-atom cursor over the stored pair lists of its walks.
+struct TripCursor {
+    unsigned* p;   // the next walk's pairs
+    int room;      // entries left for them (0: none kept)
+    int* counts;   // this atom's counts (nullptr: no list)
+    int ld;        // walks with a count
+    int w;         // walks so far
+    int total;     // pairs of all walks so far
+    bool rows;     // force kernel: the row-major walk (no lists)
+};
+
+__device__ inline TripCursor trip_cursor(const TripList& T, int i)
+{
+    TripCursor c{nullptr, 0, nullptr, 0, 0, 0, T.rows};
+    if (T.list) {
+        c.p = T.list + (size_t)i * T.cap;
+        c.room = T.cap;
+        c.counts = T.counts + (size_t)i * T.ld;
+        c.ld = T.ld;
+    }
+    return c;
+}
+
+// The walk for slot-range lists and for listed slots.  Storing side (the SF
+// kernel, store): the pairs go to the cursor's list while they fit and the
+// walk's count is recorded.  Reading side (the force kernel, read): a walk
+// with a recorded count reads its pairs back instead of testing candidates.
+template <bool GATHER, typename Work, typename Flush = NoFlush>
+__device__ inline void walk_blocks(const Stage& S, const float4* gB, const List& A, const List& B, bool same,
+                                   float rc2, TripCursor& tc, bool store, Work&& work, Flush&& flush = Flush{})
+{
+    const int w = tc.w++;
+    const bool listed = tc.counts && w < tc.ld;
+    const unsigned* src = nullptr;
+    int nsrc = 0;
+    if (listed && !store) {
+        nsrc = uni(tc.counts[w]);
+        if (nsrc >= 0) {
+            src = tc.p;
+            tc.p += nsrc;
+        }
+    }
+    unsigned* dst = listed && store ? tc.p : nullptr;
+    const int room = dst ? tc.room : 0;
+    const int n = A.contig ? for_each_triplet_blocks<true, GATHER>(S, gB, A, B, same, rc2, src, nsrc, dst, room, work, flush)
+                           : for_each_triplet_blocks<false, GATHER>(S, gB, A, B, same, rc2, src, nsrc, dst, room, work, flush);
+    tc.total += n;
+    if (dst) {
+        const bool kept = n <= room;
+        if (threadIdx.x == 0) tc.counts[w] = kept ? n : -1;
+        if (kept) {
+            tc.p += n;
+            tc.room -= n;
+        }
+    }
 }
 
 template <typename Work, typename Flush = NoFlush>
@@ -651,7 +707,7 @@ template <int NB0, int NB1, bool GATHER>
 __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stage& S, float4* gB,
                                         const MemberF* __restrict__ mf, const GroupMixF& gm, const GroupT& GT,
                                         const GroupF& g, int n, int ctype, float tanhPre, int sf0,
-                                        double* __restrict__ Gi, float* __restrict__ Ii)
+                                        double* __restrict__ Gi, float* __restrict__ Ii, TripCursor& tc)
 {
     constexpr int NBAS = 2 * (NB0 + NB1);
     constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
@@ -670,7 +726,7 @@ __device__ inline void sf32_power_group(const ModelT* __restrict__ M, const Stag
     const float lam0 = gm.lam[0], lam1 = gm.lam[1];
     int jc = -1;  // j of the cached geometry (runs of one j per lane)
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    walk_blocks<GATHER>(S, gB, A, B, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    walk_blocks<GATHER>(S, gB, A, B, same, wide ? INFINITY : g.rc2, tc, true, [&](int j, int k) {
         if (j != jc) {
             gj = S.g[j];
             cj = S.c[j];
@@ -776,7 +832,8 @@ template <int NB0, int NB1, bool GATHER>
 __device__ inline void sf32_power_merged(const ModelT* __restrict__ M, const Stage& S, float4* gB,
                                          const MemberF* __restrict__ MF, const GroupMixF* __restrict__ GM,
                                          const AngMergeF& am, const ElementT& E, int n, int ctype, float tanhPre,
-                                         char* mbase, double* __restrict__ Gi, float* __restrict__ Ii)
+                                         char* mbase, double* __restrict__ Gi, float* __restrict__ Ii,
+                                         TripCursor& tc)
 {
     constexpr int NBAS = 2 * (NB0 + NB1);
     constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
@@ -811,7 +868,7 @@ __device__ inline void sf32_power_merged(const ModelT* __restrict__ M, const Sta
     const float lam0 = gm.lam[0], lam1 = gm.lam[1];
     int jc = -1, rowj = 0;
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    walk_blocks<GATHER>(S, gB, A, A, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    walk_blocks<GATHER>(S, gB, A, A, true, wide ? INFINITY : g.rc2, tc, true, [&](int j, int k) {
         if (j != jc) {
             gj = S.g[j];
             cj = S.c[j];
@@ -873,7 +930,7 @@ template <int MAXM, bool FAST, int NB0, int NB1, bool GATHER, bool MERGE>
 __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                              const GroupMixF* __restrict__ GM, const AngMergeF* __restrict__ AM,
                                              SysView s, double* __restrict__ G, float* __restrict__ Gin, int ldG,
-                                             int ldK, int capL, int trip)
+                                             int ldK, int capL, int trip, TripList T)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
@@ -886,6 +943,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
     const int n = stage_atom(M, s, i, capL, S);
+    TripCursor tc = trip_cursor(T, i);
     double* Gi = G + (size_t)i * ldG;
     float* Ii = Gin + (size_t)i * ldK;
     for (int q = E.nsf + lane; q < ldK; q += kWave) Ii[q] = 0.f;
@@ -953,7 +1011,8 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             continue;
         }
         if constexpr (NB0 > 0) {
-            sf32_power_group<NB0, NB1, GATHER>(M, S, gB, mf, GM[E.grp0 + gi], GT, g, n, ctype, tanhPre, E.sf0, Gi, Ii);
+            sf32_power_group<NB0, NB1, GATHER>(M, S, gB, mf, GM[E.grp0 + gi], GT, g, n, ctype, tanhPre, E.sf0, Gi, Ii,
+                                               tc);
             continue;
         }
         float acc[MAXM], cmp[MAXM];
@@ -984,7 +1043,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
             }
             int jc = -1;  // j of the cached geometry (runs of one j per lane)
             float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-            walk_blocks<GATHER>(S, gB, A, B, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+            walk_blocks<GATHER>(S, gB, A, B, same, wide ? INFINITY : g.rc2, tc, true, [&](int j, int k) {
                 if (j != jc) {
                     gj = S.g[j];
                     cj = S.c[j];
@@ -1066,8 +1125,13 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
         __syncthreads();
     }
     if constexpr (MERGE && NB0 > 0) {
-        if (merged) sf32_power_merged<NB0, NB1, GATHER>(M, S, gB, MF, GM, AM[e], E, n, ctype, tanhPre, mbase, Gi, Ii);
+        if (merged)
+            sf32_power_merged<NB0, NB1, GATHER>(M, S, gB, MF, GM, AM[e], E, n, ctype, tanhPre, mbase, Gi, Ii, tc);
     }
+    // the largest pair count per atom sizes the lists
+    if (T.count && lane == 0 &&
+        tc.total > __hip_atomic_load(&s.flags[3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&s.flags[3], tc.total);
 }
 
 // ------------------------------------------------------------ BF16 MFMA NN --
@@ -1282,7 +1346,7 @@ __device__ inline void force32_power_merged(const ModelT* __restrict__ M, const 
                                             const MemberF* __restrict__ MF, const GroupMixF* __restrict__ GM,
                                             const AngMergeF& am, int n, int ctype, float tanhPre, char* mbase,
                                             const float* __restrict__ wi, double* ax, double* ay, double* az,
-                                            double& fx, double& fy, double& fz)
+                                            double& fx, double& fy, double& fz, TripCursor& tc)
 {
     constexpr int NBAS = 2 * (NB0 + NB1);
     constexpr int NBM = NB0 > NB1 ? NB0 : NB1;
@@ -1325,7 +1389,7 @@ __device__ inline void force32_power_merged(const ModelT* __restrict__ M, const 
         cur = -1;
     };
     float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;
-    walk_rows(S, A, A, true, wide ? INFINITY : g.rc2, [&](int j, int k) {
+    auto work = [&](int j, int k) {
         if (j != cur) {
             flush();
             cur = j;
@@ -1386,7 +1450,9 @@ __device__ inline void force32_power_merged(const ModelT* __restrict__ M, const 
         const float P2 = pfc * (A * (rinvijik - cost * ck.x * ck.x) - C0) + B * pr2;
         const float P3 = pfc * (A * rinvijik + C3) - B * pr3;
         force_terms(P1, P2, P3, gj, gk, sx, sy, sz, rx, ry, rz, ax + k, ay + k, az + k);
-    }, flush);
+    };
+    if (tc.rows) walk_rows(S, A, A, true, wide ? INFINITY : g.rc2, work, flush);
+    else walk_blocks<false>(S, nullptr, A, A, true, wide ? INFINITY : g.rc2, tc, false, work, flush);
     flush();
     __syncthreads();
 }
@@ -1399,7 +1465,7 @@ template <int MAXM, bool FAST, int NB0, int NB1, bool MERGE>
 __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                                 const GroupMixF* __restrict__ GM, const AngMergeF* __restrict__ AM,
                                                 SysView s, const float* __restrict__ fw, int ldG,
-                                                double* __restrict__ F, int push, int capL, int trip)
+                                                double* __restrict__ F, int push, int capL, int trip, TripList T)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
@@ -1415,6 +1481,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
     const int n = stage_atom(M, s, i, capL, S, idx);
+    TripCursor tc = trip_cursor(T, i);
     for (int k = lane; k < n; k += kWave) { ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0; }
     const float* wi = fw + (size_t)i * ldG;
     double fx = 0.0, fy = 0.0, fz = 0.0;
@@ -1507,7 +1574,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             }
         };
         float4 gj = make_float4(0.f, 0.f, 0.f, 0.f), cj = gj;  // geometry of the run's j
-        walk_rows(S, A, B, same, wide ? INFINITY : g.rc2, [&](int j, int k) {
+        auto work = [&](int j, int k) {
             if (j != cur) {
                 flush();
                 cur = j;
@@ -1630,13 +1697,16 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
             const float P2 = zw * (pfc * (A * (rinvijik - cost * ck.x * ck.x) - C2) + B * pr2);
             const float P3 = zw * (pfc * (A * rinvijik + C3) - B * pr3);
             force_terms(P1, P2, P3, gj, gk, sx, sy, sz, rx, ry, rz, ax + k, ay + k, az + k);
-        });
+        };
+        if (tc.rows) walk_rows(S, A, B, same, wide ? INFINITY : g.rc2, work);
+        else walk_blocks<false>(S, nullptr, A, B, same, wide ? INFINITY : g.rc2, tc, false, work);
         flush();
         __syncthreads();
     }
     if constexpr (MERGE && NB0 > 0) {
         if (merged)
-            force32_power_merged<NB0, NB1>(M, S, MF, GM, AM[e], n, ctype, tanhPre, mbase, wi, ax, ay, az, fx, fy, fz);
+            force32_power_merged<NB0, NB1>(M, S, MF, GM, AM[e], n, ctype, tanhPre, mbase, wi, ax, ay, az, fx, fy, fz,
+                                           tc);
     }
     fx = wave_sum(fx);
     fy = wave_sum(fy);
@@ -2113,6 +2183,17 @@ void setup_mixed(MixedModel& mm, const HostModel& hm)
         ++merged_elements;
     }
     if (merged_elements) mm.merge_avg /= merged_elements;
+    // angular walks per element (an upper bound: every non-radial group, and
+    // the merged walk)
+    mm.max_walks = 0;
+    for (int e = 0; e < t.nelem; ++e) {
+        int nw = 1;
+        for (int gi = 0; gi < t.el[e].ngrp; ++gi) {
+            const int ty = t.grp[t.el[e].grp0 + gi].type;
+            if (ty != 2 && ty != 12) ++nw;
+        }
+        mm.max_walks = std::max(mm.max_walks, nw);
+    }
     // bf16 weights, padded to 16: W [out][in] and WT [in][out], hi then lo.
     std::vector<uint16_t> w;
     std::vector<float> b;
@@ -2225,6 +2306,17 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     if (lds_f_max > 65536 || lds_sf_max > 65536)
         throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
     const bool merge_sf = use_merge(mm, mm.tune.merge_sf), merge_f = use_merge(mm, mm.tune.merge_force);
+    // with pair lists on, the angular pairs pass from the SF to the force
+    // kernel when both make the same walks (the SF kernel also counts them,
+    // which sizes the lists of later evaluations); off, the force kernel
+    // walks rows
+    const bool lists = mm.tune.pair_lists == 1;
+    TripList T{};
+    if (forces && lists && merge_sf == merge_f) {
+        T = b.trips;
+        T.count = true;
+    }
+    T.rows = !lists;
     timed("sf32", [&] {
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
             constexpr int cm = decltype(m)::value, c0 = decltype(nb0)::value, c1 = decltype(nb1)::value;
@@ -2241,7 +2333,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                 constexpr bool cg = decltype(gather)::value, cmm = decltype(merge)::value;
                 hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, cg, cmm>), dim3(s.natoms), dim3(kWave), cg ? lds1 : lds0,
                                    stream, d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.G, b.Gin, b.ldG, mm.ldK,
-                                   capL, tsf);
+                                   capL, tsf, T);
             };
             auto pick = [&](auto merge) {
                 constexpr bool cmm = decltype(merge)::value;
@@ -2309,10 +2401,10 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
             if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
             if (mg)
                 hipLaunchKernelGGL((k_force32<cm, cf, c0, c1, (c0 > 0)>), dim3(s.natoms), dim3(kWave), lds_f, stream,
-                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf);
+                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf, T);
             else
                 hipLaunchKernelGGL((k_force32<cm, cf, c0, c1, false>), dim3(s.natoms), dim3(kWave), lds_f, stream,
-                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf);
+                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf, T);
         });
         launch_sfcomp32(d_model, t, s, nullptr, nullptr, b.ldG, mm.ldK, b.dEdGf, b.F, push, true, stream);
     });

@@ -56,6 +56,39 @@ struct GroupPow64 {
     int pad;
 };
 
+// LDS staging of the mixed kernels: float4 slots of the per-group constant
+// area (member constants, two per member, on the member paths; one per term
+// on the power path) ...
+__host__ __device__ constexpr int stage_mp(int nb0, int nb1)
+{
+    return nb0 > 0 ? 2 * (nb0 + nb1) : 2 * kMaxMembersExp;
+}
+// ... and bytes of the whole staging area: per slot geometry, cutoffs, two
+// list entries and the element; the triplet buffer (trip entries and up to a
+// wave's worth carried past them); the constants; the element offsets (capL
+// is a multiple of 32 and trip of 64, so the byte arrays end 16-byte
+// aligned, as does the offset table)
+__host__ __device__ constexpr size_t stage_bytes(int capL, int trip, int mpn)
+{
+    return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)(trip + kWave) * 4 + (size_t)mpn * 16 + 48;
+}
+
+// Each atom's compacted angular pairs in device memory, walk by walk (the
+// walks of its element in group order, a merged walk counting as one): the
+// SF kernel stores them as it tests the candidates, the force kernel reads
+// them back instead of testing again.  A walk whose pairs do not fit keeps
+// count -1 and the force kernel tests its candidates itself, which visits
+// the same pairs in the same buffers.  Without lists the force kernel may
+// instead walk rows, whose order suits some models better (the tuning
+// decides).
+struct TripList {
+    unsigned* list = nullptr;  // [natoms][cap] pairs j | k << 16 (nullptr: none kept)
+    int* counts = nullptr;     // [natoms][ld] pairs of each walk, -1: not kept
+    int cap = 0, ld = 0;
+    bool count = false;        // record the largest per-atom pair count in flags[3]
+    bool rows = false;         // the force kernel walks rows (no lists kept)
+};
+
 // Angular groups of one element walked together (power path): the groups
 // share type, cutoff and term constants and differ only in their neighbour
 // element pair and members, so one triplet walk over all in-cutoff
@@ -79,6 +112,7 @@ struct Tuning {
     int sf_gather = -1;      // SF kernel partner gather: -1 auto, 0 off, 1 on
     int merge_sf = -1;       // merged angular walk in the SF kernel: -1 auto, 0 off, 1 on
     int merge_force = -1;    // ... in the force kernel
+    int pair_lists = -1;     // angular pair lists from the SF to the force kernel: -1 auto (off until tuned), 0 off, 1 on
 };
 
 struct MixedModel {
@@ -95,6 +129,7 @@ struct MixedModel {
     int powNB[2] = {0, 0};      // power path shape (NB0, NB1); {0, 0}: not used
     AngMergeF* d_amerge = nullptr;  // [nelem] merged angular walks (power path only)
     int merge_ng = 0;           // most merged groups of an element (0: no element merges)
+    int max_walks = 0;          // most angular walks of an element (unmerged)
     double merge_avg = 0.0;     // mean angular groups per merged element
     float* d_sffac = nullptr;   // [nelem*kMaxSF] force weight factor per SF
     uint16_t* d_w = nullptr;    // bf16 weights: per element/layer W hi, W lo, WT hi, WT lo
@@ -122,6 +157,7 @@ struct MixedBuffers {
     int elem_off[kMaxElem + 1];
     int nstruct;
     int cap;
+    TripList trips;    // the angular pair lists shared by the SF and force kernels
 };
 
 using TimedFn = std::function<void(const char*, const std::function<void()>&)>;

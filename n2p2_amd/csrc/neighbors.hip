@@ -329,7 +329,7 @@ void fill_int(int* p, int v, size_t n, hipStream_t stream, const int* gate)
 void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream, bool clear_flags)
 {
     if (s.natoms == 0) return;
-    if (clear_flags) fill_int(s.flags, 0, 3, stream, s.gate);
+    if (clear_flags) fill_int(s.flags, 0, 4, stream, s.gate);
     if (any_cell) {
         fill_int(s.cell_count, 0, (size_t)s.ncells, stream, s.gate);
         const int nb = (s.natoms + 255) / 256;

@@ -26,6 +26,8 @@ using namespace nnpx;
 // hipGetSymbolAddress returns for the current device (scratch_address).
 // kScratchBytes is the most device memory one such call may use.
 constexpr size_t kScratchBytes = (size_t)64 << 20;
+// device memory the mixed form's angular pair lists may take
+constexpr size_t kTripBudget = (size_t)4 << 30;
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
@@ -96,6 +98,7 @@ struct nnpx_ctx {
     // maximum count (0: the list capacity); a larger count at run time is
     // flagged (bit 4) and the evaluation repeated at full capacity
     int stage_cap = 0;
+    int trip_cap = 0;            // entries per atom of the mixed form's angular pair lists (0: none)
     int md_stage_cap = 0;        // MD: staging fitted to the in-cutoff entries (0: the list capacity)
     int md_stage_req = -1;       // NNPX_OPT_MD_STAGING
     bool nbr_valid = false;
@@ -457,11 +460,24 @@ void alloc_neighbors(nnpx_ctx* c)
     buf<double>(c, "cell_pos", (size_t)3 * c->N);
 }
 
-void read_flags(nnpx_ctx* c, int* f)
+// The four device flags: overflow bits, most list entries, most staged
+// entries, most angular pairs per atom.
+void read_flags(nnpx_ctx* c, int (&f)[4])
 {
-    hip_check(hipMemcpyAsync(f, existing<int>(c, "flags"), 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
+    hip_check(hipMemcpyAsync(f, existing<int>(c, "flags"), 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream),
               "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+}
+
+// Sizes the angular pair lists for the most pairs per atom of the last
+// evaluation (an eighth and 64 entries of headroom) within kTripBudget.  Not
+// while an MD run holds captured kernels (their list addresses stay) nor in
+// the no-allocation scratch.
+void fit_trips(nnpx_ctx* c, int most)
+{
+    if (c->md_ready || c->arena || most <= c->trip_cap) return;
+    const int cap = ((most + most / 8 + 64 + 63) / 64) * 64;
+    if ((size_t)c->N * cap * sizeof(unsigned) <= kTripBudget) c->trip_cap = cap;
 }
 
 // Builds the list; grows the capacity and rebuilds when an atom overflowed.
@@ -477,7 +493,7 @@ void build_neighbors(nnpx_ctx* c, bool check)
         timed(c, "nbr", [&] { launch_neighbors(s, list_cutoff(c), c->any_cell, c->stream); });
         c->rev_valid = false;
         if (!check) break;
-        int f[2];
+        int f[4];
         read_flags(c, f);
         c->max_nb = f[1];
         c->stage_cap = ((f[1] + 31) / 32) * 32;
@@ -658,8 +674,9 @@ void evaluate(nnpx_ctx* c, int precision, unsigned flags, bool check, bool store
 // was checked; an overflow re-runs the evaluation with a larger capacity.
 void settle(nnpx_ctx* c)
 {
-    int f[2];
+    int f[4];
     read_flags(c, f);
+    fit_trips(c, f[3]);
     if (f[0] & 2) fail(NNPX_ERR_HIP, "neighbour list is not symmetric");
     if (f[0] & 1) {
         c->cap = ((f[1] + f[1] / 4 + 31) / 32) * 32;
@@ -773,6 +790,13 @@ MixedBuffers mixed_buffers(nnpx_ctx* c)
     for (int e = 0; e <= t.nelem; ++e) m.elem_off[e] = c->elem_off[e];
     m.nstruct = c->S;
     m.cap = c->cap;
+    const size_t tl = (size_t)c->N * c->trip_cap;
+    if (c->trip_cap > 0 && c->mixed.tune.pair_lists == 1 && !c->arena && tl * sizeof(unsigned) <= kTripBudget) {
+        m.trips.cap = c->trip_cap;
+        m.trips.ld = c->mixed.max_walks;
+        m.trips.list = buf<unsigned>(c, "trips", tl);
+        m.trips.counts = buf<int>(c, "trip_n", (size_t)c->N * m.trips.ld);
+    }
     return m;
 }
 
@@ -870,6 +894,12 @@ nnpx_status nnpx_set_option(nnpx_ctx* c, int option, int64_t value)
             if (value < -1 || value > 65535) fail(NNPX_ERR_ARGUMENT, "MD staging must be -1, 0 or a slot count");
             c->md_stage_req = (int)value;
             break;
+        case NNPX_OPT_PAIR_LISTS:
+            if (value < -1 || value > 1) fail(NNPX_ERR_ARGUMENT, "pair lists must be -1, 0 or 1");
+            hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+            c->mixed.tune.pair_lists = (int)value;
+            drop_graph(c);
+            break;
         case NNPX_OPT_ANGULAR_MERGE_SF:
         case NNPX_OPT_ANGULAR_MERGE_FORCE:
             if (value < -1 || value > 1) fail(NNPX_ERR_ARGUMENT, "angular merge must be -1, 0 or 1");
@@ -897,6 +927,7 @@ nnpx_status nnpx_get_option(const nnpx_ctx* cc, int option, int64_t* value)
         case NNPX_OPT_ANGULAR_MERGE_SF: *value = c->mixed.tune.merge_sf; break;
         case NNPX_OPT_ANGULAR_MERGE_FORCE: *value = c->mixed.tune.merge_force; break;
         case NNPX_OPT_MD_STAGING: *value = c->md_stage_req; break;
+        case NNPX_OPT_PAIR_LISTS: *value = c->mixed.tune.pair_lists; break;
         default: fail(NNPX_ERR_ARGUMENT, "unknown option");
         }
     });
@@ -1953,33 +1984,56 @@ nnpx_status nnpx_autotune(nnpx_ctx* c)
         const int capL = c->cap;
         const ModelT& t = c->hm.t;
         auto fits = [&](int trip, size_t extra) {
-            return (size_t)capL * (32 + 2 + 2 + 1) + (size_t)trip * 4 + (size_t)2 * kMaxMembersExp * 16 + 48 + extra <=
-                   65536;
+            return stage_bytes(capL, trip, stage_mp(0, 0)) + extra <= 65536;
         };
         c->timing = true;
-        double best_sf = 1e30, best_f = 1e30, best_t = 1e30;
+        double best_t = 1e30;
         Tuning best = keep;
         const int ldG = t.max_sf;
         double* u = buf<double>(c, "t_u", (size_t)3 * c->N);
         double* Gdot = buf<double>(c, "t_Gdot", (size_t)c->N * ldG);
         hip_check(hipMemsetAsync(u, 0, sizeof(double) * 3 * c->N, c->stream), "hipMemsetAsync");
         try {
-            for (int trip : {128, 256, 384, 512, 768, 1024}) {
-                if (!fits(trip, (sizeof(double) * 3 + 2) * (size_t)capL + kMaxElem * kMaxElem + (size_t)kMaxMerge * 20 * 8))
-                continue;
-                tu.trip_sf = tu.trip_force = tu.trip_tangent = trip;
-                evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, true);  // warm-up
-                settle(c);
-                c->ktime.clear();
-                for (int r = 0; r < 3; ++r) evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, false);
-                SysView s = sysview(c);
-                for (int r = 0; r < 3; ++r)
-                    timed(c, "tangent32", [&] { launch_tangent32(c->mixed, t, c->d_model, s, u, Gdot, ldG, c->stream); });
-                const double a = mean_time(c, "sf32"), b = mean_time(c, "force32"), d = mean_time(c, "tangent32");
-                if (a < best_sf) { best_sf = a; best.trip_sf = trip; }
-                if (b < best_f) { best_f = b; best.trip_force = trip; }
-                if (d < best_t) { best_t = d; best.trip_tangent = trip; }
+            // triplet buffers per kernel, with and without the pair lists
+            // (the force kernel walking blocks and reading the SF kernel's
+            // pairs, or walking rows): the mode of the faster pair of kernels
+            double mode_time[2] = {1e30, 1e30};
+            Tuning mode_best[2] = {keep, keep};
+            for (int pl = 0; pl < 2; ++pl) {
+                double bsf = 1e30, bf = 1e30;
+                Tuning& mb = mode_best[pl];
+                mb.pair_lists = pl;
+                for (int trip : {128, 256, 384, 512, 768, 1024}) {
+                    if (!fits(trip, (sizeof(double) * 3 + 2) * (size_t)capL + kMaxElem * kMaxElem +
+                                        (size_t)kMaxMerge * 20 * 8))
+                        continue;
+                    tu.trip_sf = tu.trip_force = tu.trip_tangent = trip;
+                    tu.pair_lists = pl;
+                    // warm-up; its settle() sizes the lists, the second run fills them
+                    for (int r = 0; r < 2; ++r) {
+                        evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, true);
+                        settle(c);
+                    }
+                    c->ktime.clear();
+                    for (int r = 0; r < 3; ++r) evaluate(c, NNPX_MIXED, NNPX_EVAL_FORCES, false);
+                    const double a = mean_time(c, "sf32"), b = mean_time(c, "force32");
+                    if (a < bsf) { bsf = a; mb.trip_sf = trip; }
+                    if (b < bf) { bf = b; mb.trip_force = trip; }
+                    if (pl == 0) {
+                        SysView s = sysview(c);
+                        for (int r = 0; r < 3; ++r)
+                            timed(c, "tangent32",
+                                  [&] { launch_tangent32(c->mixed, t, c->d_model, s, u, Gdot, ldG, c->stream); });
+                        const double d = mean_time(c, "tangent32");
+                        if (d < best_t) { best_t = d; best.trip_tangent = trip; }
+                    }
+                }
+                mode_time[pl] = bsf + bf;
             }
+            const int pl = mode_time[1] < mode_time[0] ? 1 : 0;
+            best.trip_sf = mode_best[pl].trip_sf;
+            best.trip_force = mode_best[pl].trip_force;
+            best.pair_lists = pl;
             tu = best;
             // merged angular walks (models that have them), per kernel
             if (c->mixed.merge_ng > 0) {
@@ -1993,8 +2047,9 @@ nnpx_status nnpx_autotune(nnpx_ctx* c)
                     t_sf[mg] = mean_time(c, "sf32");
                     t_f[mg] = mean_time(c, "force32");
                 }
-                tu.merge_sf = t_sf[1] < t_sf[0] ? 1 : 0;
-                tu.merge_force = t_f[1] < t_f[0] ? 1 : 0;
+                // one choice for both kernels: the force kernel reads the SF
+                // kernel's pair lists only when both make the same walks
+                tu.merge_sf = tu.merge_force = t_sf[1] + t_f[1] < t_sf[0] + t_f[0] ? 1 : 0;
             }
             // atoms per block of the gradient kernel (FP32 network pass on the
             // current G with a zero tangent)
@@ -2206,6 +2261,8 @@ nnpx_status nnpx_md_init(nnpx_ctx* c, int precision, const double* masses, const
             const int fitted = ((m + 31) / 32) * 32;
             c->md_stage_cap = m > 0 && fitted < c->cap ? fitted : 0;
         }
+        // the pair lists at the size settle() fitted, before any step is captured
+        if (precision == NNPX_MIXED) (void)mixed_buffers(c);
         md_reset(gate, counters, ekin, c->stream);
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
         c->md_ready = true;
@@ -2324,7 +2381,7 @@ void md_steps(nnpx_ctx* c, int nsteps, int graph_steps)
         }
         c->timing = timing;
         hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-        int f[2];
+        int f[4];
         read_flags(c, f);
         if (f[0] & 1) fail(NNPX_ERR_RANGE, "neighbour capacity exceeded during MD (raise the skin or re-initialise)");
         c->sf_valid = c->nn_valid = c->dedg_valid = c->e_valid = c->f_valid = true;

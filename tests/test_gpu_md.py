@@ -29,10 +29,11 @@ def min_image(d, box):
     return d - L * np.round(d / L)
 
 
-def setup_md(d, n_cells, precision, skin, temperature=300.0, dt=1.0, seed=synth.SEED_VELOCITY):
+def setup_md(d, n_cells, precision, skin, temperature=300.0, dt=1.0, seed=synth.SEED_VELOCITY, pair_lists=-1):
     pos, elem, box = synth.fcc_lattice(n_cells=n_cells)
     m = n2p2_amd.Mode(d)
     m.set_structures(remap(pos, box), elem, box[None])
+    m.set_option(n2p2_amd.OPT_PAIR_LISTS, pair_lists)
     mass = masses_fcc()
     vel = synth.maxwell_velocities(mass[elem], temperature, seed)
     m.md_init(mass, vel, dt, skin, precision)
@@ -67,13 +68,15 @@ def test_md_plain_matches_host_verlet(fcc_dir):
     assert dx <= 1e-10 and dv <= 1e-12 and dF <= 1e-8
 
 
-@pytest.mark.parametrize("precision", [n2p2_amd.MIXED, n2p2_amd.PLAIN_FP64])
-def test_md_graph_replays_equal_eager(fcc_dir, precision):
+@pytest.mark.parametrize("precision,pair_lists", [(n2p2_amd.MIXED, 0), (n2p2_amd.MIXED, 1), (n2p2_amd.PLAIN_FP64, -1)])
+def test_md_graph_replays_equal_eager(fcc_dir, precision, pair_lists):
     """Graph-captured steps (5 per graph, four replays with host reads and
-    eager work between them) against the same steps run eagerly."""
-    say("case: MD graph vs eager, FCC 500, precision", precision)
-    a = setup_md(fcc_dir, 5, precision, skin=0.4, temperature=600.0)[0]
-    b = setup_md(fcc_dir, 5, precision, skin=0.4, temperature=600.0)[0]
+    eager work between them) against the same steps run eagerly; the mixed
+    form with and without the angular pair lists (sized at nnpx_md_init,
+    read by the captured force kernels)."""
+    say("case: MD graph vs eager, FCC 500, precision", precision, "pair lists", pair_lists)
+    a = setup_md(fcc_dir, 5, precision, skin=0.4, temperature=600.0, pair_lists=pair_lists)[0]
+    b = setup_md(fcc_dir, 5, precision, skin=0.4, temperature=600.0, pair_lists=pair_lists)[0]
     for rep in range(4):
         a.md_run(5)
         b.md_run(5, graph_steps=5)

@@ -500,3 +500,68 @@ def test_angular_merge(kind, water_dir, tmp_path):
     m.set_option(n2p2_amd.OPT_ANGULAR_MERGE_SF, -1)
     m.set_option(n2p2_amd.OPT_ANGULAR_MERGE_FORCE, -1)
     assert m.get_option(n2p2_amd.OPT_ANGULAR_MERGE_SF) == -1
+
+
+@pytest.mark.parametrize("kind", ["water", "fcc"])
+def test_pair_lists(kind, water_dir, tmp_path):
+    """Angular pair lists kept by the SF kernel for the force kernel.  With
+    lists on, the first evaluation has none yet (the force kernel tests its
+    candidates in the same block order and buffers), reading the results
+    sizes them, the next evaluation reads them: G and energies agree bit for
+    bit, forces to the order of the FP64 device atomics on F (any two
+    evaluations do).  A denser structure under the lists sized for the first
+    overflows some walks (they test again) and must still agree with an
+    evaluation without lists.  Off, the force kernel walks rows: the same
+    terms in another order, within FP32 summation order; every variant
+    meets the oracle bounds."""
+    if kind == "fcc":
+        d = synth.fcc_model(str(tmp_path / "fcc"))
+        pos, elem, box = synth.fcc_lattice(n_cells=5)
+    else:
+        d = water_dir
+        pos, elem, box = synth.water_box(4)
+    say("case: pair lists on/off, %s, %d atoms" % (kind, len(elem)))
+
+    def run(m, p, b):
+        m.set_structures(remap(p, b), elem, b[None])
+        m.evaluate(n2p2_amd.MIXED, forces=True)
+        return m.G().copy(), m.energies().copy(), m.forces().copy()
+
+    def same(a, b, what):
+        assert np.array_equal(a[0], b[0]), "%s: G differs" % what
+        assert np.array_equal(a[1], b[1]), "%s: energies differ" % what
+        df = float(np.abs(a[2] - b[2]).max() / np.abs(b[2]).max())
+        say("  %s: forces %.2e" % (what, df))
+        assert df <= 1e-13, "%s: forces differ" % what
+
+    o = Oracle(d)
+    o.set_structure(pos, elem, box)
+    o.predict(PULL, BRUTE)
+    m = n2p2_amd.Mode(d)
+    m.set_option(n2p2_amd.OPT_PAIR_LISTS, 1)
+    first = run(m, pos, box)      # no lists yet
+    second = run(m, pos, box)     # the results read above sized them
+    same(first, second, "lists read vs tested")
+    rms, mx, erel = errors(m, o)
+    say("  lists on: E rel %.2e  F rms %.2e max %.2e" % (erel, rms, mx))
+    assert erel <= 1e-5 and rms <= 1e-5
+    # denser: more pairs per atom than the lists were sized for
+    dense = (pos * 0.9, box * 0.9)
+    partly = run(m, *dense)
+    m2 = n2p2_amd.Mode(d)
+    m2.set_option(n2p2_amd.OPT_PAIR_LISTS, 1)
+    tested = run(m2, *dense)
+    same(partly, tested, "dense, lists partly kept vs tested")
+    refit = run(m, *dense)
+    same(refit, tested, "dense, lists refitted vs tested")
+    # off: rows walk
+    m.set_option(n2p2_amd.OPT_PAIR_LISTS, 0)
+    rows = run(m, pos, box)
+    assert m.get_option(n2p2_amd.OPT_PAIR_LISTS) == 0
+    dg = float(np.abs(rows[0] - second[0]).max())
+    df = float(np.abs(rows[2] - second[2]).max() / np.abs(second[2]).max())
+    say("  rows walk vs lists: G %.2e  F %.2e" % (dg, df))
+    assert dg == 0.0 and df <= 2e-5
+    rms, mx, erel = errors(m, o)
+    say("  lists off: E rel %.2e  F rms %.2e max %.2e" % (erel, rms, mx))
+    assert erel <= 1e-5 and rms <= 1e-5
