@@ -916,6 +916,69 @@ __device__ inline void sf32_power_merged(const ModelT* __restrict__ M, const Sta
     __syncthreads();
 }
 
+// SF values of an element's radial groups when all its groups share one
+// cutoff (staged cutoff values in S.c): all groups at once, each member on
+// L = 64 / members lanes (at least one) that sum every L-th neighbour of its
+// group in FP32 with compensation; the L partial sums are added in FP64 in
+// lane order through LDS (the idle triplet buffer), and the member's first
+// lane scales and stores the value.
+__device__ inline void radial_lanes(const ModelT* __restrict__ M, const Stage& S, const MemberF* __restrict__ MF,
+                                    const ElementT& E, int n, double* __restrict__ Gi, float* __restrict__ Ii)
+{
+    const int lane = threadIdx.x;
+    int nrad = 0;
+    for (int gi = 0; gi < E.ngrp; ++gi) {
+        const GroupT& GT = M->grp[E.grp0 + gi];
+        if (GT.type == 2 || GT.type == 12) nrad += GT.nmem;
+    }
+    const int L = nrad > 0 && nrad < kWave ? kWave / nrad : 1;  // lanes per member
+    const int sub = lane % L;
+    double* part = reinterpret_cast<double*>(S.tjk);  // kWave doubles fit the smallest buffer
+    for (int base = 0; base < nrad; base += kWave / L) {
+        // the lane's member: the (base + lane / L)-th radial member of the element
+        int rem = base + lane / L, gsel = -1;
+        if (lane / L < kWave / L)
+            for (int gi = 0; gi < E.ngrp && gsel < 0; ++gi) {
+                const GroupT& GT = M->grp[E.grp0 + gi];
+                if (GT.type != 2 && GT.type != 12) continue;
+                if (rem < GT.nmem) gsel = gi;
+                else rem -= GT.nmem;
+            }
+        float acc = 0.f, cmp = 0.f;
+        if (gsel >= 0) {
+            const GroupT& GT = M->grp[E.grp0 + gsel];
+            const MemberF mq = MF[GT.mem0 + rem];
+            // type 2: the slots of neighbour element e1; type 12: every slot,
+            // each term times its atomic number (SymGrpExpRadWeighted.cpp)
+            const bool wt = GT.type == 12;
+            const float rc = (float)GT.rc;
+            const int lo = wt ? 0 : S.eoff[GT.e1], hi = wt ? n : S.eoff[GT.e1 + 1];
+            for (int k = lo + sub; k < hi; k += L) {
+                const float r = S.g[k].w;
+                if (!(r < rc)) continue;
+                const float4 ck = S.c[k];
+                const float dr = r - mq.rs;
+                kahan(acc, cmp, __expf(-mq.eta * dr * dr) * (wt ? ck.y * ck.w : ck.y));
+            }
+        }
+        double v = (double)acc + (double)cmp;
+        if (L > 1) {
+            part[lane] = v;
+            __syncthreads();
+            if (sub == 0 && gsel >= 0)
+                for (int t = 1; t < L; ++t) v += part[lane + t];
+            __syncthreads();
+        }
+        if (gsel >= 0 && sub == 0) {
+            const GroupT& GT = M->grp[E.grp0 + gsel];
+            const MemberT& mt = M->mem[GT.mem0 + rem];
+            const double gs = scale_sf(M->scale[E.sf0 + mt.sf], v);
+            Gi[mt.sf] = gs;
+            Ii[mt.sf] = (float)gs;
+        }
+    }
+}
+
 // ------------------------------------------------------------- SF values --
 // Angular terms: vexp * pfc * pl^zeta evaluated as one exp2 of
 // zeta * log2(pl) + log2(pfc) - eta log2(e) * sum(r - rs)^2, with log2(pl)
@@ -948,20 +1011,23 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
     float* Ii = Gin + (size_t)i * ldK;
     for (int q = E.nsf + lane; q < ldK; q += kWave) Ii[q] = 0.f;
     const bool merged = MERGE && AM[e].ng > 0;
+    // one cutoff for all of the element's groups: the cutoff values once,
+    // and every radial member at once, one lane each
+    if (E.ucut && E.ngrp > 0) {
+        all_cutoffs(S, n, ctype, group_f(M->grp[E.grp0]), tanhPre);
+        radial_lanes(M, S, MF, E, n, Gi, Ii);
+    }
 
     for (int gi = 0; gi < E.ngrp; ++gi) {
         const GroupT& GT = M->grp[E.grp0 + gi];
         const GroupF g = group_f(GT);
         const int nmem = GT.nmem;
         const MemberF* mf = MF + GT.mem0;
-        if (E.ucut) {
-            if (gi == 0) all_cutoffs(S, n, ctype, g, tanhPre);
-        } else {
-            group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
-        }
+        if (!E.ucut) group_cutoffs(S, n, GT.e1, GT.type == 2 ? GT.e1 : GT.e2, ctype, g, tanhPre);
         if (sf_compact(GT.type)) continue;  // the compact kernel's
         if (merged && GT.type != 2 && GT.type != 12) continue;  // the merged walk's
         if (GT.type == 2 || GT.type == 12) {
+            if (E.ucut) continue;  // radial_lanes did it
             // radial members in chunks of kRadChunk (any member count);
             // type 12: every neighbour, each term times its atomic number
             // (SymGrpExpRadWeighted.cpp)
