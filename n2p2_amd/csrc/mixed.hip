@@ -214,25 +214,45 @@ __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s,
         }
         return el;
     };
+    // The entries of the first kKeep passes are kept in registers between
+    // the counting pass and the placing pass (lists up to kKeep * 64
+    // entries); later ones are recomputed.
+    constexpr int kKeep = 4;
+    float kx[kKeep], ky[kKeep], kz[kKeep], kd[kKeep], ki[kKeep];
+    int kel[kKeep];
     // pass 1: in-cutoff entries per element (lane e counts element e)
     int cnt = 0;
-    for (int b = 0; b < nl; b += kWave) {
-        double dr[3], d;
-        const int el = entry(b + lane, dr, d);
+    auto count = [&](int el) {
         for (int e = 0; e < ne; ++e) {
             const int c = __popcll(__ballot(el == e));
             if (lane == e) cnt += c;
         }
+    };
+#pragma unroll
+    for (int p = 0; p < kKeep; ++p) {
+        kel[p] = -1;
+        kx[p] = ky[p] = kz[p] = kd[p] = ki[p] = 0.f;
+        if (p * kWave < nl) {
+            double dr[3], d;
+            kel[p] = entry(p * kWave + lane, dr, d);
+            kx[p] = (float)dr[0];
+            ky[p] = (float)dr[1];
+            kz[p] = (float)dr[2];
+            kd[p] = (float)d;
+            ki[p] = (float)(1.0 / d);
+            count(kel[p]);
+        }
+    }
+    for (int b = kKeep * kWave; b < nl; b += kWave) {
+        double dr[3], d;
+        count(entry(b + lane, dr, d));
     }
     const int incl = wave_incl_scan(lane < ne ? cnt : 0);
     const int total = lane_value(incl, kWave - 1);
     int next = incl - (lane < ne ? cnt : 0);  // lane e: next slot of element e
     if (lane <= ne) S.eoff[lane] = min(lane < ne ? next : total, capL);
     // pass 2: each entry to its element's next slot
-    for (int b = 0; b < nl; b += kWave) {
-        const int k = b + lane;
-        double dr[3], d;
-        const int el = entry(k, dr, d);
+    auto place = [&](int k, int el, float x, float y, float z, float d, float id) {
         int pos = -1;
         for (int e = 0; e < ne; ++e) {
             const unsigned long long m = __ballot(el == e);
@@ -243,12 +263,20 @@ __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s,
             if (lane == e) next += __popcll(m);
         }
         if (pos >= 0 && pos < capL) {
-            S.g[pos] = make_float4((float)dr[0], (float)dr[1], (float)dr[2], (float)d);
+            S.g[pos] = make_float4(x, y, z, d);
             // .w: the neighbour's atomic number, the weight of types 12 and 13
-            S.c[pos] = make_float4((float)(1.0 / d), 0.f, 0.f, (float)M->Zel[el]);
+            S.c[pos] = make_float4(id, 0.f, 0.f, (float)M->Zel[el]);
             S.el[pos] = (unsigned char)el;
             if (idx) idx[pos] = (unsigned short)k;
         }
+    };
+#pragma unroll
+    for (int p = 0; p < kKeep; ++p)
+        if (p * kWave < nl) place(p * kWave + lane, kel[p], kx[p], ky[p], kz[p], kd[p], ki[p]);
+    for (int b = kKeep * kWave; b < nl; b += kWave) {
+        double dr[3], d;
+        const int el = entry(b + lane, dr, d);
+        place(b + lane, el, (float)dr[0], (float)dr[1], (float)dr[2], (float)d, (float)(1.0 / d));
     }
     if (lane == 0) {
         if (total > capL) atomicOr(&s.flags[0], 4);
