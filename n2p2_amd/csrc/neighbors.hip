@@ -32,13 +32,11 @@ constexpr int kCandUnroll = 4;   // candidates per lane and step
 
 __device__ inline bool gated_off(const SysView& s) { return s.gate && *s.gate == 0; }
 
-__global__ void k_cell_count(SysView s)
+// Cell of atom i (-1: its structure is built by brute force).
+__device__ inline int atom_cell_of(const SysView& s, int i)
 {
-    if (gated_off(s)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.natoms) return;
     const StructT& S = s.st[s.sid[i]];
-    if (S.algo != 1) { s.atom_cell[i] = -1; return; }
+    if (S.algo != 1) return -1;
     const double* r = s.pos + 3 * (size_t)i;
     int c[3];
     for (int k = 0; k < 3; ++k) {
@@ -47,9 +45,27 @@ __global__ void k_cell_count(SysView s)
         int q = (int)(f * S.nc[k]);
         c[k] = q < 0 ? 0 : (q >= S.nc[k] ? S.nc[k] - 1 : q);
     }
-    const int cell = S.cell0 + (c[0] * S.nc[1] + c[1]) * S.nc[2] + c[2];
+    return S.cell0 + (c[0] * S.nc[1] + c[1]) * S.nc[2] + c[2];
+}
+
+// Atom i into its cell's cell-ordered arrays at position p.
+__device__ inline void cell_place(const SysView& s, int i, int p)
+{
+    s.cell_atoms[p] = i;
+    // cell-ordered copy of the positions: candidate loads become contiguous
+    s.cell_pos[3 * (size_t)p] = s.pos[3 * (size_t)i];
+    s.cell_pos[3 * (size_t)p + 1] = s.pos[3 * (size_t)i + 1];
+    s.cell_pos[3 * (size_t)p + 2] = s.pos[3 * (size_t)i + 2];
+}
+
+__global__ void k_cell_count(SysView s)
+{
+    if (gated_off(s)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.natoms) return;
+    const int cell = atom_cell_of(s, i);
     s.atom_cell[i] = cell;
-    atomicAdd(&s.cell_count[cell], 1);
+    if (cell >= 0) atomicAdd(&s.cell_count[cell], 1);
 }
 
 // Exclusive scan of cell counts (single block; ncells is small).
@@ -87,12 +103,7 @@ __global__ void k_cell_fill(SysView s)
     if (i >= s.natoms) return;
     const int c = s.atom_cell[i];
     if (c < 0) return;
-    const int p = s.cell_start[c] + atomicAdd(&s.cell_count[c], 1);
-    s.cell_atoms[p] = i;
-    // cell-ordered copy of the positions: candidate loads become contiguous
-    s.cell_pos[3 * (size_t)p] = s.pos[3 * (size_t)i];
-    s.cell_pos[3 * (size_t)p + 1] = s.pos[3 * (size_t)i + 1];
-    s.cell_pos[3 * (size_t)p + 2] = s.pos[3 * (size_t)i + 2];
+    cell_place(s, i, s.cell_start[c] + atomicAdd(&s.cell_count[c], 1));
 }
 
 __device__ inline void emit(const SysView& s, int i, int slot, int j, int img)
@@ -315,6 +326,16 @@ __global__ void k_fill_int(int* p, int v, size_t n, const int* gate)
         p[q] = v;
 }
 
+// The flags (na entries) and the cell counts cleared by one launch.
+__global__ void k_clear_two(int* a, int na, int* b, int nb, const int* gate)
+{
+    if (gate && *gate == 0) return;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < na + nb; q += gridDim.x * blockDim.x) {
+        if (q < na) a[q] = 0;
+        else b[q - na] = 0;
+    }
+}
+
 }  // namespace
 
 // Memory clears use a fill kernel (not hipMemsetAsync) so the sequence can be
@@ -329,13 +350,16 @@ void fill_int(int* p, int v, size_t n, hipStream_t stream, const int* gate)
 void launch_neighbors(const SysView& s, double rc, bool any_cell, hipStream_t stream, bool clear_flags)
 {
     if (s.natoms == 0) return;
-    if (clear_flags) fill_int(s.flags, 0, 4, stream, s.gate);
     if (any_cell) {
-        fill_int(s.cell_count, 0, (size_t)s.ncells, stream, s.gate);
+        const int nf = clear_flags ? 4 : 0;
+        const int nc = (int)std::max<long long>(1, std::min<long long>((nf + (long long)s.ncells + 255) / 256, 4096));
+        hipLaunchKernelGGL(k_clear_two, dim3(nc), dim3(256), 0, stream, s.flags, nf, s.cell_count, s.ncells, s.gate);
         const int nb = (s.natoms + 255) / 256;
         hipLaunchKernelGGL(k_cell_count, dim3(nb), dim3(256), 0, stream, s);
         hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, stream, s);
         hipLaunchKernelGGL(k_cell_fill, dim3(nb), dim3(256), 0, stream, s);
+    } else if (clear_flags) {
+        fill_int(s.flags, 0, 4, stream, s.gate);
     }
     const int slots = sort_slots(s.cap);
     const size_t lds = any_cell ? sizeof(long long) * (size_t)slots + kCandLds : 0;
