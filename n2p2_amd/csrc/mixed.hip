@@ -193,7 +193,7 @@ __device__ inline float dist2(const float4& a, const float4& b)
 // repeats with the full capacity) and flags[2] keeps the largest count met,
 // from which callers size capL.
 __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s, int i, int capL, const Stage& S,
-                                 unsigned short* idx = nullptr)
+                                 unsigned short* idx = nullptr, const StageKeep* keep = nullptr)
 {
     const int lane = threadIdx.x;
     const StructT& St = s.st[s.sid[i]];
@@ -250,7 +250,10 @@ __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s,
     const int incl = wave_incl_scan(lane < ne ? cnt : 0);
     const int total = lane_value(incl, kWave - 1);
     int next = incl - (lane < ne ? cnt : 0);  // lane e: next slot of element e
-    if (lane <= ne) S.eoff[lane] = min(lane < ne ? next : total, capL);
+    if (lane <= ne) {
+        S.eoff[lane] = min(lane < ne ? next : total, capL);
+        if (keep) keep->off[(size_t)i * (kMaxElem + 1) + lane] = min(lane < ne ? next : total, capL);
+    }
     // pass 2: each entry to its element's next slot
     auto place = [&](int k, int el, float x, float y, float z, float d, float id) {
         int pos = -1;
@@ -268,6 +271,13 @@ __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s,
             S.c[pos] = make_float4(id, 0.f, 0.f, (float)M->Zel[el]);
             S.el[pos] = (unsigned char)el;
             if (idx) idx[pos] = (unsigned short)k;
+            if (keep) {
+                const size_t q = (size_t)i * keep->ld + pos;
+                keep->g[q] = make_float4(x, y, z, d);
+                keep->rinv[q] = id;
+                keep->nbr[q] = s.nb_j[base + k];
+                keep->el[q] = (unsigned char)el;
+            }
         }
     };
 #pragma unroll
@@ -285,6 +295,26 @@ __device__ inline int stage_atom(const ModelT* __restrict__ M, const SysView& s,
     }
     __syncthreads();
     return uni(min(total, capL));
+}
+
+// The staging of atom i as the SF kernel kept it (same slots, same values
+// as stage_atom gives); returns the slot count.
+__device__ inline int load_stage(const ModelT* __restrict__ M, const StageKeep& K, int i, const Stage& S)
+{
+    const int lane = threadIdx.x;
+    const int ne = M->nelem;
+    const int* off = K.off + (size_t)i * (kMaxElem + 1);
+    if (lane <= ne) S.eoff[lane] = off[lane];
+    const int n = uni(off[ne]);
+    const size_t b = (size_t)i * K.ld;
+    for (int k = lane; k < n; k += kWave) {
+        const int el = K.el[b + k];
+        S.g[k] = K.g[b + k];
+        S.c[k] = make_float4(K.rinv[b + k], 0.f, 0.f, (float)M->Zel[el]);
+        S.el[k] = (unsigned char)el;
+    }
+    __syncthreads();
+    return n;
 }
 
 // Neighbours of element `elem` (all elements: elem < 0) inside the group's
@@ -1021,7 +1051,7 @@ template <int MAXM, bool FAST, int NB0, int NB1, bool GATHER, bool MERGE>
 __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                              const GroupMixF* __restrict__ GM, const AngMergeF* __restrict__ AM,
                                              SysView s, double* __restrict__ G, float* __restrict__ Gin, int ldG,
-                                             int ldK, int capL, int trip, TripList T)
+                                             int ldK, int capL, int trip, TripList T, StageKeep K)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
@@ -1033,7 +1063,7 @@ __global__ __launch_bounds__(64) void k_sf32(const ModelT* __restrict__ M, const
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
-    const int n = stage_atom(M, s, i, capL, S);
+    const int n = stage_atom(M, s, i, capL, S, nullptr, K.g ? &K : nullptr);
     TripCursor tc = trip_cursor(T, i);
     double* Gi = G + (size_t)i * ldG;
     float* Ii = Gin + (size_t)i * ldK;
@@ -1559,7 +1589,8 @@ template <int MAXM, bool FAST, int NB0, int NB1, bool MERGE>
 __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, const MemberF* __restrict__ MF,
                                                 const GroupMixF* __restrict__ GM, const AngMergeF* __restrict__ AM,
                                                 SysView s, const float* __restrict__ fw, int ldG,
-                                                double* __restrict__ F, int push, int capL, int trip, TripList T)
+                                                double* __restrict__ F, int push, int capL, int trip, TripList T,
+                                                StageKeep K)
 {
     extern __shared__ float sm[];
     const Stage S = carve(sm, capL, trip, stage_mp(NB0, NB1));
@@ -1574,7 +1605,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     const ElementT& E = M->el[e];
     const int ctype = M->cutoffType;
     const float tanhPre = (float)M->tanhPre;
-    const int n = stage_atom(M, s, i, capL, S, idx);
+    const int n = K.g ? load_stage(M, K, i, S) : stage_atom(M, s, i, capL, S, idx);
     TripCursor tc = trip_cursor(T, i);
     for (int k = lane; k < n; k += kWave) { ax[k] = 0.0; ay[k] = 0.0; az[k] = 0.0; }
     const float* wi = fw + (size_t)i * ldG;
@@ -1812,7 +1843,7 @@ __global__ __launch_bounds__(64) void k_force32(const ModelT* __restrict__ M, co
     }
     const size_t base = (size_t)i * s.cap;
     for (int k = lane; k < n; k += kWave) {
-        const int j = s.nb_j[base + idx[k]];
+        const int j = K.g ? K.nbr[(size_t)i * K.ld + k] : s.nb_j[base + idx[k]];
         if (j == i && !push) continue;
         atomicAdd(F + 3 * (size_t)j, ax[k]);
         atomicAdd(F + 3 * (size_t)j + 1, ay[k]);
@@ -2411,6 +2442,8 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
         T.count = true;
     }
     T.rows = !lists;
+    // the staging passes from the SF to the force kernel (identical values)
+    const StageKeep KS = forces && b.keep.ld >= capL ? b.keep : StageKeep{};
     timed("sf32", [&] {
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
             constexpr int cm = decltype(m)::value, c0 = decltype(nb0)::value, c1 = decltype(nb1)::value;
@@ -2427,7 +2460,7 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
                 constexpr bool cg = decltype(gather)::value, cmm = decltype(merge)::value;
                 hipLaunchKernelGGL((k_sf32<cm, cf, c0, c1, cg, cmm>), dim3(s.natoms), dim3(kWave), cg ? lds1 : lds0,
                                    stream, d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.G, b.Gin, b.ldG, mm.ldK,
-                                   capL, tsf, T);
+                                   capL, tsf, T, KS);
             };
             auto pick = [&](auto merge) {
                 constexpr bool cmm = decltype(merge)::value;
@@ -2495,10 +2528,12 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
             if (lds_f > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "neighbour count too large for the mixed kernels' LDS"};
             if (mg)
                 hipLaunchKernelGGL((k_force32<cm, cf, c0, c1, (c0 > 0)>), dim3(s.natoms), dim3(kWave), lds_f, stream,
-                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf, T);
+                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf, T,
+                                   KS);
             else
                 hipLaunchKernelGGL((k_force32<cm, cf, c0, c1, false>), dim3(s.natoms), dim3(kWave), lds_f, stream,
-                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf, T);
+                                   d_model, mm.d_mem, mm.d_gmix, mm.d_amerge, s, b.dEdGf, b.ldG, b.F, ps, capL, tf, T,
+                                   KS);
         });
         launch_sfcomp32(d_model, t, s, nullptr, nullptr, b.ldG, mm.ldK, b.dEdGf, b.F, push, true, stream);
     });

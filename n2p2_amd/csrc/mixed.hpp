@@ -89,6 +89,19 @@ struct TripList {
     bool rows = false;         // the force kernel walks rows (no lists kept)
 };
 
+// Each atom's LDS staging kept in device memory by the SF kernel for the
+// force kernel of the same evaluation, which loads it (coalesced) instead of
+// gathering and sorting the neighbour list again: slots in staging order
+// with per-atom stride ld.
+struct StageKeep {
+    float4* g = nullptr;         // [natoms][ld] x, y, z, |dr| (FP32)
+    float* rinv = nullptr;       // [natoms][ld] 1/|dr|
+    int* nbr = nullptr;          // [natoms][ld] the entry's neighbour atom
+    unsigned char* el = nullptr; // [natoms][ld] its element
+    int* off = nullptr;          // [natoms][kMaxElem + 1] first slot of each element, then the slot count
+    int ld = 0;
+};
+
 // Angular groups of one element walked together (power path): the groups
 // share type, cutoff and term constants and differ only in their neighbour
 // element pair and members, so one triplet walk over all in-cutoff
@@ -158,6 +171,7 @@ struct MixedBuffers {
     int nstruct;
     int cap;
     TripList trips;    // the angular pair lists shared by the SF and force kernels
+    StageKeep keep;    // the staging shared by the SF and force kernels
 };
 
 using TimedFn = std::function<void(const char*, const std::function<void()>&)>;

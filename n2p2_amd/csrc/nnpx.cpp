@@ -790,6 +790,16 @@ MixedBuffers mixed_buffers(nnpx_ctx* c)
     for (int e = 0; e <= t.nelem; ++e) m.elem_off[e] = c->elem_off[e];
     m.nstruct = c->S;
     m.cap = c->cap;
+    // the staging shared by the SF and force kernels, one slot per list entry
+    const size_t ks = (size_t)c->N * c->cap;
+    if (!c->arena && ks * (sizeof(float4) + sizeof(float) + sizeof(int) + 1) <= kTripBudget) {
+        m.keep.ld = c->cap;
+        m.keep.g = buf<float4>(c, "keep_g", ks);
+        m.keep.rinv = buf<float>(c, "keep_r", ks);
+        m.keep.nbr = buf<int>(c, "keep_j", ks);
+        m.keep.el = buf<unsigned char>(c, "keep_e", ks);
+        m.keep.off = buf<int>(c, "keep_o", (size_t)c->N * (kMaxElem + 1));
+    }
     const size_t tl = (size_t)c->N * c->trip_cap;
     if (c->trip_cap > 0 && c->mixed.tune.pair_lists == 1 && !c->arena && tl * sizeof(unsigned) <= kTripBudget) {
         m.trips.cap = c->trip_cap;
