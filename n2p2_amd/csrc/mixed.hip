@@ -3,11 +3,14 @@
 // k_sf32     symmetry-function values in FP32, Kahan sums per lane, lanes
 //            combined in FP64, FP64 scaling (SymGrpExpRad::calculate /
 //            SymGrpExpAngn::calculate without derivatives,
-//            SymGrpExpRad.cpp:144-239, SymGrpExpAngn.cpp:156-403).
-// k_nnbf16   NeuralNetwork::propagate (NeuralNetwork.cpp:761-896) for 16
-//            atoms per wave as v_mfma_f32_16x16x16_bf16 tiles, and the
-//            adjoint dE/dG as the transposed GEMMs (replacing the reference's
-//            forward-mode NeuralNetwork::calculateDEdG, NeuralNetwork.cpp:396-442).
+//            SymGrpExpRad.cpp:144-239, SymGrpExpAngn.cpp:156-403); keeps
+//            each atom's staging, and optionally its angular pairs, in
+//            device memory for the force kernel of the same evaluation.
+// k_nnbf16   NeuralNetwork::propagate (NeuralNetwork.cpp:761-896) for a
+//            16-atom tile per block (four waves) as v_mfma_f32_16x16x16_bf16
+//            tiles, and the adjoint dE/dG as the transposed GEMMs (replacing
+//            the reference's forward-mode NeuralNetwork::calculateDEdG,
+//            NeuralNetwork.cpp:396-442).
 // k_force32  fused dG/dr backward: recomputes each pair/triplet term and
 //            contracts it with dE/dG (Mode::calculateForces semantics,
 //            Mode.cpp:1922-1984) without storing dG/dr; FP64 accumulation
