@@ -1379,11 +1379,16 @@ __global__ __launch_bounds__(kWave * kNNWaves) void k_nnbf16(const ModelT* __res
         const float norm = N.normalize ? 1.f / N.n[l - 1] : 1.f;
         for (int n0 = 16 * wv; n0 < po; n0 += 16 * kNNWaves) {
             f4 c = {0.f, 0.f, 0.f, 0.f};
+            // the next weight fragment is loaded while this one multiplies
+            s4 bh, bl;
+            b_frag(W, pi, po * pi, n0, 0, bh, bl);
             for (int k0 = 0; k0 < pi; k0 += 16) {
-                s4 ah, al, bh, bl;
+                s4 ah, al, nh = bh, nl_ = bl;
+                if (k0 + 16 < pi) b_frag(W, pi, po * pi, n0, k0 + 16, nh, nl_);
                 a_frag(in, sin, k0, ah, al);
-                b_frag(W, pi, po * pi, n0, k0, bh, bl);
                 c = mma(c, ah, al, bh, bl, a.terms);
+                bh = nh;
+                bl = nl_;
             }
             const int n = n0 + (lane & 15);
 #pragma unroll
@@ -1418,11 +1423,15 @@ __global__ __launch_bounds__(kWave * kNNWaves) void k_nnbf16(const ModelT* __res
         const float norm = N.normalize ? 1.f / N.n[l - 1] : 1.f;
         for (int m0 = 16 * wv; m0 < pi; m0 += 16 * kNNWaves) {
             f4 c = {0.f, 0.f, 0.f, 0.f};
+            s4 bh, bl;
+            b_frag(WT, po, po * pi, m0, 0, bh, bl);
             for (int k0 = 0; k0 < po; k0 += 16) {
-                s4 ah, al, bh, bl;
+                s4 ah, al, nh = bh, nl_ = bl;
+                if (k0 + 16 < po) b_frag(WT, po, po * pi, m0, k0 + 16, nh, nl_);
                 a_frag(dl, st, k0, ah, al);
-                b_frag(WT, po, po * pi, m0, k0, bh, bl);
                 c = mma(c, ah, al, bh, bl, a.terms);
+                bh = nh;
+                bl = nl_;
             }
             const int m = m0 + (lane & 15);
 #pragma unroll
