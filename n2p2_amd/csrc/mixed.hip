@@ -552,10 +552,31 @@ __device__ inline int for_each_triplet_blocks(const Stage& S, const float4* gB, 
     int ja = slot<CONTIG>(S.la, A.base, lr < min(nA, rh) ? lr : 0);
     float4 ga = a0 < nA ? S.g[ja] : make_float4(0.f, 0.f, 0.f, 0.f);
     int cnt = 0, done = 0;  // pairs in the buffer, pairs run so far
+    // stored pairs are loaded a buffer ahead (buffers of up to kAhead per
+    // lane), so the loads overlap the previous buffer's work
+    constexpr int kAhead = 6;
+    unsigned ahead[kAhead];
+    const bool pre = src && S.trip <= kAhead * kWave;
+    auto load_ahead = [&](int from) {
+        const int m = min(S.trip, nsrc - from);
+#pragma unroll
+        for (int q = 0; q < kAhead; ++q) {
+            const int x = q * kWave + lane;
+            ahead[q] = x < m ? src[from + x] : 0u;
+        }
+    };
+    if (pre) load_ahead(0);
     for (;;) {
         if (src) {
             cnt = min(S.trip, nsrc - done);
-            for (int x = lane; x < cnt; x += kWave) S.tjk[x] = src[done + x];
+            if (pre) {
+#pragma unroll
+                for (int q = 0; q < kAhead; ++q)
+                    if (q * kWave + lane < cnt) S.tjk[q * kWave + lane] = ahead[q];
+                if (done + cnt < nsrc) load_ahead(done + cnt);
+            } else {
+                for (int x = lane; x < cnt; x += kWave) S.tjk[x] = src[done + x];
+            }
         } else {
             while (a0 < nA && cnt < S.trip) {
                 // branch-free: lanes outside the block test column 0 and fail
@@ -869,11 +890,12 @@ __device__ inline void power_weights(const Stage& S, const MemberF* __restrict__
     }
     if (lane < NBAS) S.mp[lane] = make_float4(sb, sa, 0.f, 0.f);
     __syncthreads();
+    // uniform: held in scalar registers
 #pragma unroll
     for (int b = 0; b < NBAS; ++b) {
         const float4 v = S.mp[b];
-        wb[b] = v.x;
-        wa[b] = v.y;
+        wb[b] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v.x)));
+        wa[b] = __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v.y)));
     }
 }
 
