@@ -536,9 +536,10 @@ __device__ inline int for_each_triplet_blocks(const Stage& S, const float4* gB, 
     const int R = same ? (nA - 1) / 2 : nB;          // regular columns per row
     const int Rx = same && nA % 2 == 0 ? R + 1 : R;  // with the half-way column (rows a < nA / 2 only)
     const int half = nA / 2;
-    // a row's columns split into nearly equal blocks of at most 16, so
-    // the passing partners of one row stay long runs in the buffer
-    const int nblk = (Rx + 15) >> 4;
+    // a row's columns split into nearly equal blocks of at most 8: the
+    // blocks fill the wave's lanes closely, and the passing partners of one
+    // row still form runs in the buffer
+    const int nblk = (Rx + 7) >> 3;
     const int cw = nblk > 0 ? (Rx + nblk - 1) / nblk : 1, rh = kWave / cw;
     const int lr = lane / cw, lc = lane - lr * cw;  // lanes with lr >= rh stay idle
     const bool live = lr < rh;
