@@ -5,7 +5,8 @@
 //
 // Two builders share the output layout (fixed capacity per atom):
 //  * brute force -- one wave per atom, lanes over j, serial images; hits are
-//    emitted in loop order with a wave prefix sum (no sort needed);
+//    kept as a bit per image and emitted in loop order after a wave prefix
+//    sum (no sort needed);
 //  * cell list  -- for structures whose cell holds >= 3 cells of width >= rc
 //    on every axis: atoms binned by fractional coordinate, candidates from
 //    the 27 surrounding cells, hits collected in LDS and bitonic-sorted by
@@ -146,6 +147,45 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
                 }
             }
         }
+        const int n12 = (hi[1] - lo[1] + 1) * (hi[2] - lo[2] + 1);
+        const int nimg = (hi[0] - lo[0] + 1) * n12;
+        if (nimg <= kWave) {
+            // One pass: each lane records its hits as a bit per image (in loop
+            // order) and emits them from the mask after the wave's prefix sum.
+            __shared__ int img_tab[kWave];
+            if (lane < nimg) {
+                const int a0 = lane / n12, a1 = (lane - a0 * n12) / (hi[2] - lo[2] + 1);
+                const int a2 = lane - a0 * n12 - a1 * (hi[2] - lo[2] + 1);
+                img_tab[lane] = pack_image(lo[0] + a0, lo[1] + a1, lo[2] + a2);
+            }
+            __syncthreads();
+            for (int jb = 0; jb < S.natoms; jb += kWave) {
+                const int jl = jb + lane;
+                const int j = S.atom0 + jl;
+                unsigned long long hm = 0;
+                if (jl < S.natoms) {
+                    const double* rj = s.pos + 3 * (size_t)j;
+                    int t = 0;
+                    for (int b0 = lo[0]; b0 <= hi[0]; ++b0)
+                        for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
+                            for (int b2 = lo[2]; b2 <= hi[2]; ++b2, ++t) {
+                                if (i == j && b0 == 0 && b1 == 0 && b2 == 0) continue;
+                                double dr[3];
+                                pair_vector(ri, rj, S.box, b0, b1, b2, dr);
+                                if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2) hm |= 1ull << t;
+                            }
+                }
+                const int hits = __popcll(hm);
+                const int incl = wave_incl_scan(hits);
+                int slot = count + incl - hits;
+                while (hm) {
+                    emit(s, i, slot++, j, img_tab[__builtin_ctzll(hm)]);
+                    hm &= hm - 1;
+                }
+                count += lane_value(incl, kWave - 1);
+            }
+        } else {
+        // More images than lanes: count the hits, then test again to emit.
         for (int jb = 0; jb < S.natoms; jb += kWave) {
             const int jl = jb + lane;
             const int j = S.atom0 + jl;
@@ -176,6 +216,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
                         }
             }
             count += lane_value(incl, kWave - 1);
+        }
         }
     } else {
         // The 27 surrounding cells are flattened into one candidate range so
