@@ -76,7 +76,8 @@ def compare_neighbors(m, o):
     return int(c1.sum())
 
 
-@pytest.mark.parametrize("case", ["water192", "water1536_cells", "golden_cells", "small_cells"])
+@pytest.mark.parametrize("case", ["water192", "water1536_cells", "golden_cells", "small_cells",
+                                  "triclinic_cells"])
 def test_neighbor_list_identical(case, water_dir):
     say("case: neighbour list", case)
     if case == "golden_cells":
@@ -93,6 +94,14 @@ def test_neighbor_list_identical(case, water_dir):
             pos, elem, box = synth.water_box(4)
         elif case == "water1536_cells":
             pos, elem, box = synth.water_box(8)
+        elif case == "triclinic_cells":
+            # skewed cell with >= 3 cells per axis (perpendicular widths
+            # 38-41 bohr against rc = 12): the cell search's skipped cells
+            # are bounded by their largest per-axis gap here
+            rng = np.random.default_rng(13)
+            box = np.array([[41.0, 0.0, 0.0], [9.0, 39.5, 0.0], [-6.0, 5.0, 40.5]])
+            pos = remap(rng.random((900, 3)) @ box, box)
+            elem = (rng.random(900) < 0.35).astype(np.int32)
         else:
             rng = np.random.default_rng(11)
             box = np.eye(3) * 4.2
