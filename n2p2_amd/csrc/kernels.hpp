@@ -29,6 +29,7 @@ struct SysView {
     int ncells;
     int* atom_cell;        // [N]
     int* cell_count;       // [ncells]
+    int* cell_rank;        // [N] atom's place among its cell's atoms
     int* cell_start;       // [ncells+1]
     int* cell_atoms;       // [N]
     double* cell_pos;      // [3N] positions in cell order

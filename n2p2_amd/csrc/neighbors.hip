@@ -66,7 +66,8 @@ __global__ void k_cell_count(SysView s)
     if (i >= s.natoms) return;
     const int cell = atom_cell_of(s, i);
     s.atom_cell[i] = cell;
-    if (cell >= 0) atomicAdd(&s.cell_count[cell], 1);
+    // the count before this atom is its place in the cell's atom range
+    if (cell >= 0) s.cell_rank[i] = atomicAdd(&s.cell_count[cell], 1);
 }
 
 // Exclusive scan of cell counts (single block; ncells is small).
@@ -92,7 +93,6 @@ __global__ __launch_bounds__(1024) void k_scan_cells(SysView s)
     for (int q = b; q < e; ++q) {
         s.cell_start[q] = run;
         run += s.cell_count[q];
-        s.cell_count[q] = 0;  // reused as fill cursor
     }
     if (t == 1023) s.cell_start[n] = part[1023];
 }
@@ -104,7 +104,7 @@ __global__ void k_cell_fill(SysView s)
     if (i >= s.natoms) return;
     const int c = s.atom_cell[i];
     if (c < 0) return;
-    cell_place(s, i, s.cell_start[c] + atomicAdd(&s.cell_count[c], 1));
+    cell_place(s, i, s.cell_start[c] + s.cell_rank[i]);
 }
 
 __device__ inline void emit(const SysView& s, int i, int slot, int j, int img)

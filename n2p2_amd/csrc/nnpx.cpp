@@ -310,6 +310,7 @@ SysView sysview(nnpx_ctx* c)
     s.ncells = c->ncells;
     s.atom_cell = existing<int>(c, "atom_cell");
     s.cell_count = existing<int>(c, "cell_count");
+    s.cell_rank = existing<int>(c, "cell_rank");
     s.cell_start = existing<int>(c, "cell_start");
     s.cell_atoms = existing<int>(c, "cell_atoms");
     s.cell_pos = existing<double>(c, "cell_pos");
@@ -455,6 +456,7 @@ void alloc_neighbors(nnpx_ctx* c)
     buf<int>(c, "flags", 4);
     buf<int>(c, "atom_cell", c->N);
     buf<int>(c, "cell_count", std::max(c->ncells, 1));
+    buf<int>(c, "cell_rank", c->N);
     buf<int>(c, "cell_start", c->ncells + 1);
     buf<int>(c, "cell_atoms", c->N);
     buf<double>(c, "cell_pos", (size_t)3 * c->N);
