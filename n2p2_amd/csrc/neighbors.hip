@@ -9,8 +9,9 @@
 //    sum (no sort needed);
 //  * cell list  -- for structures whose cell holds >= 3 cells of width >= rc
 //    on every axis: atoms binned by fractional coordinate, candidates from
-//    the 27 surrounding cells, hits collected in LDS and bitonic-sorted by
-//    (j, image) so the order equals the brute-force loop order.
+//    the 27 surrounding cells (less those wholly beyond the cutoff), hits
+//    collected in LDS and bitonic-sorted by (j, image) so the order equals
+//    the brute-force loop order.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -113,6 +114,47 @@ __device__ inline void emit(const SysView& s, int i, int slot, int j, int img)
         s.nb_j[(size_t)i * s.cap + slot] = j;
         s.nb_img[(size_t)i * s.cap + slot] = img;
     }
+}
+
+// Whether every atom of the neighbour cell at offset (d0, d1, d2) = (o / 9,
+// o / 3 % 3, o % 3) - 1 from atom i's cell (c0, c1, c2) lies beyond the
+// cutoff, so the cell need not be searched.  Along lattice direction k the
+// cell lies at least gap_k cell widths from atom i (measured perpendicular
+// to the cell faces, gap_k = u_k for d_k = -1, 1 - u_k for d_k = 1, with u_k
+// atom i's position inside its cell); the gaps combine as a sum of squares
+// in a cell with orthogonal axes and by their maximum otherwise.  The gaps
+// are shortened by 1e-6 cell widths, far beyond any rounding of the
+// fractional coordinates, so no candidate within the cutoff is dropped.
+__device__ inline bool cell_beyond_cutoff(const StructT& S, const double* ri, int c0, int c1, int c2, int o,
+                                          double rc2)
+{
+    const int d[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+    const int c[3] = {c0, c1, c2};
+    double g[3][3], w[3], sum2 = 0.0, max2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        g[k][0] = S.invbox[k];
+        g[k][1] = S.invbox[3 + k];
+        g[k][2] = S.invbox[6 + k];
+        // cell width perpendicular to the faces of direction k
+        w[k] = 1.0 / (S.nc[k] * sqrt(g[k][0] * g[k][0] + g[k][1] * g[k][1] + g[k][2] * g[k][2]));
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (d[k] == 0) continue;
+        const double f = ri[0] * g[k][0] + ri[1] * g[k][1] + ri[2] * g[k][2];
+        const double u = f * S.nc[k] - c[k];
+        const double gap = fmax(0.0, (d[k] < 0 ? u : 1.0 - u) - 1e-6) * w[k];
+        sum2 += gap * gap;
+        max2 = fmax(max2, gap * gap);
+    }
+    bool ortho = true;
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b) {
+            const double dot = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+            const double na = g[a][0] * g[a][0] + g[a][1] * g[a][1] + g[a][2] * g[a][2];
+            const double nb = g[b][0] * g[b][0] + g[b][1] * g[b][1] + g[b][2] * g[b][2];
+            if (dot * dot > 1e-24 * na * nb) ortho = false;
+        }
+    return (ortho ? sum2 : max2) > rc2;
 }
 
 // One wave per atom.  Dynamic LDS (cell-list structures only): the sort
@@ -239,6 +281,7 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
             start = s.cell_start[cell];
             len = s.cell_start[cell + 1] - start;
             img = pack_image(b[0], b[1], b[2]);
+            if (cell_beyond_cutoff(S, ri, c0, c1, c2, lane, rc2)) len = 0;
         }
         const int incl = wave_incl_scan(len);
         if (lane < 27) {
