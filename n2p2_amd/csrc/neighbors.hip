@@ -175,15 +175,17 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
         // perpendicular width), with a margin; the loops keep the
         // reference's (j, bc0, bc1, bc2) order.
         int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        double rk[3] = {0.0, 0.0, 0.0};  // rc in fractional units of direction k, with a margin
         if (S.periodic) {
             const double rc = sqrt(rc2);
             for (int k = 0; k < 3; ++k) {
                 lo[k] = -S.pbc[k];
                 hi[k] = S.pbc[k];
+                const double a = S.invbox[k], b = S.invbox[3 + k], c = S.invbox[6 + k];
+                const double r = rc * sqrt(a * a + b * b + c * c) * (1.0 + 1e-9) + 1e-9;
+                rk[k] = r;
                 if (S.inside) {
-                    const double a = S.invbox[k], b = S.invbox[3 + k], c = S.invbox[6 + k];
                     const double f = ri[0] * a + ri[1] * b + ri[2] * c;
-                    const double r = rc * sqrt(a * a + b * b + c * c) * (1.0 + 1e-9) + 1e-9;
                     lo[k] = max(lo[k], -(int)floor(f + r));
                     hi[k] = min(hi[k], (int)floor(1.0 - f + r));
                 }
@@ -194,6 +196,12 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
         if (nimg <= kWave) {
             // One pass: each lane records its hits as a bit per image (in loop
             // order) and emits them from the mask after the wave's prefix sum.
+            // A lane tests only the images of its pair (i, j) whose slab
+            // along every lattice direction k comes within rc, i.e.
+            // |(r_i - r_j) . g_k + b_k| <= rc |g_k| (g_k the reciprocal
+            // vectors, so the left side over |g_k| is the perpendicular
+            // distance); with the margin in rk the images skipped hold no
+            // neighbour.
             __shared__ int img_tab[kWave];
             if (lane < nimg) {
                 const int a0 = lane / n12, a1 = (lane - a0 * n12) / (hi[2] - lo[2] + 1);
@@ -207,14 +215,23 @@ __global__ __launch_bounds__(64) void k_build(SysView s, double rc2, int slots)
                 unsigned long long hm = 0;
                 if (jl < S.natoms) {
                     const double* rj = s.pos + 3 * (size_t)j;
-                    int t = 0;
-                    for (int b0 = lo[0]; b0 <= hi[0]; ++b0)
-                        for (int b1 = lo[1]; b1 <= hi[1]; ++b1)
-                            for (int b2 = lo[2]; b2 <= hi[2]; ++b2, ++t) {
+                    int l[3] = {lo[0], lo[1], lo[2]}, h[3] = {hi[0], hi[1], hi[2]};
+                    if (S.periodic)
+                        for (int k = 0; k < 3; ++k) {
+                            const double df = (ri[0] - rj[0]) * S.invbox[k] + (ri[1] - rj[1]) * S.invbox[3 + k] +
+                                              (ri[2] - rj[2]) * S.invbox[6 + k];
+                            l[k] = max(l[k], (int)ceil(-df - rk[k]));
+                            h[k] = min(h[k], (int)floor(-df + rk[k]));
+                        }
+                    const int n2 = hi[2] - lo[2] + 1, n1 = hi[1] - lo[1] + 1;
+                    for (int b0 = l[0]; b0 <= h[0]; ++b0)
+                        for (int b1 = l[1]; b1 <= h[1]; ++b1)
+                            for (int b2 = l[2]; b2 <= h[2]; ++b2) {
                                 if (i == j && b0 == 0 && b1 == 0 && b2 == 0) continue;
                                 double dr[3];
                                 pair_vector(ri, rj, S.box, b0, b1, b2, dr);
-                                if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2) hm |= 1ull << t;
+                                if (dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2] <= rc2)
+                                    hm |= 1ull << (((b0 - lo[0]) * n1 + (b1 - lo[1])) * n2 + (b2 - lo[2]));
                             }
                 }
                 const int hits = __popcll(hm);
