@@ -1297,6 +1297,8 @@ struct NNMix {
     int nelem, ldK, ldG, terms;
     int stride_in;  // LDS row length of the input tile (padded input width + 4)
     int stride;     // ... of the hidden/output tiles (padded layer width + 4)
+    double* zero;   // cleared by the blocks in a grid-stride loop (the force
+    size_t nzero;   // array of the force kernel that follows), nzero doubles
 };
 
 __device__ inline float act32(int a, float x, float& d)
@@ -1370,6 +1372,8 @@ __global__ __launch_bounds__(kWave * kNNWaves) void k_nnbf16(const ModelT* __res
                                                             float* __restrict__ dEdGf)
 {
     extern __shared__ float sm[];
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < a.nzero; q += (size_t)gridDim.x * blockDim.x)
+        a.zero[q] = 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = threadIdx.x / kWave;
     int e = 0;
@@ -2543,6 +2547,11 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     a.atom_off[t.nelem] = b.elem_off[t.nelem];
     const size_t lds_nn = sizeof(float) * (size_t)kNNRows * (a.stride_in + (size_t)a.stride * (2 + kMaxLayers));
     if (lds_nn > 65536) throw Error{NNPX_ERR_UNSUPPORTED, "network too wide for the mixed NN kernel's LDS"};
+    // the NN blocks clear the forces of the force kernel (one launch fewer)
+    const size_t n3 = forces ? (size_t)3 * s.nforce : 0;
+    const bool nn_zero = blocks > 0;
+    a.zero = nn_zero ? b.F : nullptr;
+    a.nzero = nn_zero ? n3 : 0;
     timed("nnbf16", [&] {
         hipLaunchKernelGGL(k_nnbf16, dim3(blocks), dim3(kWave * kNNWaves), lds_nn, stream, d_model, a, b.Gin, b.Eatom,
                            b.dEdG, b.dEdGf);
@@ -2550,9 +2559,9 @@ void run_mixed(const MixedModel& mm, const ModelT& t, const ModelT* d_model, con
     timed("energy", [&] { launch_energy(b.Eatom, b.atom_off, b.nstruct, b.E, stream); });
     if (!forces) return;
     timed("force32", [&] {
-        const size_t n3 = (size_t)3 * s.nforce;
-        hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0, stream,
-                           b.F, n3);
+        if (!nn_zero && n3 > 0)
+            hipLaunchKernelGGL(k_zero, dim3((unsigned)std::min<size_t>((n3 + 255) / 256, 4096)), dim3(256), 0,
+                               stream, b.F, n3);
         const int ps = push ? 1 : 0;
         dispatch_angular(mm, P, [&](auto m, auto fast, auto nb0, auto nb1) {
             constexpr int cm = decltype(m)::value, c0 = decltype(nb0)::value, c1 = decltype(nb1)::value;
